@@ -1,0 +1,234 @@
+"""Benchmark: FP64 SLIC 2D Euler on a 4096 x 4096 grid (one MI355X).
+
+Workload: ideal gas gamma 1.4, dimensionally split SLIC
+with the superbee limiter, 4096^2 cells on [0,2]^2, transmissive boundaries,
+cylindrical explosion (radius 0.4) plus 64 seeded Gaussian density bumps
+(amplitude 1e-3, width 0.02), CFL 0.8, 200 steps, FP64.
+
+A "step" is one full time step X(dt/2) Y(dt) X(dt/2) of the whole grid,
+including its CFL reduction.  The initial grid is loaded into HBM before the
+timed region; the timed region is exactly --steps steps from the initial
+condition (warm-up steps run on a separate copy first).  One JSON line on
+stdout carries the throughput, the roofline of the dominant kernel (HIP-event
+timing of every launch in the timed region) and the reference CPU solver timed
+on this host on a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps 200] [--warmup 3] [--mode auto|exact|fast] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N = 4096
+LENGTH = 2.0
+CFL = 0.8
+LIMITER = 0           # superbee (Solvers/slopelimiters.cpp:62-65)
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_SWEEP = 96   # read 6 + write 6 FP64 primitive fields
+METRIC = "Mcell-updates/s FP64 SLIC 2D Euler 4096²; %HBM peak; adjoint/forward cost"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return model, cores
+
+
+def cpu_baseline(sample_n: int, steps: int):
+    """The reference CPU solver (oracle/_ref/ref_harness, compiled from the
+    reference sources) on a sample of the same workload: sample_n^2 cells of the
+    same initial condition, `steps` steps, all host cores (OpenMP)."""
+    from vanguard_amd import problems
+
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    model, cores = cpu_info()
+    w = problems.disc_2d(sample_n)
+    if os.path.exists(harness):
+        with tempfile.TemporaryDirectory() as tmp:
+            w.tofile(os.path.join(tmp, "in.f64"))
+            env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+            args = [harness, "euler_run", "2", "2", str(sample_n), str(sample_n), repr(LENGTH / sample_n), repr(CFL),
+                    "inf", str(steps), "0.0", str(LIMITER), "1.4", "0", "25", "0", "0.25", "250", "in.f64",
+                    "out.f64", "dt.f64", "0"]
+            r = subprocess.run(args, env=env, cwd=tmp, capture_output=True, text=True, timeout=600)
+            m = re.search(r"STEPS (\d+) SECONDS ([0-9.eE+-]+)", r.stderr)
+            if r.returncode != 0 or not m:
+                raise RuntimeError(f"reference harness failed: {r.stderr[-500:]}")
+            secs = float(m.group(2))
+            kind = "reference"
+            threads = cores
+    else:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        t0 = time.perf_counter()
+        O.euler_solve(w, order=2, dim=2, dx=LENGTH / sample_n, cfl=CFL, tfinal=math.inf, max_steps=steps, bias=0.0,
+                      limiter=LIMITER, params=[1.4, 0, 25, 0, 0.25, 250])
+        secs = time.perf_counter() - t0
+        kind = "port"
+        threads = 1
+    value = sample_n * sample_n * steps / secs / 1e6
+    return {"value": round(value, 4), "unit": "Mcell-updates/s", "cores": threads, "kind": kind,
+            "sample": f"{sample_n}x{sample_n} cells of the same initial condition (dx = {LENGTH}/{sample_n}), "
+                      f"{steps} SLIC steps, {secs:.2f} s, OMP_NUM_THREADS={threads}, CPU: {model}"}
+
+
+def run_mode(mode_name, w, steps, warmup, ctx):
+    from vanguard_amd import solvers as S
+
+    mode = S.EXACT if mode_name == "exact" else S.FAST
+    dx = LENGTH / N
+    if warmup:
+        g0 = S.EulerGrid(2, N, N, ctx=ctx)
+        g0.load(w)
+        g0.run(dx, CFL, max_steps=warmup, limiter=LIMITER, mode=mode)
+        g0.close()
+    g = S.EulerGrid(2, N, N, ctx=ctx)
+    g.load(w)
+    g.max_wave_speed()                    # the first step's reduction, outside the timed region
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st, dts = g.run(dx, CFL, max_steps=steps, limiter=LIMITER, mode=mode, dt_cap=steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.set_timing(False)
+    kernels = {}
+    for k in ("sweep_x_slic", "sweep_y_slic"):
+        n, ms = ctx.kernel_time(k)
+        kernels[k] = (n, ms)
+    out = g.store()
+    nonfinite = int((~(out[..., 0] > 0)).sum() + (~np.isfinite(out)).sum())
+    g.close()
+    return dict(steps=st.steps, t=st.t, elapsed=elapsed, device_ms=st.device_ms, kernels=kernels, dts=dts,
+                nonfinite=nonfinite, out=out)
+
+
+def conserved(w, gamma=1.4):
+    rho, u, v, z, p, lam = np.moveaxis(w, -1, 0)
+    E = rho * (0.5 * (u * u + v * v + z * z) + p / ((gamma - 1) * rho))
+    return [rho, rho * u, rho * v, rho * z, E, rho * lam]
+
+
+def rel_l1_max(a, b):
+    worst = 0.0
+    for x, y in zip(conserved(a), conserved(b)):
+        den = np.abs(y).sum()
+        if den > 0:
+            worst = max(worst, float(np.abs(x - y).sum() / den))
+    return worst
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--mode", choices=["auto", "exact", "fast"], default="auto")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=1024)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
+              "The 2D grid would shard into row slabs with a 2-row halo exchange per Y sweep and one 8-byte "
+              "max-allreduce per step; that path is designed, not built.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from vanguard_amd import problems
+    from vanguard_amd import solvers as S
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.init()
+    log(f"building initial condition {N}x{N} ...")
+    w = problems.disc_2d(N)
+    ctx = S.default_context()
+
+    modes = ["exact", "fast"] if args.mode == "auto" else [args.mode]
+    res = {}
+    for m in modes:
+        log(f"running {args.steps} steps in {m} mode (warm-up {args.warmup}) ...")
+        res[m] = run_mode(m, w, args.steps, args.warmup, ctx)
+        r = res[m]
+        log(f"  {m}: {r['elapsed']*1e3:.1f} ms, {N*N*args.steps/r['elapsed']/1e6:.1f} Mcell-updates/s, "
+            f"kernels {r['kernels']}")
+    # Headline: FAST when both ran and it matches EXACT within the 1e-10 parity bar, else EXACT.
+    head = "exact"
+    parity = None
+    if "fast" in res and "exact" in res:
+        l1 = rel_l1_max(res["fast"]["out"], res["exact"]["out"])
+        ddt = float(np.abs(res["fast"]["dts"] - res["exact"]["dts"]).max() / np.abs(res["exact"]["dts"]).max())
+        parity = {"fast_vs_exact_rel_l1_max": l1, "fast_vs_exact_dt_rel_max": ddt, "bar": 1e-10}
+        log(f"  fast vs exact: rel L1 {l1:.3e}, dt rel {ddt:.3e}")
+        if l1 <= 1e-10 and ddt <= 1e-10 and res["fast"]["nonfinite"] == 0:
+            head = "fast"
+    elif "fast" in res:
+        head = "fast"
+    r = res[head]
+    cells = N * N
+    value = cells * r["steps"] / r["elapsed"] / 1e6
+    ms_per_step = r["elapsed"] * 1e3 / r["steps"]
+    kern = max(r["kernels"].items(), key=lambda kv: kv[1][1])
+    name, (launches, total_ms) = kern
+    avg_ms = total_ms / launches
+    achieved = BYTES_PER_CELL_SWEEP * cells / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+                "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": BYTES_PER_CELL_SWEEP * cells}
+    step_bytes = 3 * BYTES_PER_CELL_SWEEP * cells
+    line = {
+        "metric": METRIC, "value": round(value, 2), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": r["steps"],
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "euler2d_slic_superbee_4096x4096_disc_64gauss", "grid": [N, N], "domain": [LENGTH, LENGTH],
+                   "cfl": CFL, "limiter": "superbee", "steps": r["steps"], "mode": head, "boundary": "transmissive",
+                   "parallelism": "single-gpu"},
+        "hbm_frac_step": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": roofline,
+        "parity": parity,
+        "modes": {m: {"Mcell_updates_per_s": round(cells * x["steps"] / x["elapsed"] / 1e6, 2),
+                      "ms_per_step": round(x["elapsed"] * 1e3 / x["steps"], 4),
+                      "kernels_ms_avg": {k: round(v[1] / max(v[0], 1), 4) for k, v in x["kernels"].items()},
+                      "t_final": x["t"], "nonfinite": x["nonfinite"]} for m, x in res.items()},
+    }
+    if not args.no_cpu:
+        log(f"timing the reference CPU solver on a {args.cpu_sample}^2 sample ...")
+        line["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.cpu_steps)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,136 @@
+/* vanguard.h -- C ABI of the MI355X-native explicit Godunov cell update.
+ *
+ * Drop-in replacement for the solve entry points of the reference's Solvers/
+ * directory.  Plain C: pointers, sizes and PODs only.  All functions return a
+ * status (VG_OK = 0, negative on error; vg_last_error() describes the last
+ * failure on the calling thread).  No exception crosses this boundary.
+ *
+ * Host-pointer calls (prim_in / prim_out) are synchronous.  Primitive states are
+ * arrays of structures in the reference's field order, row-major with x (the
+ * column index j) contiguous -- bit-compatible with the reference's
+ * std::vector<EulerStateVector> and std::vector<std::vector<EulerStateVector>>.
+ *
+ * Boundary conditions: VG_BC_TRANSMISSIVE reproduces the reference's ghost
+ * cells (Solvers/solvers.cpp:7-32, 88-143); VG_BC_PERIODIC has no reference
+ * counterpart.
+ */
+#ifndef VANGUARD_H
+#define VANGUARD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    VG_OK = 0,
+    VG_ERR_ARG = -1,        /* invalid argument (sizes, pointers, options) */
+    VG_ERR_HIP = -2,        /* HIP runtime failure */
+    VG_ERR_NONFINITE = -3,  /* the final state holds NaN/Inf or rho <= 0; outputs are still written */
+    VG_ERR_UNSUPPORTED = -4 /* option combination not implemented */
+};
+
+enum { VG_BC_TRANSMISSIVE = 0, VG_BC_PERIODIC = 1 };
+
+/* VG_MODE_EXACT: the reference's operation order, bit-identical to its CPU
+ * solver.  VG_MODE_FAST: algebraically reduced arithmetic (reciprocals reused,
+ * no primitive/conserved round trips), same scheme; agrees with EXACT to about
+ * 1e-13 relative and is not bit-identical. */
+enum { VG_MODE_EXACT = 0, VG_MODE_FAST = 1 };
+
+typedef struct vg_ctx vg_ctx;
+
+/* EulerMaterialParameters (Euler/eulermaterialparameters.h:30-36), same field
+ * order; EulerMaterialParameters(gamma) sets p_inf 0, cv 25, e_form 0,
+ * T_ign 0.25, k_rate 250 (eulermaterialparameters.cpp:14-23). */
+typedef struct {
+    double gamma, p_inf, cv, e_form, t_ign, k_rate;
+} vg_euler_params;
+
+typedef struct {
+    int32_t steps;          /* time steps taken by this call */
+    int32_t total_steps;    /* steps since the state was loaded */
+    double t;               /* simulated time reached */
+    double dt_min, dt_max;  /* over this call's steps */
+    double wall_ms;         /* host wall time of the call */
+    double device_ms;       /* stream time of the step loop (HIP events) */
+    int64_t nonfinite_cells;/* cells of the final state with NaN/Inf or rho <= 0 */
+    int64_t first_nonfinite;/* flat index of the first such cell, -1 if none */
+} vg_run_stats;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Creates a context on HIP device `device`: one stream, the scratch of the
+ * solvers and the kernel-timing table.  One context per host thread. */
+int vg_ctx_create(vg_ctx **ctx, int device);
+void vg_ctx_destroy(vg_ctx *ctx);
+/* Description of the last failure on this thread ("" if none). */
+const char *vg_last_error(void);
+/* Library version string. */
+const char *vg_version(void);
+
+/* Per-kernel timing: when enabled, every sweep launch is bracketed by HIP
+ * events on the context's stream; totals are kept per kernel name. */
+int vg_ctx_set_timing(vg_ctx *ctx, int enabled);
+int vg_ctx_kernel_time(vg_ctx *ctx, const char *kernel, int64_t *launches, double *total_ms);
+int vg_ctx_reset_timing(vg_ctx *ctx);
+
+/* ---- Euler: whole-solve drop-ins ---------------------------------------- */
+
+/* SecondOrderSolver::solve (Solvers/secondordersolver.h:55-56,
+ * secondordersolver.cpp:288-314) when order == 2, FirstOrderSolver::solve
+ * (firstordersolver.h:87-88, firstordersolver.cpp:435-461) when order == 1.
+ * prim_in/prim_out: n cells x 6 doubles (may alias).  max_steps 0 runs while
+ * t < t_final exactly as the reference; > 0 also stops after that many steps.
+ * dt_log (may be NULL) receives up to dt_log_cap time steps.  The reference's
+ * subcyclingIterations has no effect for Euler (an empty loop,
+ * secondordersolver.cpp:302-305) and is therefore not an argument. */
+int vg_euler_solve_1d(vg_ctx *ctx, const double *prim_in, double *prim_out, int n, double dx, double cfl,
+                      double t_final, int max_steps, double bias, int limiter, int order, int bc,
+                      const vg_euler_params *mp, int mode, double *dt_log, int dt_log_cap,
+                      vg_run_stats *stats);
+
+/* SecondOrderSolver::solve2D (secondordersolver.h:64-65, secondordersolver.cpp:392-424)
+ * when order == 2, FirstOrderSolver::solve2D (firstordersolver.h:94-95,
+ * firstordersolver.cpp:521-553) when order == 1: Strang split X(dt/2) Y(dt) X(dt/2).
+ * prim_in/prim_out: ny rows x nx columns x 6 doubles. */
+int vg_euler_solve_2d(vg_ctx *ctx, const double *prim_in, double *prim_out, int ny, int nx, double dx,
+                      double cfl, double t_final, int max_steps, double bias, int limiter, int order,
+                      int bc, const vg_euler_params *mp, int mode, double *dt_log, int dt_log_cap,
+                      vg_run_stats *stats);
+
+/* ---- Euler: device-resident state (stream-ordered) ----------------------- */
+
+typedef struct vg_euler_state vg_euler_state;
+
+/* A device-resident grid (dim 1: ny must be 1) held as structure-of-arrays
+ * primitive planes, with its time-step clock.  Replaces the per-sweep calls
+ * computeSLICTimeStep / computeX|YSLICTimeStep2D (secondordersolver.h:32-48)
+ * and Solvers::computeStableTimeStep[2D] (solvers.h:31-45) for callers that keep
+ * the grid on the device between steps. */
+int vg_euler_state_create(vg_ctx *ctx, int dim, int ny, int nx, vg_euler_state **st);
+void vg_euler_state_destroy(vg_euler_state *st);
+/* Loads an array-of-structures primitive grid (host memory when on_device is 0,
+ * device memory otherwise) and resets the clock to t = 0, step 0. */
+int vg_euler_state_load(vg_euler_state *st, const double *prim, int on_device);
+/* Writes the current grid as array-of-structures primitives. */
+int vg_euler_state_store(vg_euler_state *st, double *prim, int on_device);
+/* Device pointer of the current structure-of-arrays grid: 6 planes of
+ * ny*nx doubles at a stride of *plane doubles. */
+int vg_euler_state_planes(vg_euler_state *st, double **planes, int64_t *plane);
+/* Advances up to max_steps steps (0 = until t >= t_final) continuing the clock. */
+int vg_euler_state_run(vg_euler_state *st, int order, double dx, double cfl, double t_final, int max_steps,
+                       double bias, int limiter, int bc, const vg_euler_params *mp, int mode,
+                       double *dt_log, int dt_log_cap, vg_run_stats *stats);
+/* Solvers::computeMaximumWaveSpeed[2D] (solvers.cpp:259-276, 318-339) over the
+ * ghosted grid (2D transmissive: default-state corners included); NaN speeds
+ * are skipped as in the reference.  Order-independent, hence deterministic. */
+int vg_euler_state_max_wave_speed(vg_euler_state *st, int bc, const vg_euler_params *mp, double *smax);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VANGUARD_H */

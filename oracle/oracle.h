@@ -1,0 +1,54 @@
+/* CPU oracle: a plain-C restatement of the reference's explicit Godunov-type
+ * cell update (Euler, MHD, HPR, RGFM), used ONLY as the checker by tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg.  Nothing in the
+ * product (vanguard_amd/, include/) links, loads or calls it.
+ *
+ * Every function restates the reference's arithmetic in the reference's
+ * operation order (no FMA contraction, IEEE division and square root), so its
+ * results are bit-identical to the compiled reference; each definition cites
+ * the reference file:line it follows.  Pinning: see tests/test_oracle_*.py,
+ * which compare these functions bit for bit with golden vectors written by the
+ * compiled reference (tests/golden/) and with closed forms and invariants.
+ *
+ * Layout: primitive states are arrays of structures, nf doubles per cell, in
+ * the reference's field order (Euler: rho, u, v, w, p, lambda), row-major with
+ * x (column index j) contiguous, exactly as std::vector<std::vector<State>>.
+ */
+#ifndef VG_ORACLE_H
+#define VG_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* EulerMaterialParameters (Euler/eulermaterialparameters.h:30-36) field order. */
+typedef struct {
+    double gamma, p_inf, cv, e_form, t_ign, k_rate;
+} orc_euler_params;
+
+/* Whole-solve restatement.  dim 1: nx cells (ny ignored, pass 1); dim 2: ny*nx.
+ * order 1 = FORCE (FirstOrderSolver), 2 = SLIC (SecondOrderSolver).
+ * bc 0 = transmissive (reference ghosts), 1 = periodic (not in the reference).
+ * max_steps 0 = run while t < t_final (reference semantics).
+ * dt_log (may be NULL) receives up to dt_cap time steps.  Returns steps taken. */
+int orc_euler_solve(int order, int dim, int ny, int nx, double dx, double cfl, double t_final,
+                    int max_steps, double bias, int limiter, int bc, const orc_euler_params *mp,
+                    const double *prim_in, double *prim_out, double *dt_log, int dt_cap);
+
+/* Per-function entry points (same record layout as the reference harness's
+ * euler_units mode: 92 doubles per record of five input states). */
+void orc_euler_units(int n, double bias, int limiter, double dx, double ts, const orc_euler_params *mp,
+                     const double *states, double *out);
+void orc_limiter_units(int n, double bias, const double *r, double *out);
+
+/* One sweep (for tests of single kernels): dir 0 = x, 1 = y. */
+void orc_euler_sweep(int order, int dim, int dir, int ny, int nx, double dx, double ts, double bias,
+                     int limiter, int bc, const orc_euler_params *mp, const double *prim_in,
+                     double *prim_out);
+/* Maximum wave speed over the ghosted grid, corners included in 2D. */
+double orc_euler_max_wave_speed(int dim, int ny, int nx, const orc_euler_params *mp, const double *prim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

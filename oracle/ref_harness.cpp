@@ -1,0 +1,299 @@
+// Golden-vector harness for the reference CPU solver.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is compiled (by oracle/Makefile) against
+// the reference sources where they lie under /root/reference, into
+// oracle/_ref/ref_harness.  It is used to write golden fixtures in this
+// container (tests/golden/gen_golden.py) and, as a prebuilt binary, as the
+// CPU-baseline timer in bench.py.  Nothing in the product links or calls it.
+//
+// The reference API has no step cap, so the run modes drive the reference's own
+// public static per-step functions in loops that mirror the bodies of
+//   SecondOrderSolver::solve    (Solvers/secondordersolver.cpp:288-314)
+//   SecondOrderSolver::solve2D  (Solvers/secondordersolver.cpp:392-424)
+//   FirstOrderSolver::solve     (Solvers/firstordersolver.cpp:435-461)
+//   FirstOrderSolver::solve2D   (Solvers/firstordersolver.cpp:521-553)
+// and record every time step at full precision.  With max_steps == 0 the loop
+// runs to the final time exactly like the reference and, when asked, the
+// reference's own solve/solve2D is called too and compared bit for bit.
+
+#include "Solvers/secondordersolver.h"
+#include "Solvers/firstordersolver.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <cstdint>
+#include <chrono>
+#include <iostream>
+#include <limits>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace {
+
+std::vector<double> readDoubles(const char *path, size_t count)
+{
+    std::vector<double> v(count);
+    FILE *f = std::fopen(path, "rb");
+    if (!f) { std::fprintf(stderr, "cannot open %s\n", path); std::exit(2); }
+    size_t got = std::fread(v.data(), sizeof(double), count, f);
+    std::fclose(f);
+    if (got != count) { std::fprintf(stderr, "short read %s: %zu of %zu\n", path, got, count); std::exit(2); }
+    return v;
+}
+
+void writeDoubles(const char *path, const std::vector<double> &v)
+{
+    FILE *f = std::fopen(path, "wb");
+    if (!f) { std::fprintf(stderr, "cannot write %s\n", path); std::exit(2); }
+    std::fwrite(v.data(), sizeof(double), v.size(), f);
+    std::fclose(f);
+}
+
+EulerStateVector stateFrom(const double *p)
+{
+    return EulerStateVector(p[0], p[1], p[2], p[3], p[4], p[5]);
+}
+
+void putState(EulerStateVector s, double *p)
+{
+    p[0] = s.getDensity();
+    p[1] = s.getXVelocity();
+    p[2] = s.getYVelocity();
+    p[3] = s.getZVelocity();
+    p[4] = s.getPressure();
+    p[5] = s.getReactionProgress();
+}
+
+void putVec(const std::vector<double> &v, double *p)
+{
+    for (size_t k = 0; k < v.size(); k++) p[k] = v[k];
+}
+
+bool sameBits(double a, double b)
+{
+    uint64_t x, y;
+    std::memcpy(&x, &a, 8);
+    std::memcpy(&y, &b, 8);
+    return x == y;
+}
+
+// Silences the per-step status line the reference prints to stdout.
+struct MuteCout
+{
+    std::ostringstream sink;
+    std::streambuf *old;
+    MuteCout() : old(std::cout.rdbuf(sink.rdbuf())) {}
+    ~MuteCout() { std::cout.rdbuf(old); }
+};
+
+EulerMaterialParameters eulerParams(char **a)
+{
+    return EulerMaterialParameters(std::atof(a[0]), std::atof(a[1]), std::atof(a[2]), std::atof(a[3]), std::atof(a[4]), std::atof(a[5]));
+}
+
+// euler_run order dim ny nx dx cfl tfinal max_steps bias limiter g pinf cv ef tign krate in out_state out_dt check_api
+int eulerRun(int argc, char **argv)
+{
+    if (argc < 22) { std::fprintf(stderr, "euler_run: too few arguments\n"); return 2; }
+    int order = std::atoi(argv[2]);
+    int dim = std::atoi(argv[3]);
+    int ny = std::atoi(argv[4]);
+    int nx = std::atoi(argv[5]);
+    double dx = std::atof(argv[6]);
+    double cfl = std::atof(argv[7]);
+    double tfinal = std::strcmp(argv[8], "inf") == 0 ? std::numeric_limits<double>::infinity() : std::atof(argv[8]);
+    int maxSteps = std::atoi(argv[9]);
+    double bias = std::atof(argv[10]);
+    int limiter = std::atoi(argv[11]);
+    EulerMaterialParameters mp = eulerParams(argv + 12);
+    const char *inPath = argv[18];
+    const char *outState = argv[19];
+    const char *outDt = argv[20];
+    int checkApi = std::atoi(argv[21]);
+    int ghost = (order == 1) ? 1 : 2;
+
+    std::vector<double> in = readDoubles(inPath, (size_t)ny * nx * 6);
+    std::vector<double> dts;
+    std::vector<double> out((size_t)ny * nx * 6);
+    MuteCout mute;
+    auto t0 = std::chrono::steady_clock::now();
+
+    if (dim == 1)
+    {
+        std::vector<EulerStateVector> cells(nx);
+        for (int j = 0; j < nx; j++) cells[j] = stateFrom(&in[(size_t)j * 6]);
+        std::vector<EulerStateVector> initial = cells;
+        double t = 0.0;
+        int it = 0;
+        while (t < tfinal && (maxSteps == 0 || it < maxSteps))
+        {
+            std::vector<EulerStateVector> withBoundary = Solvers::insertBoundaryCells(cells, ghost);
+            double dt = Solvers::computeStableTimeStep(withBoundary, dx, cfl, t, tfinal, it, mp);
+            if (order == 1)
+                FirstOrderSolver::computeFORCETimeStep(cells, withBoundary, dx, dt, mp);
+            else
+                SecondOrderSolver::computeSLICTimeStep(cells, withBoundary, dx, dt, bias, limiter, mp);
+            t += dt;
+            it += 1;
+            dts.push_back(dt);
+        }
+        for (int j = 0; j < nx; j++) putState(cells[j], &out[(size_t)j * 6]);
+        if (checkApi && maxSteps == 0)
+        {
+            std::vector<EulerStateVector> api = (order == 1) ? FirstOrderSolver::solve(initial, dx, cfl, tfinal, 0, mp)
+                                                             : SecondOrderSolver::solve(initial, dx, cfl, tfinal, bias, limiter, 0, mp);
+            bool match = true;
+            for (int j = 0; j < nx; j++)
+            {
+                double p[6];
+                putState(api[j], p);
+                for (int f = 0; f < 6; f++) match = match && sameBits(p[f], out[(size_t)j * 6 + f]);
+            }
+            std::fprintf(stderr, "API_MATCH %d\n", match ? 1 : 0);
+        }
+    }
+    else
+    {
+        std::vector<std::vector<EulerStateVector> > cells(ny, std::vector<EulerStateVector>(nx));
+        for (int i = 0; i < ny; i++)
+            for (int j = 0; j < nx; j++) cells[i][j] = stateFrom(&in[((size_t)i * nx + j) * 6]);
+        std::vector<std::vector<EulerStateVector> > initial = cells;
+        double t = 0.0;
+        int it = 0;
+        while (t < tfinal && (maxSteps == 0 || it < maxSteps))
+        {
+            std::vector<std::vector<EulerStateVector> > withBoundary = Solvers::insertBoundaryCells2D(cells, ghost);
+            double dt = Solvers::computeStableTimeStep2D(withBoundary, dx, cfl, t, tfinal, it, mp);
+            if (order == 1)
+            {
+                FirstOrderSolver::computeXFORCETimeStep2D(cells, withBoundary, dx, 0.5 * dt, mp);
+                withBoundary = Solvers::insertBoundaryCells2D(cells, ghost);
+                FirstOrderSolver::computeYFORCETimeStep2D(cells, withBoundary, dx, dt, mp);
+                withBoundary = Solvers::insertBoundaryCells2D(cells, ghost);
+                FirstOrderSolver::computeXFORCETimeStep2D(cells, withBoundary, dx, 0.5 * dt, mp);
+            }
+            else
+            {
+                SecondOrderSolver::computeXSLICTimeStep2D(cells, withBoundary, dx, 0.5 * dt, bias, limiter, mp);
+                withBoundary = Solvers::insertBoundaryCells2D(cells, ghost);
+                SecondOrderSolver::computeYSLICTimeStep2D(cells, withBoundary, dx, dt, bias, limiter, mp);
+                withBoundary = Solvers::insertBoundaryCells2D(cells, ghost);
+                SecondOrderSolver::computeXSLICTimeStep2D(cells, withBoundary, dx, 0.5 * dt, bias, limiter, mp);
+            }
+            t += dt;
+            it += 1;
+            dts.push_back(dt);
+        }
+        for (int i = 0; i < ny; i++)
+            for (int j = 0; j < nx; j++) putState(cells[i][j], &out[((size_t)i * nx + j) * 6]);
+        if (checkApi && maxSteps == 0)
+        {
+            std::vector<std::vector<EulerStateVector> > api = (order == 1) ? FirstOrderSolver::solve2D(initial, dx, cfl, tfinal, 0, mp)
+                                                                           : SecondOrderSolver::solve2D(initial, dx, cfl, tfinal, bias, limiter, 0, mp);
+            bool match = true;
+            for (int i = 0; i < ny; i++)
+                for (int j = 0; j < nx; j++)
+                {
+                    double p[6];
+                    putState(api[i][j], p);
+                    for (int f = 0; f < 6; f++) match = match && sameBits(p[f], out[((size_t)i * nx + j) * 6 + f]);
+                }
+            std::fprintf(stderr, "API_MATCH %d\n", match ? 1 : 0);
+        }
+    }
+    double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::fprintf(stderr, "STEPS %zu SECONDS %.6f\n", dts.size(), secs);
+    writeDoubles(outState, out);
+    writeDoubles(outDt, dts);
+    return 0;
+}
+
+// Per-function vectors.  Each input record is five primitive states a,b,c,d,e
+// (30 doubles); each output record holds, in order:
+//   slope(a,b,c) [6] | Fx(cons b) [6] | Fy(cons b) [6]
+//   evolveX(a,b,c) side0, side1 [12] | evolveY(a,b,c) side0, side1 [12]
+//   FORCEx(b,c) [6] | FORCEy(b,c) [6] | SLICx(a,b,c,d) [6] | SLICy(a,b,c,d) [6]
+//   wave speed 1D of b [1] | wave speed 2D of b [1]
+//   cons(b) [6] | prim(cons(b)) [6] | LFx(b,c) [6] | RIx(b,c) [6]
+// = 92 doubles.
+// euler_units n bias limiter dx ts g pinf cv ef tign krate in out
+int eulerUnits(int argc, char **argv)
+{
+    if (argc < 15) { std::fprintf(stderr, "euler_units: too few arguments\n"); return 2; }
+    int n = std::atoi(argv[2]);
+    double bias = std::atof(argv[3]);
+    int limiter = std::atoi(argv[4]);
+    double dx = std::atof(argv[5]);
+    double ts = std::atof(argv[6]);
+    EulerMaterialParameters mp = eulerParams(argv + 7);
+    std::vector<double> in = readDoubles(argv[13], (size_t)n * 30);
+    const int rec = 92;
+    std::vector<double> out((size_t)n * rec);
+    for (int r = 0; r < n; r++)
+    {
+        const double *s = &in[(size_t)r * 30];
+        EulerStateVector a = stateFrom(s), b = stateFrom(s + 6), c = stateFrom(s + 12), d = stateFrom(s + 18);
+        double *o = &out[(size_t)r * rec];
+        putVec(SlopeLimiters::computeSlopeVector(a, b, c, bias, limiter, mp), o); o += 6;
+        putVec(EulerStateVector::computeXFluxVector(b.computeConservedVariableVector(mp), mp), o); o += 6;
+        putVec(EulerStateVector::computeYFluxVector(b.computeConservedVariableVector(mp), mp), o); o += 6;
+        putState(Solvers::evolveStateByHalfXTimeStep(a, b, c, dx, ts, bias, limiter, 0, mp), o); o += 6;
+        putState(Solvers::evolveStateByHalfXTimeStep(a, b, c, dx, ts, bias, limiter, 1, mp), o); o += 6;
+        putState(Solvers::evolveStateByHalfYTimeStep(a, b, c, dx, ts, bias, limiter, 0, mp), o); o += 6;
+        putState(Solvers::evolveStateByHalfYTimeStep(a, b, c, dx, ts, bias, limiter, 1, mp), o); o += 6;
+        putVec(FirstOrderSolver::computeXFORCEFlux(b, c, dx, ts, mp), o); o += 6;
+        putVec(FirstOrderSolver::computeYFORCEFlux(b, c, dx, ts, mp), o); o += 6;
+        putVec(SecondOrderSolver::computeXSLICFlux(a, b, c, d, dx, ts, bias, limiter, mp), o); o += 6;
+        putVec(SecondOrderSolver::computeYSLICFlux(a, b, c, d, dx, ts, bias, limiter, mp), o); o += 6;
+        {
+            std::vector<EulerStateVector> one(1, b);
+            *o++ = Solvers::computeMaximumWaveSpeed(one, mp);
+            std::vector<std::vector<EulerStateVector> > one2(1, one);
+            *o++ = Solvers::computeMaximumWaveSpeed2D(one2, mp);
+        }
+        std::vector<double> cb = b.computeConservedVariableVector(mp);
+        putVec(cb, o); o += 6;
+        EulerStateVector back;
+        back.setConservedVariableVector(cb, mp);
+        putState(back, o); o += 6;
+        putVec(FirstOrderSolver::computeXLaxFriedrichsFlux(b, c, dx, ts, mp), o); o += 6;
+        putVec(FirstOrderSolver::computeXRichtmyerFlux(b, c, dx, ts, mp), o); o += 6;
+    }
+    writeDoubles(argv[14], out);
+    return 0;
+}
+
+// limiter_units n bias in out: per r -> superbee, van Leer, minbee, R coefficient, code 7 (-> minbee)
+int limiterUnits(int argc, char **argv)
+{
+    if (argc < 6) { std::fprintf(stderr, "limiter_units: too few arguments\n"); return 2; }
+    int n = std::atoi(argv[2]);
+    double bias = std::atof(argv[3]);
+    std::vector<double> in = readDoubles(argv[4], (size_t)n);
+    std::vector<double> out((size_t)n * 5);
+    for (int r = 0; r < n; r++)
+    {
+        out[(size_t)r * 5 + 0] = SlopeLimiters::computeSlopeLimiter(in[r], bias, 0);
+        out[(size_t)r * 5 + 1] = SlopeLimiters::computeSlopeLimiter(in[r], bias, 1);
+        out[(size_t)r * 5 + 2] = SlopeLimiters::computeSlopeLimiter(in[r], bias, 2);
+        out[(size_t)r * 5 + 3] = SlopeLimiters::computeRCoefficient(in[r], bias);
+        out[(size_t)r * 5 + 4] = SlopeLimiters::computeSlopeLimiter(in[r], bias, 7);
+    }
+    writeDoubles(argv[5], out);
+    return 0;
+}
+
+} // namespace
+
+int main(int argc, char **argv)
+{
+    if (argc < 2) { std::fprintf(stderr, "usage: ref_harness <mode> ...\n"); return 2; }
+    std::string mode = argv[1];
+    if (mode == "euler_run") return eulerRun(argc, argv);
+    if (mode == "euler_units") return eulerUnits(argc, argv);
+    if (mode == "limiter_units") return limiterUnits(argc, argv);
+    std::fprintf(stderr, "unknown mode %s\n", mode.c_str());
+    return 2;
+}

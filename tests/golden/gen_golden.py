@@ -1,0 +1,197 @@
+"""Write the golden fixtures under tests/golden/ from the compiled reference.
+
+Runs oracle/_ref/ref_harness (built by `make -C oracle ref` from the reference
+sources under /root/reference) single-threaded -- the reference's OpenMP
+wave-speed maximum is racy, so only OMP_NUM_THREADS=1 runs are deterministic --
+and stores inputs and outputs as compressed .npz fixtures (plain float64 arrays,
+no pickles).  Run in the build container only:
+
+    python tests/golden/gen_golden.py [name ...]
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from vanguard_amd import problems  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+IDEAL = (1.4, 0.0, 25.0, 0.0, 0.25, 250.0)
+# Stiffened gas with an energy of formation, so every EOS term is exercised.
+STIFF = (3.0, 0.1, 25.0, 0.5, 0.25, 250.0)
+
+
+def run(args, tmp):
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    r = subprocess.run([HARNESS] + [str(a) for a in args], env=env, capture_output=True, text=True, cwd=tmp)
+    if r.returncode != 0:
+        raise RuntimeError(f"harness failed: {r.stderr}")
+    return r.stderr
+
+
+def euler_run(name, w, *, order, dim, dx, cfl, tfinal, max_steps, bias, limiter, params, check_api):
+    with tempfile.TemporaryDirectory() as tmp:
+        ny, nx = (1, w.shape[0]) if dim == 1 else w.shape[:2]
+        np.ascontiguousarray(w, dtype=np.float64).tofile(os.path.join(tmp, "in.f64"))
+        t = "inf" if np.isinf(tfinal) else repr(float(tfinal))
+        log = run(["euler_run", order, dim, ny, nx, repr(dx), repr(cfl), t, max_steps, repr(bias), limiter,
+                   *[repr(p) for p in params], "in.f64", "state.f64", "dt.f64", int(check_api)], tmp)
+        state = np.fromfile(os.path.join(tmp, "state.f64")).reshape(w.shape)
+        dt = np.fromfile(os.path.join(tmp, "dt.f64"))
+    api = None
+    for line in log.splitlines():
+        if line.startswith("API_MATCH"):
+            api = int(line.split()[1])
+    if check_api and max_steps == 0 and api != 1:
+        raise RuntimeError(f"{name}: mirror loop differs from the reference's own solve")
+    meta = dict(order=order, dim=dim, dx=dx, cfl=cfl, tfinal=tfinal, max_steps=max_steps, bias=bias,
+                limiter=limiter, params=list(params), api_checked=bool(check_api and max_steps == 0))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), prim_in=w, prim_out=state, dt=dt,
+                        meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
+    print(f"{name}: {len(dt)} steps")
+
+
+def random_states(rng, n, stiff):
+    """Five primitive states per record, with neighbours close enough that many
+    conserved differences fall below the reference's 1e-5 slope clamp and that
+    every limiter branch is reached."""
+    base = np.empty((n, 6))
+    base[:, 0] = rng.uniform(0.2, 2.0, n)
+    base[:, 1:4] = rng.uniform(-1.0, 1.0, (n, 3))
+    base[:, 4] = rng.uniform(0.2, 2.0, n)
+    base[:, 5] = rng.uniform(0.3, 1.0, n) if stiff else 1.0
+    out = np.empty((n, 5, 6))
+    for k in range(5):
+        scale = rng.choice([1e-7, 1e-6, 1e-3, 0.05, 0.3], size=(n, 1))
+        pert = rng.uniform(-1.0, 1.0, (n, 6)) * scale
+        s = base + pert * np.abs(base).clip(0.1)
+        s[:, 0] = np.abs(s[:, 0]) + 0.05
+        s[:, 4] = np.abs(s[:, 4]) + 0.05
+        if not stiff:
+            s[:, 5] = 1.0
+        # some records with a repeated neighbour (zero differences)
+        same = rng.random(n) < 0.1
+        if k > 0:
+            s[same] = out[same, k - 1]
+        out[:, k] = s
+    return out.reshape(n, 30)
+
+
+def euler_units(name, n, seed, bias, limiter, params, stiff):
+    rng = np.random.default_rng(seed)
+    states = random_states(rng, n, stiff)
+    dx, ts = 1e-2, 2e-3
+    with tempfile.TemporaryDirectory() as tmp:
+        states.tofile(os.path.join(tmp, "in.f64"))
+        run(["euler_units", n, repr(bias), limiter, repr(dx), repr(ts), *[repr(p) for p in params],
+             "in.f64", "out.f64"], tmp)
+        out = np.fromfile(os.path.join(tmp, "out.f64")).reshape(n, 92)
+    meta = dict(bias=bias, limiter=limiter, dx=dx, ts=ts, params=list(params))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), states=states, out=out,
+                        meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
+    print(f"{name}: {n} records")
+
+
+def limiter_units(name, bias):
+    r = np.concatenate([np.array([-1.0, -0.0, 0.0, 1e-300, 0.25, 0.5, 0.75, 1.0, 1.5, 2.0, 3.0, 1e300,
+                                  np.nextafter(0.5, 0), np.nextafter(1.0, 0), np.nextafter(1.0, 2)]),
+                        np.random.default_rng(7).uniform(-2, 5, 500)])
+    with tempfile.TemporaryDirectory() as tmp:
+        r.tofile(os.path.join(tmp, "in.f64"))
+        run(["limiter_units", r.size, repr(bias), "in.f64", "out.f64"], tmp)
+        out = np.fromfile(os.path.join(tmp, "out.f64")).reshape(r.size, 5)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), r=r, out=out, bias=np.array(bias))
+    print(f"{name}: {r.size} values")
+
+
+def stiff_1d(n):
+    w = np.empty((n, 6))
+    x = (np.arange(n) + 0.5) / n
+    left = x < 0.4
+    w[left] = (1.0, 0.2, 0.1, -0.05, 1.0, 1.0)
+    w[~left] = (0.5, -0.1, 0.0, 0.02, 0.2, 0.3)
+    return w
+
+
+def rect_2d(ny, nx):
+    """Off-centre, anisotropic 2D problem with flow in x and y (pins the i = y,
+    j = x orientation and the y-direction flux)."""
+    w = np.empty((ny, nx, 6))
+    i = np.arange(ny)[:, None]
+    j = np.arange(nx)[None, :]
+    inside = (i - 0.3 * ny) ** 2 + (0.5 * (j - 0.6 * nx)) ** 2 <= (0.2 * ny) ** 2
+    w[..., 0] = np.where(inside, 1.0, 0.2)
+    w[..., 1] = np.where(inside, 0.3, -0.1)
+    w[..., 2] = np.where(inside, -0.2, 0.15)
+    w[..., 3] = 0.05
+    w[..., 4] = np.where(inside, 1.5, 0.3)
+    w[..., 5] = 1.0
+    return w
+
+
+JOBS = {
+    # config C1 exactly (1000 cells, minbee, CFL 0.9, T 0.25); also checks the
+    # reference's own SecondOrderSolver::solve against the mirror loop.
+    "euler_c1_sod1000": lambda: euler_run("euler_c1_sod1000", problems.sod_1d(1000), order=2, dim=1, dx=1e-3,
+                                          cfl=0.9, tfinal=0.25, max_steps=0, bias=0.0, limiter=2,
+                                          params=IDEAL, check_api=True),
+    # EulerTests::solveToroTest1 settings (eulertests.cpp:26) at 100 cells.
+    "euler_toro100": lambda: euler_run("euler_toro100", problems.sod_1d(100), order=2, dim=1, dx=1e-2, cfl=0.8,
+                                       tfinal=0.25, max_steps=0, bias=0.0, limiter=0, params=IDEAL,
+                                       check_api=True),
+    "euler_1d_stiff_bias": lambda: euler_run("euler_1d_stiff_bias", stiff_1d(200), order=2, dim=1, dx=5e-3,
+                                             cfl=0.7, tfinal=0.05, max_steps=0, bias=0.4, limiter=1,
+                                             params=STIFF, check_api=True),
+    "euler_force1d": lambda: euler_run("euler_force1d", problems.sod_1d(200), order=1, dim=1, dx=5e-3, cfl=0.9,
+                                       tfinal=0.2, max_steps=0, bias=0.0, limiter=0, params=IDEAL,
+                                       check_api=True),
+    "euler_2d_disc64_l0": lambda: euler_run("euler_2d_disc64_l0", problems.disc_2d(64, gaussians=0, length=1.0),
+                                            order=2, dim=2, dx=1.0 / 64, cfl=0.8, tfinal=np.inf, max_steps=200,
+                                            bias=0.0, limiter=0, params=IDEAL, check_api=False),
+    "euler_2d_disc64_l1": lambda: euler_run("euler_2d_disc64_l1", problems.disc_2d(64, gaussians=0, length=1.0),
+                                            order=2, dim=2, dx=1.0 / 64, cfl=0.8, tfinal=np.inf, max_steps=120,
+                                            bias=0.0, limiter=1, params=IDEAL, check_api=False),
+    "euler_2d_disc64_l2": lambda: euler_run("euler_2d_disc64_l2", problems.disc_2d(64, gaussians=0, length=1.0),
+                                            order=2, dim=2, dx=1.0 / 64, cfl=0.8, tfinal=np.inf, max_steps=120,
+                                            bias=0.0, limiter=2, params=IDEAL, check_api=False),
+    # config C2's initial condition (disc + 64 Gaussians, [0,2]^2) at 64^2.
+    "euler_2d_c2ic64": lambda: euler_run("euler_2d_c2ic64", problems.disc_2d(64), order=2, dim=2, dx=2.0 / 64,
+                                         cfl=0.8, tfinal=np.inf, max_steps=200, bias=0.0, limiter=0,
+                                         params=IDEAL, check_api=False),
+    "euler_2d_rect": lambda: euler_run("euler_2d_rect", rect_2d(40, 72), order=2, dim=2, dx=1.0 / 72, cfl=0.6,
+                                       tfinal=np.inf, max_steps=60, bias=0.2, limiter=0, params=STIFF,
+                                       check_api=False),
+    "euler_force2d": lambda: euler_run("euler_force2d", problems.disc_2d(48, gaussians=0, length=1.0), order=1,
+                                       dim=2, dx=1.0 / 48, cfl=0.8, tfinal=np.inf, max_steps=100, bias=0.0,
+                                       limiter=0, params=IDEAL, check_api=False),
+    # solve2D to a final time through the reference's own entry point.
+    "euler_2d_api32": lambda: euler_run("euler_2d_api32", problems.disc_2d(32, gaussians=0, length=1.0), order=2,
+                                        dim=2, dx=1.0 / 32, cfl=0.8, tfinal=0.05, max_steps=0, bias=0.0,
+                                        limiter=0, params=IDEAL, check_api=True),
+    "euler_units_ideal": lambda: euler_units("euler_units_ideal", 1000, 11, 0.0, 0, IDEAL, False),
+    "euler_units_vl_bias": lambda: euler_units("euler_units_vl_bias", 1000, 12, 0.3, 1, STIFF, True),
+    "euler_units_minbee": lambda: euler_units("euler_units_minbee", 1000, 13, -0.2, 2, STIFF, True),
+    "limiter_units_b0": lambda: limiter_units("limiter_units_b0", 0.0),
+    "limiter_units_b03": lambda: limiter_units("limiter_units_b03", 0.3),
+}
+
+
+def main(names):
+    if not os.path.exists(HARNESS):
+        raise SystemExit("build the reference harness first: make -C oracle ref")
+    for n in names or JOBS:
+        JOBS[n]()
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

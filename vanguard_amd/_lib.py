@@ -1,0 +1,97 @@
+"""ctypes bindings of libvanguard_hip.so (the C ABI declared in include/vanguard.h).
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libvanguard_hip.so")
+
+VG_OK = 0
+VG_ERR_ARG = -1
+VG_ERR_HIP = -2
+VG_ERR_NONFINITE = -3
+VG_ERR_UNSUPPORTED = -4
+
+VG_BC_TRANSMISSIVE = 0
+VG_BC_PERIODIC = 1
+VG_MODE_EXACT = 0
+VG_MODE_FAST = 1
+
+dptr = C.POINTER(C.c_double)
+vptr = C.c_void_p
+
+
+class EulerParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("gamma", "p_inf", "cv", "e_form", "t_ign", "k_rate")]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("total_steps", C.c_int32), ("t", C.c_double), ("dt_min", C.c_double),
+                ("dt_max", C.c_double), ("wall_ms", C.c_double), ("device_ms", C.c_double),
+                ("nonfinite_cells", C.c_int64), ("first_nonfinite", C.c_int64)]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "vg_ctx_create": (C.c_int, [C.POINTER(vptr), C.c_int]),
+    "vg_ctx_destroy": (None, [vptr]),
+    "vg_last_error": (C.c_char_p, []),
+    "vg_version": (C.c_char_p, []),
+    "vg_ctx_set_timing": (C.c_int, [vptr, C.c_int]),
+    "vg_ctx_kernel_time": (C.c_int, [vptr, C.c_char_p, C.POINTER(C.c_int64), dptr]),
+    "vg_ctx_reset_timing": (C.c_int, [vptr]),
+    "vg_euler_solve_1d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                    C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(EulerParams), C.c_int,
+                                    dptr, C.c_int, C.POINTER(RunStats)]),
+    "vg_euler_solve_2d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                    C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(EulerParams),
+                                    C.c_int, dptr, C.c_int, C.POINTER(RunStats)]),
+    "vg_euler_state_create": (C.c_int, [vptr, C.c_int, C.c_int, C.c_int, C.POINTER(vptr)]),
+    "vg_euler_state_destroy": (None, [vptr]),
+    "vg_euler_state_load": (C.c_int, [vptr, vptr, C.c_int]),
+    "vg_euler_state_store": (C.c_int, [vptr, vptr, C.c_int]),
+    "vg_euler_state_planes": (C.c_int, [vptr, C.POINTER(vptr), C.POINTER(C.c_int64)]),
+    "vg_euler_state_run": (C.c_int, [vptr, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double,
+                                     C.c_int, C.c_int, C.POINTER(EulerParams), C.c_int, dptr, C.c_int,
+                                     C.POINTER(RunStats)]),
+    "vg_euler_state_max_wave_speed": (C.c_int, [vptr, C.c_int, C.POINTER(EulerParams), dptr]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Loads the HIP library (raises if it is absent or incomplete)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m vanguard_amd.build`")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class VanguardError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        super().__init__(f"{what} failed with status {status}: {load().vg_last_error().decode()}")
+        self.status = status
+
+
+def check(status: int, what: str) -> int:
+    if status != VG_OK:
+        raise VanguardError(status, what)
+    return status
+
+
+def as_dptr(a):
+    return a.ctypes.data_as(dptr)

@@ -1,0 +1,73 @@
+// Device-side building blocks shared by every sweep kernel (gfx950, wave64).
+//
+// All arithmetic here is IEEE binary64 with no contraction: the library is
+// compiled with -ffp-contract=off, so a*b+c stays a multiply and an add, and
+// '/' and sqrt are the correctly rounded AMDGPU expansions.  That is what makes
+// EXACT mode bit-identical to the reference's x86-64 SSE2 arithmetic.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vg {
+
+constexpr int kWave = 64;
+
+// std::min / std::max with libstdc++'s argument order (they differ from fmin/fmax
+// for NaN and signed zeros).
+__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+
+// Ghost-cell index for a cell index k in [-2, n+1].  Transmissive ghosts are
+// mirror-ordered copies of the interior (Solvers/solvers.cpp:17-23, 112-131:
+// g[-1] = c[0], g[-2] = c[1], g[n] = c[n-1], g[n+1] = c[n-2]); the one-layer
+// copy (:12-16, 94-109) is the same map.  Periodic wraps.
+__device__ __forceinline__ int ghost_index(int k, int n, int bc)
+{
+    if (k >= 0 && k < n) return k;
+    if (bc == 1) return k < 0 ? k + n : k - n;
+    return k < 0 ? -1 - k : 2 * n - 1 - k;
+}
+
+// Key for an order-independent maximum of non-negative doubles: for x >= 0 the
+// IEEE bit pattern is monotone in x, so an unsigned 64-bit max of the bits is
+// the max of the values.  NaN (and anything not > 0) maps to 0, which is how the
+// reference's `waveSpeed > maximumWaveSpeed` test skips it
+// (Solvers/solvers.cpp:269-272).
+__device__ __forceinline__ unsigned long long max_key(double s)
+{
+    return (s > 0.0) ? (unsigned long long)__double_as_longlong(s) : 0ull;
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
+{
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(v, off, kWave);
+        v = (o > v) ? o : v;
+    }
+    return v;
+}
+
+// Per-grid time-step clock kept in device memory so that a whole run can be
+// enqueued (or captured in a graph) without host round trips.  The step logic
+// is Solvers::computeStableTimeStep (Solvers/solvers.cpp:389-395, 413-419,
+// 437-452) and the solve loop condition `while (currentTime < finalTime)`
+// (Solvers/secondordersolver.cpp:295, 399).
+struct Clock {
+    double t;            // current simulated time
+    double t_final;
+    double dt;           // time step of the step in progress
+    double cfl, dx;
+    double corner;       // wave speed of the default-state corner ghosts (2D transmissive), else 0
+    unsigned long long smax;  // max_key of the wave speeds of the current grid
+    int it;              // steps completed
+    int max_steps;       // 0 = no cap; else stop when it reaches this (absolute)
+    int done;            // set once the loop condition fails
+    int pending;         // a step was started and t/it not yet advanced
+    double dt_min, dt_max;
+    int steps_this_run;
+    int pad;
+};
+
+}  // namespace vg

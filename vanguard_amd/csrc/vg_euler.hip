@@ -1,0 +1,179 @@
+// Euler C ABI: whole-solve drop-ins and the device-resident state.
+#include <cmath>
+#include <cstring>
+
+#include "vg_grid.h"
+#include "vg_phys_euler.h"
+
+using EulerGrid = vg::GridState<vg::EulerPhys>;
+
+struct vg_euler_state {
+    EulerGrid g;
+};
+
+namespace {
+
+vg::EulerPhys::Params to_params(const vg_euler_params *p)
+{
+    vg::EulerPhys::Params m;
+    m.gamma = p->gamma;
+    m.p_inf = p->p_inf;
+    m.cv = p->cv;
+    m.e_form = p->e_form;
+    m.t_ign = p->t_ign;
+    m.k_rate = p->k_rate;
+    return m;
+}
+
+int check_run_args(int dim, int ny, int nx, double dx, double cfl, double t_final, int max_steps, int limiter,
+                   int order, int bc, const vg_euler_params *mp, int mode)
+{
+    (void)limiter;
+    VG_CHECK(mp != nullptr, "null material parameters");
+    VG_CHECK(order == 1 || order == 2, "order must be 1 (FORCE) or 2 (SLIC)");
+    VG_CHECK(mode == VG_MODE_EXACT || mode == VG_MODE_FAST, "mode must be VG_MODE_EXACT or VG_MODE_FAST");
+    VG_CHECK(bc == VG_BC_TRANSMISSIVE || bc == VG_BC_PERIODIC, "unknown boundary condition");
+    VG_CHECK(dim == 1 || dim == 2, "dim must be 1 or 2");
+    VG_CHECK(nx >= 2 && ny >= 1 && (dim == 1 ? ny == 1 : ny >= 2), "grid needs at least 2 cells per direction");
+    VG_CHECK((long long)ny * nx <= (1ll << 31), "grid too large for one state");
+    VG_CHECK(dx > 0.0 && std::isfinite(dx), "cell spacing must be positive");
+    VG_CHECK(cfl > 0.0 && std::isfinite(cfl), "CFL coefficient must be positive");
+    VG_CHECK(max_steps >= 0, "max_steps must be >= 0");
+    VG_CHECK(max_steps > 0 || std::isfinite(t_final), "an infinite final time needs a step cap");
+    return VG_OK;
+}
+
+int solve(vg_ctx *ctx, int dim, const double *prim_in, double *prim_out, int ny, int nx, double dx, double cfl,
+          double t_final, int max_steps, double bias, int limiter, int order, int bc, const vg_euler_params *mp,
+          int mode, double *dt_log, int dt_log_cap, vg_run_stats *stats)
+{
+    vg::clear_error();
+    VG_CHECK(ctx != nullptr, "null context");
+    VG_CHECK(prim_in != nullptr && prim_out != nullptr, "null state pointer");
+    int rc = check_run_args(dim, ny, nx, dx, cfl, t_final, max_steps, limiter, order, bc, mp, mode);
+    if (rc) return rc;
+    rc = vg::bind(ctx);
+    if (rc) return rc;
+    EulerGrid g;
+    rc = g.create(ctx, dim, ny, nx);
+    if (!rc) rc = g.load(prim_in, 0);
+    vg_run_stats local{};
+    if (!rc) rc = g.run(order, mode, dx, cfl, t_final, max_steps, bias, limiter, bc, to_params(mp), dt_log, dt_log_cap, &local);
+    if (!rc) rc = g.store(prim_out, 0);
+    if (!rc) rc = g.scan_nonfinite(&local.nonfinite_cells, &local.first_nonfinite);
+    g.destroy();
+    if (stats) *stats = local;
+    if (!rc && local.nonfinite_cells > 0) {
+        vg::set_error("final state holds " + std::to_string(local.nonfinite_cells) +
+                      " non-finite or non-positive-density cells, first at flat index " +
+                      std::to_string(local.first_nonfinite));
+        return VG_ERR_NONFINITE;
+    }
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vg_euler_solve_1d(vg_ctx *ctx, const double *prim_in, double *prim_out, int n, double dx, double cfl,
+                      double t_final, int max_steps, double bias, int limiter, int order, int bc,
+                      const vg_euler_params *mp, int mode, double *dt_log, int dt_log_cap, vg_run_stats *stats)
+{
+    return solve(ctx, 1, prim_in, prim_out, 1, n, dx, cfl, t_final, max_steps, bias, limiter, order, bc, mp, mode,
+                 dt_log, dt_log_cap, stats);
+}
+
+int vg_euler_solve_2d(vg_ctx *ctx, const double *prim_in, double *prim_out, int ny, int nx, double dx, double cfl,
+                      double t_final, int max_steps, double bias, int limiter, int order, int bc,
+                      const vg_euler_params *mp, int mode, double *dt_log, int dt_log_cap, vg_run_stats *stats)
+{
+    return solve(ctx, 2, prim_in, prim_out, ny, nx, dx, cfl, t_final, max_steps, bias, limiter, order, bc, mp, mode,
+                 dt_log, dt_log_cap, stats);
+}
+
+int vg_euler_state_create(vg_ctx *ctx, int dim, int ny, int nx, vg_euler_state **st)
+{
+    vg::clear_error();
+    VG_CHECK(ctx != nullptr && st != nullptr, "null argument");
+    VG_CHECK(dim == 1 || dim == 2, "dim must be 1 or 2");
+    VG_CHECK(nx >= 2 && (dim == 1 ? ny == 1 : ny >= 2), "grid needs at least 2 cells per direction");
+    int rc = vg::bind(ctx);
+    if (rc) return rc;
+    vg_euler_state *s = new vg_euler_state();
+    rc = s->g.create(ctx, dim, ny, nx);
+    if (rc) {
+        s->g.destroy();
+        delete s;
+        return rc;
+    }
+    *st = s;
+    return VG_OK;
+}
+
+void vg_euler_state_destroy(vg_euler_state *st)
+{
+    if (!st) return;
+    vg::bind(st->g.ctx);
+    hipStreamSynchronize(st->g.ctx->stream);
+    st->g.destroy();
+    delete st;
+}
+
+int vg_euler_state_load(vg_euler_state *st, const double *prim, int on_device)
+{
+    vg::clear_error();
+    VG_CHECK(st != nullptr && prim != nullptr, "null argument");
+    int rc = vg::bind(st->g.ctx);
+    return rc ? rc : st->g.load(prim, on_device);
+}
+
+int vg_euler_state_store(vg_euler_state *st, double *prim, int on_device)
+{
+    vg::clear_error();
+    VG_CHECK(st != nullptr && prim != nullptr, "null argument");
+    int rc = vg::bind(st->g.ctx);
+    return rc ? rc : st->g.store(prim, on_device);
+}
+
+int vg_euler_state_planes(vg_euler_state *st, double **planes, int64_t *plane)
+{
+    VG_CHECK(st != nullptr && planes != nullptr && plane != nullptr, "null argument");
+    *planes = st->g.buf[st->g.cur];
+    *plane = st->g.plane;
+    return VG_OK;
+}
+
+int vg_euler_state_run(vg_euler_state *st, int order, double dx, double cfl, double t_final, int max_steps,
+                       double bias, int limiter, int bc, const vg_euler_params *mp, int mode, double *dt_log,
+                       int dt_log_cap, vg_run_stats *stats)
+{
+    vg::clear_error();
+    VG_CHECK(st != nullptr, "null state");
+    int rc = check_run_args(st->g.dim, st->g.ny, st->g.nx, dx, cfl, t_final, max_steps, limiter, order, bc, mp, mode);
+    if (rc) return rc;
+    rc = vg::bind(st->g.ctx);
+    if (rc) return rc;
+    return st->g.run(order, mode, dx, cfl, t_final, max_steps, bias, limiter, bc, to_params(mp), dt_log, dt_log_cap,
+                     stats);
+}
+
+int vg_euler_state_max_wave_speed(vg_euler_state *st, int bc, const vg_euler_params *mp, double *smax)
+{
+    vg::clear_error();
+    VG_CHECK(st != nullptr && mp != nullptr && smax != nullptr, "null argument");
+    int rc = vg::bind(st->g.ctx);
+    if (rc) return rc;
+    rc = st->g.refresh_max(bc, to_params(mp));
+    if (rc) return rc;
+    vg::Clock h;
+    VG_HIP(hipMemcpyAsync(&h, st->g.clk, sizeof(h), hipMemcpyDeviceToHost, st->g.ctx->stream));
+    VG_HIP(hipStreamSynchronize(st->g.ctx->stream));
+    double s;
+    std::memcpy(&s, &h.smax, sizeof(s));
+    if (h.corner > s) s = h.corner;
+    *smax = s;
+    return VG_OK;
+}
+
+}  // extern "C"

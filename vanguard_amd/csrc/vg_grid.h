@@ -1,0 +1,295 @@
+// Generic device-resident grid and time loop for one physics P.
+//
+// The loop mirrors the reference's solve/solve2D drivers
+// (Solvers/secondordersolver.cpp:288-314, 392-424; firstordersolver.cpp:435-461,
+// 521-553): each step computes dt from the wave-speed maximum of the current
+// grid, then runs one X sweep (1D) or the Strang split X(dt/2) Y(dt) X(dt/2)
+// (2D).  Everything stays on the device: dt lives in a Clock in device memory,
+// the maximum is folded into the last sweep of the previous step, and the host
+// only enqueues launches, reading the clock back once per chunk of steps.
+#pragma once
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "vg_runtime.h"
+#include "vg_sweep.h"
+
+namespace vg {
+
+constexpr int kBSX = 256;
+constexpr int kBSY = 256;
+constexpr int kHY = 32;
+
+template <class P>
+struct GridState {
+    static constexpr int NF = P::NF;
+    vg_ctx *ctx = nullptr;
+    int dim = 2, ny = 1, nx = 1;
+    long long plane = 0, cells = 0;
+    double *buf[2] = {nullptr, nullptr};
+    int cur = 0;
+    Clock *clk = nullptr;          // device
+    double *staging = nullptr;     // AoS staging for host transfers (lazy)
+    double *dt_dev = nullptr;      // device dt log (lazy)
+    int dt_dev_cap = 0;
+    unsigned long long *scan = nullptr;  // 2 counters for the non-finite scan
+    bool max_valid = false;
+    typename P::Params last_params{};
+    int last_bc = -1;
+
+    int create(vg_ctx *c, int d, int y, int x)
+    {
+        ctx = c;
+        dim = d;
+        ny = y;
+        nx = x;
+        cells = (long long)ny * nx;
+        plane = (cells + 63) / 64 * 64;
+        VG_HIP(hipMalloc(&buf[0], sizeof(double) * NF * plane));
+        VG_HIP(hipMalloc(&buf[1], sizeof(double) * NF * plane));
+        VG_HIP(hipMalloc(&clk, sizeof(Clock)));
+        VG_HIP(hipMalloc(&scan, 2 * sizeof(unsigned long long)));
+        Clock h{};
+        VG_HIP(hipMemcpy(clk, &h, sizeof(Clock), hipMemcpyHostToDevice));
+        return VG_OK;
+    }
+
+    void destroy()
+    {
+        hipFree(buf[0]);
+        hipFree(buf[1]);
+        hipFree(clk);
+        hipFree(staging);
+        hipFree(dt_dev);
+        hipFree(scan);
+    }
+
+    int ensure_staging()
+    {
+        if (!staging) VG_HIP(hipMalloc(&staging, sizeof(double) * NF * cells));
+        return VG_OK;
+    }
+
+    static int blocks_for(long long n) { return (int)((n + 255) / 256 > 4096 ? 4096 : (n + 255) / 256); }
+
+    int load(const double *prim, int on_device)
+    {
+        const double *src = prim;
+        if (!on_device) {
+            int rc = ensure_staging();
+            if (rc) return rc;
+            VG_HIP(hipMemcpyAsync(staging, prim, sizeof(double) * NF * cells, hipMemcpyHostToDevice, ctx->stream));
+            src = staging;
+        }
+        aos_to_soa_kernel<NF><<<blocks_for(cells), 256, 0, ctx->stream>>>(src, buf[0], plane, cells);
+        int rc = launch_status("aos_to_soa");
+        if (rc) return rc;
+        cur = 0;
+        Clock h{};
+        VG_HIP(hipMemcpyAsync(clk, &h, sizeof(Clock), hipMemcpyHostToDevice, ctx->stream));
+        VG_HIP(hipStreamSynchronize(ctx->stream));
+        max_valid = false;
+        return VG_OK;
+    }
+
+    int store(double *prim, int on_device)
+    {
+        double *dst = prim;
+        if (!on_device) {
+            int rc = ensure_staging();
+            if (rc) return rc;
+            dst = staging;
+        }
+        soa_to_aos_kernel<NF><<<blocks_for(cells), 256, 0, ctx->stream>>>(buf[cur], dst, plane, cells);
+        int rc = launch_status("soa_to_aos");
+        if (rc) return rc;
+        if (!on_device)
+            VG_HIP(hipMemcpyAsync(prim, staging, sizeof(double) * NF * cells, hipMemcpyDeviceToHost, ctx->stream));
+        VG_HIP(hipStreamSynchronize(ctx->stream));
+        return VG_OK;
+    }
+
+    // Wave-speed maximum of the current grid into clk->smax (and the corner term).
+    int refresh_max(int bc, const typename P::Params &m)
+    {
+        const int blocks = blocks_for(cells) < 2048 ? blocks_for(cells) : 2048;
+        VG_HIP(hipMemsetAsync(&clk->smax, 0, sizeof(unsigned long long), ctx->stream));
+        wave_max_kernel<P><<<blocks, 256, 0, ctx->stream>>>(buf[cur], plane, cells, dim, m, clk);
+        int rc = launch_status("wave_max");
+        if (rc) return rc;
+        if (dim == 2 && bc == VG_BC_TRANSMISSIVE) {
+            corner_speed_kernel<P><<<1, 1, 0, ctx->stream>>>(m, dim, clk);
+            rc = launch_status("corner_speed");
+            if (rc) return rc;
+        } else {
+            VG_HIP(hipMemsetAsync(&clk->corner, 0, sizeof(double), ctx->stream));
+        }
+        max_valid = true;
+        last_params = m;
+        last_bc = bc;
+        return VG_OK;
+    }
+
+    template <int ORDER, int MODE>
+    int launch_sweep(int dir, const double *in, double *out, double dt_scale, int fuse, int bc, double dx,
+                     double bias, int limiter, const typename P::Params &m)
+    {
+        SweepArgs<P> a;
+        a.in = in;
+        a.out = out;
+        a.plane = plane;
+        a.ny = ny;
+        a.nx = nx;
+        a.bc = bc;
+        a.limiter = limiter;
+        a.dx = dx;
+        a.bias = bias;
+        a.dt_scale = dt_scale;
+        a.dim = dim;
+        a.fuse_max = fuse;
+        a.m = m;
+        a.clk = clk;
+        if (dir == 0) {
+            constexpr int G = (ORDER == 2) ? 2 : 1;
+            dim3 grid((nx + (kBSX - 2 * G) - 1) / (kBSX - 2 * G), ny);
+            return ctx->timed(ORDER == 2 ? "sweep_x_slic" : "sweep_x_force", [&] {
+                sweep_x_kernel<P, ORDER, MODE, kBSX><<<grid, kBSX, 0, ctx->stream>>>(a);
+            });
+        }
+        dim3 grid((nx + kBSY - 1) / kBSY, (ny + kHY - 1) / kHY);
+        return ctx->timed(ORDER == 2 ? "sweep_y_slic" : "sweep_y_force", [&] {
+            sweep_y_kernel<P, ORDER, MODE, kBSY, kHY><<<grid, kBSY, 0, ctx->stream>>>(a);
+        });
+    }
+
+    template <int ORDER, int MODE>
+    int enqueue_step(int bc, double dx, double bias, int limiter, const typename P::Params &m, double *dt_log, int dt_cap)
+    {
+        clock_begin_kernel<<<1, 1, 0, ctx->stream>>>(clk, dt_log, dt_cap);
+        int rc = launch_status("clock_begin");
+        if (rc) return rc;
+        double *a = buf[cur], *b = buf[1 - cur];
+        if (dim == 1) {
+            rc = launch_sweep<ORDER, MODE>(0, a, b, 1.0, 1, bc, dx, bias, limiter, m);
+        } else {
+            rc = launch_sweep<ORDER, MODE>(0, a, b, 0.5, 0, bc, dx, bias, limiter, m);
+            if (!rc) rc = launch_sweep<ORDER, MODE>(1, b, a, 1.0, 0, bc, dx, bias, limiter, m);
+            if (!rc) rc = launch_sweep<ORDER, MODE>(0, a, b, 0.5, 1, bc, dx, bias, limiter, m);
+        }
+        if (rc) return launch_status("sweep");
+        cur = 1 - cur;
+        return VG_OK;
+    }
+
+    template <int ORDER, int MODE>
+    int run_impl(double dx, double cfl, double t_final, int max_steps, double bias, int limiter, int bc,
+                 const typename P::Params &m, double *dt_log, int dt_cap, vg_run_stats *stats)
+    {
+        auto w0 = std::chrono::steady_clock::now();
+        if (!max_valid || last_bc != bc || std::memcmp(&last_params, &m, sizeof(m)) != 0) {
+            int rc = refresh_max(bc, m);
+            if (rc) return rc;
+        }
+        Clock h;
+        VG_HIP(hipMemcpyAsync(&h, clk, sizeof(Clock), hipMemcpyDeviceToHost, ctx->stream));
+        VG_HIP(hipStreamSynchronize(ctx->stream));
+        const int start_it = h.it;
+        h.t_final = t_final;
+        h.cfl = cfl;
+        h.dx = dx;
+        h.max_steps = (max_steps > 0) ? start_it + max_steps : 0;
+        h.done = 0;
+        h.pending = 0;
+        h.steps_this_run = 0;
+        h.dt_min = h.dt_max = 0.0;
+        VG_HIP(hipMemcpyAsync(clk, &h, sizeof(Clock), hipMemcpyHostToDevice, ctx->stream));
+        if (dt_log && dt_cap > 0 && dt_dev_cap < dt_cap) {
+            hipFree(dt_dev);
+            dt_dev = nullptr;
+            VG_HIP(hipMalloc(&dt_dev, sizeof(double) * dt_cap));
+            dt_dev_cap = dt_cap;
+        }
+        double *dlog = (dt_log && dt_cap > 0) ? dt_dev : nullptr;
+
+        hipEvent_t e0, e1;
+        VG_HIP(hipEventCreate(&e0));
+        VG_HIP(hipEventCreate(&e1));
+        VG_HIP(hipEventRecord(e0, ctx->stream));
+        const int start_cur = cur;
+        const int chunk = (max_steps > 0 && max_steps <= 4096) ? max_steps : 256;
+        for (;;) {
+            for (int s = 0; s < chunk; s++) {
+                int rc = enqueue_step<ORDER, MODE>(bc, dx, bias, limiter, m, dlog, dt_cap);
+                if (rc) return rc;
+            }
+            VG_HIP(hipMemcpyAsync(&h, clk, sizeof(Clock), hipMemcpyDeviceToHost, ctx->stream));
+            VG_HIP(hipStreamSynchronize(ctx->stream));
+            if (h.done) break;
+            if (max_steps > 0 && h.steps_this_run >= max_steps) break;
+        }
+        clock_finish_kernel<<<1, 1, 0, ctx->stream>>>(clk);
+        int rc = launch_status("clock_finish");
+        if (rc) return rc;
+        VG_HIP(hipEventRecord(e1, ctx->stream));
+        VG_HIP(hipMemcpyAsync(&h, clk, sizeof(Clock), hipMemcpyDeviceToHost, ctx->stream));
+        VG_HIP(hipStreamSynchronize(ctx->stream));
+        float dev_ms = 0.0f;
+        VG_HIP(hipEventElapsedTime(&dev_ms, e0, e1));
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        // Each executed step moved the grid to the other buffer; steps enqueued
+        // after the loop condition failed did nothing.
+        cur = start_cur ^ (h.steps_this_run & 1);
+        if (dlog) {
+            const int n = h.steps_this_run < dt_cap ? h.steps_this_run : dt_cap;
+            if (n > 0) VG_HIP(hipMemcpy(dt_log, dt_dev, sizeof(double) * n, hipMemcpyDeviceToHost));
+        }
+        if (ctx->timing) {
+            rc = ctx->resolve_timing();
+            if (rc) return rc;
+        }
+        if (stats) {
+            stats->steps = h.steps_this_run;
+            stats->total_steps = h.it;
+            stats->t = h.t;
+            stats->dt_min = h.dt_min;
+            stats->dt_max = h.dt_max;
+            stats->device_ms = dev_ms;
+            stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+            stats->nonfinite_cells = 0;
+            stats->first_nonfinite = -1;
+        }
+        return VG_OK;
+    }
+
+    int run(int order, int mode, double dx, double cfl, double t_final, int max_steps, double bias, int limiter,
+            int bc, const typename P::Params &m, double *dt_log, int dt_cap, vg_run_stats *stats)
+    {
+        if (order == 2 && mode == 0) return run_impl<2, 0>(dx, cfl, t_final, max_steps, bias, limiter, bc, m, dt_log, dt_cap, stats);
+        if (order == 2 && mode == 1) return run_impl<2, 1>(dx, cfl, t_final, max_steps, bias, limiter, bc, m, dt_log, dt_cap, stats);
+        if (order == 1 && mode == 0) return run_impl<1, 0>(dx, cfl, t_final, max_steps, bias, limiter, bc, m, dt_log, dt_cap, stats);
+        if (order == 1 && mode == 1) return run_impl<1, 1>(dx, cfl, t_final, max_steps, bias, limiter, bc, m, dt_log, dt_cap, stats);
+        set_error("run: order must be 1 or 2 and mode 0 or 1");
+        return VG_ERR_ARG;
+    }
+
+    // Non-finite / non-positive-density scan of the current grid.
+    int scan_nonfinite(int64_t *count, int64_t *first)
+    {
+        unsigned long long init[2] = {0ull, ~0ull};
+        VG_HIP(hipMemcpyAsync(scan, init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
+        nonfinite_kernel<NF><<<blocks_for(cells), 256, 0, ctx->stream>>>(buf[cur], plane, cells, scan, scan + 1);
+        int rc = launch_status("nonfinite");
+        if (rc) return rc;
+        unsigned long long res[2];
+        VG_HIP(hipMemcpyAsync(res, scan, sizeof(res), hipMemcpyDeviceToHost, ctx->stream));
+        VG_HIP(hipStreamSynchronize(ctx->stream));
+        *count = (int64_t)res[0];
+        *first = res[0] ? (int64_t)res[1] : -1;
+        return VG_OK;
+    }
+};
+
+}  // namespace vg

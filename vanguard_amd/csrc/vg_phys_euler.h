@@ -1,0 +1,151 @@
+// Euler equations with a stiffened-gas EOS and a reaction-progress variable:
+// the physics of the reference's EulerStateVector, as device functions on
+// register arrays.  Primitive order (rho, u, v, w, p, lambda); conserved order
+// (rho, rho u, rho v, rho w, E, rho lambda) -- Euler/eulerstatevector.h:55-61.
+#pragma once
+
+#include "vg_device.h"
+
+namespace vg {
+
+struct EulerPhys {
+    static constexpr int NF = 6;
+    static constexpr const char *name = "euler";
+
+    // EulerMaterialParameters (Euler/eulermaterialparameters.h:30-36).
+    struct Params {
+        double gamma, p_inf, cv, e_form, t_ign, k_rate;
+    };
+
+    // Specific internal energy, pressure and sound speed of the stiffened gas
+    // (Euler/eulerequationofstate.cpp:7-29).
+    __device__ static __forceinline__ double eos_e(const Params &m, double rho, double p)
+    {
+        return (p + (m.gamma * m.p_inf)) / ((m.gamma - 1) * rho);
+    }
+    __device__ static __forceinline__ double eos_p(const Params &m, double rho, double e)
+    {
+        return (e * (m.gamma - 1) * rho) - (m.gamma * m.p_inf);
+    }
+    __device__ static __forceinline__ double eos_c(const Params &m, double rho, double p)
+    {
+        return sqrt((m.gamma * (p + m.p_inf)) / rho);
+    }
+
+    // computeConservedVariableVector + computeTotalEnergy
+    // (Euler/eulerstatevector.cpp:78-91, 192-198).
+    __device__ static __forceinline__ void prim_to_cons(const Params &m, const double (&w)[NF], double (&U)[NF])
+    {
+        const double rho = w[0], u = w[1], v = w[2], z = w[3], p = w[4], lam = w[5];
+        const double vsq = (u * u) + (v * v) + (z * z);
+        U[0] = rho;
+        U[1] = rho * u;
+        U[2] = rho * v;
+        U[3] = rho * z;
+        U[4] = rho * ((0.5 * vsq) + eos_e(m, rho, p) + (m.e_form * (lam - 1.0)));
+        U[5] = rho * lam;
+    }
+
+    // setConservedVariableVector (Euler/eulerstatevector.cpp:47-61).
+    __device__ static __forceinline__ void cons_to_prim(const Params &m, const double (&U)[NF], double (&w)[NF])
+    {
+        const double rho = U[0];
+        const double u = U[1] / rho, v = U[2] / rho, z = U[3] / rho;
+        const double lam = U[5] / rho;
+        const double vsq = (u * u) + (v * v) + (z * z);
+        const double e = (U[4] / rho) - (0.5 * vsq) - (m.e_form * (lam - 1.0));
+        w[0] = rho;
+        w[1] = u;
+        w[2] = v;
+        w[3] = z;
+        w[4] = eos_p(m, rho, e);
+        w[5] = lam;
+    }
+
+    // computeXFluxVector / computeYFluxVector of a conserved vector
+    // (Euler/eulerstatevector.cpp:93-120, 127-154).
+    template <int DIR>
+    __device__ static __forceinline__ void flux(const Params &m, const double (&U)[NF], double (&F)[NF])
+    {
+        const double rho = U[0];
+        const double u = U[1] / rho, v = U[2] / rho, z = U[3] / rho;
+        const double lam = U[5] / rho;
+        const double vsq = (u * u) + (v * v) + (z * z);
+        const double et = U[4] / rho;
+        const double e = et - (0.5 * vsq) - (m.e_form * (lam - 1.0));
+        const double p = eos_p(m, rho, e);
+        if (DIR == 0) {
+            F[0] = rho * u;
+            F[1] = (rho * (u * u)) + p;
+            F[2] = rho * (u * v);
+            F[3] = rho * (u * z);
+            F[4] = (rho * (u * et)) + (u * p);
+            F[5] = rho * (u * lam);
+        } else {
+            F[0] = rho * v;
+            F[1] = rho * (v * u);
+            F[2] = (rho * (v * v)) + p;
+            F[3] = rho * (v * z);
+            F[4] = (rho * (v * et)) + (v * p);
+            F[5] = rho * (v * lam);
+        }
+    }
+
+    // Wave speed of Solvers::computeMaximumWaveSpeed (1D, |u| + c) and
+    // computeMaximumWaveSpeed2D (max(|u|, |v|) + c) (Solvers/solvers.cpp:267, 329).
+    __device__ static __forceinline__ double wave_speed(const Params &m, const double (&w)[NF], int dim)
+    {
+        const double a = (dim == 1) ? fabs(w[1]) : smax(fabs(w[1]), fabs(w[2]));
+        return a + eos_c(m, w[0], w[4]);
+    }
+
+    // EulerStateVector() (Euler/eulerstatevector.cpp:3-12): fills the 2D corner ghosts.
+    __device__ static __forceinline__ void default_state(double (&w)[NF])
+    {
+        w[0] = 1.0; w[1] = 0.0; w[2] = 0.0; w[3] = 0.0; w[4] = 1.0; w[5] = 1.0;
+    }
+
+    // ---- FAST mode ----------------------------------------------------------
+    // Same scheme, fewer divisions: one reciprocal of rho per state and the
+    // EOS constants folded.  Not bit-identical to the reference.
+    __device__ static __forceinline__ void prim_to_cons_fast(const Params &m, const double (&w)[NF], double (&U)[NF])
+    {
+        const double rho = w[0], u = w[1], v = w[2], z = w[3], p = w[4], lam = w[5];
+        const double inv_gm1 = 1.0 / (m.gamma - 1);
+        U[0] = rho;
+        U[1] = rho * u;
+        U[2] = rho * v;
+        U[3] = rho * z;
+        U[4] = 0.5 * rho * ((u * u) + (v * v) + (z * z)) + (p + m.gamma * m.p_inf) * inv_gm1
+               + rho * (m.e_form * (lam - 1.0));
+        U[5] = rho * lam;
+    }
+    __device__ static __forceinline__ void cons_to_prim_fast(const Params &m, const double (&U)[NF], double (&w)[NF])
+    {
+        const double rho = U[0], r = 1.0 / rho;
+        const double u = U[1] * r, v = U[2] * r, z = U[3] * r, lam = U[5] * r;
+        const double ke = 0.5 * ((U[1] * u) + (U[2] * v) + (U[3] * z));
+        w[0] = rho;
+        w[1] = u;
+        w[2] = v;
+        w[3] = z;
+        w[4] = (m.gamma - 1) * (U[4] - ke - m.e_form * (U[5] - rho)) - m.gamma * m.p_inf;
+        w[5] = lam;
+    }
+    // Flux from a primitive state (no conserved round trip).
+    template <int DIR>
+    __device__ static __forceinline__ void flux_prim_fast(const Params &m, const double (&w)[NF], const double (&U)[NF],
+                                                          double (&F)[NF])
+    {
+        const double vn = (DIR == 0) ? w[1] : w[2];
+        const double p = w[4];
+        F[0] = U[0] * vn;
+        F[1] = U[1] * vn + (DIR == 0 ? p : 0.0);
+        F[2] = U[2] * vn + (DIR == 1 ? p : 0.0);
+        F[3] = U[3] * vn;
+        F[4] = (U[4] + p) * vn;
+        F[5] = U[5] * vn;
+    }
+};
+
+}  // namespace vg

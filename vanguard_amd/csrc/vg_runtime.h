@@ -1,0 +1,75 @@
+// Host-side runtime shared by the solvers: the context (device, stream,
+// kernel-timing table), error reporting, and launch helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/vanguard.h"
+
+namespace vg {
+
+void set_error(const std::string &msg);
+void clear_error();
+
+#define VG_HIP(call)                                                                        \
+    do {                                                                                    \
+        hipError_t vg_e_ = (call);                                                          \
+        if (vg_e_ != hipSuccess) {                                                          \
+            ::vg::set_error(std::string(#call) + ": " + hipGetErrorString(vg_e_));          \
+            return VG_ERR_HIP;                                                              \
+        }                                                                                   \
+    } while (0)
+
+#define VG_CHECK(cond, msg)                                                                 \
+    do {                                                                                    \
+        if (!(cond)) {                                                                      \
+            ::vg::set_error(msg);                                                           \
+            return VG_ERR_ARG;                                                              \
+        }                                                                                   \
+    } while (0)
+
+struct KernelTimer {
+    int64_t launches = 0;
+    double total_ms = 0.0;
+};
+
+}  // namespace vg
+
+struct vg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    std::map<std::string, vg::KernelTimer> timers;
+    // Event pairs recorded around launches and not yet resolved.
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> event_pool;
+
+    hipEvent_t take_event();
+    // Brackets a launch with events when timing is on.
+    template <class F>
+    int timed(const char *name, F &&launch)
+    {
+        if (!timing) {
+            launch();
+            return hipPeekAtLastError() == hipSuccess ? VG_OK : VG_ERR_HIP;
+        }
+        hipEvent_t a = take_event(), b = take_event();
+        hipEventRecord(a, stream);
+        launch();
+        hipEventRecord(b, stream);
+        pending.push_back({name, {a, b}});
+        return hipPeekAtLastError() == hipSuccess ? VG_OK : VG_ERR_HIP;
+    }
+    // Synchronises the stream and folds the recorded pairs into the table.
+    int resolve_timing();
+};
+
+namespace vg {
+// Makes the context's device current; returns VG_ERR_HIP on failure.
+int bind(vg_ctx *ctx);
+int launch_status(const char *what);
+}  // namespace vg

@@ -1,0 +1,130 @@
+// Context, errors and kernel timing.
+#include "vg_runtime.h"
+
+#include <cstring>
+
+namespace vg {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+void clear_error() { g_last_error.clear(); }
+
+int bind(vg_ctx *ctx)
+{
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) {
+        set_error(std::string("hipSetDevice: ") + hipGetErrorString(e));
+        return VG_ERR_HIP;
+    }
+    return VG_OK;
+}
+
+int launch_status(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string(what) + ": " + hipGetErrorString(e));
+        return VG_ERR_HIP;
+    }
+    return VG_OK;
+}
+
+}  // namespace vg
+
+hipEvent_t vg_ctx::take_event()
+{
+    if (!event_pool.empty()) {
+        hipEvent_t e = event_pool.back();
+        event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    hipEventCreate(&e);
+    return e;
+}
+
+int vg_ctx::resolve_timing()
+{
+    VG_HIP(hipStreamSynchronize(stream));
+    for (auto &p : pending) {
+        float ms = 0.0f;
+        VG_HIP(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+        auto &t = timers[p.first];
+        t.launches += 1;
+        t.total_ms += ms;
+        event_pool.push_back(p.second.first);
+        event_pool.push_back(p.second.second);
+    }
+    pending.clear();
+    return VG_OK;
+}
+
+extern "C" {
+
+const char *vg_last_error(void) { return vg::g_last_error.c_str(); }
+
+const char *vg_version(void) { return "vanguard-mi355x 0.1 (gfx950)"; }
+
+int vg_ctx_create(vg_ctx **out, int device)
+{
+    vg::clear_error();
+    VG_CHECK(out != nullptr, "vg_ctx_create: null output pointer");
+    int count = 0;
+    VG_HIP(hipGetDeviceCount(&count));
+    VG_CHECK(device >= 0 && device < count, "vg_ctx_create: no such device");
+    vg_ctx *ctx = new vg_ctx();
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        vg::set_error(std::string("vg_ctx_create: ") + hipGetErrorString(e));
+        delete ctx;
+        return VG_ERR_HIP;
+    }
+    *out = ctx;
+    return VG_OK;
+}
+
+void vg_ctx_destroy(vg_ctx *ctx)
+{
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (auto &p : ctx->pending) {
+        hipEventDestroy(p.second.first);
+        hipEventDestroy(p.second.second);
+    }
+    for (hipEvent_t e : ctx->event_pool) hipEventDestroy(e);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int vg_ctx_set_timing(vg_ctx *ctx, int enabled)
+{
+    VG_CHECK(ctx != nullptr, "vg_ctx_set_timing: null context");
+    ctx->timing = enabled != 0;
+    return VG_OK;
+}
+
+int vg_ctx_kernel_time(vg_ctx *ctx, const char *kernel, int64_t *launches, double *total_ms)
+{
+    VG_CHECK(ctx != nullptr && kernel != nullptr, "vg_ctx_kernel_time: null argument");
+    int rc = ctx->resolve_timing();
+    if (rc) return rc;
+    auto it = ctx->timers.find(kernel);
+    if (launches) *launches = (it == ctx->timers.end()) ? 0 : it->second.launches;
+    if (total_ms) *total_ms = (it == ctx->timers.end()) ? 0.0 : it->second.total_ms;
+    return VG_OK;
+}
+
+int vg_ctx_reset_timing(vg_ctx *ctx)
+{
+    VG_CHECK(ctx != nullptr, "vg_ctx_reset_timing: null context");
+    int rc = ctx->resolve_timing();
+    if (rc) return rc;
+    ctx->timers.clear();
+    return VG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,158 @@
+// The SLIC / FORCE building blocks, generic over the physics P (Euler, MHD,
+// HPR) and the sweep direction, on register arrays.  EXACT mode follows the
+// reference's operation order (see the citations); FAST mode is the same scheme
+// with the primitive/conserved round trips removed.
+#pragma once
+
+#include "vg_device.h"
+
+namespace vg {
+
+enum { MODE_EXACT = 0, MODE_FAST = 1 };
+
+// SlopeLimiters::computeRCoefficient and computeSlopeLimiter
+// (Solvers/slopelimiters.cpp:7-74): 0 superbee, 1 van Leer, anything else minbee.
+__device__ __forceinline__ double r_coefficient(double r, double bias)
+{
+    return 2.0 / ((1.0 - bias) + ((1.0 + bias) * r));
+}
+
+__device__ __forceinline__ double limiter_value(double r, double bias, int limiter)
+{
+    if (r < 0.0) return 0.0;
+    if (limiter == 0) {
+        if (r < 0.5) return 2.0 * r;
+        if (r < 1.0) return 1.0;
+        return smin(smin(r, r_coefficient(r, bias)), 2.0);
+    }
+    if (limiter == 1) return smin((2.0 * r) / (1.0 + r), r_coefficient(r, bias));
+    if (r < 1.0) return r;
+    return smin(1.0, r_coefficient(r, bias));
+}
+
+// Conserved state plus its physical flux in the sweep direction.
+template <int NF>
+struct ConsFlux {
+    double U[NF];
+    double F[NF];
+};
+
+template <class P, int DIR, int MODE>
+struct Scheme {
+    static constexpr int NF = P::NF;
+    using Params = typename P::Params;
+
+    __device__ static __forceinline__ void cons(const Params &m, const double (&w)[NF], double (&U)[NF])
+    {
+        if (MODE == MODE_EXACT) P::prim_to_cons(m, w, U);
+        else P::prim_to_cons_fast(m, w, U);
+    }
+
+    // Flux of a conserved vector.
+    __device__ static __forceinline__ void flux_of_cons(const Params &m, const double (&U)[NF], double (&F)[NF])
+    {
+        if (MODE == MODE_EXACT) {
+            P::template flux<DIR>(m, U, F);
+        } else {
+            double w[NF];
+            P::cons_to_prim_fast(m, U, w);
+            P::template flux_prim_fast<DIR>(m, w, U, F);
+        }
+    }
+
+    // Conserved vector and flux of a cell given as primitives.  EXACT: the
+    // reference's state-overload flux, computeXFluxVector(computeConserved...)
+    // (Euler/eulerstatevector.cpp:122-125).
+    __device__ static __forceinline__ void cons_flux_of_prim(const Params &m, const double (&w)[NF], ConsFlux<NF> &cf)
+    {
+        cons(m, w, cf.U);
+        if (MODE == MODE_EXACT) P::template flux<DIR>(m, cf.U, cf.F);
+        else P::template flux_prim_fast<DIR>(m, w, cf.U, cf.F);
+    }
+
+    // MUSCL-Hancock half-step evolution of the middle cell's two boundary
+    // extrapolations: SlopeLimiters::computeSlopeVector (slopelimiters.cpp:217-246)
+    // then Solvers::evolveStateByHalfX/YTimeStep (Solvers/solvers.cpp:454-466,
+    // 502-514) with computeEvolutionVector (:601-609) and evolveStateByHalfTimeStep
+    // (:550-565).  Produces, for the SLIC face fluxes, the conserved vector and
+    // flux of both evolved states: side 0 (left) and side 1 (right).
+    // half_c = 0.5 * (ts / dx).
+    __device__ static __forceinline__ void evolve(const Params &m, const double (&UL)[NF], const double (&UM)[NF],
+                                                  const double (&UR)[NF], double bias, int limiter, double half_c,
+                                                  ConsFlux<NF> &left, ConsFlux<NF> &right)
+    {
+        double xl[NF], xr[NF];
+        const double cl = 0.5 * (1.0 + bias), cr = 0.5 * (1.0 - bias);
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+            const double dl = UM[k] - UL[k];
+            const double dr = UR[k] - UM[k];
+            double d = (cl * dl) + (cr * dr);
+            // |difference| < 1e-5 is replaced by +1e-5 (sign dropped) before the ratio.
+            const double num = (fabs(dl) < 1e-5) ? 1e-5 : dl;
+            const double den = (fabs(dr) < 1e-5) ? 1e-5 : dr;
+            d *= limiter_value(num / den, bias, limiter);
+            xl[k] = UM[k] - (0.5 * d);
+            xr[k] = UM[k] + (0.5 * d);
+        }
+        double Fl[NF], Fr[NF];
+        flux_of_cons(m, xl, Fl);
+        flux_of_cons(m, xr, Fr);
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+            const double evo = half_c * (Fl[k] - Fr[k]);
+            xl[k] = xl[k] + evo;
+            xr[k] = xr[k] + evo;
+        }
+        if (MODE == MODE_EXACT) {
+            // setConservedVariableVector, then the FORCE flux converts the state
+            // back: computeConservedVariableVector + state-overload flux.
+            double w[NF];
+            P::cons_to_prim(m, xl, w);
+            cons_flux_of_prim(m, w, left);
+            P::cons_to_prim(m, xr, w);
+            cons_flux_of_prim(m, w, right);
+        } else {
+            double w[NF];
+#pragma unroll
+            for (int k = 0; k < NF; k++) { left.U[k] = xl[k]; right.U[k] = xr[k]; }
+            P::cons_to_prim_fast(m, xl, w);
+            P::template flux_prim_fast<DIR>(m, w, left.U, left.F);
+            P::cons_to_prim_fast(m, xr, w);
+            P::template flux_prim_fast<DIR>(m, w, right.U, right.F);
+        }
+    }
+
+    // FORCE flux between two states: FirstOrderSolver::computeLaxFriedrichsFlux
+    // (firstordersolver.cpp:79-85), computeRichtmyerFlux (:165-171) followed by the
+    // flux of the intermediate state (:96-97), and computeFORCEFlux (:227-230).
+    // lf_c = dx / ts, ri_c = ts / dx.
+    __device__ static __forceinline__ void force_face(const Params &m, const ConsFlux<NF> &a, const ConsFlux<NF> &b,
+                                                      double lf_c, double ri_c, double (&F)[NF])
+    {
+        double us[NF], lf[NF];
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+            lf[k] = 0.5 * ((a.F[k] + b.F[k]) + (lf_c * (a.U[k] - b.U[k])));
+            us[k] = 0.5 * ((a.U[k] + b.U[k]) + (ri_c * (a.F[k] - b.F[k])));
+        }
+        double ri[NF];
+        flux_of_cons(m, us, ri);
+#pragma unroll
+        for (int k = 0; k < NF; k++) F[k] = 0.5 * (lf[k] + ri[k]);
+    }
+
+    // FirstOrderSolver::computeFORCEUpdate (firstordersolver.cpp:428-433) followed
+    // by setConservedVariableVector.  up_c = ts / dx.
+    __device__ static __forceinline__ void update(const Params &m, const double (&U)[NF], const double (&Fm)[NF],
+                                                  const double (&Fp)[NF], double up_c, double (&w)[NF])
+    {
+        double N[NF];
+#pragma unroll
+        for (int k = 0; k < NF; k++) N[k] = U[k] + (up_c * (Fm[k] - Fp[k]));
+        if (MODE == MODE_EXACT) P::cons_to_prim(m, N, w);
+        else P::cons_to_prim_fast(m, N, w);
+    }
+};
+
+}  // namespace vg

@@ -1,0 +1,95 @@
+"""Initial-condition builders for the benchmark and test problems.
+
+Host-side NumPy code that produces primitive-state arrays in the reference's
+layout: arrays of structures, fields last, row-major with x (column j)
+contiguous, as ``std::vector<std::vector<EulerStateVector>>`` holds them
+(Euler field order rho, u, v, w, p, lambda: Euler/eulerstatevector.h:55-61).
+
+The random draws use splitmix64 so that every problem is reproducible from its
+seed on any machine.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+_MASK = (1 << 64) - 1
+
+
+def splitmix64(seed: int, count: int) -> np.ndarray:
+    """``count`` successive splitmix64 outputs (uint64) from ``seed``."""
+    out = np.empty(count, dtype=np.uint64)
+    state = seed & _MASK
+    for k in range(count):
+        state = (state + 0x9E3779B97F4A7C15) & _MASK
+        z = state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _MASK
+        out[k] = z ^ (z >> 31)
+    return out
+
+
+def uniform01(seed: int, count: int) -> np.ndarray:
+    """Doubles in [0, 1) from the top 53 bits of splitmix64 draws."""
+    u = splitmix64(seed, count)
+    return (u >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+
+def sod_1d(n: int) -> np.ndarray:
+    """Toro test 1 / Sod shock tube on n cells, discontinuity at i <= n/2.
+
+    Same cell classification as EulerTests::solveToroTest1
+    (Euler/eulertests.cpp:14-24): left (1, 0, 0, 0, 1, 1), right
+    (0.125, 0, 0, 0, 0.1, 1).
+    """
+    w = np.empty((n, 6))
+    left = np.arange(n) <= 0.5 * n
+    w[left] = (1.0, 0.0, 0.0, 0.0, 1.0, 1.0)
+    w[~left] = (0.125, 0.0, 0.0, 0.0, 0.1, 1.0)
+    return w
+
+
+def disc_2d(n: int, gaussians: int = 64, seed: int = 20261202, amplitude: float = 1e-3,
+            width: float = 0.02, length: float = 2.0, ny: int | None = None) -> np.ndarray:
+    """Cylindrical explosion on an n x n grid over [0, length]^2, with Gaussian bumps.
+
+    The disc follows EulerTests::solve2DToroTest1 (Euler/eulertests.cpp:36-49):
+    cell (i, j) is inside when sqrt((i - n/2)^2 + (j - n/2)^2) <= 0.2 n, giving
+    (rho, p) = (1, 1) inside and (0.125, 0.1) outside, at rest, lambda = 1.
+
+    ``gaussians`` density bumps rho += amplitude * exp(-|x - c_k|^2 / (2 width^2))
+    are added at centres c_k drawn from splitmix64(seed): draws 2k and 2k+1 give
+    (x_k, y_k) = length * (u >> 11) * 2^-53.  Cell centres are
+    ((j + 1/2) dx, (i + 1/2) dx).  Each bump is evaluated as the separable product
+    exp(-(x-cx)^2/(2w^2)) * exp(-(y-cy)^2/(2w^2)) inside a window of 12 widths
+    (outside it the bump is below 1e-31 and cannot change a double of the sum).
+    """
+    ny = n if ny is None else ny
+    nx = n
+    dx = length / nx
+    w = np.empty((ny, nx, 6))
+    i = np.arange(ny, dtype=np.float64)[:, None]
+    j = np.arange(nx, dtype=np.float64)[None, :]
+    r = np.sqrt((i - 0.5 * ny) * (i - 0.5 * ny) + (j - 0.5 * nx) * (j - 0.5 * nx))
+    inside = r <= 0.2 * n
+    w[..., 1:4] = 0.0
+    w[..., 5] = 1.0
+    w[..., 0] = np.where(inside, 1.0, 0.125)
+    w[..., 4] = np.where(inside, 1.0, 0.1)
+    if gaussians:
+        c = uniform01(seed, 2 * gaussians) * length
+        bump = np.zeros((ny, nx))
+        xc = (np.arange(nx) + 0.5) * dx
+        yc = (np.arange(ny) + 0.5) * dx
+        half = 12.0 * width
+        for k in range(gaussians):
+            cx, cy = c[2 * k], c[2 * k + 1]
+            jx = np.nonzero(np.abs(xc - cx) <= half)[0]
+            iy = np.nonzero(np.abs(yc - cy) <= half)[0]
+            if jx.size == 0 or iy.size == 0:
+                continue
+            gx = np.exp(-((xc[jx] - cx) ** 2) / (2.0 * width * width))
+            gy = np.exp(-((yc[iy] - cy) ** 2) / (2.0 * width * width))
+            bump[iy[0]:iy[-1] + 1, jx[0]:jx[-1] + 1] += np.outer(gy, gx)
+        w[..., 0] += amplitude * bump
+    return w

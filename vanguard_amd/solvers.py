@@ -1,0 +1,207 @@
+"""Python mirror of the reference's solver interface, over the C ABI.
+
+Same names and argument meaning as the reference's static C++ methods
+(Solvers/secondordersolver.h:55-65, Solvers/firstordersolver.h:87-95), with
+states held as NumPy arrays of primitives in the reference's layout (cells x 6,
+or rows x columns x 6).  The arithmetic runs in the HIP kernels of
+libvanguard_hip.so; this module only moves arguments.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+EXACT = _lib.VG_MODE_EXACT
+FAST = _lib.VG_MODE_FAST
+TRANSMISSIVE = _lib.VG_BC_TRANSMISSIVE
+PERIODIC = _lib.VG_BC_PERIODIC
+
+
+class Context:
+    """A vg_ctx on one HIP device (a stream plus solver scratch)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        self.handle = C.c_void_p()
+        check(self.lib.vg_ctx_create(C.byref(self.handle), device), "vg_ctx_create")
+
+    def close(self):
+        if self.handle:
+            self.lib.vg_ctx_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_timing(self, on: bool):
+        check(self.lib.vg_ctx_set_timing(self.handle, int(on)), "vg_ctx_set_timing")
+
+    def kernel_time(self, name: str):
+        n = C.c_int64()
+        ms = C.c_double()
+        check(self.lib.vg_ctx_kernel_time(self.handle, name.encode(), C.byref(n), C.byref(ms)), "vg_ctx_kernel_time")
+        return n.value, ms.value
+
+    def reset_timing(self):
+        check(self.lib.vg_ctx_reset_timing(self.handle), "vg_ctx_reset_timing")
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+@dataclass
+class EulerMaterialParameters:
+    """EulerMaterialParameters (Euler/eulermaterialparameters.h:5-37); the
+    one-argument constructor's defaults (eulermaterialparameters.cpp:14-23)."""
+
+    gamma: float = 1.0
+    p_inf: float = 0.0
+    cv: float = 25.0
+    e_form: float = 0.0
+    t_ign: float = 0.25
+    k_rate: float = 250.0
+
+    def c_struct(self) -> _lib.EulerParams:
+        return _lib.EulerParams(self.gamma, self.p_inf, self.cv, self.e_form, self.t_ign, self.k_rate)
+
+
+@dataclass
+class SolveResult:
+    cells: np.ndarray
+    dt: np.ndarray
+    stats: _lib.RunStats
+    status: int
+
+
+def euler_solve(prim: np.ndarray, dim: int, dx: float, cfl: float, t_final: float, *, order: int = 2,
+                bias: float = 0.0, limiter: int = 0, params: EulerMaterialParameters | None = None,
+                max_steps: int = 0, mode: int = EXACT, bc: int = TRANSMISSIVE, dt_cap: int = 100000,
+                ctx: Context | None = None, raise_on_error: bool = True) -> SolveResult:
+    """Whole solve through vg_euler_solve_1d / vg_euler_solve_2d."""
+    ctx = ctx or default_context()
+    params = params or EulerMaterialParameters(1.4)
+    w = np.ascontiguousarray(prim, dtype=np.float64)
+    out = np.empty_like(w)
+    dts = np.empty(dt_cap)
+    st = _lib.RunStats()
+    mp = params.c_struct()
+    if dim == 1:
+        if w.ndim != 2 or w.shape[1] != 6:
+            raise ValueError("1D state must be (cells, 6)")
+        rc = ctx.lib.vg_euler_solve_1d(ctx.handle, _lib.as_dptr(w), _lib.as_dptr(out), w.shape[0], dx, cfl, t_final,
+                                       max_steps, bias, limiter, order, bc, C.byref(mp), mode, _lib.as_dptr(dts),
+                                       dt_cap, C.byref(st))
+    else:
+        if w.ndim != 3 or w.shape[2] != 6:
+            raise ValueError("2D state must be (rows, columns, 6)")
+        rc = ctx.lib.vg_euler_solve_2d(ctx.handle, _lib.as_dptr(w), _lib.as_dptr(out), w.shape[0], w.shape[1], dx,
+                                       cfl, t_final, max_steps, bias, limiter, order, bc, C.byref(mp), mode,
+                                       _lib.as_dptr(dts), dt_cap, C.byref(st))
+    if raise_on_error and rc != _lib.VG_OK:
+        check(rc, "vg_euler_solve")
+    return SolveResult(out, dts[:min(st.steps, dt_cap)].copy(), st, rc)
+
+
+class SecondOrderSolver:
+    """SLIC (MUSCL-Hancock + FORCE), Solvers/secondordersolver.h:7-72."""
+
+    @staticmethod
+    def solve(initial_cells, cell_spacing, cfl_coefficient, final_time, bias, slope_limiter,
+              subcycling_iterations, material_parameters, **kw):
+        # subcyclingIterations is an empty loop for Euler (secondordersolver.cpp:302-305).
+        return euler_solve(initial_cells, 1, cell_spacing, cfl_coefficient, final_time, order=2, bias=bias,
+                           limiter=slope_limiter, params=material_parameters, **kw).cells
+
+    @staticmethod
+    def solve2D(initial_cells, cell_spacing, cfl_coefficient, final_time, bias, slope_limiter,
+                subcycling_iterations, material_parameters, **kw):
+        return euler_solve(initial_cells, 2, cell_spacing, cfl_coefficient, final_time, order=2, bias=bias,
+                           limiter=slope_limiter, params=material_parameters, **kw).cells
+
+
+class FirstOrderSolver:
+    """FORCE, Solvers/firstordersolver.h:7-101."""
+
+    @staticmethod
+    def solve(initial_cells, cell_spacing, cfl_coefficient, final_time, subcycling_iterations,
+              material_parameters, **kw):
+        return euler_solve(initial_cells, 1, cell_spacing, cfl_coefficient, final_time, order=1,
+                           params=material_parameters, **kw).cells
+
+    @staticmethod
+    def solve2D(initial_cells, cell_spacing, cfl_coefficient, final_time, subcycling_iterations,
+                material_parameters, **kw):
+        return euler_solve(initial_cells, 2, cell_spacing, cfl_coefficient, final_time, order=1,
+                           params=material_parameters, **kw).cells
+
+
+class EulerGrid:
+    """Device-resident grid (vg_euler_state): load once, advance many times."""
+
+    def __init__(self, dim: int, ny: int, nx: int, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.lib = self.ctx.lib
+        self.dim, self.ny, self.nx = dim, ny, nx
+        self.handle = C.c_void_p()
+        check(self.lib.vg_euler_state_create(self.ctx.handle, dim, ny, nx, C.byref(self.handle)),
+              "vg_euler_state_create")
+
+    def close(self):
+        if self.handle:
+            self.lib.vg_euler_state_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load(self, prim: np.ndarray):
+        w = np.ascontiguousarray(prim, dtype=np.float64)
+        check(self.lib.vg_euler_state_load(self.handle, w.ctypes.data_as(C.c_void_p), 0), "vg_euler_state_load")
+
+    def load_device(self, ptr: int):
+        check(self.lib.vg_euler_state_load(self.handle, C.c_void_p(ptr), 1), "vg_euler_state_load")
+
+    def store(self) -> np.ndarray:
+        shape = (self.nx, 6) if self.dim == 1 else (self.ny, self.nx, 6)
+        out = np.empty(shape)
+        check(self.lib.vg_euler_state_store(self.handle, out.ctypes.data_as(C.c_void_p), 0), "vg_euler_state_store")
+        return out
+
+    def run(self, dx, cfl, t_final=math.inf, max_steps=0, *, order=2, bias=0.0, limiter=0, bc=TRANSMISSIVE,
+            params: EulerMaterialParameters | None = None, mode=EXACT, dt_cap=0):
+        params = params or EulerMaterialParameters(1.4)
+        mp = params.c_struct()
+        st = _lib.RunStats()
+        dts = np.empty(max(dt_cap, 1))
+        check(self.lib.vg_euler_state_run(self.handle, order, dx, cfl, t_final, max_steps, bias, limiter, bc,
+                                          C.byref(mp), mode, _lib.as_dptr(dts) if dt_cap else None, dt_cap,
+                                          C.byref(st)), "vg_euler_state_run")
+        return st, dts[:min(st.steps, dt_cap)].copy()
+
+    def max_wave_speed(self, params: EulerMaterialParameters | None = None, bc=TRANSMISSIVE) -> float:
+        params = params or EulerMaterialParameters(1.4)
+        mp = params.c_struct()
+        s = C.c_double()
+        check(self.lib.vg_euler_state_max_wave_speed(self.handle, bc, C.byref(mp), C.byref(s)),
+              "vg_euler_state_max_wave_speed")
+        return s.value
