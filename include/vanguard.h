@@ -28,7 +28,8 @@ enum {
     VG_OK = 0,
     VG_ERR_ARG = -1,        /* invalid argument (sizes, pointers, options) */
     VG_ERR_HIP = -2,        /* HIP runtime failure */
-    VG_ERR_NONFINITE = -3,  /* the final state holds NaN/Inf or rho <= 0; outputs are still written */
+    VG_ERR_NONFINITE = -3,  /* the final state holds NaN/Inf or inadmissible cells (Euler: rho <= 0 or
+                               p + p_inf <= 0); outputs are still written */
     VG_ERR_UNSUPPORTED = -4 /* option combination not implemented */
 };
 
@@ -56,7 +57,7 @@ typedef struct {
     double dt_min, dt_max;  /* over this call's steps */
     double wall_ms;         /* host wall time of the call */
     double device_ms;       /* stream time of the step loop (HIP events) */
-    int64_t nonfinite_cells;/* cells of the final state with NaN/Inf or rho <= 0 */
+    int64_t nonfinite_cells;/* cells of the final state that are non-finite or inadmissible */
     int64_t first_nonfinite;/* flat index of the first such cell, -1 if none */
 } vg_run_stats;
 

@@ -18,6 +18,18 @@ constexpr int kWave = 64;
 __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
 
+// Reciprocal for FAST mode: the hardware estimate refined by two Newton steps
+// (explicit FMAs), within an ulp of 1/x for normal x > 0; not correctly rounded,
+// so never used in EXACT mode.
+__device__ __forceinline__ double rcp_nr(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
 // Ghost-cell index for a cell index k in [-2, n+1].  Transmissive ghosts are
 // mirror-ordered copies of the interior (Solvers/solvers.cpp:17-23, 112-131:
 // g[-1] = c[0], g[-2] = c[1], g[n] = c[n-1], g[n+1] = c[n-2]); the one-layer

@@ -22,6 +22,9 @@ vg::EulerPhys::Params to_params(const vg_euler_params *p)
     m.e_form = p->e_form;
     m.t_ign = p->t_ign;
     m.k_rate = p->k_rate;
+    m.gm1 = p->gamma - 1;
+    m.gpinf = p->gamma * p->p_inf;
+    m.inv_gm1 = 1.0 / (p->gamma - 1);
     return m;
 }
 
@@ -60,12 +63,12 @@ int solve(vg_ctx *ctx, int dim, const double *prim_in, double *prim_out, int ny,
     vg_run_stats local{};
     if (!rc) rc = g.run(order, mode, dx, cfl, t_final, max_steps, bias, limiter, bc, to_params(mp), dt_log, dt_log_cap, &local);
     if (!rc) rc = g.store(prim_out, 0);
-    if (!rc) rc = g.scan_nonfinite(&local.nonfinite_cells, &local.first_nonfinite);
+    if (!rc) rc = g.scan_nonfinite(to_params(mp), &local.nonfinite_cells, &local.first_nonfinite);
     g.destroy();
     if (stats) *stats = local;
     if (!rc && local.nonfinite_cells > 0) {
         vg::set_error("final state holds " + std::to_string(local.nonfinite_cells) +
-                      " non-finite or non-positive-density cells, first at flat index " +
+                      " non-finite or inadmissible cells (rho <= 0 or p + p_inf <= 0), first at flat index " +
                       std::to_string(local.first_nonfinite));
         return VG_ERR_NONFINITE;
     }

@@ -11,6 +11,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 
 #include "vg_runtime.h"
@@ -20,7 +21,8 @@ namespace vg {
 
 constexpr int kBSX = 256;
 constexpr int kBSY = 256;
-constexpr int kHY = 32;
+constexpr int kHY = 64;
+constexpr int kRX = 4;
 
 template <class P>
 struct GridState {
@@ -35,6 +37,8 @@ struct GridState {
     double *dt_dev = nullptr;      // device dt log (lazy)
     int dt_dev_cap = 0;
     unsigned long long *scan = nullptr;  // 2 counters for the non-finite scan
+    unsigned long long *partials = nullptr;  // per-block wave-speed keys
+    int n_partials = 0;                      // slots written by the last producer
     bool max_valid = false;
     typename P::Params last_params{};
     int last_bc = -1;
@@ -51,6 +55,9 @@ struct GridState {
         VG_HIP(hipMalloc(&buf[1], sizeof(double) * NF * plane));
         VG_HIP(hipMalloc(&clk, sizeof(Clock)));
         VG_HIP(hipMalloc(&scan, 2 * sizeof(unsigned long long)));
+        const long long gx = (nx + kBSX - 4 - 1) / (kBSX - 4) + 1;
+        const long long slots = std::max<long long>(gx * ny, 4096);
+        VG_HIP(hipMalloc(&partials, sizeof(unsigned long long) * slots));
         Clock h{};
         VG_HIP(hipMemcpy(clk, &h, sizeof(Clock), hipMemcpyHostToDevice));
         return VG_OK;
@@ -64,6 +71,7 @@ struct GridState {
         hipFree(staging);
         hipFree(dt_dev);
         hipFree(scan);
+        hipFree(partials);
     }
 
     int ensure_staging()
@@ -115,9 +123,12 @@ struct GridState {
     int refresh_max(int bc, const typename P::Params &m)
     {
         const int blocks = blocks_for(cells) < 2048 ? blocks_for(cells) : 2048;
-        VG_HIP(hipMemsetAsync(&clk->smax, 0, sizeof(unsigned long long), ctx->stream));
-        wave_max_kernel<P><<<blocks, 256, 0, ctx->stream>>>(buf[cur], plane, cells, dim, m, clk);
+        wave_max_kernel<P><<<blocks, 256, 0, ctx->stream>>>(buf[cur], plane, cells, dim, m, partials);
         int rc = launch_status("wave_max");
+        if (rc) return rc;
+        n_partials = blocks;
+        reduce_max_kernel<<<1, kReduceThreads, 0, ctx->stream>>>(partials, n_partials, clk);
+        rc = launch_status("reduce_max");
         if (rc) return rc;
         if (dim == 2 && bc == VG_BC_TRANSMISSIVE) {
             corner_speed_kernel<P><<<1, 1, 0, ctx->stream>>>(m, dim, clk);
@@ -151,14 +162,17 @@ struct GridState {
         a.fuse_max = fuse;
         a.m = m;
         a.clk = clk;
+        a.partials = partials;
         if (dir == 0) {
             constexpr int G = (ORDER == 2) ? 2 : 1;
-            dim3 grid((nx + (kBSX - 2 * G) - 1) / (kBSX - 2 * G), ny);
+            dim3 grid((nx + (kBSX - 2 * G) - 1) / (kBSX - 2 * G), (ny + kRX - 1) / kRX);
+            if (fuse) n_partials = (int)(grid.x * grid.y);
             return ctx->timed(ORDER == 2 ? "sweep_x_slic" : "sweep_x_force", [&] {
-                sweep_x_kernel<P, ORDER, MODE, kBSX><<<grid, kBSX, 0, ctx->stream>>>(a);
+                sweep_x_kernel<P, ORDER, MODE, kBSX, kRX><<<grid, kBSX, 0, ctx->stream>>>(a);
             });
         }
         dim3 grid((nx + kBSY - 1) / kBSY, (ny + kHY - 1) / kHY);
+        if (fuse) n_partials = (int)(grid.x * grid.y);
         return ctx->timed(ORDER == 2 ? "sweep_y_slic" : "sweep_y_force", [&] {
             sweep_y_kernel<P, ORDER, MODE, kBSY, kHY><<<grid, kBSY, 0, ctx->stream>>>(a);
         });
@@ -167,7 +181,7 @@ struct GridState {
     template <int ORDER, int MODE>
     int enqueue_step(int bc, double dx, double bias, int limiter, const typename P::Params &m, double *dt_log, int dt_cap)
     {
-        clock_begin_kernel<<<1, 1, 0, ctx->stream>>>(clk, dt_log, dt_cap);
+        clock_begin_kernel<<<1, kReduceThreads, 0, ctx->stream>>>(clk, partials, n_partials, dt_log, dt_cap);
         int rc = launch_status("clock_begin");
         if (rc) return rc;
         double *a = buf[cur], *b = buf[1 - cur];
@@ -275,12 +289,12 @@ struct GridState {
         return VG_ERR_ARG;
     }
 
-    // Non-finite / non-positive-density scan of the current grid.
-    int scan_nonfinite(int64_t *count, int64_t *first)
+    // Scan of the current grid for non-finite or inadmissible cells.
+    int scan_nonfinite(const typename P::Params &m, int64_t *count, int64_t *first)
     {
         unsigned long long init[2] = {0ull, ~0ull};
         VG_HIP(hipMemcpyAsync(scan, init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
-        nonfinite_kernel<NF><<<blocks_for(cells), 256, 0, ctx->stream>>>(buf[cur], plane, cells, scan, scan + 1);
+        nonfinite_kernel<P><<<blocks_for(cells), 256, 0, ctx->stream>>>(buf[cur], plane, cells, m, scan, scan + 1);
         int rc = launch_status("nonfinite");
         if (rc) return rc;
         unsigned long long res[2];

@@ -13,19 +13,23 @@ struct EulerPhys {
     static constexpr const char *name = "euler";
 
     // EulerMaterialParameters (Euler/eulermaterialparameters.h:30-36).
+    // Derived constants (gm1 = gamma - 1, gpinf = gamma * p_inf, inv_gm1) are
+    // filled on the host; each is the same IEEE value the reference recomputes
+    // at every use, so EXACT mode stays bit-identical.
     struct Params {
         double gamma, p_inf, cv, e_form, t_ign, k_rate;
+        double gm1, gpinf, inv_gm1;
     };
 
     // Specific internal energy, pressure and sound speed of the stiffened gas
     // (Euler/eulerequationofstate.cpp:7-29).
     __device__ static __forceinline__ double eos_e(const Params &m, double rho, double p)
     {
-        return (p + (m.gamma * m.p_inf)) / ((m.gamma - 1) * rho);
+        return (p + m.gpinf) / (m.gm1 * rho);
     }
     __device__ static __forceinline__ double eos_p(const Params &m, double rho, double e)
     {
-        return (e * (m.gamma - 1) * rho) - (m.gamma * m.p_inf);
+        return (e * m.gm1 * rho) - m.gpinf;
     }
     __device__ static __forceinline__ double eos_c(const Params &m, double rho, double p)
     {
@@ -99,6 +103,12 @@ struct EulerPhys {
         return a + eos_c(m, w[0], w[4]);
     }
 
+    // A state whose sound speed is real: rho > 0 and p + p_inf > 0.
+    __device__ static __forceinline__ bool admissible(const Params &m, const double (&w)[NF])
+    {
+        return w[0] > 0.0 && (w[4] + m.p_inf) > 0.0;
+    }
+
     // EulerStateVector() (Euler/eulerstatevector.cpp:3-12): fills the 2D corner ghosts.
     __device__ static __forceinline__ void default_state(double (&w)[NF])
     {
@@ -106,37 +116,37 @@ struct EulerPhys {
     }
 
     // ---- FAST mode ----------------------------------------------------------
-    // Same scheme, fewer divisions: one reciprocal of rho per state and the
-    // EOS constants folded.  Not bit-identical to the reference.
+    // Same scheme, fewer VALU operations: one Newton-refined reciprocal of rho
+    // per state, the EOS constants folded, fluxes taken from primitives without
+    // a conserved round trip.  Not bit-identical to the reference.
     __device__ static __forceinline__ void prim_to_cons_fast(const Params &m, const double (&w)[NF], double (&U)[NF])
     {
         const double rho = w[0], u = w[1], v = w[2], z = w[3], p = w[4], lam = w[5];
-        const double inv_gm1 = 1.0 / (m.gamma - 1);
         U[0] = rho;
         U[1] = rho * u;
         U[2] = rho * v;
         U[3] = rho * z;
-        U[4] = 0.5 * rho * ((u * u) + (v * v) + (z * z)) + (p + m.gamma * m.p_inf) * inv_gm1
-               + rho * (m.e_form * (lam - 1.0));
+        U[4] = 0.5 * (U[1] * u + U[2] * v + U[3] * z) + (p + m.gpinf) * m.inv_gm1 + m.e_form * (rho * lam - rho);
         U[5] = rho * lam;
     }
     __device__ static __forceinline__ void cons_to_prim_fast(const Params &m, const double (&U)[NF], double (&w)[NF])
     {
-        const double rho = U[0], r = 1.0 / rho;
+        const double rho = U[0], r = rcp_nr(rho);
         const double u = U[1] * r, v = U[2] * r, z = U[3] * r, lam = U[5] * r;
         const double ke = 0.5 * ((U[1] * u) + (U[2] * v) + (U[3] * z));
         w[0] = rho;
         w[1] = u;
         w[2] = v;
         w[3] = z;
-        w[4] = (m.gamma - 1) * (U[4] - ke - m.e_form * (U[5] - rho)) - m.gamma * m.p_inf;
+        w[4] = m.gm1 * (U[4] - ke - m.e_form * (U[5] - rho)) - m.gpinf;
         w[5] = lam;
     }
-    // Flux from a primitive state (no conserved round trip).
+    // Flux from a primitive state and its conserved vector.
     template <int DIR>
     __device__ static __forceinline__ void flux_prim_fast(const Params &m, const double (&w)[NF], const double (&U)[NF],
                                                           double (&F)[NF])
     {
+        (void)m;
         const double vn = (DIR == 0) ? w[1] : w[2];
         const double p = w[4];
         F[0] = U[0] * vn;

@@ -30,6 +30,23 @@ __device__ __forceinline__ double limiter_value(double r, double bias, int limit
     return smin(1.0, r_coefficient(r, bias));
 }
 
+// FAST-mode limiter: the same piecewise functions with Newton-refined reciprocals.
+__device__ __forceinline__ double limiter_value_fast(double r, double bias, int limiter)
+{
+    if (r < 0.0) return 0.0;
+    if (limiter == 0) {
+        if (r < 0.5) return 2.0 * r;
+        if (r < 1.0) return 1.0;
+        return smin(smin(r, 2.0 * rcp_nr((1.0 - bias) + (1.0 + bias) * r)), 2.0);
+    }
+    if (limiter == 1) {
+        const double R = 2.0 * rcp_nr((1.0 - bias) + (1.0 + bias) * r);
+        return smin(2.0 * r * rcp_nr(1.0 + r), R);
+    }
+    if (r < 1.0) return r;
+    return smin(1.0, 2.0 * rcp_nr((1.0 - bias) + (1.0 + bias) * r));
+}
+
 // Conserved state plus its physical flux in the sweep direction.
 template <int NF>
 struct ConsFlux {
@@ -91,7 +108,8 @@ struct Scheme {
             // |difference| < 1e-5 is replaced by +1e-5 (sign dropped) before the ratio.
             const double num = (fabs(dl) < 1e-5) ? 1e-5 : dl;
             const double den = (fabs(dr) < 1e-5) ? 1e-5 : dr;
-            d *= limiter_value(num / den, bias, limiter);
+            if (MODE == MODE_EXACT) d *= limiter_value(num / den, bias, limiter);
+            else d *= limiter_value_fast(num * rcp_nr(den), bias, limiter);
             xl[k] = UM[k] - (0.5 * d);
             xr[k] = UM[k] + (0.5 * d);
         }

@@ -34,6 +34,7 @@ struct SweepArgs {
     int fuse_max;
     typename P::Params m;
     Clock *clk;
+    unsigned long long *partials;   // one wave-speed key per block when fuse_max
 };
 
 // Per-step scalars, computed identically in every thread.
@@ -52,7 +53,7 @@ template <int NF>
 __device__ __forceinline__ void load_prim(const double *__restrict__ in, long long plane, long long idx, double (&w)[NF])
 {
 #pragma unroll
-    for (int f = 0; f < NF; f++) w[f] = __builtin_nontemporal_load(in + f * plane + idx);
+    for (int f = 0; f < NF; f++) w[f] = in[f * plane + idx];
 }
 
 template <int NF>
@@ -62,13 +63,25 @@ __device__ __forceinline__ void store_prim(double *__restrict__ out, long long p
     for (int f = 0; f < NF; f++) out[f * plane + idx] = w[f];
 }
 
-__device__ __forceinline__ void flush_max(Clock *clk, unsigned long long key)
+// Block-level maximum of the wave-speed keys, written to this block's slot of
+// the partials array (no atomics: every block owns one slot, and the 1-block
+// clock kernel reduces the slots, so the result does not depend on ordering).
+template <int BS>
+__device__ __forceinline__ void block_max_to(unsigned long long *partials, unsigned long long key)
 {
+    __shared__ unsigned long long s_w[BS / kWave];
     key = wave_max_u64(key);
-    if ((threadIdx.x & (kWave - 1)) == 0 && key) atomicMax(&clk->smax, key);
+    if ((threadIdx.x & (kWave - 1)) == 0) s_w[threadIdx.x / kWave] = key;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long mx = 0;
+#pragma unroll
+        for (int k = 0; k < BS / kWave; k++) mx = (s_w[k] > mx) ? s_w[k] : mx;
+        partials[(long long)blockIdx.y * gridDim.x + blockIdx.x] = mx;
+    }
 }
 
-template <class P, int ORDER, int MODE, int BS>
+template <class P, int ORDER, int MODE, int BS, int RX>
 __global__ __launch_bounds__(BS) void sweep_x_kernel(SweepArgs<P> a)
 {
     constexpr int NF = P::NF;
@@ -84,83 +97,96 @@ __global__ __launch_bounds__(BS) void sweep_x_kernel(SweepArgs<P> a)
     const typename P::Params m = a.m;
 
     const int tid = threadIdx.x;
-    const int i = blockIdx.y;
     const int j = blockIdx.x * OUT - G + tid;
     const int nx = a.nx;
     const int jj = (j >= -G && j < nx + G) ? ghost_index(j, nx, a.bc) : (j < 0 ? 0 : nx - 1);
-    const long long row = (long long)i * nx;
-
-    double w[NF], U[NF];
-    load_prim<NF>(a.in, a.plane, row + jj, w);
-    double Fp[NF];
+    const int i_begin = blockIdx.y * RX;
+    const int i_end = (i_begin + RX < a.ny) ? i_begin + RX : a.ny;
     unsigned long long key = 0;
 
-    if (ORDER == 2) {
-        S::cons(m, w, U);
+    // Each block walks RX rows; the next row's cells are loaded while the
+    // current row is computed.
+    double wnext[NF];
+    load_prim<NF>(a.in, a.plane, (long long)i_begin * nx + jj, wnext);
+    for (int i = i_begin; i < i_end; i++) {
+        const long long row = (long long)i * nx;
+        double w[NF], U[NF];
 #pragma unroll
-        for (int f = 0; f < NF; f++) s_a[f * BS + tid] = U[f];
-        __syncthreads();
-        ConsFlux<NF> left, right;
-        if (tid >= 1 && tid <= BS - 2) {
-            double UL[NF], UR[NF];
-#pragma unroll
-            for (int f = 0; f < NF; f++) {
-                UL[f] = s_a[f * BS + tid - 1];
-                UR[f] = s_a[f * BS + tid + 1];
-            }
-            S::evolve(m, UL, U, UR, a.bias, a.limiter, sc.half_c, left, right);
-#pragma unroll
-            for (int f = 0; f < NF; f++) {
-                s_e[f * BS + tid] = left.U[f];
-                s_e[(NF + f) * BS + tid] = left.F[f];
-            }
-        }
-        __syncthreads();
-        if (tid >= 1 && tid <= BS - 3) {
-            ConsFlux<NF> nb;
-#pragma unroll
-            for (int f = 0; f < NF; f++) {
-                nb.U[f] = s_e[f * BS + tid + 1];
-                nb.F[f] = s_e[(NF + f) * BS + tid + 1];
-            }
-            S::force_face(m, right, nb, sc.lf_c, sc.ri_c, Fp);   // face j + 1/2
-#pragma unroll
-            for (int f = 0; f < NF; f++) s_a[f * BS + tid] = Fp[f];
-        }
-        __syncthreads();
-    } else {
-        ConsFlux<NF> cf;
-        S::cons_flux_of_prim(m, w, cf);
-#pragma unroll
-        for (int f = 0; f < NF; f++) {
-            U[f] = cf.U[f];
-            s_e[f * BS + tid] = cf.U[f];
-            s_e[(NF + f) * BS + tid] = cf.F[f];
-        }
-        __syncthreads();
-        if (tid <= BS - 2) {
-            ConsFlux<NF> nb;
-#pragma unroll
-            for (int f = 0; f < NF; f++) {
-                nb.U[f] = s_e[f * BS + tid + 1];
-                nb.F[f] = s_e[(NF + f) * BS + tid + 1];
-            }
-            S::force_face(m, cf, nb, sc.lf_c, sc.ri_c, Fp);
-#pragma unroll
-            for (int f = 0; f < NF; f++) s_a[f * BS + tid] = Fp[f];
-        }
-        __syncthreads();
-    }
+        for (int f = 0; f < NF; f++) w[f] = wnext[f];
+        if (i + 1 < i_end) load_prim<NF>(a.in, a.plane, row + nx + jj, wnext);
+        double Fp[NF];
 
-    if (tid >= G && tid < BS - G && j < nx) {
-        double Fm[NF], wn[NF];
+        if (ORDER == 2) {
+            S::cons(m, w, U);
 #pragma unroll
-        for (int f = 0; f < NF; f++) Fm[f] = s_a[f * BS + tid - 1];
-        S::update(m, U, Fm, Fp, sc.up_c, wn);
-        store_prim<NF>(a.out, a.plane, row + j, wn);
-        if (a.fuse_max) key = max_key(P::wave_speed(m, wn, a.dim));
+            for (int f = 0; f < NF; f++) s_a[f * BS + tid] = U[f];
+            __syncthreads();
+            ConsFlux<NF> left, right;
+            if (tid >= 1 && tid <= BS - 2) {
+                double UL[NF], UR[NF];
+#pragma unroll
+                for (int f = 0; f < NF; f++) {
+                    UL[f] = s_a[f * BS + tid - 1];
+                    UR[f] = s_a[f * BS + tid + 1];
+                }
+                S::evolve(m, UL, U, UR, a.bias, a.limiter, sc.half_c, left, right);
+#pragma unroll
+                for (int f = 0; f < NF; f++) {
+                    s_e[f * BS + tid] = left.U[f];
+                    s_e[(NF + f) * BS + tid] = left.F[f];
+                }
+            }
+            __syncthreads();
+            if (tid >= 1 && tid <= BS - 3) {
+                ConsFlux<NF> nb;
+#pragma unroll
+                for (int f = 0; f < NF; f++) {
+                    nb.U[f] = s_e[f * BS + tid + 1];
+                    nb.F[f] = s_e[(NF + f) * BS + tid + 1];
+                }
+                S::force_face(m, right, nb, sc.lf_c, sc.ri_c, Fp);   // face j + 1/2
+#pragma unroll
+                for (int f = 0; f < NF; f++) s_a[f * BS + tid] = Fp[f];
+            }
+            __syncthreads();
+        } else {
+            ConsFlux<NF> cf;
+            S::cons_flux_of_prim(m, w, cf);
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                U[f] = cf.U[f];
+                s_e[f * BS + tid] = cf.U[f];
+                s_e[(NF + f) * BS + tid] = cf.F[f];
+            }
+            __syncthreads();
+            if (tid <= BS - 2) {
+                ConsFlux<NF> nb;
+#pragma unroll
+                for (int f = 0; f < NF; f++) {
+                    nb.U[f] = s_e[f * BS + tid + 1];
+                    nb.F[f] = s_e[(NF + f) * BS + tid + 1];
+                }
+                S::force_face(m, cf, nb, sc.lf_c, sc.ri_c, Fp);
+#pragma unroll
+                for (int f = 0; f < NF; f++) s_a[f * BS + tid] = Fp[f];
+            }
+            __syncthreads();
+        }
+
+        if (tid >= G && tid < BS - G && j < nx) {
+            double Fm[NF], wn[NF];
+#pragma unroll
+            for (int f = 0; f < NF; f++) Fm[f] = s_a[f * BS + tid - 1];
+            S::update(m, U, Fm, Fp, sc.up_c, wn);
+            store_prim<NF>(a.out, a.plane, row + j, wn);
+            if (a.fuse_max) {
+                const unsigned long long k2 = max_key(P::wave_speed(m, wn, a.dim));
+                key = (k2 > key) ? k2 : key;
+            }
+        }
+        __syncthreads();   // s_a / s_e are rewritten by the next row
     }
-    if (a.fuse_max) flush_max(a.clk, key);
+    if (a.fuse_max) block_max_to<BS>(a.partials, key);
 }
 
 template <class P, int ORDER, int MODE, int BS, int H>
@@ -188,11 +214,12 @@ __global__ __launch_bounds__(BS) void sweep_y_kernel(SweepArgs<P> a)
             S::cons(m, w, A);
             load_prim<NF>(a.in, a.plane, row_of(i0 - 1), w);
             S::cons(m, w, B);
+            load_prim<NF>(a.in, a.plane, row_of(i0), w);   // row e + 1 of the first iteration
             ConsFlux<NF> er_prev;
             double Fprev[NF];
             for (int e = i0 - 1; e <= i1; e++) {
-                load_prim<NF>(a.in, a.plane, row_of(e + 1), w);
                 S::cons(m, w, C);
+                if (e + 2 <= i1 + 1) load_prim<NF>(a.in, a.plane, row_of(e + 2), w);   // prefetch
                 ConsFlux<NF> L, R;
                 S::evolve(m, A, B, C, a.bias, a.limiter, sc.half_c, L, R);
                 double Fnew[NF];
@@ -245,14 +272,14 @@ __global__ __launch_bounds__(BS) void sweep_y_kernel(SweepArgs<P> a)
             }
         }
     }
-    if (a.fuse_max) flush_max(a.clk, key);
+    if (a.fuse_max) block_max_to<BS>(a.partials, key);
 }
 
 // Wave-speed maximum of a whole grid (the first step, or after the caller
 // changed the state).
 template <class P>
 __global__ __launch_bounds__(256) void wave_max_kernel(const double *__restrict__ in, long long plane, long long cells,
-                                                       int dim, typename P::Params m, Clock *clk)
+                                                       int dim, typename P::Params m, unsigned long long *partials)
 {
     constexpr int NF = P::NF;
     unsigned long long key = 0;
@@ -262,7 +289,32 @@ __global__ __launch_bounds__(256) void wave_max_kernel(const double *__restrict_
         const unsigned long long k2 = max_key(P::wave_speed(m, w, dim));
         key = (k2 > key) ? k2 : key;
     }
-    flush_max(clk, key);
+    block_max_to<256>(partials, key);
+}
+
+// Reduces n partial keys with one block of kReduceThreads threads; returns the
+// maximum in thread 0.
+constexpr int kReduceThreads = 1024;
+
+__device__ __forceinline__ unsigned long long reduce_partials(const unsigned long long *partials, int n)
+{
+    __shared__ unsigned long long s_r[kReduceThreads / kWave];
+    unsigned long long key = 0;
+    for (int k = threadIdx.x; k < n; k += kReduceThreads) key = (partials[k] > key) ? partials[k] : key;
+    key = wave_max_u64(key);
+    if ((threadIdx.x & (kWave - 1)) == 0) s_r[threadIdx.x / kWave] = key;
+    __syncthreads();
+    unsigned long long mx = 0;
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < kReduceThreads / kWave; k++) mx = (s_r[k] > mx) ? s_r[k] : mx;
+    }
+    return mx;
+}
+
+__global__ __launch_bounds__(kReduceThreads) void reduce_max_kernel(const unsigned long long *partials, int n, Clock *clk)
+{
+    const unsigned long long mx = reduce_partials(partials, n);
+    if (threadIdx.x == 0) clk->smax = mx;
 }
 
 // Default-state wave speed (2D transmissive corner ghosts) into clk->corner.
@@ -278,8 +330,13 @@ __global__ void corner_speed_kernel(typename P::Params m, int dim, Clock *clk)
 // secondordersolver.cpp:307-308), evaluate the loop condition, then
 // dt = CFL * (dx / S), x0.2 for the first six steps, clipped to the final time
 // (Solvers/solvers.cpp:392, 437-452).
-__global__ void clock_begin_kernel(Clock *c, double *dt_log, int dt_cap)
+__global__ __launch_bounds__(kReduceThreads) void clock_begin_kernel(Clock *c, const unsigned long long *partials,
+                                                                    int n_partials, double *dt_log, int dt_cap)
 {
+    if (c->done) return;   // uniform: nothing pending once done
+    const unsigned long long mx = reduce_partials(partials, n_partials);
+    if (threadIdx.x != 0) return;
+    c->smax = mx;
     if (c->pending) {
         c->t += c->dt;
         c->it += 1;
@@ -290,9 +347,8 @@ __global__ void clock_begin_kernel(Clock *c, double *dt_log, int dt_cap)
         c->done = 1;
         return;
     }
-    double S = __longlong_as_double((long long)c->smax);
+    double S = __longlong_as_double((long long)mx);
     if (c->corner > S) S = c->corner;
-    c->smax = 0ull;
     double dt = c->cfl * (c->dx / S);
     if (c->it <= 5) dt *= 0.2;
     if ((c->t + dt) > c->t_final) dt = c->t_final - c->t;
@@ -334,16 +390,23 @@ __global__ __launch_bounds__(256) void soa_to_aos_kernel(const double *__restric
     }
 }
 
-// Counts cells whose state is not finite or whose density is not positive, and
-// records the smallest such flat index.
-template <int NF>
+// Counts cells whose state is not finite or not admissible for the physics
+// (P::admissible: e.g. a real sound speed), and records the smallest such flat index.
+template <class P>
 __global__ __launch_bounds__(256) void nonfinite_kernel(const double *__restrict__ soa, long long plane, long long cells,
-                                                        unsigned long long *count, unsigned long long *first)
+                                                        typename P::Params m, unsigned long long *count,
+                                                        unsigned long long *first)
 {
+    constexpr int NF = P::NF;
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += (long long)gridDim.x * blockDim.x) {
-        bool bad = !(soa[c] > 0.0);
+        double w[NF];
+        bool bad = false;
 #pragma unroll
-        for (int f = 0; f < NF; f++) bad = bad || !isfinite(soa[f * plane + c]);
+        for (int f = 0; f < NF; f++) {
+            w[f] = soa[f * plane + c];
+            bad = bad || !isfinite(w[f]);
+        }
+        bad = bad || !P::admissible(m, w);
         if (bad) {
             atomicAdd(count, 1ull);
             atomicMin(first, (unsigned long long)c);
