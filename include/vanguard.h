@@ -130,6 +130,51 @@ int vg_euler_state_run(vg_euler_state *st, int order, double dx, double cfl, dou
  * are skipped as in the reference.  Order-independent, hence deterministic. */
 int vg_euler_state_max_wave_speed(vg_euler_state *st, int bc, const vg_euler_params *mp, double *smax);
 
+/* Batched 1D Euler solves in one persistent launch: n_inst independent
+ * SecondOrderSolver::solve (order 2) / FirstOrderSolver::solve (order 1) calls
+ * (secondordersolver.cpp:288-314, firstordersolver.cpp:435-461) with their own
+ * sizes, spacings, material parameters and final times; transmissive boundaries.
+ * offsets: n_inst + 1 cell offsets into prim_in/prim_out (6 doubles per cell).
+ * steps/t_out (may be NULL) receive each instance's step count and final time. */
+int vg_euler_solve_batch(vg_ctx *ctx, int n_inst, const int64_t *offsets, const double *prim_in, double *prim_out,
+                         const double *dx, const vg_euler_params *mp, const double *t_final, double cfl,
+                         int max_steps, double bias, int limiter, int order, int mode, int32_t *steps,
+                         double *t_out, vg_run_stats *stats);
+
+/* ---- ideal MHD with GLM cleaning ----------------------------------------- */
+
+/* MHDMaterialParameters (MHD/mhdmaterialparameters.h) field order: gamma, p_inf,
+ * GLM hyperbolic speed c_h, parabolic damping c_p.  As in the reference the
+ * solvers set c_h to each step's maximum wave speed and c_p = sqrt(0.18 c_h)
+ * (mhdsecondordersolver.cpp:330-332), so the c_h/c_p given here are unused. */
+typedef struct {
+    double gamma, p_inf, ch, cp;
+} vg_mhd_params;
+
+/* MHDSecondOrderSolver::solve (mhdsecondordersolver.h:58, mhdsecondordersolver.cpp:318-360)
+ * when order == 2, MHDFirstOrderSolver::solve when order == 1, with zero
+ * sub-cycles.  Primitive order (rho, u, v, w, p, Bx, By, Bz, psi), 9 doubles
+ * per cell.  Runs as one persistent workgroup. */
+int vg_mhd_solve_1d(vg_ctx *ctx, const double *prim_in, double *prim_out, int n, double dx, double cfl,
+                    double t_final, int max_steps, double bias, int limiter, int order, int bc,
+                    const vg_mhd_params *mp, int mode, double *dt_log, int dt_log_cap, vg_run_stats *stats);
+
+/* MHDSecondOrderSolver::solve2D (mhdsecondordersolver.h:70, mhdsecondordersolver.cpp:501-546)
+ * when order == 2, MHDFirstOrderSolver::solve2D when order == 1. */
+int vg_mhd_solve_2d(vg_ctx *ctx, const double *prim_in, double *prim_out, int ny, int nx, double dx, double cfl,
+                    double t_final, int max_steps, double bias, int limiter, int order, int bc,
+                    const vg_mhd_params *mp, int mode, double *dt_log, int dt_log_cap, vg_run_stats *stats);
+
+/* n_inst independent MHDSecondOrderSolver::solve calls (transmissive
+ * boundaries, zero sub-cycles) in one persistent launch; each instance has its
+ * own size, spacing, parameters, final time, dt sequence and c_h.  Returns
+ * VG_ERR_NONFINITE if an instance had to stop because its time step could not
+ * advance t (where the reference would loop forever). */
+int vg_mhd_solve_batch(vg_ctx *ctx, int n_inst, const int64_t *offsets, const double *prim_in, double *prim_out,
+                       const double *dx, const vg_mhd_params *mp, const double *t_final, double cfl, int max_steps,
+                       double bias, int limiter, int order, int mode, int32_t *steps, double *t_out,
+                       vg_run_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,8 +45,24 @@ void orc_limiter_units(int n, double bias, const double *r, double *out);
 void orc_euler_sweep(int order, int dim, int dir, int ny, int nx, double dx, double ts, double bias,
                      int limiter, int bc, const orc_euler_params *mp, const double *prim_in,
                      double *prim_out);
-/* Maximum wave speed over the ghosted grid, corners included in 2D. */
+/* Maximum wave speed over the cells (no ghost corners). */
 double orc_euler_max_wave_speed(int dim, int ny, int nx, const orc_euler_params *mp, const double *prim);
+
+/* MHDMaterialParameters (MHD/mhdmaterialparameters.h) field order: adiabatic
+ * index, stiffening parameter, GLM hyperbolic wave speed c_h, parabolic
+ * damping c_p.  MHDMaterialParameters(gamma) sets the last three to 0. */
+typedef struct {
+    double gamma, p_inf, ch, cp;
+} orc_mhd_params;
+
+/* MHDSecondOrderSolver::solve/solve2D and MHDFirstOrderSolver::solve/solve2D with
+ * zero sub-cycles; primitive order (rho, u, v, w, p, Bx, By, Bz, psi). */
+int orc_mhd_solve(int order, int dim, int ny, int nx, double dx, double cfl, double t_final, int max_steps,
+                  double bias, int limiter, int bc, const orc_mhd_params *mp, const double *prim_in,
+                  double *prim_out, double *dt_log, int dt_cap);
+/* Per-function records of five input states (45 doubles) -> 85 doubles. */
+void orc_mhd_units(int n, double bias, int limiter, double dx, double ts, const orc_mhd_params *mp,
+                   const double *states, double *out);
 
 #ifdef __cplusplus
 }

@@ -18,6 +18,8 @@
 
 #include "Solvers/secondordersolver.h"
 #include "Solvers/firstordersolver.h"
+#include "Solvers/mhdsecondordersolver.h"
+#include "Solvers/mhdfirstordersolver.h"
 
 #include <cstdio>
 #include <cstdlib>
@@ -285,6 +287,179 @@ int limiterUnits(int argc, char **argv)
     return 0;
 }
 
+MHDStateVector mhdStateFrom(const double *p)
+{
+    return MHDStateVector(p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8]);
+}
+
+void putMhdState(MHDStateVector s, double *p)
+{
+    std::vector<double> w = s.computePrimitiveVariableVector();
+    for (int k = 0; k < 9; k++) p[k] = w[k];
+}
+
+// mhd_run order dim ny nx dx cfl tfinal max_steps bias limiter gamma pinf in out_state out_dt check_api
+// Mirrors MHDSecondOrderSolver::solve / solve2D (mhdsecondordersolver.cpp:318-360, 501-546)
+// and MHDFirstOrderSolver::solve / solve2D with zero sub-cycles.
+int mhdRun(int argc, char **argv)
+{
+    if (argc < 18) { std::fprintf(stderr, "mhd_run: too few arguments\n"); return 2; }
+    int order = std::atoi(argv[2]);
+    int dim = std::atoi(argv[3]);
+    int ny = std::atoi(argv[4]);
+    int nx = std::atoi(argv[5]);
+    double dx = std::atof(argv[6]);
+    double cfl = std::atof(argv[7]);
+    double tfinal = std::strcmp(argv[8], "inf") == 0 ? std::numeric_limits<double>::infinity() : std::atof(argv[8]);
+    int maxSteps = std::atoi(argv[9]);
+    double bias = std::atof(argv[10]);
+    int limiter = std::atoi(argv[11]);
+    MHDMaterialParameters mp(std::atof(argv[12]));
+    mp.setStiffeningParameter(std::atof(argv[13]));
+    const char *inPath = argv[14];
+    const char *outState = argv[15];
+    const char *outDt = argv[16];
+    int checkApi = std::atoi(argv[17]);
+    int ghost = (order == 1) ? 1 : 2;
+    const int nf = 9;
+
+    std::vector<double> in = readDoubles(inPath, (size_t)ny * nx * nf);
+    std::vector<double> dts;
+    std::vector<double> out((size_t)ny * nx * nf);
+    MuteCout mute;
+    auto t0 = std::chrono::steady_clock::now();
+    if (dim == 1)
+    {
+        std::vector<MHDStateVector> cells(nx);
+        for (int j = 0; j < nx; j++) cells[j] = mhdStateFrom(&in[(size_t)j * nf]);
+        std::vector<MHDStateVector> initial = cells;
+        MHDMaterialParameters p = mp;
+        double t = 0.0;
+        int it = 0;
+        while (t < tfinal && (maxSteps == 0 || it < maxSteps))
+        {
+            std::vector<MHDStateVector> withBoundary = MHDSolvers::insertBoundaryCells(cells, ghost);
+            double dt = MHDSolvers::computeStableTimeStep(withBoundary, dx, cfl, t, tfinal, it, p);
+            double smax = MHDSolvers::computeMaximumWaveSpeed(withBoundary, p);
+            p.setHyperbolicWaveSpeed(smax);
+            p.configureParabolicDamping();
+            withBoundary = MHDSolvers::insertBoundaryCells(cells, ghost);
+            if (order == 1)
+                MHDFirstOrderSolver::computeFORCETimeStep(cells, withBoundary, dx, dt, p);
+            else
+                MHDSecondOrderSolver::computeSLICTimeStep(cells, withBoundary, dx, dt, bias, limiter, p);
+            t += dt;
+            it += 1;
+            dts.push_back(dt);
+        }
+        for (int j = 0; j < nx; j++) putMhdState(cells[j], &out[(size_t)j * nf]);
+        if (checkApi && maxSteps == 0 && order == 2)
+        {
+            std::vector<MHDStateVector> api = MHDSecondOrderSolver::solve(initial, dx, cfl, tfinal, bias, limiter, 0, mp);
+            bool match = true;
+            for (int j = 0; j < nx; j++)
+            {
+                double q[9];
+                putMhdState(api[j], q);
+                for (int f = 0; f < nf; f++) match = match && sameBits(q[f], out[(size_t)j * nf + f]);
+            }
+            std::fprintf(stderr, "API_MATCH %d\n", match ? 1 : 0);
+        }
+    }
+    else
+    {
+        std::vector<std::vector<MHDStateVector> > cells(ny, std::vector<MHDStateVector>(nx));
+        for (int i = 0; i < ny; i++)
+            for (int j = 0; j < nx; j++) cells[i][j] = mhdStateFrom(&in[((size_t)i * nx + j) * nf]);
+        MHDMaterialParameters p = mp;
+        double t = 0.0;
+        int it = 0;
+        while (t < tfinal && (maxSteps == 0 || it < maxSteps))
+        {
+            std::vector<std::vector<MHDStateVector> > withBoundary = MHDSolvers::insertBoundaryCells2D(cells, ghost);
+            double dt = MHDSolvers::computeStableTimeStep2D(withBoundary, dx, cfl, t, tfinal, it, p);
+            double smax = MHDSolvers::computeMaximumWaveSpeed2D(withBoundary, p);
+            p.setHyperbolicWaveSpeed(smax);
+            p.configureParabolicDamping();
+            withBoundary = MHDSolvers::insertBoundaryCells2D(cells, ghost);
+            if (order == 1)
+            {
+                MHDFirstOrderSolver::computeXFORCETimeStep2D(cells, withBoundary, dx, 0.5 * dt, p);
+                withBoundary = MHDSolvers::insertBoundaryCells2D(cells, ghost);
+                MHDFirstOrderSolver::computeYFORCETimeStep2D(cells, withBoundary, dx, dt, p);
+                withBoundary = MHDSolvers::insertBoundaryCells2D(cells, ghost);
+                MHDFirstOrderSolver::computeXFORCETimeStep2D(cells, withBoundary, dx, 0.5 * dt, p);
+            }
+            else
+            {
+                MHDSecondOrderSolver::computeXSLICTimeStep2D(cells, withBoundary, dx, 0.5 * dt, bias, limiter, p);
+                withBoundary = MHDSolvers::insertBoundaryCells2D(cells, ghost);
+                MHDSecondOrderSolver::computeYSLICTimeStep2D(cells, withBoundary, dx, dt, bias, limiter, p);
+                withBoundary = MHDSolvers::insertBoundaryCells2D(cells, ghost);
+                MHDSecondOrderSolver::computeXSLICTimeStep2D(cells, withBoundary, dx, 0.5 * dt, bias, limiter, p);
+            }
+            t += dt;
+            it += 1;
+            dts.push_back(dt);
+        }
+        for (int i = 0; i < ny; i++)
+            for (int j = 0; j < nx; j++) putMhdState(cells[i][j], &out[((size_t)i * nx + j) * nf]);
+    }
+    double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::fprintf(stderr, "STEPS %zu SECONDS %.6f\n", dts.size(), secs);
+    writeDoubles(outState, out);
+    writeDoubles(outDt, dts);
+    return 0;
+}
+
+// mhd_units n bias limiter dx ts gamma pinf ch in out: records of five states
+// (45 doubles) -> slope(9) Fx(9) Fy(9) evolveX side0/1 (18) FORCEx(9) SLICx(9)
+// speed1D speed2D cons(9) prim(cons)(9) fast_x alfven = 85 doubles.
+int mhdUnits(int argc, char **argv)
+{
+    if (argc < 12) { std::fprintf(stderr, "mhd_units: too few arguments\n"); return 2; }
+    int n = std::atoi(argv[2]);
+    double bias = std::atof(argv[3]);
+    int limiter = std::atoi(argv[4]);
+    double dx = std::atof(argv[5]);
+    double ts = std::atof(argv[6]);
+    MHDMaterialParameters mp(std::atof(argv[7]));
+    mp.setStiffeningParameter(std::atof(argv[8]));
+    mp.setHyperbolicWaveSpeed(std::atof(argv[9]));
+    mp.configureParabolicDamping();
+    std::vector<double> in = readDoubles(argv[10], (size_t)n * 45);
+    const int rec = 85;
+    std::vector<double> out((size_t)n * rec);
+    for (int r = 0; r < n; r++)
+    {
+        const double *s = &in[(size_t)r * 45];
+        MHDStateVector a = mhdStateFrom(s), b = mhdStateFrom(s + 9), c = mhdStateFrom(s + 18), d = mhdStateFrom(s + 27);
+        double *o = &out[(size_t)r * rec];
+        putVec(SlopeLimiters::computeSlopeVector(a, b, c, bias, limiter, mp), o); o += 9;
+        putVec(MHDStateVector::computeXFluxVector(b.computeConservedVariableVector(mp), mp), o); o += 9;
+        putVec(MHDStateVector::computeYFluxVector(b.computeConservedVariableVector(mp), mp), o); o += 9;
+        putMhdState(MHDSolvers::evolveStateByHalfXTimeStep(a, b, c, dx, ts, bias, limiter, 0, mp), o); o += 9;
+        putMhdState(MHDSolvers::evolveStateByHalfXTimeStep(a, b, c, dx, ts, bias, limiter, 1, mp), o); o += 9;
+        putVec(MHDFirstOrderSolver::computeXFORCEFlux(b, c, dx, ts, mp), o); o += 9;
+        putVec(MHDSecondOrderSolver::computeXSLICFlux(a, b, c, d, dx, ts, bias, limiter, mp), o); o += 9;
+        {
+            std::vector<MHDStateVector> one(1, b);
+            *o++ = MHDSolvers::computeMaximumWaveSpeed(one, mp);
+            std::vector<std::vector<MHDStateVector> > one2(1, one);
+            *o++ = MHDSolvers::computeMaximumWaveSpeed2D(one2, mp);
+        }
+        std::vector<double> cb = b.computeConservedVariableVector(mp);
+        putVec(cb, o); o += 9;
+        MHDStateVector back;
+        back.setConservedVariableVector(cb, mp);
+        putMhdState(back, o); o += 9;
+        *o++ = b.computeXFastMagnetoAcousticSpeed(mp);
+        *o++ = b.computeAlfvenWaveSpeed();
+    }
+    writeDoubles(argv[11], out);
+    return 0;
+}
+
 } // namespace
 
 int main(int argc, char **argv)
@@ -294,6 +469,8 @@ int main(int argc, char **argv)
     if (mode == "euler_run") return eulerRun(argc, argv);
     if (mode == "euler_units") return eulerUnits(argc, argv);
     if (mode == "limiter_units") return limiterUnits(argc, argv);
+    if (mode == "mhd_run") return mhdRun(argc, argv);
+    if (mode == "mhd_units") return mhdUnits(argc, argv);
     std::fprintf(stderr, "unknown mode %s\n", mode.c_str());
     return 2;
 }

@@ -1,0 +1,52 @@
+/* Generic SLIC / FORCE scheme of the oracle, over a physics table.
+ * TEST INFRASTRUCTURE ONLY (see oracle.h). */
+#ifndef VG_SCHEME_ORACLE_H
+#define VG_SCHEME_ORACLE_H
+
+#define ORC_MAX_NF 17
+
+typedef struct orc_phys orc_phys;
+struct orc_phys {
+    int nf;
+    /* primitive <-> conserved, in the reference's operation order */
+    void (*prim_to_cons)(const orc_phys *ph, const double *w, double *U);
+    void (*cons_to_prim)(const orc_phys *ph, const double *U, double *w);
+    /* flux of a conserved vector, dir 0 = x, 1 = y */
+    void (*flux)(const orc_phys *ph, int dir, const double *U, double *F);
+    /* per-cell wave speed entering the CFL maximum (dim 1 or 2) */
+    double (*wave_speed)(const orc_phys *ph, const double *w, int dim);
+    /* the state's default constructor (2D corner ghosts) */
+    const double *default_state;
+    /* called once per step with the step's maximum wave speed (MHD sets c_h) */
+    void (*begin_step)(orc_phys *ph, double smax);
+    void *params;
+};
+
+double orc_smin(double a, double b);
+double orc_smax(double a, double b);
+double orc_limiter_value(double r, double bias, int limiter);
+double orc_r_coefficient(double r, double bias);
+void orc_slope_vector(int nf, const double *UL, const double *UM, const double *UR, double bias, int limiter,
+                      double *d);
+void orc_evolve_half(const orc_phys *ph, int dir, const double *wl, const double *wm, const double *wr, double dx,
+                     double ts, double bias, int limiter, int side, double *out);
+void orc_force_flux(const orc_phys *ph, int dir, const double *wl, const double *wr, double dx, double ts,
+                    double *F);
+void orc_lax_friedrichs(int nf, const double *Ul, const double *Ur, const double *Fl, const double *Fr, double dx,
+                        double ts, double *out);
+void orc_richtmyer_state(int nf, const double *Ul, const double *Ur, const double *Fl, const double *Fr, double dx,
+                         double ts, double *out);
+void orc_slic_flux(const orc_phys *ph, int dir, const double *wa, const double *wb, const double *wc,
+                   const double *wd, double dx, double ts, double bias, int limiter, double *F);
+/* One sweep over a 1D (dim 1) or 2D grid of primitive states, AoS. */
+void orc_sweep(const orc_phys *ph, int order, int dim, int dir, int ny, int nx, double dx, double ts, double bias,
+               int limiter, int bc, const double *prim_in, double *prim_out);
+/* Maximum over the ghosted grid, 2D transmissive corners included. */
+double orc_ghosted_max(const orc_phys *ph, int dim, int ny, int nx, int bc, const double *prim);
+double orc_stable_time_step(double S, double dx, double cfl, double t, double T, int it);
+/* The solve loop; returns steps taken. */
+int orc_solve(orc_phys *ph, int order, int dim, int ny, int nx, double dx, double cfl, double t_final,
+              int max_steps, double bias, int limiter, int bc, const double *prim_in, double *prim_out,
+              double *dt_log, int dt_cap);
+
+#endif

@@ -139,7 +139,106 @@ def rect_2d(ny, nx):
     return w
 
 
+def mhd_call(w, *, order, dim, dx, cfl, tfinal, max_steps, bias, limiter, gamma, p_inf, check_api):
+    with tempfile.TemporaryDirectory() as tmp:
+        ny, nx = (1, w.shape[0]) if dim == 1 else w.shape[:2]
+        np.ascontiguousarray(w, dtype=np.float64).tofile(os.path.join(tmp, "in.f64"))
+        t = "inf" if np.isinf(tfinal) else repr(float(tfinal))
+        log = run(["mhd_run", order, dim, ny, nx, repr(dx), repr(cfl), t, max_steps, repr(bias), limiter, repr(gamma),
+                   repr(p_inf), "in.f64", "state.f64", "dt.f64", int(check_api)], tmp)
+        state = np.fromfile(os.path.join(tmp, "state.f64")).reshape(w.shape)
+        dt = np.fromfile(os.path.join(tmp, "dt.f64"))
+    if check_api and max_steps == 0 and order == 2 and dim == 1 and "API_MATCH 1" not in log:
+        raise RuntimeError("MHD mirror loop differs from the reference's own solve")
+    return state, dt
+
+
+def mhd_run(name, w, **kw):
+    state, dt = mhd_call(w, **kw)
+    meta = dict(kw)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), prim_in=w, prim_out=state, dt=dt,
+                        meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
+    print(f"{name}: {len(dt)} steps")
+
+
+def mhd_batch_fixture(name, n_inst, seed, n_min, n_max, tfinal):
+    prim, offsets, gamma, dx = problems.mhd_batch(n_inst, seed=seed, n_min=n_min, n_max=n_max)
+    out = np.empty_like(prim)
+    steps = np.empty(n_inst, dtype=np.int64)
+    tend = np.empty(n_inst)
+    for q in range(n_inst):
+        a, b = offsets[q], offsets[q + 1]
+        s, d = mhd_call(prim[a:b], order=2, dim=1, dx=float(dx[q]), cfl=0.8, tfinal=tfinal, max_steps=0, bias=0.0,
+                        limiter=1, gamma=float(gamma[q]), p_inf=0.0, check_api=False)
+        out[a:b] = s
+        steps[q] = len(d)
+        t = 0.0
+        for x in d:
+            t += x
+        tend[q] = t
+    meta = dict(n_inst=n_inst, seed=seed, n_min=n_min, n_max=n_max, tfinal=tfinal, cfl=0.8, limiter=1, bias=0.0)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), prim_in=prim, prim_out=out, offsets=offsets, gamma=gamma,
+                        dx=dx, steps=steps, t=tend, meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
+    print(f"{name}: {n_inst} instances, {int(steps.sum())} steps")
+
+
+def mhd_units(name, n, seed, bias, limiter, gamma, ch):
+    rng = np.random.default_rng(seed)
+    base = np.empty((n, 9))
+    base[:, 0] = rng.uniform(0.2, 2.0, n)
+    base[:, 1:4] = rng.uniform(-1.0, 1.0, (n, 3))
+    base[:, 4] = rng.uniform(0.2, 2.0, n)
+    base[:, 5:8] = rng.uniform(-1.5, 1.5, (n, 3))
+    base[:, 8] = rng.uniform(-0.1, 0.1, n)
+    st = np.empty((n, 5, 9))
+    for k in range(5):
+        scale = rng.choice([1e-7, 1e-6, 1e-3, 0.05, 0.3], size=(n, 1))
+        s = base + rng.uniform(-1.0, 1.0, (n, 9)) * scale * np.abs(base).clip(0.1)
+        s[:, 0] = np.abs(s[:, 0]) + 0.05
+        s[:, 4] = np.abs(s[:, 4]) + 0.05
+        st[:, k] = s
+    states = st.reshape(n, 45)
+    dx, ts = 1e-2, 2e-3
+    with tempfile.TemporaryDirectory() as tmp:
+        states.tofile(os.path.join(tmp, "in.f64"))
+        run(["mhd_units", n, repr(bias), limiter, repr(dx), repr(ts), repr(gamma), "0.0", repr(ch), "in.f64",
+             "out.f64"], tmp)
+        out = np.fromfile(os.path.join(tmp, "out.f64")).reshape(n, 85)
+    meta = dict(bias=bias, limiter=limiter, dx=dx, ts=ts, gamma=gamma, p_inf=0.0, ch=ch)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), states=states, out=out,
+                        meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
+    print(f"{name}: {n} records")
+
+
+def mhd_2d_ic(n):
+    """Orszag-Tang-like smooth vortex with a density step (2D MHD fixture)."""
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x)
+    w = np.zeros((n, n, 9))
+    w[..., 0] = np.where(X < 0.5, 1.0, 0.5)
+    w[..., 1] = -np.sin(2 * np.pi * Y)
+    w[..., 2] = np.sin(2 * np.pi * X)
+    w[..., 3] = 0.1
+    w[..., 4] = 0.6
+    w[..., 5] = -np.sin(2 * np.pi * Y) * 0.5
+    w[..., 6] = np.sin(4 * np.pi * X) * 0.5
+    w[..., 7] = 0.2
+    return w
+
+
 JOBS = {
+    "mhd_briowu256": lambda: mhd_run("mhd_briowu256", problems.brio_wu(256), order=2, dim=1, dx=1.0 / 256, cfl=0.8,
+                                     tfinal=0.1, max_steps=0, bias=0.0, limiter=1, gamma=2.0, p_inf=0.0,
+                                     check_api=True),
+    "mhd_force_briowu128": lambda: mhd_run("mhd_force_briowu128", problems.brio_wu(128), order=1, dim=1,
+                                           dx=1.0 / 128, cfl=0.8, tfinal=0.1, max_steps=0, bias=0.0, limiter=0,
+                                           gamma=2.0, p_inf=0.0, check_api=False),
+    "mhd_2d_ot32": lambda: mhd_run("mhd_2d_ot32", mhd_2d_ic(32), order=2, dim=2, dx=1.0 / 32, cfl=0.6,
+                                   tfinal=np.inf, max_steps=30, bias=0.0, limiter=0, gamma=5.0 / 3.0, p_inf=0.0,
+                                   check_api=False),
+    "mhd_batch12": lambda: mhd_batch_fixture("mhd_batch12", 12, 3, 64, 256, 0.02),
+    "mhd_units": lambda: mhd_units("mhd_units", 500, 21, 0.0, 1, 2.0, 1.7),
+
     # config C1 exactly (1000 cells, minbee, CFL 0.9, T 0.25); also checks the
     # reference's own SecondOrderSolver::solve against the mirror loop.
     "euler_c1_sod1000": lambda: euler_run("euler_c1_sod1000", problems.sod_1d(1000), order=2, dim=1, dx=1e-3,

@@ -46,8 +46,48 @@ def load():
                                     C.c_int, C.c_int, C.POINTER(OrcEulerParams), dptr, dptr]
     lib.orc_euler_max_wave_speed.restype = C.c_double
     lib.orc_euler_max_wave_speed.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(OrcEulerParams), dptr]
+    lib.orc_mhd_solve.restype = C.c_int
+    lib.orc_mhd_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                  C.c_double, C.c_int, C.c_int, C.POINTER(OrcMHDParams), dptr, dptr, dptr, C.c_int]
+    lib.orc_mhd_units.restype = None
+    lib.orc_mhd_units.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.POINTER(OrcMHDParams),
+                                  dptr, dptr]
     _lib = lib
     return lib
+
+
+class OrcMHDParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("gamma", "p_inf", "ch", "cp")]
+
+
+def mhd_solve(prim, *, order, dim, dx, cfl, tfinal, max_steps, bias, limiter, gamma, p_inf=0.0, bc=0,
+              dt_cap=200000):
+    lib = load()
+    w = np.ascontiguousarray(prim, dtype=np.float64)
+    out = np.empty_like(w)
+    dts = np.empty(dt_cap)
+    ny, nx = (1, w.shape[0]) if dim == 1 else w.shape[:2]
+    mp = OrcMHDParams(gamma, p_inf, 0.0, 0.0)
+    n = lib.orc_mhd_solve(order, dim, ny, nx, dx, cfl, tfinal, max_steps, bias, limiter, bc, C.byref(mp), _p(w),
+                          _p(out), _p(dts), dt_cap)
+    return out, dts[:min(n, dt_cap)].copy()
+
+
+def mhd_units(states, *, bias, limiter, dx, ts, gamma, p_inf, ch):
+    lib = load()
+    s = np.ascontiguousarray(states, dtype=np.float64)
+    out = np.empty((s.shape[0], 85))
+    mp = OrcMHDParams(gamma, p_inf, ch, 0.0)
+    lib.orc_mhd_units(s.shape[0], bias, limiter, dx, ts, C.byref(mp), _p(s), _p(out))
+    return out
+
+
+def seq_sum(x):
+    """Left-to-right floating-point sum (the solve loop's t += dt)."""
+    t = 0.0
+    for v in x:
+        t += float(v)
+    return t
 
 
 def _p(a):

@@ -1,0 +1,105 @@
+"""MHD on the GPU through the C ABI, and the batched persistent solves.
+
+EXACT mode is bit-identical to the compiled reference's golden vectors (1D with
+the per-step GLM speed, FORCE, 2D) and to the oracle; the batched launch gives
+every instance exactly what its own solve call gives.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from vanguard_amd import _lib, problems
+from vanguard_amd import solvers as S
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", ["mhd_briowu256", "mhd_force_briowu128", "mhd_2d_ot32"])
+def test_mhd_exact_bit_identical_to_reference(vg_ctx, name):
+    g = O.golden(name)
+    m = g["meta"]
+    r = S.mhd_solve(g["prim_in"], m["dim"], m["dx"], m["cfl"], m["tfinal"], order=m["order"], bias=m["bias"],
+                    limiter=m["limiter"], params=S.MHDMaterialParameters(m["gamma"], m["p_inf"]),
+                    max_steps=m["max_steps"])
+    print(f"{name}: steps={r.stats.steps}", flush=True)
+    O.assert_bitwise(r.dt, g["dt"], f"{name} dt")
+    O.assert_bitwise(r.cells, g["prim_out"], f"{name} state")
+
+
+def test_mhd_batch_bit_identical_to_reference(vg_ctx):
+    g = O.golden("mhd_batch12")
+    m = g["meta"]
+    r = S.mhd_solve_batch(g["prim_in"], g["offsets"], g["dx"], g["gamma"], m["tfinal"], m["cfl"],
+                          limiter=m["limiter"])
+    np.testing.assert_array_equal(r.steps, g["steps"])
+    O.assert_bitwise(r.t, g["t"], "final times")
+    O.assert_bitwise(r.cells, g["prim_out"], "states")
+
+
+def test_mhd_batch_uneven_sizes_match_oracle(vg_ctx):
+    """Sizes from 2 cells to a few thousand, mixed in one launch, in the
+    config-3 recipe; more instances than one wave of workgroups."""
+    prim, off, gamma, dx = problems.mhd_batch(600, seed=11, n_min=2, n_max=3000)
+    tf = np.full(600, 2e-3)
+    r = S.mhd_solve_batch(prim, off, dx, gamma, tf, 0.8, limiter=1)
+    rng = np.random.default_rng(0)
+    for q in rng.choice(600, 12, replace=False):
+        a, b = off[q], off[q + 1]
+        ref, dts = O.mhd_solve(prim[a:b], order=2, dim=1, dx=float(dx[q]), cfl=0.8, tfinal=2e-3, max_steps=0,
+                               bias=0.0, limiter=1, gamma=float(gamma[q]))
+        assert r.steps[q] == len(dts)
+        assert r.t[q] == O.seq_sum(dts)
+        O.assert_bitwise(r.cells[a:b], ref, f"instance {q} (n={b - a})")
+
+
+def test_mhd_fast_mode_within_tolerance(vg_ctx):
+    g = O.golden("mhd_briowu256")
+    m = g["meta"]
+    r = S.mhd_solve(g["prim_in"], 1, m["dx"], m["cfl"], m["tfinal"], limiter=m["limiter"],
+                    params=S.MHDMaterialParameters(m["gamma"]), mode=S.FAST)
+    assert r.stats.steps == len(g["dt"])
+    for f in range(9):
+        ref = g["prim_out"][:, f]
+        if np.abs(ref).sum() > 0:
+            assert np.abs(r.cells[:, f] - ref).sum() / np.abs(ref).sum() <= 1e-10, f
+
+
+def test_euler_batch_matches_single_solves(vg_ctx):
+    rng = np.random.default_rng(4)
+    sizes = rng.integers(2, 2000, 40)
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    prim = np.concatenate([problems.sod_1d(int(n)) for n in sizes])
+    dx = 1.0 / sizes
+    tf = rng.uniform(0.01, 0.05, 40)
+    params = [S.EulerMaterialParameters(float(g)) for g in rng.uniform(1.2, 1.8, 40)]
+    r = S.euler_solve_batch(prim, off, dx, params, tf, 0.9, limiter=2)
+    for q in range(0, 40, 7):
+        a, b = off[q], off[q + 1]
+        ref, dts = O.euler_solve(prim[a:b], order=2, dim=1, dx=float(dx[q]), cfl=0.9, tfinal=float(tf[q]),
+                                 max_steps=0, bias=0.0, limiter=2, params=[params[q].gamma, 0, 25, 0, 0.25, 250])
+        assert r.steps[q] == len(dts)
+        O.assert_bitwise(r.cells[a:b], ref, f"euler instance {q}")
+
+
+def test_euler_1d_device_state_path(vg_ctx):
+    """The multi-workgroup grid path for 1D (device-resident state) agrees with the
+    persistent single-workgroup path used by vg_euler_solve_1d."""
+    w = problems.sod_1d(3000)
+    grid = S.EulerGrid(1, 1, 3000)
+    grid.load(w)
+    st, d = grid.run(1 / 3000, 0.9, 0.05, limiter=2, dt_cap=10000)
+    one = S.euler_solve(w, 1, 1 / 3000, 0.9, 0.05, limiter=2)
+    O.assert_bitwise(d, one.dt, "dt")
+    O.assert_bitwise(grid.store(), one.cells, "state")
+    grid.close()
+
+
+def test_batch_rejects_bad_offsets(vg_ctx):
+    prim = np.zeros((10, 9))
+    prim[:, 0] = 1
+    prim[:, 4] = 1
+    r = S.mhd_solve_batch(prim, [0, 1, 10], [0.1, 0.1], [2.0, 2.0], [0.1, 0.1], 0.8, raise_on_error=False)
+    assert r.status == _lib.VG_ERR_ARG

@@ -30,6 +30,10 @@ class EulerParams(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("gamma", "p_inf", "cv", "e_form", "t_ign", "k_rate")]
 
 
+class MHDParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("gamma", "p_inf", "ch", "cp")]
+
+
 class RunStats(C.Structure):
     _fields_ = [("steps", C.c_int32), ("total_steps", C.c_int32), ("t", C.c_double), ("dt_min", C.c_double),
                 ("dt_max", C.c_double), ("wall_ms", C.c_double), ("device_ms", C.c_double),
@@ -60,6 +64,18 @@ SIGNATURES = {
                                      C.c_int, C.c_int, C.POINTER(EulerParams), C.c_int, dptr, C.c_int,
                                      C.POINTER(RunStats)]),
     "vg_euler_state_max_wave_speed": (C.c_int, [vptr, C.c_int, C.POINTER(EulerParams), dptr]),
+    "vg_euler_solve_batch": (C.c_int, [vptr, C.c_int, C.POINTER(C.c_int64), dptr, dptr, dptr,
+                                       C.POINTER(EulerParams), dptr, C.c_double, C.c_int, C.c_double, C.c_int,
+                                       C.c_int, C.c_int, C.POINTER(C.c_int32), dptr, C.POINTER(RunStats)]),
+    "vg_mhd_solve_1d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                  C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(MHDParams), C.c_int, dptr,
+                                  C.c_int, C.POINTER(RunStats)]),
+    "vg_mhd_solve_2d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                  C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(MHDParams), C.c_int, dptr,
+                                  C.c_int, C.POINTER(RunStats)]),
+    "vg_mhd_solve_batch": (C.c_int, [vptr, C.c_int, C.POINTER(C.c_int64), dptr, dptr, dptr, C.POINTER(MHDParams),
+                                     dptr, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
+                                     C.POINTER(C.c_int32), dptr, C.POINTER(RunStats)]),
 }
 
 _lib = None

@@ -79,7 +79,8 @@ struct Clock {
     int pending;         // a step was started and t/it not yet advanced
     double dt_min, dt_max;
     int steps_this_run;
-    int pad;
+    int stalled;         // set when a step would not advance t (dt <= 0 or NaN while t < t_final)
+    double S;            // maximum wave speed of the step in progress (MHD: its c_h)
 };
 
 }  // namespace vg

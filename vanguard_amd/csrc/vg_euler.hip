@@ -2,6 +2,9 @@
 #include <cmath>
 #include <cstring>
 
+#include <vector>
+
+#include "vg_batch_host.h"
 #include "vg_grid.h"
 #include "vg_phys_euler.h"
 
@@ -12,6 +15,9 @@ struct vg_euler_state {
 };
 
 namespace {
+
+// 1D grids up to this size run as one persistent workgroup.
+constexpr int kPersistent1D = 32768;
 
 vg::EulerPhys::Params to_params(const vg_euler_params *p)
 {
@@ -57,6 +63,38 @@ int solve(vg_ctx *ctx, int dim, const double *prim_in, double *prim_out, int ny,
     if (rc) return rc;
     rc = vg::bind(ctx);
     if (rc) return rc;
+    if (dim == 1 && nx <= kPersistent1D) {
+        // A small 1D grid runs its whole time loop in one persistent workgroup:
+        // one launch instead of a launch per step.
+        const int64_t offsets[2] = {0, nx};
+        std::vector<vg::EulerPhys::Params> params(1, to_params(mp));
+        int32_t steps = 0;
+        double tend = 0.0;
+        vg_run_stats local{};
+        rc = vg::run_batch<vg::EulerPhys, 256>(ctx, order, mode, 1, offsets, prim_in, prim_out, &dx, params, &t_final,
+                                               cfl, max_steps, bias, limiter, bc, dt_log, dt_log_cap, &steps, &tend,
+                                               &local);
+        local.steps = local.total_steps = steps;
+        local.t = tend;
+        local.first_nonfinite = -1;
+        for (int j = 0; j < nx && rc == VG_OK; j++) {
+            const double *w = prim_out + 6 * (size_t)j;
+            bool bad = !(w[0] > 0.0) || !((w[4] + mp->p_inf) > 0.0);
+            for (int f = 0; f < 6; f++) bad = bad || !std::isfinite(w[f]);
+            if (bad) {
+                if (local.first_nonfinite < 0) local.first_nonfinite = j;
+                local.nonfinite_cells += 1;
+            }
+        }
+        if (stats) *stats = local;
+        if (!rc && local.nonfinite_cells > 0) {
+            vg::set_error("final state holds " + std::to_string(local.nonfinite_cells) +
+                          " non-finite or inadmissible cells, first at flat index " +
+                          std::to_string(local.first_nonfinite));
+            return VG_ERR_NONFINITE;
+        }
+        return rc;
+    }
     EulerGrid g;
     rc = g.create(ctx, dim, ny, nx);
     if (!rc) rc = g.load(prim_in, 0);
@@ -180,3 +218,19 @@ int vg_euler_state_max_wave_speed(vg_euler_state *st, int bc, const vg_euler_par
 }
 
 }  // extern "C"
+
+extern "C" int vg_euler_solve_batch(vg_ctx *ctx, int n_inst, const int64_t *offsets, const double *prim_in,
+                                    double *prim_out, const double *dx, const vg_euler_params *mp,
+                                    const double *t_final, double cfl, int max_steps, double bias, int limiter,
+                                    int order, int mode, int32_t *steps, double *t_out, vg_run_stats *stats)
+{
+    vg::clear_error();
+    VG_CHECK(ctx != nullptr && mp != nullptr && n_inst >= 1, "null argument");
+    int rc = vg::bind(ctx);
+    if (rc) return rc;
+    std::vector<vg::EulerPhys::Params> params(n_inst);
+    for (int q = 0; q < n_inst; q++) params[q] = to_params(&mp[q]);
+    return vg::run_batch<vg::EulerPhys, 256>(ctx, order, mode, n_inst, offsets, prim_in, prim_out, dx, params, t_final,
+                                             cfl, max_steps, bias, limiter, VG_BC_TRANSMISSIVE, nullptr, 0, steps,
+                                             t_out, stats);
+}

@@ -1,0 +1,103 @@
+// MHD C ABI: 1D and 2D solve drop-ins and the batched persistent solve.
+#include <cmath>
+#include <vector>
+
+#include "vg_batch_host.h"
+#include "vg_grid.h"
+#include "vg_phys_mhd.h"
+
+namespace {
+
+using vg::MHDPhys;
+
+MHDPhys::Params to_params(const vg_mhd_params *p)
+{
+    MHDPhys::Params m;
+    m.gamma = p->gamma;
+    m.p_inf = p->p_inf;
+    m.ch = p->ch;
+    m.cp = p->cp;
+    m.gm1 = p->gamma - 1;
+    m.gpinf = p->gamma * p->p_inf;
+    m.inv_gm1 = 1.0 / (p->gamma - 1);
+    return m;
+}
+
+constexpr int kBatchBS = 128;
+
+}  // namespace
+
+extern "C" {
+
+int vg_mhd_solve_1d(vg_ctx *ctx, const double *prim_in, double *prim_out, int n, double dx, double cfl,
+                    double t_final, int max_steps, double bias, int limiter, int order, int bc,
+                    const vg_mhd_params *mp, int mode, double *dt_log, int dt_log_cap, vg_run_stats *stats)
+{
+    vg::clear_error();
+    VG_CHECK(ctx != nullptr, "null context");
+    VG_CHECK(mp != nullptr, "null material parameters");
+    int rc = vg::bind(ctx);
+    if (rc) return rc;
+    const int64_t offsets[2] = {0, n};
+    std::vector<MHDPhys::Params> params(1, to_params(mp));
+    int32_t steps = 0;
+    double tend = 0.0;
+    rc = vg::run_batch<MHDPhys, kBatchBS>(ctx, order, mode, 1, offsets, prim_in, prim_out, &dx, params, &t_final, cfl,
+                                          max_steps, bias, limiter, bc, dt_log, dt_log_cap, &steps, &tend, stats);
+    if (stats) {
+        stats->steps = steps;
+        stats->total_steps = steps;
+        stats->t = tend;
+    }
+    return rc;
+}
+
+int vg_mhd_solve_2d(vg_ctx *ctx, const double *prim_in, double *prim_out, int ny, int nx, double dx, double cfl,
+                    double t_final, int max_steps, double bias, int limiter, int order, int bc,
+                    const vg_mhd_params *mp, int mode, double *dt_log, int dt_log_cap, vg_run_stats *stats)
+{
+    vg::clear_error();
+    VG_CHECK(ctx != nullptr && prim_in != nullptr && prim_out != nullptr, "null argument");
+    VG_CHECK(mp != nullptr, "null material parameters");
+    VG_CHECK(order == 1 || order == 2, "order must be 1 (FORCE) or 2 (SLIC)");
+    VG_CHECK(mode == VG_MODE_EXACT || mode == VG_MODE_FAST, "unknown mode");
+    VG_CHECK(bc == VG_BC_TRANSMISSIVE || bc == VG_BC_PERIODIC, "unknown boundary condition");
+    VG_CHECK(nx >= 2 && ny >= 2, "grid needs at least 2 cells per direction");
+    VG_CHECK(dx > 0.0 && cfl > 0.0, "cell spacing and CFL must be positive");
+    VG_CHECK(max_steps >= 0 && (max_steps > 0 || std::isfinite(t_final)), "an infinite final time needs a step cap");
+    int rc = vg::bind(ctx);
+    if (rc) return rc;
+    vg::GridState<MHDPhys> g;
+    rc = g.create(ctx, 2, ny, nx);
+    if (!rc) rc = g.load(prim_in, 0);
+    vg_run_stats local{};
+    if (!rc) rc = g.run(order, mode, dx, cfl, t_final, max_steps, bias, limiter, bc, to_params(mp), dt_log, dt_log_cap, &local);
+    if (!rc) rc = g.store(prim_out, 0);
+    if (!rc) rc = g.scan_nonfinite(to_params(mp), &local.nonfinite_cells, &local.first_nonfinite);
+    g.destroy();
+    if (stats) *stats = local;
+    if (!rc && local.nonfinite_cells > 0) {
+        vg::set_error("final state holds non-finite or inadmissible cells, first at flat index " +
+                      std::to_string(local.first_nonfinite));
+        return VG_ERR_NONFINITE;
+    }
+    return rc;
+}
+
+int vg_mhd_solve_batch(vg_ctx *ctx, int n_inst, const int64_t *offsets, const double *prim_in, double *prim_out,
+                       const double *dx, const vg_mhd_params *mp, const double *t_final, double cfl, int max_steps,
+                       double bias, int limiter, int order, int mode, int32_t *steps, double *t_out,
+                       vg_run_stats *stats)
+{
+    vg::clear_error();
+    VG_CHECK(ctx != nullptr && mp != nullptr && n_inst >= 1, "null argument");
+    int rc = vg::bind(ctx);
+    if (rc) return rc;
+    std::vector<MHDPhys::Params> params(n_inst);
+    for (int q = 0; q < n_inst; q++) params[q] = to_params(&mp[q]);
+    return vg::run_batch<MHDPhys, kBatchBS>(ctx, order, mode, n_inst, offsets, prim_in, prim_out, dx, params, t_final,
+                                            cfl, max_steps, bias, limiter, VG_BC_TRANSMISSIVE, nullptr, 0, steps,
+                                            t_out, stats);
+}
+
+}  // extern "C"

@@ -21,6 +21,9 @@ struct EulerPhys {
         double gm1, gpinf, inv_gm1;
     };
 
+    // Euler has no parameter that follows the time step.
+    __device__ static __forceinline__ void begin_step(Params &, double) {}
+
     // Specific internal energy, pressure and sound speed of the stiffened gas
     // (Euler/eulerequationofstate.cpp:7-29).
     __device__ static __forceinline__ double eos_e(const Params &m, double rho, double p)

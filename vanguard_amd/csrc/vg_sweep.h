@@ -81,20 +81,105 @@ __device__ __forceinline__ void block_max_to(unsigned long long *partials, unsig
     }
 }
 
+// One tile of an x sweep: thread tid holds the primitive state w of cell
+// j = j0 - G + tid (boundary-mapped by the caller); threads G .. BS-G-1 with
+// `write` set compute and store the updated cell at out[f * plane + out_idx].
+// Returns the wave-speed key of the stored cell when want_key (else 0).  Ends
+// with a barrier, so the LDS arrays can be reused by the next tile.
+template <class P, int ORDER, int MODE, int BS>
+__device__ __forceinline__ unsigned long long row_tile(const typename P::Params &m, const StepScalars &sc, double bias,
+                                                       int limiter, const double (&w)[P::NF], bool write,
+                                                       double *__restrict__ out, long long plane, long long out_idx,
+                                                       int dim, bool want_key, double *s_a, double *s_e)
+{
+    constexpr int NF = P::NF;
+    constexpr int G = (ORDER == 2) ? 2 : 1;
+    using S = Scheme<P, 0, MODE>;
+    const int tid = threadIdx.x;
+    double U[NF], Fp[NF];
+    if (ORDER == 2) {
+        S::cons(m, w, U);
+#pragma unroll
+        for (int f = 0; f < NF; f++) s_a[f * BS + tid] = U[f];
+        __syncthreads();
+        ConsFlux<NF> left, right;
+        if (tid >= 1 && tid <= BS - 2) {
+            double UL[NF], UR[NF];
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                UL[f] = s_a[f * BS + tid - 1];
+                UR[f] = s_a[f * BS + tid + 1];
+            }
+            S::evolve(m, UL, U, UR, bias, limiter, sc.half_c, left, right);
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                s_e[f * BS + tid] = left.U[f];
+                s_e[(NF + f) * BS + tid] = left.F[f];
+            }
+        }
+        __syncthreads();
+        if (tid >= 1 && tid <= BS - 3) {
+            ConsFlux<NF> nb;
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                nb.U[f] = s_e[f * BS + tid + 1];
+                nb.F[f] = s_e[(NF + f) * BS + tid + 1];
+            }
+            S::force_face(m, right, nb, sc.lf_c, sc.ri_c, Fp);   // face j + 1/2
+#pragma unroll
+            for (int f = 0; f < NF; f++) s_a[f * BS + tid] = Fp[f];
+        }
+        __syncthreads();
+    } else {
+        ConsFlux<NF> cf;
+        S::cons_flux_of_prim(m, w, cf);
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            U[f] = cf.U[f];
+            s_e[f * BS + tid] = cf.U[f];
+            s_e[(NF + f) * BS + tid] = cf.F[f];
+        }
+        __syncthreads();
+        if (tid <= BS - 2) {
+            ConsFlux<NF> nb;
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                nb.U[f] = s_e[f * BS + tid + 1];
+                nb.F[f] = s_e[(NF + f) * BS + tid + 1];
+            }
+            S::force_face(m, cf, nb, sc.lf_c, sc.ri_c, Fp);
+#pragma unroll
+            for (int f = 0; f < NF; f++) s_a[f * BS + tid] = Fp[f];
+        }
+        __syncthreads();
+    }
+    unsigned long long key = 0;
+    if (tid >= G && tid < BS - G && write) {
+        double Fm[NF], wn[NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++) Fm[f] = s_a[f * BS + tid - 1];
+        S::update(m, U, Fm, Fp, sc.up_c, wn);
+        store_prim<NF>(out, plane, out_idx, wn);
+        if (want_key) key = max_key(P::wave_speed(m, wn, dim));
+    }
+    __syncthreads();
+    return key;
+}
+
 template <class P, int ORDER, int MODE, int BS, int RX>
 __global__ __launch_bounds__(BS) void sweep_x_kernel(SweepArgs<P> a)
 {
     constexpr int NF = P::NF;
     constexpr int G = (ORDER == 2) ? 2 : 1;
     constexpr int OUT = BS - 2 * G;
-    using S = Scheme<P, 0, MODE>;
     __shared__ double s_a[NF * BS];       // cell conserved vectors, later face fluxes
     __shared__ double s_e[2 * NF * BS];   // left evolved state (SLIC) / cell state (FORCE): U and F
 
     if (a.clk->done) return;
     const double ts = a.clk->dt * a.dt_scale;
     const StepScalars sc(ts, a.dx);
-    const typename P::Params m = a.m;
+    typename P::Params m = a.m;
+    P::begin_step(m, a.clk->S);
 
     const int tid = threadIdx.x;
     const int j = blockIdx.x * OUT - G + tid;
@@ -110,81 +195,13 @@ __global__ __launch_bounds__(BS) void sweep_x_kernel(SweepArgs<P> a)
     load_prim<NF>(a.in, a.plane, (long long)i_begin * nx + jj, wnext);
     for (int i = i_begin; i < i_end; i++) {
         const long long row = (long long)i * nx;
-        double w[NF], U[NF];
+        double w[NF];
 #pragma unroll
         for (int f = 0; f < NF; f++) w[f] = wnext[f];
         if (i + 1 < i_end) load_prim<NF>(a.in, a.plane, row + nx + jj, wnext);
-        double Fp[NF];
-
-        if (ORDER == 2) {
-            S::cons(m, w, U);
-#pragma unroll
-            for (int f = 0; f < NF; f++) s_a[f * BS + tid] = U[f];
-            __syncthreads();
-            ConsFlux<NF> left, right;
-            if (tid >= 1 && tid <= BS - 2) {
-                double UL[NF], UR[NF];
-#pragma unroll
-                for (int f = 0; f < NF; f++) {
-                    UL[f] = s_a[f * BS + tid - 1];
-                    UR[f] = s_a[f * BS + tid + 1];
-                }
-                S::evolve(m, UL, U, UR, a.bias, a.limiter, sc.half_c, left, right);
-#pragma unroll
-                for (int f = 0; f < NF; f++) {
-                    s_e[f * BS + tid] = left.U[f];
-                    s_e[(NF + f) * BS + tid] = left.F[f];
-                }
-            }
-            __syncthreads();
-            if (tid >= 1 && tid <= BS - 3) {
-                ConsFlux<NF> nb;
-#pragma unroll
-                for (int f = 0; f < NF; f++) {
-                    nb.U[f] = s_e[f * BS + tid + 1];
-                    nb.F[f] = s_e[(NF + f) * BS + tid + 1];
-                }
-                S::force_face(m, right, nb, sc.lf_c, sc.ri_c, Fp);   // face j + 1/2
-#pragma unroll
-                for (int f = 0; f < NF; f++) s_a[f * BS + tid] = Fp[f];
-            }
-            __syncthreads();
-        } else {
-            ConsFlux<NF> cf;
-            S::cons_flux_of_prim(m, w, cf);
-#pragma unroll
-            for (int f = 0; f < NF; f++) {
-                U[f] = cf.U[f];
-                s_e[f * BS + tid] = cf.U[f];
-                s_e[(NF + f) * BS + tid] = cf.F[f];
-            }
-            __syncthreads();
-            if (tid <= BS - 2) {
-                ConsFlux<NF> nb;
-#pragma unroll
-                for (int f = 0; f < NF; f++) {
-                    nb.U[f] = s_e[f * BS + tid + 1];
-                    nb.F[f] = s_e[(NF + f) * BS + tid + 1];
-                }
-                S::force_face(m, cf, nb, sc.lf_c, sc.ri_c, Fp);
-#pragma unroll
-                for (int f = 0; f < NF; f++) s_a[f * BS + tid] = Fp[f];
-            }
-            __syncthreads();
-        }
-
-        if (tid >= G && tid < BS - G && j < nx) {
-            double Fm[NF], wn[NF];
-#pragma unroll
-            for (int f = 0; f < NF; f++) Fm[f] = s_a[f * BS + tid - 1];
-            S::update(m, U, Fm, Fp, sc.up_c, wn);
-            store_prim<NF>(a.out, a.plane, row + j, wn);
-            if (a.fuse_max) {
-                const unsigned long long k2 = max_key(P::wave_speed(m, wn, a.dim));
-                key = (k2 > key) ? k2 : key;
-            }
-        }
-        __syncthreads();   // s_a / s_e are rewritten by the next row
+        const unsigned long long k2 = row_tile<P, ORDER, MODE, BS>(m, sc, a.bias, a.limiter, w, j < nx, a.out, a.plane,
+                                                                   row + j, a.dim, a.fuse_max != 0, s_a, s_e);
+        key = (k2 > key) ? k2 : key;
     }
     if (a.fuse_max) block_max_to<BS>(a.partials, key);
 }
@@ -198,7 +215,8 @@ __global__ __launch_bounds__(BS) void sweep_y_kernel(SweepArgs<P> a)
     if (a.clk->done) return;
     const double ts = a.clk->dt * a.dt_scale;
     const StepScalars sc(ts, a.dx);
-    const typename P::Params m = a.m;
+    typename P::Params m = a.m;
+    P::begin_step(m, a.clk->S);
 
     const int j = blockIdx.x * BS + threadIdx.x;
     const int ny = a.ny, nx = a.nx;
@@ -311,7 +329,7 @@ __device__ __forceinline__ unsigned long long reduce_partials(const unsigned lon
     return mx;
 }
 
-__global__ __launch_bounds__(kReduceThreads) void reduce_max_kernel(const unsigned long long *partials, int n, Clock *clk)
+static __global__ __launch_bounds__(kReduceThreads) void reduce_max_kernel(const unsigned long long *partials, int n, Clock *clk)
 {
     const unsigned long long mx = reduce_partials(partials, n);
     if (threadIdx.x == 0) clk->smax = mx;
@@ -330,7 +348,7 @@ __global__ void corner_speed_kernel(typename P::Params m, int dim, Clock *clk)
 // secondordersolver.cpp:307-308), evaluate the loop condition, then
 // dt = CFL * (dx / S), x0.2 for the first six steps, clipped to the final time
 // (Solvers/solvers.cpp:392, 437-452).
-__global__ __launch_bounds__(kReduceThreads) void clock_begin_kernel(Clock *c, const unsigned long long *partials,
+static __global__ __launch_bounds__(kReduceThreads) void clock_begin_kernel(Clock *c, const unsigned long long *partials,
                                                                     int n_partials, double *dt_log, int dt_cap)
 {
     if (c->done) return;   // uniform: nothing pending once done
@@ -352,7 +370,15 @@ __global__ __launch_bounds__(kReduceThreads) void clock_begin_kernel(Clock *c, c
     double dt = c->cfl * (c->dx / S);
     if (c->it <= 5) dt *= 0.2;
     if ((c->t + dt) > c->t_final) dt = c->t_final - c->t;
+    if (!(dt > 0.0) || !(c->t + dt > c->t)) {
+        // The reference would repeat this step forever (t never reaches t_final);
+        // stop and report instead.
+        c->done = 1;
+        c->stalled = 1;
+        return;
+    }
     c->dt = dt;
+    c->S = S;
     if (dt_log && c->steps_this_run < dt_cap) dt_log[c->steps_this_run] = dt;
     if (c->steps_this_run == 0 || dt < c->dt_min) c->dt_min = dt;
     if (c->steps_this_run == 0 || dt > c->dt_max) c->dt_max = dt;
@@ -360,7 +386,7 @@ __global__ __launch_bounds__(kReduceThreads) void clock_begin_kernel(Clock *c, c
     c->pending = 1;
 }
 
-__global__ void clock_finish_kernel(Clock *c)
+static __global__ void clock_finish_kernel(Clock *c)
 {
     if (c->pending) {
         c->t += c->dt;

@@ -93,3 +93,48 @@ def disc_2d(n: int, gaussians: int = 64, seed: int = 20261202, amplitude: float 
             bump[iy[0]:iy[-1] + 1, jx[0]:jx[-1] + 1] += np.outer(gy, gx)
         w[..., 0] += amplitude * bump
     return w
+
+
+def brio_wu(n: int) -> np.ndarray:
+    """Brio-Wu shock tube as MHDTests::solveDumbserTest1 sets it up
+    (MHD/mhdtests.cpp:12-24), gamma 2: left (1, 0, 0, 0, 1, 0.75, 1, 0, 0),
+    right (0.125, 0, 0, 0, 0.1, 0.75, -1, 0, 0), split at i <= n/2.
+    Primitive order (rho, u, v, w, p, Bx, By, Bz, psi)."""
+    w = np.empty((n, 9))
+    left = np.arange(n) <= 0.5 * n
+    w[left] = (1.0, 0.0, 0.0, 0.0, 1.0, 0.75, 1.0, 0.0, 0.0)
+    w[~left] = (0.125, 0.0, 0.0, 0.0, 0.1, 0.75, -1.0, 0.0, 0.0)
+    return w
+
+
+def mhd_batch(n_inst: int, seed: int = 3, n_min: int = 256, n_max: int = 16384):
+    """Independent 1D ideal-MHD Riemann problems (benchmark config 3 recipe).
+
+    Per instance, 17 draws u_k in [0, 1) from splitmix64(seed) in this order:
+    N = n_min + floor(u (n_max - n_min + 1)); gamma = 1.3 + 0.7 u; Bx = 2u - 1
+    (shared by both sides); then for the left and then the right state:
+    rho = 10^(2u - 1), p = 10^(4u - 2), u, v, w = 4u - 2 each, By, Bz = 4u - 2 each.
+    psi = 0.  Cells i <= N/2 take the left state.  dx = 1/N.
+
+    Returns (prim, offsets, gamma, dx): prim is the concatenation of all
+    instances (total cells x 9), offsets the N+1 cell offsets.
+    """
+    u = uniform01(seed, 17 * n_inst).reshape(n_inst, 17)
+    sizes = (n_min + np.floor(u[:, 0] * (n_max - n_min + 1))).astype(np.int64)
+    sizes = np.minimum(sizes, n_max)
+    gamma = 1.3 + 0.7 * u[:, 1]
+    bx = 2.0 * u[:, 2] - 1.0
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    prim = np.empty((int(offsets[-1]), 9))
+    for q in range(n_inst):
+        side = []
+        for s in range(2):
+            d = u[q, 3 + 7 * s: 10 + 7 * s]
+            side.append((10.0 ** (2 * d[0] - 1), 4 * d[2] - 2, 4 * d[3] - 2, 4 * d[4] - 2, 10.0 ** (4 * d[1] - 2),
+                         bx[q], 4 * d[5] - 2, 4 * d[6] - 2, 0.0))
+        n = int(sizes[q])
+        blk = prim[offsets[q]:offsets[q + 1]]
+        left = np.arange(n) <= 0.5 * n
+        blk[left] = side[0]
+        blk[~left] = side[1]
+    return prim, offsets, gamma, 1.0 / sizes.astype(np.float64)

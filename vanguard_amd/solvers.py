@@ -205,3 +205,131 @@ class EulerGrid:
         check(self.lib.vg_euler_state_max_wave_speed(self.handle, bc, C.byref(mp), C.byref(s)),
               "vg_euler_state_max_wave_speed")
         return s.value
+
+
+# ---- batched 1D solves ---------------------------------------------------------
+
+@dataclass
+class BatchResult:
+    cells: np.ndarray
+    steps: np.ndarray
+    t: np.ndarray
+    stats: _lib.RunStats
+    status: int
+
+
+def _batch_arrays(prim, offsets, dx, t_final, nf):
+    w = np.ascontiguousarray(prim, dtype=np.float64)
+    if w.ndim != 2 or w.shape[1] != nf:
+        raise ValueError(f"batched state must be (cells, {nf})")
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    n_inst = off.size - 1
+    dx = np.ascontiguousarray(np.broadcast_to(np.asarray(dx, dtype=np.float64), (n_inst,)))
+    tf = np.ascontiguousarray(np.broadcast_to(np.asarray(t_final, dtype=np.float64), (n_inst,)))
+    return w, off, n_inst, dx, tf
+
+
+def euler_solve_batch(prim, offsets, dx, params, t_final, cfl, *, max_steps=0, bias=0.0, limiter=0, order=2,
+                      mode=EXACT, ctx: Context | None = None, raise_on_error=True) -> BatchResult:
+    """n_inst independent 1D Euler solves in one persistent launch (vg_euler_solve_batch)."""
+    ctx = ctx or default_context()
+    w, off, n_inst, dx, tf = _batch_arrays(prim, offsets, dx, t_final, 6)
+    if isinstance(params, EulerMaterialParameters):
+        params = [params] * n_inst
+    mp = (_lib.EulerParams * n_inst)(*[p.c_struct() for p in params])
+    out = np.empty_like(w)
+    steps = np.empty(n_inst, dtype=np.int32)
+    tend = np.empty(n_inst)
+    st = _lib.RunStats()
+    rc = ctx.lib.vg_euler_solve_batch(ctx.handle, n_inst, off.ctypes.data_as(C.POINTER(C.c_int64)), _lib.as_dptr(w),
+                                      _lib.as_dptr(out), _lib.as_dptr(dx), mp, _lib.as_dptr(tf), cfl, max_steps, bias,
+                                      limiter, order, mode, steps.ctypes.data_as(C.POINTER(C.c_int32)),
+                                      _lib.as_dptr(tend), C.byref(st))
+    if raise_on_error and rc != _lib.VG_OK:
+        check(rc, "vg_euler_solve_batch")
+    return BatchResult(out, steps, tend, st, rc)
+
+
+# ---- ideal MHD ---------------------------------------------------------------------
+
+@dataclass
+class MHDMaterialParameters:
+    """MHDMaterialParameters (MHD/mhdmaterialparameters.h); the one-argument
+    constructor sets p_inf, c_h and c_p to 0 (mhdmaterialparameters.cpp:12-19)."""
+
+    gamma: float = 1.0
+    p_inf: float = 0.0
+    ch: float = 0.0
+    cp: float = 0.0
+
+    def c_struct(self) -> _lib.MHDParams:
+        return _lib.MHDParams(self.gamma, self.p_inf, self.ch, self.cp)
+
+
+def mhd_solve(prim: np.ndarray, dim: int, dx: float, cfl: float, t_final: float, *, order: int = 2,
+              bias: float = 0.0, limiter: int = 0, params: MHDMaterialParameters | None = None, max_steps: int = 0,
+              mode: int = EXACT, bc: int = TRANSMISSIVE, dt_cap: int = 200000, ctx: Context | None = None,
+              raise_on_error: bool = True) -> SolveResult:
+    """Whole MHD solve through vg_mhd_solve_1d / vg_mhd_solve_2d."""
+    ctx = ctx or default_context()
+    params = params or MHDMaterialParameters(2.0)
+    w = np.ascontiguousarray(prim, dtype=np.float64)
+    out = np.empty_like(w)
+    dts = np.empty(dt_cap)
+    st = _lib.RunStats()
+    mp = params.c_struct()
+    if dim == 1:
+        if w.ndim != 2 or w.shape[1] != 9:
+            raise ValueError("1D MHD state must be (cells, 9)")
+        rc = ctx.lib.vg_mhd_solve_1d(ctx.handle, _lib.as_dptr(w), _lib.as_dptr(out), w.shape[0], dx, cfl, t_final,
+                                     max_steps, bias, limiter, order, bc, C.byref(mp), mode, _lib.as_dptr(dts), dt_cap,
+                                     C.byref(st))
+    else:
+        if w.ndim != 3 or w.shape[2] != 9:
+            raise ValueError("2D MHD state must be (rows, columns, 9)")
+        rc = ctx.lib.vg_mhd_solve_2d(ctx.handle, _lib.as_dptr(w), _lib.as_dptr(out), w.shape[0], w.shape[1], dx, cfl,
+                                     t_final, max_steps, bias, limiter, order, bc, C.byref(mp), mode,
+                                     _lib.as_dptr(dts), dt_cap, C.byref(st))
+    if raise_on_error and rc != _lib.VG_OK:
+        check(rc, "vg_mhd_solve")
+    return SolveResult(out, dts[:min(st.steps, dt_cap)].copy(), st, rc)
+
+
+def mhd_solve_batch(prim, offsets, dx, gamma, t_final, cfl, *, max_steps=0, bias=0.0, limiter=1, order=2,
+                    mode=EXACT, p_inf=0.0, ctx: Context | None = None, raise_on_error=True) -> BatchResult:
+    """n_inst independent MHDSecondOrderSolver::solve calls in one persistent launch."""
+    ctx = ctx or default_context()
+    w, off, n_inst, dx, tf = _batch_arrays(prim, offsets, dx, t_final, 9)
+    g = np.broadcast_to(np.asarray(gamma, dtype=np.float64), (n_inst,))
+    mp = (_lib.MHDParams * n_inst)(*[_lib.MHDParams(float(x), p_inf, 0.0, 0.0) for x in g])
+    out = np.empty_like(w)
+    steps = np.empty(n_inst, dtype=np.int32)
+    tend = np.empty(n_inst)
+    st = _lib.RunStats()
+    rc = ctx.lib.vg_mhd_solve_batch(ctx.handle, n_inst, off.ctypes.data_as(C.POINTER(C.c_int64)), _lib.as_dptr(w),
+                                    _lib.as_dptr(out), _lib.as_dptr(dx), mp, _lib.as_dptr(tf), cfl, max_steps, bias,
+                                    limiter, order, mode, steps.ctypes.data_as(C.POINTER(C.c_int32)),
+                                    _lib.as_dptr(tend), C.byref(st))
+    if raise_on_error and rc != _lib.VG_OK:
+        check(rc, "vg_mhd_solve_batch")
+    return BatchResult(out, steps, tend, st, rc)
+
+
+class MHDSecondOrderSolver:
+    """MHDSecondOrderSolver (Solvers/mhdsecondordersolver.h); zero sub-cycles."""
+
+    @staticmethod
+    def solve(initial_cells, cell_spacing, cfl_coefficient, final_time, bias, slope_limiter, subcycling_iterations,
+              material_parameters, **kw):
+        if subcycling_iterations:
+            raise NotImplementedError("the GLM source sub-cycles are not part of this build")
+        return mhd_solve(initial_cells, 1, cell_spacing, cfl_coefficient, final_time, order=2, bias=bias,
+                         limiter=slope_limiter, params=material_parameters, **kw).cells
+
+    @staticmethod
+    def solve2D(initial_cells, cell_spacing, cfl_coefficient, final_time, bias, slope_limiter, subcycling_iterations,
+                material_parameters, **kw):
+        if subcycling_iterations:
+            raise NotImplementedError("the GLM source sub-cycles are not part of this build")
+        return mhd_solve(initial_cells, 2, cell_spacing, cfl_coefficient, final_time, order=2, bias=bias,
+                         limiter=slope_limiter, params=material_parameters, **kw).cells
