@@ -17,6 +17,8 @@
 #ifndef VG_ORACLE_H
 #define VG_ORACLE_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -62,6 +64,24 @@ int orc_mhd_solve(int order, int dim, int ny, int nx, double dx, double cfl, dou
                   double *prim_out, double *dt_log, int dt_cap);
 /* Per-function records of five input states (45 doubles) -> 85 doubles. */
 void orc_mhd_units(int n, double bias, int limiter, double dx, double ts, const orc_mhd_params *mp,
+                   const double *states, double *out);
+
+/* HPRMaterialParameters (HPR/hprmaterialparameters.h), stiffened-gas EOS: the
+ * fields the solver reads.  tau1/tau2 are the strain-dissipation and thermal
+ * relaxation times as configureStrainDissipationTime /
+ * configureThermalImpulseRelaxationTime set them (hprmaterialparameters.cpp:101-119). */
+typedef struct {
+    int32_t is_thermal, is_plastic;
+    double rho0, t_ref, t0, cv, gamma, p_inf, cs, tau1, tau2, alpha;
+} orc_hpr_params;
+
+/* HPRSecondOrderSolver::solve / solve2D with `subcycles` RK4 source sub-cycles on
+ * each side of the SLIC step.  Primitive order (rho, p, A row-major, u, v, w, J). */
+int orc_hpr_solve(int order, int dim, int ny, int nx, double dx, double cfl, double t_final, int max_steps,
+                  double bias, int limiter, int bc, int subcycles, const orc_hpr_params *mp, const double *prim_in,
+                  double *prim_out, double *dt_log, int dt_cap);
+/* Per-function records of five states (85 doubles) -> 156 doubles. */
+void orc_hpr_units(int n, double bias, int limiter, double dx, double ts, const orc_hpr_params *mp,
                    const double *states, double *out);
 
 #ifdef __cplusplus

@@ -20,6 +20,10 @@
 #include "Solvers/firstordersolver.h"
 #include "Solvers/mhdsecondordersolver.h"
 #include "Solvers/mhdfirstordersolver.h"
+#include "Solvers/hprsecondordersolver.h"
+#include "Solvers/hprfirstordersolver.h"
+#include "Solvers/hprforcingsolver.h"
+#include "HPR/hpracoustictensor.h"
 
 #include <cstdio>
 #include <cstdlib>
@@ -460,6 +464,204 @@ int mhdUnits(int argc, char **argv)
     return 0;
 }
 
+// HPR material parameters from argv: is_thermal rho0 p0 t_ref t0 cv gamma p_inf cs tau1 tau2 alpha
+HPRMaterialParameters hprParams(char **a)
+{
+    HPRMaterialParameters mp("StiffenedGas", std::atoi(a[0]) != 0, false, 8.31445985, std::atof(a[1]), std::atof(a[2]), std::atof(a[3]),
+                             std::atof(a[4]), std::atof(a[5]), std::atof(a[6]), std::atof(a[7]), 0.0, 0.0, 0.0, 0.0, std::atof(a[8]),
+                             std::atof(a[9]), std::atof(a[10]), 1e-2, 0.75, 0.0, 1.0, std::atof(a[11]), 0.0, 0.0, 0.0);
+    return mp;
+}
+
+HPRStateVector hprStateFrom(const double *p)
+{
+    std::vector<std::vector<double> > A(3, std::vector<double>(3));
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) A[i][j] = p[2 + 3 * i + j];
+    return HPRStateVector(p[0], p[1], A, p[11], p[12], p[13], p[14], p[15], p[16]);
+}
+
+void putHprState(HPRStateVector s, double *p)
+{
+    p[0] = s.getDensity();
+    p[1] = s.getPressure();
+    std::vector<std::vector<double> > A = s.getDistortionTensor();
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) p[2 + 3 * i + j] = A[i][j];
+    p[11] = s.getXVelocity();
+    p[12] = s.getYVelocity();
+    p[13] = s.getZVelocity();
+    p[14] = s.getXThermalImpulse();
+    p[15] = s.getYThermalImpulse();
+    p[16] = s.getZThermalImpulse();
+}
+
+// hpr_run order dim ny nx dx cfl tfinal max_steps bias limiter subcycles <12 params> in out_state out_dt check_api
+// Mirrors HPRSecondOrderSolver::solve / solve2D (hprsecondordersolver.cpp:314-352, 502-546).
+int hprRun(int argc, char **argv)
+{
+    if (argc < 29) { std::fprintf(stderr, "hpr_run: too few arguments\n"); return 2; }
+    int order = std::atoi(argv[2]);
+    int dim = std::atoi(argv[3]);
+    int ny = std::atoi(argv[4]);
+    int nx = std::atoi(argv[5]);
+    double dx = std::atof(argv[6]);
+    double cfl = std::atof(argv[7]);
+    double tfinal = std::strcmp(argv[8], "inf") == 0 ? std::numeric_limits<double>::infinity() : std::atof(argv[8]);
+    int maxSteps = std::atoi(argv[9]);
+    double bias = std::atof(argv[10]);
+    int limiter = std::atoi(argv[11]);
+    int sub = std::atoi(argv[12]);
+    HPRMaterialParameters mp = hprParams(argv + 13);
+    const char *inPath = argv[25];
+    const char *outState = argv[26];
+    const char *outDt = argv[27];
+    int checkApi = std::atoi(argv[28]);
+    const int nf = 17;
+    std::vector<double> in = readDoubles(inPath, (size_t)ny * nx * nf);
+    std::vector<double> dts;
+    std::vector<double> out((size_t)ny * nx * nf);
+    MuteCout mute;
+    auto t0 = std::chrono::steady_clock::now();
+    if (dim == 1)
+    {
+        std::vector<HPRStateVector> cells(nx);
+        for (int j = 0; j < nx; j++) cells[j] = hprStateFrom(&in[(size_t)j * nf]);
+        std::vector<HPRStateVector> initial = cells;
+        double t = 0.0;
+        int it = 0;
+        while (t < tfinal && (maxSteps == 0 || it < maxSteps))
+        {
+            std::vector<HPRStateVector> withBoundary = HPRSolvers::insertBoundaryCells(cells, 2);
+            double dt = HPRSolvers::computeStableTimeStep(withBoundary, dx, cfl, t, tfinal, it, mp);
+            for (int s = 0; s < sub; s++)
+            {
+                withBoundary = HPRSolvers::insertBoundaryCells(cells, 1);
+                HPRForcingSolver::computeRungeKuttaTimeStep(cells, withBoundary, dx, 0.5 * (dt / sub), bias, limiter, mp);
+            }
+            withBoundary = HPRSolvers::insertBoundaryCells(cells, order == 1 ? 1 : 2);
+            if (order == 1)
+                HPRFirstOrderSolver::computeFORCETimeStep(cells, withBoundary, dx, dt, mp);
+            else
+                HPRSecondOrderSolver::computeSLICTimeStep(cells, withBoundary, dx, dt, bias, limiter, mp);
+            for (int s = 0; s < sub; s++)
+            {
+                withBoundary = HPRSolvers::insertBoundaryCells(cells, 1);
+                HPRForcingSolver::computeRungeKuttaTimeStep(cells, withBoundary, dx, 0.5 * (dt / sub), bias, limiter, mp);
+            }
+            t += dt;
+            it += 1;
+            dts.push_back(dt);
+        }
+        for (int j = 0; j < nx; j++) putHprState(cells[j], &out[(size_t)j * nf]);
+        if (checkApi && maxSteps == 0 && order == 2)
+        {
+            std::vector<HPRStateVector> api = HPRSecondOrderSolver::solve(initial, dx, cfl, tfinal, bias, limiter, sub, mp);
+            bool match = true;
+            for (int j = 0; j < nx; j++)
+            {
+                double q[17];
+                putHprState(api[j], q);
+                for (int f = 0; f < nf; f++) match = match && sameBits(q[f], out[(size_t)j * nf + f]);
+            }
+            std::fprintf(stderr, "API_MATCH %d\n", match ? 1 : 0);
+        }
+    }
+    else
+    {
+        std::vector<std::vector<HPRStateVector> > cells(ny, std::vector<HPRStateVector>(nx));
+        for (int i = 0; i < ny; i++)
+            for (int j = 0; j < nx; j++) cells[i][j] = hprStateFrom(&in[((size_t)i * nx + j) * nf]);
+        double t = 0.0;
+        int it = 0;
+        while (t < tfinal && (maxSteps == 0 || it < maxSteps))
+        {
+            std::vector<std::vector<HPRStateVector> > withBoundary = HPRSolvers::insertBoundaryCells2D(cells, 2);
+            double dt = HPRSolvers::computeStableTimeStep2D(withBoundary, dx, cfl, t, tfinal, it, mp);
+            for (int s = 0; s < sub; s++)
+            {
+                withBoundary = HPRSolvers::insertBoundaryCells2D(cells, 1);
+                HPRForcingSolver::computeRungeKuttaTimeStep2D(cells, withBoundary, dx, 0.5 * (dt / sub), bias, limiter, mp);
+            }
+            withBoundary = HPRSolvers::insertBoundaryCells2D(cells, 2);
+            HPRSecondOrderSolver::computeXSLICTimeStep2D(cells, withBoundary, dx, 0.5 * dt, bias, limiter, mp);
+            withBoundary = HPRSolvers::insertBoundaryCells2D(cells, 2);
+            HPRSecondOrderSolver::computeYSLICTimeStep2D(cells, withBoundary, dx, dt, bias, limiter, mp);
+            withBoundary = HPRSolvers::insertBoundaryCells2D(cells, 2);
+            HPRSecondOrderSolver::computeXSLICTimeStep2D(cells, withBoundary, dx, 0.5 * dt, bias, limiter, mp);
+            for (int s = 0; s < sub; s++)
+            {
+                withBoundary = HPRSolvers::insertBoundaryCells2D(cells, 1);
+                HPRForcingSolver::computeRungeKuttaTimeStep2D(cells, withBoundary, dx, 0.5 * (dt / sub), bias, limiter, mp);
+            }
+            t += dt;
+            it += 1;
+            dts.push_back(dt);
+        }
+        for (int i = 0; i < ny; i++)
+            for (int j = 0; j < nx; j++) putHprState(cells[i][j], &out[((size_t)i * nx + j) * nf]);
+    }
+    double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::fprintf(stderr, "STEPS %zu SECONDS %.6f\n", dts.size(), secs);
+    writeDoubles(outState, out);
+    writeDoubles(outDt, dts);
+    return 0;
+}
+
+// hpr_units n bias limiter dx ts <12 params> in out: records of five states (85 doubles) -> 156 doubles,
+// the layout of orc_hpr_units.
+int hprUnits(int argc, char **argv)
+{
+    if (argc < 20) { std::fprintf(stderr, "hpr_units: too few arguments\n"); return 2; }
+    int n = std::atoi(argv[2]);
+    double bias = std::atof(argv[3]);
+    int limiter = std::atoi(argv[4]);
+    double dx = std::atof(argv[5]);
+    double ts = std::atof(argv[6]);
+    HPRMaterialParameters mp = hprParams(argv + 7);
+    std::vector<double> in = readDoubles(argv[19], (size_t)n * 85);
+    const int rec = 156, nf = 17;
+    std::vector<double> out((size_t)n * rec);
+    for (int r = 0; r < n; r++)
+    {
+        const double *s = &in[(size_t)r * 85];
+        HPRStateVector a = hprStateFrom(s), b = hprStateFrom(s + 17), c = hprStateFrom(s + 34), d = hprStateFrom(s + 51);
+        double *o = &out[(size_t)r * rec];
+        std::vector<double> cb = b.computeConservedVariableVector(mp);
+        putVec(cb, o); o += nf;
+        HPRStateVector back;
+        back.setConservedVariableVector(cb, mp);
+        putHprState(back, o);
+        if (!mp.getIsThermal()) { o[14] = b.getXThermalImpulse(); o[15] = b.getYThermalImpulse(); o[16] = b.getZThermalImpulse(); }
+        o += nf;
+        putVec(HPRStateVector::computeXFluxVector(cb, mp), o); o += nf;
+        putVec(HPRStateVector::computeYFluxVector(cb, mp), o); o += nf;
+        putVec(HPRStateVector::computeSourceTermVector(cb, mp), o); o += nf;
+        putHprState(HPRSolvers::evolveStateByHalfXTimeStep(a, b, c, dx, ts, bias, limiter, 0, mp), o);
+        if (!mp.getIsThermal()) { o[14] = o[15] = o[16] = 0.0; }
+        o += nf;
+        putVec(HPRFirstOrderSolver::computeXFORCEFlux(b, c, dx, ts, mp), o); o += nf;
+        putVec(HPRSecondOrderSolver::computeXSLICFlux(a, b, c, d, dx, ts, bias, limiter, mp), o); o += nf;
+        {
+            std::vector<double> nu = HPRForcingSolver::evolveConservedVariableVector(a.computeConservedVariableVector(mp), cb, c.computeConservedVariableVector(mp),
+                                                                                    dx, ts, bias, limiter, mp);
+            HPRStateVector ev;
+            ev.setConservedVariableVector(nu, mp);
+            putHprState(ev, o);
+            if (!mp.getIsThermal()) { o[14] = o[15] = o[16] = 0.0; }
+            o += nf;
+        }
+        *o++ = HPRAcousticTensor::computeMaximumWaveSpeed(b, mp, 0);
+        *o++ = HPRAcousticTensor::computeMaximumWaveSpeed(b, mp, 1);
+        {
+            std::vector<std::vector<HPRStateVector> > one2(1, std::vector<HPRStateVector>(1, b));
+            *o++ = HPRSolvers::computeMaximumWaveSpeed2D(one2, mp);
+        }
+    }
+    writeDoubles(argv[20], out);
+    return 0;
+}
+
 } // namespace
 
 int main(int argc, char **argv)
@@ -471,6 +673,8 @@ int main(int argc, char **argv)
     if (mode == "limiter_units") return limiterUnits(argc, argv);
     if (mode == "mhd_run") return mhdRun(argc, argv);
     if (mode == "mhd_units") return mhdUnits(argc, argv);
+    if (mode == "hpr_run") return hprRun(argc, argv);
+    if (mode == "hpr_units") return hprUnits(argc, argv);
     std::fprintf(stderr, "unknown mode %s\n", mode.c_str());
     return 2;
 }

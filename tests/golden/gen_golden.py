@@ -226,7 +226,134 @@ def mhd_2d_ic(n):
     return w
 
 
+def hpr_params(thermal, *, rho0=1.0, p0=1.0 / 1.4, t_ref=0.0, t0=1.0, cv=2.5, gamma=1.4, p_inf=0.0, cs=1.0, mu=1e-2,
+               alpha=2.0, kappa=1e-2):
+    """HPR parameters as the reference's configure* helpers set the relaxation
+    times (hprmaterialparameters.cpp:101-119)."""
+    tau1 = 6.0 * (mu / (p0 * (cs * cs)))
+    tau2 = kappa * (rho0 / (t0 * (alpha * alpha))) if thermal else 0.0
+    return dict(is_thermal=int(thermal), rho0=rho0, p0=p0, t_ref=t_ref, t0=t0, cv=cv, gamma=gamma, p_inf=p_inf, cs=cs,
+                tau1=tau1, tau2=tau2, alpha=alpha)
+
+
+HPR_ARG_ORDER = ("is_thermal", "rho0", "p0", "t_ref", "t0", "cv", "gamma", "p_inf", "cs", "tau1", "tau2", "alpha")
+
+
+def hpr_call(w, *, order, dim, dx, cfl, tfinal, max_steps, bias, limiter, subcycles, params, check_api):
+    with tempfile.TemporaryDirectory() as tmp:
+        ny, nx = (1, w.shape[0]) if dim == 1 else w.shape[:2]
+        np.ascontiguousarray(w, dtype=np.float64).tofile(os.path.join(tmp, "in.f64"))
+        t = "inf" if np.isinf(tfinal) else repr(float(tfinal))
+        log = run(["hpr_run", order, dim, ny, nx, repr(dx), repr(cfl), t, max_steps, repr(bias), limiter, subcycles,
+                   *[repr(params[k]) if k != "is_thermal" else params[k] for k in HPR_ARG_ORDER], "in.f64",
+                   "state.f64", "dt.f64", int(check_api)], tmp)
+        state = np.fromfile(os.path.join(tmp, "state.f64")).reshape(w.shape)
+        dt = np.fromfile(os.path.join(tmp, "dt.f64"))
+    if check_api and max_steps == 0 and order == 2 and dim == 1 and "API_MATCH 1" not in log:
+        raise RuntimeError("HPR mirror loop differs from the reference's own solve")
+    return state, dt
+
+
+def hpr_run(name, w, **kw):
+    state, dt = hpr_call(w, **kw)
+    meta = dict(kw)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), prim_in=w, prim_out=state, dt=dt,
+                        meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
+    print(f"{name}: {len(dt)} steps")
+
+
+def hpr_state(rho, p, u, v, w=0.0, A=None, J=(0.0, 0.0, 0.0)):
+    s = np.zeros(17)
+    s[0], s[1] = rho, p
+    s[2:11] = (np.eye(3) if A is None else A).ravel()
+    s[11:14] = (u, v, w)
+    s[14:17] = J
+    return s
+
+
+def hpr_stokes(n):
+    """HPRTests::solveStokesFirstProblem (hprtests.cpp:7-31): v = -0.1 / +0.1 shear layer."""
+    return np.stack([hpr_state(1.0, 1.0 / 1.4, 0.0, -0.1 if i <= 0.5 * n else 0.1) for i in range(n)])
+
+
+def hpr_heat(n):
+    """HPRTests::solveHeatConductionProblem (hprtests.cpp:33-57): density jump at equal pressure."""
+    rows = []
+    for i in range(n):
+        rho = 2.0 if i <= 0.5 * n else 0.5
+        rows.append(hpr_state(rho, 1.0, 0.0, 0.0, A=(rho ** (1.0 / 3.0)) * np.eye(3)))
+    return np.stack(rows)
+
+
+def hpr_2d(n, seed=0):
+    rng = np.random.default_rng(seed)
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x)
+    w = np.zeros((n, n, 17))
+    w[..., 0] = 1.0 + 0.1 * np.sin(2 * np.pi * X)
+    w[..., 1] = 1.0 / 1.4 + 0.05 * np.cos(2 * np.pi * Y)
+    A = np.eye(3)[None, None] + 0.02 * rng.standard_normal((n, n, 3, 3))
+    w[..., 2:11] = A.reshape(n, n, 9)
+    w[..., 11] = 0.1 * np.sin(2 * np.pi * Y)
+    w[..., 12] = -0.1 * np.sin(2 * np.pi * X)
+    w[..., 13] = 0.01
+    return w
+
+
+def hpr_units(name, n, seed, thermal):
+    rng = np.random.default_rng(seed)
+    params = hpr_params(thermal)
+    base = np.zeros((n, 17))
+    base[:, 0] = rng.uniform(0.8, 1.2, n)
+    base[:, 1] = rng.uniform(0.5, 1.0, n)
+    base[:, 2:11] = (np.eye(3).ravel()[None] + 0.05 * rng.standard_normal((n, 9)))
+    base[:, 11:14] = rng.uniform(-0.2, 0.2, (n, 3))
+    if thermal:
+        base[:, 14:17] = rng.uniform(-0.1, 0.1, (n, 3))
+    st = np.empty((n, 5, 17))
+    for k in range(5):
+        scale = rng.choice([1e-7, 1e-3, 0.02], size=(n, 1))
+        s = base + rng.standard_normal((n, 17)) * scale * np.abs(base).clip(0.05)
+        if not thermal:
+            s[:, 14:17] = 0.0
+        st[:, k] = s
+    states = st.reshape(n, 85)
+    dx, ts = 1e-2, 2e-3
+    with tempfile.TemporaryDirectory() as tmp:
+        states.tofile(os.path.join(tmp, "in.f64"))
+        run(["hpr_units", n, "0.0", 0, repr(dx), repr(ts),
+             *[repr(params[k]) if k != "is_thermal" else params[k] for k in HPR_ARG_ORDER], "in.f64", "out.f64"], tmp)
+        out = np.fromfile(os.path.join(tmp, "out.f64")).reshape(n, 156)
+    meta = dict(bias=0.0, limiter=0, dx=dx, ts=ts, params=params)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), states=states, out=out,
+                        meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
+    print(f"{name}: {n} records")
+
+
 JOBS = {
+    "hpr_stokes100": lambda: hpr_run("hpr_stokes100", hpr_stokes(100), order=2, dim=1, dx=0.01, cfl=0.8, tfinal=0.05,
+                                     max_steps=0, bias=0.0, limiter=0, subcycles=0, params=hpr_params(False),
+                                     check_api=True),
+    "hpr_stokes100_sub1": lambda: hpr_run("hpr_stokes100_sub1", hpr_stokes(100), order=2, dim=1, dx=0.01, cfl=0.8,
+                                          tfinal=0.05, max_steps=0, bias=0.0, limiter=0, subcycles=1,
+                                          params=hpr_params(False), check_api=True),
+    "hpr_heat100": lambda: hpr_run("hpr_heat100", hpr_heat(100), order=2, dim=1, dx=0.01, cfl=0.8, tfinal=0.02,
+                                   max_steps=0, bias=0.0, limiter=1, subcycles=0, params=hpr_params(True),
+                                   check_api=True),
+    "hpr_heat100_sub1": lambda: hpr_run("hpr_heat100_sub1", hpr_heat(100), order=2, dim=1, dx=0.01, cfl=0.8,
+                                        tfinal=0.02, max_steps=0, bias=0.0, limiter=1, subcycles=1,
+                                        params=hpr_params(True), check_api=True),
+    "hpr_force_stokes64": lambda: hpr_run("hpr_force_stokes64", hpr_stokes(64), order=1, dim=1, dx=1 / 64, cfl=0.8,
+                                          tfinal=0.05, max_steps=0, bias=0.0, limiter=0, subcycles=0,
+                                          params=hpr_params(False), check_api=False),
+    "hpr_2d16": lambda: hpr_run("hpr_2d16", hpr_2d(16), order=2, dim=2, dx=1 / 16, cfl=0.8, tfinal=np.inf,
+                                max_steps=8, bias=0.0, limiter=0, subcycles=0, params=hpr_params(False),
+                                check_api=False),
+    "hpr_2d16_thermal_sub1": lambda: hpr_run("hpr_2d16_thermal_sub1", hpr_2d(16, 1), order=2, dim=2, dx=1 / 16,
+                                             cfl=0.8, tfinal=np.inf, max_steps=5, bias=0.0, limiter=1, subcycles=1,
+                                             params=hpr_params(True), check_api=False),
+    "hpr_units": lambda: hpr_units("hpr_units", 300, 31, False),
+    "hpr_units_thermal": lambda: hpr_units("hpr_units_thermal", 300, 32, True),
     "mhd_briowu256": lambda: mhd_run("mhd_briowu256", problems.brio_wu(256), order=2, dim=1, dx=1.0 / 256, cfl=0.8,
                                      tfinal=0.1, max_steps=0, bias=0.0, limiter=1, gamma=2.0, p_inf=0.0,
                                      check_api=True),

@@ -52,8 +52,59 @@ def load():
     lib.orc_mhd_units.restype = None
     lib.orc_mhd_units.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.POINTER(OrcMHDParams),
                                   dptr, dptr]
+    lib.orc_hpr_solve.restype = C.c_int
+    lib.orc_hpr_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                  C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(OrcHPRParams), dptr, dptr, dptr,
+                                  C.c_int]
+    lib.orc_hpr_units.restype = None
+    lib.orc_hpr_units.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.POINTER(OrcHPRParams),
+                                  dptr, dptr]
     _lib = lib
     return lib
+
+
+class OrcHPRParams(C.Structure):
+    _fields_ = [("is_thermal", C.c_int32), ("is_plastic", C.c_int32)] + [
+        (n, C.c_double) for n in ("rho0", "t_ref", "t0", "cv", "gamma", "p_inf", "cs", "tau1", "tau2", "alpha")]
+
+
+def hpr_params(thermal, *, rho0=1.0, p0=1.0 / 1.4, t_ref=0.0, t0=1.0, cv=2.5, gamma=1.4, p_inf=0.0, cs=1.0, mu=1e-2,
+               alpha=2.0, kappa=1e-2):
+    """HPR parameters with the relaxation times the reference's
+    configureStrainDissipationTime / configureThermalImpulseRelaxationTime set
+    (hprmaterialparameters.cpp:101-119): tau1 = 6 mu / (p0 cs^2),
+    tau2 = kappa rho0 / (T0 alpha^2)."""
+    tau1 = 6.0 * (mu / (p0 * (cs * cs)))
+    tau2 = kappa * (rho0 / (t0 * (alpha * alpha))) if thermal else 0.0
+    return dict(is_thermal=int(thermal), rho0=rho0, p0=p0, t_ref=t_ref, t0=t0, cv=cv, gamma=gamma, p_inf=p_inf, cs=cs,
+                tau1=tau1, tau2=tau2, alpha=alpha)
+
+
+def _hpr_struct(p):
+    return OrcHPRParams(int(p["is_thermal"]), 0, p["rho0"], p["t_ref"], p["t0"], p["cv"], p["gamma"], p["p_inf"],
+                        p["cs"], p["tau1"], p["tau2"], p["alpha"])
+
+
+def hpr_solve(prim, *, order, dim, dx, cfl, tfinal, max_steps, bias, limiter, subcycles, params, bc=0,
+              dt_cap=100000):
+    lib = load()
+    w = np.ascontiguousarray(prim, dtype=np.float64)
+    out = np.empty_like(w)
+    dts = np.empty(dt_cap)
+    ny, nx = (1, w.shape[0]) if dim == 1 else w.shape[:2]
+    mp = _hpr_struct(params)
+    n = lib.orc_hpr_solve(order, dim, ny, nx, dx, cfl, tfinal, max_steps, bias, limiter, bc, subcycles, C.byref(mp),
+                          _p(w), _p(out), _p(dts), dt_cap)
+    return out, dts[:min(n, dt_cap)].copy()
+
+
+def hpr_units(states, *, bias, limiter, dx, ts, params):
+    lib = load()
+    s = np.ascontiguousarray(states, dtype=np.float64)
+    out = np.empty((s.shape[0], 156))
+    mp = _hpr_struct(params)
+    lib.orc_hpr_units(s.shape[0], bias, limiter, dx, ts, C.byref(mp), _p(s), _p(out))
+    return out
 
 
 class OrcMHDParams(C.Structure):

@@ -1,0 +1,146 @@
+"""Pins the HPR oracle (oracle/hpr_oracle.c).
+
+Bit for bit against golden vectors written by the compiled reference
+(HPRSecondOrderSolver / HPRFirstOrderSolver runs with and without RK4 source
+sub-cycles, 1D and 2D, with and without heat conduction, and per-function
+records), plus what the mathematics fixes: an undistorted cell (A = I) carries no
+distortion energy and no relaxation source, its largest acoustic-tensor speed is
+the longitudinal speed sqrt(gamma p / rho + 4/3 c_s^2), a uniform state is a fixed
+point of the whole solve, a periodic grid conserves mass and momentum, and the
+sub-cycled source step converges at fourth order to the relaxation ODE.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+RUNS = ["hpr_stokes100", "hpr_stokes100_sub1", "hpr_heat100", "hpr_heat100_sub1", "hpr_force_stokes64", "hpr_2d16",
+        "hpr_2d16_thermal_sub1"]
+
+
+def run_meta(g, **over):
+    m = dict(g["meta"])
+    m.update(over)
+    return dict(order=m["order"], dim=m["dim"], dx=m["dx"], cfl=m["cfl"], tfinal=m["tfinal"],
+                max_steps=m["max_steps"], bias=m["bias"], limiter=m["limiter"], subcycles=m["subcycles"],
+                params=m["params"])
+
+
+@pytest.mark.parametrize("name", RUNS)
+def test_oracle_matches_reference_hpr_run(name):
+    g = O.golden(name)
+    out, dts = O.hpr_solve(g["prim_in"], **run_meta(g))
+    O.assert_bitwise(dts, g["dt"], f"{name} dt")
+    O.assert_bitwise(out, g["prim_out"], f"{name} state")
+
+
+FIELDS = [("cons", 0, 17), ("cons_roundtrip", 17, 34), ("flux_x", 34, 51), ("flux_y", 51, 68), ("source", 68, 85),
+          ("evolve_x", 85, 102), ("force_x", 102, 119), ("slic_x", 119, 136), ("forcing_1d", 136, 153),
+          ("acoustic_x", 153, 154), ("acoustic_y", 154, 155), ("wave_2d", 155, 156)]
+
+
+@pytest.mark.parametrize("name", ["hpr_units", "hpr_units_thermal"])
+def test_oracle_matches_reference_hpr_functions(name):
+    g = O.golden(name)
+    m = g["meta"]
+    out = O.hpr_units(g["states"], bias=m["bias"], limiter=m["limiter"], dx=m["dx"], ts=m["ts"], params=m["params"])
+    for field, a, b in FIELDS:
+        O.assert_bitwise(out[:, a:b], g["out"][:, a:b], f"{name}:{field}")
+
+
+def state(rho, p, u=0.0, v=0.0, w=0.0, A=None, J=(0.0, 0.0, 0.0)):
+    s = np.zeros(17)
+    s[0], s[1] = rho, p
+    s[2:11] = (np.eye(3) if A is None else np.asarray(A)).ravel()
+    s[11:14] = (u, v, w)
+    s[14:17] = J
+    return s
+
+
+def units_of(states, params, dx=1e-2, ts=1e-3):
+    return O.hpr_units(np.asarray(states).reshape(-1, 85), bias=0.0, limiter=0, dx=dx, ts=ts, params=params)
+
+
+@pytest.mark.parametrize("thermal", [False, True])
+def test_undistorted_cell_closed_forms(thermal):
+    """A = I: rho E = p/(gamma-1) + rho |u|^2 / 2 (+ rho alpha^2 |J|^2 / 2), zero
+    source in A, momentum flux p + rho u^2, and the acoustic speed along x is
+    sqrt(gamma p / rho + 4/3 c_s^2) (the power iteration's ten steps leave a
+    small error, hence the tolerance)."""
+    params = O.hpr_params(thermal)
+    g = params["gamma"]
+    rows = []
+    for rho, p, u in [(1.0, 1.0 / 1.4, 0.0), (2.0, 3.0, 0.1), (0.5, 0.2, -0.3)]:
+        J = (0.05, -0.02, 0.01) if thermal else (0.0, 0.0, 0.0)
+        s = state(rho, p, u, 0.0, 0.0, J=J)
+        out = units_of(np.tile(s, 5), params)[0]
+        E = p / (g - 1.0) + 0.5 * rho * u * u
+        if thermal:
+            E += 0.5 * rho * params["alpha"] ** 2 * sum(j * j for j in J)
+        assert math.isclose(out[1], E, rel_tol=1e-14)
+        np.testing.assert_array_equal(out[68 + 2:68 + 11], 0.0)
+        assert math.isclose(out[34 + 11], p + rho * u * u, rel_tol=1e-14)
+        if not thermal:
+            cl = math.sqrt(g * p / rho + 4.0 / 3.0 * params["cs"] ** 2)
+            assert math.isclose(out[153], cl, rel_tol=2e-3)
+            rows.append(out[153] / cl)
+    if rows:
+        assert all(r <= 1.0 + 1e-12 for r in rows)   # power iteration approaches from below
+
+
+def test_uniform_state_is_fixed_point():
+    params = O.hpr_params(False)
+    s = state(1.0, 1.0 / 1.4, 0.0, 0.0, 0.0)
+    for dim, shape in ((1, (32,)), (2, (8, 12))):
+        w = np.broadcast_to(s, shape + (17,)).copy()
+        out, dts = O.hpr_solve(w, order=2, dim=dim, dx=0.05, cfl=0.8, tfinal=0.05, max_steps=0, bias=0.0, limiter=0,
+                               subcycles=2, params=params)
+        assert len(dts) > 3
+        np.testing.assert_allclose(out, w, rtol=0, atol=1e-14)
+
+
+def test_periodic_conservation():
+    """Periodic SLIC sweeps conserve total mass, momentum and energy exactly up
+    to rounding (flux differences telescope); the relaxation source only acts on
+    A (and J), so the sub-cycled run conserves them too."""
+    params = O.hpr_params(False)
+    n = 64
+    x = (np.arange(n) + 0.5) / n
+    w = np.stack([state(1.0 + 0.2 * math.sin(2 * math.pi * xi), 1.0 / 1.4, 0.1 * math.cos(2 * math.pi * xi),
+                        0.1 * math.sin(4 * math.pi * xi), 0.0) for xi in x])
+    out, dts = O.hpr_solve(w, order=2, dim=1, dx=1.0 / n, cfl=0.8, tfinal=0.1, max_steps=0, bias=0.0, limiter=1,
+                           subcycles=1, params=params, bc=1)
+    assert len(dts) > 5
+
+    def totals(a):
+        units = units_of(np.repeat(a, 5, axis=0).reshape(-1, 85), params)
+        U = units[:, 0:17]
+        return U[:, [0, 1, 11, 12, 13]].sum(axis=0)
+
+    np.testing.assert_allclose(totals(out), totals(w), rtol=0, atol=1e-12)
+
+
+def test_source_subcycle_converges_fourth_order():
+    """Uniform neighbours make every fractional MUSCL predictor trivial, so a
+    sub-cycle is classical RK4 on dA/dt = S(A).  Against a converged reference
+    (many small steps) the error falls by about 2^4 per halving of the step."""
+    params = O.hpr_params(False)
+    A0 = np.array([[1.05, 0.02, 0.0], [0.01, 0.97, 0.03], [0.0, -0.02, 1.0]])
+    s = state(1.0, 1.0 / 1.4, A=A0)
+
+    def advance(ts, nsteps):
+        cur = s.copy()
+        for _ in range(nsteps):
+            out = units_of(np.tile(cur, 5), params, ts=ts)[0]
+            cur = out[136:153].copy()
+        return cur[2:11]
+
+    T = 0.01
+    ref = advance(T / 512, 512)
+    errs = [np.abs(advance(T / k, k) - ref).max() for k in (2, 4, 8, 16)]
+    assert errs[0] > 1e-7
+    for e0, e1 in zip(errs, errs[1:]):
+        assert 13.0 < e0 / e1 < 21.0
