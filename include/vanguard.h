@@ -175,6 +175,64 @@ int vg_mhd_solve_batch(vg_ctx *ctx, int n_inst, const int64_t *offsets, const do
                        double bias, int limiter, int order, int mode, int32_t *steps, double *t_out,
                        vg_run_stats *stats);
 
+/* ---- Godunov-Peshkov-Romenski (HPR) continuum ---------------------------- */
+
+enum {
+    VG_HPR_EOS_STIFFENED_GAS = 0,       /* "StiffenedGas" */
+    VG_HPR_EOS_SHOCK_MIE_GRUNEISEN = 1, /* "ShockMieGruneisen" (not supported: VG_ERR_UNSUPPORTED) */
+    VG_HPR_EOS_GODUNOV_ROMENSKI = 2     /* "GodunovRomenski"   (not supported: VG_ERR_UNSUPPORTED) */
+};
+
+/* HPRMaterialParameters (HPR/hprmaterialparameters.h:10-15), the full
+ * constructor argument list in its order, the EOS name as an enum.  The
+ * stiffened-gas path reads eos, is_thermal, is_plastic (must be 0),
+ * reference_density, reference_temperature, initial_temperature,
+ * specific_heat_capacity, adiabatic_index, stiffening_parameter,
+ * transverse_wave_speed, strain_dissipation_time,
+ * thermal_impulse_relaxation_time and heat_wave_speed.  As in the reference,
+ * the two relaxation times are taken as given (the reference's
+ * configureStrainDissipationTime / configureThermalImpulseRelaxationTime,
+ * hprmaterialparameters.cpp:101-119, are helpers the caller applies). */
+typedef struct {
+    int32_t eos, is_thermal, is_plastic, reserved;
+    double ideal_gas_constant, reference_density, reference_pressure, reference_temperature, initial_temperature,
+        specific_heat_capacity, adiabatic_index, stiffening_parameter, reference_sound_speed,
+        reference_gruneisen_coefficient, hugoniot_slope_coefficient, reference_internal_energy, transverse_wave_speed,
+        strain_dissipation_time, thermal_impulse_relaxation_time, dynamic_viscosity_coefficient, prandtl_number,
+        elastic_plastic_transition_parameter, power_law_index, heat_wave_speed, alpha_parameter, beta_parameter,
+        gamma_parameter;
+} vg_hpr_params;
+
+/* HPRSecondOrderSolver::solve (hprsecondordersolver.h:56, hprsecondordersolver.cpp:314-352)
+ * when order == 2, HPRFirstOrderSolver::solve when order == 1.  Each step:
+ * `subcycles` RK4 relaxation sub-cycles over dt/2, the sweep, `subcycles` more
+ * (HPRForcingSolver::computeRungeKuttaTimeStep, hprforcingsolver.cpp:341-356).
+ * Primitive order (rho, p, A[3][3] row-major, u, v, w, J[3]), 17 doubles per
+ * cell; J is carried unchanged when is_thermal is 0, as in the reference. */
+int vg_hpr_solve_1d(vg_ctx *ctx, const double *prim_in, double *prim_out, int n, double dx, double cfl,
+                    double t_final, int max_steps, double bias, int limiter, int order, int bc, int subcycles,
+                    const vg_hpr_params *mp, int mode, double *dt_log, int dt_log_cap, vg_run_stats *stats);
+
+/* HPRSecondOrderSolver::solve2D (hprsecondordersolver.h:67, hprsecondordersolver.cpp:502-546):
+ * source(dt/2), X(dt/2) Y(dt) X(dt/2), source(dt/2). */
+int vg_hpr_solve_2d(vg_ctx *ctx, const double *prim_in, double *prim_out, int ny, int nx, double dx, double cfl,
+                    double t_final, int max_steps, double bias, int limiter, int order, int bc, int subcycles,
+                    const vg_hpr_params *mp, int mode, double *dt_log, int dt_log_cap, vg_run_stats *stats);
+
+/* Device-resident HPR grid (as vg_euler_state). */
+typedef struct vg_hpr_state vg_hpr_state;
+int vg_hpr_state_create(vg_ctx *ctx, int dim, int ny, int nx, vg_hpr_state **st);
+void vg_hpr_state_destroy(vg_hpr_state *st);
+int vg_hpr_state_load(vg_hpr_state *st, const double *prim, int on_device);
+int vg_hpr_state_store(vg_hpr_state *st, double *prim, int on_device);
+int vg_hpr_state_planes(vg_hpr_state *st, double **planes, int64_t *plane);
+int vg_hpr_state_run(vg_hpr_state *st, int order, double dx, double cfl, double t_final, int max_steps, double bias,
+                     int limiter, int bc, int subcycles, const vg_hpr_params *mp, int mode, double *dt_log,
+                     int dt_log_cap, vg_run_stats *stats);
+/* Maximum wave speed (HPRSolvers::computeMaximumWaveSpeed[2D], hprsolvers.cpp:286-303,
+ * 362-385) over the ghosted grid. */
+int vg_hpr_state_max_wave_speed(vg_hpr_state *st, int bc, const vg_hpr_params *mp, double *smax);
+
 #ifdef __cplusplus
 }
 #endif

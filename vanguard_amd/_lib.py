@@ -34,6 +34,24 @@ class MHDParams(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("gamma", "p_inf", "ch", "cp")]
 
 
+VG_HPR_EOS_STIFFENED_GAS = 0
+VG_HPR_EOS_SHOCK_MIE_GRUNEISEN = 1
+VG_HPR_EOS_GODUNOV_ROMENSKI = 2
+
+HPR_DOUBLE_FIELDS = (
+    "ideal_gas_constant", "reference_density", "reference_pressure", "reference_temperature", "initial_temperature",
+    "specific_heat_capacity", "adiabatic_index", "stiffening_parameter", "reference_sound_speed",
+    "reference_gruneisen_coefficient", "hugoniot_slope_coefficient", "reference_internal_energy",
+    "transverse_wave_speed", "strain_dissipation_time", "thermal_impulse_relaxation_time",
+    "dynamic_viscosity_coefficient", "prandtl_number", "elastic_plastic_transition_parameter", "power_law_index",
+    "heat_wave_speed", "alpha_parameter", "beta_parameter", "gamma_parameter")
+
+
+class HPRParams(C.Structure):
+    _fields_ = [("eos", C.c_int32), ("is_thermal", C.c_int32), ("is_plastic", C.c_int32), ("reserved", C.c_int32)] + [
+        (n, C.c_double) for n in HPR_DOUBLE_FIELDS]
+
+
 class RunStats(C.Structure):
     _fields_ = [("steps", C.c_int32), ("total_steps", C.c_int32), ("t", C.c_double), ("dt_min", C.c_double),
                 ("dt_max", C.c_double), ("wall_ms", C.c_double), ("device_ms", C.c_double),
@@ -76,6 +94,21 @@ SIGNATURES = {
     "vg_mhd_solve_batch": (C.c_int, [vptr, C.c_int, C.POINTER(C.c_int64), dptr, dptr, dptr, C.POINTER(MHDParams),
                                      dptr, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
                                      C.POINTER(C.c_int32), dptr, C.POINTER(RunStats)]),
+    "vg_hpr_solve_1d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                  C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(HPRParams), C.c_int, dptr,
+                                  C.c_int, C.POINTER(RunStats)]),
+    "vg_hpr_solve_2d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                  C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(HPRParams), C.c_int, dptr,
+                                  C.c_int, C.POINTER(RunStats)]),
+    "vg_hpr_state_create": (C.c_int, [vptr, C.c_int, C.c_int, C.c_int, C.POINTER(vptr)]),
+    "vg_hpr_state_destroy": (None, [vptr]),
+    "vg_hpr_state_load": (C.c_int, [vptr, vptr, C.c_int]),
+    "vg_hpr_state_store": (C.c_int, [vptr, vptr, C.c_int]),
+    "vg_hpr_state_planes": (C.c_int, [vptr, C.POINTER(vptr), C.POINTER(C.c_int64)]),
+    "vg_hpr_state_run": (C.c_int, [vptr, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int,
+                                   C.c_int, C.c_int, C.POINTER(HPRParams), C.c_int, dptr, C.c_int,
+                                   C.POINTER(RunStats)]),
+    "vg_hpr_state_max_wave_speed": (C.c_int, [vptr, C.c_int, C.POINTER(HPRParams), dptr]),
 }
 
 _lib = None

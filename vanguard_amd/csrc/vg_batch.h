@@ -1,13 +1,20 @@
-// Batched 1D solves in one persistent launch.
+// Batched 1D solves in one launch.
 //
 // Thousands of independent 1D problems of uneven size (each the reference's
 // whole solve(), e.g. MHDSecondOrderSolver::solve, mhdsecondordersolver.cpp:318-360)
-// run in one kernel: every workgroup takes the next instance from a work queue
-// (an atomic counter over instances sorted by decreasing estimated work), runs
+// run in one kernel: workgroup b takes instance order[b] (instances sorted by
+// decreasing estimated work, so the hardware dispatcher, which hands out
+// workgroups in index order as CUs free up, schedules longest-first) and runs
 // that instance's complete time loop -- CFL reduction, dt, SLIC/FORCE sweep in
-// tiles of BS cells -- and then takes the next one.  Per-instance time steps and
-// parameters (MHD: c_h) are kept in registers; the grid lives in HBM/L2 as
-// per-instance structure-of-arrays planes, ping-ponged between two buffers.
+// tiles of BS cells.  Per-instance time steps and parameters (MHD: c_h) are kept
+// in registers; the grid lives in HBM/L2 as per-instance structure-of-arrays
+// planes, ping-ponged between two buffers.
+//
+// There is deliberately no software work queue: a loop of the form
+// `for (;;) { if (tid == 0) s_q = atomicAdd(..); __syncthreads(); ... }` is
+// restructured by the compiler so that lane 0 leaves the loop to fetch while
+// the other lanes of its wave re-enter the barrier, and the block never
+// reconverges.  Every barrier below sits in wave-uniform control flow.
 #pragma once
 
 #include "vg_sweep.h"
@@ -29,7 +36,6 @@ struct BatchArgs {
     double cfl, bias;
     int limiter, max_steps, bc;
     double *buf0, *buf1;              // instance q: planes of n_q doubles at NF * offsets[q]
-    int *counter;                     // work-queue head
     int *steps_out;
     double *t_out;
     int *where;                       // 0: result in buf0, 1: in buf1
@@ -60,75 +66,68 @@ __global__ __launch_bounds__(BS) void batch_solve_kernel(BatchArgs<P> a)
     __shared__ double s_a[NF * BS];
     __shared__ double s_e[2 * NF * BS];
     __shared__ unsigned long long s_red[BS / kWave];
-    __shared__ int s_q;
     const int tid = threadIdx.x;
+    const int q = blockIdx.x;
+    if (q >= a.n_inst) return;
+    const int inst = a.order ? a.order[q] : q;
+    const long long off = a.offsets[inst];
+    const int n = (int)(a.offsets[inst + 1] - off);
+    typename P::Params m = a.params[inst];
+    const double dx = a.dx[inst];
+    const double T = a.t_final[inst];
+    double *A = a.buf0 + (long long)NF * off;
+    double *B = a.buf1 + (long long)NF * off;
 
+    unsigned long long key = 0;
+    for (int j = tid; j < n; j += BS) {
+        double w[NF];
+        load_prim<NF>(A, n, j, w);
+        const unsigned long long k2 = max_key(P::wave_speed(m, w, 1));
+        key = (k2 > key) ? k2 : key;
+    }
+    key = block_allreduce_max<BS>(key, s_red);
+
+    // The solve loop (secondordersolver.cpp:295-311): dt from the maximum
+    // wave speed, x0.2 for the first six steps, clipped to the final time.
+    double t = 0.0;
+    int it = 0, stalled = 0;
     for (;;) {
-        if (tid == 0) s_q = atomicAdd(a.counter, 1);
-        __syncthreads();
-        const int q = s_q;
-        __syncthreads();
-        if (q >= a.n_inst) break;
-        const int inst = a.order ? a.order[q] : q;
-        const long long off = a.offsets[inst];
-        const int n = (int)(a.offsets[inst + 1] - off);
-        typename P::Params m = a.params[inst];
-        const double dx = a.dx[inst];
-        const double T = a.t_final[inst];
-        double *A = a.buf0 + (long long)NF * off;
-        double *B = a.buf1 + (long long)NF * off;
-
-        unsigned long long key = 0;
-        for (int j = tid; j < n; j += BS) {
+        if (!(t < T) || (a.max_steps > 0 && it >= a.max_steps)) break;
+        const double S = __longlong_as_double((long long)key);
+        double dt = a.cfl * (dx / S);
+        if (it <= 5) dt *= 0.2;
+        if ((t + dt) > T) dt = T - t;
+        // A step that cannot advance t would make the reference loop forever;
+        // stop the instance and report it instead (also past kStepGuard steps).
+        if (!(dt > 0.0) || !(t + dt > t) || it >= kStepGuard) {
+            stalled = 1;
+            break;
+        }
+        if (a.dt_log && it < a.dt_cap && tid == 0) a.dt_log[it] = dt;
+        P::begin_step(m, S);
+        const StepScalars sc(dt, dx);
+        key = 0;
+        for (int j0 = 0; j0 < n; j0 += OUT) {
+            const int j = j0 - G + tid;
+            const int jj = (j >= -G && j < n + G) ? ghost_index(j, n, a.bc) : (j < 0 ? 0 : n - 1);
             double w[NF];
-            load_prim<NF>(A, n, j, w);
-            const unsigned long long k2 = max_key(P::wave_speed(m, w, 1));
+            load_prim<NF>(A, n, jj, w);
+            const unsigned long long k2 =
+                row_tile<P, ORDER, MODE, BS>(m, sc, a.bias, a.limiter, w, j < n, B, n, j, 1, true, s_a, s_e);
             key = (k2 > key) ? k2 : key;
         }
-        key = block_allreduce_max<BS>(key, s_red);
-
-        // The solve loop (secondordersolver.cpp:295-311): dt from the maximum
-        // wave speed, x0.2 for the first six steps, clipped to the final time.
-        double t = 0.0;
-        int it = 0, stalled = 0;
-        for (;;) {
-            if (!(t < T) || (a.max_steps > 0 && it >= a.max_steps)) break;
-            const double S = __longlong_as_double((long long)key);
-            double dt = a.cfl * (dx / S);
-            if (it <= 5) dt *= 0.2;
-            if ((t + dt) > T) dt = T - t;
-            // A step that cannot advance t would make the reference loop forever;
-            // stop the instance and report it instead (also past kStepGuard steps).
-            if (!(dt > 0.0) || !(t + dt > t) || it >= kStepGuard) {
-                stalled = 1;
-                break;
-            }
-            if (a.dt_log && it < a.dt_cap && tid == 0) a.dt_log[it] = dt;
-            P::begin_step(m, S);
-            const StepScalars sc(dt, dx);
-            key = 0;
-            for (int j0 = 0; j0 < n; j0 += OUT) {
-                const int j = j0 - G + tid;
-                const int jj = (j >= -G && j < n + G) ? ghost_index(j, n, a.bc) : (j < 0 ? 0 : n - 1);
-                double w[NF];
-                load_prim<NF>(A, n, jj, w);
-                const unsigned long long k2 =
-                    row_tile<P, ORDER, MODE, BS>(m, sc, a.bias, a.limiter, w, j < n, B, n, j, 1, true, s_a, s_e);
-                key = (k2 > key) ? k2 : key;
-            }
-            key = block_allreduce_max<BS>(key, s_red);   // its barriers also publish B for the next step
-            double *tmp = A;
-            A = B;
-            B = tmp;
-            t += dt;
-            it += 1;
-        }
-        if (tid == 0) {
-            a.steps_out[inst] = it;
-            a.t_out[inst] = t;
-            a.where[inst] = it & 1;
-            a.stalled[inst] = stalled;
-        }
+        key = block_allreduce_max<BS>(key, s_red);   // its barriers also publish B for the next step
+        double *tmp = A;
+        A = B;
+        B = tmp;
+        t += dt;
+        it += 1;
+    }
+    if (tid == 0) {
+        a.steps_out[inst] = it;
+        a.t_out[inst] = t;
+        a.where[inst] = it & 1;
+        a.stalled[inst] = stalled;
     }
 }
 

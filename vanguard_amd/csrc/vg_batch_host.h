@@ -1,7 +1,7 @@
-// Host side of the batched persistent solve: uploads the instances, orders the
-// work queue by decreasing estimated work (cells^2 x initial wave speed: the
-// step count of an instance grows like N * S / CFL), sizes the persistent grid
-// to the device's resident capacity, launches once and collects results.
+// Host side of the batched solve: uploads the instances, orders them by
+// decreasing estimated work (cells^2 x initial wave speed: the step count of an
+// instance grows like N * S / CFL), launches one workgroup per instance in that
+// order and collects results.
 #pragma once
 
 #include <algorithm>
@@ -21,11 +21,11 @@ struct BatchBuffers {
     double *dx = nullptr, *t_final = nullptr, *t_out = nullptr, *speed = nullptr, *dt_log = nullptr;
     typename P::Params *params = nullptr;
     double *buf0 = nullptr, *buf1 = nullptr, *staging = nullptr;
-    int *counter = nullptr, *steps = nullptr, *where = nullptr, *stalled = nullptr, *order = nullptr;
+    int *steps = nullptr, *where = nullptr, *stalled = nullptr, *order = nullptr;
     ~BatchBuffers()
     {
         hipFree(offsets); hipFree(dx); hipFree(t_final); hipFree(t_out); hipFree(speed); hipFree(dt_log);
-        hipFree(params); hipFree(buf0); hipFree(buf1); hipFree(staging); hipFree(counter); hipFree(steps);
+        hipFree(params); hipFree(buf0); hipFree(buf1); hipFree(staging); hipFree(steps);
         hipFree(where); hipFree(stalled); hipFree(order);
     }
 };
@@ -33,12 +33,8 @@ struct BatchBuffers {
 template <class P, int ORDER, int MODE, int BS>
 int launch_batch(vg_ctx *ctx, const BatchArgs<P> &args, int n_inst)
 {
-    int cus = 0, per_cu = 0;
-    VG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-    VG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_solve_kernel<P, ORDER, MODE, BS>, BS, 0));
-    const int blocks = std::max(1, std::min(n_inst, cus * std::max(per_cu, 1)));
     return ctx->timed("batch_solve", [&] {
-        batch_solve_kernel<P, ORDER, MODE, BS><<<blocks, BS, 0, ctx->stream>>>(args);
+        batch_solve_kernel<P, ORDER, MODE, BS><<<n_inst, BS, 0, ctx->stream>>>(args);
     });
 }
 
@@ -76,7 +72,6 @@ int run_batch(vg_ctx *ctx, int order, int mode, int n_inst, const int64_t *offse
     VG_HIP(hipMalloc(&b.buf0, sizeof(double) * NF * cells));
     VG_HIP(hipMalloc(&b.buf1, sizeof(double) * NF * cells));
     VG_HIP(hipMalloc(&b.staging, sizeof(double) * NF * cells));
-    VG_HIP(hipMalloc(&b.counter, sizeof(int)));
     VG_HIP(hipMalloc(&b.steps, sizeof(int) * n_inst));
     VG_HIP(hipMalloc(&b.where, sizeof(int) * n_inst));
     VG_HIP(hipMalloc(&b.stalled, sizeof(int) * n_inst));
@@ -94,7 +89,7 @@ int run_batch(vg_ctx *ctx, int order, int mode, int n_inst, const int64_t *offse
     int rc = launch_status("batch_aos_to_planes");
     if (rc) return rc;
 
-    // Work queue order: decreasing cells^2 * initial wave speed.
+    // Launch order: decreasing cells^2 * initial wave speed.
     std::vector<int> order_h(n_inst);
     std::iota(order_h.begin(), order_h.end(), 0);
     if (n_inst > 1) {
@@ -112,7 +107,6 @@ int run_batch(vg_ctx *ctx, int order, int mode, int n_inst, const int64_t *offse
         std::stable_sort(order_h.begin(), order_h.end(), [&](int x, int y) { return est[x] > est[y]; });
     }
     VG_HIP(hipMemcpyAsync(b.order, order_h.data(), sizeof(int) * n_inst, hipMemcpyHostToDevice, s));
-    VG_HIP(hipMemsetAsync(b.counter, 0, sizeof(int), s));
 
     BatchArgs<P> a;
     a.n_inst = n_inst;
@@ -128,7 +122,6 @@ int run_batch(vg_ctx *ctx, int order, int mode, int n_inst, const int64_t *offse
     a.bc = bc;
     a.buf0 = b.buf0;
     a.buf1 = b.buf1;
-    a.counter = b.counter;
     a.steps_out = b.steps;
     a.t_out = b.t_out;
     a.where = b.where;

@@ -61,6 +61,14 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
     return v;
 }
 
+// Tile shapes of the grid sweeps for a physics P: x-sweep block size and rows
+// per block, y-sweep block size and rows per column strip.  The x tile keeps
+// 3 * NF * bsx doubles in LDS, so wide states use narrower tiles.
+template <class P>
+struct Tiles {
+    static constexpr int bsx = 256, rx = 4, bsy = 256, hy = 64;
+};
+
 // Per-grid time-step clock kept in device memory so that a whole run can be
 // enqueued (or captured in a graph) without host round trips.  The step logic
 // is Solvers::computeStableTimeStep (Solvers/solvers.cpp:389-395, 413-419,

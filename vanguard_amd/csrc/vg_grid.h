@@ -19,14 +19,23 @@
 
 namespace vg {
 
-constexpr int kBSX = 256;
-constexpr int kBSY = 256;
-constexpr int kHY = 64;
-constexpr int kRX = 4;
+// Optional per-step source stage (the HPR relaxation sub-cycles): launched
+// `subcycles` times before and after the sweeps of every step, each launch
+// reading one buffer and writing the other.  The default has none.
+template <class P>
+struct SourceStage {
+    static constexpr bool kActive = false;
+    template <class GS>
+    static int launch(GS &, const double *, double *, int, int, double, double, int, const typename P::Params &)
+    {
+        return VG_OK;
+    }
+};
 
 template <class P>
 struct GridState {
     static constexpr int NF = P::NF;
+    static constexpr int kBSX = Tiles<P>::bsx, kRX = Tiles<P>::rx, kBSY = Tiles<P>::bsy, kHY = Tiles<P>::hy;
     vg_ctx *ctx = nullptr;
     int dim = 2, ny = 1, nx = 1;
     long long plane = 0, cells = 0;
@@ -40,6 +49,7 @@ struct GridState {
     unsigned long long *partials = nullptr;  // per-block wave-speed keys
     int n_partials = 0;                      // slots written by the last producer
     bool max_valid = false;
+    int subcycles = 0;             // source sub-cycles per half step (SourceStage)
     typename P::Params last_params{};
     int last_bc = -1;
 
@@ -56,7 +66,7 @@ struct GridState {
         VG_HIP(hipMalloc(&clk, sizeof(Clock)));
         VG_HIP(hipMalloc(&scan, 2 * sizeof(unsigned long long)));
         const long long gx = (nx + kBSX - 4 - 1) / (kBSX - 4) + 1;
-        const long long slots = std::max<long long>(gx * ny, 4096);
+        const long long slots = std::max<long long>(std::max<long long>(gx * ny, cells / 64 + 1), 4096);
         VG_HIP(hipMalloc(&partials, sizeof(unsigned long long) * slots));
         Clock h{};
         VG_HIP(hipMemcpy(clk, &h, sizeof(Clock), hipMemcpyHostToDevice));
@@ -185,15 +195,31 @@ struct GridState {
         int rc = launch_status("clock_begin");
         if (rc) return rc;
         double *a = buf[cur], *b = buf[1 - cur];
+        // Source half step before the sweeps (hprsecondordersolver.cpp:326-330, 514-520).
+        const int nsub = SourceStage<P>::kActive ? subcycles : 0;
+        for (int s = 0; s < nsub; s++) {
+            rc = SourceStage<P>::launch(*this, a, b, 0, bc, dx, bias, limiter, m);
+            if (rc) return rc;
+            std::swap(a, b);
+        }
+        const int fuse = (nsub == 0) ? 1 : 0;
         if (dim == 1) {
-            rc = launch_sweep<ORDER, MODE>(0, a, b, 1.0, 1, bc, dx, bias, limiter, m);
+            rc = launch_sweep<ORDER, MODE>(0, a, b, 1.0, fuse, bc, dx, bias, limiter, m);
         } else {
             rc = launch_sweep<ORDER, MODE>(0, a, b, 0.5, 0, bc, dx, bias, limiter, m);
             if (!rc) rc = launch_sweep<ORDER, MODE>(1, b, a, 1.0, 0, bc, dx, bias, limiter, m);
-            if (!rc) rc = launch_sweep<ORDER, MODE>(0, a, b, 0.5, 1, bc, dx, bias, limiter, m);
+            if (!rc) rc = launch_sweep<ORDER, MODE>(0, a, b, 0.5, fuse, bc, dx, bias, limiter, m);
         }
         if (rc) return launch_status("sweep");
-        cur = 1 - cur;
+        std::swap(a, b);
+        // ... and after them (:337-341, 531-537); the last launch folds in the CFL maximum.
+        for (int s = 0; s < nsub; s++) {
+            rc = SourceStage<P>::launch(*this, a, b, s == nsub - 1, bc, dx, bias, limiter, m);
+            if (rc) return rc;
+            std::swap(a, b);
+        }
+        // Every step moves the grid an odd number of times (1 + 2 nsub).
+        cur = (a == buf[0]) ? 0 : 1;
         return VG_OK;
     }
 

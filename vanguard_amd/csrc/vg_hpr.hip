@@ -1,0 +1,268 @@
+// HPR C ABI: 1D/2D whole-solve drop-ins and the device-resident state, with the
+// RK4 relaxation sub-cycles run as a source stage around the sweeps.
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "vg_grid.h"
+#include "vg_hpr_forcing.h"
+#include "vg_phys_hpr.h"
+
+namespace vg {
+
+// The relaxation sub-cycle launch for GridState<HPRPhys>::enqueue_step.
+template <>
+struct SourceStage<HPRPhys> {
+    static constexpr bool kActive = true;
+    template <class GS>
+    static int launch(GS &g, const double *in, double *out, int fuse, int bc, double dx, double bias, int limiter,
+                      const HPRPhys::Params &m)
+    {
+        ForcingArgs a;
+        a.in = in;
+        a.out = out;
+        a.plane = g.plane;
+        a.ny = g.ny;
+        a.nx = g.nx;
+        a.dim = g.dim;
+        a.bc = bc;
+        a.limiter = limiter;
+        a.subcycles = g.subcycles;
+        a.dx = dx;
+        a.bias = bias;
+        a.fuse_max = fuse;
+        a.m = m;
+        a.clk = g.clk;
+        a.partials = g.partials;
+        const long long blocks = (g.cells + kForceBS - 1) / kForceBS;
+        if (fuse) g.n_partials = (int)blocks;
+        return g.ctx->timed("hpr_source_rk4", [&] {
+            if (g.dim == 1)
+                hpr_forcing_kernel<1><<<(unsigned)blocks, kForceBS, 0, g.ctx->stream>>>(a);
+            else
+                hpr_forcing_kernel<2><<<(unsigned)blocks, kForceBS, 0, g.ctx->stream>>>(a);
+        });
+    }
+};
+
+}  // namespace vg
+
+using HPRGrid = vg::GridState<vg::HPRPhys>;
+
+struct vg_hpr_state {
+    HPRGrid g;
+};
+
+namespace {
+
+using vg::HPRPhys;
+
+HPRPhys::Params to_params(const vg_hpr_params *p)
+{
+    HPRPhys::Params m;
+    m.is_thermal = p->is_thermal ? 1 : 0;
+    m.is_plastic = p->is_plastic ? 1 : 0;
+    m.rho0 = p->reference_density;
+    m.t_ref = p->reference_temperature;
+    m.t0 = p->initial_temperature;
+    m.cv = p->specific_heat_capacity;
+    m.gamma = p->adiabatic_index;
+    m.p_inf = p->stiffening_parameter;
+    m.cs = p->transverse_wave_speed;
+    m.tau1 = p->strain_dissipation_time;
+    m.tau2 = p->thermal_impulse_relaxation_time;
+    m.alpha = p->heat_wave_speed;
+    // The same IEEE values the reference recomputes per call.
+    m.G = m.gamma - 1.0;
+    m.cs2 = m.cs * m.cs;
+    m.a2 = m.alpha * m.alpha;
+    return m;
+}
+
+int check_params(const vg_hpr_params *mp, int subcycles)
+{
+    VG_CHECK(mp != nullptr, "null material parameters");
+    if (mp->eos != VG_HPR_EOS_STIFFENED_GAS) {
+        vg::set_error("HPR: only the StiffenedGas equation of state is implemented");
+        return VG_ERR_UNSUPPORTED;
+    }
+    if (mp->is_plastic) {
+        vg::set_error("HPR: the plastic (power-law) strain relaxation is not implemented");
+        return VG_ERR_UNSUPPORTED;
+    }
+    VG_CHECK(subcycles >= 0, "subcycles must be >= 0");
+    VG_CHECK(subcycles == 0 || (mp->strain_dissipation_time > 0.0 &&
+                                (!mp->is_thermal || mp->thermal_impulse_relaxation_time > 0.0)),
+             "relaxation times must be positive when sub-cycling the source");
+    return VG_OK;
+}
+
+int check_run_args(int dim, int ny, int nx, double dx, double cfl, double t_final, int max_steps, int order, int bc,
+                   int subcycles, const vg_hpr_params *mp, int mode)
+{
+    int rc = check_params(mp, subcycles);
+    if (rc) return rc;
+    VG_CHECK(order == 1 || order == 2, "order must be 1 (FORCE) or 2 (SLIC)");
+    VG_CHECK(mode == VG_MODE_EXACT || mode == VG_MODE_FAST, "mode must be VG_MODE_EXACT or VG_MODE_FAST");
+    VG_CHECK(bc == VG_BC_TRANSMISSIVE || bc == VG_BC_PERIODIC, "unknown boundary condition");
+    VG_CHECK(dim == 1 || dim == 2, "dim must be 1 or 2");
+    VG_CHECK(nx >= 2 && ny >= 1 && (dim == 1 ? ny == 1 : ny >= 2), "grid needs at least 2 cells per direction");
+    VG_CHECK((long long)ny * nx <= (1ll << 30), "grid too large for one state");
+    VG_CHECK(dx > 0.0 && std::isfinite(dx), "cell spacing must be positive");
+    VG_CHECK(cfl > 0.0 && std::isfinite(cfl), "CFL coefficient must be positive");
+    VG_CHECK(max_steps >= 0, "max_steps must be >= 0");
+    VG_CHECK(max_steps > 0 || std::isfinite(t_final), "an infinite final time needs a step cap");
+    return VG_OK;
+}
+
+int run_grid(HPRGrid &g, int order, double dx, double cfl, double t_final, int max_steps, double bias, int limiter,
+             int bc, int subcycles, const vg_hpr_params *mp, int mode, double *dt_log, int dt_log_cap,
+             vg_run_stats *stats)
+{
+    g.subcycles = subcycles;
+    // HPR has no separate FAST arithmetic: both modes run the exact kernels.
+    (void)mode;
+    return g.run(order, VG_MODE_EXACT, dx, cfl, t_final, max_steps, bias, limiter, bc, to_params(mp), dt_log,
+                 dt_log_cap, stats);
+}
+
+int solve(vg_ctx *ctx, int dim, const double *prim_in, double *prim_out, int ny, int nx, double dx, double cfl,
+          double t_final, int max_steps, double bias, int limiter, int order, int bc, int subcycles,
+          const vg_hpr_params *mp, int mode, double *dt_log, int dt_log_cap, vg_run_stats *stats)
+{
+    vg::clear_error();
+    VG_CHECK(ctx != nullptr, "null context");
+    VG_CHECK(prim_in != nullptr && prim_out != nullptr, "null state pointer");
+    int rc = check_run_args(dim, ny, nx, dx, cfl, t_final, max_steps, order, bc, subcycles, mp, mode);
+    if (rc) return rc;
+    rc = vg::bind(ctx);
+    if (rc) return rc;
+    HPRGrid g;
+    rc = g.create(ctx, dim, ny, nx);
+    if (!rc) rc = g.load(prim_in, 0);
+    vg_run_stats local{};
+    if (!rc) rc = run_grid(g, order, dx, cfl, t_final, max_steps, bias, limiter, bc, subcycles, mp, mode, dt_log,
+                           dt_log_cap, &local);
+    if (!rc) rc = g.store(prim_out, 0);
+    if (!rc) rc = g.scan_nonfinite(to_params(mp), &local.nonfinite_cells, &local.first_nonfinite);
+    g.destroy();
+    if (stats) *stats = local;
+    if (!rc && local.nonfinite_cells > 0) {
+        vg::set_error("final state holds " + std::to_string(local.nonfinite_cells) +
+                      " non-finite or inadmissible cells (rho <= 0 or p + p_inf <= 0), first at flat index " +
+                      std::to_string(local.first_nonfinite));
+        return VG_ERR_NONFINITE;
+    }
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vg_hpr_solve_1d(vg_ctx *ctx, const double *prim_in, double *prim_out, int n, double dx, double cfl,
+                    double t_final, int max_steps, double bias, int limiter, int order, int bc, int subcycles,
+                    const vg_hpr_params *mp, int mode, double *dt_log, int dt_log_cap, vg_run_stats *stats)
+{
+    return solve(ctx, 1, prim_in, prim_out, 1, n, dx, cfl, t_final, max_steps, bias, limiter, order, bc, subcycles,
+                 mp, mode, dt_log, dt_log_cap, stats);
+}
+
+int vg_hpr_solve_2d(vg_ctx *ctx, const double *prim_in, double *prim_out, int ny, int nx, double dx, double cfl,
+                    double t_final, int max_steps, double bias, int limiter, int order, int bc, int subcycles,
+                    const vg_hpr_params *mp, int mode, double *dt_log, int dt_log_cap, vg_run_stats *stats)
+{
+    return solve(ctx, 2, prim_in, prim_out, ny, nx, dx, cfl, t_final, max_steps, bias, limiter, order, bc, subcycles,
+                 mp, mode, dt_log, dt_log_cap, stats);
+}
+
+int vg_hpr_state_create(vg_ctx *ctx, int dim, int ny, int nx, vg_hpr_state **st)
+{
+    vg::clear_error();
+    VG_CHECK(ctx != nullptr && st != nullptr, "null argument");
+    VG_CHECK(dim == 1 || dim == 2, "dim must be 1 or 2");
+    VG_CHECK(nx >= 2 && (dim == 1 ? ny == 1 : ny >= 2), "grid needs at least 2 cells per direction");
+    VG_CHECK((long long)ny * nx <= (1ll << 30), "grid too large for one state");
+    int rc = vg::bind(ctx);
+    if (rc) return rc;
+    vg_hpr_state *s = new vg_hpr_state();
+    rc = s->g.create(ctx, dim, ny, nx);
+    if (rc) {
+        s->g.destroy();
+        delete s;
+        return rc;
+    }
+    *st = s;
+    return VG_OK;
+}
+
+void vg_hpr_state_destroy(vg_hpr_state *st)
+{
+    if (!st) return;
+    vg::bind(st->g.ctx);
+    hipStreamSynchronize(st->g.ctx->stream);
+    st->g.destroy();
+    delete st;
+}
+
+int vg_hpr_state_load(vg_hpr_state *st, const double *prim, int on_device)
+{
+    vg::clear_error();
+    VG_CHECK(st != nullptr && prim != nullptr, "null argument");
+    int rc = vg::bind(st->g.ctx);
+    return rc ? rc : st->g.load(prim, on_device);
+}
+
+int vg_hpr_state_store(vg_hpr_state *st, double *prim, int on_device)
+{
+    vg::clear_error();
+    VG_CHECK(st != nullptr && prim != nullptr, "null argument");
+    int rc = vg::bind(st->g.ctx);
+    return rc ? rc : st->g.store(prim, on_device);
+}
+
+int vg_hpr_state_planes(vg_hpr_state *st, double **planes, int64_t *plane)
+{
+    vg::clear_error();
+    VG_CHECK(st != nullptr && planes != nullptr && plane != nullptr, "null argument");
+    *planes = st->g.buf[st->g.cur];
+    *plane = st->g.plane;
+    return VG_OK;
+}
+
+int vg_hpr_state_run(vg_hpr_state *st, int order, double dx, double cfl, double t_final, int max_steps, double bias,
+                     int limiter, int bc, int subcycles, const vg_hpr_params *mp, int mode, double *dt_log,
+                     int dt_log_cap, vg_run_stats *stats)
+{
+    vg::clear_error();
+    VG_CHECK(st != nullptr, "null state");
+    int rc = check_run_args(st->g.dim, st->g.ny, st->g.nx, dx, cfl, t_final, max_steps, order, bc, subcycles, mp,
+                            mode);
+    if (rc) return rc;
+    rc = vg::bind(st->g.ctx);
+    if (rc) return rc;
+    return run_grid(st->g, order, dx, cfl, t_final, max_steps, bias, limiter, bc, subcycles, mp, mode, dt_log,
+                    dt_log_cap, stats);
+}
+
+int vg_hpr_state_max_wave_speed(vg_hpr_state *st, int bc, const vg_hpr_params *mp, double *smax)
+{
+    vg::clear_error();
+    VG_CHECK(st != nullptr && smax != nullptr, "null argument");
+    int rc = check_params(mp, 0);
+    if (rc) return rc;
+    rc = vg::bind(st->g.ctx);
+    if (rc) return rc;
+    rc = st->g.refresh_max(bc, to_params(mp));
+    if (rc) return rc;
+    vg::Clock h;
+    VG_HIP(hipMemcpyAsync(&h, st->g.clk, sizeof(h), hipMemcpyDeviceToHost, st->g.ctx->stream));
+    VG_HIP(hipStreamSynchronize(st->g.ctx->stream));
+    double s;
+    std::memcpy(&s, &h.smax, sizeof(s));
+    if (h.corner > s) s = h.corner;
+    *smax = s;
+    return VG_OK;
+}
+
+}  // extern "C"

@@ -138,3 +138,52 @@ def mhd_batch(n_inst: int, seed: int = 3, n_min: int = 256, n_max: int = 16384):
         blk[left] = side[0]
         blk[~left] = side[1]
     return prim, offsets, gamma, 1.0 / sizes.astype(np.float64)
+
+
+def hpr_state(rho: float, p: float, u: float = 0.0, v: float = 0.0, w: float = 0.0, A=None,
+              J=(0.0, 0.0, 0.0)) -> np.ndarray:
+    """One HPR primitive state: rho, p, A (3x3 row-major), u, v, w, J
+    (HPRStateVector field order, HPR/hprstatevector.h)."""
+    s = np.zeros(17)
+    s[0], s[1] = rho, p
+    s[2:11] = (np.eye(3) if A is None else np.asarray(A, dtype=np.float64)).ravel()
+    s[11:14] = (u, v, w)
+    s[14:17] = J
+    return s
+
+
+def hpr_stokes_1d(n: int, v0: float = 0.1) -> np.ndarray:
+    """HPRTests::solveStokesFirstProblem (HPR/hprtests.cpp:7-31): rho = 1,
+    p = 1/1.4, A = I, v = -v0 for i <= n/2 and +v0 beyond."""
+    return np.stack([hpr_state(1.0, 1.0 / 1.4, 0.0, -v0 if i <= 0.5 * n else v0) for i in range(n)])
+
+
+def hpr_fourier_2d(n: int, seed: int = 4, modes: int = 16, amplitude: float = 0.1, gamma: float = 1.4,
+                   ny: int | None = None) -> np.ndarray:
+    """Viscous-flow IC on [0,1]^2, periodic: rho = 1, p = 1/gamma, A = I, J = 0,
+    and each velocity component a sum of ``modes`` Fourier modes
+    amplitude * a_m * sin(2 pi (k_m . x) + phi_m) with k_m in {1..4}^2 and
+    a_m in [-1, 1), phi_m in [0, 2 pi) from splitmix64(seed) (u and v draw
+    their own modes: 4 draws per mode (k_x, k_y, a, phi), u first)."""
+    ny = n if ny is None else ny
+    r = uniform01(seed, 2 * modes * 4).reshape(2, modes, 4)
+    x = (np.arange(n) + 0.5) / n
+    y = (np.arange(ny) + 0.5) / ny
+    X, Y = np.meshgrid(x, y)
+    vel = []
+    for c in range(2):
+        f = np.zeros((ny, n))
+        for m in range(modes):
+            kx = 1 + int(r[c, m, 0] * 4)
+            ky = 1 + int(r[c, m, 1] * 4)
+            a = 2.0 * r[c, m, 2] - 1.0
+            phi = 2.0 * np.pi * r[c, m, 3]
+            f += amplitude * a * np.sin(2.0 * np.pi * (kx * X + ky * Y) + phi)
+        vel.append(f)
+    w = np.zeros((ny, n, 17))
+    w[..., 0] = 1.0
+    w[..., 1] = 1.0 / gamma
+    w[..., 2] = w[..., 6] = w[..., 10] = 1.0
+    w[..., 11] = vel[0]
+    w[..., 12] = vel[1]
+    return w

@@ -333,3 +333,179 @@ class MHDSecondOrderSolver:
             raise NotImplementedError("the GLM source sub-cycles are not part of this build")
         return mhd_solve(initial_cells, 2, cell_spacing, cfl_coefficient, final_time, order=2, bias=bias,
                          limiter=slope_limiter, params=material_parameters, **kw).cells
+
+
+# ---- Godunov-Peshkov-Romenski (HPR) ----------------------------------------------
+
+@dataclass
+class HPRMaterialParameters:
+    """HPRMaterialParameters (HPR/hprmaterialparameters.h:10-15): the
+    constructor's arguments in order, the EOS name as a string."""
+
+    equation_of_state: str = "StiffenedGas"
+    is_thermal: bool = False
+    is_plastic: bool = False
+    ideal_gas_constant: float = 8.31445985
+    reference_density: float = 1.0
+    reference_pressure: float = 1.0 / 1.4
+    reference_temperature: float = 0.0
+    initial_temperature: float = 1.0
+    specific_heat_capacity: float = 2.5
+    adiabatic_index: float = 1.4
+    stiffening_parameter: float = 0.0
+    reference_sound_speed: float = 0.0
+    reference_gruneisen_coefficient: float = 0.0
+    hugoniot_slope_coefficient: float = 0.0
+    reference_internal_energy: float = 0.0
+    transverse_wave_speed: float = 1.0
+    strain_dissipation_time: float = 0.0
+    thermal_impulse_relaxation_time: float = 0.0
+    dynamic_viscosity_coefficient: float = 1e-2
+    prandtl_number: float = 0.75
+    elastic_plastic_transition_parameter: float = 0.0
+    power_law_index: float = 1.0
+    heat_wave_speed: float = 2.0
+    alpha_parameter: float = 0.0
+    beta_parameter: float = 0.0
+    gamma_parameter: float = 0.0
+
+    _EOS = {"StiffenedGas": _lib.VG_HPR_EOS_STIFFENED_GAS, "ShockMieGruneisen": _lib.VG_HPR_EOS_SHOCK_MIE_GRUNEISEN,
+            "GodunovRomenski": _lib.VG_HPR_EOS_GODUNOV_ROMENSKI}
+
+    def configure_strain_dissipation_time(self):
+        """tau1 = 6 mu / (p_ref c_s^2) (hprmaterialparameters.cpp:101-104)."""
+        self.strain_dissipation_time = 6.0 * (self.dynamic_viscosity_coefficient /
+                                              (self.reference_pressure * (self.transverse_wave_speed ** 2)))
+        return self
+
+    def configure_thermal_impulse_relaxation_time(self, heat_conduction_coefficient: float):
+        """tau2 = kappa rho0 / (T0 alpha^2) (hprmaterialparameters.cpp:112-119)."""
+        self.thermal_impulse_relaxation_time = heat_conduction_coefficient * (
+            self.reference_density / (self.initial_temperature * (self.heat_wave_speed ** 2)))
+        return self
+
+    def c_struct(self) -> _lib.HPRParams:
+        if self.equation_of_state not in self._EOS:
+            raise ValueError(f"unknown equation of state {self.equation_of_state!r}")
+        return _lib.HPRParams(self._EOS[self.equation_of_state], int(self.is_thermal), int(self.is_plastic), 0,
+                              *[float(getattr(self, n)) for n in _lib.HPR_DOUBLE_FIELDS])
+
+
+def hpr_solve(prim: np.ndarray, dim: int, dx: float, cfl: float, t_final: float, *, order: int = 2,
+              bias: float = 0.0, limiter: int = 0, subcycles: int = 0, params: HPRMaterialParameters | None = None,
+              max_steps: int = 0, mode: int = EXACT, bc: int = TRANSMISSIVE, dt_cap: int = 100000,
+              ctx: Context | None = None, raise_on_error: bool = True) -> SolveResult:
+    """Whole HPR solve through vg_hpr_solve_1d / vg_hpr_solve_2d."""
+    ctx = ctx or default_context()
+    params = params or HPRMaterialParameters()
+    w = np.ascontiguousarray(prim, dtype=np.float64)
+    out = np.empty_like(w)
+    dts = np.empty(dt_cap)
+    st = _lib.RunStats()
+    mp = params.c_struct()
+    if dim == 1:
+        if w.ndim != 2 or w.shape[1] != 17:
+            raise ValueError("1D HPR state must be (cells, 17)")
+        rc = ctx.lib.vg_hpr_solve_1d(ctx.handle, _lib.as_dptr(w), _lib.as_dptr(out), w.shape[0], dx, cfl, t_final,
+                                     max_steps, bias, limiter, order, bc, subcycles, C.byref(mp), mode,
+                                     _lib.as_dptr(dts), dt_cap, C.byref(st))
+    else:
+        if w.ndim != 3 or w.shape[2] != 17:
+            raise ValueError("2D HPR state must be (rows, columns, 17)")
+        rc = ctx.lib.vg_hpr_solve_2d(ctx.handle, _lib.as_dptr(w), _lib.as_dptr(out), w.shape[0], w.shape[1], dx, cfl,
+                                     t_final, max_steps, bias, limiter, order, bc, subcycles, C.byref(mp), mode,
+                                     _lib.as_dptr(dts), dt_cap, C.byref(st))
+    if raise_on_error and rc != _lib.VG_OK:
+        check(rc, "vg_hpr_solve")
+    return SolveResult(out, dts[:min(st.steps, dt_cap)].copy(), st, rc)
+
+
+class HPRSecondOrderSolver:
+    """HPRSecondOrderSolver (Solvers/hprsecondordersolver.h:56, 67)."""
+
+    @staticmethod
+    def solve(initial_cells, cell_spacing, cfl_coefficient, final_time, bias, slope_limiter, subcycling_iterations,
+              material_parameters, **kw):
+        return hpr_solve(initial_cells, 1, cell_spacing, cfl_coefficient, final_time, order=2, bias=bias,
+                         limiter=slope_limiter, subcycles=subcycling_iterations, params=material_parameters,
+                         **kw).cells
+
+    @staticmethod
+    def solve2D(initial_cells, cell_spacing, cfl_coefficient, final_time, bias, slope_limiter, subcycling_iterations,
+                material_parameters, **kw):
+        return hpr_solve(initial_cells, 2, cell_spacing, cfl_coefficient, final_time, order=2, bias=bias,
+                         limiter=slope_limiter, subcycles=subcycling_iterations, params=material_parameters,
+                         **kw).cells
+
+
+class HPRFirstOrderSolver:
+    """HPRFirstOrderSolver (Solvers/hprfirstordersolver.h): FORCE."""
+
+    @staticmethod
+    def solve(initial_cells, cell_spacing, cfl_coefficient, final_time, subcycling_iterations, material_parameters,
+              **kw):
+        return hpr_solve(initial_cells, 1, cell_spacing, cfl_coefficient, final_time, order=1,
+                         subcycles=subcycling_iterations, params=material_parameters, **kw).cells
+
+    @staticmethod
+    def solve2D(initial_cells, cell_spacing, cfl_coefficient, final_time, subcycling_iterations, material_parameters,
+                **kw):
+        return hpr_solve(initial_cells, 2, cell_spacing, cfl_coefficient, final_time, order=1,
+                         subcycles=subcycling_iterations, params=material_parameters, **kw).cells
+
+
+class HPRGrid:
+    """Device-resident HPR grid (vg_hpr_state)."""
+
+    def __init__(self, dim: int, ny: int, nx: int, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.lib = self.ctx.lib
+        self.dim, self.ny, self.nx = dim, ny, nx
+        self.handle = C.c_void_p()
+        check(self.lib.vg_hpr_state_create(self.ctx.handle, dim, ny, nx, C.byref(self.handle)), "vg_hpr_state_create")
+
+    def close(self):
+        if self.handle:
+            self.lib.vg_hpr_state_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load(self, prim: np.ndarray):
+        w = np.ascontiguousarray(prim, dtype=np.float64)
+        check(self.lib.vg_hpr_state_load(self.handle, w.ctypes.data_as(C.c_void_p), 0), "vg_hpr_state_load")
+
+    def load_device(self, ptr: int):
+        check(self.lib.vg_hpr_state_load(self.handle, C.c_void_p(ptr), 1), "vg_hpr_state_load")
+
+    def store(self) -> np.ndarray:
+        shape = (self.nx, 17) if self.dim == 1 else (self.ny, self.nx, 17)
+        out = np.empty(shape)
+        check(self.lib.vg_hpr_state_store(self.handle, out.ctypes.data_as(C.c_void_p), 0), "vg_hpr_state_store")
+        return out
+
+    def store_device(self, ptr: int):
+        check(self.lib.vg_hpr_state_store(self.handle, C.c_void_p(ptr), 1), "vg_hpr_state_store")
+
+    def run(self, dx, cfl, t_final=math.inf, max_steps=0, *, order=2, bias=0.0, limiter=0, bc=TRANSMISSIVE,
+            subcycles=0, params: HPRMaterialParameters | None = None, dt_cap=0):
+        params = params or HPRMaterialParameters()
+        mp = params.c_struct()
+        st = _lib.RunStats()
+        dts = np.empty(max(dt_cap, 1))
+        check(self.lib.vg_hpr_state_run(self.handle, order, dx, cfl, t_final, max_steps, bias, limiter, bc, subcycles,
+                                        C.byref(mp), EXACT, _lib.as_dptr(dts) if dt_cap else None, dt_cap,
+                                        C.byref(st)), "vg_hpr_state_run")
+        return st, dts[:min(st.steps, dt_cap)].copy()
+
+    def max_wave_speed(self, params: HPRMaterialParameters | None = None, bc=TRANSMISSIVE) -> float:
+        params = params or HPRMaterialParameters()
+        mp = params.c_struct()
+        s = C.c_double()
+        check(self.lib.vg_hpr_state_max_wave_speed(self.handle, bc, C.byref(mp), C.byref(s)),
+              "vg_hpr_state_max_wave_speed")
+        return s.value
