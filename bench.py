@@ -14,6 +14,13 @@ timing of every launch in the timed region) and the reference CPU solver timed
 on this host on a bounded sample of the same workload.
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 3] [--mode auto|exact|fast] [--no-cpu]
+
+Secondary workloads (one JSON line each, same contract):
+
+    --workload mhd_batch   batched 1D ideal-MHD Riemann problems (8192 instances,
+                           256..16384 cells, SLIC van Leer, CFL 0.8, t = 0.1)
+    --workload hpr2d       2D HPR viscous flow, 1024^2 periodic, 16 Fourier modes,
+                           mu = 1e-3, heat conduction, 1 relaxation sub-cycle per half step
 """
 
 from __future__ import annotations
@@ -146,8 +153,120 @@ def rel_l1_max(a, b):
     return worst
 
 
+def kernel_table(ctx, names):
+    out = {}
+    for k in names:
+        n, ms = ctx.kernel_time(k)
+        if n:
+            out[k] = {"launches": n, "total_ms": round(ms, 3), "avg_ms": round(ms / n, 4)}
+    return out
+
+
+def bench_mhd_batch(args):
+    """Batched 1D MHD: value = total cell-updates (sum over instances of
+    cells x steps) / wall time of the one batched launch."""
+    import torch
+    from vanguard_amd import problems
+    from vanguard_amd import solvers as S
+
+    n_inst = args.instances
+    prim, off, gamma, dx = problems.mhd_batch(n_inst, seed=3, n_min=256, n_max=args.max_cells)
+    tf = np.full(n_inst, 0.1)
+    ctx = S.default_context()
+    cells = np.diff(off)
+    if args.warmup:
+        S.mhd_solve_batch(prim[:off[64]], off[:65], dx[:64], gamma[:64], np.full(64, 1e-3), 0.8, limiter=1, ctx=ctx)
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = S.mhd_solve_batch(prim, off, dx, gamma, tf, 0.8, limiter=1, mode=S.EXACT if args.mode != "fast" else S.FAST,
+                          ctx=ctx)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.set_timing(False)
+    kt = kernel_table(ctx, ["batch_solve"])
+    updates = float((cells * r.steps.astype(np.float64)).sum())
+    dev_s = kt["batch_solve"]["total_ms"] * 1e-3
+    bytes_per_update = 144   # 9 FP64 primitive fields read + written per cell-update
+    achieved = bytes_per_update * updates / dev_s / 1e9
+    line = {
+        "metric": "Mcell-updates/s FP64 batched 1D MHD SLIC (sum over instances)", "value": round(updates / elapsed / 1e6, 2),
+        "unit": "Mcell-updates/s", "n_gpus": 1, "steps": int(r.steps.sum()), "warmup": args.warmup,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"mhd1d_batch_{n_inst}inst_256-{args.max_cells}cells_vanleer_t0.1",
+                   "instances": n_inst, "cells_total": int(cells.sum()), "mode": "fast" if args.mode == "fast" else "exact"},
+        "wall_s": round(elapsed, 3), "device_s": round(dev_s, 3),
+        "steps_per_instance": {"min": int(r.steps.min()), "max": int(r.steps.max()), "mean": float(r.steps.mean())},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "batch_solve"},
+        "kernels": kt, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_hpr2d(args):
+    """2D HPR, 1024^2 periodic: value = cell-updates/s over the timed steps."""
+    import torch
+    from vanguard_amd import problems
+    from vanguard_amd import solvers as S
+
+    n = args.hpr_n
+    w = problems.hpr_fourier_2d(n, seed=4)
+    p = S.HPRMaterialParameters(is_thermal=True, dynamic_viscosity_coefficient=1e-3, heat_wave_speed=2.0)
+    p.configure_strain_dissipation_time()
+    p.configure_thermal_impulse_relaxation_time(1e-3 * 1.4 * p.specific_heat_capacity / p.prandtl_number)
+    ctx = S.default_context()
+    dx = 1.0 / n
+    sub = args.subcycles
+    if args.warmup:
+        g0 = S.HPRGrid(2, n, n, ctx=ctx)
+        g0.load(w)
+        g0.run(dx, CFL, max_steps=args.warmup, subcycles=sub, params=p, bc=S.PERIODIC)
+        g0.close()
+    g = S.HPRGrid(2, n, n, ctx=ctx)
+    g.load(w)
+    g.max_wave_speed(p, bc=S.PERIODIC)
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st, _ = g.run(dx, CFL, max_steps=args.steps, subcycles=sub, params=p, bc=S.PERIODIC)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.set_timing(False)
+    kt = kernel_table(ctx, ["sweep_x_slic", "sweep_y_slic", "hpr_source_rk4"])
+    out = g.store()
+    g.close()
+    cells = n * n
+    bytes_per_update = 816 + 544 * sub
+    name = max(kt, key=lambda k: kt[k]["total_ms"])
+    avg = kt[name]["avg_ms"]
+    per_launch = {"sweep_x_slic": 2 * 17 * 8, "sweep_y_slic": 2 * 17 * 8, "hpr_source_rk4": 2 * 17 * 8}[name] * cells
+    achieved = per_launch / (avg * 1e-3) / 1e9
+    line = {
+        "metric": "Mcell-updates/s FP64 2D HPR SLIC + RK4 relaxation", "value": round(cells * st.steps / elapsed / 1e6, 3),
+        "unit": "Mcell-updates/s", "n_gpus": 1, "steps": st.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / st.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"hpr2d_{n}x{n}_periodic_fourier16_mu1e-3_thermal_sub{sub}", "grid": [n, n],
+                   "subcycles": sub, "cfl": CFL, "limiter": "superbee"},
+        "algorithmic_bytes_per_cell_update": bytes_per_update,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name},
+        "kernels": kt, "finite": bool(np.isfinite(out).all()), "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["euler2d", "mhd_batch", "hpr2d"], default="euler2d")
+    ap.add_argument("--instances", type=int, default=8192)
+    ap.add_argument("--max-cells", type=int, default=16384)
+    ap.add_argument("--hpr-n", type=int, default=1024)
+    ap.add_argument("--subcycles", type=int, default=1)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
@@ -171,6 +290,10 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
     torch.cuda.init()
+    if args.workload == "mhd_batch":
+        return bench_mhd_batch(args)
+    if args.workload == "hpr2d":
+        return bench_hpr2d(args)
     log(f"building initial condition {N}x{N} ...")
     w = problems.disc_2d(N)
     ctx = S.default_context()

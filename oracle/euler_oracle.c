@@ -190,3 +190,7 @@ void orc_limiter_units(int n, double bias, const double *r, double *out)
         out[5 * (size_t)k + 4] = orc_limiter_value(r[k], bias, 7);
     }
 }
+
+/* Entry points for the RGFM oracle (rgfm_oracle.c). */
+orc_phys orc_euler_phys_table(const orc_euler_params *mp) { return euler_phys(mp); }
+double orc_euler_sound_speed(double rho, double p, const orc_euler_params *mp) { return eos_c(rho, p, mp); }

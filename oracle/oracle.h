@@ -84,6 +84,29 @@ int orc_hpr_solve(int order, int dim, int ny, int nx, double dx, double cfl, dou
 void orc_hpr_units(int n, double bias, int limiter, double dx, double ts, const orc_hpr_params *mp,
                    const double *states, double *out);
 
+/* RGFMSolver::solveHLLC / solveHLLC2D (riemann 0) and solveExact / solveExact2D
+ * (riemann 1).  Two full-domain material grids (primitive AoS, 6 doubles per
+ * cell) and the level set (1 double per cell).  dy is used by the Y sweep and
+ * the Y level-set pass and min(dx, dy) by the time step; dy == dx is the
+ * reference. */
+int orc_rgfm_solve(int riemann, int dim, int ny, int nx, double dx, double dy, double cfl, double t_final,
+                   int max_steps, double bias, int limiter, const orc_euler_params *p1, const orc_euler_params *p2,
+                   const double *m1_in, const double *m2_in, const double *phi_in, double *m1_out, double *m2_out,
+                   double *phi_out, double *dt_log, int dt_cap);
+/* applyRGFMBoundaryConditions{HLLC,Exact}[2D], in place. */
+void orc_rgfm_ghost_fill(int riemann, int dim, int ny, int nx, const orc_euler_params *p1,
+                         const orc_euler_params *p2, double *m1, double *m2, const double *phi);
+/* updateLevelSetFunction (dim 1) / update2DLevelSetFunctionX (dir 0) / Y (dir 1). */
+void orc_rgfm_level_set(int dim, int dir, int ny, int nx, double h, double dt, const double *m1, const double *m2,
+                        const double *phi_in, double *phi_out);
+/* HLLC records (12 doubles -> 24) and exact-solver records (12 -> 10). */
+void orc_hllc_units(int n, const orc_euler_params *p1, const orc_euler_params *p2, const double *states,
+                    double *out);
+void orc_exact_units(int n, double s, const orc_euler_params *p1, const orc_euler_params *p2, const double *states,
+                     double *out);
+/* Interface circulation over phi < 0 from material 1's velocity. */
+double orc_rgfm_circulation(int ny, int nx, double dx, double dy, const double *m1, const double *phi);
+
 #ifdef __cplusplus
 }
 #endif

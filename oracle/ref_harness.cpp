@@ -24,6 +24,7 @@
 #include "Solvers/hprfirstordersolver.h"
 #include "Solvers/hprforcingsolver.h"
 #include "HPR/hpracoustictensor.h"
+#include "RGFM/rgfmsolver.h"
 
 #include <cstdio>
 #include <cstdlib>
@@ -664,6 +665,189 @@ int hprUnits(int argc, char **argv)
 
 } // namespace
 
+// rgfm_run dim ny nx dx cfl tfinal max_steps bias limiter g1 pinf1 cv1 ef1 tign1 k1 g2 pinf2 cv2 ef2 tign2 k2
+//          in1 in2 phi out1 out2 out_phi out_dt check_api
+// A loop over the reference's public per-step functions mirroring
+// RGFMSolver::solveHLLC (rgfmsolver.cpp:745-784) and solveHLLC2D (:786-840).
+int rgfmRun(int argc, char **argv)
+{
+    if (argc < 31) { std::fprintf(stderr, "rgfm_run: too few arguments\n"); return 2; }
+    int dim = std::atoi(argv[2]);
+    int ny = std::atoi(argv[3]);
+    int nx = std::atoi(argv[4]);
+    double dx = std::atof(argv[5]);
+    double cfl = std::atof(argv[6]);
+    double tfinal = std::strcmp(argv[7], "inf") == 0 ? std::numeric_limits<double>::infinity() : std::atof(argv[7]);
+    int maxSteps = std::atoi(argv[8]);
+    double bias = std::atof(argv[9]);
+    int limiter = std::atoi(argv[10]);
+    EulerMaterialParameters p1 = eulerParams(argv + 11);
+    EulerMaterialParameters p2 = eulerParams(argv + 17);
+    const size_t cells = (size_t)ny * nx;
+    std::vector<double> in1 = readDoubles(argv[23], cells * 6);
+    std::vector<double> in2 = readDoubles(argv[24], cells * 6);
+    std::vector<double> phiIn = readDoubles(argv[25], cells);
+    int checkApi = std::atoi(argv[30]);
+    std::vector<double> out1(cells * 6), out2(cells * 6), outPhi(cells), dts;
+    MuteCout mute;
+    auto t0 = std::chrono::steady_clock::now();
+    if (dim == 1)
+    {
+        std::vector<EulerStateVector> m1(nx), m2(nx);
+        std::vector<double> phi(phiIn.begin(), phiIn.end());
+        for (int j = 0; j < nx; j++) { m1[j] = stateFrom(&in1[(size_t)j * 6]); m2[j] = stateFrom(&in2[(size_t)j * 6]); }
+        MultimaterialSystem initial(m1, m2, phi);
+        double t = 0.0;
+        int it = 0;
+        while (t < tfinal && (maxSteps == 0 || it < maxSteps))
+        {
+            MultimaterialSystem g = RGFMSolver::applyRGFMBoundaryConditionsHLLC(MultimaterialSystem(m1, m2, phi), p1, p2);
+            m1 = g.getMaterial1Cells();
+            m2 = g.getMaterial2Cells();
+            std::vector<EulerStateVector> b1 = Solvers::insertBoundaryCells(m1, 2);
+            std::vector<EulerStateVector> b2 = Solvers::insertBoundaryCells(m2, 2);
+            double dt = std::min(Solvers::computeStableTimeStep(b1, dx, cfl, t, tfinal, it, p1),
+                                 Solvers::computeStableTimeStep(b2, dx, cfl, t, tfinal, it, p2));
+            phi = RGFMSolver::updateLevelSetFunction(phi, dx, dt, m1, m2);
+            SecondOrderSolver::computeSLICTimeStep(m1, b1, dx, dt, bias, limiter, p1);
+            SecondOrderSolver::computeSLICTimeStep(m2, b2, dx, dt, bias, limiter, p2);
+            t += dt;
+            it += 1;
+            dts.push_back(dt);
+        }
+        for (int j = 0; j < nx; j++) { putState(m1[j], &out1[(size_t)j * 6]); putState(m2[j], &out2[(size_t)j * 6]); }
+        for (int j = 0; j < nx; j++) outPhi[j] = phi[j];
+        if (checkApi && maxSteps == 0)
+        {
+            MultimaterialSystem api = RGFMSolver::solveHLLC(initial, dx, cfl, tfinal, bias, limiter, 0, 0, p1, p2);
+            std::vector<EulerStateVector> a1 = api.getMaterial1Cells(), a2 = api.getMaterial2Cells();
+            std::vector<double> ap = api.getLevelSetFunction();
+            bool match = true;
+            for (int j = 0; j < nx; j++)
+            {
+                double q1[6], q2[6];
+                putState(a1[j], q1);
+                putState(a2[j], q2);
+                for (int f = 0; f < 6; f++)
+                    match = match && sameBits(q1[f], out1[(size_t)j * 6 + f]) && sameBits(q2[f], out2[(size_t)j * 6 + f]);
+                match = match && sameBits(ap[j], outPhi[j]);
+            }
+            std::fprintf(stderr, "API_MATCH %d\n", match ? 1 : 0);
+        }
+    }
+    else
+    {
+        typedef std::vector<std::vector<EulerStateVector> > Grid;
+        Grid m1(ny, std::vector<EulerStateVector>(nx)), m2 = m1;
+        std::vector<std::vector<double> > phi(ny, std::vector<double>(nx));
+        for (int i = 0; i < ny; i++)
+            for (int j = 0; j < nx; j++)
+            {
+                m1[i][j] = stateFrom(&in1[((size_t)i * nx + j) * 6]);
+                m2[i][j] = stateFrom(&in2[((size_t)i * nx + j) * 6]);
+                phi[i][j] = phiIn[(size_t)i * nx + j];
+            }
+        MultimaterialSystem initial(m1, m2, phi);
+        double t = 0.0;
+        int it = 0;
+        while (t < tfinal && (maxSteps == 0 || it < maxSteps))
+        {
+            MultimaterialSystem g = RGFMSolver::applyRGFMBoundaryConditionsHLLC2D(MultimaterialSystem(m1, m2, phi), p1, p2);
+            m1 = g.getMaterial1Cells2D();
+            m2 = g.getMaterial2Cells2D();
+            Grid b1 = Solvers::insertBoundaryCells2D(m1, 2);
+            Grid b2 = Solvers::insertBoundaryCells2D(m2, 2);
+            double dt = std::min(Solvers::computeStableTimeStep2D(b1, dx, cfl, t, tfinal, it, p1),
+                                 Solvers::computeStableTimeStep2D(b2, dx, cfl, t, tfinal, it, p2));
+            phi = RGFMSolver::update2DLevelSetFunctionX(phi, dx, 0.5 * dt, m1, m2);
+            phi = RGFMSolver::update2DLevelSetFunctionY(phi, dx, dt, m1, m2);
+            phi = RGFMSolver::update2DLevelSetFunctionX(phi, dx, 0.5 * dt, m1, m2);
+            SecondOrderSolver::computeXSLICTimeStep2D(m1, b1, dx, 0.5 * dt, bias, limiter, p1);
+            b1 = Solvers::insertBoundaryCells2D(m1, 2);
+            SecondOrderSolver::computeYSLICTimeStep2D(m1, b1, dx, dt, bias, limiter, p1);
+            b1 = Solvers::insertBoundaryCells2D(m1, 2);
+            SecondOrderSolver::computeXSLICTimeStep2D(m1, b1, dx, 0.5 * dt, bias, limiter, p1);
+            SecondOrderSolver::computeXSLICTimeStep2D(m2, b2, dx, 0.5 * dt, bias, limiter, p2);
+            b2 = Solvers::insertBoundaryCells2D(m2, 2);
+            SecondOrderSolver::computeYSLICTimeStep2D(m2, b2, dx, dt, bias, limiter, p2);
+            b2 = Solvers::insertBoundaryCells2D(m2, 2);
+            SecondOrderSolver::computeXSLICTimeStep2D(m2, b2, dx, 0.5 * dt, bias, limiter, p2);
+            t += dt;
+            it += 1;
+            dts.push_back(dt);
+        }
+        for (int i = 0; i < ny; i++)
+            for (int j = 0; j < nx; j++)
+            {
+                putState(m1[i][j], &out1[((size_t)i * nx + j) * 6]);
+                putState(m2[i][j], &out2[((size_t)i * nx + j) * 6]);
+                outPhi[(size_t)i * nx + j] = phi[i][j];
+            }
+        if (checkApi && maxSteps == 0)
+        {
+            MultimaterialSystem api = RGFMSolver::solveHLLC2D(initial, dx, cfl, tfinal, bias, limiter, 0, 0, p1, p2);
+            Grid a1 = api.getMaterial1Cells2D(), a2 = api.getMaterial2Cells2D();
+            std::vector<std::vector<double> > ap = api.getLevelSetFunction2D();
+            bool match = true;
+            for (int i = 0; i < ny; i++)
+                for (int j = 0; j < nx; j++)
+                {
+                    double q1[6], q2[6];
+                    putState(a1[i][j], q1);
+                    putState(a2[i][j], q2);
+                    for (int f = 0; f < 6; f++)
+                        match = match && sameBits(q1[f], out1[((size_t)i * nx + j) * 6 + f]) &&
+                                sameBits(q2[f], out2[((size_t)i * nx + j) * 6 + f]);
+                    match = match && sameBits(ap[i][j], outPhi[(size_t)i * nx + j]);
+                }
+            std::fprintf(stderr, "API_MATCH %d\n", match ? 1 : 0);
+        }
+    }
+    double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    writeDoubles(argv[26], out1);
+    writeDoubles(argv[27], out2);
+    writeDoubles(argv[28], outPhi);
+    writeDoubles(argv[29], dts);
+    std::fprintf(stderr, "STEPS %zu SECONDS %.6f\n", dts.size(), secs);
+    return 0;
+}
+
+// hllc_units n g1 pinf1 cv1 ef1 tign1 k1 g2 pinf2 cv2 ef2 tign2 k2 in out
+// Records of two primitive states (12 doubles) -> 24 doubles:
+//   solveX state (6) | left star rho | right star rho | star u | star p_x | S_L | S_R |
+//   solveY state (6) | top star rho | bottom star rho | star v | star p_y | S_T | S_B
+int hllcUnits(int argc, char **argv)
+{
+    if (argc < 17) { std::fprintf(stderr, "hllc_units: too few arguments\n"); return 2; }
+    int n = std::atoi(argv[2]);
+    EulerMaterialParameters p1 = eulerParams(argv + 3);
+    EulerMaterialParameters p2 = eulerParams(argv + 9);
+    std::vector<double> in = readDoubles(argv[15], (size_t)n * 12);
+    std::vector<double> out((size_t)n * 24);
+    for (int r = 0; r < n; r++)
+    {
+        EulerStateVector a = stateFrom(&in[(size_t)r * 12]);
+        EulerStateVector b = stateFrom(&in[(size_t)r * 12 + 6]);
+        double *o = &out[(size_t)r * 24];
+        putState(HLLCSolver::solveX(a, b, p1, p2), o);
+        o[6] = HLLCSolver::computeLeftStarRegionDensity(a, b, p1, p2);
+        o[7] = HLLCSolver::computeRightStarRegionDensity(a, b, p1, p2);
+        o[8] = HLLCSolver::computeStarRegionXVelocity(a, b, p1, p2);
+        o[9] = HLLCSolver::computeStarRegionXPressure(a, b, p1, p2);
+        o[10] = HLLCSolver::computeLeftWaveSpeed(a, b, p1, p2);
+        o[11] = HLLCSolver::computeRightWaveSpeed(a, b, p1, p2);
+        putState(HLLCSolver::solveY(a, b, p1, p2), o + 12);
+        o[18] = HLLCSolver::computeTopStarRegionDensity(a, b, p1, p2);
+        o[19] = HLLCSolver::computeBottomStarRegionDensity(a, b, p1, p2);
+        o[20] = HLLCSolver::computeStarRegionYVelocity(a, b, p1, p2);
+        o[21] = HLLCSolver::computeStarRegionYPressure(a, b, p1, p2);
+        o[22] = HLLCSolver::computeTopWaveSpeed(a, b, p1, p2);
+        o[23] = HLLCSolver::computeBottomWaveSpeed(a, b, p1, p2);
+    }
+    writeDoubles(argv[16], out);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) { std::fprintf(stderr, "usage: ref_harness <mode> ...\n"); return 2; }
@@ -675,6 +859,8 @@ int main(int argc, char **argv)
     if (mode == "mhd_units") return mhdUnits(argc, argv);
     if (mode == "hpr_run") return hprRun(argc, argv);
     if (mode == "hpr_units") return hprUnits(argc, argv);
+    if (mode == "rgfm_run") return rgfmRun(argc, argv);
+    if (mode == "hllc_units") return hllcUnits(argc, argv);
     std::fprintf(stderr, "unknown mode %s\n", mode.c_str());
     return 2;
 }

@@ -49,4 +49,9 @@ int orc_solve(orc_phys *ph, int order, int dim, int ny, int nx, double dx, doubl
               int max_steps, double bias, int limiter, int bc, const double *prim_in, double *prim_out,
               double *dt_log, int dt_cap);
 
+/* Euler physics table and EOS sound speed (euler_oracle.c), for the RGFM oracle. */
+#include "oracle.h"
+orc_phys orc_euler_phys_table(const orc_euler_params *mp);
+double orc_euler_sound_speed(double rho, double p, const orc_euler_params *mp);
+
 #endif

@@ -330,7 +330,121 @@ def hpr_units(name, n, seed, thermal):
     print(f"{name}: {n} records")
 
 
+def rgfm_call(m1, m2, phi, *, dim, dx, cfl, tfinal, max_steps, bias, limiter, p1, p2, check_api):
+    with tempfile.TemporaryDirectory() as tmp:
+        ny, nx = (1, phi.shape[0]) if dim == 1 else phi.shape[:2]
+        for nm, a in (("in1.f64", m1), ("in2.f64", m2), ("phi.f64", phi)):
+            np.ascontiguousarray(a, dtype=np.float64).tofile(os.path.join(tmp, nm))
+        t = "inf" if np.isinf(tfinal) else repr(float(tfinal))
+        log = run(["rgfm_run", dim, ny, nx, repr(dx), repr(cfl), t, max_steps, repr(bias), limiter,
+                   *[repr(v) for v in p1], *[repr(v) for v in p2], "in1.f64", "in2.f64", "phi.f64", "o1.f64",
+                   "o2.f64", "ophi.f64", "dt.f64", int(check_api)], tmp)
+        o1 = np.fromfile(os.path.join(tmp, "o1.f64")).reshape(m1.shape)
+        o2 = np.fromfile(os.path.join(tmp, "o2.f64")).reshape(m2.shape)
+        ophi = np.fromfile(os.path.join(tmp, "ophi.f64")).reshape(phi.shape)
+        dt = np.fromfile(os.path.join(tmp, "dt.f64"))
+    if check_api and max_steps == 0 and "API_MATCH 1" not in log:
+        raise RuntimeError("RGFM mirror loop differs from the reference's own solveHLLC")
+    return o1, o2, ophi, dt
+
+
+def rgfm_run(name, m1, m2, phi, **kw):
+    o1, o2, ophi, dt = rgfm_call(m1, m2, phi, **kw)
+    meta = dict(kw)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), m1_in=m1, m2_in=m2, phi_in=phi, m1_out=o1, m2_out=o2,
+                        phi_out=ophi, dt=dt, meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
+    print(f"{name}: {len(dt)} steps")
+
+
+def rgfm_toro_1d(n):
+    """EulerRGFMTests::solveToroTest1HLLC (eulerrgfmtests.cpp:27-45)."""
+    m1 = np.tile([1.0, 0.0, 0.0, 0.0, 1.0, 1.0], (n, 1))
+    m2 = np.tile([0.125, 0.0, 0.0, 0.0, 0.1, 1.0], (n, 1))
+    phi = (1.0 / n) * np.arange(n) - 0.5
+    return m1, m2, phi
+
+
+def rgfm_fedkiw_1d(n):
+    """EulerRGFMTests::solveFedkiwTestHLLC (eulerrgfmtests.cpp:76-103)."""
+    x = (1.0 / n) * np.arange(n)
+    m1 = np.tile([1.0, 0.0, 0.0, 0.0, 1e5, 1.0], (n, 1))
+    m1[x <= 0.05] = [1.3333, 0.3535 * np.sqrt(1e5), 0.0, 0.0, 1.5e5, 1.0]
+    m2 = np.tile([0.1379, 0.0, 0.0, 0.0, 1e5, 1.0], (n, 1))
+    return m1, m2, x - 0.5
+
+
+def rgfm_disc_2d(n, fedkiw):
+    """EulerRGFMTests::solve2DToroTest1HLLC / solve2DFedkiwTestHLLC
+    (eulerrgfmtests.cpp:128-148, 183-213)."""
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    r = np.sqrt((i - 0.5 * n) ** 2 + (j - 0.5 * n) ** 2)
+    m1 = np.zeros((n, n, 6))
+    m2 = np.zeros((n, n, 6))
+    m1[..., 5] = m2[..., 5] = 1.0
+    if fedkiw:
+        m1[..., 0], m1[..., 4] = 1.0, 1e5
+        inner = r <= 0.02 * n
+        m1[inner, 0], m1[inner, 1], m1[inner, 4] = 1.3333, 0.3535 * np.sqrt(1e5), 1.5e5
+        m2[..., 0], m2[..., 4] = 0.1379, 1e5
+    else:
+        m1[..., 0], m1[..., 4] = 1.0, 1.0
+        m2[..., 0], m2[..., 4] = 0.125, 0.1
+    return m1, m2, r / n - 0.2
+
+
+def hllc_units(name, n, seed):
+    rng = np.random.default_rng(seed)
+    sets = [((1.4, 0.0, 25.0, 0.0, 0.25, 250.0), (1.648, 0.0, 25.0, 0.0, 0.25, 250.0)),
+            ((1.4, 0.0, 25.0, 0.0, 0.25, 250.0), (1.4, 0.0, 25.0, 0.0, 0.25, 250.0)),
+            ((4.4, 0.5, 25.0, 0.0, 0.25, 250.0), (1.4, 0.0, 25.0, 0.0, 0.25, 250.0))]
+    out = {}
+    for k, (p1, p2) in enumerate(sets):
+        st = np.empty((n, 12))
+        for s in (0, 6):
+            st[:, s + 0] = 10.0 ** rng.uniform(-1, 1, n)
+            st[:, s + 1:s + 4] = rng.uniform(-1.5, 1.5, (n, 3))
+            st[:, s + 4] = 10.0 ** rng.uniform(-1, 1.5, n)
+            st[:, s + 5] = 1.0
+        # a third of the pairs with nearly equal pressures (the PVRS region)
+        near = rng.random(n) < 0.33
+        st[near, 10] = st[near, 4] * rng.uniform(0.8, 1.25, near.sum())
+        with tempfile.TemporaryDirectory() as tmp:
+            st.tofile(os.path.join(tmp, "in.f64"))
+            run(["hllc_units", n, *[repr(v) for v in p1], *[repr(v) for v in p2], "in.f64", "out.f64"], tmp)
+            out[f"out{k}"] = np.fromfile(os.path.join(tmp, "out.f64")).reshape(n, 24)
+        out[f"states{k}"] = st
+        out[f"p1_{k}"] = np.array(p1)
+        out[f"p2_{k}"] = np.array(p2)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print(f"{name}: {n} x {len(sets)} records")
+
+
+def rgfm_bubble_small():
+    m1, m2, phi, dx, dy = problems.shock_bubble_2d(96, 32, length=(0.325, 0.325 * 32 / 96), center=(0.175, 0.325 / 6),
+                                                  radius=0.03)
+    return m1, m2, phi, dx
+
+
 JOBS = {
+    "rgfm_toro1d_hllc": lambda: rgfm_run("rgfm_toro1d_hllc", *rgfm_toro_1d(200), dim=1, dx=1 / 200, cfl=0.8,
+                                         tfinal=0.25, max_steps=0, bias=0.0, limiter=0, p1=IDEAL, p2=IDEAL,
+                                         check_api=True),
+    "rgfm_fedkiw1d_hllc": lambda: rgfm_run("rgfm_fedkiw1d_hllc", *rgfm_fedkiw_1d(200), dim=1, dx=1 / 200, cfl=0.8,
+                                           tfinal=0.0012, max_steps=0, bias=0.0, limiter=0, p1=IDEAL,
+                                           p2=(1.67, 0.0, 25.0, 0.0, 0.25, 250.0), check_api=True),
+    "rgfm_2d_toro32": lambda: rgfm_run("rgfm_2d_toro32", *rgfm_disc_2d(32, False), dim=2, dx=1 / 32, cfl=0.8,
+                                       tfinal=np.inf, max_steps=12, bias=0.0, limiter=0, p1=IDEAL, p2=IDEAL,
+                                       check_api=False),
+    "rgfm_2d_fedkiw32": lambda: rgfm_run("rgfm_2d_fedkiw32", *rgfm_disc_2d(32, True), dim=2, dx=1 / 32, cfl=0.8,
+                                         tfinal=np.inf, max_steps=12, bias=0.0, limiter=1, p1=IDEAL,
+                                         p2=(1.67, 0.0, 25.0, 0.0, 0.25, 250.0), check_api=False),
+    "rgfm_2d_fedkiw24_api": lambda: rgfm_run("rgfm_2d_fedkiw24_api", *rgfm_disc_2d(24, True), dim=2, dx=1 / 24,
+                                             cfl=0.8, tfinal=2e-5, max_steps=0, bias=0.0, limiter=0, p1=IDEAL,
+                                             p2=(1.67, 0.0, 25.0, 0.0, 0.25, 250.0), check_api=True),
+    "rgfm_2d_bubble96": lambda: rgfm_run("rgfm_2d_bubble96", *rgfm_bubble_small()[:3], dim=2, dx=rgfm_bubble_small()[3],
+                                         cfl=0.8, tfinal=np.inf, max_steps=15, bias=0.0, limiter=0,
+                                         p1=problems.HELIUM, p2=problems.AIR, check_api=False),
+    "hllc_units": lambda: hllc_units("hllc_units", 300, 41),
     "hpr_stokes100": lambda: hpr_run("hpr_stokes100", hpr_stokes(100), order=2, dim=1, dx=0.01, cfl=0.8, tfinal=0.05,
                                      max_steps=0, bias=0.0, limiter=0, subcycles=0, params=hpr_params(False),
                                      check_api=True),

@@ -59,8 +59,74 @@ def load():
     lib.orc_hpr_units.restype = None
     lib.orc_hpr_units.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.POINTER(OrcHPRParams),
                                   dptr, dptr]
+    P = C.POINTER(OrcEulerParams)
+    lib.orc_rgfm_solve.restype = C.c_int
+    lib.orc_rgfm_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                   C.c_int, C.c_double, C.c_int, P, P, dptr, dptr, dptr, dptr, dptr, dptr, dptr,
+                                   C.c_int]
+    lib.orc_rgfm_ghost_fill.restype = None
+    lib.orc_rgfm_ghost_fill.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, P, P, dptr, dptr, dptr]
+    lib.orc_rgfm_level_set.restype = None
+    lib.orc_rgfm_level_set.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, dptr, dptr, dptr,
+                                       dptr]
+    lib.orc_hllc_units.restype = None
+    lib.orc_hllc_units.argtypes = [C.c_int, P, P, dptr, dptr]
+    lib.orc_exact_units.restype = None
+    lib.orc_exact_units.argtypes = [C.c_int, C.c_double, P, P, dptr, dptr]
+    lib.orc_rgfm_circulation.restype = C.c_double
+    lib.orc_rgfm_circulation.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, dptr, dptr]
     _lib = lib
     return lib
+
+
+def rgfm_solve(m1, m2, phi, *, dim, dx, cfl, tfinal, max_steps, bias, limiter, p1, p2, dy=None, riemann=0,
+               dt_cap=200000):
+    lib = load()
+    a1 = np.ascontiguousarray(m1, dtype=np.float64)
+    a2 = np.ascontiguousarray(m2, dtype=np.float64)
+    ph = np.ascontiguousarray(phi, dtype=np.float64)
+    o1, o2, oph = np.empty_like(a1), np.empty_like(a2), np.empty_like(ph)
+    dts = np.empty(dt_cap)
+    ny, nx = (1, ph.shape[0]) if dim == 1 else ph.shape
+    q1, q2 = OrcEulerParams(*p1), OrcEulerParams(*p2)
+    n = lib.orc_rgfm_solve(riemann, dim, ny, nx, dx, dx if dy is None else dy, cfl, tfinal, max_steps, bias, limiter,
+                           C.byref(q1), C.byref(q2), _p(a1), _p(a2), _p(ph), _p(o1), _p(o2), _p(oph), _p(dts), dt_cap)
+    return o1, o2, oph, dts[:min(n, dt_cap)].copy()
+
+
+def hllc_units(states, p1, p2):
+    lib = load()
+    s = np.ascontiguousarray(states, dtype=np.float64)
+    out = np.empty((s.shape[0], 24))
+    q1, q2 = OrcEulerParams(*p1), OrcEulerParams(*p2)
+    lib.orc_hllc_units(s.shape[0], C.byref(q1), C.byref(q2), _p(s), _p(out))
+    return out
+
+
+def exact_units(states, p1, p2, s=0.0):
+    lib = load()
+    st = np.ascontiguousarray(states, dtype=np.float64)
+    out = np.empty((st.shape[0], 10))
+    q1, q2 = OrcEulerParams(*p1), OrcEulerParams(*p2)
+    lib.orc_exact_units(st.shape[0], s, C.byref(q1), C.byref(q2), _p(st), _p(out))
+    return out
+
+
+def rgfm_level_set(dim, direction, phi, m1, m2, h, dt):
+    lib = load()
+    ph = np.ascontiguousarray(phi, dtype=np.float64)
+    out = np.empty_like(ph)
+    ny, nx = (1, ph.shape[0]) if dim == 1 else ph.shape
+    lib.orc_rgfm_level_set(dim, direction, ny, nx, h, dt, _p(np.ascontiguousarray(m1, dtype=np.float64)),
+                           _p(np.ascontiguousarray(m2, dtype=np.float64)), _p(ph), _p(out))
+    return out
+
+
+def rgfm_circulation(m1, phi, dx, dy):
+    lib = load()
+    a = np.ascontiguousarray(m1, dtype=np.float64)
+    ph = np.ascontiguousarray(phi, dtype=np.float64)
+    return lib.orc_rgfm_circulation(ph.shape[0], ph.shape[1], dx, dy, _p(a), _p(ph))
 
 
 class OrcHPRParams(C.Structure):

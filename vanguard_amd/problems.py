@@ -187,3 +187,64 @@ def hpr_fourier_2d(n: int, seed: int = 4, modes: int = 16, amplitude: float = 0.
     w[..., 11] = vel[0]
     w[..., 12] = vel[1]
     return w
+
+
+# ---- two-material (RGFM) problems ---------------------------------------------------
+
+AIR = (1.4, 0.0, 25.0, 0.0, 0.25, 250.0)        # EulerMaterialParameters(1.4)
+HELIUM = (1.648, 0.0, 25.0, 0.0, 0.25, 250.0)   # EulerMaterialParameters(1.648)
+
+
+def post_shock_state(rho0: float, p0: float, gamma: float, mach: float):
+    """Rankine-Hugoniot state behind a normal shock of Mach ``mach`` running
+    into gas at rest (rho0, p0): returns (rho1, |u1|, p1) with
+    p1/p0 = 1 + 2 gamma (M^2 - 1)/(gamma + 1),
+    rho1/rho0 = (gamma + 1) M^2 / ((gamma - 1) M^2 + 2),
+    |u1| = 2 c0 (M - 1/M) / (gamma + 1), the gas following the shock."""
+    m2 = mach * mach
+    c0 = np.sqrt(gamma * p0 / rho0)
+    p1 = p0 * (1.0 + 2.0 * gamma * (m2 - 1.0) / (gamma + 1.0))
+    rho1 = rho0 * (gamma + 1.0) * m2 / ((gamma - 1.0) * m2 + 2.0)
+    u1 = 2.0 * c0 * (mach - 1.0 / mach) / (gamma + 1.0)
+    return rho1, u1, p1
+
+
+def shock_bubble_2d(nx: int, ny: int, *, length=(0.325, 0.089), center=(0.175, 0.0445), radius: float = 0.025,
+                    mach: float = 1.22, shock_x: float = 0.225, air=(1.0, 1e5), helium=(0.138, 1e5)):
+    """Helium bubble in air hit by a planar shock (the Haas-Sturtevant /
+    Quirk-Karni geometry), cell-centred on an nx x ny grid over
+    [0, Lx] x [0, Ly].
+
+    Material 1 is helium (real inside the bubble, phi < 0), material 2 air (real
+    outside; the rows start in air, which is how the 2D ghost fill's toggle
+    starts, rgfmsolver.cpp:374-379).  Both materials are defined everywhere:
+    helium at rest, air at rest for x < shock_x and in the Rankine-Hugoniot
+    post-shock state (moving -x) beyond.  phi is the signed distance to the bubble.
+    Returns (m1, m2, phi, dx, dy)."""
+    lx, ly = length
+    dx, dy = lx / nx, ly / ny
+    x = (np.arange(nx) + 0.5) * dx
+    y = (np.arange(ny) + 0.5) * dy
+    X, Y = np.meshgrid(x, y)
+    m1 = np.zeros((ny, nx, 6))
+    m1[..., 0] = helium[0]
+    m1[..., 4] = helium[1]
+    m1[..., 5] = 1.0
+    m2 = np.zeros((ny, nx, 6))
+    m2[..., 0] = air[0]
+    m2[..., 4] = air[1]
+    m2[..., 5] = 1.0
+    rho1, u1, p1 = post_shock_state(air[0], air[1], AIR[0], mach)
+    behind = X > shock_x
+    m2[behind, 0] = rho1
+    m2[behind, 1] = -u1
+    m2[behind, 4] = p1
+    phi = np.sqrt((X - center[0]) ** 2 + (Y - center[1]) ** 2) - radius
+    return m1, m2, phi, dx, dy
+
+
+def shock_bubble_batch(n_inst: int = 16, seed: int = 6):
+    """Per-instance (radius, mach): radius uniform in [0.015, 0.03], Mach uniform
+    in [1.1, 1.5], two splitmix64 draws per instance (radius first)."""
+    u = uniform01(seed, 2 * n_inst).reshape(n_inst, 2)
+    return 0.015 + 0.015 * u[:, 0], 1.1 + 0.4 * u[:, 1]
