@@ -175,6 +175,47 @@ int vg_mhd_solve_batch(vg_ctx *ctx, int n_inst, const int64_t *offsets, const do
                        double bias, int limiter, int order, int mode, int32_t *steps, double *t_out,
                        vg_run_stats *stats);
 
+/* ---- two-material Euler: real ghost fluid method (RGFM) ----------------- */
+
+enum {
+    VG_RIEMANN_HLLC = 0,  /* HLLCSolver (RGFM/hllcsolver.cpp): RGFMSolver::solveHLLC[2D] */
+    VG_RIEMANN_EXACT = 1  /* ExactSolver (RGFM/exactsolver.cpp): RGFMSolver::solveExact[2D] */
+};
+
+/* RGFMSolver::solveHLLC / solveExact (rgfmsolver.h:30-36, rgfmsolver.cpp:648-687, 745-784),
+ * zero sub-cycles and no reinitialisation, as the reference runs them.
+ * MultimaterialSystem (multimaterialsystem.h:8-39) is passed as its three
+ * arrays: material 1 and material 2 primitive states over the whole domain
+ * (n x 6 doubles each, EulerStateVector order) and the level set (n doubles).
+ * The level set's sign changes mark the interface; the material flag starts at
+ * material 1 on the left. */
+int vg_rgfm_solve_1d(vg_ctx *ctx, const double *m1_in, const double *m2_in, const double *phi_in, double *m1_out,
+                     double *m2_out, double *phi_out, int n, double dx, double cfl, double t_final, int max_steps,
+                     double bias, int limiter, int riemann, const vg_euler_params *p1, const vg_euler_params *p2,
+                     int mode, double *dt_log, int dt_log_cap, vg_run_stats *stats);
+
+/* RGFMSolver::solveHLLC2D / solveExact2D (rgfmsolver.h:33-38, rgfmsolver.cpp:689-743, 786-840):
+ * ny x nx grids, rows contiguous; each row and column starts in material 2 (the
+ * reference's 2D flag starts "in the ghost region").  dy is the y spacing
+ * (dy == dx is the reference; dy != dx uses dy in the y sweeps and the y
+ * level-set pass and min(dx, dy) in the time step).  `circulation` (may be NULL)
+ * receives sum over interior cells with phi < 0 of (dv/dx - du/dy) dx dy of
+ * material 1's final velocity (central differences). */
+int vg_rgfm_solve_2d(vg_ctx *ctx, const double *m1_in, const double *m2_in, const double *phi_in, double *m1_out,
+                     double *m2_out, double *phi_out, int ny, int nx, double dx, double dy, double cfl,
+                     double t_final, int max_steps, double bias, int limiter, int riemann,
+                     const vg_euler_params *p1, const vg_euler_params *p2, int mode, double *dt_log, int dt_log_cap,
+                     vg_run_stats *stats, double *circulation);
+
+/* n_inst independent vg_rgfm_solve_2d calls on grids of one shape (states
+ * concatenated instance by instance), each with its own final time; per-instance
+ * steps, final times and circulations out (each may be NULL). */
+int vg_rgfm_solve_batch_2d(vg_ctx *ctx, int n_inst, const double *m1_in, const double *m2_in, const double *phi_in,
+                           double *m1_out, double *m2_out, double *phi_out, int ny, int nx, double dx, double dy,
+                           double cfl, const double *t_final, int max_steps, double bias, int limiter, int riemann,
+                           const vg_euler_params *p1, const vg_euler_params *p2, int mode, int32_t *steps,
+                           double *t_out, double *circulation, vg_run_stats *stats);
+
 /* ---- Godunov-Peshkov-Romenski (HPR) continuum ---------------------------- */
 
 enum {

@@ -34,6 +34,9 @@ class MHDParams(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("gamma", "p_inf", "ch", "cp")]
 
 
+VG_RIEMANN_HLLC = 0
+VG_RIEMANN_EXACT = 1
+
 VG_HPR_EOS_STIFFENED_GAS = 0
 VG_HPR_EOS_SHOCK_MIE_GRUNEISEN = 1
 VG_HPR_EOS_GODUNOV_ROMENSKI = 2
@@ -94,6 +97,17 @@ SIGNATURES = {
     "vg_mhd_solve_batch": (C.c_int, [vptr, C.c_int, C.POINTER(C.c_int64), dptr, dptr, dptr, C.POINTER(MHDParams),
                                      dptr, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
                                      C.POINTER(C.c_int32), dptr, C.POINTER(RunStats)]),
+    "vg_rgfm_solve_1d": (C.c_int, [vptr, dptr, dptr, dptr, dptr, dptr, dptr, C.c_int, C.c_double, C.c_double,
+                                   C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(EulerParams),
+                                   C.POINTER(EulerParams), C.c_int, dptr, C.c_int, C.POINTER(RunStats)]),
+    "vg_rgfm_solve_2d": (C.c_int, [vptr, dptr, dptr, dptr, dptr, dptr, dptr, C.c_int, C.c_int, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int,
+                                   C.POINTER(EulerParams), C.POINTER(EulerParams), C.c_int, dptr, C.c_int,
+                                   C.POINTER(RunStats), dptr]),
+    "vg_rgfm_solve_batch_2d": (C.c_int, [vptr, C.c_int, dptr, dptr, dptr, dptr, dptr, dptr, C.c_int, C.c_int,
+                                         C.c_double, C.c_double, C.c_double, dptr, C.c_int, C.c_double, C.c_int,
+                                         C.c_int, C.POINTER(EulerParams), C.POINTER(EulerParams), C.c_int,
+                                         C.POINTER(C.c_int32), dptr, dptr, C.POINTER(RunStats)]),
     "vg_hpr_solve_1d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                                   C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(HPRParams), C.c_int, dptr,
                                   C.c_int, C.POINTER(RunStats)]),

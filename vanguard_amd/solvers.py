@@ -509,3 +509,127 @@ class HPRGrid:
         check(self.lib.vg_hpr_state_max_wave_speed(self.handle, bc, C.byref(mp), C.byref(s)),
               "vg_hpr_state_max_wave_speed")
         return s.value
+
+
+# ---- two-material Euler: RGFM ------------------------------------------------------
+
+HLLC = _lib.VG_RIEMANN_HLLC
+EXACT_RIEMANN = _lib.VG_RIEMANN_EXACT
+
+
+@dataclass
+class MultimaterialSystem:
+    """MultimaterialSystem (RGFM/multimaterialsystem.h:8-39): both materials'
+    primitive states over the whole domain and the level set."""
+
+    material1: np.ndarray
+    material2: np.ndarray
+    level_set: np.ndarray
+
+
+@dataclass
+class RGFMResult:
+    system: MultimaterialSystem
+    dt: np.ndarray
+    stats: _lib.RunStats
+    status: int
+    circulation: float | None = None
+
+
+def _params(p):
+    if isinstance(p, EulerMaterialParameters):
+        return p
+    return EulerMaterialParameters(*p) if isinstance(p, (tuple, list)) else EulerMaterialParameters(float(p))
+
+
+def rgfm_solve(system: MultimaterialSystem, dx: float, cfl: float, t_final: float, params1, params2, *,
+               dy: float | None = None, bias: float = 0.0, limiter: int = 0, riemann: int = HLLC, max_steps: int = 0,
+               mode: int = EXACT, dt_cap: int = 200000, ctx: Context | None = None,
+               raise_on_error: bool = True) -> RGFMResult:
+    """RGFMSolver::solveHLLC[2D] / solveExact[2D] through vg_rgfm_solve_1d / _2d."""
+    ctx = ctx or default_context()
+    m1 = np.ascontiguousarray(system.material1, dtype=np.float64)
+    m2 = np.ascontiguousarray(system.material2, dtype=np.float64)
+    phi = np.ascontiguousarray(system.level_set, dtype=np.float64)
+    if m1.shape != m2.shape or m1.shape[:-1] != phi.shape or m1.shape[-1] != 6:
+        raise ValueError("materials must be (..., 6) over the level set's shape")
+    o1, o2, oph = np.empty_like(m1), np.empty_like(m2), np.empty_like(phi)
+    dts = np.empty(dt_cap)
+    st = _lib.RunStats()
+    q1, q2 = _params(params1).c_struct(), _params(params2).c_struct()
+    circ = C.c_double(math.nan)
+    P = _lib.as_dptr
+    if phi.ndim == 1:
+        rc = ctx.lib.vg_rgfm_solve_1d(ctx.handle, P(m1), P(m2), P(phi), P(o1), P(o2), P(oph), phi.shape[0], dx, cfl,
+                                      t_final, max_steps, bias, limiter, riemann, C.byref(q1), C.byref(q2), mode,
+                                      P(dts), dt_cap, C.byref(st))
+    else:
+        rc = ctx.lib.vg_rgfm_solve_2d(ctx.handle, P(m1), P(m2), P(phi), P(o1), P(o2), P(oph), phi.shape[0],
+                                      phi.shape[1], dx, dx if dy is None else dy, cfl, t_final, max_steps, bias,
+                                      limiter, riemann, C.byref(q1), C.byref(q2), mode, P(dts), dt_cap, C.byref(st),
+                                      C.byref(circ))
+    if raise_on_error and rc != _lib.VG_OK:
+        check(rc, "vg_rgfm_solve")
+    return RGFMResult(MultimaterialSystem(o1, o2, oph), dts[:min(st.steps, dt_cap)].copy(), st, rc,
+                      None if phi.ndim == 1 else circ.value)
+
+
+@dataclass
+class RGFMBatchResult:
+    material1: np.ndarray
+    material2: np.ndarray
+    level_set: np.ndarray
+    steps: np.ndarray
+    t: np.ndarray
+    circulation: np.ndarray
+    stats: _lib.RunStats
+    status: int
+
+
+def rgfm_solve_batch_2d(m1, m2, phi, dx, dy, cfl, t_final, params1, params2, *, bias=0.0, limiter=0, riemann=HLLC,
+                        max_steps=0, mode=EXACT, ctx: Context | None = None, raise_on_error=True) -> RGFMBatchResult:
+    """n_inst instances (leading axis) through vg_rgfm_solve_batch_2d."""
+    ctx = ctx or default_context()
+    a1 = np.ascontiguousarray(m1, dtype=np.float64)
+    a2 = np.ascontiguousarray(m2, dtype=np.float64)
+    ph = np.ascontiguousarray(phi, dtype=np.float64)
+    n_inst, ny, nx = ph.shape
+    tf = np.ascontiguousarray(np.broadcast_to(np.asarray(t_final, dtype=np.float64), (n_inst,)))
+    o1, o2, oph = np.empty_like(a1), np.empty_like(a2), np.empty_like(ph)
+    steps = np.zeros(n_inst, dtype=np.int32)
+    tend = np.zeros(n_inst)
+    circ = np.zeros(n_inst)
+    st = _lib.RunStats()
+    q1, q2 = _params(params1).c_struct(), _params(params2).c_struct()
+    P = _lib.as_dptr
+    rc = ctx.lib.vg_rgfm_solve_batch_2d(ctx.handle, n_inst, P(a1), P(a2), P(ph), P(o1), P(o2), P(oph), ny, nx, dx, dy,
+                                        cfl, P(tf), max_steps, bias, limiter, riemann, C.byref(q1), C.byref(q2), mode,
+                                        steps.ctypes.data_as(C.POINTER(C.c_int32)), P(tend), P(circ), C.byref(st))
+    if raise_on_error and rc != _lib.VG_OK:
+        check(rc, "vg_rgfm_solve_batch_2d")
+    return RGFMBatchResult(o1, o2, oph, steps, tend, circ, st, rc)
+
+
+class RGFMSolver:
+    """RGFMSolver (RGFM/rgfmsolver.h:8-41): the whole-solve entry points."""
+
+    @staticmethod
+    def _solve(system, cell_spacing, cfl, final_time, bias, limiter, p1, p2, riemann, **kw):
+        return rgfm_solve(system, cell_spacing, cfl, final_time, p1, p2, bias=bias, limiter=limiter, riemann=riemann,
+                          **kw).system
+
+    @staticmethod
+    def solveHLLC(system, cell_spacing, cfl_coefficient, final_time, bias, slope_limiter, subcycling_iterations,
+                  reinitialisation_frequency, material1_parameters, material2_parameters, **kw):
+        return RGFMSolver._solve(system, cell_spacing, cfl_coefficient, final_time, bias, slope_limiter,
+                                 material1_parameters, material2_parameters, HLLC, **kw)
+
+    solveHLLC2D = solveHLLC
+
+    @staticmethod
+    def solveExact(system, cell_spacing, cfl_coefficient, final_time, bias, slope_limiter, subcycling_iterations,
+                   reinitialisation_frequency, material1_parameters, material2_parameters, **kw):
+        return RGFMSolver._solve(system, cell_spacing, cfl_coefficient, final_time, bias, slope_limiter,
+                                 material1_parameters, material2_parameters, EXACT_RIEMANN, **kw)
+
+    solveExact2D = solveExact
