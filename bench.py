@@ -21,6 +21,10 @@ Secondary workloads (one JSON line each, same contract):
                            256..16384 cells, SLIC van Leer, CFL 0.8, t = 0.1)
     --workload hpr2d       2D HPR viscous flow, 1024^2 periodic, 16 Fourier modes,
                            mu = 1e-3, heat conduction, 1 relaxation sub-cycle per half step
+    --workload rgfm        two-material RGFM shock-bubble batch: 16 instances of
+                           2048 x 1024 cells on 0.325 x 0.089 (dx != dy), helium bubble
+                           radius and shock Mach from the batch recipe, HLLC, t = 170 us
+    --workload copy        a plain 1 GiB device copy (calibrates HBM byte counters)
 """
 
 from __future__ import annotations
@@ -46,6 +50,10 @@ CFL = 0.8
 LIMITER = 0           # superbee (Solvers/slopelimiters.cpp:62-65)
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_SWEEP = 96   # read 6 + write 6 FP64 primitive fields
+# HBM bytes per launch of the 4096^2 sweeps, FETCH_SIZE x 2 + WRITE_SIZE x 1
+# (scales from a plain 1 GiB copy), measured with rocprofv3 --pmc on the FAST
+# kernels; profiles/euler2d_4096_slic_fast_hbm_traffic.md.
+MEASURED_TRAFFIC = {"sweep_x_slic": 1673988183, "sweep_y_slic": 1660324546}
 METRIC = "Mcell-updates/s FP64 SLIC 2D Euler 4096²; %HBM peak; adjoint/forward cost"
 
 
@@ -260,9 +268,83 @@ def bench_hpr2d(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_rgfm(args):
+    """RGFM shock-bubble batch: value = cell-updates (two materials count once
+    per cell) per second over the whole batch."""
+    import torch
+    from vanguard_amd import problems
+    from vanguard_amd import solvers as S
+
+    nx, ny, n_inst = args.rgfm_nx, args.rgfm_ny, args.instances
+    radii, machs = problems.shock_bubble_batch(16, seed=6)
+    radii, machs = radii[:n_inst], machs[:n_inst]
+    ins = [problems.shock_bubble_2d(nx, ny, radius=float(r), mach=float(m)) for r, m in zip(radii, machs)]
+    dx, dy = ins[0][3], ins[0][4]
+    m1 = np.stack([a[0] for a in ins])
+    m2 = np.stack([a[1] for a in ins])
+    phi = np.stack([a[2] for a in ins])
+    tf = np.full(n_inst, args.t_final)
+    ctx = S.default_context()
+    if args.warmup:
+        S.rgfm_solve_batch_2d(m1[:1], m2[:1], phi[:1], dx, dy, 0.8, tf[:1], problems.HELIUM, problems.AIR,
+                              max_steps=args.warmup, ctx=ctx)
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = S.rgfm_solve_batch_2d(m1, m2, phi, dx, dy, 0.8, tf, problems.HELIUM, problems.AIR, ctx=ctx,
+                              max_steps=args.max_steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.set_timing(False)
+    kt = kernel_table(ctx, ["rgfm_sweep_x", "rgfm_sweep_y", "rgfm_fill_x", "rgfm_fill_y", "rgfm_level_set"])
+    updates = float(nx * ny) * float(r.steps.sum())
+    name = max(kt, key=lambda k: kt[k]["total_ms"])
+    per_launch = 96 * nx * ny if name.startswith("rgfm_sweep") else None
+    achieved = per_launch / (kt[name]["avg_ms"] * 1e-3) / 1e9 if per_launch else None
+    line = {
+        "metric": "Mcell-updates/s FP64 2D two-material RGFM (HLLC ghost fill, level set, SLIC per material)",
+        "value": round(updates / elapsed / 1e6, 2), "unit": "Mcell-updates/s", "n_gpus": 1,
+        "steps": int(r.steps.sum()), "warmup": args.warmup, "ms_per_step": round(elapsed * 1e3 / r.steps.sum(), 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"rgfm_shock_bubble_{n_inst}inst_{nx}x{ny}_hllc_t{args.t_final:g}", "grid": [nx, ny],
+                   "dx": dx, "dy": dy, "instances": n_inst, "riemann": "hllc"},
+        "algorithmic_bytes_per_cell_update": 672,
+        "roofline": {"bound": "hbm", "achieved": None if achieved is None else round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": name},
+        "steps_per_instance": [int(x) for x in r.steps], "circulation": [float(x) for x in r.circulation],
+        "kernels": kt, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_copy(args):
+    """Plain device copy of 1 GiB, three times (torch's copy kernel): the known
+    byte count that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    n = 128 * 1024 * 1024
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(json.dumps({"metric": "device copy GB/s (read + write)", "value": round(3 * 2 * n * 8 / el / 1e9, 1),
+                      "unit": "GB/s", "bytes_per_copy": {"read": n * 8, "write": n * 8}, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["euler2d", "mhd_batch", "hpr2d"], default="euler2d")
+    ap.add_argument("--workload", choices=["euler2d", "mhd_batch", "hpr2d", "rgfm", "copy"], default="euler2d")
+    ap.add_argument("--rgfm-nx", type=int, default=2048)
+    ap.add_argument("--rgfm-ny", type=int, default=1024)
+    ap.add_argument("--t-final", type=float, default=170e-6)
+    ap.add_argument("--max-steps", type=int, default=0)
     ap.add_argument("--instances", type=int, default=8192)
     ap.add_argument("--max-cells", type=int, default=16384)
     ap.add_argument("--hpr-n", type=int, default=1024)
@@ -294,6 +376,12 @@ def main():
         return bench_mhd_batch(args)
     if args.workload == "hpr2d":
         return bench_hpr2d(args)
+    if args.workload == "rgfm":
+        if args.instances > 16:
+            args.instances = 16
+        return bench_rgfm(args)
+    if args.workload == "copy":
+        return bench_copy(args)
     log(f"building initial condition {N}x{N} ...")
     w = problems.disc_2d(N)
     ctx = S.default_context()
@@ -327,7 +415,8 @@ def main():
     avg_ms = total_ms / launches
     achieved = BYTES_PER_CELL_SWEEP * cells / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": MEASURED_TRAFFIC.get(name) if head == "fast" else None, "kernel": name,
                 "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": BYTES_PER_CELL_SWEEP * cells}
     step_bytes = 3 * BYTES_PER_CELL_SWEEP * cells
     line = {
