@@ -297,10 +297,10 @@ def bench_rgfm(args):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ctx.set_timing(False)
-    kt = kernel_table(ctx, ["rgfm_sweep_x", "rgfm_sweep_y", "rgfm_fill_x", "rgfm_fill_y", "rgfm_level_set"])
+    kt = kernel_table(ctx, ["rgfm_sweep_x", "sweep_y_slic", "rgfm_fill_x", "rgfm_fill_y", "rgfm_level_set"])
     updates = float(nx * ny) * float(r.steps.sum())
     name = max(kt, key=lambda k: kt[k]["total_ms"])
-    per_launch = 96 * nx * ny if name.startswith("rgfm_sweep") else None
+    per_launch = 96 * nx * ny if "sweep" in name else None
     achieved = per_launch / (kt[name]["avg_ms"] * 1e-3) / 1e9 if per_launch else None
     line = {
         "metric": "Mcell-updates/s FP64 2D two-material RGFM (HLLC ghost fill, level set, SLIC per material)",

@@ -78,6 +78,18 @@ int vg_ctx_set_timing(vg_ctx *ctx, int enabled);
 int vg_ctx_kernel_time(vg_ctx *ctx, const char *kernel, int64_t *launches, double *total_ms);
 int vg_ctx_reset_timing(vg_ctx *ctx);
 
+/* Step replay as HIP graphs (default on): runs capture a chunk of time steps
+ * (all launches of each step, no memory operations) once per grid and replay
+ * it until the clock stops.  Used only while per-kernel timing is off.  No
+ * reference counterpart (the reference has no device). */
+int vg_ctx_set_graphs(vg_ctx *ctx, int enabled);
+/* Tile-shape autotuner (default on): the first run on a grid shape times the
+ * candidate y-sweep shapes (threads per block x rows per strip) on that grid
+ * and keeps the fastest for the context.  vg_ctx_tuned_shape returns the chosen
+ * variant for a key "physics/order/mode/ny/nx" (-1 if not tuned yet). */
+int vg_ctx_set_autotune(vg_ctx *ctx, int enabled);
+int vg_ctx_tuned_shape(vg_ctx *ctx, const char *key, int *variant);
+
 /* ---- Euler: whole-solve drop-ins ---------------------------------------- */
 
 /* SecondOrderSolver::solve (Solvers/secondordersolver.h:55-56,

@@ -63,10 +63,12 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
 
 // Tile shapes of the grid sweeps for a physics P: x-sweep block size and rows
 // per block, y-sweep block size and rows per column strip.  The x tile keeps
-// 3 * NF * bsx doubles in LDS, so wide states use narrower tiles.
+// 3 * NF * bsx doubles in LDS, so wide states use narrower tiles.  `tune`: the
+// y-sweep shape is chosen per grid by the autotuner among fixed candidates.
 template <class P>
 struct Tiles {
     static constexpr int bsx = 256, rx = 4, bsy = 256, hy = 64;
+    static constexpr bool tune = true;
 };
 
 // Per-grid time-step clock kept in device memory so that a whole run can be

@@ -205,7 +205,7 @@ int vg_euler_state_max_wave_speed(vg_euler_state *st, int bc, const vg_euler_par
     VG_CHECK(st != nullptr && mp != nullptr && smax != nullptr, "null argument");
     int rc = vg::bind(st->g.ctx);
     if (rc) return rc;
-    rc = st->g.refresh_max(bc, to_params(mp));
+    rc = st->g.refresh_max(bc, to_params(mp), st->g.fused_count(2));
     if (rc) return rc;
     vg::Clock h;
     VG_HIP(hipMemcpyAsync(&h, st->g.clk, sizeof(h), hipMemcpyDeviceToHost, st->g.ctx->stream));

@@ -30,7 +30,86 @@ struct SourceStage {
     {
         return VG_OK;
     }
+    // Blocks of one launch (the CFL partials it writes when it is the last of a step).
+    template <class GS>
+    static long long blocks(const GS &)
+    {
+        return 0;
+    }
 };
+
+// ---- y-sweep shapes and their autotuner --------------------------------------
+
+// Candidate (threads per block, rows per strip) of the y sweep: variant 0 is
+// Tiles<P>'s shape; 1-3 trade strip length for more workgroups on small grids.
+constexpr int kYVariants = 4;
+
+template <class P, int ORDER, int MODE, int BS, int H>
+int launch_y_shape(vg_ctx *ctx, const SweepArgs<P> &a)
+{
+    dim3 grid((a.nx + BS - 1) / BS, (a.ny + H - 1) / H);
+    return ctx->timed(ORDER == 2 ? "sweep_y_slic" : "sweep_y_force",
+                      [&] { sweep_y_kernel<P, ORDER, MODE, BS, H><<<grid, BS, 0, ctx->stream>>>(a); });
+}
+
+template <class P, int ORDER, int MODE>
+int launch_y(vg_ctx *ctx, int variant, const SweepArgs<P> &a)
+{
+    if constexpr (Tiles<P>::tune) {
+        switch (variant) {
+        case 1: return launch_y_shape<P, ORDER, MODE, 256, 16>(ctx, a);
+        case 2: return launch_y_shape<P, ORDER, MODE, 128, 32>(ctx, a);
+        case 3: return launch_y_shape<P, ORDER, MODE, 64, 64>(ctx, a);
+        default: break;
+        }
+    }
+    return launch_y_shape<P, ORDER, MODE, Tiles<P>::bsy, Tiles<P>::hy>(ctx, a);
+}
+
+inline std::string tune_key(const char *phys, int order, int mode, int ny, int nx)
+{
+    return std::string(phys) + "/" + std::to_string(order) + "/" + std::to_string(mode) + "/" + std::to_string(ny) +
+           "/" + std::to_string(nx);
+}
+
+// Picks the y-sweep shape for a grid: the context's table when the shape was
+// tuned before; otherwise each candidate runs once to warm up and three times
+// timed on the caller's grid (a.in -> a.out, with a.clk a scratch clock holding
+// a small positive dt, no CFL fusion), and the fastest is recorded.
+template <class P, int ORDER, int MODE>
+int tune_y(vg_ctx *ctx, const std::string &key, SweepArgs<P> a, int &variant)
+{
+    auto it = ctx->tuned.find(key);
+    if (it != ctx->tuned.end()) {
+        variant = it->second;
+        return VG_OK;
+    }
+    variant = 0;
+    if (!Tiles<P>::tune || !ctx->autotune || ctx->timing) return VG_OK;
+    a.fuse_max = 0;
+    hipEvent_t e0, e1;
+    VG_HIP(hipEventCreate(&e0));
+    VG_HIP(hipEventCreate(&e1));
+    float best = 0.0f;
+    for (int v = 0; v < kYVariants; v++) {
+        int rc = launch_y<P, ORDER, MODE>(ctx, v, a);
+        VG_HIP(hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < 3 && !rc; r++) rc = launch_y<P, ORDER, MODE>(ctx, v, a);
+        VG_HIP(hipEventRecord(e1, ctx->stream));
+        VG_HIP(hipEventSynchronize(e1));
+        if (rc) return launch_status("autotune");
+        float ms = 0.0f;
+        VG_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (v == 0 || ms < best) {
+            best = ms;
+            variant = v;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    ctx->tuned[key] = variant;
+    return VG_OK;
+}
 
 template <class P>
 struct GridState {
@@ -52,6 +131,11 @@ struct GridState {
     int subcycles = 0;             // source sub-cycles per half step (SourceStage)
     typename P::Params last_params{};
     int last_bc = -1;
+    int y_variant = 0;             // y-sweep shape (autotuned per grid shape)
+    Clock *tune_clk = nullptr;     // scratch clock for tuning launches (lazy)
+    hipGraphExec_t graph = nullptr;  // captured chunk of steps
+    std::string graph_key;           // arguments the graph was captured with
+    static constexpr int kGraphSteps = 8;   // steps per captured chunk (even: buffers return to their start)
 
     int create(vg_ctx *c, int d, int y, int x)
     {
@@ -82,6 +166,8 @@ struct GridState {
         hipFree(dt_dev);
         hipFree(scan);
         hipFree(partials);
+        hipFree(tune_clk);
+        if (graph) hipGraphExecDestroy(graph);
     }
 
     int ensure_staging()
@@ -129,10 +215,22 @@ struct GridState {
         return VG_OK;
     }
 
-    // Wave-speed maximum of the current grid into clk->smax (and the corner term).
-    int refresh_max(int bc, const typename P::Params &m)
+    // Blocks of the launch that ends a step and folds in the next step's CFL
+    // maximum: the last source sub-cycle, else the last x sweep.
+    int fused_count(int order) const
     {
-        const int blocks = blocks_for(cells) < 2048 ? blocks_for(cells) : 2048;
+        if (SourceStage<P>::kActive && subcycles > 0) return (int)SourceStage<P>::blocks(*this);
+        const int G = (order == 2) ? 2 : 1;
+        return (int)(((nx + (kBSX - 2 * G) - 1) / (kBSX - 2 * G)) * (long long)((ny + kRX - 1) / kRX));
+    }
+
+    // Wave-speed maximum of the current grid into clk->smax (and the corner
+    // term), written as `blocks` per-block partials: the same count the step's
+    // fused producer writes, so that every clock_begin of a run -- and of a
+    // replayed graph -- reduces the same number of slots.
+    int refresh_max(int bc, const typename P::Params &m, int blocks)
+    {
+        if (blocks < 1) blocks = 1;
         wave_max_kernel<P><<<blocks, 256, 0, ctx->stream>>>(buf[cur], plane, cells, dim, m, partials);
         int rc = launch_status("wave_max");
         if (rc) return rc;
@@ -181,11 +279,9 @@ struct GridState {
                 sweep_x_kernel<P, ORDER, MODE, kBSX, kRX><<<grid, kBSX, 0, ctx->stream>>>(a);
             });
         }
-        dim3 grid((nx + kBSY - 1) / kBSY, (ny + kHY - 1) / kHY);
-        if (fuse) n_partials = (int)(grid.x * grid.y);
-        return ctx->timed(ORDER == 2 ? "sweep_y_slic" : "sweep_y_force", [&] {
-            sweep_y_kernel<P, ORDER, MODE, kBSY, kHY><<<grid, kBSY, 0, ctx->stream>>>(a);
-        });
+        // The y sweep never ends a step, so it never writes the CFL partials.
+        a.fuse_max = 0;
+        return launch_y<P, ORDER, MODE>(ctx, y_variant, a);
     }
 
     template <int ORDER, int MODE>
@@ -228,8 +324,33 @@ struct GridState {
                  const typename P::Params &m, double *dt_log, int dt_cap, vg_run_stats *stats)
     {
         auto w0 = std::chrono::steady_clock::now();
-        if (!max_valid || last_bc != bc || std::memcmp(&last_params, &m, sizeof(m)) != 0) {
-            int rc = refresh_max(bc, m);
+        const int fused = fused_count(ORDER);
+        if (!max_valid || last_bc != bc || std::memcmp(&last_params, &m, sizeof(m)) != 0 || n_partials != fused) {
+            int rc = refresh_max(bc, m, fused);
+            if (rc) return rc;
+        }
+        if (dim == 2) {
+            if (!tune_clk) VG_HIP(hipMalloc(&tune_clk, sizeof(Clock)));
+            Clock tc{};
+            tc.dt = 1e-3 * dx;
+            VG_HIP(hipMemcpyAsync(tune_clk, &tc, sizeof(Clock), hipMemcpyHostToDevice, ctx->stream));
+            SweepArgs<P> ta;
+            ta.in = buf[cur];
+            ta.out = buf[1 - cur];
+            ta.plane = plane;
+            ta.ny = ny;
+            ta.nx = nx;
+            ta.bc = bc;
+            ta.limiter = limiter;
+            ta.dx = dx;
+            ta.bias = bias;
+            ta.dt_scale = 1.0;
+            ta.dim = dim;
+            ta.fuse_max = 0;
+            ta.m = m;
+            ta.clk = tune_clk;
+            ta.partials = nullptr;
+            int rc = tune_y<P, ORDER, MODE>(ctx, tune_key(P::name, ORDER, MODE, ny, nx), ta, y_variant);
             if (rc) return rc;
         }
         Clock h;
@@ -259,15 +380,63 @@ struct GridState {
         VG_HIP(hipEventRecord(e0, ctx->stream));
         const int start_cur = cur;
         const int chunk = (max_steps > 0 && max_steps <= 4096) ? max_steps : 256;
-        for (;;) {
-            for (int s = 0; s < chunk; s++) {
-                int rc = enqueue_step<ORDER, MODE>(bc, dx, bias, limiter, m, dlog, dt_cap);
-                if (rc) return rc;
+        if (ctx->graphs && !ctx->timing) {
+            // Capture kGraphSteps steps once (all kernels; no memory operations),
+            // replay until the clock stops.  kGraphSteps is even, so every replay
+            // starts from the same buffer; steps past the stop do nothing.
+            std::string key(reinterpret_cast<const char *>(&m), sizeof(m));
+            const double num[2] = {dx, bias};
+            key.append(reinterpret_cast<const char *>(num), sizeof(num));
+            key += "|" + std::to_string(ORDER) + "|" + std::to_string(MODE) + "|" + std::to_string(bc) + "|" +
+                   std::to_string(limiter) + "|" + std::to_string((unsigned long long)(uintptr_t)dlog) + "|" +
+                   std::to_string(dt_cap) + "|" + std::to_string(subcycles) + "|" + std::to_string(y_variant) + "|" +
+                   std::to_string(cur) + "|" + std::to_string(n_partials);
+            if (!graph || key != graph_key) {
+                if (graph) {
+                    hipGraphExecDestroy(graph);
+                    graph = nullptr;
+                }
+                hipGraph_t g = nullptr;
+                VG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                int rc = VG_OK;
+                for (int s = 0; s < kGraphSteps && !rc; s++) rc = enqueue_step<ORDER, MODE>(bc, dx, bias, limiter, m, dlog, dt_cap);
+                hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+                if (rc) {
+                    if (g) hipGraphDestroy(g);
+                    return rc;
+                }
+                if (ec != hipSuccess) {
+                    set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+                    return VG_ERR_HIP;
+                }
+                ec = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+                hipGraphDestroy(g);
+                if (ec != hipSuccess) {
+                    graph = nullptr;
+                    set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ec));
+                    return VG_ERR_HIP;
+                }
+                graph_key = key;
             }
-            VG_HIP(hipMemcpyAsync(&h, clk, sizeof(Clock), hipMemcpyDeviceToHost, ctx->stream));
-            VG_HIP(hipStreamSynchronize(ctx->stream));
-            if (h.done) break;
-            if (max_steps > 0 && h.steps_this_run >= max_steps) break;
+            const int reps = std::max(1, chunk / kGraphSteps);
+            for (;;) {
+                for (int r = 0; r < reps; r++) VG_HIP(hipGraphLaunch(graph, ctx->stream));
+                VG_HIP(hipMemcpyAsync(&h, clk, sizeof(Clock), hipMemcpyDeviceToHost, ctx->stream));
+                VG_HIP(hipStreamSynchronize(ctx->stream));
+                if (h.done) break;
+                if (max_steps > 0 && h.steps_this_run >= max_steps) break;
+            }
+        } else {
+            for (;;) {
+                for (int s = 0; s < chunk; s++) {
+                    int rc = enqueue_step<ORDER, MODE>(bc, dx, bias, limiter, m, dlog, dt_cap);
+                    if (rc) return rc;
+                }
+                VG_HIP(hipMemcpyAsync(&h, clk, sizeof(Clock), hipMemcpyDeviceToHost, ctx->stream));
+                VG_HIP(hipStreamSynchronize(ctx->stream));
+                if (h.done) break;
+                if (max_steps > 0 && h.steps_this_run >= max_steps) break;
+            }
         }
         clock_finish_kernel<<<1, 1, 0, ctx->stream>>>(clk);
         int rc = launch_status("clock_finish");

@@ -15,6 +15,11 @@ template <>
 struct SourceStage<HPRPhys> {
     static constexpr bool kActive = true;
     template <class GS>
+    static long long blocks(const GS &g)
+    {
+        return (g.cells + kForceBS - 1) / kForceBS;
+    }
+    template <class GS>
     static int launch(GS &g, const double *in, double *out, int fuse, int bc, double dx, double bias, int limiter,
                       const HPRPhys::Params &m)
     {
@@ -253,7 +258,7 @@ int vg_hpr_state_max_wave_speed(vg_hpr_state *st, int bc, const vg_hpr_params *m
     if (rc) return rc;
     rc = vg::bind(st->g.ctx);
     if (rc) return rc;
-    rc = st->g.refresh_max(bc, to_params(mp));
+    rc = st->g.refresh_max(bc, to_params(mp), st->g.fused_count(2));
     if (rc) return rc;
     vg::Clock h;
     VG_HIP(hipMemcpyAsync(&h, st->g.clk, sizeof(h), hipMemcpyDeviceToHost, st->g.ctx->stream));

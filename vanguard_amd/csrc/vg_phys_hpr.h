@@ -435,6 +435,7 @@ struct HPRPhys {
 template <>
 struct Tiles<HPRPhys> {
     static constexpr int bsx = 128, rx = 4, bsy = 128, hy = 32;
+    static constexpr bool tune = false;
 };
 
 }  // namespace vg

@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "vg_grid.h"
 #include "vg_rgfm.h"
 #include "vg_runtime.h"
 
@@ -47,9 +48,13 @@ struct RGFMGrid {
     double *rec = nullptr, *staging = nullptr, *circ = nullptr, *dt_dev = nullptr;
     int dt_dev_cap = 0;
     int cur = 0, pcur = 0;
-    Clock *clk = nullptr;
+    Clock *clk = nullptr, *tune_clk = nullptr;
     unsigned long long *partials = nullptr;
     int max_blocks = 0, circ_blocks = 0;
+    int y_variant = 0;                  // autotuned y-sweep shape
+    hipGraphExec_t graph = nullptr;     // captured chunk of steps
+    std::string graph_key;
+    static constexpr int kGraphSteps = 8;   // even: buffers return to their start
 
     int create(vg_ctx *c, int d, int y, int x)
     {
@@ -87,6 +92,8 @@ struct RGFMGrid {
         hipFree(partials);
         hipFree(circ);
         hipFree(dt_dev);
+        hipFree(tune_clk);
+        if (graph) hipGraphExecDestroy(graph);
     }
 
     static int blocks_for(long long n) { return (int)std::min<long long>((n + 255) / 256, 4096); }
@@ -164,10 +171,7 @@ struct RGFMGrid {
                 sweep_x_kernel<EulerPhys, 2, MODE, BSX, RX><<<grid, BSX, 0, ctx->stream>>>(a);
             });
         }
-        dim3 grid((nx + BSY - 1) / BSY, (ny + HY - 1) / HY);
-        return ctx->timed("rgfm_sweep_y", [&] {
-            sweep_y_kernel<EulerPhys, 2, MODE, BSY, HY><<<grid, BSY, 0, ctx->stream>>>(a);
-        });
+        return launch_y<EulerPhys, 2, MODE>(ctx, y_variant, a);
     }
 
     template <int MODE>
@@ -251,22 +255,98 @@ struct RGFMGrid {
             dt_dev_cap = dt_cap;
         }
         double *dlog = (dt_log && dt_cap > 0) ? dt_dev : nullptr;
+        if (dim == 2) {
+            // y-sweep shape for this grid (shared with Euler grids of the same shape).
+            if (!tune_clk) VG_HIP(hipMalloc(&tune_clk, sizeof(Clock)));
+            Clock tc{};
+            tc.dt = 1e-3 * std::min(dx, dy);
+            VG_HIP(hipMemcpyAsync(tune_clk, &tc, sizeof(Clock), hipMemcpyHostToDevice, s));
+            SweepArgs<EulerPhys> ta;
+            ta.in = m1[cur];
+            ta.out = m1[1 - cur];
+            ta.plane = plane;
+            ta.ny = ny;
+            ta.nx = nx;
+            ta.bc = VG_BC_TRANSMISSIVE;
+            ta.limiter = limiter;
+            ta.dx = dy;
+            ta.bias = bias;
+            ta.dt_scale = 1.0;
+            ta.dim = 2;
+            ta.fuse_max = 0;
+            ta.m = p1;
+            ta.clk = tune_clk;
+            ta.partials = nullptr;
+            int rc = (mode == VG_MODE_FAST)
+                         ? tune_y<EulerPhys, 2, MODE_FAST>(ctx, tune_key("euler", 2, mode, ny, nx), ta, y_variant)
+                         : tune_y<EulerPhys, 2, MODE_EXACT>(ctx, tune_key("euler", 2, mode, ny, nx), ta, y_variant);
+            if (rc) return rc;
+        }
         hipEvent_t e0, e1;
         VG_HIP(hipEventCreate(&e0));
         VG_HIP(hipEventCreate(&e1));
         VG_HIP(hipEventRecord(e0, s));
         const int start_cur = cur, start_pcur = pcur;
         const int chunk = (max_steps > 0 && max_steps <= 1024) ? max_steps : 128;
-        for (;;) {
-            for (int k = 0; k < chunk; k++) {
-                int rc = (mode == VG_MODE_FAST)
-                             ? enqueue_step<MODE_FAST>(riemann, dx, dy, bias, limiter, p1, p2, dlog, dt_cap)
-                             : enqueue_step<MODE_EXACT>(riemann, dx, dy, bias, limiter, p1, p2, dlog, dt_cap);
-                if (rc) return rc;
+        auto enqueue = [&]() {
+            return (mode == VG_MODE_FAST) ? enqueue_step<MODE_FAST>(riemann, dx, dy, bias, limiter, p1, p2, dlog, dt_cap)
+                                          : enqueue_step<MODE_EXACT>(riemann, dx, dy, bias, limiter, p1, p2, dlog, dt_cap);
+        };
+        if (ctx->graphs && !ctx->timing) {
+            // Capture kGraphSteps steps once per argument set and replay them;
+            // every step moves both materials and the level set once, so an even
+            // count returns all buffers to their start.
+            std::string key(reinterpret_cast<const char *>(&p1), sizeof(p1));
+            key.append(reinterpret_cast<const char *>(&p2), sizeof(p2));
+            const double num[3] = {dx, dy, bias};
+            key.append(reinterpret_cast<const char *>(num), sizeof(num));
+            key += "|" + std::to_string(riemann) + "|" + std::to_string(mode) + "|" + std::to_string(limiter) + "|" +
+                   std::to_string((unsigned long long)(uintptr_t)dlog) + "|" + std::to_string(dt_cap) + "|" +
+                   std::to_string(y_variant) + "|" + std::to_string(cur) + "|" + std::to_string(pcur);
+            if (!graph || key != graph_key) {
+                if (graph) {
+                    hipGraphExecDestroy(graph);
+                    graph = nullptr;
+                }
+                hipGraph_t g = nullptr;
+                VG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                int rc = VG_OK;
+                for (int k = 0; k < kGraphSteps && !rc; k++) rc = enqueue();
+                hipError_t ec = hipStreamEndCapture(s, &g);
+                if (rc) {
+                    if (g) hipGraphDestroy(g);
+                    return rc;
+                }
+                if (ec != hipSuccess) {
+                    set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+                    return VG_ERR_HIP;
+                }
+                ec = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+                hipGraphDestroy(g);
+                if (ec != hipSuccess) {
+                    graph = nullptr;
+                    set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ec));
+                    return VG_ERR_HIP;
+                }
+                graph_key = key;
             }
-            VG_HIP(hipMemcpyAsync(&h, clk, sizeof(Clock), hipMemcpyDeviceToHost, s));
-            VG_HIP(hipStreamSynchronize(s));
-            if (h.done) break;
+            const int reps = std::max(1, chunk / kGraphSteps);
+            for (;;) {
+                for (int r = 0; r < reps; r++) VG_HIP(hipGraphLaunch(graph, s));
+                VG_HIP(hipMemcpyAsync(&h, clk, sizeof(Clock), hipMemcpyDeviceToHost, s));
+                VG_HIP(hipStreamSynchronize(s));
+                if (h.done) break;
+            }
+        } else {
+            for (;;) {
+                for (int k = 0; k < chunk; k++) {
+                    int rc = enqueue();
+                    if (rc) return rc;
+                }
+                VG_HIP(hipMemcpyAsync(&h, clk, sizeof(Clock), hipMemcpyDeviceToHost, s));
+                VG_HIP(hipStreamSynchronize(s));
+                if (h.done) break;
+            }
         }
         clock_finish_kernel<<<1, 1, 0, s>>>(clk);
         VG_HIP(hipEventRecord(e1, s));

@@ -43,6 +43,13 @@ struct vg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool timing = false;
+    // Replay chunks of time steps as captured HIP graphs (only while per-kernel
+    // timing is off: timing brackets each launch with events).
+    bool graphs = true;
+    // Time the candidate tile shapes on first use of a grid shape and keep the
+    // fastest (key: physics/order/mode/ny/nx -> variant index).
+    bool autotune = true;
+    std::map<std::string, int> tuned;
     std::map<std::string, vg::KernelTimer> timers;
     // Event pairs recorded around launches and not yet resolved.
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;

@@ -118,6 +118,28 @@ int vg_ctx_kernel_time(vg_ctx *ctx, const char *kernel, int64_t *launches, doubl
     return VG_OK;
 }
 
+int vg_ctx_set_graphs(vg_ctx *ctx, int enabled)
+{
+    VG_CHECK(ctx != nullptr, "vg_ctx_set_graphs: null context");
+    ctx->graphs = enabled != 0;
+    return VG_OK;
+}
+
+int vg_ctx_set_autotune(vg_ctx *ctx, int enabled)
+{
+    VG_CHECK(ctx != nullptr, "vg_ctx_set_autotune: null context");
+    ctx->autotune = enabled != 0;
+    return VG_OK;
+}
+
+int vg_ctx_tuned_shape(vg_ctx *ctx, const char *key, int *variant)
+{
+    VG_CHECK(ctx != nullptr && key != nullptr && variant != nullptr, "vg_ctx_tuned_shape: null argument");
+    auto it = ctx->tuned.find(key);
+    *variant = (it == ctx->tuned.end()) ? -1 : it->second;
+    return VG_OK;
+}
+
 int vg_ctx_reset_timing(vg_ctx *ctx)
 {
     VG_CHECK(ctx != nullptr, "vg_ctx_reset_timing: null context");

@@ -55,6 +55,20 @@ class Context:
     def reset_timing(self):
         check(self.lib.vg_ctx_reset_timing(self.handle), "vg_ctx_reset_timing")
 
+    def set_graphs(self, on: bool):
+        """Replay chunks of time steps as captured HIP graphs (default on)."""
+        check(self.lib.vg_ctx_set_graphs(self.handle, int(on)), "vg_ctx_set_graphs")
+
+    def set_autotune(self, on: bool):
+        """Time the candidate y-sweep shapes on first use of a grid shape (default on)."""
+        check(self.lib.vg_ctx_set_autotune(self.handle, int(on)), "vg_ctx_set_autotune")
+
+    def tuned_shape(self, physics: str, order: int, mode: int, ny: int, nx: int) -> int:
+        v = C.c_int()
+        key = f"{physics}/{order}/{mode}/{ny}/{nx}".encode()
+        check(self.lib.vg_ctx_tuned_shape(self.handle, key, C.byref(v)), "vg_ctx_tuned_shape")
+        return v.value
+
 
 _default_ctx = None
 
