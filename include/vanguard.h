@@ -153,6 +153,37 @@ int vg_euler_solve_batch(vg_ctx *ctx, int n_inst, const int64_t *offsets, const 
                          int max_steps, double bias, int limiter, int order, int mode, int32_t *steps,
                          double *t_out, vg_run_stats *stats);
 
+/* ---- Euler: adjoint (reverse-mode derivative) ---------------------------- */
+
+/* Reverse-mode derivative of the SLIC solve (SecondOrderSolver::solve /
+ * solve2D, EXACT arithmetic) with the time steps held at the values the forward
+ * run chose.  No reference counterpart: the reference has no derivatives.
+ * Runs the forward (prim_out receives its final state, dt_log its time steps),
+ * keeping a checkpoint every `checkpoint_every` steps (0: ceil(sqrt(steps))),
+ * then the adjoint backwards with recomputation per segment.  seed: the
+ * derivative of a scalar loss with respect to the final primitive state
+ * (array of structures, 6 doubles per cell); grad receives its derivative with
+ * respect to the initial primitive state.  The slope limiter is differentiated
+ * piecewise: at a kink, the derivative of the branch the forward comparison
+ * selected; the 1e-5 slope clamp counts as a constant. */
+int vg_euler_adjoint_1d(vg_ctx *ctx, const double *prim_in, double *prim_out, const double *seed, double *grad, int n,
+                        double dx, double cfl, double t_final, int max_steps, double bias, int limiter, int bc,
+                        const vg_euler_params *mp, int checkpoint_every, double *dt_log, int dt_log_cap,
+                        vg_run_stats *stats);
+int vg_euler_adjoint_2d(vg_ctx *ctx, const double *prim_in, double *prim_out, const double *seed, double *grad, int ny,
+                        int nx, double dx, double cfl, double t_final, int max_steps, double bias, int limiter, int bc,
+                        const vg_euler_params *mp, int checkpoint_every, double *dt_log, int dt_log_cap,
+                        vg_run_stats *stats);
+
+/* The SLIC solve with prescribed time steps dts[0..n_steps) (EXACT arithmetic):
+ * the map the adjoint differentiates, for frozen-dt finite-difference checks. */
+int vg_euler_solve_fixed_dt_1d(vg_ctx *ctx, const double *prim_in, double *prim_out, int n, double dx,
+                               const double *dts, int n_steps, double bias, int limiter, int bc,
+                               const vg_euler_params *mp);
+int vg_euler_solve_fixed_dt_2d(vg_ctx *ctx, const double *prim_in, double *prim_out, int ny, int nx, double dx,
+                               const double *dts, int n_steps, double bias, int limiter, int bc,
+                               const vg_euler_params *mp);
+
 /* ---- ideal MHD with GLM cleaning ----------------------------------------- */
 
 /* MHDMaterialParameters (MHD/mhdmaterialparameters.h) field order: gamma, p_inf,

@@ -50,6 +50,17 @@ void orc_euler_sweep(int order, int dim, int dir, int ny, int nx, double dx, dou
 /* Maximum wave speed over the cells (no ghost corners). */
 double orc_euler_max_wave_speed(int dim, int ny, int nx, const orc_euler_params *mp, const double *prim);
 
+/* SLIC solve with prescribed time steps dts[0..n_steps), and (when tangent_in
+ * is not NULL) its forward-mode derivative along tangent_in, by dual numbers
+ * (euler_dual.cpp).  Limiter branches follow the primal comparisons; the 1e-5
+ * slope clamp is a constant.  Same layout as orc_euler_solve. */
+void orc_euler_fixed_dt_jvp(int dim, int ny, int nx, double dx, const double *dts, int n_steps, double bias,
+                            int limiter, int bc, const orc_euler_params *mp, const double *prim_in,
+                            const double *tangent_in, double *prim_out, double *tangent_out);
+/* Hash of every branch decision (slope clamps, limiter cases, min selections)
+ * of the last orc_euler_fixed_dt_jvp call: equal hashes, same smooth pieces. */
+uint64_t orc_euler_branch_hash(void);
+
 /* MHDMaterialParameters (MHD/mhdmaterialparameters.h) field order: adiabatic
  * index, stiffening parameter, GLM hyperbolic wave speed c_h, parabolic
  * damping c_p.  MHDMaterialParameters(gamma) sets the last three to 0. */

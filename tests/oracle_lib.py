@@ -73,6 +73,11 @@ def load():
     lib.orc_hllc_units.argtypes = [C.c_int, P, P, dptr, dptr]
     lib.orc_exact_units.restype = None
     lib.orc_exact_units.argtypes = [C.c_int, C.c_double, P, P, dptr, dptr]
+    lib.orc_euler_fixed_dt_jvp.restype = None
+    lib.orc_euler_fixed_dt_jvp.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, dptr, C.c_int, C.c_double, C.c_int,
+                                           C.c_int, C.POINTER(OrcEulerParams), dptr, dptr, dptr, dptr]
+    lib.orc_euler_branch_hash.restype = C.c_uint64
+    lib.orc_euler_branch_hash.argtypes = []
     lib.orc_rgfm_circulation.restype = C.c_double
     lib.orc_rgfm_circulation.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, dptr, dptr]
     _lib = lib
@@ -221,6 +226,33 @@ def euler_solve(prim, *, order, dim, dx, cfl, tfinal, max_steps, bias, limiter, 
     n = lib.orc_euler_solve(order, dim, ny, nx, dx, cfl, tfinal, max_steps, bias, limiter, bc, C.byref(mp), _p(w),
                             _p(out), _p(dts), dt_cap)
     return out, dts[:min(n, dt_cap)].copy()
+
+
+def euler_fixed_dt(prim, dts, *, dim, dx, bias, limiter, params, bc=0, tangent=None):
+    """SLIC solve with prescribed time steps; with `tangent`, also the
+    forward-mode derivative of the final state along it (dual numbers)."""
+    lib = load()
+    w = np.ascontiguousarray(prim, dtype=np.float64)
+    d = np.ascontiguousarray(dts, dtype=np.float64)
+    out = np.empty_like(w)
+    ny, nx = (1, w.shape[0]) if dim == 1 else w.shape[:2]
+    mp = OrcEulerParams(*params)
+    if tangent is None:
+        lib.orc_euler_fixed_dt_jvp(dim, ny, nx, dx, _p(d), d.size, bias, limiter, bc, C.byref(mp), _p(w), None,
+                                   _p(out), None)
+        return out
+    tv = np.ascontiguousarray(tangent, dtype=np.float64)
+    assert tv.shape == w.shape
+    tout = np.empty_like(w)
+    lib.orc_euler_fixed_dt_jvp(dim, ny, nx, dx, _p(d), d.size, bias, limiter, bc, C.byref(mp), _p(w), _p(tv),
+                               _p(out), _p(tout))
+    return out, tout
+
+
+def euler_branch_hash(prim, dts, **kw):
+    """Hash of the branch decisions the fixed-dt solve takes from `prim`."""
+    euler_fixed_dt(prim, dts, **kw)
+    return int(load().orc_euler_branch_hash())
 
 
 def euler_units(states, *, bias, limiter, dx, ts, params):

@@ -59,7 +59,9 @@ def test_device_state_graph_replays_between_host_work(fresh_ctx):
 
 
 def test_hpr_graph_replay_matches_eager(fresh_ctx):
-    p = S.HPRMaterialParameters(is_thermal=True, dynamic_viscosity_coefficient=1e-3)
+    # mu = 1e-2: at 32^2 the stiffer mu = 1e-3 relaxation is unstable for the
+    # reference's RK4 sub-cycle too (the oracle blows up the same way).
+    p = S.HPRMaterialParameters(is_thermal=True, dynamic_viscosity_coefficient=1e-2)
     p.configure_strain_dissipation_time()
     p.configure_thermal_impulse_relaxation_time(1e-2)
     w = problems.hpr_fourier_2d(32, seed=4)

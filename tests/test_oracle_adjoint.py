@@ -1,0 +1,215 @@
+"""Pins the dual-number restatement of the SLIC solve (oracle/euler_dual.cpp),
+the independent check of the hand-written GPU adjoint.
+
+1. Its primal (plain doubles) reproduces the oracle's solve bit for bit, given
+   the solve's own time steps: it is the same discrete map.
+2. Its tangent matches central finite differences of that map on smooth data,
+   where no limiter branch or slope clamp changes inside the stencil of the
+   difference.
+3. What the mathematics fixes: a uniform state stays uniform, so the tangent
+   along a uniform direction is that direction; with periodic boundaries mass
+   and momentum are conserved, so their linearisations are too; the tangent is
+   linear in the direction.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+IDEAL = [1.4, 0.0, 25.0, 0.0, 0.25, 250.0]
+REACTIVE = [1.4, 0.0, 2.5, 10.0, 0.25, 250.0]
+
+
+def smooth_1d(n, *, periodic=True):
+    x = (np.arange(n) + 0.5) / n
+    w = np.zeros((n, 6))
+    w[:, 0] = 1.0 + 0.2 * np.exp(-((x - 0.5) / 0.15) ** 2)
+    w[:, 1] = 0.1 * np.sin(2 * np.pi * x)
+    w[:, 2] = 0.05 * np.cos(2 * np.pi * x)
+    w[:, 4] = 1.0 + 0.1 * np.cos(2 * np.pi * x + 0.3)
+    w[:, 5] = 0.8 + 0.1 * np.sin(2 * np.pi * x + 1.0)
+    return w
+
+
+def smooth_2d(ny, nx):
+    y = ((np.arange(ny) + 0.5) / ny)[:, None]
+    x = ((np.arange(nx) + 0.5) / nx)[None, :]
+    w = np.zeros((ny, nx, 6))
+    w[..., 0] = 1.0 + 0.2 * np.exp(-((x - 0.45) ** 2 + (y - 0.55) ** 2) / 0.05)
+    w[..., 1] = 0.1 * np.sin(2 * np.pi * y)
+    w[..., 2] = -0.08 * np.sin(2 * np.pi * x)
+    w[..., 3] = 0.02 * np.cos(2 * np.pi * (x + y))
+    w[..., 4] = 1.0 + 0.1 * np.cos(2 * np.pi * x) * np.cos(2 * np.pi * y)
+    w[..., 5] = 0.9 + 0.05 * np.sin(2 * np.pi * (x - y))
+    return w
+
+
+def sod_1d(n):
+    w = np.zeros((n, 6))
+    w[:, 0] = np.where(np.arange(n) < n // 2, 1.0, 0.125)
+    w[:, 4] = np.where(np.arange(n) < n // 2, 1.0, 0.1)
+    w[:, 5] = 1.0
+    return w
+
+
+def sod_2d(ny, nx):
+    w = np.zeros((ny, nx, 6))
+    yy, xx = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    inside = (xx - nx / 2) ** 2 + (yy - ny / 2) ** 2 < (0.3 * nx) ** 2
+    w[..., 0] = np.where(inside, 1.0, 0.125)
+    w[..., 4] = np.where(inside, 1.0, 0.1)
+    w[..., 5] = 1.0
+    return w
+
+
+@pytest.mark.parametrize("case", [
+    dict(dim=1, w=lambda: sod_1d(64), limiter=0, bias=0.0, bc=0, params=IDEAL),
+    dict(dim=1, w=lambda: sod_1d(64), limiter=1, bias=0.3, bc=0, params=REACTIVE),
+    dict(dim=1, w=lambda: smooth_1d(48), limiter=2, bias=-0.2, bc=1, params=IDEAL),
+    dict(dim=2, w=lambda: sod_2d(12, 16), limiter=0, bias=0.0, bc=0, params=IDEAL),
+    dict(dim=2, w=lambda: smooth_2d(16, 12), limiter=1, bias=0.0, bc=1, params=REACTIVE),
+])
+def test_dual_primal_is_the_oracle_solve(case):
+    w = case["w"]()
+    kw = dict(dim=case["dim"], dx=1.0 / w.shape[-2], bias=case["bias"], limiter=case["limiter"],
+              params=case["params"], bc=case["bc"])
+    ref, dts = O.euler_solve(w, order=2, cfl=0.8, tfinal=math.inf, max_steps=12, **kw)
+    assert dts.size == 12
+    out = O.euler_fixed_dt(w, dts, **kw)
+    O.assert_bitwise(out, ref, "fixed-dt primal")
+    out2, _ = O.euler_fixed_dt(w, dts, tangent=np.zeros_like(w), **kw)
+    O.assert_bitwise(out2, ref, "dual primal")
+
+
+def _dts(w, dim, bc, params, steps):
+    return O.euler_solve(w, order=2, dim=dim, dx=1.0 / w.shape[-2], cfl=0.6, tfinal=math.inf, max_steps=steps,
+                         bias=0.0, limiter=0, params=params, bc=bc)[1]
+
+
+def local_direction(rng, shape, dim, width=3):
+    """A random direction supported on a width^dim patch of cells."""
+    v = np.zeros(shape)
+    if dim == 1:
+        c = rng.integers(0, shape[0] - width)
+        v[c:c + width] = rng.standard_normal((width, 6))
+    else:
+        cy, cx = rng.integers(0, shape[0] - width), rng.integers(0, shape[1] - width)
+        v[cy:cy + width, cx:cx + width] = rng.standard_normal((width, width, 6))
+    return v
+
+
+@pytest.mark.parametrize("dim,bc,limiter", [(1, 1, 0), (1, 0, 1), (1, 1, 2), (2, 1, 0), (2, 0, 2)])
+def test_dual_tangent_matches_central_differences(dim, bc, limiter):
+    w = smooth_1d(40) if dim == 1 else smooth_2d(12, 10)
+    dts = _dts(w, dim, bc, IDEAL, 6)
+    kw = dict(dim=dim, dx=1.0 / w.shape[-2], bias=0.0, limiter=limiter, params=IDEAL, bc=bc)
+    rng = np.random.default_rng(11 + dim + 3 * limiter)
+    v = rng.standard_normal(w.shape)
+    _, jv = O.euler_fixed_dt(w, dts, tangent=v, **kw)
+    # h small enough that no slope crosses the 1e-5 clamp or a limiter kink
+    # (constant fields sit exactly at the clamp: |dl| = 0 < 1e-5), large enough
+    # that rounding (~1e-16 / h) stays far below the tolerance; the branch
+    # hashes certify that w - hv, w and w + hv take the same branches
+    h = 1e-8
+    h0 = O.euler_branch_hash(w, dts, **kw)
+    assert O.euler_branch_hash(w + h * v, dts, **kw) == h0 == O.euler_branch_hash(w - h * v, dts, **kw)
+    fd = (O.euler_fixed_dt(w + h * v, dts, **kw) - O.euler_fixed_dt(w - h * v, dts, **kw)) / (2 * h)
+    err = np.abs(fd - jv).max() / np.abs(jv).max()
+    assert err < 1e-6, err
+    # a planted error the comparison must see: the tangent of a different step count
+    _, jv5 = O.euler_fixed_dt(w, dts[:5], tangent=v, **kw)
+    assert np.abs(fd - jv5).max() / np.abs(jv).max() > 1e-3
+
+
+def test_branch_hash_sees_a_crossing():
+    """At h = 1e-4 the constant fields' zero slopes leave the 1e-5 clamp: the
+    hash must change, and the difference quotient is then no derivative."""
+    w = smooth_1d(40)
+    dts = _dts(w, 1, 1, IDEAL, 6)
+    kw = dict(dim=1, dx=1.0 / 40, bias=0.0, limiter=0, params=IDEAL, bc=1)
+    v = np.random.default_rng(15).standard_normal(w.shape)
+    h = 1e-4
+    assert O.euler_branch_hash(w + h * v, dts, **kw) != O.euler_branch_hash(w, dts, **kw)
+    _, jv = O.euler_fixed_dt(w, dts, tangent=v, **kw)
+    fd = (O.euler_fixed_dt(w + h * v, dts, **kw) - O.euler_fixed_dt(w - h * v, dts, **kw)) / (2 * h)
+    assert np.abs(fd - jv).max() / np.abs(jv).max() > 1e-3
+
+
+@pytest.mark.parametrize("make,dim,bc,steps", [(lambda: smooth_1d(200), 1, 1, 15), (lambda: sod_1d(100), 1, 0, 20),
+                                                (lambda: smooth_2d(40, 32), 2, 1, 15),
+                                                (lambda: sod_2d(24, 20), 2, 0, 8)])
+def test_dual_tangent_local_directions(make, dim, bc, steps):
+    """Longer runs on finer grids: local directions whose perturbed runs take
+    the same branches (equal hashes) match central differences of <seed, .>."""
+    w = make()
+    dts = _dts(w, dim, bc, IDEAL, steps)
+    kw = dict(dim=dim, dx=1.0 / w.shape[-2], bias=0.0, limiter=0, params=IDEAL, bc=bc)
+    rng = np.random.default_rng(steps + dim)
+    seed = rng.standard_normal(w.shape)
+    h, h0, used = 1e-8, O.euler_branch_hash(w, dts, **kw), 0
+    for _ in range(6):
+        v = local_direction(rng, w.shape, dim)
+        if not (O.euler_branch_hash(w + h * v, dts, **kw) == h0 == O.euler_branch_hash(w - h * v, dts, **kw)):
+            continue
+        _, jv = O.euler_fixed_dt(w, dts, tangent=v, **kw)
+        fd = np.sum(seed * (O.euler_fixed_dt(w + h * v, dts, **kw) - O.euler_fixed_dt(w - h * v, dts, **kw))) / (2 * h)
+        assert abs(fd - np.sum(seed * jv)) <= 1e-6 * np.sum(np.abs(seed * jv))
+        used += 1
+    assert used >= 3
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_uniform_direction_at_uniform_state(dim):
+    shape = (24, 6) if dim == 1 else (8, 10, 6)
+    w = np.zeros(shape)
+    w[..., 0] = 1.3
+    w[..., 1] = 0.2
+    w[..., 2] = -0.1
+    w[..., 4] = 0.9
+    w[..., 5] = 0.7
+    v = np.zeros(shape)
+    v[...] = np.array([0.3, -0.7, 0.4, 0.2, 1.1, -0.5])
+    dts = np.full(5, 0.01)
+    for bc in (0, 1):
+        out, jv = O.euler_fixed_dt(w, dts, dim=dim, dx=0.05, bias=0.0, limiter=0, params=REACTIVE, bc=bc, tangent=v)
+        np.testing.assert_allclose(out, w, rtol=1e-13, atol=1e-13)
+        np.testing.assert_allclose(jv, v, rtol=1e-10, atol=1e-10)
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_periodic_conservation_of_the_tangent(dim):
+    """Sum(rho) and Sum(rho u) are invariants of the periodic scheme; their
+    derivatives along any direction are therefore preserved too."""
+    w = sod_1d(50) if dim == 1 else sod_2d(10, 12)
+    w[..., 1] += 0.1
+    dts = _dts(w, dim, 1, IDEAL, 8)
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal(w.shape)
+    out, jv = O.euler_fixed_dt(w, dts, dim=dim, dx=1.0 / w.shape[-2], bias=0.0, limiter=0, params=IDEAL, bc=1,
+                               tangent=v)
+
+    def lin(state, tan):
+        rho, u = state[..., 0], state[..., 1]
+        return np.array([tan[..., 0].sum(), (tan[..., 0] * u + rho * tan[..., 1]).sum()])
+
+    a, b = lin(w, v), lin(out, jv)
+    np.testing.assert_allclose(b, a, rtol=1e-10, atol=1e-10 * np.abs(a).max())
+    # and a tangent with a term dropped breaks it
+    bad = jv.copy()
+    bad[..., 1] = 0.0
+    assert np.abs(lin(out, bad) - a).max() > 1e-3
+
+
+def test_tangent_is_linear_in_the_direction():
+    w = sod_1d(40)
+    dts = _dts(w, 1, 0, IDEAL, 10)
+    rng = np.random.default_rng(9)
+    v1, v2 = rng.standard_normal(w.shape), rng.standard_normal(w.shape)
+    kw = dict(dim=1, dx=1.0 / 40, bias=0.0, limiter=0, params=IDEAL, bc=0)
+    j1 = O.euler_fixed_dt(w, dts, tangent=v1, **kw)[1]
+    j2 = O.euler_fixed_dt(w, dts, tangent=v2, **kw)[1]
+    j12 = O.euler_fixed_dt(w, dts, tangent=2.0 * v1 - 3.0 * v2, **kw)[1]
+    np.testing.assert_allclose(j12, 2.0 * j1 - 3.0 * j2, rtol=1e-9, atol=1e-9 * np.abs(j12).max())

@@ -79,6 +79,16 @@ SIGNATURES = {
     "vg_euler_solve_2d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                     C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(EulerParams),
                                     C.c_int, dptr, C.c_int, C.POINTER(RunStats)]),
+    "vg_euler_adjoint_1d": (C.c_int, [vptr, dptr, dptr, dptr, dptr, C.c_int, C.c_double, C.c_double, C.c_double,
+                                      C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(EulerParams), C.c_int, dptr,
+                                      C.c_int, C.POINTER(RunStats)]),
+    "vg_euler_adjoint_2d": (C.c_int, [vptr, dptr, dptr, dptr, dptr, C.c_int, C.c_int, C.c_double, C.c_double,
+                                      C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(EulerParams),
+                                      C.c_int, dptr, C.c_int, C.POINTER(RunStats)]),
+    "vg_euler_solve_fixed_dt_1d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_double, dptr, C.c_int, C.c_double,
+                                             C.c_int, C.c_int, C.POINTER(EulerParams)]),
+    "vg_euler_solve_fixed_dt_2d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_int, C.c_double, dptr, C.c_int,
+                                             C.c_double, C.c_int, C.c_int, C.POINTER(EulerParams)]),
     "vg_euler_state_create": (C.c_int, [vptr, C.c_int, C.c_int, C.c_int, C.POINTER(vptr)]),
     "vg_euler_state_destroy": (None, [vptr]),
     "vg_euler_state_load": (C.c_int, [vptr, vptr, C.c_int]),

@@ -1,0 +1,363 @@
+// Adjoint C ABI for the Euler SLIC solve, and the fixed-time-step forward it
+// differentiates.
+//
+// vg_euler_adjoint_{1d,2d}: forward solve (the same kernels and arithmetic as
+// vg_euler_solve_*, EXACT mode) that records every dt and keeps a checkpoint of
+// the grid every K steps (K = ceil(sqrt(steps)) unless given); then, segment by
+// segment from the end, the segment is recomputed from its checkpoint with the
+// recorded dts, storing the input of every sweep, and the adjoint sweeps run
+// backwards through it: 2D steps X(dt/2) Y(dt) X(dt/2) reverse as
+// X*(dt/2) Y*(dt) X*(dt/2).  Memory: (steps / K) checkpoints + (3K + 1) sweep
+// inputs (2D) of one grid each.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "vg_adjoint.h"
+#include "vg_grid.h"
+#include "vg_phys_euler.h"
+
+namespace {
+
+using vg::EulerPhys;
+using EGrid = vg::GridState<EulerPhys>;
+
+EulerPhys::Params to_params(const vg_euler_params *p)
+{
+    EulerPhys::Params m;
+    m.gamma = p->gamma;
+    m.p_inf = p->p_inf;
+    m.cv = p->cv;
+    m.e_form = p->e_form;
+    m.t_ign = p->t_ign;
+    m.k_rate = p->k_rate;
+    m.gm1 = p->gamma - 1;
+    m.gpinf = p->gamma * p->p_inf;
+    m.inv_gm1 = 1.0 / (p->gamma - 1);
+    return m;
+}
+
+int check_common(vg_ctx *ctx, int dim, int ny, int nx, double dx, int limiter, int bc, const vg_euler_params *mp)
+{
+    (void)limiter;
+    VG_CHECK(ctx != nullptr, "null context");
+    VG_CHECK(mp != nullptr, "null material parameters");
+    VG_CHECK(bc == VG_BC_TRANSMISSIVE || bc == VG_BC_PERIODIC, "unknown boundary condition");
+    VG_CHECK(nx >= 2 && (dim == 1 ? ny == 1 : ny >= 2), "grid needs at least 2 cells per direction");
+    VG_CHECK(dx > 0.0 && std::isfinite(dx), "cell spacing must be positive");
+    return VG_OK;
+}
+
+// One forward sweep with the time step taken from the clock (dt x scale).
+template <int MODE>
+int fwd_sweep(vg_ctx *ctx, int dir, const double *in, double *out, long long plane, int ny, int nx, int dim, int bc,
+              double dx, double bias, int limiter, double scale, const EulerPhys::Params &m, vg::Clock *clk,
+              int y_variant)
+{
+    vg::SweepArgs<EulerPhys> a;
+    a.in = in;
+    a.out = out;
+    a.plane = plane;
+    a.ny = ny;
+    a.nx = nx;
+    a.bc = bc;
+    a.limiter = limiter;
+    a.dx = dx;
+    a.bias = bias;
+    a.dt_scale = scale;
+    a.dim = dim;
+    a.fuse_max = 0;
+    a.m = m;
+    a.clk = clk;
+    a.partials = nullptr;
+    if (dir == 0) {
+        constexpr int BS = vg::Tiles<EulerPhys>::bsx, RX = vg::Tiles<EulerPhys>::rx;
+        dim3 grid((nx + (BS - 4) - 1) / (BS - 4), (ny + RX - 1) / RX);
+        vg::sweep_x_kernel<EulerPhys, 2, MODE, BS, RX><<<grid, BS, 0, ctx->stream>>>(a);
+        return vg::launch_status("sweep_x");
+    }
+    return vg::launch_y<EulerPhys, 2, MODE>(ctx, y_variant, a);
+}
+
+// One adjoint sweep: wbar_in = (d sweep / d w)^T wbar_out at input w.
+int adj_sweep(vg_ctx *ctx, int dir, const double *w, const double *wbar_out, double *wbar_in, double *ghost,
+              long long plane, int ny, int nx, int bc, double h, double bias, int limiter, double ts,
+              const EulerPhys::Params &m)
+{
+    vg::AdjointArgs a;
+    a.w = w;
+    a.wbar_out = wbar_out;
+    a.wbar_in = wbar_in;
+    a.ghost = ghost;
+    a.plane = plane;
+    a.ny = ny;
+    a.nx = nx;
+    a.bc = bc;
+    a.limiter = limiter;
+    a.dx = h;
+    a.bias = bias;
+    a.ts = ts;
+    a.m = m;
+    const int n = (dir == 0) ? nx : ny, lines = (dir == 0) ? ny : nx;
+    dim3 grid((n + 4 + vg::kAdjOut - 1) / vg::kAdjOut, lines);
+    return ctx->timed(dir == 0 ? "adjoint_sweep_x" : "adjoint_sweep_y", [&] {
+        if (dir == 0) {
+            vg::euler_adjoint_sweep_kernel<0><<<grid, vg::kAdjBS, 0, ctx->stream>>>(a);
+            vg::euler_adjoint_fold_kernel<0><<<(lines + 255) / 256, 256, 0, ctx->stream>>>(wbar_in, ghost, plane, ny,
+                                                                                            nx, bc);
+        } else {
+            vg::euler_adjoint_sweep_kernel<1><<<grid, vg::kAdjBS, 0, ctx->stream>>>(a);
+            vg::euler_adjoint_fold_kernel<1><<<(lines + 255) / 256, 256, 0, ctx->stream>>>(wbar_in, ghost, plane, ny,
+                                                                                            nx, bc);
+        }
+    });
+}
+
+struct DeviceBuffers {
+    std::vector<double *> ptrs;
+    ~DeviceBuffers()
+    {
+        for (double *p : ptrs) hipFree(p);
+    }
+    int alloc(double **p, size_t bytes)
+    {
+        hipError_t e = hipMalloc(p, bytes);
+        if (e != hipSuccess) {
+            vg::set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+            return VG_ERR_HIP;
+        }
+        ptrs.push_back(*p);
+        return VG_OK;
+    }
+};
+
+// Fixed-dt forward over steps [s0, s1) from `in`: with `store` (3K+1 grid slots
+// for 2D, K+1 for 1D) every sweep input is kept; otherwise ping-pong.
+int forward_fixed(vg_ctx *ctx, EGrid &g, const double *dts_dev, int s0, int s1, double dx, double bias, int limiter,
+                  int bc, const EulerPhys::Params &m, const std::vector<double *> &slots)
+{
+    vg::Clock h{};
+    h.it = s0;
+    h.t_final = INFINITY;
+    h.max_steps = s1;
+    VG_HIP(hipMemcpyAsync(g.clk, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
+    const int per = (g.dim == 2) ? 3 : 1;
+    for (int s = s0; s < s1; s++) {
+        vg::clock_fixed_kernel<<<1, 1, 0, ctx->stream>>>(g.clk, dts_dev, s1);
+        const int base = (s - s0) * per;
+        int rc;
+        if (g.dim == 1) {
+            rc = fwd_sweep<vg::MODE_EXACT>(ctx, 0, slots[base], slots[base + 1], g.plane, g.ny, g.nx, 1, bc, dx, bias,
+                                           limiter, 1.0, m, g.clk, 0);
+        } else {
+            rc = fwd_sweep<vg::MODE_EXACT>(ctx, 0, slots[base], slots[base + 1], g.plane, g.ny, g.nx, 2, bc, dx, bias,
+                                           limiter, 0.5, m, g.clk, g.y_variant);
+            if (!rc)
+                rc = fwd_sweep<vg::MODE_EXACT>(ctx, 1, slots[base + 1], slots[base + 2], g.plane, g.ny, g.nx, 2, bc, dx,
+                                               bias, limiter, 1.0, m, g.clk, g.y_variant);
+            if (!rc)
+                rc = fwd_sweep<vg::MODE_EXACT>(ctx, 0, slots[base + 2], slots[base + 3], g.plane, g.ny, g.nx, 2, bc, dx,
+                                               bias, limiter, 0.5, m, g.clk, g.y_variant);
+        }
+        if (rc) return rc;
+    }
+    return VG_OK;
+}
+
+int adjoint(vg_ctx *ctx, int dim, const double *prim_in, double *prim_out, const double *seed, double *grad, int ny,
+            int nx, double dx, double cfl, double t_final, int max_steps, double bias, int limiter, int bc,
+            const vg_euler_params *mp, int checkpoint_every, double *dt_log, int dt_log_cap, vg_run_stats *stats)
+{
+    vg::clear_error();
+    int rc = check_common(ctx, dim, ny, nx, dx, limiter, bc, mp);
+    if (rc) return rc;
+    VG_CHECK(prim_in && prim_out && seed && grad, "null state pointer");
+    VG_CHECK(cfl > 0.0 && std::isfinite(cfl), "CFL coefficient must be positive");
+    VG_CHECK(max_steps >= 0 && (max_steps > 0 || std::isfinite(t_final)), "an infinite final time needs a step cap");
+    VG_CHECK(checkpoint_every >= 0, "checkpoint_every must be >= 0");
+    rc = vg::bind(ctx);
+    if (rc) return rc;
+    auto w0 = std::chrono::steady_clock::now();
+    const EulerPhys::Params m = to_params(mp);
+    hipStream_t s = ctx->stream;
+    EGrid g;
+    rc = g.create(ctx, dim, ny, nx);
+    if (!rc) rc = g.load(prim_in, 0);
+    if (rc) {
+        g.destroy();
+        return rc;
+    }
+    const size_t gbytes = sizeof(double) * 6 * g.plane;
+    DeviceBuffers mem;
+    // ---- forward with checkpoints ---------------------------------------------------
+    const int cap = max_steps > 0 ? max_steps : 1 << 22;
+    std::vector<double> dts;
+    std::vector<double *> ckpt;
+    int K = checkpoint_every;
+    if (K == 0) K = (max_steps > 0) ? std::max(1, (int)std::ceil(std::sqrt((double)max_steps))) : 16;
+    vg_run_stats fst{};
+    double *slot = nullptr;
+    for (;;) {
+        if ((int)dts.size() >= cap) break;
+        if ((rc = mem.alloc(&slot, gbytes))) break;
+        VG_HIP(hipMemcpyAsync(slot, g.buf[g.cur], gbytes, hipMemcpyDeviceToDevice, s));
+        ckpt.push_back(slot);
+        std::vector<double> part(K);
+        vg_run_stats st{};
+        const int want = std::min(K, cap - (int)dts.size());
+        rc = g.run(2, VG_MODE_EXACT, dx, cfl, t_final, want, bias, limiter, bc, m, part.data(), K, &st);
+        if (rc) break;
+        dts.insert(dts.end(), part.begin(), part.begin() + st.steps);
+        fst = st;
+        if (st.steps < want) break;   // reached t_final
+    }
+    if (rc) {
+        g.destroy();
+        return rc;
+    }
+    const int steps = (int)dts.size();
+    rc = g.store(prim_out, 0);
+    // ---- reverse, segment by segment ----------------------------------------------------
+    double *dts_dev = nullptr, *wb[2] = {nullptr, nullptr}, *ghost = nullptr;
+    const int per = (dim == 2) ? 3 : 1;
+    std::vector<double *> slots(per * K + 1, nullptr);
+    if (!rc) rc = mem.alloc(&dts_dev, sizeof(double) * std::max(steps, 1));
+    if (!rc) rc = mem.alloc(&wb[0], gbytes);
+    if (!rc) rc = mem.alloc(&wb[1], gbytes);
+    if (!rc) rc = mem.alloc(&ghost, sizeof(double) * 4 * 6 * (size_t)std::max(nx, ny));
+    for (int k = 1; k < per * K + 1 && !rc; k++) rc = mem.alloc(&slots[k], gbytes);
+    if (rc) {
+        g.destroy();
+        return rc;
+    }
+    if (steps > 0) VG_HIP(hipMemcpyAsync(dts_dev, dts.data(), sizeof(double) * steps, hipMemcpyHostToDevice, s));
+    // seed: array of structures -> planes
+    VG_HIP(hipMemcpyAsync(g.staging, seed, sizeof(double) * 6 * g.cells, hipMemcpyHostToDevice, s));
+    vg::aos_to_soa_kernel<6><<<EGrid::blocks_for(g.cells), 256, 0, s>>>(g.staging, wb[0], g.plane, g.cells);
+    int cw = 0;
+    const double dy = dx;
+    for (int seg = (int)ckpt.size() - 1; seg >= 0 && !rc; seg--) {
+        const int s0 = seg * K, s1 = std::min(steps, s0 + K);
+        if (s0 >= s1) continue;
+        slots[0] = ckpt[seg];
+        rc = forward_fixed(ctx, g, dts_dev, s0, s1, dx, bias, limiter, bc, m, slots);
+        for (int st = s1 - 1; st >= s0 && !rc; st--) {
+            const int base = (st - s0) * per;
+            const double dt = dts[st];
+            if (dim == 1) {
+                rc = adj_sweep(ctx, 0, slots[base], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias, limiter,
+                               dt, m);
+                cw = 1 - cw;
+            } else {
+                rc = adj_sweep(ctx, 0, slots[base + 2], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias,
+                               limiter, dt * 0.5, m);
+                cw = 1 - cw;
+                if (!rc)
+                    rc = adj_sweep(ctx, 1, slots[base + 1], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dy, bias,
+                                   limiter, dt, m);
+                cw = 1 - cw;
+                if (!rc)
+                    rc = adj_sweep(ctx, 0, slots[base], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias,
+                                   limiter, dt * 0.5, m);
+                cw = 1 - cw;
+            }
+        }
+    }
+    if (!rc) {
+        vg::soa_to_aos_kernel<6><<<EGrid::blocks_for(g.cells), 256, 0, s>>>(wb[cw], g.staging, g.plane, g.cells);
+        rc = vg::launch_status("adjoint store");
+    }
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(grad, g.staging, sizeof(double) * 6 * g.cells, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            vg::set_error(std::string("adjoint copy: ") + hipGetErrorString(e));
+            rc = VG_ERR_HIP;
+        }
+    }
+    if (!rc && ctx->timing) rc = ctx->resolve_timing();
+    if (dt_log) std::memcpy(dt_log, dts.data(), sizeof(double) * std::min(steps, std::max(dt_log_cap, 0)));
+    if (stats) {
+        *stats = fst;
+        stats->steps = steps;
+        stats->total_steps = steps;
+        stats->dt_min = steps ? *std::min_element(dts.begin(), dts.end()) : 0.0;
+        stats->dt_max = steps ? *std::max_element(dts.begin(), dts.end()) : 0.0;
+        stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+        stats->nonfinite_cells = 0;
+        stats->first_nonfinite = -1;
+    }
+    g.destroy();
+    return rc;
+}
+
+int solve_fixed(vg_ctx *ctx, int dim, const double *prim_in, double *prim_out, int ny, int nx, double dx,
+                const double *dts, int n_steps, double bias, int limiter, int bc, const vg_euler_params *mp)
+{
+    vg::clear_error();
+    int rc = check_common(ctx, dim, ny, nx, dx, limiter, bc, mp);
+    if (rc) return rc;
+    VG_CHECK(prim_in && prim_out && (dts || n_steps == 0) && n_steps >= 0, "null argument");
+    rc = vg::bind(ctx);
+    if (rc) return rc;
+    const EulerPhys::Params m = to_params(mp);
+    EGrid g;
+    rc = g.create(ctx, dim, ny, nx);
+    if (!rc) rc = g.load(prim_in, 0);
+    DeviceBuffers mem;
+    double *dts_dev = nullptr;
+    if (!rc) rc = mem.alloc(&dts_dev, sizeof(double) * std::max(n_steps, 1));
+    if (!rc && n_steps > 0)
+        VG_HIP(hipMemcpyAsync(dts_dev, dts, sizeof(double) * n_steps, hipMemcpyHostToDevice, ctx->stream));
+    for (int s = 0; s < n_steps && !rc; s++) {
+        // ping-pong through the grid's two buffers, one step at a time
+        std::vector<double *> slots;
+        if (dim == 1) slots = {g.buf[g.cur], g.buf[1 - g.cur]};
+        else slots = {g.buf[g.cur], g.buf[1 - g.cur], g.buf[g.cur], g.buf[1 - g.cur]};
+        rc = forward_fixed(ctx, g, dts_dev, s, s + 1, dx, bias, limiter, bc, m, slots);
+        g.cur = 1 - g.cur;
+    }
+    if (!rc) rc = g.store(prim_out, 0);
+    g.destroy();
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vg_euler_adjoint_1d(vg_ctx *ctx, const double *prim_in, double *prim_out, const double *seed, double *grad, int n,
+                        double dx, double cfl, double t_final, int max_steps, double bias, int limiter, int bc,
+                        const vg_euler_params *mp, int checkpoint_every, double *dt_log, int dt_log_cap,
+                        vg_run_stats *stats)
+{
+    return adjoint(ctx, 1, prim_in, prim_out, seed, grad, 1, n, dx, cfl, t_final, max_steps, bias, limiter, bc, mp,
+                   checkpoint_every, dt_log, dt_log_cap, stats);
+}
+
+int vg_euler_adjoint_2d(vg_ctx *ctx, const double *prim_in, double *prim_out, const double *seed, double *grad, int ny,
+                        int nx, double dx, double cfl, double t_final, int max_steps, double bias, int limiter, int bc,
+                        const vg_euler_params *mp, int checkpoint_every, double *dt_log, int dt_log_cap,
+                        vg_run_stats *stats)
+{
+    return adjoint(ctx, 2, prim_in, prim_out, seed, grad, ny, nx, dx, cfl, t_final, max_steps, bias, limiter, bc, mp,
+                   checkpoint_every, dt_log, dt_log_cap, stats);
+}
+
+int vg_euler_solve_fixed_dt_1d(vg_ctx *ctx, const double *prim_in, double *prim_out, int n, double dx,
+                               const double *dts, int n_steps, double bias, int limiter, int bc,
+                               const vg_euler_params *mp)
+{
+    return solve_fixed(ctx, 1, prim_in, prim_out, 1, n, dx, dts, n_steps, bias, limiter, bc, mp);
+}
+
+int vg_euler_solve_fixed_dt_2d(vg_ctx *ctx, const double *prim_in, double *prim_out, int ny, int nx, double dx,
+                               const double *dts, int n_steps, double bias, int limiter, int bc,
+                               const vg_euler_params *mp)
+{
+    return solve_fixed(ctx, 2, prim_in, prim_out, ny, nx, dx, dts, n_steps, bias, limiter, bc, mp);
+}
+
+}  // extern "C"

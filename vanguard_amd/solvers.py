@@ -133,6 +133,76 @@ def euler_solve(prim: np.ndarray, dim: int, dx: float, cfl: float, t_final: floa
     return SolveResult(out, dts[:min(st.steps, dt_cap)].copy(), st, rc)
 
 
+@dataclass
+class AdjointResult:
+    cells: np.ndarray      # final primitive state of the forward solve
+    grad: np.ndarray       # d loss / d initial primitive state
+    dt: np.ndarray         # the time steps the forward chose (held fixed)
+    stats: _lib.RunStats
+
+
+def euler_adjoint(prim: np.ndarray, seed: np.ndarray, dim: int, dx: float, cfl: float, t_final: float, *,
+                  bias: float = 0.0, limiter: int = 0, params: EulerMaterialParameters | None = None,
+                  max_steps: int = 0, bc: int = TRANSMISSIVE, checkpoint_every: int = 0, dt_cap: int = 100000,
+                  ctx: Context | None = None) -> AdjointResult:
+    """Reverse-mode derivative of the SLIC solve (vg_euler_adjoint_1d / _2d).
+
+    `seed` is d loss / d final primitive state (same shape as `prim`); the
+    result's `grad` is d loss / d initial primitive state, with every time step
+    held at the value the forward run chose."""
+    ctx = ctx or default_context()
+    params = params or EulerMaterialParameters(1.4)
+    w = np.ascontiguousarray(prim, dtype=np.float64)
+    sd = np.ascontiguousarray(seed, dtype=np.float64)
+    if sd.shape != w.shape:
+        raise ValueError("seed must have the shape of the state")
+    out = np.empty_like(w)
+    grad = np.empty_like(w)
+    dts = np.empty(dt_cap)
+    st = _lib.RunStats()
+    mp = params.c_struct()
+    if dim == 1:
+        if w.ndim != 2 or w.shape[1] != 6:
+            raise ValueError("1D state must be (cells, 6)")
+        rc = ctx.lib.vg_euler_adjoint_1d(ctx.handle, _lib.as_dptr(w), _lib.as_dptr(out), _lib.as_dptr(sd),
+                                         _lib.as_dptr(grad), w.shape[0], dx, cfl, t_final, max_steps, bias, limiter,
+                                         bc, C.byref(mp), checkpoint_every, _lib.as_dptr(dts), dt_cap, C.byref(st))
+    else:
+        if w.ndim != 3 or w.shape[2] != 6:
+            raise ValueError("2D state must be (rows, columns, 6)")
+        rc = ctx.lib.vg_euler_adjoint_2d(ctx.handle, _lib.as_dptr(w), _lib.as_dptr(out), _lib.as_dptr(sd),
+                                         _lib.as_dptr(grad), w.shape[0], w.shape[1], dx, cfl, t_final, max_steps,
+                                         bias, limiter, bc, C.byref(mp), checkpoint_every, _lib.as_dptr(dts), dt_cap,
+                                         C.byref(st))
+    check(rc, "vg_euler_adjoint")
+    return AdjointResult(out, grad, dts[:min(st.steps, dt_cap)].copy(), st)
+
+
+def euler_solve_fixed_dt(prim: np.ndarray, dim: int, dx: float, dts, *, bias: float = 0.0, limiter: int = 0,
+                         params: EulerMaterialParameters | None = None, bc: int = TRANSMISSIVE,
+                         ctx: Context | None = None) -> np.ndarray:
+    """SLIC solve with prescribed time steps (vg_euler_solve_fixed_dt_1d / _2d)."""
+    ctx = ctx or default_context()
+    params = params or EulerMaterialParameters(1.4)
+    w = np.ascontiguousarray(prim, dtype=np.float64)
+    d = np.ascontiguousarray(dts, dtype=np.float64).ravel()
+    out = np.empty_like(w)
+    mp = params.c_struct()
+    if dim == 1:
+        if w.ndim != 2 or w.shape[1] != 6:
+            raise ValueError("1D state must be (cells, 6)")
+        rc = ctx.lib.vg_euler_solve_fixed_dt_1d(ctx.handle, _lib.as_dptr(w), _lib.as_dptr(out), w.shape[0], dx,
+                                                _lib.as_dptr(d), d.size, bias, limiter, bc, C.byref(mp))
+    else:
+        if w.ndim != 3 or w.shape[2] != 6:
+            raise ValueError("2D state must be (rows, columns, 6)")
+        rc = ctx.lib.vg_euler_solve_fixed_dt_2d(ctx.handle, _lib.as_dptr(w), _lib.as_dptr(out), w.shape[0],
+                                                w.shape[1], dx, _lib.as_dptr(d), d.size, bias, limiter, bc,
+                                                C.byref(mp))
+    check(rc, "vg_euler_solve_fixed_dt")
+    return out
+
+
 class SecondOrderSolver:
     """SLIC (MUSCL-Hancock + FORCE), Solvers/secondordersolver.h:7-72."""
 
