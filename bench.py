@@ -24,6 +24,9 @@ Secondary workloads (one JSON line each, same contract):
     --workload rgfm        two-material RGFM shock-bubble batch: 16 instances of
                            2048 x 1024 cells on 0.325 x 0.089 (dx != dy), helium bubble
                            radius and shock Mach from the batch recipe, HLLC, t = 170 us
+    --workload adjoint     reverse-mode derivative of the same 2D Euler solve (EXACT
+                           arithmetic, time steps frozen, sqrt(N) checkpoints): cost of
+                           forward + adjoint over the plain EXACT forward of the same run
     --workload copy        a plain 1 GiB device copy (calibrates HBM byte counters)
 """
 
@@ -320,6 +323,68 @@ def bench_rgfm(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_adjoint(args):
+    """Forward + adjoint of the 2D Euler SLIC solve (vg_euler_adjoint_2d: the CFL
+    forward with checkpoints, then per segment the fixed-dt recomputation and the
+    adjoint sweeps) over the plain EXACT forward of the same steps.  value = that
+    cost ratio (lower is better); the forward is timed device-resident, the
+    adjoint call includes its host copies of state, seed and gradient."""
+    import torch
+    from vanguard_amd import problems
+    from vanguard_amd import solvers as S
+
+    n, steps = args.adj_n, args.steps
+    w = problems.disc_2d(n)
+    dx = LENGTH / n
+    ctx = S.default_context()
+    seed = np.random.default_rng(1).standard_normal(w.shape)
+    # warm-up: compile paths, tune the y sweep, size the graph caches
+    if args.warmup:
+        S.euler_adjoint(w[:64, :64].copy(), seed[:64, :64].copy(), 2, dx, CFL, math.inf, limiter=LIMITER,
+                        max_steps=args.warmup, ctx=ctx)
+        g0 = S.EulerGrid(2, n, n, ctx=ctx)
+        g0.load(w)
+        g0.run(dx, CFL, max_steps=args.warmup, limiter=LIMITER, mode=S.EXACT)
+        g0.close()
+    g = S.EulerGrid(2, n, n, ctx=ctx)
+    g.load(w)
+    g.max_wave_speed()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st, _ = g.run(dx, CFL, max_steps=steps, limiter=LIMITER, mode=S.EXACT)
+    torch.cuda.synchronize()
+    fwd = time.perf_counter() - t0
+    fwd_out = g.store()
+    g.close()
+    log(f"plain EXACT forward: {st.steps} steps, {fwd*1e3:.1f} ms")
+    t0 = time.perf_counter()
+    res = S.euler_adjoint(w, seed, 2, dx, CFL, math.inf, limiter=LIMITER, max_steps=steps, ctx=ctx,
+                          checkpoint_every=args.checkpoint_every)
+    adj = time.perf_counter() - t0
+    log(f"forward + adjoint: {adj*1e3:.1f} ms (device-side wall {res.stats.wall_ms:.1f} ms)")
+    same = bool(np.array_equal(res.cells.view(np.uint64), fwd_out.view(np.uint64)))
+    # kernel breakdown in a separate timed run (per-launch events change the timing)
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    S.euler_adjoint(w, seed, 2, dx, CFL, math.inf, limiter=LIMITER, max_steps=min(steps, 20), ctx=ctx)
+    ctx.set_timing(False)
+    kt = kernel_table(ctx, ["sweep_x_slic", "sweep_y_slic", "adjoint_sweep_x", "adjoint_sweep_y"])
+    k = int(args.checkpoint_every) or int(math.ceil(math.sqrt(steps)))
+    grids = (steps + k - 1) // k + 3 * k + 1 + 4
+    line = {
+        "metric": "adjoint/forward cost FP64 SLIC 2D Euler (forward + adjoint wall / plain forward wall)",
+        "value": round(adj / fwd, 3), "unit": "x", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": round(adj * 1e3 / steps, 4), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"euler2d_slic_superbee_{n}x{n}_disc_adjoint", "grid": [n, n], "cfl": CFL,
+                   "limiter": "superbee", "checkpoint_every": k, "device_grids": grids},
+        "forward_ms": round(fwd * 1e3, 2), "forward_adjoint_ms": round(adj * 1e3, 2),
+        "adjoint_forward_matches_solve": same, "target": 3.0,
+        "kernels_20_steps": kt, "grad_finite": bool(np.isfinite(res.grad).all()), "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_copy(args):
     """Plain device copy of 1 GiB, three times (torch's copy kernel): the known
     byte count that scales the FETCH_SIZE / WRITE_SIZE counters."""
@@ -340,7 +405,10 @@ def bench_copy(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["euler2d", "mhd_batch", "hpr2d", "rgfm", "copy"], default="euler2d")
+    ap.add_argument("--workload", choices=["euler2d", "mhd_batch", "hpr2d", "rgfm", "adjoint", "copy"],
+                    default="euler2d")
+    ap.add_argument("--adj-n", type=int, default=4096)
+    ap.add_argument("--checkpoint-every", type=int, default=0)
     ap.add_argument("--rgfm-nx", type=int, default=2048)
     ap.add_argument("--rgfm-ny", type=int, default=1024)
     ap.add_argument("--t-final", type=float, default=170e-6)
@@ -380,6 +448,8 @@ def main():
         if args.instances > 16:
             args.instances = 16
         return bench_rgfm(args)
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
     if args.workload == "copy":
         return bench_copy(args)
     log(f"building initial condition {N}x{N} ...")

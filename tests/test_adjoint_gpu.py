@@ -197,6 +197,40 @@ def test_checkpoint_interval_changes_nothing(dim):
         O.assert_bitwise(other.cells, base.cells, f"final state, checkpoint every {k}")
 
 
+def test_adjoint_tile_variants_change_nothing():
+    """Every tile variant of the adjoint sweeps (shape and occupancy target)
+    gives the same gradient bit for bit (each output is computed with the same
+    arithmetic), and the autotuner records its picks under the grid's keys."""
+    w = sod_2d(70, 45)
+    seed = np.random.default_rng(12).standard_normal(w.shape)
+    w1 = sod_1d(300)
+    seed1 = np.random.default_rng(13).standard_normal(w1.shape)
+    ctx = S.Context(0)
+    try:
+        kx, ky, k1 = "euler_adjoint_x/70/45", "euler_adjoint_y/70/45", "euler_adjoint_x/1/300"
+        base = base1 = None
+        for vx in range(4):
+            for vy in range(4):
+                ctx.set_tuned_key(kx, vx)
+                ctx.set_tuned_key(ky, vy)
+                r = S.euler_adjoint(w, seed, 2, 1.0 / 45, 0.6, math.inf, params=IDEAL_M, max_steps=6, bc=1, ctx=ctx)
+                if base is None:
+                    base = r.grad
+                O.assert_bitwise(r.grad, base, f"tile variants x {vx} y {vy}")
+            ctx.set_tuned_key(k1, vx)
+            r1 = S.euler_adjoint(w1, seed1, 1, 1.0 / 300, 0.6, math.inf, params=IDEAL_M, max_steps=9, ctx=ctx)
+            if base1 is None:
+                base1 = r1.grad
+            O.assert_bitwise(r1.grad, base1, f"1D tile variant {vx}")
+        for k in (kx, ky):
+            ctx.set_tuned_key(k, -1)
+            assert ctx.tuned_key(k) == -1
+        S.euler_adjoint(w, seed, 2, 1.0 / 45, 0.6, math.inf, params=IDEAL_M, max_steps=2, bc=1, ctx=ctx)
+        assert 0 <= ctx.tuned_key(kx) <= 3 and 0 <= ctx.tuned_key(ky) <= 3
+    finally:
+        ctx.close()
+
+
 def test_adjoint_to_final_time():
     """Stop at t_final (the last step is cut to land on it): the recorded steps
     sum to t_final and the gradient still passes the dot-product test."""

@@ -73,6 +73,7 @@ SIGNATURES = {
     "vg_ctx_set_graphs": (C.c_int, [vptr, C.c_int]),
     "vg_ctx_set_autotune": (C.c_int, [vptr, C.c_int]),
     "vg_ctx_tuned_shape": (C.c_int, [vptr, C.c_char_p, C.POINTER(C.c_int)]),
+    "vg_ctx_set_tuned_shape": (C.c_int, [vptr, C.c_char_p, C.c_int]),
     "vg_euler_solve_1d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                                     C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(EulerParams), C.c_int,
                                     dptr, C.c_int, C.POINTER(RunStats)]),

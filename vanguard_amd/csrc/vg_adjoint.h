@@ -19,8 +19,17 @@
 // the same `<` tests -- a one-sided subgradient).  The 1e-5 clamp is a constant
 // (zero derivative), as in the forward.
 //
-// One workgroup handles a tile of BS consecutive cells along the sweep, a halo
-// of 4 on each side: outputs need N-bar of cells j-2..j+2.  Positions beyond the
+// Only the branch decisions need the forward's exact values, and they depend
+// only on U = p2c(w) and its differences: those are recomputed with the
+// forward's IEEE arithmetic, so every comparison comes out as in the forward.
+// Everything else is a linearisation point, where a last-bit difference moves
+// the derivative by a relative O(1e-16): it is evaluated with Newton-refined
+// reciprocals, and the round trip U' = p2c(c2p(X)) is taken as the identity it
+// is in exact arithmetic (U' = X, Jacobian I).
+//
+// One workgroup handles a tile of LW adjacent lines x BS / LW consecutive
+// positions along the sweep, a halo of 4 on each side: outputs need N-bar of
+// cells j-2..j+2.  Positions beyond the
 // real grid carry no output adjoint (their N-bar is 0); the adjoint that lands on
 // the two ghost positions on each side is written to a per-line ghost buffer and
 // folded into the interior cells it copies (transmissive mirror or periodic wrap).
@@ -52,7 +61,7 @@ __device__ __forceinline__ void vjp_p2c(const EPar &m, const double (&w)[6], con
 // p = (gamma-1)(U4 - |m|^2/(2 U0) - E_f (U5 - U0)) - gamma p_inf.
 __device__ __forceinline__ void vjp_c2p(const EPar &m, const double (&U)[6], const double (&wb)[6], double (&Ub)[6])
 {
-    const double q = 1.0 / U[0];
+    const double q = rcp_nr(U[0]);
     const double u = U[1] * q, v = U[2] * q, z = U[3] * q, lam = U[5] * q;
     const double g1 = m.gm1;
     const double vsq = u * u + v * v + z * z;
@@ -69,7 +78,7 @@ __device__ __forceinline__ void vjp_c2p(const EPar &m, const double (&U)[6], con
 template <int DIR>
 __device__ __forceinline__ void vjp_flux(const EPar &m, const double (&U)[6], const double (&Fb)[6], double (&Ub)[6])
 {
-    const double q = 1.0 / U[0];
+    const double q = rcp_nr(U[0]);
     const double vel[3] = {U[1] * q, U[2] * q, U[3] * q};
     const double lam = U[5] * q;
     const double un = vel[DIR];
@@ -121,36 +130,84 @@ __device__ __forceinline__ void vjp_flux(const EPar &m, const double (&U)[6], co
     for (int k = 0; k < 6; k++) Ub[k] = acc[k];
 }
 
-// d xi / d r for the branch the forward evaluation takes (slopelimiters.cpp:7-74).
-__device__ __forceinline__ double limiter_derivative(double r, double bias, int limiter)
+// Pieces of the slope limiters (Solvers/slopelimiters.cpp:7-74): which
+// expression of r the forward's comparisons select.
+enum LimiterPiece : int { kXiZero = 0, kXiTwoR, kXiOne, kXiR, kXiRcoef, kXiTwo, kXiVanLeer };
+
+// The forward's comparisons, in its IEEE arithmetic (limiter_value with
+// smin(a, b) = b < a ? b : a), reduced to the piece they select.
+__device__ __forceinline__ int limiter_piece(double r, double bias, int limiter)
 {
-    if (r < 0.0) return 0.0;
-    const double den = (1.0 - bias) + (1.0 + bias) * r;
-    const double dR = -2.0 * (1.0 + bias) / (den * den);   // d/dr of 2 / ((1-w) + (1+w) r)
-    const double R = 2.0 / den;
+    if (r < 0.0) return kXiZero;
     if (limiter == 0) {
-        if (r < 0.5) return 2.0;
-        if (r < 1.0) return 0.0;
-        // smin(smin(r, R), 2): the first argument wins unless the second is smaller
-        const double a = (R < r) ? R : r;
-        const double da = (R < r) ? dR : 1.0;
-        return (2.0 < a) ? 0.0 : da;
+        if (r < 0.5) return kXiTwoR;
+        if (r < 1.0) return kXiOne;
+        const double R = r_coefficient(r, bias);
+        const double a = (R < r) ? R : r;     // smin(r, R)
+        if (2.0 < a) return kXiTwo;           // smin(a, 2)
+        return (R < r) ? kXiRcoef : kXiR;
     }
     if (limiter == 1) {
+        const double R = r_coefficient(r, bias);
         const double vl = (2.0 * r) / (1.0 + r);
-        const double dvl = 2.0 / ((1.0 + r) * (1.0 + r));
-        return (R < vl) ? dR : dvl;
+        return (R < vl) ? kXiRcoef : kXiVanLeer;   // smin(vl, R)
     }
-    if (r < 1.0) return 1.0;
-    return (R < 1.0) ? dR : 0.0;
+    if (r < 1.0) return kXiR;
+    return (r_coefficient(r, bias) < 1.0) ? kXiRcoef : kXiOne;   // smin(1, R)
 }
 
-// ---- forward pieces (EXACT arithmetic, as vg_scheme.h) ------------------------------
-
-template <int DIR>
-__device__ __forceinline__ void fwd_flux(const EPar &m, const double (&U)[6], double (&F)[6])
+// Value and r-derivative of a piece at a linearisation point.
+__device__ __forceinline__ void limiter_piece_eval(int piece, double r, double bias, double &xi, double &dxi)
 {
-    EulerPhys::template flux<DIR>(m, U, F);
+    switch (piece) {
+    case kXiTwoR: xi = 2.0 * r; dxi = 2.0; break;
+    case kXiOne: xi = 1.0; dxi = 0.0; break;
+    case kXiR: xi = r; dxi = 1.0; break;
+    case kXiTwo: xi = 2.0; dxi = 0.0; break;
+    case kXiRcoef: {
+        const double iv = rcp_nr((1.0 - bias) + (1.0 + bias) * r);
+        xi = 2.0 * iv;
+        dxi = -2.0 * (1.0 + bias) * (iv * iv);
+        break;
+    }
+    case kXiVanLeer: {
+        const double iv = rcp_nr(1.0 + r);
+        xi = 2.0 * r * iv;
+        dxi = 2.0 * (iv * iv);
+        break;
+    }
+    default: xi = 0.0; dxi = 0.0; break;
+    }
+}
+
+// Slope of one field at a linearisation point, given the forward's piece:
+// base = cl dl + cr dr, r = num / den with |.| < 1e-5 replaced by +1e-5.
+struct SlopeLin {
+    double base, xi, dxi, num, iden;
+    bool cln, cld;
+};
+
+__device__ __forceinline__ SlopeLin slope_lin(double dl, double dr, int piece, double cl, double cr, double bias)
+{
+    SlopeLin s;
+    s.base = (cl * dl) + (cr * dr);
+    s.cln = fabs(dl) < 1e-5;
+    s.cld = fabs(dr) < 1e-5;
+    s.num = s.cln ? 1e-5 : dl;
+    s.iden = rcp_nr(s.cld ? 1e-5 : dr);
+    limiter_piece_eval(piece, s.num * s.iden, bias, s.xi, s.dxi);
+    return s;
+}
+
+// ---- forward pieces at linearisation points ---------------------------------------
+
+// Flux of a conserved vector with one refined reciprocal of rho.
+template <int DIR>
+__device__ __forceinline__ void lin_flux(const EPar &m, const double (&U)[6], double (&F)[6])
+{
+    double w[6];
+    EulerPhys::cons_to_prim_fast(m, U, w);
+    EulerPhys::template flux_prim_fast<DIR>(m, w, U, F);
 }
 
 // ---- the adjoint tile kernel ----------------------------------------------------------
@@ -166,9 +223,7 @@ struct AdjointArgs {
     EPar m;
 };
 
-constexpr int kAdjBS = 128;
 constexpr int kAdjHalo = 4;
-constexpr int kAdjOut = kAdjBS - 2 * kAdjHalo;
 
 template <int DIR>
 __device__ __forceinline__ long long adj_cell(int line, int k, int nx)
@@ -176,63 +231,96 @@ __device__ __forceinline__ long long adj_cell(int line, int k, int nx)
     return (DIR == 0) ? (long long)line * nx + k : (long long)k * nx + line;
 }
 
-template <int DIR>
-__global__ __launch_bounds__(kAdjBS) void euler_adjoint_sweep_kernel(AdjointArgs a)
+// Tile of one block: LW adjacent lines (the fastest-varying thread index, so a
+// Y sweep reads LW consecutive doubles per row) x BS / LW positions along the
+// sweep, of which the middle BS / LW - 2 * kAdjHalo are outputs.
+template <int BS, int LW>
+struct AdjTile {
+    static constexpr int lines = LW, len = BS / LW, out = BS / LW - 2 * kAdjHalo;
+    static_assert(BS % LW == 0 && out > 0, "tile too short for the halo");
+};
+
+// Hides a register value's origin from the optimiser (an empty asm that may
+// "modify" it), so a later recomputation from it is not merged with the
+// earlier one and the earlier intermediates need not stay live.
+template <int K>
+__device__ __forceinline__ void opaque(double (&v)[K])
+{
+#pragma unroll
+    for (int k = 0; k < K; k++) asm volatile("" : "+v"(v[k]));
+}
+
+// The tile runs in phases separated by barriers; three LDS planes of six
+// fields carry what neighbours exchange, and only w, the slopes' differences
+// and the evolved states stay in registers from the forward recomputation to
+// the reverse pass (the rest is recomputed there), to keep occupancy up.
+//   F1  U = p2c(w)                                   L0 <- U
+//   F2  slopes, evolved XL/XR                        L1 <- XL, L2 <- f(XL)
+//   F3  face q+1/2 flux (needs the next cell's XL)   L0 <- F
+//   F4  N = U + c (F_{q-1/2} - F_{q+1/2}), N-bar     L1 <- N-bar
+//   R1  face q+1/2 reverse                           L0 <- U'-bar, L2 <- F'-bar (next cell's left state)
+//   R2  evolved states, evolution and slope reverse  L0 <- to the cell before, L2 <- to the cell after
+//   R3  gather, w-bar = (d p2c / d w)^T U-bar
+//
+// WPE is the occupancy the register allocation must allow (waves per SIMD): 2
+// keeps every value in registers (about 234 live at the peak), 3 spills some
+// to scratch for half as many waves again in flight.  The arithmetic is the same.
+template <int DIR, int BS, int LW, int WPE>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void euler_adjoint_sweep_kernel(
+    AdjointArgs a)
 {
     constexpr int NF = 6;
-    __shared__ double s1[2 * NF * kAdjBS];
-    __shared__ double s2[NF * kAdjBS];
+    using T = AdjTile<BS, LW>;
+    __shared__ double L0[NF * BS], L1[NF * BS], L2[NF * BS];
     const int tid = threadIdx.x;
-    const int line = blockIdx.y;
-    const int n = (DIR == 0) ? a.nx : a.ny;
-    const int pos = -2 + (int)blockIdx.x * kAdjOut - kAdjHalo + tid;   // position along the line
+    const int q = tid / LW;                                  // position index within the tile
+    const int n = (DIR == 0) ? a.nx : a.ny, n_lines = (DIR == 0) ? a.ny : a.nx;
+    const int line_raw = (int)blockIdx.y * LW + tid % LW;
+    const bool line_ok = line_raw < n_lines;
+    const int line = line_ok ? line_raw : n_lines - 1;
+    const int pos = -2 + (int)blockIdx.x * T::out - kAdjHalo + q;   // position along the line
     const EPar &m = a.m;
     const double half_c = 0.5 * (a.ts / a.dx), lf_c = a.dx / a.ts, ri_c = a.ts / a.dx, up_c = a.ts / a.dx;
     const double cl = 0.5 * (1.0 + a.bias), cr = 0.5 * (1.0 - a.bias);
     const bool real = (pos >= 0 && pos < n);
+    const bool has_l = q > 0, has_r = q < T::len - 1;
+    const int tl = has_l ? tid - LW : tid, tr = has_r ? tid + LW : tid;
 
-    // ---- forward recompute -------------------------------------------------------
-    double w[NF], U[NF];
+    // ---- F1 ---------------------------------------------------------------------------
+    double w[NF];
     {
         const int k = (pos >= -2 && pos < n + 2) ? ghost_index(pos, n, a.bc) : (pos < 0 ? 0 : n - 1);
         const long long c = adj_cell<DIR>(line, k, a.nx);
 #pragma unroll
         for (int f = 0; f < NF; f++) w[f] = a.w[f * a.plane + c];
-    }
-    EulerPhys::prim_to_cons(m, w, U);
+        double U[NF];
+        EulerPhys::prim_to_cons(m, w, U);
 #pragma unroll
-    for (int f = 0; f < NF; f++) s2[f * kAdjBS + tid] = U[f];
+        for (int f = 0; f < NF; f++) L0[f * BS + tid] = U[f];
+    }
     __syncthreads();
-    double UL[NF], UR[NF];
-    const int tl = tid > 0 ? tid - 1 : tid, tr = tid < kAdjBS - 1 ? tid + 1 : tid;
-#pragma unroll
-    for (int f = 0; f < NF; f++) {
-        UL[f] = s2[f * kAdjBS + tl];
-        UR[f] = s2[f * kAdjBS + tr];
-    }
-    // slope, extrapolation, evolution (keeping what the reverse pass needs)
-    double dl[NF], dr[NF], base[NF], xi[NF], rr[NF], xl[NF], xr[NF];
-    bool cln[NF], cld[NF];
-#pragma unroll
-    for (int k = 0; k < NF; k++) {
-        dl[k] = U[k] - UL[k];
-        dr[k] = UR[k] - U[k];
-        base[k] = (cl * dl[k]) + (cr * dr[k]);
-        cln[k] = fabs(dl[k]) < 1e-5;
-        cld[k] = fabs(dr[k]) < 1e-5;
-        const double num = cln[k] ? 1e-5 : dl[k];
-        const double den = cld[k] ? 1e-5 : dr[k];
-        rr[k] = num / den;
-        xi[k] = limiter_value(rr[k], a.bias, a.limiter);
-        const double d = base[k] * xi[k];
-        xl[k] = U[k] - (0.5 * d);
-        xr[k] = U[k] + (0.5 * d);
-    }
-    double XL[NF], XR[NF];
+    // ---- F2 ---------------------------------------------------------------------------
+    double dl[NF], dr[NF], XL[NF], XR[NF], URp[NF], FRp[NF];
+    int pieces = 0;   // 3 bits per field: the limiter piece the forward selects
     {
+        double xl[NF], xr[NF];
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+            const double u = L0[k * BS + tid];
+            dl[k] = u - L0[k * BS + tl];
+            dr[k] = L0[k * BS + tr] - u;
+            const double num = (fabs(dl[k]) < 1e-5) ? 1e-5 : dl[k];
+            const double den = (fabs(dr[k]) < 1e-5) ? 1e-5 : dr[k];
+            const int pc = limiter_piece(num / den, a.bias, a.limiter);
+            pieces |= pc << (3 * k);
+            const SlopeLin sl = slope_lin(dl[k], dr[k], pc, cl, cr, a.bias);
+            const double d = sl.base * sl.xi;
+            xl[k] = u - (0.5 * d);
+            xr[k] = u + (0.5 * d);
+        }
         double Fl[NF], Fr[NF];
-        fwd_flux<DIR>(m, xl, Fl);
-        fwd_flux<DIR>(m, xr, Fr);
+        lin_flux<DIR>(m, xl, Fl);
+        lin_flux<DIR>(m, xr, Fr);
 #pragma unroll
         for (int k = 0; k < NF; k++) {
             const double e = half_c * (Fl[k] - Fr[k]);
@@ -240,51 +328,41 @@ __global__ __launch_bounds__(kAdjBS) void euler_adjoint_sweep_kernel(AdjointArgs
             XR[k] = xr[k] + e;
         }
     }
-    // round trips of the evolved states: W = c2p(X), U' = p2c(W), F' = f(U')
-    double WL[NF], WR[NF], ULp[NF], URp[NF], FLp[NF], FRp[NF];
-    EulerPhys::cons_to_prim(m, XL, WL);
-    EulerPhys::prim_to_cons(m, WL, ULp);
-    fwd_flux<DIR>(m, ULp, FLp);
-    EulerPhys::cons_to_prim(m, XR, WR);
-    EulerPhys::prim_to_cons(m, WR, URp);
-    fwd_flux<DIR>(m, URp, FRp);
-    __syncthreads();
+    {
+        double Fp[NF];
+        lin_flux<DIR>(m, XL, Fp);
 #pragma unroll
-    for (int f = 0; f < NF; f++) {
-        s1[f * kAdjBS + tid] = ULp[f];
-        s1[(NF + f) * kAdjBS + tid] = FLp[f];
+        for (int f = 0; f < NF; f++) {
+            L1[f * BS + tid] = XL[f];
+            L2[f * BS + tid] = Fp[f];
+        }
+#pragma unroll
+        for (int f = 0; f < NF; f++) URp[f] = XR[f];
+        lin_flux<DIR>(m, URp, FRp);
     }
     __syncthreads();
-    // face tid+1/2: a = right state of this cell, b = left state of the next
-    double Ub[NF], Fb[NF], Us[NF], Fface[NF];
-#pragma unroll
-    for (int f = 0; f < NF; f++) {
-        Ub[f] = s1[f * kAdjBS + tr];
-        Fb[f] = s1[(NF + f) * kAdjBS + tr];
-    }
+    // ---- F3: face q+1/2, a = right state of this cell, b = left state of the next ----
+    double Us[NF];
     {
         double lf[NF], ri[NF];
 #pragma unroll
         for (int k = 0; k < NF; k++) {
-            lf[k] = 0.5 * ((FRp[k] + Fb[k]) + (lf_c * (URp[k] - Ub[k])));
-            Us[k] = 0.5 * ((URp[k] + Ub[k]) + (ri_c * (FRp[k] - Fb[k])));
+            const double ub = L1[k * BS + tr], fb = L2[k * BS + tr];
+            lf[k] = 0.5 * ((FRp[k] + fb) + (lf_c * (URp[k] - ub)));
+            Us[k] = 0.5 * ((URp[k] + ub) + (ri_c * (FRp[k] - fb)));
         }
-        fwd_flux<DIR>(m, Us, ri);
+        lin_flux<DIR>(m, Us, ri);
 #pragma unroll
-        for (int k = 0; k < NF; k++) Fface[k] = 0.5 * (lf[k] + ri[k]);
+        for (int k = 0; k < NF; k++) L0[k * BS + tid] = 0.5 * (lf[k] + ri[k]);
     }
     __syncthreads();
-#pragma unroll
-    for (int f = 0; f < NF; f++) s2[f * kAdjBS + tid] = Fface[f];
-    __syncthreads();
-    double N[NF];
-#pragma unroll
-    for (int k = 0; k < NF; k++) N[k] = U[k] + (up_c * (s2[k * kAdjBS + tl] - Fface[k]));
-
-    // ---- reverse ----------------------------------------------------------------------
-    double Nb[NF];
+    // ---- F4 ---------------------------------------------------------------------------
+    opaque(w);
     {
-        double wb[NF];
+        double U[NF], N[NF], wb[NF], Nb[NF];
+        EulerPhys::prim_to_cons_fast(m, w, U);
+#pragma unroll
+        for (int k = 0; k < NF; k++) N[k] = U[k] + (up_c * (L0[k * BS + tl] - L0[k * BS + tid]));
         if (real) {
             const long long c = adj_cell<DIR>(line, pos, a.nx);
 #pragma unroll
@@ -294,105 +372,90 @@ __global__ __launch_bounds__(kAdjBS) void euler_adjoint_sweep_kernel(AdjointArgs
             for (int f = 0; f < NF; f++) wb[f] = 0.0;
         }
         vjp_c2p(m, N, wb, Nb);
+#pragma unroll
+        for (int f = 0; f < NF; f++) L1[f * BS + tid] = Nb[f];
     }
     __syncthreads();
-#pragma unroll
-    for (int f = 0; f < NF; f++) s2[f * kAdjBS + tid] = Nb[f];
-    __syncthreads();
-    // face tid+1/2: F-bar = up_c (N-bar_{t+1} - N-bar_t)
-    double Uab[NF], Fab[NF], Ubb[NF], Fbb[NF];
+    // ---- R1: face q+1/2, F-bar = up_c (N-bar_{q+1} - N-bar_q) -------------------------
+    double Uab[NF], Fab[NF];
     {
-        double Fbar[NF], LFb[NF], RIb[NF], Usb[NF];
+        double Fbar[NF], RIb[NF], Usb[NF];
 #pragma unroll
         for (int k = 0; k < NF; k++) {
-            Fbar[k] = (tid < kAdjBS - 1) ? up_c * (s2[k * kAdjBS + tr] - Nb[k]) : 0.0;
-            LFb[k] = 0.5 * Fbar[k];
+            Fbar[k] = has_r ? up_c * (L1[k * BS + tr] - L1[k * BS + tid]) : 0.0;
             RIb[k] = 0.5 * Fbar[k];
         }
         vjp_flux<DIR>(m, Us, RIb, Usb);
 #pragma unroll
         for (int k = 0; k < NF; k++) {
-            Uab[k] = 0.5 * lf_c * LFb[k] + 0.5 * Usb[k];
-            Ubb[k] = -0.5 * lf_c * LFb[k] + 0.5 * Usb[k];
-            Fab[k] = 0.5 * LFb[k] + 0.5 * ri_c * Usb[k];
-            Fbb[k] = 0.5 * LFb[k] - 0.5 * ri_c * Usb[k];
+            const double LFb = 0.5 * Fbar[k];
+            Uab[k] = 0.5 * lf_c * LFb + 0.5 * Usb[k];
+            Fab[k] = 0.5 * LFb + 0.5 * ri_c * Usb[k];
+            L0[k * BS + tid] = -0.5 * lf_c * LFb + 0.5 * Usb[k];   // next cell's left state: U'-bar
+            L2[k * BS + tid] = 0.5 * LFb - 0.5 * ri_c * Usb[k];    // and F'-bar
         }
     }
     __syncthreads();
+    // ---- R2 ---------------------------------------------------------------------------
+    double Ut[NF], Ft[NF];
 #pragma unroll
     for (int f = 0; f < NF; f++) {
-        s1[f * kAdjBS + tid] = Ubb[f];
-        s1[(NF + f) * kAdjBS + tid] = Fbb[f];
+        Ut[f] = has_l ? L0[f * BS + tl] : 0.0;
+        Ft[f] = has_l ? L2[f * BS + tl] : 0.0;
     }
-    __syncthreads();
-    // round-trip chains of both evolved states -> X-bar
-    double XLb[NF], XRb[NF];
+    __syncthreads();   // L0 and L2 are free for the slope exchange below
+    opaque(w);
+    opaque(dl);
+    opaque(dr);
+    double Ubar[NF];
     {
-        double Ut[NF], Ft[NF], t1[NF], Wb[NF];
-        // left state of this cell receives (U-bar, F-bar) from face tid-1/2
+        double XLb[NF], XRb[NF];
+        vjp_flux<DIR>(m, XL, Ft, XLb);
+        vjp_flux<DIR>(m, XR, Fab, XRb);
 #pragma unroll
-        for (int f = 0; f < NF; f++) {
-            Ut[f] = (tid > 0) ? s1[f * kAdjBS + tl] : 0.0;
-            Ft[f] = (tid > 0) ? s1[(NF + f) * kAdjBS + tl] : 0.0;
+        for (int k = 0; k < NF; k++) {
+            XLb[k] += Ut[k];
+            XRb[k] += Uab[k];
         }
-        vjp_flux<DIR>(m, ULp, Ft, t1);
+        // evolution and extrapolation -> U-bar (own) and d-bar, then the slope
+        double U[NF], xl[NF], xr[NF], eb[NF], tl1[NF], tr1[NF];
+        EulerPhys::prim_to_cons_fast(m, w, U);
 #pragma unroll
-        for (int k = 0; k < NF; k++) t1[k] += Ut[k];
-        vjp_p2c(m, WL, t1, Wb);
-        vjp_c2p(m, XL, Wb, XLb);
-        // right state of this cell: from this thread's face
-        vjp_flux<DIR>(m, URp, Fab, t1);
-#pragma unroll
-        for (int k = 0; k < NF; k++) t1[k] += Uab[k];
-        vjp_p2c(m, WR, t1, Wb);
-        vjp_c2p(m, XR, Wb, XRb);
-    }
-    // evolution and extrapolation -> U-bar (own) and d-bar
-    double Ubar[NF], db[NF];
-    {
-        double eb[NF], tl1[NF], tr1[NF];
-#pragma unroll
-        for (int k = 0; k < NF; k++) eb[k] = half_c * (XLb[k] + XRb[k]);
+        for (int k = 0; k < NF; k++) {
+            const SlopeLin sl = slope_lin(dl[k], dr[k], (pieces >> (3 * k)) & 7, cl, cr, a.bias);
+            const double d = sl.base * sl.xi;
+            xl[k] = U[k] - (0.5 * d);
+            xr[k] = U[k] + (0.5 * d);
+            eb[k] = half_c * (XLb[k] + XRb[k]);
+        }
         vjp_flux<DIR>(m, xl, eb, tl1);
         vjp_flux<DIR>(m, xr, eb, tr1);
 #pragma unroll
         for (int k = 0; k < NF; k++) {
             const double xlb = XLb[k] + tl1[k];
             const double xrb = XRb[k] - tr1[k];
-            Ubar[k] = Nb[k] + xlb + xrb;
-            db[k] = 0.5 * (xrb - xlb);
+            Ubar[k] = L1[k * BS + tid] + xlb + xrb;
+            const double db = 0.5 * (xrb - xlb);
+            const SlopeLin sl = slope_lin(dl[k], dr[k], (pieces >> (3 * k)) & 7, cl, cr, a.bias);
+            const double bb = db * sl.xi;
+            const double rb = db * sl.base * sl.dxi;
+            const double dlb = cl * bb + (sl.cln ? 0.0 : rb * sl.iden);
+            const double drb = cr * bb + (sl.cld ? 0.0 : -rb * sl.num * (sl.iden * sl.iden));
+            Ubar[k] += dlb - drb;
+            L0[k * BS + tid] = -dlb;   // for the cell before
+            L2[k * BS + tid] = drb;    // for the cell after
         }
     }
-    // slope -> contributions to U-bar of this cell and both neighbours
-    double toL[NF], toR[NF];
-#pragma unroll
-    for (int k = 0; k < NF; k++) {
-        const double bb = db[k] * xi[k];
-        const double rb = db[k] * base[k] * limiter_derivative(rr[k], a.bias, a.limiter);
-        const double num = cln[k] ? 1e-5 : dl[k];
-        const double den = cld[k] ? 1e-5 : dr[k];
-        const double dlb = cl * bb + (cln[k] ? 0.0 : rb / den);
-        const double drb = cr * bb + (cld[k] ? 0.0 : -rb * num / (den * den));
-        Ubar[k] += dlb - drb;
-        toL[k] = -dlb;
-        toR[k] = drb;
-    }
     __syncthreads();
+    // ---- R3 ---------------------------------------------------------------------------
 #pragma unroll
     for (int f = 0; f < NF; f++) {
-        s1[f * kAdjBS + tid] = toL[f];          // for cell tid-1
-        s1[(NF + f) * kAdjBS + tid] = toR[f];   // for cell tid+1
-    }
-    __syncthreads();
-#pragma unroll
-    for (int f = 0; f < NF; f++) {
-        if (tid < kAdjBS - 1) Ubar[f] += s1[f * kAdjBS + tr];
-        if (tid > 0) Ubar[f] += s1[(NF + f) * kAdjBS + tl];
+        if (has_r) Ubar[f] += L0[f * BS + tr];
+        if (has_l) Ubar[f] += L2[f * BS + tl];
     }
     double wb[NF];
     vjp_p2c(m, w, Ubar, wb);
-    // ---- store: interior cells, or the ghost buffer ---------------------------------
-    if (tid >= kAdjHalo && tid < kAdjBS - kAdjHalo && pos >= -2 && pos < n + 2) {
+    if (line_ok && q >= kAdjHalo && q < T::len - kAdjHalo && pos >= -2 && pos < n + 2) {
         if (real) {
             const long long c = adj_cell<DIR>(line, pos, a.nx);
 #pragma unroll

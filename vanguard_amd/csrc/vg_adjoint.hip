@@ -81,10 +81,43 @@ int fwd_sweep(vg_ctx *ctx, int dir, const double *in, double *out, long long pla
     return vg::launch_y<EulerPhys, 2, MODE>(ctx, y_variant, a);
 }
 
-// One adjoint sweep: wbar_in = (d sweep / d w)^T wbar_out at input w.
-int adj_sweep(vg_ctx *ctx, int dir, const double *w, const double *wbar_out, double *wbar_in, double *ghost,
-              long long plane, int ny, int nx, int bc, double h, double bias, int limiter, double ts,
-              const EulerPhys::Params &m)
+template <int DIR, int BS, int LW, int WPE>
+void launch_adjoint_tile(vg_ctx *ctx, const vg::AdjointArgs &a)
+{
+    const int n = (DIR == 0) ? a.nx : a.ny, lines = (DIR == 0) ? a.ny : a.nx;
+    using T = vg::AdjTile<BS, LW>;
+    dim3 grid((n + 4 + T::out - 1) / T::out, (lines + LW - 1) / LW);
+    vg::euler_adjoint_sweep_kernel<DIR, BS, LW, WPE><<<grid, BS, 0, ctx->stream>>>(a);
+}
+
+// Tile variants per direction (threads, lines per block, waves per SIMD).  X:
+// one line per block; Y: LW adjacent columns per block, so each row read is LW
+// consecutive doubles.
+constexpr int kAdjVariants = 4;
+constexpr int kAdjDefault[2] = {0, 0};
+
+void launch_adjoint_dir(vg_ctx *ctx, int dir, int variant, const vg::AdjointArgs &a)
+{
+    if (dir == 0) {
+        switch (variant) {
+        case 1: launch_adjoint_tile<0, 128, 1, 3>(ctx, a); break;
+        case 2: launch_adjoint_tile<0, 256, 1, 2>(ctx, a); break;
+        case 3: launch_adjoint_tile<0, 64, 1, 2>(ctx, a); break;
+        default: launch_adjoint_tile<0, 128, 1, 2>(ctx, a); break;
+        }
+        return;
+    }
+    switch (variant) {
+    case 1: launch_adjoint_tile<1, 256, 8, 3>(ctx, a); break;
+    case 2: launch_adjoint_tile<1, 256, 16, 2>(ctx, a); break;
+    case 3: launch_adjoint_tile<1, 256, 4, 2>(ctx, a); break;
+    default: launch_adjoint_tile<1, 256, 8, 2>(ctx, a); break;
+    }
+}
+
+vg::AdjointArgs adjoint_args(const double *w, const double *wbar_out, double *wbar_in, double *ghost,
+                             long long plane, int ny, int nx, int bc, double h, double bias, int limiter, double ts,
+                             const EulerPhys::Params &m)
 {
     vg::AdjointArgs a;
     a.w = w;
@@ -100,19 +133,71 @@ int adj_sweep(vg_ctx *ctx, int dir, const double *w, const double *wbar_out, dou
     a.bias = bias;
     a.ts = ts;
     a.m = m;
-    const int n = (dir == 0) ? nx : ny, lines = (dir == 0) ? ny : nx;
-    dim3 grid((n + 4 + vg::kAdjOut - 1) / vg::kAdjOut, lines);
+    return a;
+}
+
+// One adjoint sweep: wbar_in = (d sweep / d w)^T wbar_out at input w.  Every
+// tile variant computes each output with the same arithmetic (the halo is
+// recomputed, not shared), so the variant changes nothing in the result.
+int adj_sweep(vg_ctx *ctx, int dir, int variant, const double *w, const double *wbar_out, double *wbar_in,
+              double *ghost, long long plane, int ny, int nx, int bc, double h, double bias, int limiter, double ts,
+              const EulerPhys::Params &m)
+{
+    const vg::AdjointArgs a = adjoint_args(w, wbar_out, wbar_in, ghost, plane, ny, nx, bc, h, bias, limiter, ts, m);
+    const int lines = (dir == 0) ? ny : nx;
     return ctx->timed(dir == 0 ? "adjoint_sweep_x" : "adjoint_sweep_y", [&] {
-        if (dir == 0) {
-            vg::euler_adjoint_sweep_kernel<0><<<grid, vg::kAdjBS, 0, ctx->stream>>>(a);
+        launch_adjoint_dir(ctx, dir, variant, a);
+        if (dir == 0)
             vg::euler_adjoint_fold_kernel<0><<<(lines + 255) / 256, 256, 0, ctx->stream>>>(wbar_in, ghost, plane, ny,
                                                                                             nx, bc);
-        } else {
-            vg::euler_adjoint_sweep_kernel<1><<<grid, vg::kAdjBS, 0, ctx->stream>>>(a);
+        else
             vg::euler_adjoint_fold_kernel<1><<<(lines + 255) / 256, 256, 0, ctx->stream>>>(wbar_in, ghost, plane, ny,
                                                                                             nx, bc);
-        }
     });
+}
+
+std::string adjoint_key(int dir, int ny, int nx)
+{
+    return std::string(dir == 0 ? "euler_adjoint_x/" : "euler_adjoint_y/") + std::to_string(ny) + "/" +
+           std::to_string(nx);
+}
+
+// Tile variant of one direction for an ny x nx grid: the context's table, else
+// each variant runs once to warm up and twice timed on the caller's buffers
+// (the real sweep overwrites what they write), and the fastest is recorded.
+int tune_adjoint(vg_ctx *ctx, int dir, const vg::AdjointArgs &a, int &variant)
+{
+    const std::string key = adjoint_key(dir, a.ny, a.nx);
+    auto it = ctx->tuned.find(key);
+    if (it != ctx->tuned.end()) {
+        variant = it->second;
+        return VG_OK;
+    }
+    variant = kAdjDefault[dir];
+    if (!ctx->autotune || ctx->timing) return VG_OK;
+    hipEvent_t e0, e1;
+    VG_HIP(hipEventCreate(&e0));
+    VG_HIP(hipEventCreate(&e1));
+    float best = 0.0f;
+    for (int v = 0; v < kAdjVariants; v++) {
+        launch_adjoint_dir(ctx, dir, v, a);
+        VG_HIP(hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < 2; r++) launch_adjoint_dir(ctx, dir, v, a);
+        VG_HIP(hipEventRecord(e1, ctx->stream));
+        VG_HIP(hipEventSynchronize(e1));
+        int rc = vg::launch_status("adjoint autotune");
+        if (rc) return rc;
+        float ms = 0.0f;
+        VG_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (v == 0 || ms < best) {
+            best = ms;
+            variant = v;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    ctx->tuned[key] = variant;
+    return VG_OK;
 }
 
 struct DeviceBuffers {
@@ -238,6 +323,14 @@ int adjoint(vg_ctx *ctx, int dim, const double *prim_in, double *prim_out, const
     vg::aos_to_soa_kernel<6><<<EGrid::blocks_for(g.cells), 256, 0, s>>>(g.staging, wb[0], g.plane, g.cells);
     int cw = 0;
     const double dy = dx;
+    int xv = kAdjDefault[0], yv = kAdjDefault[1];
+    if (steps > 0 && !rc) {
+        // tune on the checkpointed initial grid with the first time step
+        const vg::AdjointArgs ta =
+            adjoint_args(ckpt[0], wb[0], wb[1], ghost, g.plane, ny, nx, bc, dx, bias, limiter, dts[0], m);
+        rc = tune_adjoint(ctx, 0, ta, xv);
+        if (!rc && dim == 2) rc = tune_adjoint(ctx, 1, ta, yv);
+    }
     for (int seg = (int)ckpt.size() - 1; seg >= 0 && !rc; seg--) {
         const int s0 = seg * K, s1 = std::min(steps, s0 + K);
         if (s0 >= s1) continue;
@@ -247,19 +340,19 @@ int adjoint(vg_ctx *ctx, int dim, const double *prim_in, double *prim_out, const
             const int base = (st - s0) * per;
             const double dt = dts[st];
             if (dim == 1) {
-                rc = adj_sweep(ctx, 0, slots[base], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias, limiter,
-                               dt, m);
+                rc = adj_sweep(ctx, 0, xv, slots[base], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias,
+                               limiter, dt, m);
                 cw = 1 - cw;
             } else {
-                rc = adj_sweep(ctx, 0, slots[base + 2], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias,
+                rc = adj_sweep(ctx, 0, xv, slots[base + 2], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias,
                                limiter, dt * 0.5, m);
                 cw = 1 - cw;
                 if (!rc)
-                    rc = adj_sweep(ctx, 1, slots[base + 1], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dy, bias,
-                                   limiter, dt, m);
+                    rc = adj_sweep(ctx, 1, yv, slots[base + 1], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dy,
+                                   bias, limiter, dt, m);
                 cw = 1 - cw;
                 if (!rc)
-                    rc = adj_sweep(ctx, 0, slots[base], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias,
+                    rc = adj_sweep(ctx, 0, xv, slots[base], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias,
                                    limiter, dt * 0.5, m);
                 cw = 1 - cw;
             }

@@ -140,6 +140,15 @@ int vg_ctx_tuned_shape(vg_ctx *ctx, const char *key, int *variant)
     return VG_OK;
 }
 
+int vg_ctx_set_tuned_shape(vg_ctx *ctx, const char *key, int variant)
+{
+    VG_CHECK(ctx != nullptr && key != nullptr, "vg_ctx_set_tuned_shape: null argument");
+    VG_CHECK(variant >= -1 && variant < 4, "vg_ctx_set_tuned_shape: variant must be -1..3");
+    if (variant < 0) ctx->tuned.erase(key);
+    else ctx->tuned[key] = variant;
+    return VG_OK;
+}
+
 int vg_ctx_reset_timing(vg_ctx *ctx)
 {
     VG_CHECK(ctx != nullptr, "vg_ctx_reset_timing: null context");

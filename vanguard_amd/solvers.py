@@ -69,6 +69,16 @@ class Context:
         check(self.lib.vg_ctx_tuned_shape(self.handle, key, C.byref(v)), "vg_ctx_tuned_shape")
         return v.value
 
+    def tuned_key(self, key: str) -> int:
+        """Variant recorded for a raw key (e.g. "euler_adjoint_y/ny/nx"), -1 if none."""
+        v = C.c_int()
+        check(self.lib.vg_ctx_tuned_shape(self.handle, key.encode(), C.byref(v)), "vg_ctx_tuned_shape")
+        return v.value
+
+    def set_tuned_key(self, key: str, variant: int):
+        """Pin a tile-shape variant for a key; -1 forgets it."""
+        check(self.lib.vg_ctx_set_tuned_shape(self.handle, key.encode(), int(variant)), "vg_ctx_set_tuned_shape")
+
 
 _default_ctx = None
 
