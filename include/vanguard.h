@@ -94,6 +94,9 @@ int vg_ctx_set_graphs(vg_ctx *ctx, int enabled);
 int vg_ctx_set_autotune(vg_ctx *ctx, int enabled);
 int vg_ctx_tuned_shape(vg_ctx *ctx, const char *key, int *variant);
 int vg_ctx_set_tuned_shape(vg_ctx *ctx, const char *key, int variant);
+/* Frees the device workspace the context keeps between calls (the adjoint's
+ * checkpoints and sweep inputs). */
+int vg_ctx_release_workspace(vg_ctx *ctx);
 
 /* ---- Euler: whole-solve drop-ins ---------------------------------------- */
 
@@ -164,8 +167,10 @@ int vg_euler_solve_batch(vg_ctx *ctx, int n_inst, const int64_t *offsets, const 
  * solve2D, EXACT arithmetic) with the time steps held at the values the forward
  * run chose.  No reference counterpart: the reference has no derivatives.
  * Runs the forward (prim_out receives its final state, dt_log its time steps),
- * keeping a checkpoint every `checkpoint_every` steps (0: ceil(sqrt(steps))),
- * then the adjoint backwards with recomputation per segment.  seed: the
+ * keeping a checkpoint every `checkpoint_every` steps (0: every step when that
+ * fits in free device memory and max_steps > 0, else ceil(sqrt(max_steps)), 16
+ * for runs to t_final), then the adjoint backwards with recomputation per
+ * segment.  seed: the
  * derivative of a scalar loss with respect to the final primitive state
  * (array of structures, 6 doubles per cell); grad receives its derivative with
  * respect to the initial primitive state.  The slope limiter is differentiated
@@ -179,6 +184,20 @@ int vg_euler_adjoint_2d(vg_ctx *ctx, const double *prim_in, double *prim_out, co
                         int nx, double dx, double cfl, double t_final, int max_steps, double bias, int limiter, int bc,
                         const vg_euler_params *mp, int checkpoint_every, double *dt_log, int dt_log_cap,
                         vg_run_stats *stats);
+/* The same with prim_in, prim_out, seed and grad in device memory (same
+ * array-of-structures layout); dt_log and stats stay on the host.  Checkpoints
+ * and sweep inputs live in the context's workspace, kept between calls
+ * (vg_ctx_release_workspace frees it); checkpoint_every 0 picks 1 when a grid
+ * per step fits in free device memory and the step count is capped, else
+ * ceil(sqrt(max_steps)) (16 for runs to t_final). */
+int vg_euler_adjoint_1d_device(vg_ctx *ctx, const double *prim_in, double *prim_out, const double *seed, double *grad,
+                               int n, double dx, double cfl, double t_final, int max_steps, double bias, int limiter,
+                               int bc, const vg_euler_params *mp, int checkpoint_every, double *dt_log, int dt_log_cap,
+                               vg_run_stats *stats);
+int vg_euler_adjoint_2d_device(vg_ctx *ctx, const double *prim_in, double *prim_out, const double *seed, double *grad,
+                               int ny, int nx, double dx, double cfl, double t_final, int max_steps, double bias,
+                               int limiter, int bc, const vg_euler_params *mp, int checkpoint_every, double *dt_log,
+                               int dt_log_cap, vg_run_stats *stats);
 
 /* The SLIC solve with prescribed time steps dts[0..n_steps) (EXACT arithmetic):
  * the map the adjoint differentiates, for frozen-dt finite-difference checks. */

@@ -231,6 +231,25 @@ def test_adjoint_tile_variants_change_nothing():
         ctx.close()
 
 
+@pytest.mark.parametrize("dim", [1, 2])
+def test_device_entry_points_match_host(dim):
+    """vg_euler_adjoint_*_device on device arrays gives the host entry point's
+    results bit for bit, and a second call reuses the context's workspace."""
+    w = sod_1d(150) if dim == 1 else sod_2d(30, 26)
+    seed = np.random.default_rng(31).standard_normal(w.shape)
+    shape = w.shape[:-1]
+    host = run_adjoint(w, seed, dim=dim, steps=13, bc=1)
+    wd, sd = torch.from_numpy(w).cuda(), torch.from_numpy(seed).cuda()
+    od, gd = torch.empty_like(wd), torch.empty_like(wd)
+    for _ in range(2):
+        dts, st = S.euler_adjoint_device(wd.data_ptr(), od.data_ptr(), sd.data_ptr(), gd.data_ptr(), shape,
+                                         1.0 / w.shape[-2], 0.6, math.inf, params=IDEAL_M, max_steps=13, bc=1)
+        O.assert_bitwise(dts, host.dt, "time steps")
+        O.assert_bitwise(od.cpu().numpy(), host.cells, "final state")
+        O.assert_bitwise(gd.cpu().numpy(), host.grad, "gradient")
+    S.default_context().release_workspace()
+
+
 def test_adjoint_to_final_time():
     """Stop at t_final (the last step is cut to land on it): the recorded steps
     sum to t_final and the gradient still passes the dot-product test."""

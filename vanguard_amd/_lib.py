@@ -74,6 +74,7 @@ SIGNATURES = {
     "vg_ctx_set_autotune": (C.c_int, [vptr, C.c_int]),
     "vg_ctx_tuned_shape": (C.c_int, [vptr, C.c_char_p, C.POINTER(C.c_int)]),
     "vg_ctx_set_tuned_shape": (C.c_int, [vptr, C.c_char_p, C.c_int]),
+    "vg_ctx_release_workspace": (C.c_int, [vptr]),
     "vg_euler_solve_1d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                                     C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(EulerParams), C.c_int,
                                     dptr, C.c_int, C.POINTER(RunStats)]),
@@ -86,6 +87,12 @@ SIGNATURES = {
     "vg_euler_adjoint_2d": (C.c_int, [vptr, dptr, dptr, dptr, dptr, C.c_int, C.c_int, C.c_double, C.c_double,
                                       C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(EulerParams),
                                       C.c_int, dptr, C.c_int, C.POINTER(RunStats)]),
+    "vg_euler_adjoint_1d_device": (C.c_int, [vptr, vptr, vptr, vptr, vptr, C.c_int, C.c_double, C.c_double,
+                                             C.c_double, C.c_int, C.c_double, C.c_int, C.c_int,
+                                             C.POINTER(EulerParams), C.c_int, dptr, C.c_int, C.POINTER(RunStats)]),
+    "vg_euler_adjoint_2d_device": (C.c_int, [vptr, vptr, vptr, vptr, vptr, C.c_int, C.c_int, C.c_double, C.c_double,
+                                             C.c_double, C.c_int, C.c_double, C.c_int, C.c_int,
+                                             C.POINTER(EulerParams), C.c_int, dptr, C.c_int, C.POINTER(RunStats)]),
     "vg_euler_solve_fixed_dt_1d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_double, dptr, C.c_int, C.c_double,
                                              C.c_int, C.c_int, C.POINTER(EulerParams)]),
     "vg_euler_solve_fixed_dt_2d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_int, C.c_double, dptr, C.c_int,

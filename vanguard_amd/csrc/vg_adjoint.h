@@ -74,7 +74,11 @@ __device__ __forceinline__ void vjp_c2p(const EPar &m, const double (&U)[6], con
 }
 
 // U-bar = (d f_DIR / d U)^T F-bar for the flux of a conserved vector
-// (eulerstatevector.cpp:93-120, 127-154): F = (m_n, m_n u + p e_n, ..., n (E + p), m_n lam).
+// (eulerstatevector.cpp:93-120, 127-154): with n = DIR, u_n = U_{1+n} / U0,
+//   F0 = U_{1+n},  F_{1+c} = U_{1+n} vel_c + p [c == n],  F4 = u_n (U4 + p),  F5 = u_n U5,
+//   p = (gamma-1)(U4 - |m|^2 / (2 U0) - E_f (U5 - U0)) - gamma p_inf.
+// Collecting terms: U-bar = s dp/dU + (terms of u_n) + u_n F-bar on the own
+// components, with s = F-bar_{1+n} + u_n F-bar_4 the weight of the pressure.
 template <int DIR>
 __device__ __forceinline__ void vjp_flux(const EPar &m, const double (&U)[6], const double (&Fb)[6], double (&Ub)[6])
 {
@@ -85,49 +89,18 @@ __device__ __forceinline__ void vjp_flux(const EPar &m, const double (&U)[6], co
     const double g1 = m.gm1;
     const double vsq = vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2];
     const double p = g1 * (U[4] - 0.5 * U[0] * vsq - m.e_form * (U[5] - U[0])) - m.gpinf;
-    // dp/dU
-    double dp[6];
-    dp[0] = g1 * (0.5 * vsq + m.e_form);
-    dp[1] = -g1 * vel[0];
-    dp[2] = -g1 * vel[1];
-    dp[3] = -g1 * vel[2];
-    dp[4] = g1;
-    dp[5] = -g1 * m.e_form;
-    // momentum rows: F_{1+c} = m_n vel_c + p [c == DIR]; d(m_n vel_c)/dU:
-    //   dU0: -un vel_c, dU_{1+DIR}: vel_c, dU_{1+c}: un (both when c == DIR: 2 un)
-    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    // mass row F0 = U_{1+DIR}
-    acc[1 + DIR] += Fb[0];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const double b = Fb[1 + c];
-        acc[0] += -un * vel[c] * b;
-        acc[1 + DIR] += vel[c] * b;
-        acc[1 + c] += un * b;
-        if (c == DIR) {
-#pragma unroll
-            for (int k = 0; k < 6; k++) acc[k] += dp[k] * b;
-        }
-    }
-    // energy row F4 = un (U4 + p)
-    {
-        const double b = Fb[4];
-        const double H = (U[4] + p) * q;
-        acc[0] += (-un * H) * b;
-        acc[1 + DIR] += H * b;
-        acc[4] += un * b;
-#pragma unroll
-        for (int k = 0; k < 6; k++) acc[k] += un * dp[k] * b;
-    }
-    // reaction row F5 = un U5
-    {
-        const double b = Fb[5];
-        acc[0] += (-un * lam) * b;
-        acc[1 + DIR] += lam * b;
-        acc[5] += un * b;
-    }
-#pragma unroll
-    for (int k = 0; k < 6; k++) Ub[k] = acc[k];
+    const double H = (U[4] + p) * q;
+    // d(u_n)/dU0 = -u_n / U0, d(u_n)/dU_{1+n} = 1 / U0: every row with a u_n
+    // factor contributes T (its coefficient of u_n, times U0) to both
+    const double T = vel[0] * Fb[1] + vel[1] * Fb[2] + vel[2] * Fb[3] + H * Fb[4] + lam * Fb[5];
+    const double sp = g1 * (Fb[1 + DIR] + un * Fb[4]);   // weight of dp/dU, (gamma-1) folded in
+    Ub[0] = sp * (0.5 * vsq + m.e_form) - un * T;
+    Ub[1] = un * Fb[1] - sp * vel[0];
+    Ub[2] = un * Fb[2] - sp * vel[1];
+    Ub[3] = un * Fb[3] - sp * vel[2];
+    Ub[1 + DIR] += T + Fb[0];
+    Ub[4] = sp + un * Fb[4];
+    Ub[5] = un * Fb[5] - sp * m.e_form;
 }
 
 // Pieces of the slope limiters (Solvers/slopelimiters.cpp:7-74): which
@@ -156,28 +129,28 @@ __device__ __forceinline__ int limiter_piece(double r, double bias, int limiter)
     return (r_coefficient(r, bias) < 1.0) ? kXiRcoef : kXiOne;   // smin(1, R)
 }
 
-// Value and r-derivative of a piece at a linearisation point.
-__device__ __forceinline__ void limiter_piece_eval(int piece, double r, double bias, double &xi, double &dxi)
+// Value and r-derivative of a piece at a linearisation point, by selection
+// (no divergent branches: the pieces of one limiter are all cheap).
+__device__ __forceinline__ void limiter_piece_eval(int piece, double r, double bias, int limiter, double &xi,
+                                                   double &dxi)
 {
-    switch (piece) {
-    case kXiTwoR: xi = 2.0 * r; dxi = 2.0; break;
-    case kXiOne: xi = 1.0; dxi = 0.0; break;
-    case kXiR: xi = r; dxi = 1.0; break;
-    case kXiTwo: xi = 2.0; dxi = 0.0; break;
-    case kXiRcoef: {
-        const double iv = rcp_nr((1.0 - bias) + (1.0 + bias) * r);
-        xi = 2.0 * iv;
-        dxi = -2.0 * (1.0 + bias) * (iv * iv);
-        break;
+    const double iv = rcp_nr((1.0 - bias) + (1.0 + bias) * r);
+    double x = 0.0, d = 0.0;
+    x = (piece == kXiTwoR) ? 2.0 * r : x;
+    d = (piece == kXiTwoR) ? 2.0 : d;
+    x = (piece == kXiOne) ? 1.0 : x;
+    x = (piece == kXiR) ? r : x;
+    d = (piece == kXiR) ? 1.0 : d;
+    x = (piece == kXiTwo) ? 2.0 : x;
+    x = (piece == kXiRcoef) ? 2.0 * iv : x;
+    d = (piece == kXiRcoef) ? -2.0 * (1.0 + bias) * (iv * iv) : d;
+    if (limiter == 1) {   // uniform over the launch
+        const double ivl = rcp_nr(1.0 + r);
+        x = (piece == kXiVanLeer) ? 2.0 * r * ivl : x;
+        d = (piece == kXiVanLeer) ? 2.0 * (ivl * ivl) : d;
     }
-    case kXiVanLeer: {
-        const double iv = rcp_nr(1.0 + r);
-        xi = 2.0 * r * iv;
-        dxi = 2.0 * (iv * iv);
-        break;
-    }
-    default: xi = 0.0; dxi = 0.0; break;
-    }
+    xi = x;
+    dxi = d;
 }
 
 // Slope of one field at a linearisation point, given the forward's piece:
@@ -187,7 +160,8 @@ struct SlopeLin {
     bool cln, cld;
 };
 
-__device__ __forceinline__ SlopeLin slope_lin(double dl, double dr, int piece, double cl, double cr, double bias)
+__device__ __forceinline__ SlopeLin slope_lin(double dl, double dr, int piece, double cl, double cr, double bias,
+                                              int limiter)
 {
     SlopeLin s;
     s.base = (cl * dl) + (cr * dr);
@@ -195,7 +169,7 @@ __device__ __forceinline__ SlopeLin slope_lin(double dl, double dr, int piece, d
     s.cld = fabs(dr) < 1e-5;
     s.num = s.cln ? 1e-5 : dl;
     s.iden = rcp_nr(s.cld ? 1e-5 : dr);
-    limiter_piece_eval(piece, s.num * s.iden, bias, s.xi, s.dxi);
+    limiter_piece_eval(piece, s.num * s.iden, bias, limiter, s.xi, s.dxi);
     return s;
 }
 
@@ -313,7 +287,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void 
             const double den = (fabs(dr[k]) < 1e-5) ? 1e-5 : dr[k];
             const int pc = limiter_piece(num / den, a.bias, a.limiter);
             pieces |= pc << (3 * k);
-            const SlopeLin sl = slope_lin(dl[k], dr[k], pc, cl, cr, a.bias);
+            const SlopeLin sl = slope_lin(dl[k], dr[k], pc, cl, cr, a.bias, a.limiter);
             const double d = sl.base * sl.xi;
             xl[k] = u - (0.5 * d);
             xr[k] = u + (0.5 * d);
@@ -422,7 +396,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         EulerPhys::prim_to_cons_fast(m, w, U);
 #pragma unroll
         for (int k = 0; k < NF; k++) {
-            const SlopeLin sl = slope_lin(dl[k], dr[k], (pieces >> (3 * k)) & 7, cl, cr, a.bias);
+            const SlopeLin sl = slope_lin(dl[k], dr[k], (pieces >> (3 * k)) & 7, cl, cr, a.bias, a.limiter);
             const double d = sl.base * sl.xi;
             xl[k] = U[k] - (0.5 * d);
             xr[k] = U[k] + (0.5 * d);
@@ -436,7 +410,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void 
             const double xrb = XRb[k] - tr1[k];
             Ubar[k] = L1[k * BS + tid] + xlb + xrb;
             const double db = 0.5 * (xrb - xlb);
-            const SlopeLin sl = slope_lin(dl[k], dr[k], (pieces >> (3 * k)) & 7, cl, cr, a.bias);
+            const SlopeLin sl = slope_lin(dl[k], dr[k], (pieces >> (3 * k)) & 7, cl, cr, a.bias, a.limiter);
             const double bb = db * sl.xi;
             const double rb = db * sl.base * sl.dxi;
             const double dlb = cl * bb + (sl.cln ? 0.0 : rb * sl.iden);

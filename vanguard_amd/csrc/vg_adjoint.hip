@@ -1,14 +1,16 @@
 // Adjoint C ABI for the Euler SLIC solve, and the fixed-time-step forward it
 // differentiates.
 //
-// vg_euler_adjoint_{1d,2d}: forward solve (the same kernels and arithmetic as
-// vg_euler_solve_*, EXACT mode) that records every dt and keeps a checkpoint of
-// the grid every K steps (K = ceil(sqrt(steps)) unless given); then, segment by
+// vg_euler_adjoint_{1d,2d}[_device]: forward solve (the same kernels and
+// arithmetic as vg_euler_solve_*, EXACT mode) that records every dt and keeps a
+// checkpoint of the grid every K steps (K = 1 when a grid per step fits in free
+// device memory, else ceil(sqrt(steps)), unless given); then, segment by
 // segment from the end, the segment is recomputed from its checkpoint with the
-// recorded dts, storing the input of every sweep, and the adjoint sweeps run
-// backwards through it: 2D steps X(dt/2) Y(dt) X(dt/2) reverse as
-// X*(dt/2) Y*(dt) X*(dt/2).  Memory: (steps / K) checkpoints + (3K + 1) sweep
-// inputs (2D) of one grid each.
+// recorded dts, storing the input of every sweep (the last sweep of a segment
+// is not needed), and the adjoint sweeps run backwards through it: 2D steps
+// X(dt/2) Y(dt) X(dt/2) reverse as X*(dt/2) Y*(dt) X*(dt/2).  Memory: (steps / K)
+// checkpoints + 3K - 1 sweep inputs (2D) of one grid each, kept in the
+// context's workspace between calls.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -218,10 +220,11 @@ struct DeviceBuffers {
     }
 };
 
-// Fixed-dt forward over steps [s0, s1) from `in`: with `store` (3K+1 grid slots
-// for 2D, K+1 for 1D) every sweep input is kept; otherwise ping-pong.
+// Fixed-dt forward over steps [s0, s1) through the grid slots: slot[base + k]
+// is the input of sweep k of step s0 + base / per.  With skip_last the final
+// sweep (whose output the adjoint never reads) is not run.
 int forward_fixed(vg_ctx *ctx, EGrid &g, const double *dts_dev, int s0, int s1, double dx, double bias, int limiter,
-                  int bc, const EulerPhys::Params &m, const std::vector<double *> &slots)
+                  int bc, const EulerPhys::Params &m, const std::vector<double *> &slots, bool skip_last = false)
 {
     vg::Clock h{};
     h.it = s0;
@@ -232,17 +235,19 @@ int forward_fixed(vg_ctx *ctx, EGrid &g, const double *dts_dev, int s0, int s1, 
     for (int s = s0; s < s1; s++) {
         vg::clock_fixed_kernel<<<1, 1, 0, ctx->stream>>>(g.clk, dts_dev, s1);
         const int base = (s - s0) * per;
-        int rc;
+        const bool last = skip_last && s == s1 - 1;
+        int rc = VG_OK;
         if (g.dim == 1) {
-            rc = fwd_sweep<vg::MODE_EXACT>(ctx, 0, slots[base], slots[base + 1], g.plane, g.ny, g.nx, 1, bc, dx, bias,
-                                           limiter, 1.0, m, g.clk, 0);
+            if (!last)
+                rc = fwd_sweep<vg::MODE_EXACT>(ctx, 0, slots[base], slots[base + 1], g.plane, g.ny, g.nx, 1, bc, dx,
+                                               bias, limiter, 1.0, m, g.clk, 0);
         } else {
             rc = fwd_sweep<vg::MODE_EXACT>(ctx, 0, slots[base], slots[base + 1], g.plane, g.ny, g.nx, 2, bc, dx, bias,
                                            limiter, 0.5, m, g.clk, g.y_variant);
             if (!rc)
                 rc = fwd_sweep<vg::MODE_EXACT>(ctx, 1, slots[base + 1], slots[base + 2], g.plane, g.ny, g.nx, 2, bc, dx,
                                                bias, limiter, 1.0, m, g.clk, g.y_variant);
-            if (!rc)
+            if (!rc && !last)
                 rc = fwd_sweep<vg::MODE_EXACT>(ctx, 0, slots[base + 2], slots[base + 3], g.plane, g.ny, g.nx, 2, bc, dx,
                                                bias, limiter, 0.5, m, g.clk, g.y_variant);
         }
@@ -251,9 +256,26 @@ int forward_fixed(vg_ctx *ctx, EGrid &g, const double *dts_dev, int s0, int s1, 
     return VG_OK;
 }
 
+// Checkpoint interval: the caller's; else every step when the run's length is
+// known and one grid per step fits in free device memory (the recomputation
+// then re-runs two sweeps per 2D step and none in 1D); else ceil(sqrt(steps)).
+int choose_interval(vg_ctx *ctx, int checkpoint_every, int max_steps, int per, size_t gbytes)
+{
+    if (checkpoint_every > 0) return checkpoint_every;
+    if (max_steps <= 0) return 16;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+        const double avail = 0.85 * (double)(free_b + ctx->workspace_bytes());
+        const double need = (double)gbytes * ((double)max_steps + 1.0 + per + 1.0 + 2.0) + 64.0 * 1024 * 1024;
+        if (need < avail) return 1;
+    }
+    return std::max(1, (int)std::ceil(std::sqrt((double)max_steps)));
+}
+
 int adjoint(vg_ctx *ctx, int dim, const double *prim_in, double *prim_out, const double *seed, double *grad, int ny,
             int nx, double dx, double cfl, double t_final, int max_steps, double bias, int limiter, int bc,
-            const vg_euler_params *mp, int checkpoint_every, double *dt_log, int dt_log_cap, vg_run_stats *stats)
+            const vg_euler_params *mp, int checkpoint_every, double *dt_log, int dt_log_cap, vg_run_stats *stats,
+            int on_device)
 {
     vg::clear_error();
     int rc = check_common(ctx, dim, ny, nx, dx, limiter, bc, mp);
@@ -269,24 +291,33 @@ int adjoint(vg_ctx *ctx, int dim, const double *prim_in, double *prim_out, const
     hipStream_t s = ctx->stream;
     EGrid g;
     rc = g.create(ctx, dim, ny, nx);
-    if (!rc) rc = g.load(prim_in, 0);
+    if (!rc) rc = g.load(prim_in, on_device);
+    if (!rc && !on_device) rc = g.ensure_staging();
     if (rc) {
         g.destroy();
         return rc;
     }
     const size_t gbytes = sizeof(double) * 6 * g.plane;
-    DeviceBuffers mem;
-    // ---- forward with checkpoints ---------------------------------------------------
+    const int per = (dim == 2) ? 3 : 1;
+    const int K = choose_interval(ctx, checkpoint_every, max_steps, per, gbytes);
+    // Workspace buffers: 0 dts, 1-2 adjoint ping-pong, 3 ghost adjoints,
+    // 4 .. 4 + per * K - 1 sweep inputs, then the checkpoints.
     const int cap = max_steps > 0 ? max_steps : 1 << 22;
+    auto wsp = [&](size_t i, size_t bytes, double **p) {
+        void *v = nullptr;
+        int r = ctx->workspace_buffer(i, bytes, &v);
+        *p = static_cast<double *>(v);
+        return r;
+    };
+    const size_t ck0 = 4 + (size_t)per * K;
+    // ---- forward with checkpoints ---------------------------------------------------
     std::vector<double> dts;
     std::vector<double *> ckpt;
-    int K = checkpoint_every;
-    if (K == 0) K = (max_steps > 0) ? std::max(1, (int)std::ceil(std::sqrt((double)max_steps))) : 16;
     vg_run_stats fst{};
-    double *slot = nullptr;
     for (;;) {
         if ((int)dts.size() >= cap) break;
-        if ((rc = mem.alloc(&slot, gbytes))) break;
+        double *slot = nullptr;
+        if ((rc = wsp(ck0 + ckpt.size(), gbytes, &slot))) break;
         VG_HIP(hipMemcpyAsync(slot, g.buf[g.cur], gbytes, hipMemcpyDeviceToDevice, s));
         ckpt.push_back(slot);
         std::vector<double> part(K);
@@ -303,24 +334,29 @@ int adjoint(vg_ctx *ctx, int dim, const double *prim_in, double *prim_out, const
         return rc;
     }
     const int steps = (int)dts.size();
-    rc = g.store(prim_out, 0);
+    rc = g.store(prim_out, on_device);
     // ---- reverse, segment by segment ----------------------------------------------------
     double *dts_dev = nullptr, *wb[2] = {nullptr, nullptr}, *ghost = nullptr;
-    const int per = (dim == 2) ? 3 : 1;
     std::vector<double *> slots(per * K + 1, nullptr);
-    if (!rc) rc = mem.alloc(&dts_dev, sizeof(double) * std::max(steps, 1));
-    if (!rc) rc = mem.alloc(&wb[0], gbytes);
-    if (!rc) rc = mem.alloc(&wb[1], gbytes);
-    if (!rc) rc = mem.alloc(&ghost, sizeof(double) * 4 * 6 * (size_t)std::max(nx, ny));
-    for (int k = 1; k < per * K + 1 && !rc; k++) rc = mem.alloc(&slots[k], gbytes);
+    if (!rc) rc = wsp(0, sizeof(double) * std::max(steps, 1), &dts_dev);
+    if (!rc) rc = wsp(1, gbytes, &wb[0]);
+    if (!rc) rc = wsp(2, gbytes, &wb[1]);
+    if (!rc) rc = wsp(3, sizeof(double) * 4 * 6 * (size_t)std::max(nx, ny), &ghost);
+    // slot 0 of a segment is its checkpoint; the last step's final sweep is not
+    // recomputed, so slot per * K is never written
+    for (int k = 1; k < per * K && !rc; k++) rc = wsp(4 + k - 1, gbytes, &slots[k]);
     if (rc) {
         g.destroy();
         return rc;
     }
     if (steps > 0) VG_HIP(hipMemcpyAsync(dts_dev, dts.data(), sizeof(double) * steps, hipMemcpyHostToDevice, s));
     // seed: array of structures -> planes
-    VG_HIP(hipMemcpyAsync(g.staging, seed, sizeof(double) * 6 * g.cells, hipMemcpyHostToDevice, s));
-    vg::aos_to_soa_kernel<6><<<EGrid::blocks_for(g.cells), 256, 0, s>>>(g.staging, wb[0], g.plane, g.cells);
+    const double *seed_aos = seed;
+    if (!on_device) {
+        VG_HIP(hipMemcpyAsync(g.staging, seed, sizeof(double) * 6 * g.cells, hipMemcpyHostToDevice, s));
+        seed_aos = g.staging;
+    }
+    vg::aos_to_soa_kernel<6><<<EGrid::blocks_for(g.cells), 256, 0, s>>>(seed_aos, wb[0], g.plane, g.cells);
     int cw = 0;
     const double dy = dx;
     int xv = kAdjDefault[0], yv = kAdjDefault[1];
@@ -335,7 +371,7 @@ int adjoint(vg_ctx *ctx, int dim, const double *prim_in, double *prim_out, const
         const int s0 = seg * K, s1 = std::min(steps, s0 + K);
         if (s0 >= s1) continue;
         slots[0] = ckpt[seg];
-        rc = forward_fixed(ctx, g, dts_dev, s0, s1, dx, bias, limiter, bc, m, slots);
+        rc = forward_fixed(ctx, g, dts_dev, s0, s1, dx, bias, limiter, bc, m, slots, true);
         for (int st = s1 - 1; st >= s0 && !rc; st--) {
             const int base = (st - s0) * per;
             const double dt = dts[st];
@@ -359,11 +395,13 @@ int adjoint(vg_ctx *ctx, int dim, const double *prim_in, double *prim_out, const
         }
     }
     if (!rc) {
-        vg::soa_to_aos_kernel<6><<<EGrid::blocks_for(g.cells), 256, 0, s>>>(wb[cw], g.staging, g.plane, g.cells);
+        double *dst = on_device ? grad : g.staging;
+        vg::soa_to_aos_kernel<6><<<EGrid::blocks_for(g.cells), 256, 0, s>>>(wb[cw], dst, g.plane, g.cells);
         rc = vg::launch_status("adjoint store");
     }
     if (!rc) {
-        hipError_t e = hipMemcpyAsync(grad, g.staging, sizeof(double) * 6 * g.cells, hipMemcpyDeviceToHost, s);
+        hipError_t e = hipSuccess;
+        if (!on_device) e = hipMemcpyAsync(grad, g.staging, sizeof(double) * 6 * g.cells, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) {
             vg::set_error(std::string("adjoint copy: ") + hipGetErrorString(e));
@@ -427,7 +465,16 @@ int vg_euler_adjoint_1d(vg_ctx *ctx, const double *prim_in, double *prim_out, co
                         vg_run_stats *stats)
 {
     return adjoint(ctx, 1, prim_in, prim_out, seed, grad, 1, n, dx, cfl, t_final, max_steps, bias, limiter, bc, mp,
-                   checkpoint_every, dt_log, dt_log_cap, stats);
+                   checkpoint_every, dt_log, dt_log_cap, stats, 0);
+}
+
+int vg_euler_adjoint_1d_device(vg_ctx *ctx, const double *prim_in, double *prim_out, const double *seed, double *grad,
+                               int n, double dx, double cfl, double t_final, int max_steps, double bias, int limiter,
+                               int bc, const vg_euler_params *mp, int checkpoint_every, double *dt_log, int dt_log_cap,
+                               vg_run_stats *stats)
+{
+    return adjoint(ctx, 1, prim_in, prim_out, seed, grad, 1, n, dx, cfl, t_final, max_steps, bias, limiter, bc, mp,
+                   checkpoint_every, dt_log, dt_log_cap, stats, 1);
 }
 
 int vg_euler_adjoint_2d(vg_ctx *ctx, const double *prim_in, double *prim_out, const double *seed, double *grad, int ny,
@@ -436,7 +483,16 @@ int vg_euler_adjoint_2d(vg_ctx *ctx, const double *prim_in, double *prim_out, co
                         vg_run_stats *stats)
 {
     return adjoint(ctx, 2, prim_in, prim_out, seed, grad, ny, nx, dx, cfl, t_final, max_steps, bias, limiter, bc, mp,
-                   checkpoint_every, dt_log, dt_log_cap, stats);
+                   checkpoint_every, dt_log, dt_log_cap, stats, 0);
+}
+
+int vg_euler_adjoint_2d_device(vg_ctx *ctx, const double *prim_in, double *prim_out, const double *seed, double *grad,
+                               int ny, int nx, double dx, double cfl, double t_final, int max_steps, double bias,
+                               int limiter, int bc, const vg_euler_params *mp, int checkpoint_every, double *dt_log,
+                               int dt_log_cap, vg_run_stats *stats)
+{
+    return adjoint(ctx, 2, prim_in, prim_out, seed, grad, ny, nx, dx, cfl, t_final, max_steps, bias, limiter, bc, mp,
+                   checkpoint_every, dt_log, dt_log_cap, stats, 1);
 }
 
 int vg_euler_solve_fixed_dt_1d(vg_ctx *ctx, const double *prim_in, double *prim_out, int n, double dx,

@@ -73,6 +73,13 @@ struct vg_ctx {
     }
     // Synchronises the stream and folds the recorded pairs into the table.
     int resolve_timing();
+    // Device buffers kept between calls (the adjoint's checkpoints and sweep
+    // inputs): buffer i is reused when it is large enough, else reallocated.
+    // One call at a time per context uses them.
+    std::vector<std::pair<void *, size_t>> workspace;
+    int workspace_buffer(size_t i, size_t bytes, void **out);
+    size_t workspace_bytes() const;
+    void release_workspace();
 };
 
 namespace vg {

@@ -60,6 +60,37 @@ int vg_ctx::resolve_timing()
     return VG_OK;
 }
 
+int vg_ctx::workspace_buffer(size_t i, size_t bytes, void **out)
+{
+    if (workspace.size() <= i) workspace.resize(i + 1, {nullptr, 0});
+    auto &b = workspace[i];
+    if (b.second < bytes) {
+        if (b.first) {
+            VG_HIP(hipStreamSynchronize(stream));
+            VG_HIP(hipFree(b.first));
+        }
+        b = {nullptr, 0};
+        VG_HIP(hipMalloc(&b.first, bytes));
+        b.second = bytes;
+    }
+    *out = b.first;
+    return VG_OK;
+}
+
+size_t vg_ctx::workspace_bytes() const
+{
+    size_t n = 0;
+    for (auto &b : workspace) n += b.second;
+    return n;
+}
+
+void vg_ctx::release_workspace()
+{
+    for (auto &b : workspace)
+        if (b.first) hipFree(b.first);
+    workspace.clear();
+}
+
 extern "C" {
 
 const char *vg_last_error(void) { return vg::g_last_error.c_str(); }
@@ -91,6 +122,7 @@ void vg_ctx_destroy(vg_ctx *ctx)
     if (!ctx) return;
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    ctx->release_workspace();
     for (auto &p : ctx->pending) {
         hipEventDestroy(p.second.first);
         hipEventDestroy(p.second.second);
@@ -137,6 +169,16 @@ int vg_ctx_tuned_shape(vg_ctx *ctx, const char *key, int *variant)
     VG_CHECK(ctx != nullptr && key != nullptr && variant != nullptr, "vg_ctx_tuned_shape: null argument");
     auto it = ctx->tuned.find(key);
     *variant = (it == ctx->tuned.end()) ? -1 : it->second;
+    return VG_OK;
+}
+
+int vg_ctx_release_workspace(vg_ctx *ctx)
+{
+    VG_CHECK(ctx != nullptr, "vg_ctx_release_workspace: null context");
+    int rc = vg::bind(ctx);
+    if (rc) return rc;
+    VG_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->release_workspace();
     return VG_OK;
 }
 

@@ -75,6 +75,10 @@ class Context:
         check(self.lib.vg_ctx_tuned_shape(self.handle, key.encode(), C.byref(v)), "vg_ctx_tuned_shape")
         return v.value
 
+    def release_workspace(self):
+        """Free the device workspace kept between calls (adjoint checkpoints)."""
+        check(self.lib.vg_ctx_release_workspace(self.handle), "vg_ctx_release_workspace")
+
     def set_tuned_key(self, key: str, variant: int):
         """Pin a tile-shape variant for a key; -1 forgets it."""
         check(self.lib.vg_ctx_set_tuned_shape(self.handle, key.encode(), int(variant)), "vg_ctx_set_tuned_shape")
@@ -186,6 +190,32 @@ def euler_adjoint(prim: np.ndarray, seed: np.ndarray, dim: int, dx: float, cfl: 
                                          C.byref(st))
     check(rc, "vg_euler_adjoint")
     return AdjointResult(out, grad, dts[:min(st.steps, dt_cap)].copy(), st)
+
+
+def euler_adjoint_device(prim_ptr: int, prim_out_ptr: int, seed_ptr: int, grad_ptr: int, shape, dx: float,
+                         cfl: float, t_final: float, *, bias: float = 0.0, limiter: int = 0,
+                         params: EulerMaterialParameters | None = None, max_steps: int = 0, bc: int = TRANSMISSIVE,
+                         checkpoint_every: int = 0, dt_cap: int = 100000, ctx: Context | None = None):
+    """vg_euler_adjoint_1d_device / _2d_device: the four state arrays are device
+    pointers (array of structures, 6 doubles per cell) of a grid of `shape`
+    ((n,) or (ny, nx)).  Returns (time steps, stats)."""
+    ctx = ctx or default_context()
+    params = params or EulerMaterialParameters(1.4)
+    dts = np.empty(dt_cap)
+    st = _lib.RunStats()
+    mp = params.c_struct()
+    V = C.c_void_p
+    if len(shape) == 1:
+        rc = ctx.lib.vg_euler_adjoint_1d_device(ctx.handle, V(prim_ptr), V(prim_out_ptr), V(seed_ptr), V(grad_ptr),
+                                                shape[0], dx, cfl, t_final, max_steps, bias, limiter, bc, C.byref(mp),
+                                                checkpoint_every, _lib.as_dptr(dts), dt_cap, C.byref(st))
+    else:
+        rc = ctx.lib.vg_euler_adjoint_2d_device(ctx.handle, V(prim_ptr), V(prim_out_ptr), V(seed_ptr), V(grad_ptr),
+                                                shape[0], shape[1], dx, cfl, t_final, max_steps, bias, limiter, bc,
+                                                C.byref(mp), checkpoint_every, _lib.as_dptr(dts), dt_cap,
+                                                C.byref(st))
+    check(rc, "vg_euler_adjoint_device")
+    return dts[:min(st.steps, dt_cap)].copy(), st
 
 
 def euler_solve_fixed_dt(prim: np.ndarray, dim: int, dx: float, dts, *, bias: float = 0.0, limiter: int = 0,
