@@ -212,24 +212,31 @@ int vg_euler_solve_fixed_dt_2d(vg_ctx *ctx, const double *prim_in, double *prim_
 
 /* MHDMaterialParameters (MHD/mhdmaterialparameters.h) field order: gamma, p_inf,
  * GLM hyperbolic speed c_h, parabolic damping c_p.  As in the reference the
- * solvers set c_h to each step's maximum wave speed and c_p = sqrt(0.18 c_h)
- * (mhdsecondordersolver.cpp:330-332), so the c_h/c_p given here are unused. */
+ * second-order solvers set c_h to each step's maximum wave speed and
+ * c_p = sqrt(0.18 c_h) (mhdsecondordersolver.cpp:330-332); the first-order
+ * solvers use the c_h given here (MHDMaterialParameters(gamma) has c_h = 0). */
 typedef struct {
     double gamma, p_inf, ch, cp;
 } vg_mhd_params;
 
 /* MHDSecondOrderSolver::solve (mhdsecondordersolver.h:58, mhdsecondordersolver.cpp:318-360)
- * when order == 2, MHDFirstOrderSolver::solve when order == 1, with zero
- * sub-cycles.  Primitive order (rho, u, v, w, p, Bx, By, Bz, psi), 9 doubles
- * per cell.  Runs as one persistent workgroup. */
+ * when order == 2, with `subcycles` RK4 GLM-damping sub-cycles
+ * (MHDForcingSolver::computeRungeKuttaTimeStep, mhdforcingsolver.cpp:7-44,
+ * 341-356) before and after every SLIC step; MHDFirstOrderSolver::solve
+ * (mhdfirstordersolver.cpp:457-477) when order == 1, which neither sub-cycles
+ * nor sets c_h (the c_h, c_p given here are its GLM constants).  Primitive
+ * order (rho, u, v, w, p, Bx, By, Bz, psi), 9 doubles per cell.  Without
+ * sub-cycles it runs as one persistent workgroup. */
 int vg_mhd_solve_1d(vg_ctx *ctx, const double *prim_in, double *prim_out, int n, double dx, double cfl,
-                    double t_final, int max_steps, double bias, int limiter, int order, int bc,
+                    double t_final, int max_steps, double bias, int limiter, int order, int bc, int subcycles,
                     const vg_mhd_params *mp, int mode, double *dt_log, int dt_log_cap, vg_run_stats *stats);
 
-/* MHDSecondOrderSolver::solve2D (mhdsecondordersolver.h:70, mhdsecondordersolver.cpp:501-546)
- * when order == 2, MHDFirstOrderSolver::solve2D when order == 1. */
+/* MHDSecondOrderSolver::solve2D (mhdsecondordersolver.h:70, mhdsecondordersolver.cpp:501-546,
+ * sub-cycles by computeRungeKuttaTimeStep2D, mhdforcingsolver.cpp:134-194, 394-417)
+ * when order == 2, MHDFirstOrderSolver::solve2D (mhdfirstordersolver.cpp:552-578)
+ * when order == 1. */
 int vg_mhd_solve_2d(vg_ctx *ctx, const double *prim_in, double *prim_out, int ny, int nx, double dx, double cfl,
-                    double t_final, int max_steps, double bias, int limiter, int order, int bc,
+                    double t_final, int max_steps, double bias, int limiter, int order, int bc, int subcycles,
                     const vg_mhd_params *mp, int mode, double *dt_log, int dt_log_cap, vg_run_stats *stats);
 
 /* n_inst independent MHDSecondOrderSolver::solve calls (transmissive

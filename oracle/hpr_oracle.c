@@ -383,6 +383,18 @@ static void source(const orc_hpr_params *m, const double *U, double *S)
     }
 }
 
+static void source_of(const orc_phys *ph, const double *U, double *S) { source(P_(ph), U, S); }
+
+/* computeRungeKuttaTimeStep[2D] leaves a cell's J alone without heat conduction. */
+static void keep_j(const orc_phys *ph, const double *w_in, double *w_out)
+{
+    if (!P_(ph)->is_thermal) {
+        w_out[14] = w_in[14];
+        w_out[15] = w_in[15];
+        w_out[16] = w_in[16];
+    }
+}
+
 static orc_phys hpr_phys(const orc_hpr_params *m)
 {
     orc_phys ph;
@@ -393,149 +405,23 @@ static orc_phys hpr_phys(const orc_hpr_params *m)
     ph.flux = flux;
     ph.wave_speed = wave_speed;
     ph.default_state = DEFAULT_STATE;
+    ph.source = source_of;
+    ph.forcing_finish = keep_j;
     ph.params = (void *)m;
     return ph;
 }
 
-/* HPRSolvers::evolveStateByFractionalX/YTimeStep (hprsolvers.cpp:606-620, 673-687):
- * half-slope extrapolations, flux difference scaled by frac * (ts / dx), then
- * evolveStateByFractionalTimeStep (:791-797) on the middle conserved vector. */
-static void evolve_fractional(const orc_phys *ph, int dir, double frac, const double *wl, const double *wm,
-                              const double *wr, double dx, double ts, double bias, int limiter, double *out)
-{
-    double Ul[NF], Um[NF], Ur[NF], d[NF], xl[NF], xr[NF], Fl[NF], Fr[NF], N[NF];
-    prim_to_cons(ph, wl, Ul);
-    prim_to_cons(ph, wm, Um);
-    prim_to_cons(ph, wr, Ur);
-    orc_slope_vector(NF, Ul, Um, Ur, bias, limiter, d);
-    for (int k = 0; k < NF; k++) {
-        xl[k] = Um[k] - (0.5 * d[k]);
-        xr[k] = Um[k] + (0.5 * d[k]);
-    }
-    flux(ph, dir, xl, Fl);
-    flux(ph, dir, xr, Fr);
-    double c = frac * (ts / dx);
-    for (int k = 0; k < NF; k++) N[k] = Um[k] + (c * (Fl[k] - Fr[k]));
-    cons_to_prim(ph, N, out);
-}
-
-/* HPRForcingSolver::evolveConservedVariableVector[2D] (hprforcingsolver.cpp:7-44,
- * 134-192) and computeEvolvedConservedVariableVector (:332-339): RK4 on the
- * source, each stage predicting the middle state with a fractional MUSCL step
- * (X in 1D; X(1/4) Y(1/2) X(1/4) etc. in 2D) from fixed neighbours. */
-static void forcing_cell(const orc_phys *ph, int dim, const double *Ul_, const double *Um, const double *Ur_,
-                         const double *Ut_, const double *Ub_, double dx, double ts, double bias, int limiter,
-                         double *Unew)
-{
-    const orc_hpr_params *m = P_(ph);
-    double wl[NF], wr[NF], wt[NF], wb[NF], wm[NF], e1[NF], e2[NF], e3[NF];
-    double k1[NF], k2[NF], k3[NF], k4[NF], S[NF], Us[NF], Ue[NF];
-    cons_to_prim(ph, Ul_, wl);
-    cons_to_prim(ph, Ur_, wr);
-    if (dim == 2) {
-        cons_to_prim(ph, Ut_, wt);
-        cons_to_prim(ph, Ub_, wb);
-    }
-    /* stage fractions: X, Y, X parts */
-    static const double fx2[3] = {0.25, 0.25, 0.5}, fy2[3] = {0.5, 0.5, 1.0}, fx1[3] = {0.5, 0.5, 1.0};
-    double *ks[4] = {k1, k2, k3, k4};
-    source(m, Um, S);
-    for (int k = 0; k < NF; k++) k1[k] = ts * S[k];
-    for (int st = 0; st < 3; st++) {
-        const double *kp = ks[st];
-        for (int k = 0; k < NF; k++) Us[k] = Um[k] + (st < 2 ? (0.5 * kp[k]) : kp[k]);
-        cons_to_prim(ph, Us, wm);
-        if (dim == 1) {
-            evolve_fractional(ph, 0, fx1[st], wl, wm, wr, dx, ts, bias, limiter, e3);
-        } else {
-            evolve_fractional(ph, 0, fx2[st], wl, wm, wr, dx, ts, bias, limiter, e1);
-            evolve_fractional(ph, 1, fy2[st], wt, e1, wb, dx, ts, bias, limiter, e2);
-            evolve_fractional(ph, 0, fx2[st], wl, e2, wr, dx, ts, bias, limiter, e3);
-        }
-        prim_to_cons(ph, e3, Ue);
-        source(m, Ue, S);
-        for (int k = 0; k < NF; k++) ks[st + 1][k] = ts * S[k];
-    }
-    for (int k = 0; k < NF; k++)
-        Unew[k] = Um[k] + ((1.0 / 6.0) * (k1[k] + ((2.0 * k2[k]) + ((2.0 * k3[k]) + k4[k]))));
-}
-
-/* HPRForcingSolver::computeRungeKuttaTimeStep[2D] (hprforcingsolver.cpp:341-356,
- * 394-417) over a one-layer ghosted grid. */
-static void forcing_step(const orc_phys *ph, int dim, int ny, int nx, int bc, double dx, double ts, double bias,
-                         int limiter, const double *prim_in, double *prim_out)
-{
-    size_t cells = (size_t)(dim == 1 ? 1 : ny) * nx;
-    double *U = (double *)malloc(sizeof(double) * NF * cells);
-    for (size_t c = 0; c < cells; c++) prim_to_cons(ph, prim_in + NF * c, U + NF * c);
-#define GI(k, n) ((k) >= 0 && (k) < (n) ? (k) : (bc == 1 ? ((k) < 0 ? (k) + (n) : (k) - (n)) : ((k) < 0 ? -1 - (k) : 2 * (n) - 1 - (k))))
-    for (int i = 0; i < (dim == 1 ? 1 : ny); i++) {
-        for (int j = 0; j < nx; j++) {
-            size_t c = (size_t)i * nx + j;
-            const double *L = U + NF * ((size_t)i * nx + GI(j - 1, nx));
-            const double *R = U + NF * ((size_t)i * nx + GI(j + 1, nx));
-            const double *T = NULL, *B = NULL;
-            if (dim == 2) {
-                T = U + NF * ((size_t)GI(i - 1, ny) * nx + j);
-                B = U + NF * ((size_t)GI(i + 1, ny) * nx + j);
-            }
-            double N[NF];
-            forcing_cell(ph, dim, L, U + NF * c, R, T, B, dx, ts, bias, limiter, N);
-            cons_to_prim(ph, N, prim_out + NF * c);
-            if (!P_(ph)->is_thermal) {   /* the cell's own J is left untouched */
-                prim_out[NF * c + 14] = prim_in[NF * c + 14];
-                prim_out[NF * c + 15] = prim_in[NF * c + 15];
-                prim_out[NF * c + 16] = prim_in[NF * c + 16];
-            }
-        }
-    }
-#undef GI
-    free(U);
-}
-
 /* HPRSecondOrderSolver::solve / solve2D (hprsecondordersolver.cpp:314-352,
  * 502-546): dt, [n sub-cycles of the source at 0.5 dt / n], SLIC (X in 1D;
- * X(1/2) Y X(1/2) in 2D), [n sub-cycles]. */
+ * X(1/2) Y X(1/2) in 2D), [n sub-cycles]; HPRFirstOrderSolver::solve never
+ * sub-cycles. */
 int orc_hpr_solve(int order, int dim, int ny, int nx, double dx, double cfl, double t_final, int max_steps,
                   double bias, int limiter, int bc, int subcycles, const orc_hpr_params *mp, const double *prim_in,
                   double *prim_out, double *dt_log, int dt_cap)
 {
     orc_phys ph = hpr_phys(mp);
-    size_t cells = (size_t)(dim == 1 ? 1 : ny) * nx;
-    double *a = (double *)malloc(sizeof(double) * NF * cells);
-    double *b = (double *)malloc(sizeof(double) * NF * cells);
-    memcpy(a, prim_in, sizeof(double) * NF * cells);
-    double t = 0.0;
-    int it = 0;
-    while (t < t_final && (max_steps == 0 || it < max_steps)) {
-        double S = orc_ghosted_max(&ph, dim, ny, nx, bc, a);
-        double dt = orc_stable_time_step(S, dx, cfl, t, t_final, it);
-        double tsub = 0.5 * (dt / subcycles);
-        for (int s = 0; s < subcycles; s++) {
-            forcing_step(&ph, dim, ny, nx, bc, dx, tsub, bias, limiter, a, b);
-            double *tmp = a; a = b; b = tmp;
-        }
-        if (dim == 1) {
-            orc_sweep(&ph, order, 1, 0, 1, nx, dx, dt, bias, limiter, bc, a, b);
-        } else {
-            double half = 0.5 * dt;
-            orc_sweep(&ph, order, 2, 0, ny, nx, dx, half, bias, limiter, bc, a, b);
-            orc_sweep(&ph, order, 2, 1, ny, nx, dx, dt, bias, limiter, bc, b, a);
-            orc_sweep(&ph, order, 2, 0, ny, nx, dx, half, bias, limiter, bc, a, b);
-        }
-        { double *tmp = a; a = b; b = tmp; }
-        for (int s = 0; s < subcycles; s++) {
-            forcing_step(&ph, dim, ny, nx, bc, dx, tsub, bias, limiter, a, b);
-            double *tmp = a; a = b; b = tmp;
-        }
-        if (dt_log && it < dt_cap) dt_log[it] = dt;
-        t += dt;
-        it += 1;
-    }
-    memcpy(prim_out, a, sizeof(double) * NF * cells);
-    free(a);
-    free(b);
-    return it;
+    return orc_solve_sub(&ph, order, dim, ny, nx, dx, cfl, t_final, max_steps, bias, limiter, bc, subcycles, prim_in,
+                         prim_out, dt_log, dt_cap);
 }
 
 /* Per-function records of five input states (85 doubles) -> 17*9 + 3 = 156:
@@ -562,7 +448,7 @@ void orc_hpr_units(int n, double bias, int limiter, double dx, double ts, const 
         orc_slic_flux(&ph, 0, a, b, c, d, dx, ts, bias, limiter, o); o += NF;
         {
             double N[NF];
-            forcing_cell(&ph, 1, Ua, Ub, Uc, NULL, NULL, dx, ts, bias, limiter, N);
+            orc_forcing_cell(&ph, 1, Ua, Ub, Uc, NULL, NULL, dx, ts, bias, limiter, N);
             cons_to_prim(&ph, N, o); o += NF;
         }
         *o++ = acoustic_speed(mp, b, 0);

@@ -157,6 +157,16 @@ static void begin_step(orc_phys *ph, double smax)
     mp->cp = sqrt(mp->ch * 0.18);
 }
 
+/* MHDStateVector::computeSourceTermVector (MHD/mhdstatevector.cpp:222-234): the
+ * GLM damping of psi, -(c_h^2 / c_p^2) psi; every other slot 0. */
+static void source(const orc_phys *ph, const double *U, double *S)
+{
+    const orc_mhd_params *mp = (const orc_mhd_params *)ph->params;
+    for (int k = 0; k < NF; k++) S[k] = 0.0;
+    double chs = mp->ch * mp->ch, cps = mp->cp * mp->cp;
+    S[8] = -(chs / cps) * U[8];
+}
+
 /* MHDStateVector default constructor (MHD/mhdstatevector.cpp:3-16). */
 static const double DEFAULT_STATE[NF] = {1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0};
 
@@ -171,18 +181,19 @@ static orc_phys mhd_phys(orc_mhd_params *mp)
     ph.wave_speed = wave_speed;
     ph.default_state = DEFAULT_STATE;
     ph.begin_step = begin_step;
+    ph.source = source;
     ph.params = mp;
     return ph;
 }
 
 int orc_mhd_solve(int order, int dim, int ny, int nx, double dx, double cfl, double t_final, int max_steps,
-                  double bias, int limiter, int bc, const orc_mhd_params *mp, const double *prim_in,
+                  double bias, int limiter, int bc, int subcycles, const orc_mhd_params *mp, const double *prim_in,
                   double *prim_out, double *dt_log, int dt_cap)
 {
     orc_mhd_params local = *mp;   /* solve takes the parameters by value and mutates its copy */
     orc_phys ph = mhd_phys(&local);
-    return orc_solve(&ph, order, dim, ny, nx, dx, cfl, t_final, max_steps, bias, limiter, bc, prim_in, prim_out,
-                     dt_log, dt_cap);
+    return orc_solve_sub(&ph, order, dim, ny, nx, dx, cfl, t_final, max_steps, bias, limiter, bc, subcycles, prim_in,
+                         prim_out, dt_log, dt_cap);
 }
 
 void orc_mhd_units(int n, double bias, int limiter, double dx, double ts, const orc_mhd_params *mp,

@@ -68,10 +68,12 @@ typedef struct {
     double gamma, p_inf, ch, cp;
 } orc_mhd_params;
 
-/* MHDSecondOrderSolver::solve/solve2D and MHDFirstOrderSolver::solve/solve2D with
- * zero sub-cycles; primitive order (rho, u, v, w, p, Bx, By, Bz, psi). */
+/* MHDSecondOrderSolver::solve/solve2D with `subcycles` RK4 GLM sub-cycles
+ * (MHDForcingSolver) on each side of every step, and MHDFirstOrderSolver::solve/
+ * solve2D (which neither sub-cycles nor sets c_h, so mp's c_h is used);
+ * primitive order (rho, u, v, w, p, Bx, By, Bz, psi). */
 int orc_mhd_solve(int order, int dim, int ny, int nx, double dx, double cfl, double t_final, int max_steps,
-                  double bias, int limiter, int bc, const orc_mhd_params *mp, const double *prim_in,
+                  double bias, int limiter, int bc, int subcycles, const orc_mhd_params *mp, const double *prim_in,
                   double *prim_out, double *dt_log, int dt_cap);
 /* Per-function records of five input states (45 doubles) -> 85 doubles. */
 void orc_mhd_units(int n, double bias, int limiter, double dx, double ts, const orc_mhd_params *mp,

@@ -23,6 +23,7 @@
 #include "Solvers/hprsecondordersolver.h"
 #include "Solvers/hprfirstordersolver.h"
 #include "Solvers/hprforcingsolver.h"
+#include "Solvers/mhdforcingsolver.h"
 #include "HPR/hpracoustictensor.h"
 #include "RGFM/rgfmsolver.h"
 
@@ -303,12 +304,15 @@ void putMhdState(MHDStateVector s, double *p)
     for (int k = 0; k < 9; k++) p[k] = w[k];
 }
 
-// mhd_run order dim ny nx dx cfl tfinal max_steps bias limiter gamma pinf in out_state out_dt check_api
-// Mirrors MHDSecondOrderSolver::solve / solve2D (mhdsecondordersolver.cpp:318-360, 501-546)
-// and MHDFirstOrderSolver::solve / solve2D with zero sub-cycles.
+// mhd_run order dim ny nx dx cfl tfinal max_steps bias limiter gamma pinf in out_state out_dt check_api [subcycles]
+// Mirrors MHDSecondOrderSolver::solve / solve2D (mhdsecondordersolver.cpp:318-360, 501-546),
+// RK4 GLM sub-cycles included (MHDForcingSolver::computeRungeKuttaTimeStep[2D]), and
+// MHDFirstOrderSolver::solve / solve2D (mhdfirstordersolver.cpp:457-477, 552-578), which
+// never set c_h and ignore sub-cycles.
 int mhdRun(int argc, char **argv)
 {
     if (argc < 18) { std::fprintf(stderr, "mhd_run: too few arguments\n"); return 2; }
+    const int sub = (argc > 18) ? std::atoi(argv[18]) : 0;
     int order = std::atoi(argv[2]);
     int dim = std::atoi(argv[3]);
     int ny = std::atoi(argv[4]);
@@ -345,22 +349,39 @@ int mhdRun(int argc, char **argv)
         {
             std::vector<MHDStateVector> withBoundary = MHDSolvers::insertBoundaryCells(cells, ghost);
             double dt = MHDSolvers::computeStableTimeStep(withBoundary, dx, cfl, t, tfinal, it, p);
-            double smax = MHDSolvers::computeMaximumWaveSpeed(withBoundary, p);
-            p.setHyperbolicWaveSpeed(smax);
-            p.configureParabolicDamping();
+            if (order == 2)
+            {
+                double smax = MHDSolvers::computeMaximumWaveSpeed(withBoundary, p);
+                p.setHyperbolicWaveSpeed(smax);
+                p.configureParabolicDamping();
+                for (int s = 0; s < sub; s++)
+                {
+                    withBoundary = MHDSolvers::insertBoundaryCells(cells, 1);
+                    MHDForcingSolver::computeRungeKuttaTimeStep(cells, withBoundary, dx, 0.5 * (dt / sub), bias, limiter, p);
+                }
+            }
             withBoundary = MHDSolvers::insertBoundaryCells(cells, ghost);
             if (order == 1)
                 MHDFirstOrderSolver::computeFORCETimeStep(cells, withBoundary, dx, dt, p);
             else
+            {
                 MHDSecondOrderSolver::computeSLICTimeStep(cells, withBoundary, dx, dt, bias, limiter, p);
+                for (int s = 0; s < sub; s++)
+                {
+                    withBoundary = MHDSolvers::insertBoundaryCells(cells, 1);
+                    MHDForcingSolver::computeRungeKuttaTimeStep(cells, withBoundary, dx, 0.5 * (dt / sub), bias, limiter, p);
+                }
+            }
             t += dt;
             it += 1;
             dts.push_back(dt);
         }
         for (int j = 0; j < nx; j++) putMhdState(cells[j], &out[(size_t)j * nf]);
-        if (checkApi && maxSteps == 0 && order == 2)
+        if (checkApi && maxSteps == 0)
         {
-            std::vector<MHDStateVector> api = MHDSecondOrderSolver::solve(initial, dx, cfl, tfinal, bias, limiter, 0, mp);
+            std::vector<MHDStateVector> api = (order == 2)
+                ? MHDSecondOrderSolver::solve(initial, dx, cfl, tfinal, bias, limiter, sub, mp)
+                : MHDFirstOrderSolver::solve(initial, dx, cfl, tfinal, sub, mp);
             bool match = true;
             for (int j = 0; j < nx; j++)
             {
@@ -383,9 +404,17 @@ int mhdRun(int argc, char **argv)
         {
             std::vector<std::vector<MHDStateVector> > withBoundary = MHDSolvers::insertBoundaryCells2D(cells, ghost);
             double dt = MHDSolvers::computeStableTimeStep2D(withBoundary, dx, cfl, t, tfinal, it, p);
-            double smax = MHDSolvers::computeMaximumWaveSpeed2D(withBoundary, p);
-            p.setHyperbolicWaveSpeed(smax);
-            p.configureParabolicDamping();
+            if (order == 2)
+            {
+                double smax = MHDSolvers::computeMaximumWaveSpeed2D(withBoundary, p);
+                p.setHyperbolicWaveSpeed(smax);
+                p.configureParabolicDamping();
+                for (int s = 0; s < sub; s++)
+                {
+                    withBoundary = MHDSolvers::insertBoundaryCells2D(cells, 1);
+                    MHDForcingSolver::computeRungeKuttaTimeStep2D(cells, withBoundary, dx, 0.5 * (dt / sub), bias, limiter, p);
+                }
+            }
             withBoundary = MHDSolvers::insertBoundaryCells2D(cells, ghost);
             if (order == 1)
             {
@@ -402,6 +431,11 @@ int mhdRun(int argc, char **argv)
                 MHDSecondOrderSolver::computeYSLICTimeStep2D(cells, withBoundary, dx, dt, bias, limiter, p);
                 withBoundary = MHDSolvers::insertBoundaryCells2D(cells, ghost);
                 MHDSecondOrderSolver::computeXSLICTimeStep2D(cells, withBoundary, dx, 0.5 * dt, bias, limiter, p);
+                for (int s = 0; s < sub; s++)
+                {
+                    withBoundary = MHDSolvers::insertBoundaryCells2D(cells, 1);
+                    MHDForcingSolver::computeRungeKuttaTimeStep2D(cells, withBoundary, dx, 0.5 * (dt / sub), bias, limiter, p);
+                }
             }
             t += dt;
             it += 1;
@@ -409,6 +443,24 @@ int mhdRun(int argc, char **argv)
         }
         for (int i = 0; i < ny; i++)
             for (int j = 0; j < nx; j++) putMhdState(cells[i][j], &out[((size_t)i * nx + j) * nf]);
+        if (checkApi && maxSteps == 0)
+        {
+            std::vector<std::vector<MHDStateVector> > init2(ny, std::vector<MHDStateVector>(nx));
+            for (int i = 0; i < ny; i++)
+                for (int j = 0; j < nx; j++) init2[i][j] = mhdStateFrom(&in[((size_t)i * nx + j) * nf]);
+            std::vector<std::vector<MHDStateVector> > api = (order == 2)
+                ? MHDSecondOrderSolver::solve2D(init2, dx, cfl, tfinal, bias, limiter, sub, mp)
+                : MHDFirstOrderSolver::solve2D(init2, dx, cfl, tfinal, sub, mp);
+            bool match = true;
+            for (int i = 0; i < ny; i++)
+                for (int j = 0; j < nx; j++)
+                {
+                    double q[9];
+                    putMhdState(api[i][j], q);
+                    for (int f = 0; f < nf; f++) match = match && sameBits(q[f], out[((size_t)i * nx + j) * nf + f]);
+                }
+            std::fprintf(stderr, "API_MATCH %d\n", match ? 1 : 0);
+        }
     }
     double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::fprintf(stderr, "STEPS %zu SECONDS %.6f\n", dts.size(), secs);
@@ -535,7 +587,9 @@ int hprRun(int argc, char **argv)
         {
             std::vector<HPRStateVector> withBoundary = HPRSolvers::insertBoundaryCells(cells, 2);
             double dt = HPRSolvers::computeStableTimeStep(withBoundary, dx, cfl, t, tfinal, it, mp);
-            for (int s = 0; s < sub; s++)
+            // HPRFirstOrderSolver::solve never sub-cycles (hprfirstordersolver.cpp:455-475)
+            const int nsub = (order == 2) ? sub : 0;
+            for (int s = 0; s < nsub; s++)
             {
                 withBoundary = HPRSolvers::insertBoundaryCells(cells, 1);
                 HPRForcingSolver::computeRungeKuttaTimeStep(cells, withBoundary, dx, 0.5 * (dt / sub), bias, limiter, mp);
@@ -545,7 +599,7 @@ int hprRun(int argc, char **argv)
                 HPRFirstOrderSolver::computeFORCETimeStep(cells, withBoundary, dx, dt, mp);
             else
                 HPRSecondOrderSolver::computeSLICTimeStep(cells, withBoundary, dx, dt, bias, limiter, mp);
-            for (int s = 0; s < sub; s++)
+            for (int s = 0; s < nsub; s++)
             {
                 withBoundary = HPRSolvers::insertBoundaryCells(cells, 1);
                 HPRForcingSolver::computeRungeKuttaTimeStep(cells, withBoundary, dx, 0.5 * (dt / sub), bias, limiter, mp);
@@ -555,9 +609,11 @@ int hprRun(int argc, char **argv)
             dts.push_back(dt);
         }
         for (int j = 0; j < nx; j++) putHprState(cells[j], &out[(size_t)j * nf]);
-        if (checkApi && maxSteps == 0 && order == 2)
+        if (checkApi && maxSteps == 0)
         {
-            std::vector<HPRStateVector> api = HPRSecondOrderSolver::solve(initial, dx, cfl, tfinal, bias, limiter, sub, mp);
+            std::vector<HPRStateVector> api = (order == 2)
+                ? HPRSecondOrderSolver::solve(initial, dx, cfl, tfinal, bias, limiter, sub, mp)
+                : HPRFirstOrderSolver::solve(initial, dx, cfl, tfinal, sub, mp);
             bool match = true;
             for (int j = 0; j < nx; j++)
             {

@@ -252,15 +252,120 @@ double orc_stable_time_step(double S, double dx, double cfl, double t, double T,
     return dt;
 }
 
-/* The solve loops of SecondOrderSolver/FirstOrderSolver::solve[2D]
- * (secondordersolver.cpp:288-314, 392-424; firstordersolver.cpp:435-461,
- * 521-553) and MHDSecondOrderSolver::solve[2D] (mhdsecondordersolver.cpp:318-360,
- * 501-546, with begin_step setting c_h), with an optional step cap. */
-int orc_solve(orc_phys *ph, int order, int dim, int ny, int nx, double dx, double cfl, double t_final,
-              int max_steps, double bias, int limiter, int bc, const double *prim_in, double *prim_out,
-              double *dt_log, int dt_cap)
+/* Solvers::evolveStateByFractionalX/YTimeStep (HPR: hprsolvers.cpp:606-620,
+ * 673-687; MHD: mhdsolvers.cpp:612-626 and the Y twin): half-slope
+ * extrapolations, flux difference scaled by frac * (ts / dx) added to the middle
+ * conserved vector (computeFractionalEvolutionVector), then
+ * evolveStateByFractionalTimeStep: the new primitive state. */
+static void evolve_fractional(const orc_phys *ph, int dir, double frac, const double *wl, const double *wm,
+                              const double *wr, double dx, double ts, double bias, int limiter, double *out)
 {
     const int nf = ph->nf;
+    double Ul[ORC_MAX_NF], Um[ORC_MAX_NF], Ur[ORC_MAX_NF], d[ORC_MAX_NF], xl[ORC_MAX_NF], xr[ORC_MAX_NF];
+    double Fl[ORC_MAX_NF], Fr[ORC_MAX_NF], N[ORC_MAX_NF];
+    ph->prim_to_cons(ph, wl, Ul);
+    ph->prim_to_cons(ph, wm, Um);
+    ph->prim_to_cons(ph, wr, Ur);
+    orc_slope_vector(nf, Ul, Um, Ur, bias, limiter, d);
+    for (int k = 0; k < nf; k++) {
+        xl[k] = Um[k] - (0.5 * d[k]);
+        xr[k] = Um[k] + (0.5 * d[k]);
+    }
+    ph->flux(ph, dir, xl, Fl);
+    ph->flux(ph, dir, xr, Fr);
+    double c = frac * (ts / dx);
+    for (int k = 0; k < nf; k++) N[k] = Um[k] + (c * (Fl[k] - Fr[k]));
+    ph->cons_to_prim(ph, N, out);
+}
+
+/* HPRForcingSolver / MHDForcingSolver::evolveConservedVariableVector[2D]
+ * (hprforcingsolver.cpp:7-44, 134-192; mhdforcingsolver.cpp:7-44, 134-194) and
+ * HPRForcingSolver::computeEvolvedConservedVariableVector (:332-339): RK4 on the
+ * source, each stage predicting the middle state with fractional MUSCL steps
+ * (X(1/2) in 1D; X(1/4) Y(1/2) X(1/4), then X(1/2) Y(1) X(1/2), in 2D) from
+ * fixed neighbours. */
+void orc_forcing_cell(const orc_phys *ph, int dim, const double *Ul_, const double *Um, const double *Ur_,
+                      const double *Ut_, const double *Ub_, double dx, double ts, double bias, int limiter,
+                      double *Unew)
+{
+    const int nf = ph->nf;
+    double wl[ORC_MAX_NF], wr[ORC_MAX_NF], wt[ORC_MAX_NF], wb[ORC_MAX_NF], wm[ORC_MAX_NF];
+    double e1[ORC_MAX_NF], e2[ORC_MAX_NF], e3[ORC_MAX_NF];
+    double k1[ORC_MAX_NF], k2[ORC_MAX_NF], k3[ORC_MAX_NF], k4[ORC_MAX_NF], S[ORC_MAX_NF], Us[ORC_MAX_NF];
+    double Ue[ORC_MAX_NF];
+    ph->cons_to_prim(ph, Ul_, wl);
+    ph->cons_to_prim(ph, Ur_, wr);
+    if (dim == 2) {
+        ph->cons_to_prim(ph, Ut_, wt);
+        ph->cons_to_prim(ph, Ub_, wb);
+    }
+    static const double fx2[3] = {0.25, 0.25, 0.5}, fy2[3] = {0.5, 0.5, 1.0}, fx1[3] = {0.5, 0.5, 1.0};
+    double *ks[4] = {k1, k2, k3, k4};
+    ph->source(ph, Um, S);
+    for (int k = 0; k < nf; k++) k1[k] = ts * S[k];
+    for (int st = 0; st < 3; st++) {
+        const double *kp = ks[st];
+        for (int k = 0; k < nf; k++) Us[k] = Um[k] + (st < 2 ? (0.5 * kp[k]) : kp[k]);
+        ph->cons_to_prim(ph, Us, wm);
+        if (dim == 1) {
+            evolve_fractional(ph, 0, fx1[st], wl, wm, wr, dx, ts, bias, limiter, e3);
+        } else {
+            evolve_fractional(ph, 0, fx2[st], wl, wm, wr, dx, ts, bias, limiter, e1);
+            evolve_fractional(ph, 1, fy2[st], wt, e1, wb, dx, ts, bias, limiter, e2);
+            evolve_fractional(ph, 0, fx2[st], wl, e2, wr, dx, ts, bias, limiter, e3);
+        }
+        ph->prim_to_cons(ph, e3, Ue);
+        ph->source(ph, Ue, S);
+        for (int k = 0; k < nf; k++) ks[st + 1][k] = ts * S[k];
+    }
+    for (int k = 0; k < nf; k++)
+        Unew[k] = Um[k] + ((1.0 / 6.0) * (k1[k] + ((2.0 * k2[k]) + ((2.0 * k3[k]) + k4[k]))));
+}
+
+/* HPRForcingSolver / MHDForcingSolver::computeRungeKuttaTimeStep[2D]
+ * (hprforcingsolver.cpp:341-356, 394-417; mhdforcingsolver.cpp:341-356, 394-417)
+ * over a one-layer ghosted grid: every cell's conserved vector from its
+ * primitive state, the RK4 update, setConservedVariableVector. */
+void orc_forcing_step(const orc_phys *ph, int dim, int ny, int nx, int bc, double dx, double ts, double bias,
+                      int limiter, const double *prim_in, double *prim_out)
+{
+    const int nf = ph->nf;
+    size_t cells = (size_t)(dim == 1 ? 1 : ny) * nx;
+    double *U = (double *)malloc(sizeof(double) * nf * cells);
+    for (size_t c = 0; c < cells; c++) ph->prim_to_cons(ph, prim_in + nf * c, U + nf * c);
+    for (int i = 0; i < (dim == 1 ? 1 : ny); i++) {
+        for (int j = 0; j < nx; j++) {
+            size_t c = (size_t)i * nx + j;
+            const double *L = U + nf * ((size_t)i * nx + ghost_index(j - 1, nx, bc));
+            const double *R = U + nf * ((size_t)i * nx + ghost_index(j + 1, nx, bc));
+            const double *T = NULL, *B = NULL;
+            if (dim == 2) {
+                T = U + nf * ((size_t)ghost_index(i - 1, ny, bc) * nx + j);
+                B = U + nf * ((size_t)ghost_index(i + 1, ny, bc) * nx + j);
+            }
+            double N[ORC_MAX_NF];
+            orc_forcing_cell(ph, dim, L, U + nf * c, R, T, B, dx, ts, bias, limiter, N);
+            ph->cons_to_prim(ph, N, prim_out + nf * c);
+            if (ph->forcing_finish) ph->forcing_finish(ph, prim_in + nf * c, prim_out + nf * c);
+        }
+    }
+    free(U);
+}
+
+/* The solve loops of SecondOrderSolver/FirstOrderSolver::solve[2D]
+ * (secondordersolver.cpp:288-314, 392-424; firstordersolver.cpp:435-461,
+ * 521-553), MHDSecondOrderSolver::solve[2D] (mhdsecondordersolver.cpp:318-360,
+ * 501-546: c_h set from the step's maximum, sub-cycles around the SLIC step),
+ * HPRSecondOrderSolver::solve[2D] (hprsecondordersolver.cpp:314-352, 502-546)
+ * and the first-order MHD/HPR solvers, which neither set c_h nor sub-cycle
+ * (mhdfirstordersolver.cpp:457-477, 552-578; hprfirstordersolver.cpp:455-475),
+ * with an optional step cap. */
+int orc_solve_sub(orc_phys *ph, int order, int dim, int ny, int nx, double dx, double cfl, double t_final,
+                  int max_steps, double bias, int limiter, int bc, int subcycles, const double *prim_in,
+                  double *prim_out, double *dt_log, int dt_cap)
+{
+    const int nf = ph->nf;
+    const int nsub = (order == 2 && ph->source) ? subcycles : 0;
     size_t cells = (size_t)(dim == 1 ? 1 : ny) * nx;
     double *a = (double *)malloc(sizeof(double) * nf * cells);
     double *b = (double *)malloc(sizeof(double) * nf * cells);
@@ -270,7 +375,11 @@ int orc_solve(orc_phys *ph, int order, int dim, int ny, int nx, double dx, doubl
     while (t < t_final && (max_steps == 0 || it < max_steps)) {
         double S = orc_ghosted_max(ph, dim, ny, nx, bc, a);
         double dt = orc_stable_time_step(S, dx, cfl, t, t_final, it);
-        if (ph->begin_step) ph->begin_step(ph, S);
+        if (order == 2 && ph->begin_step) ph->begin_step(ph, S);
+        for (int s = 0; s < nsub; s++) {
+            orc_forcing_step(ph, dim, ny, nx, bc, dx, 0.5 * (dt / subcycles), bias, limiter, a, b);
+            double *tmp = a; a = b; b = tmp;
+        }
         if (dim == 1) {
             orc_sweep(ph, order, 1, 0, 1, nx, dx, dt, bias, limiter, bc, a, b);
         } else {
@@ -279,7 +388,11 @@ int orc_solve(orc_phys *ph, int order, int dim, int ny, int nx, double dx, doubl
             orc_sweep(ph, order, 2, 1, ny, nx, dx, dt, bias, limiter, bc, b, a);
             orc_sweep(ph, order, 2, 0, ny, nx, dx, half, bias, limiter, bc, a, b);
         }
-        double *tmp = a; a = b; b = tmp;
+        { double *tmp = a; a = b; b = tmp; }
+        for (int s = 0; s < nsub; s++) {
+            orc_forcing_step(ph, dim, ny, nx, bc, dx, 0.5 * (dt / subcycles), bias, limiter, a, b);
+            double *tmp = a; a = b; b = tmp;
+        }
         if (dt_log && it < dt_cap) dt_log[it] = dt;
         t += dt;
         it += 1;
@@ -288,4 +401,12 @@ int orc_solve(orc_phys *ph, int order, int dim, int ny, int nx, double dx, doubl
     free(a);
     free(b);
     return it;
+}
+
+int orc_solve(orc_phys *ph, int order, int dim, int ny, int nx, double dx, double cfl, double t_final,
+              int max_steps, double bias, int limiter, int bc, const double *prim_in, double *prim_out,
+              double *dt_log, int dt_cap)
+{
+    return orc_solve_sub(ph, order, dim, ny, nx, dx, cfl, t_final, max_steps, bias, limiter, bc, 0, prim_in,
+                         prim_out, dt_log, dt_cap);
 }

@@ -17,8 +17,14 @@ struct orc_phys {
     double (*wave_speed)(const orc_phys *ph, const double *w, int dim);
     /* the state's default constructor (2D corner ghosts) */
     const double *default_state;
-    /* called once per step with the step's maximum wave speed (MHD sets c_h) */
+    /* called once per second-order step with the step's maximum wave speed
+     * (MHD sets c_h; the first-order solvers never do) */
     void (*begin_step)(orc_phys *ph, double smax);
+    /* source term S(U) of the RK4 sub-cycles (NULL: no source stage) */
+    void (*source)(const orc_phys *ph, const double *U, double *S);
+    /* after a sub-cycle, per cell: fix-ups of the new primitive state w_out
+     * given the old one w_in (HPR without heat conduction keeps its own J) */
+    void (*forcing_finish)(const orc_phys *ph, const double *w_in, double *w_out);
     void *params;
 };
 
@@ -48,6 +54,18 @@ double orc_stable_time_step(double S, double dx, double cfl, double t, double T,
 int orc_solve(orc_phys *ph, int order, int dim, int ny, int nx, double dx, double cfl, double t_final,
               int max_steps, double bias, int limiter, int bc, const double *prim_in, double *prim_out,
               double *dt_log, int dt_cap);
+/* The same with `subcycles` RK4 source sub-cycles on each side of every
+ * second-order step (first order never sub-cycles). */
+int orc_solve_sub(orc_phys *ph, int order, int dim, int ny, int nx, double dx, double cfl, double t_final,
+                  int max_steps, double bias, int limiter, int bc, int subcycles, const double *prim_in,
+                  double *prim_out, double *dt_log, int dt_cap);
+/* One RK4 source cell update from conserved neighbours (Ut/Ub NULL in 1D). */
+void orc_forcing_cell(const orc_phys *ph, int dim, const double *Ul, const double *Um, const double *Ur,
+                      const double *Ut, const double *Ub, double dx, double ts, double bias, int limiter,
+                      double *Unew);
+/* One sub-cycle over the grid (one-layer ghosts). */
+void orc_forcing_step(const orc_phys *ph, int dim, int ny, int nx, int bc, double dx, double ts, double bias,
+                      int limiter, const double *prim_in, double *prim_out);
 
 /* Euler physics table and EOS sound speed (euler_oracle.c), for the RGFM oracle. */
 #include "oracle.h"

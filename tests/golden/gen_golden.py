@@ -139,16 +139,16 @@ def rect_2d(ny, nx):
     return w
 
 
-def mhd_call(w, *, order, dim, dx, cfl, tfinal, max_steps, bias, limiter, gamma, p_inf, check_api):
+def mhd_call(w, *, order, dim, dx, cfl, tfinal, max_steps, bias, limiter, gamma, p_inf, check_api, subcycles=0):
     with tempfile.TemporaryDirectory() as tmp:
         ny, nx = (1, w.shape[0]) if dim == 1 else w.shape[:2]
         np.ascontiguousarray(w, dtype=np.float64).tofile(os.path.join(tmp, "in.f64"))
         t = "inf" if np.isinf(tfinal) else repr(float(tfinal))
         log = run(["mhd_run", order, dim, ny, nx, repr(dx), repr(cfl), t, max_steps, repr(bias), limiter, repr(gamma),
-                   repr(p_inf), "in.f64", "state.f64", "dt.f64", int(check_api)], tmp)
+                   repr(p_inf), "in.f64", "state.f64", "dt.f64", int(check_api), subcycles], tmp)
         state = np.fromfile(os.path.join(tmp, "state.f64")).reshape(w.shape)
         dt = np.fromfile(os.path.join(tmp, "dt.f64"))
-    if check_api and max_steps == 0 and order == 2 and dim == 1 and "API_MATCH 1" not in log:
+    if check_api and max_steps == 0 and "API_MATCH 1" not in log:
         raise RuntimeError("MHD mirror loop differs from the reference's own solve")
     return state, dt
 
@@ -208,6 +208,16 @@ def mhd_units(name, n, seed, bias, limiter, gamma, ch):
     np.savez_compressed(os.path.join(HERE, name + ".npz"), states=states, out=out,
                         meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
     print(f"{name}: {n} records")
+
+
+def mhd_psi_1d(n):
+    """Brio-Wu with a Gaussian bump of the GLM field psi and a ripple in Bx, so
+    the damping source and the psi/Bx coupling of the flux both act."""
+    w = problems.brio_wu(n)
+    x = (np.arange(n) + 0.5) / n
+    w[:, 8] = 0.2 * np.exp(-((x - 0.3) / 0.05) ** 2)
+    w[:, 5] = 0.75 + 0.1 * np.sin(2 * np.pi * x)
+    return w
 
 
 def mhd_2d_ic(n):
@@ -473,7 +483,18 @@ JOBS = {
                                      check_api=True),
     "mhd_force_briowu128": lambda: mhd_run("mhd_force_briowu128", problems.brio_wu(128), order=1, dim=1,
                                            dx=1.0 / 128, cfl=0.8, tfinal=0.1, max_steps=0, bias=0.0, limiter=0,
-                                           gamma=2.0, p_inf=0.0, check_api=False),
+                                           gamma=2.0, p_inf=0.0, check_api=True),
+    # first order never sets c_h: with a varying Bx the GLM terms stay switched off
+    "mhd_force_2d_ot16": lambda: mhd_run("mhd_force_2d_ot16", mhd_2d_ic(16), order=1, dim=2, dx=1.0 / 16, cfl=0.6,
+                                         tfinal=0.05, max_steps=0, bias=0.0, limiter=0, gamma=5.0 / 3.0, p_inf=0.0,
+                                         check_api=True),
+    # RK4 GLM sub-cycles (MHDForcingSolver) around every SLIC step
+    "mhd_psi128_sub2": lambda: mhd_run("mhd_psi128_sub2", mhd_psi_1d(128), order=2, dim=1, dx=1.0 / 128,
+                                       cfl=0.8, tfinal=0.05, max_steps=0, bias=0.0, limiter=1, gamma=2.0,
+                                       p_inf=0.0, check_api=True, subcycles=2),
+    "mhd_2d_ot16_sub1": lambda: mhd_run("mhd_2d_ot16_sub1", mhd_2d_ic(16), order=2, dim=2, dx=1.0 / 16, cfl=0.6,
+                                        tfinal=0.03, max_steps=0, bias=0.0, limiter=0, gamma=5.0 / 3.0, p_inf=0.0,
+                                        check_api=True, subcycles=1),
     "mhd_2d_ot32": lambda: mhd_run("mhd_2d_ot32", mhd_2d_ic(32), order=2, dim=2, dx=1.0 / 32, cfl=0.6,
                                    tfinal=np.inf, max_steps=30, bias=0.0, limiter=0, gamma=5.0 / 3.0, p_inf=0.0,
                                    check_api=False),

@@ -48,7 +48,8 @@ def load():
     lib.orc_euler_max_wave_speed.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(OrcEulerParams), dptr]
     lib.orc_mhd_solve.restype = C.c_int
     lib.orc_mhd_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
-                                  C.c_double, C.c_int, C.c_int, C.POINTER(OrcMHDParams), dptr, dptr, dptr, C.c_int]
+                                  C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(OrcMHDParams), dptr, dptr, dptr,
+                                  C.c_int]
     lib.orc_mhd_units.restype = None
     lib.orc_mhd_units.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.POINTER(OrcMHDParams),
                                   dptr, dptr]
@@ -183,15 +184,15 @@ class OrcMHDParams(C.Structure):
 
 
 def mhd_solve(prim, *, order, dim, dx, cfl, tfinal, max_steps, bias, limiter, gamma, p_inf=0.0, bc=0,
-              dt_cap=200000):
+              dt_cap=200000, subcycles=0, ch=0.0, cp=0.0):
     lib = load()
     w = np.ascontiguousarray(prim, dtype=np.float64)
     out = np.empty_like(w)
     dts = np.empty(dt_cap)
     ny, nx = (1, w.shape[0]) if dim == 1 else w.shape[:2]
-    mp = OrcMHDParams(gamma, p_inf, 0.0, 0.0)
-    n = lib.orc_mhd_solve(order, dim, ny, nx, dx, cfl, tfinal, max_steps, bias, limiter, bc, C.byref(mp), _p(w),
-                          _p(out), _p(dts), dt_cap)
+    mp = OrcMHDParams(gamma, p_inf, ch, cp)
+    n = lib.orc_mhd_solve(order, dim, ny, nx, dx, cfl, tfinal, max_steps, bias, limiter, bc, subcycles, C.byref(mp),
+                          _p(w), _p(out), _p(dts), dt_cap)
     return out, dts[:min(n, dt_cap)].copy()
 
 

@@ -17,13 +17,14 @@ from vanguard_amd import solvers as S
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", ["mhd_briowu256", "mhd_force_briowu128", "mhd_2d_ot32"])
+@pytest.mark.parametrize("name", ["mhd_briowu256", "mhd_force_briowu128", "mhd_2d_ot32", "mhd_force_2d_ot16",
+                                  "mhd_psi128_sub2", "mhd_2d_ot16_sub1"])
 def test_mhd_exact_bit_identical_to_reference(vg_ctx, name):
     g = O.golden(name)
     m = g["meta"]
     r = S.mhd_solve(g["prim_in"], m["dim"], m["dx"], m["cfl"], m["tfinal"], order=m["order"], bias=m["bias"],
                     limiter=m["limiter"], params=S.MHDMaterialParameters(m["gamma"], m["p_inf"]),
-                    max_steps=m["max_steps"])
+                    max_steps=m["max_steps"], subcycles=m.get("subcycles", 0))
     print(f"{name}: steps={r.stats.steps}", flush=True)
     O.assert_bitwise(r.dt, g["dt"], f"{name} dt")
     O.assert_bitwise(r.cells, g["prim_out"], f"{name} state")
@@ -53,6 +54,48 @@ def test_mhd_batch_uneven_sizes_match_oracle(vg_ctx):
         assert r.steps[q] == len(dts)
         assert r.t[q] == O.seq_sum(dts)
         O.assert_bitwise(r.cells[a:b], ref, f"instance {q} (n={b - a})")
+
+
+@pytest.mark.parametrize("dim,bc,sub,limiter", [(1, 1, 3, 0), (1, 0, 1, 2), (2, 1, 2, 1), (2, 0, 1, 0)])
+def test_mhd_sub_cycles_match_oracle(vg_ctx, dim, bc, sub, limiter):
+    """RK4 GLM sub-cycles on sizes and boundaries the fixtures do not cover,
+    bit for bit against the oracle (including the first-order path, which
+    ignores them and keeps the given c_h)."""
+    rng = np.random.default_rng(40 + dim + 3 * sub)
+    shape = (300, 9) if dim == 1 else (22, 31, 9)
+    w = np.zeros(shape)
+    w[..., 0] = rng.uniform(0.5, 1.5, shape[:-1])
+    w[..., 1:4] = rng.uniform(-0.3, 0.3, shape[:-1] + (3,))
+    w[..., 4] = rng.uniform(0.5, 1.5, shape[:-1])
+    w[..., 5:8] = rng.uniform(-0.8, 0.8, shape[:-1] + (3,))
+    w[..., 8] = rng.uniform(-0.1, 0.1, shape[:-1])
+    dx = 1.0 / shape[-2]
+    for order in (2, 1):
+        p = S.MHDMaterialParameters(5.0 / 3.0, 0.0, 0.7, 0.4)
+        r = S.mhd_solve(w, dim, dx, 0.6, np.inf, order=order, limiter=limiter, params=p, max_steps=12, bc=bc,
+                        subcycles=sub)
+        ref, dts = O.mhd_solve(w, order=order, dim=dim, dx=dx, cfl=0.6, tfinal=np.inf, max_steps=12, bias=0.0,
+                               limiter=limiter, gamma=5.0 / 3.0, bc=bc, subcycles=sub, ch=0.7, cp=0.4)
+        O.assert_bitwise(r.dt, dts, f"order {order} dt")
+        O.assert_bitwise(r.cells, ref, f"order {order} state")
+
+
+def test_mhd_sub_cycled_graph_replay_matches_eager():
+    """Captured step chunks with the source stage give the eager result bit for bit."""
+    g = O.golden("mhd_2d_ot16_sub1")
+    m = g["meta"]
+    out = []
+    for graphs in (True, False):
+        ctx = S.Context(0)
+        try:
+            ctx.set_graphs(graphs)
+            r = S.mhd_solve(g["prim_in"], 2, m["dx"], m["cfl"], m["tfinal"], limiter=m["limiter"],
+                            params=S.MHDMaterialParameters(m["gamma"]), subcycles=1, ctx=ctx)
+            out.append(r.cells)
+        finally:
+            ctx.close()
+    O.assert_bitwise(out[0], out[1], "graphs vs eager")
+    O.assert_bitwise(out[0], g["prim_out"], "graphs vs reference")
 
 
 def test_mhd_fast_mode_within_tolerance(vg_ctx):

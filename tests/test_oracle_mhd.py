@@ -15,15 +15,58 @@ import pytest
 import oracle_lib as O
 
 
-@pytest.mark.parametrize("name", ["mhd_briowu256", "mhd_force_briowu128", "mhd_2d_ot32"])
+MHD_RUNS = ["mhd_briowu256", "mhd_force_briowu128", "mhd_2d_ot32", "mhd_force_2d_ot16", "mhd_psi128_sub2",
+            "mhd_2d_ot16_sub1"]
+
+
+@pytest.mark.parametrize("name", MHD_RUNS)
 def test_oracle_matches_reference_mhd_run(name):
     g = O.golden(name)
     m = g["meta"]
     out, dts = O.mhd_solve(g["prim_in"], order=m["order"], dim=m["dim"], dx=m["dx"], cfl=m["cfl"],
                            tfinal=m["tfinal"], max_steps=m["max_steps"], bias=m["bias"], limiter=m["limiter"],
-                           gamma=m["gamma"], p_inf=m["p_inf"])
+                           gamma=m["gamma"], p_inf=m["p_inf"], subcycles=m.get("subcycles", 0))
     O.assert_bitwise(dts, g["dt"], f"{name} dt")
     O.assert_bitwise(out, g["prim_out"], f"{name} state")
+
+
+def test_sub_cycles_and_first_order_c_h_are_seen():
+    """The fixtures pin what they claim: the sub-cycled runs differ from runs
+    without sub-cycles, and the first-order 2D run differs from one whose c_h
+    follows the step's maximum wave speed (psi would move)."""
+    for name in ("mhd_psi128_sub2", "mhd_2d_ot16_sub1"):
+        g = O.golden(name)
+        m = g["meta"]
+        plain, _ = O.mhd_solve(g["prim_in"], order=2, dim=m["dim"], dx=m["dx"], cfl=m["cfl"], tfinal=m["tfinal"],
+                               max_steps=0, bias=0.0, limiter=m["limiter"], gamma=m["gamma"])
+        assert np.abs(plain[..., 8] - g["prim_out"][..., 8]).max() > 1e-4, name
+    g = O.golden("mhd_force_2d_ot16")
+    assert np.all(g["prim_out"][..., 8] == 0.0)   # c_h = 0: psi never moves
+    m = g["meta"]
+    moved, _ = O.mhd_solve(g["prim_in"], order=1, dim=2, dx=m["dx"], cfl=m["cfl"], tfinal=m["tfinal"], max_steps=0,
+                           bias=0.0, limiter=0, gamma=m["gamma"], ch=1.0, cp=1.0)
+    assert np.abs(moved[..., 8]).max() > 1e-6
+
+
+def test_glm_source_damps_psi_exponentially():
+    """A uniform state with psi != 0: the fluxes cancel, so each sub-cycle is RK4
+    on psi' = -(c_h^2 / c_p^2) psi with c_p^2 = 0.18 c_h; after one step with n
+    sub-cycles per half step psi is multiplied by the RK4 amplification factor
+    R(z)^(2n), z = -(c_h / 0.18) * dt / (2n)."""
+    w = np.zeros((32, 9))
+    w[:, 0] = 1.0
+    w[:, 4] = 1.0
+    w[:, 5] = 0.3
+    w[:, 8] = 0.25
+    for n in (1, 3):
+        out, dts = O.mhd_solve(w, order=2, dim=1, dx=1.0 / 32, cfl=0.5, tfinal=np.inf, max_steps=1, bias=0.0,
+                               limiter=0, gamma=1.4, bc=1, subcycles=n)
+        # c_h = the step's max |u| + c_f, with u = 0 and B = (0.3, 0, 0)
+        cf = np.sqrt(0.5 * ((1.4 + 0.09) + np.sqrt((1.4 + 0.09) ** 2 - 4 * 1.4 * 0.09)))
+        z = -(cf / 0.18) * (0.5 * dts[0] / n)
+        R = 1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24
+        np.testing.assert_allclose(out[:, 8], 0.25 * R ** (2 * n), rtol=1e-12)
+        np.testing.assert_allclose(out[:, :8], w[:, :8], rtol=0, atol=1e-15)
 
 
 def test_oracle_matches_reference_mhd_batch_instances():

@@ -104,7 +104,7 @@ __global__ __launch_bounds__(BS) void batch_solve_kernel(BatchArgs<P> a)
             break;
         }
         if (a.dt_log && it < a.dt_cap && tid == 0) a.dt_log[it] = dt;
-        P::begin_step(m, S);
+        if (ORDER == 2) P::begin_step(m, S);   // first-order solvers never set c_h
         const StepScalars sc(dt, dx);
         key = 0;
         for (int j0 = 0; j0 < n; j0 += OUT) {

@@ -19,9 +19,10 @@
 
 namespace vg {
 
-// Optional per-step source stage (the HPR relaxation sub-cycles): launched
-// `subcycles` times before and after the sweeps of every step, each launch
-// reading one buffer and writing the other.  The default has none.
+// Optional per-step source stage (the HPR relaxation and MHD GLM damping
+// sub-cycles): launched `subcycles` times before and after the sweeps of every
+// second-order step, each launch reading one buffer and writing the other.  The
+// first-order solvers never sub-cycle.  The default has none.
 template <class P>
 struct SourceStage {
     static constexpr bool kActive = false;
@@ -219,7 +220,7 @@ struct GridState {
     // maximum: the last source sub-cycle, else the last x sweep.
     int fused_count(int order) const
     {
-        if (SourceStage<P>::kActive && subcycles > 0) return (int)SourceStage<P>::blocks(*this);
+        if (SourceStage<P>::kActive && subcycles > 0 && order == 2) return (int)SourceStage<P>::blocks(*this);
         const int G = (order == 2) ? 2 : 1;
         return (int)(((nx + (kBSX - 2 * G) - 1) / (kBSX - 2 * G)) * (long long)((ny + kRX - 1) / kRX));
     }
@@ -292,7 +293,7 @@ struct GridState {
         if (rc) return rc;
         double *a = buf[cur], *b = buf[1 - cur];
         // Source half step before the sweeps (hprsecondordersolver.cpp:326-330, 514-520).
-        const int nsub = SourceStage<P>::kActive ? subcycles : 0;
+        const int nsub = (SourceStage<P>::kActive && ORDER == 2) ? subcycles : 0;
         for (int s = 0; s < nsub; s++) {
             rc = SourceStage<P>::launch(*this, a, b, 0, bc, dx, bias, limiter, m);
             if (rc) return rc;

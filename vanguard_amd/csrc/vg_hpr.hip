@@ -12,43 +12,7 @@ namespace vg {
 
 // The relaxation sub-cycle launch for GridState<HPRPhys>::enqueue_step.
 template <>
-struct SourceStage<HPRPhys> {
-    static constexpr bool kActive = true;
-    template <class GS>
-    static long long blocks(const GS &g)
-    {
-        return (g.cells + kForceBS - 1) / kForceBS;
-    }
-    template <class GS>
-    static int launch(GS &g, const double *in, double *out, int fuse, int bc, double dx, double bias, int limiter,
-                      const HPRPhys::Params &m)
-    {
-        ForcingArgs a;
-        a.in = in;
-        a.out = out;
-        a.plane = g.plane;
-        a.ny = g.ny;
-        a.nx = g.nx;
-        a.dim = g.dim;
-        a.bc = bc;
-        a.limiter = limiter;
-        a.subcycles = g.subcycles;
-        a.dx = dx;
-        a.bias = bias;
-        a.fuse_max = fuse;
-        a.m = m;
-        a.clk = g.clk;
-        a.partials = g.partials;
-        const long long blocks = (g.cells + kForceBS - 1) / kForceBS;
-        if (fuse) g.n_partials = (int)blocks;
-        return g.ctx->timed("hpr_source_rk4", [&] {
-            if (g.dim == 1)
-                hpr_forcing_kernel<1><<<(unsigned)blocks, kForceBS, 0, g.ctx->stream>>>(a);
-            else
-                hpr_forcing_kernel<2><<<(unsigned)blocks, kForceBS, 0, g.ctx->stream>>>(a);
-        });
-    }
-};
+struct SourceStage<HPRPhys> : ForcingStage<HPRPhys> {};
 
 }  // namespace vg
 

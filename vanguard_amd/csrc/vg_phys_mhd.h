@@ -29,6 +29,16 @@ struct MHDPhys {
         m.cp = sqrt(m.ch * 0.18);
     }
 
+    // MHDStateVector::computeSourceTermVector (MHD/mhdstatevector.cpp:222-234):
+    // GLM damping of psi, -(c_h^2 / c_p^2) psi; every other slot 0.
+    __device__ static __forceinline__ void source(const Params &m, const double (&U)[NF], double (&S)[NF])
+    {
+#pragma unroll
+        for (int k = 0; k < NF; k++) S[k] = 0.0;
+        const double chs = m.ch * m.ch, cps = m.cp * m.cp;
+        S[8] = -(chs / cps) * U[8];
+    }
+
     __device__ static __forceinline__ double eos_e(const Params &m, double rho, double p)
     {
         return (p + m.gpinf) / (m.gm1 * rho);

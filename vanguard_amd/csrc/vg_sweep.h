@@ -179,7 +179,7 @@ __global__ __launch_bounds__(BS) void sweep_x_kernel(SweepArgs<P> a)
     const double ts = a.clk->dt * a.dt_scale;
     const StepScalars sc(ts, a.dx);
     typename P::Params m = a.m;
-    P::begin_step(m, a.clk->S);
+    if (ORDER == 2) P::begin_step(m, a.clk->S);   // first-order solvers never set c_h
 
     const int tid = threadIdx.x;
     const int j = blockIdx.x * OUT - G + tid;
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(BS) void sweep_y_kernel(SweepArgs<P> a)
     const double ts = a.clk->dt * a.dt_scale;
     const StepScalars sc(ts, a.dx);
     typename P::Params m = a.m;
-    P::begin_step(m, a.clk->S);
+    if (ORDER == 2) P::begin_step(m, a.clk->S);   // first-order solvers never set c_h
 
     const int j = blockIdx.x * BS + threadIdx.x;
     const int ny = a.ny, nx = a.nx;

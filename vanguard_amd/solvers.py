@@ -392,9 +392,10 @@ class MHDMaterialParameters:
 
 def mhd_solve(prim: np.ndarray, dim: int, dx: float, cfl: float, t_final: float, *, order: int = 2,
               bias: float = 0.0, limiter: int = 0, params: MHDMaterialParameters | None = None, max_steps: int = 0,
-              mode: int = EXACT, bc: int = TRANSMISSIVE, dt_cap: int = 200000, ctx: Context | None = None,
-              raise_on_error: bool = True) -> SolveResult:
-    """Whole MHD solve through vg_mhd_solve_1d / vg_mhd_solve_2d."""
+              mode: int = EXACT, bc: int = TRANSMISSIVE, subcycles: int = 0, dt_cap: int = 200000,
+              ctx: Context | None = None, raise_on_error: bool = True) -> SolveResult:
+    """Whole MHD solve through vg_mhd_solve_1d / vg_mhd_solve_2d; `subcycles`
+    RK4 GLM-damping sub-cycles around every second-order step."""
     ctx = ctx or default_context()
     params = params or MHDMaterialParameters(2.0)
     w = np.ascontiguousarray(prim, dtype=np.float64)
@@ -406,13 +407,13 @@ def mhd_solve(prim: np.ndarray, dim: int, dx: float, cfl: float, t_final: float,
         if w.ndim != 2 or w.shape[1] != 9:
             raise ValueError("1D MHD state must be (cells, 9)")
         rc = ctx.lib.vg_mhd_solve_1d(ctx.handle, _lib.as_dptr(w), _lib.as_dptr(out), w.shape[0], dx, cfl, t_final,
-                                     max_steps, bias, limiter, order, bc, C.byref(mp), mode, _lib.as_dptr(dts), dt_cap,
-                                     C.byref(st))
+                                     max_steps, bias, limiter, order, bc, subcycles, C.byref(mp), mode,
+                                     _lib.as_dptr(dts), dt_cap, C.byref(st))
     else:
         if w.ndim != 3 or w.shape[2] != 9:
             raise ValueError("2D MHD state must be (rows, columns, 9)")
         rc = ctx.lib.vg_mhd_solve_2d(ctx.handle, _lib.as_dptr(w), _lib.as_dptr(out), w.shape[0], w.shape[1], dx, cfl,
-                                     t_final, max_steps, bias, limiter, order, bc, C.byref(mp), mode,
+                                     t_final, max_steps, bias, limiter, order, bc, subcycles, C.byref(mp), mode,
                                      _lib.as_dptr(dts), dt_cap, C.byref(st))
     if raise_on_error and rc != _lib.VG_OK:
         check(rc, "vg_mhd_solve")
@@ -440,23 +441,39 @@ def mhd_solve_batch(prim, offsets, dx, gamma, t_final, cfl, *, max_steps=0, bias
 
 
 class MHDSecondOrderSolver:
-    """MHDSecondOrderSolver (Solvers/mhdsecondordersolver.h); zero sub-cycles."""
+    """MHDSecondOrderSolver (Solvers/mhdsecondordersolver.h:58, 70), RK4 GLM
+    sub-cycles (MHDForcingSolver) included."""
 
     @staticmethod
     def solve(initial_cells, cell_spacing, cfl_coefficient, final_time, bias, slope_limiter, subcycling_iterations,
               material_parameters, **kw):
-        if subcycling_iterations:
-            raise NotImplementedError("the GLM source sub-cycles are not part of this build")
         return mhd_solve(initial_cells, 1, cell_spacing, cfl_coefficient, final_time, order=2, bias=bias,
-                         limiter=slope_limiter, params=material_parameters, **kw).cells
+                         limiter=slope_limiter, params=material_parameters, subcycles=subcycling_iterations,
+                         **kw).cells
 
     @staticmethod
     def solve2D(initial_cells, cell_spacing, cfl_coefficient, final_time, bias, slope_limiter, subcycling_iterations,
                 material_parameters, **kw):
-        if subcycling_iterations:
-            raise NotImplementedError("the GLM source sub-cycles are not part of this build")
         return mhd_solve(initial_cells, 2, cell_spacing, cfl_coefficient, final_time, order=2, bias=bias,
-                         limiter=slope_limiter, params=material_parameters, **kw).cells
+                         limiter=slope_limiter, params=material_parameters, subcycles=subcycling_iterations,
+                         **kw).cells
+
+
+class MHDFirstOrderSolver:
+    """MHDFirstOrderSolver (Solvers/mhdfirstordersolver.h): FORCE; never
+    sub-cycles and never sets c_h (the parameters' c_h is used)."""
+
+    @staticmethod
+    def solve(initial_cells, cell_spacing, cfl_coefficient, final_time, subcycling_iterations, material_parameters,
+              **kw):
+        return mhd_solve(initial_cells, 1, cell_spacing, cfl_coefficient, final_time, order=1,
+                         params=material_parameters, **kw).cells
+
+    @staticmethod
+    def solve2D(initial_cells, cell_spacing, cfl_coefficient, final_time, subcycling_iterations, material_parameters,
+                **kw):
+        return mhd_solve(initial_cells, 2, cell_spacing, cfl_coefficient, final_time, order=1,
+                         params=material_parameters, **kw).cells
 
 
 # ---- Godunov-Peshkov-Romenski (HPR) ----------------------------------------------
