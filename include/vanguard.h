@@ -290,6 +290,41 @@ int vg_rgfm_solve_batch_2d(vg_ctx *ctx, int n_inst, const double *m1_in, const d
                            const vg_euler_params *p1, const vg_euler_params *p2, int mode, int32_t *steps,
                            double *t_out, double *circulation, vg_run_stats *stats);
 
+/* Reverse-mode derivative of the HLLC RGFM solve (RGFMSolver::solveHLLC /
+ * solveHLLC2D, EXACT arithmetic) with the time steps held at the values the
+ * forward run chose.  No reference counterpart: the reference has no
+ * derivatives.  Runs the forward (outputs as vg_rgfm_solve_*, dt_log its
+ * steps), checkpointing both materials and the level set every
+ * `checkpoint_every` steps (0: ceil(sqrt(max_steps)), 64 for runs to t_final),
+ * then the adjoint backwards: per step the SLIC sweeps of both materials and
+ * the ghost fills.  seed1/seed2: derivative of a scalar loss with respect to
+ * the final states of materials 1 and 2 (array of structures, 6 doubles per
+ * cell); grad1/grad2 receive its derivative with respect to their initial
+ * states.  The level set only decides where the ghost fills write and carries
+ * no derivative; the slope limiter is differentiated piecewise as in
+ * vg_euler_adjoint_*, the HLLC interface state on the branch its comparisons
+ * select. */
+int vg_rgfm_adjoint_1d(vg_ctx *ctx, const double *m1_in, const double *m2_in, const double *phi_in, double *m1_out,
+                       double *m2_out, double *phi_out, const double *seed1, const double *seed2, double *grad1,
+                       double *grad2, int n, double dx, double cfl, double t_final, int max_steps, double bias,
+                       int limiter, const vg_euler_params *p1, const vg_euler_params *p2, int checkpoint_every,
+                       double *dt_log, int dt_log_cap, vg_run_stats *stats);
+int vg_rgfm_adjoint_2d(vg_ctx *ctx, const double *m1_in, const double *m2_in, const double *phi_in, double *m1_out,
+                       double *m2_out, double *phi_out, const double *seed1, const double *seed2, double *grad1,
+                       double *grad2, int ny, int nx, double dx, double dy, double cfl, double t_final, int max_steps,
+                       double bias, int limiter, const vg_euler_params *p1, const vg_euler_params *p2,
+                       int checkpoint_every, double *dt_log, int dt_log_cap, vg_run_stats *stats);
+/* The HLLC RGFM solve with prescribed time steps dts[0..n_steps) (EXACT
+ * arithmetic): the map the adjoint differentiates, for finite-difference checks. */
+int vg_rgfm_solve_fixed_dt_1d(vg_ctx *ctx, const double *m1_in, const double *m2_in, const double *phi_in,
+                              double *m1_out, double *m2_out, double *phi_out, int n, double dx, const double *dts,
+                              int n_steps, double bias, int limiter, const vg_euler_params *p1,
+                              const vg_euler_params *p2);
+int vg_rgfm_solve_fixed_dt_2d(vg_ctx *ctx, const double *m1_in, const double *m2_in, const double *phi_in,
+                              double *m1_out, double *m2_out, double *phi_out, int ny, int nx, double dx, double dy,
+                              const double *dts, int n_steps, double bias, int limiter, const vg_euler_params *p1,
+                              const vg_euler_params *p2);
+
 /* ---- Godunov-Peshkov-Romenski (HPR) continuum ---------------------------- */
 
 enum {

@@ -117,6 +117,14 @@ void orc_hllc_units(int n, const orc_euler_params *p1, const orc_euler_params *p
                     double *out);
 void orc_exact_units(int n, double s, const orc_euler_params *p1, const orc_euler_params *p2, const double *states,
                      double *out);
+/* HLLC RGFM solve with prescribed time steps and (t1_in not NULL) the
+ * forward-mode derivative of both final material states along (t1_in, t2_in),
+ * by dual numbers (euler_dual.cpp); the level set is not differentiated.
+ * orc_euler_branch_hash() afterwards hashes every branch decision taken. */
+void orc_rgfm_fixed_dt_jvp(int dim, int ny, int nx, double dx, double dy, const double *dts, int n_steps, double bias,
+                           int limiter, const orc_euler_params *p1, const orc_euler_params *p2, const double *m1_in,
+                           const double *m2_in, const double *phi_in, const double *t1_in, const double *t2_in,
+                           double *m1_out, double *m2_out, double *phi_out, double *t1_out, double *t2_out);
 /* Interface circulation over phi < 0 from material 1's velocity. */
 double orc_rgfm_circulation(int ny, int nx, double dx, double dy, const double *m1, const double *phi);
 

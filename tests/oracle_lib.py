@@ -77,6 +77,10 @@ def load():
     lib.orc_euler_fixed_dt_jvp.restype = None
     lib.orc_euler_fixed_dt_jvp.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, dptr, C.c_int, C.c_double, C.c_int,
                                            C.c_int, C.POINTER(OrcEulerParams), dptr, dptr, dptr, dptr]
+    lib.orc_rgfm_fixed_dt_jvp.restype = None
+    lib.orc_rgfm_fixed_dt_jvp.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, dptr, C.c_int, C.c_double,
+                                          C.c_int, C.POINTER(OrcEulerParams), C.POINTER(OrcEulerParams), dptr, dptr,
+                                          dptr, dptr, dptr, dptr, dptr, dptr, dptr, dptr]
     lib.orc_euler_branch_hash.restype = C.c_uint64
     lib.orc_euler_branch_hash.argtypes = []
     lib.orc_rgfm_circulation.restype = C.c_double
@@ -248,6 +252,35 @@ def euler_fixed_dt(prim, dts, *, dim, dx, bias, limiter, params, bc=0, tangent=N
     lib.orc_euler_fixed_dt_jvp(dim, ny, nx, dx, _p(d), d.size, bias, limiter, bc, C.byref(mp), _p(w), _p(tv),
                                _p(out), _p(tout))
     return out, tout
+
+
+def rgfm_fixed_dt(m1, m2, phi, dts, *, dim, dx, bias, limiter, p1, p2, dy=None, tangent=None):
+    """HLLC RGFM solve with prescribed time steps; with tangent = (t1, t2), also
+    the forward-mode derivative of both final material states along it.
+    Returns (m1, m2, phi[, t1, t2]) and sets the branch hash."""
+    lib = load()
+    a1 = np.ascontiguousarray(m1, dtype=np.float64)
+    a2 = np.ascontiguousarray(m2, dtype=np.float64)
+    ph = np.ascontiguousarray(phi, dtype=np.float64)
+    d = np.ascontiguousarray(dts, dtype=np.float64)
+    o1, o2, oph = np.empty_like(a1), np.empty_like(a2), np.empty_like(ph)
+    ny, nx = (1, ph.shape[0]) if dim == 1 else ph.shape
+    q1, q2 = OrcEulerParams(*p1), OrcEulerParams(*p2)
+    dyv = dx if dy is None else dy
+    if tangent is None:
+        lib.orc_rgfm_fixed_dt_jvp(dim, ny, nx, dx, dyv, _p(d), d.size, bias, limiter, C.byref(q1), C.byref(q2), _p(a1),
+                                  _p(a2), _p(ph), None, None, _p(o1), _p(o2), _p(oph), None, None)
+        return o1, o2, oph
+    t1 = np.ascontiguousarray(tangent[0], dtype=np.float64)
+    t2 = np.ascontiguousarray(tangent[1], dtype=np.float64)
+    u1, u2 = np.empty_like(a1), np.empty_like(a2)
+    lib.orc_rgfm_fixed_dt_jvp(dim, ny, nx, dx, dyv, _p(d), d.size, bias, limiter, C.byref(q1), C.byref(q2), _p(a1),
+                              _p(a2), _p(ph), _p(t1), _p(t2), _p(o1), _p(o2), _p(oph), _p(u1), _p(u2))
+    return o1, o2, oph, u1, u2
+
+
+def branch_hash():
+    return int(load().orc_euler_branch_hash())
 
 
 def euler_branch_hash(prim, dts, **kw):

@@ -213,3 +213,148 @@ def test_tangent_is_linear_in_the_direction():
     j2 = O.euler_fixed_dt(w, dts, tangent=v2, **kw)[1]
     j12 = O.euler_fixed_dt(w, dts, tangent=2.0 * v1 - 3.0 * v2, **kw)[1]
     np.testing.assert_allclose(j12, 2.0 * j1 - 3.0 * j2, rtol=1e-9, atol=1e-9 * np.abs(j12).max())
+
+
+# ---- the RGFM dual restatement (orc_rgfm_fixed_dt_jvp) -------------------------------
+
+def rgfm_case(dim, mach=1.3, tie=True):
+    """Small shock-bubble (2D) or shock-interface (1D) problems.  With tie the
+    interface is as in the benchmark, helium and air at rest at equal pressure:
+    its Riemann problem has star velocity exactly 0, where the choice between
+    the two star states is a kink.  Without, helium moves and is slightly
+    compressed, so every interface solve stays on one side."""
+    from vanguard_amd import problems
+    if dim == 2:
+        m1, m2, phi, dx, dy = problems.shock_bubble_2d(40, 20, radius=0.03, mach=mach)
+        if not tie:
+            m1[..., 1] = 7.0
+            m1[..., 4] = 1.03e5
+        return m1, m2, phi, dx, dy
+    n = 80
+    x = (np.arange(n) + 0.5) / n
+    m1 = np.zeros((n, 6))
+    m1[:, 0], m1[:, 4], m1[:, 5] = 0.138, 1e5, 1.0
+    m2 = np.zeros((n, 6))
+    m2[:, 0], m2[:, 4], m2[:, 5] = 1.0, 1e5, 1.0
+    rho1, u1, p1 = problems.post_shock_state(1.0, 1e5, 1.4, mach)
+    behind = x > 0.7
+    m2[behind, 0], m2[behind, 1], m2[behind, 4] = rho1, -u1, p1
+    phi = x - 0.45
+    if not tie:
+        m1[:, 1] = 7.0
+        m1[:, 4] = 1.03e5
+    return m1, m2, phi, 1.0 / n, 1.0 / n
+
+
+def test_rgfm_dual_primal_is_the_oracle_solve():
+    from vanguard_amd.problems import AIR, HELIUM
+    for dim in (1, 2):
+        m1, m2, phi, dx, dy = rgfm_case(dim)
+        kw = dict(dim=dim, dx=dx, dy=dy, bias=0.0, limiter=0, p1=HELIUM, p2=AIR)
+        r1, r2, rph, dts = O.rgfm_solve(m1, m2, phi, cfl=0.8, tfinal=math.inf, max_steps=25, **kw)
+        assert dts.size == 25
+        o1, o2, oph = O.rgfm_fixed_dt(m1, m2, phi, dts, **kw)
+        O.assert_bitwise(o1, r1, f"{dim}D material 1")
+        O.assert_bitwise(o2, r2, f"{dim}D material 2")
+        O.assert_bitwise(oph, rph, f"{dim}D level set")
+        d1, d2, dph, _, _ = O.rgfm_fixed_dt(m1, m2, phi, dts, tangent=(np.zeros_like(m1), np.zeros_like(m2)), **kw)
+        O.assert_bitwise(d1, r1, "dual primal 1")
+        O.assert_bitwise(d2, r2, "dual primal 2")
+
+
+def rgfm_parameter_directions(dim, mach=1.3, tie=False):
+    """Directions that move a whole uniform region (the parameters of the
+    problem): helium density, helium pressure, air density ahead of the shock,
+    and the Mach number (d initial state / d M, by differences of the exact
+    initial condition).  Uniform regions keep their zero slopes exactly zero,
+    so these stay inside the reference's absolute 1e-5 slope clamp, where
+    random per-cell directions cannot (pressures are about 1e5)."""
+    m1, m2, phi, dx, dy = rgfm_case(dim, mach, tie=tie)
+    ahead = m2[..., 0] <= 1.0
+    dirs = []
+    v1 = np.zeros_like(m1)
+    v1[..., 0] = 0.138
+    dirs.append(("helium density", v1, np.zeros_like(m2)))
+    v1 = np.zeros_like(m1)
+    v1[..., 4] = 1e5
+    dirs.append(("helium pressure", v1, np.zeros_like(m2)))
+    v2 = np.zeros_like(m2)
+    v2[ahead, 0] = 1.0
+    dirs.append(("air density ahead", np.zeros_like(m1), v2))
+    a, b = rgfm_case(dim, mach + 1e-6, tie=tie), rgfm_case(dim, mach - 1e-6, tie=tie)
+    dirs.append(("mach", (a[0] - b[0]) / 2e-6, (a[1] - b[1]) / 2e-6))
+    return (m1, m2, phi, dx, dy), dirs
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_rgfm_dual_tangent_matches_central_differences(dim):
+    """Along each parameter direction, with the branch hashes of w - hv, w and
+    w + hv equal (limiters, clamps, HLLC branches, crossings, level-set
+    toggles), the tangent matches central differences field by field."""
+    from vanguard_amd.problems import AIR, HELIUM
+    (m1, m2, phi, dx, dy), dirs = rgfm_parameter_directions(dim)
+    kw = dict(dim=dim, dx=dx, dy=dy, bias=0.0, limiter=0, p1=HELIUM, p2=AIR)
+    dts = O.rgfm_solve(m1, m2, phi, cfl=0.8, tfinal=math.inf, max_steps=20, **kw)[3]
+    O.rgfm_fixed_dt(m1, m2, phi, dts, **kw)
+    h0, used = O.branch_hash(), 0
+    for name, v1, v2 in dirs:
+        for h in (1e-6, 1e-7, 1e-8):   # the largest step that keeps every branch
+            p = O.rgfm_fixed_dt(m1 + h * v1, m2 + h * v2, phi, dts, **kw)
+            hp = O.branch_hash()
+            q = O.rgfm_fixed_dt(m1 - h * v1, m2 - h * v2, phi, dts, **kw)
+            if hp == h0 == O.branch_hash():
+                break
+        else:
+            continue
+        _, _, _, t1, t2 = O.rgfm_fixed_dt(m1, m2, phi, dts, tangent=(v1, v2), **kw)
+        for f in range(6):
+            ref = max(np.abs(t1[..., f]).max(), np.abs(t2[..., f]).max())
+            if ref == 0.0:
+                continue
+            err = max(np.abs((p[0] - q[0])[..., f] / (2 * h) - t1[..., f]).max(),
+                      np.abs((p[1] - q[1])[..., f] / (2 * h) - t2[..., f]).max())
+            assert err <= 2e-6 * ref, (name, h, f, err / ref)
+        used += 1
+    assert used >= 3
+
+
+def test_rgfm_interface_tie_is_seen():
+    """At rest and at equal pressure the interface Riemann problem has star
+    velocity exactly 0, a kink (left or right star state): the hash must see
+    a pressure perturbation of helium flip it."""
+    from vanguard_amd.problems import AIR, HELIUM
+    m1, m2, phi, dx, dy = rgfm_case(1, tie=True)
+    kw = dict(dim=1, dx=dx, dy=dy, bias=0.0, limiter=0, p1=HELIUM, p2=AIR)
+    dts = O.rgfm_solve(m1, m2, phi, cfl=0.8, tfinal=math.inf, max_steps=5, **kw)[3]
+    O.rgfm_fixed_dt(m1, m2, phi, dts, **kw)
+    h0 = O.branch_hash()
+    v1 = np.zeros_like(m1)
+    v1[..., 4] = 1.0
+    O.rgfm_fixed_dt(m1 + 1e-3 * v1, m2, phi, dts, **kw)
+    hp = O.branch_hash()
+    O.rgfm_fixed_dt(m1 - 1e-3 * v1, m2, phi, dts, **kw)
+    assert hp != h0 or O.branch_hash() != h0
+
+
+def test_rgfm_dual_mach_derivative_matches_central_differences():
+    """The tangent along d(initial state)/d(Mach) against central differences of
+    the whole run in the Mach number, time steps frozen."""
+    from vanguard_amd import problems
+    from vanguard_amd.problems import AIR, HELIUM
+    M, h = 1.3, 1e-7
+    m1, m2, phi, dx, dy = rgfm_case(2, M)
+    kw = dict(dim=2, dx=dx, dy=dy, bias=0.0, limiter=0, p1=HELIUM, p2=AIR)
+    dts = O.rgfm_solve(m1, m2, phi, cfl=0.8, tfinal=math.inf, max_steps=20, **kw)[3]
+    a = rgfm_case(2, M + h)
+    b = rgfm_case(2, M - h)
+    t2 = (a[1] - b[1]) / (2 * h)       # the initial condition is smooth in M
+    O.rgfm_fixed_dt(m1, m2, phi, dts, **kw)
+    h0 = O.branch_hash()
+    pa = O.rgfm_fixed_dt(a[0], a[1], a[2], dts, **kw)
+    assert O.branch_hash() == h0
+    pb = O.rgfm_fixed_dt(b[0], b[1], b[2], dts, **kw)
+    assert O.branch_hash() == h0
+    _, _, _, u1, u2 = O.rgfm_fixed_dt(m1, m2, phi, dts, tangent=(np.zeros_like(m1), t2), **kw)
+    for k, u in ((0, u1), (1, u2)):
+        fd = (pa[k] - pb[k]) / (2 * h)
+        assert np.abs(fd - u).max() <= 1e-5 * np.abs(u).max(), k

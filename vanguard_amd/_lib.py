@@ -129,6 +129,20 @@ SIGNATURES = {
                                          C.c_double, C.c_double, C.c_double, dptr, C.c_int, C.c_double, C.c_int,
                                          C.c_int, C.POINTER(EulerParams), C.POINTER(EulerParams), C.c_int,
                                          C.POINTER(C.c_int32), dptr, dptr, C.POINTER(RunStats)]),
+    "vg_rgfm_adjoint_1d": (C.c_int, [vptr, dptr, dptr, dptr, dptr, dptr, dptr, dptr, dptr, dptr, dptr, C.c_int,
+                                     C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int,
+                                     C.POINTER(EulerParams), C.POINTER(EulerParams), C.c_int, dptr, C.c_int,
+                                     C.POINTER(RunStats)]),
+    "vg_rgfm_adjoint_2d": (C.c_int, [vptr, dptr, dptr, dptr, dptr, dptr, dptr, dptr, dptr, dptr, dptr, C.c_int,
+                                     C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double,
+                                     C.c_int, C.POINTER(EulerParams), C.POINTER(EulerParams), C.c_int, dptr,
+                                     C.c_int, C.POINTER(RunStats)]),
+    "vg_rgfm_solve_fixed_dt_1d": (C.c_int, [vptr, dptr, dptr, dptr, dptr, dptr, dptr, C.c_int, C.c_double, dptr,
+                                            C.c_int, C.c_double, C.c_int, C.POINTER(EulerParams),
+                                            C.POINTER(EulerParams)]),
+    "vg_rgfm_solve_fixed_dt_2d": (C.c_int, [vptr, dptr, dptr, dptr, dptr, dptr, dptr, C.c_int, C.c_int, C.c_double,
+                                            C.c_double, dptr, C.c_int, C.c_double, C.c_int, C.POINTER(EulerParams),
+                                            C.POINTER(EulerParams)]),
     "vg_hpr_solve_1d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                                   C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(HPRParams), C.c_int, dptr,
                                   C.c_int, C.POINTER(RunStats)]),

@@ -463,27 +463,4 @@ __global__ void euler_adjoint_fold_kernel(double *wbar_in, const double *ghost, 
     }
 }
 
-// Clock step with a prescribed time step: dts[it] instead of the CFL formula
-// (the forward of a frozen-dt derivative check, and recomputation in the
-// adjoint).  Same loop test and t / it bookkeeping as clock_begin_kernel.
-static __global__ void clock_fixed_kernel(Clock *c, const double *dts, int n_dts)
-{
-    if (c->done) return;
-    if (c->pending) {
-        c->t += c->dt;
-        c->it += 1;
-        c->pending = 0;
-    }
-    if (c->it >= n_dts || (c->max_steps > 0 && c->it >= c->max_steps)) {
-        c->done = 1;
-        return;
-    }
-    const double dt = dts[c->it];
-    c->dt = dt;
-    if (c->steps_this_run == 0 || dt < c->dt_min) c->dt_min = dt;
-    if (c->steps_this_run == 0 || dt > c->dt_max) c->dt_max = dt;
-    c->steps_this_run += 1;
-    c->pending = 1;
-}
-
 }  // namespace vg

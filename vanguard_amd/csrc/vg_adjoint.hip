@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "vg_adjoint.h"
+#include "vg_adjoint_host.h"
 #include "vg_grid.h"
 #include "vg_phys_euler.h"
 
@@ -25,6 +26,10 @@ namespace {
 
 using vg::EulerPhys;
 using EGrid = vg::GridState<EulerPhys>;
+using vg::adj_sweep;
+using vg::adjoint_args;
+using vg::kAdjDefault;
+using vg::tune_adjoint;
 
 EulerPhys::Params to_params(const vg_euler_params *p)
 {
@@ -81,125 +86,6 @@ int fwd_sweep(vg_ctx *ctx, int dir, const double *in, double *out, long long pla
         return vg::launch_status("sweep_x");
     }
     return vg::launch_y<EulerPhys, 2, MODE>(ctx, y_variant, a);
-}
-
-template <int DIR, int BS, int LW, int WPE>
-void launch_adjoint_tile(vg_ctx *ctx, const vg::AdjointArgs &a)
-{
-    const int n = (DIR == 0) ? a.nx : a.ny, lines = (DIR == 0) ? a.ny : a.nx;
-    using T = vg::AdjTile<BS, LW>;
-    dim3 grid((n + 4 + T::out - 1) / T::out, (lines + LW - 1) / LW);
-    vg::euler_adjoint_sweep_kernel<DIR, BS, LW, WPE><<<grid, BS, 0, ctx->stream>>>(a);
-}
-
-// Tile variants per direction (threads, lines per block, waves per SIMD).  X:
-// one line per block; Y: LW adjacent columns per block, so each row read is LW
-// consecutive doubles.
-constexpr int kAdjVariants = 4;
-constexpr int kAdjDefault[2] = {0, 0};
-
-void launch_adjoint_dir(vg_ctx *ctx, int dir, int variant, const vg::AdjointArgs &a)
-{
-    if (dir == 0) {
-        switch (variant) {
-        case 1: launch_adjoint_tile<0, 128, 1, 3>(ctx, a); break;
-        case 2: launch_adjoint_tile<0, 256, 1, 2>(ctx, a); break;
-        case 3: launch_adjoint_tile<0, 64, 1, 2>(ctx, a); break;
-        default: launch_adjoint_tile<0, 128, 1, 2>(ctx, a); break;
-        }
-        return;
-    }
-    switch (variant) {
-    case 1: launch_adjoint_tile<1, 256, 8, 3>(ctx, a); break;
-    case 2: launch_adjoint_tile<1, 256, 16, 2>(ctx, a); break;
-    case 3: launch_adjoint_tile<1, 256, 4, 2>(ctx, a); break;
-    default: launch_adjoint_tile<1, 256, 8, 2>(ctx, a); break;
-    }
-}
-
-vg::AdjointArgs adjoint_args(const double *w, const double *wbar_out, double *wbar_in, double *ghost,
-                             long long plane, int ny, int nx, int bc, double h, double bias, int limiter, double ts,
-                             const EulerPhys::Params &m)
-{
-    vg::AdjointArgs a;
-    a.w = w;
-    a.wbar_out = wbar_out;
-    a.wbar_in = wbar_in;
-    a.ghost = ghost;
-    a.plane = plane;
-    a.ny = ny;
-    a.nx = nx;
-    a.bc = bc;
-    a.limiter = limiter;
-    a.dx = h;
-    a.bias = bias;
-    a.ts = ts;
-    a.m = m;
-    return a;
-}
-
-// One adjoint sweep: wbar_in = (d sweep / d w)^T wbar_out at input w.  Every
-// tile variant computes each output with the same arithmetic (the halo is
-// recomputed, not shared), so the variant changes nothing in the result.
-int adj_sweep(vg_ctx *ctx, int dir, int variant, const double *w, const double *wbar_out, double *wbar_in,
-              double *ghost, long long plane, int ny, int nx, int bc, double h, double bias, int limiter, double ts,
-              const EulerPhys::Params &m)
-{
-    const vg::AdjointArgs a = adjoint_args(w, wbar_out, wbar_in, ghost, plane, ny, nx, bc, h, bias, limiter, ts, m);
-    const int lines = (dir == 0) ? ny : nx;
-    return ctx->timed(dir == 0 ? "adjoint_sweep_x" : "adjoint_sweep_y", [&] {
-        launch_adjoint_dir(ctx, dir, variant, a);
-        if (dir == 0)
-            vg::euler_adjoint_fold_kernel<0><<<(lines + 255) / 256, 256, 0, ctx->stream>>>(wbar_in, ghost, plane, ny,
-                                                                                            nx, bc);
-        else
-            vg::euler_adjoint_fold_kernel<1><<<(lines + 255) / 256, 256, 0, ctx->stream>>>(wbar_in, ghost, plane, ny,
-                                                                                            nx, bc);
-    });
-}
-
-std::string adjoint_key(int dir, int ny, int nx)
-{
-    return std::string(dir == 0 ? "euler_adjoint_x/" : "euler_adjoint_y/") + std::to_string(ny) + "/" +
-           std::to_string(nx);
-}
-
-// Tile variant of one direction for an ny x nx grid: the context's table, else
-// each variant runs once to warm up and twice timed on the caller's buffers
-// (the real sweep overwrites what they write), and the fastest is recorded.
-int tune_adjoint(vg_ctx *ctx, int dir, const vg::AdjointArgs &a, int &variant)
-{
-    const std::string key = adjoint_key(dir, a.ny, a.nx);
-    auto it = ctx->tuned.find(key);
-    if (it != ctx->tuned.end()) {
-        variant = it->second;
-        return VG_OK;
-    }
-    variant = kAdjDefault[dir];
-    if (!ctx->autotune || ctx->timing) return VG_OK;
-    hipEvent_t e0, e1;
-    VG_HIP(hipEventCreate(&e0));
-    VG_HIP(hipEventCreate(&e1));
-    float best = 0.0f;
-    for (int v = 0; v < kAdjVariants; v++) {
-        launch_adjoint_dir(ctx, dir, v, a);
-        VG_HIP(hipEventRecord(e0, ctx->stream));
-        for (int r = 0; r < 2; r++) launch_adjoint_dir(ctx, dir, v, a);
-        VG_HIP(hipEventRecord(e1, ctx->stream));
-        VG_HIP(hipEventSynchronize(e1));
-        int rc = vg::launch_status("adjoint autotune");
-        if (rc) return rc;
-        float ms = 0.0f;
-        VG_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (v == 0 || ms < best) {
-            best = ms;
-            variant = v;
-        }
-    }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    ctx->tuned[key] = variant;
-    return VG_OK;
 }
 
 struct DeviceBuffers {

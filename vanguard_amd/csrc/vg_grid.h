@@ -19,6 +19,29 @@
 
 namespace vg {
 
+// Clock step with a prescribed time step: dts[it] instead of the CFL formula
+// (the forward of a frozen-dt derivative check, and recomputation in the
+// adjoint).  Same loop test and t / it bookkeeping as clock_begin_kernel.
+static __global__ void clock_fixed_kernel(Clock *c, const double *dts, int n_dts)
+{
+    if (c->done) return;
+    if (c->pending) {
+        c->t += c->dt;
+        c->it += 1;
+        c->pending = 0;
+    }
+    if (c->it >= n_dts || (c->max_steps > 0 && c->it >= c->max_steps)) {
+        c->done = 1;
+        return;
+    }
+    const double dt = dts[c->it];
+    c->dt = dt;
+    if (c->steps_this_run == 0 || dt < c->dt_min) c->dt_min = dt;
+    if (c->steps_this_run == 0 || dt > c->dt_max) c->dt_max = dt;
+    c->steps_this_run += 1;
+    c->pending = 1;
+}
+
 // Optional per-step source stage (the HPR relaxation and MHD GLM damping
 // sub-cycles): launched `subcycles` times before and after the sweeps of every
 // second-order step, each launch reading one buffer and writing the other.  The

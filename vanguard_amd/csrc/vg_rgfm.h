@@ -25,81 +25,164 @@ typedef double State[6];
 
 __device__ __forceinline__ double snd(const State &w, const EP &m) { return EulerPhys::eos_c(m, w[0], w[4]); }
 
+// ---- scalar types of the interface solver ------------------------------------------
+// The HLLC interface state is written once over a scalar type T: double for the
+// fill (every operation exactly as before, the EOS and conversions being
+// EulerPhys's own), and a forward-mode dual number for the adjoint of the fill,
+// whose branch tests read the value part (the same values as the double run).
+
+struct Dual {
+    double v, d;
+};
+using ::pow;
+using ::sqrt;
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ Dual operator/(Dual a, Dual b) { return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
+__device__ __forceinline__ Dual operator+(Dual a, double b) { return {a.v + b, a.d}; }
+__device__ __forceinline__ Dual operator+(double a, Dual b) { return {a + b.v, b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, double b) { return {a.v - b, a.d}; }
+__device__ __forceinline__ Dual operator-(double a, Dual b) { return {a - b.v, -b.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, double b) { return {a.v * b, a.d * b}; }
+__device__ __forceinline__ Dual operator*(double a, Dual b) { return {a * b.v, a * b.d}; }
+__device__ __forceinline__ Dual operator/(Dual a, double b) { return {a.v / b, a.d / b}; }
+__device__ __forceinline__ Dual operator/(double a, Dual b) { return {a / b.v, -a * b.d / (b.v * b.v)}; }
+__device__ __forceinline__ Dual sqrt(Dual a)
+{
+    const double r = sqrt(a.v);
+    return {r, a.d / (2.0 * r)};
+}
+__device__ __forceinline__ Dual pow(Dual a, double e)
+{
+    const double r = pow(a.v, e);
+    return {r, e * pow(a.v, e - 1.0) * a.d};
+}
+__device__ __forceinline__ double val(double x) { return x; }
+__device__ __forceinline__ double val(Dual x) { return x.v; }
+// A constant of the scalar type of `like`.
+__device__ __forceinline__ double t_const(double c, double) { return c; }
+__device__ __forceinline__ Dual t_const(double c, Dual) { return {c, 0.0}; }
+template <class T>
+__device__ __forceinline__ T tmin(T a, T b) { return (val(b) < val(a)) ? b : a; }
+template <class T>
+__device__ __forceinline__ T tmax(T a, T b) { return (val(a) < val(b)) ? b : a; }
+
+__device__ __forceinline__ double t_snd(const double (&w)[6], const EP &m) { return EulerPhys::eos_c(m, w[0], w[4]); }
+__device__ __forceinline__ Dual t_snd(const Dual (&w)[6], const EP &m)
+{
+    return sqrt((m.gamma * (w[4] + m.p_inf)) / w[0]);
+}
+__device__ __forceinline__ void t_p2c(const EP &m, const double (&w)[6], double (&U)[6])
+{
+    EulerPhys::prim_to_cons(m, w, U);
+}
+__device__ __forceinline__ void t_p2c(const EP &m, const Dual (&w)[6], Dual (&U)[6])
+{
+    const Dual rho = w[0], u = w[1], v = w[2], z = w[3], p = w[4], lam = w[5];
+    const Dual vsq = (u * u) + (v * v) + (z * z);
+    U[0] = rho;
+    U[1] = rho * u;
+    U[2] = rho * v;
+    U[3] = rho * z;
+    U[4] = rho * ((0.5 * vsq) + ((p + m.gpinf) / (m.gm1 * rho)) + (m.e_form * (lam - 1.0)));
+    U[5] = rho * lam;
+}
+__device__ __forceinline__ void t_c2p(const EP &m, const double (&U)[6], double (&w)[6])
+{
+    EulerPhys::cons_to_prim(m, U, w);
+}
+__device__ __forceinline__ void t_c2p(const EP &m, const Dual (&U)[6], Dual (&w)[6])
+{
+    const Dual rho = U[0];
+    const Dual u = U[1] / rho, v = U[2] / rho, z = U[3] / rho;
+    const Dual lam = U[5] / rho;
+    const Dual vsq = (u * u) + (v * v) + (z * z);
+    const Dual e = (U[4] / rho) - (0.5 * vsq) - (m.e_form * (lam - 1.0));
+    w[0] = rho;
+    w[1] = u;
+    w[2] = v;
+    w[3] = z;
+    w[4] = (e * m.gm1 * rho) - m.gpinf;
+    w[5] = lam;
+}
+
 // ---- HLLC (hllcsolver.cpp) ---------------------------------------------------
 
 // computeStarRegionDensity (hllcsolver.cpp:7-10)
-__device__ __forceinline__ double hllc_star_rho(double rho, double S, double u, double Ss)
+template <class T>
+__device__ __forceinline__ T hllc_star_rho(T rho, T S, T u, T Ss)
 {
     return rho * ((S - u) / (S - Ss));
 }
 
 // PVRS (:182-254); the product 0.25 (rho_a + rho_b)(c_a + c_b).
-template <int DIR>
-__device__ __forceinline__ double pvrs_prod(const State &a, const State &b, const EP &m1, const EP &m2)
+template <int DIR, class T>
+__device__ __forceinline__ T pvrs_prod(const T (&a)[6], const T (&b)[6], const EP &m1, const EP &m2)
 {
-    return 0.25 * (a[0] + b[0]) * (snd(a, m1) + snd(b, m2));
+    return 0.25 * (a[0] + b[0]) * (t_snd(a, m1) + t_snd(b, m2));
 }
-template <int DIR>
-__device__ __forceinline__ double pvrs_vel(const State &a, const State &b, const EP &m1, const EP &m2)
+template <int DIR, class T>
+__device__ __forceinline__ T pvrs_vel(const T (&a)[6], const T (&b)[6], const EP &m1, const EP &m2)
 {
-    const double prod = pvrs_prod<DIR>(a, b, m1, m2);
+    const T prod = pvrs_prod<DIR>(a, b, m1, m2);
     return (0.5 * (a[1 + DIR] + b[1 + DIR])) + (0.5 * (a[4] - b[4]) / prod);
 }
-template <int DIR>
-__device__ __forceinline__ double pvrs_pres(const State &a, const State &b, const EP &m1, const EP &m2)
+template <int DIR, class T>
+__device__ __forceinline__ T pvrs_pres(const T (&a)[6], const T (&b)[6], const EP &m1, const EP &m2)
 {
-    const double prod = pvrs_prod<DIR>(a, b, m1, m2);
+    const T prod = pvrs_prod<DIR>(a, b, m1, m2);
     return (0.5 * (a[4] + b[4])) + (0.5 * (a[1 + DIR] - b[1 + DIR]) * prod);
 }
 
 // Two-rarefaction estimates (:256-348), equal-gamma materials only.
-template <int DIR>
-__device__ __forceinline__ double two_rare_vel(const State &a, const State &b, const EP &m1, const EP &m2)
+template <int DIR, class T>
+__device__ __forceinline__ T two_rare_vel(const T (&a)[6], const T (&b)[6], const EP &m1, const EP &m2)
 {
-    const double ca = snd(a, m1), cb = snd(b, m2);
+    const T ca = t_snd(a, m1), cb = t_snd(b, m2);
     const double g = 0.5 * (m1.gamma + m2.gamma);
     const double ex = (g - 1.0) / (2.0 * g);
-    const double pr = pow((a[4] / b[4]), ex);
+    const T pr = pow((a[4] / b[4]), ex);
     const double co = 2.0 / (g - 1.0);
     return ((pr * (a[1 + DIR] / ca)) + (b[1 + DIR] / cb) + (co * (pr - 1.0))) / ((pr / ca) + (1.0 / cb));
 }
-template <int DIR>
-__device__ __forceinline__ double two_rare_pres(const State &a, const State &b, const EP &m1, const EP &m2)
+template <int DIR, class T>
+__device__ __forceinline__ T two_rare_pres(const T (&a)[6], const T (&b)[6], const EP &m1, const EP &m2)
 {
-    const double ca = snd(a, m1), cb = snd(b, m2);
+    const T ca = t_snd(a, m1), cb = t_snd(b, m2);
     const double g = 0.5 * (m1.gamma + m2.gamma);
     const double co = 0.5 * (g - 1.0);
-    const double us = two_rare_vel<DIR>(a, b, m1, m2);
+    const T us = two_rare_vel<DIR>(a, b, m1, m2);
     const double ex = 2.0 * (g / (g - 1.0));
-    const double ld = 1.0 + (co * ((a[1 + DIR] - us) / ca));
-    const double rd = 1.0 + (co * ((us - b[1 + DIR]) / cb));
+    const T ld = 1.0 + (co * ((a[1 + DIR] - us) / ca));
+    const T rd = 1.0 + (co * ((us - b[1 + DIR]) / cb));
     return 0.5 * ((a[4] * pow(ld, ex)) + (b[4] * pow(rd, ex)));
 }
 
 // Two-shock estimates (:350-462).
-template <int DIR>
-__device__ __forceinline__ void two_shock_coeffs(const State &a, const State &b, const EP &m1, const EP &m2, double &ga,
-                                                 double &gb)
+template <int DIR, class T>
+__device__ __forceinline__ void two_shock_coeffs(const T (&a)[6], const T (&b)[6], const EP &m1, const EP &m2, T &ga,
+                                                 T &gb)
 {
     const double g = 0.5 * (m1.gamma + m2.gamma);
-    const double ppv = pvrs_pres<DIR>(a, b, m1, m2);
+    const T ppv = pvrs_pres<DIR>(a, b, m1, m2);
     const double A = 2.0 / (g + 1.0);
     const double B = (g - 1.0) / (g + 1.0);
     ga = sqrt((A / a[0]) / ((B * a[4]) + ppv));
     gb = sqrt((A / b[0]) / ((B * b[4]) + ppv));
 }
-template <int DIR>
-__device__ __forceinline__ double two_shock_pres(const State &a, const State &b, const EP &m1, const EP &m2)
+template <int DIR, class T>
+__device__ __forceinline__ T two_shock_pres(const T (&a)[6], const T (&b)[6], const EP &m1, const EP &m2)
 {
-    double ga, gb;
+    T ga, gb;
     two_shock_coeffs<DIR>(a, b, m1, m2, ga, gb);
     return ((ga * a[4]) + (gb * b[4]) - (b[1 + DIR] - a[1 + DIR])) / (ga + gb);
 }
-template <int DIR>
-__device__ __forceinline__ double two_shock_vel(const State &a, const State &b, const EP &m1, const EP &m2)
+template <int DIR, class T>
+__device__ __forceinline__ T two_shock_vel(const T (&a)[6], const T (&b)[6], const EP &m1, const EP &m2)
 {
-    const double ps = two_shock_pres<DIR>(a, b, m1, m2);
-    double ga, gb;
+    const T ps = two_shock_pres<DIR>(a, b, m1, m2);
+    T ga, gb;
     two_shock_coeffs<DIR>(a, b, m1, m2, ga, gb);
     return (0.5 * (a[1 + DIR] + b[1 + DIR])) + (0.5 * ((gb * (ps - b[4])) - (ga * (ps - a[4]))));
 }
@@ -114,72 +197,73 @@ __device__ __forceinline__ bool use_pvrs(double pa, double pb, double ppv, const
     return ((ratio <= 2.0) && (pmin <= ppv) && (ppv <= pmax)) || fabs(m1.gamma - m2.gamma) > 1e-8 || m1.p_inf > 0.0 ||
            m2.p_inf > 0.0;
 }
-template <int DIR>
-__device__ __forceinline__ double hllc_star_vel(const State &a, const State &b, const EP &m1, const EP &m2)
+template <int DIR, class T>
+__device__ __forceinline__ T hllc_star_vel(const T (&a)[6], const T (&b)[6], const EP &m1, const EP &m2)
 {
-    const double ppv = (DIR == 0) ? pvrs_pres<0>(a, b, m1, m2) : pvrs_vel<1>(a, b, m1, m2);
-    if (use_pvrs(a[4], b[4], ppv, m1, m2)) return pvrs_vel<DIR>(a, b, m1, m2);
-    if (ppv < smin(a[4], b[4])) return two_rare_vel<DIR>(a, b, m1, m2);
+    const T ppv = (DIR == 0) ? pvrs_pres<0>(a, b, m1, m2) : pvrs_vel<1>(a, b, m1, m2);
+    if (use_pvrs(val(a[4]), val(b[4]), val(ppv), m1, m2)) return pvrs_vel<DIR>(a, b, m1, m2);
+    if (val(ppv) < smin(val(a[4]), val(b[4]))) return two_rare_vel<DIR>(a, b, m1, m2);
     return two_shock_vel<DIR>(a, b, m1, m2);
 }
-template <int DIR>
-__device__ __forceinline__ double hllc_star_pres(const State &a, const State &b, const EP &m1, const EP &m2)
+template <int DIR, class T>
+__device__ __forceinline__ T hllc_star_pres(const T (&a)[6], const T (&b)[6], const EP &m1, const EP &m2)
 {
-    const double ppv = pvrs_pres<DIR>(a, b, m1, m2);
-    if (use_pvrs(a[4], b[4], ppv, m1, m2)) return ppv;
-    if (ppv < smin(a[4], b[4])) return two_rare_pres<DIR>(a, b, m1, m2);
+    const T ppv = pvrs_pres<DIR>(a, b, m1, m2);
+    if (use_pvrs(val(a[4]), val(b[4]), val(ppv), m1, m2)) return ppv;
+    if (val(ppv) < smin(val(a[4]), val(b[4]))) return two_rare_pres<DIR>(a, b, m1, m2);
     return two_shock_pres<DIR>(a, b, m1, m2);
 }
 
 // computeWaveSpeedWeighting (:608-623).
-__device__ __forceinline__ double hllc_weighting(double ps, const State &w, const EP &m)
+template <class T>
+__device__ __forceinline__ T hllc_weighting(T ps, const T (&w)[6], const EP &m)
 {
-    const double p = w[4];
-    if (ps <= p) return 1.0;
+    const T p = w[4];
+    if (val(ps) <= val(p)) return t_const(1.0, p);
     return sqrt(1.0 + ((m.gamma + 1.0) / (2.0 * m.gamma)) * ((ps + m.p_inf) / (p + m.p_inf) - 1.0));
 }
 
 // The interface state of HLLCSolver::solveX/solveY (:114-180) -- only its
 // pressure and normal velocity are used -- and the two star densities
 // (computeLeft/Right/Top/BottomStarRegionDensity, :12-58).
-template <int DIR>
-__device__ __forceinline__ void hllc_interface(const State &a, const State &b, const EP &m1, const EP &m2, double &p,
-                                               double &u, double &ra, double &rb)
+template <int DIR, class T>
+__device__ __forceinline__ void hllc_interface(const T (&a)[6], const T (&b)[6], const EP &m1, const EP &m2, T &p,
+                                               T &u, T &ra, T &rb)
 {
-    const double Ss = hllc_star_vel<DIR>(a, b, m1, m2);
-    const double ps = hllc_star_pres<DIR>(a, b, m1, m2);
-    const double Sa = a[1 + DIR] - (hllc_weighting(ps, a, m1) * snd(a, m1));
-    const double Sb = b[1 + DIR] + (hllc_weighting(ps, b, m2) * snd(b, m2));
-    if (0.0 <= Sa) {
+    const T Ss = hllc_star_vel<DIR>(a, b, m1, m2);
+    const T ps = hllc_star_pres<DIR>(a, b, m1, m2);
+    const T Sa = a[1 + DIR] - (hllc_weighting(ps, a, m1) * t_snd(a, m1));
+    const T Sb = b[1 + DIR] + (hllc_weighting(ps, b, m2) * t_snd(b, m2));
+    if (0.0 <= val(Sa)) {
         p = a[4];
         u = a[1 + DIR];
-    } else if (Sb <= 0.0) {
+    } else if (val(Sb) <= 0.0) {
         p = b[4];
         u = b[1 + DIR];
     } else {
         // computeStarRegionX/YConservedVariableVector (:60-112), then
         // setConservedVariableVector with that side's material.
-        const bool left = (0.0 <= Ss);
-        const State &w = left ? a : b;
+        const bool left = (0.0 <= val(Ss));
+        const T(&w)[6] = left ? a : b;
         const EP &m = left ? m1 : m2;
-        const double S = left ? Sa : Sb;
-        const double rho = w[0], pw = w[4], un = w[1 + DIR];
-        double W[6];
+        const T S = left ? Sa : Sb;
+        const T rho = w[0], pw = w[4], un = w[1 + DIR];
+        T W[6];
 #pragma unroll
         for (int f = 0; f < 6; f++) W[f] = w[f];
-        double Uw[6];
-        EulerPhys::prim_to_cons(m, W, Uw);
-        const double rs = hllc_star_rho(rho, S, un, Ss);
-        const double et = (Ss - un) * (Ss + (pw / (rho * (S - un))));
-        double U[6];
+        T Uw[6];
+        t_p2c(m, W, Uw);
+        const T rs = hllc_star_rho(rho, S, un, Ss);
+        const T et = (Ss - un) * (Ss + (pw / (rho * (S - un))));
+        T U[6];
         U[0] = rs;
         U[1] = rs * (DIR == 0 ? Ss : w[1]);
         U[2] = rs * (DIR == 1 ? Ss : w[2]);
         U[3] = rs * w[3];
         U[4] = rs * ((Uw[4] / rho) + et);
         U[5] = rs;
-        double out[6];
-        EulerPhys::cons_to_prim(m, U, out);
+        T out[6];
+        t_c2p(m, U, out);
         p = out[4];
         u = out[1 + DIR];
     }

@@ -13,8 +13,10 @@
 #include <string>
 #include <vector>
 
+#include "vg_adjoint_host.h"
 #include "vg_grid.h"
 #include "vg_rgfm.h"
+#include "vg_rgfm_adjoint.h"
 #include "vg_runtime.h"
 
 namespace vg {
@@ -146,8 +148,23 @@ struct RGFMGrid {
         return a;
     }
 
+    // One ghost-fill pass, HLLC, in place on (a1, a2) with level set ph, under
+    // the clock c (which must not be done).
+    int fill_pass(int dir, double *a1, double *a2, const double *ph, const EP &p1, const EP &p2, Clock *c)
+    {
+        LineArgs la = line_args(VG_RIEMANN_HLLC, p1, p2);
+        la.m1 = a1;
+        la.m2 = a2;
+        la.phi = ph;
+        la.clk = c;
+        if (dir == 0) rgfm_fill_kernel<0><<<ny, kLineBS, 0, ctx->stream>>>(la);
+        else rgfm_fill_kernel<1><<<nx, kLineBS, 0, ctx->stream>>>(la);
+        return launch_status("rgfm fill (recomputation)");
+    }
+
     template <int MODE>
-    int sweep(int dir, const double *in, double *out, double scale, double h, double bias, int limiter, const EP &m)
+    int sweep(int dir, const double *in, double *out, double scale, double h, double bias, int limiter, const EP &m,
+              Clock *use_clk = nullptr)
     {
         SweepArgs<EulerPhys> a;
         a.in = in;
@@ -163,7 +180,7 @@ struct RGFMGrid {
         a.dim = dim;
         a.fuse_max = 0;
         a.m = m;
-        a.clk = clk;
+        a.clk = use_clk ? use_clk : clk;
         a.partials = nullptr;
         if (dir == 0) {
             dim3 grid((nx + (BSX - 4) - 1) / (BSX - 4), (ny + RX - 1) / RX);
@@ -174,9 +191,11 @@ struct RGFMGrid {
         return launch_y<EulerPhys, 2, MODE>(ctx, y_variant, a);
     }
 
+    // One step; with fixed_dts the time step is fixed_dts[it] (the clock's
+    // step counter) instead of the CFL formula.
     template <int MODE>
     int enqueue_step(int riemann, double dx, double dy, double bias, int limiter, const EP &p1, const EP &p2,
-                     double *dt_log, int dt_cap)
+                     double *dt_log, int dt_cap, const double *fixed_dts = nullptr, int n_fixed = 0)
     {
         hipStream_t s = ctx->stream;
         clock_check_kernel<<<1, 1, 0, s>>>(clk);
@@ -184,9 +203,14 @@ struct RGFMGrid {
         int rc = ctx->timed("rgfm_fill_x", [&] { rgfm_fill_kernel<0><<<ny, kLineBS, 0, s>>>(la); });
         if (!rc && dim == 2) rc = ctx->timed("rgfm_fill_y", [&] { rgfm_fill_kernel<1><<<nx, kLineBS, 0, s>>>(la); });
         if (rc) return launch_status("rgfm_fill");
-        wave_max_kernel<EulerPhys><<<max_blocks, 256, 0, s>>>(m1[cur], plane, cells, dim, p1, partials);
-        wave_max_kernel<EulerPhys><<<max_blocks, 256, 0, s>>>(m2[cur], plane, cells, dim, p2, partials + max_blocks);
-        clock_begin_kernel<<<1, kReduceThreads, 0, s>>>(clk, partials, 2 * max_blocks, dt_log, dt_cap);
+        if (fixed_dts) {
+            clock_fixed_kernel<<<1, 1, 0, s>>>(clk, fixed_dts, n_fixed);
+        } else {
+            wave_max_kernel<EulerPhys><<<max_blocks, 256, 0, s>>>(m1[cur], plane, cells, dim, p1, partials);
+            wave_max_kernel<EulerPhys><<<max_blocks, 256, 0, s>>>(m2[cur], plane, cells, dim, p2,
+                                                                  partials + max_blocks);
+            clock_begin_kernel<<<1, kReduceThreads, 0, s>>>(clk, partials, 2 * max_blocks, dt_log, dt_cap);
+        }
         rc = launch_status("rgfm clock");
         if (rc) return rc;
         if (dim == 1) {
@@ -232,16 +256,26 @@ struct RGFMGrid {
 
     // Runs to t_final or max_steps.  Each executed step moves both materials
     // and the level set to the other buffer once.
+    // With resume the clock continues (t, step count, start-up ramp) from the
+    // previous run on this grid and max_steps counts from there.
     int run(int riemann, int mode, double dx, double dy, double cfl, double t_final, int max_steps, double bias,
-            int limiter, const EP &p1, const EP &p2, double *dt_log, int dt_cap, vg_run_stats *stats)
+            int limiter, const EP &p1, const EP &p2, double *dt_log, int dt_cap, vg_run_stats *stats,
+            bool resume = false)
     {
         auto w0 = std::chrono::steady_clock::now();
         hipStream_t s = ctx->stream;
         Clock h{};
+        if (resume) {
+            Clock prev;
+            VG_HIP(hipMemcpyAsync(&prev, clk, sizeof(Clock), hipMemcpyDeviceToHost, s));
+            VG_HIP(hipStreamSynchronize(s));
+            h.t = prev.t;
+            h.it = prev.it;
+        }
         h.t_final = t_final;
         h.cfl = cfl;
         h.dx = (dim == 1) ? dx : std::min(dx, dy);
-        h.max_steps = max_steps;
+        h.max_steps = (max_steps > 0) ? h.it + max_steps : 0;
         VG_HIP(hipMemcpyAsync(clk, &h, sizeof(Clock), hipMemcpyHostToDevice, s));
         if (dim == 2) {
             rgfm_corner_kernel<<<1, 1, 0, s>>>(p1, p2, clk);
@@ -464,6 +498,291 @@ int solve(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, const 
     return rc ? rc : run_rc;
 }
 
+// ---- adjoint (no reference counterpart) ---------------------------------------------
+
+// The fixed-time-step map the adjoint differentiates: steps as `solve` takes
+// them, with dts[it] for the time step.
+int solve_fixed(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, const double *phi_in,
+                double *m1_out, double *m2_out, double *phi_out, int ny, int nx, double dx, double dy,
+                const double *dts, int n_steps, double bias, int limiter, const vg_euler_params *p1,
+                const vg_euler_params *p2)
+{
+    vg::clear_error();
+    VG_CHECK(ctx != nullptr, "null context");
+    VG_CHECK(m1_in && m2_in && phi_in && m1_out && m2_out && phi_out && (dts || n_steps == 0) && n_steps >= 0,
+             "null argument");
+    int rc = check_args(dim, ny, nx, dx, dy, 1.0, 1.0, 0, limiter, VG_RIEMANN_HLLC, p1, p2, VG_MODE_EXACT);
+    if (rc) return rc;
+    rc = vg::bind(ctx);
+    if (rc) return rc;
+    const vg::EP q1 = to_params(p1), q2 = to_params(p2);
+    vg::RGFMGrid g;
+    rc = g.create(ctx, dim, ny, nx);
+    if (!rc) rc = g.load(m1_in, m2_in, phi_in);
+    double *dts_dev = nullptr;
+    if (!rc) {
+        void *v = nullptr;
+        rc = ctx->workspace_buffer(0, sizeof(double) * std::max(n_steps, 1), &v);
+        dts_dev = static_cast<double *>(v);
+    }
+    if (!rc && n_steps > 0)
+        VG_HIP(hipMemcpyAsync(dts_dev, dts, sizeof(double) * n_steps, hipMemcpyHostToDevice, ctx->stream));
+    if (!rc) {
+        vg::Clock h{};
+        h.t_final = INFINITY;
+        h.max_steps = n_steps;
+        VG_HIP(hipMemcpyAsync(g.clk, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
+    }
+    for (int st = 0; st < n_steps && !rc; st++)
+        rc = g.enqueue_step<vg::MODE_EXACT>(VG_RIEMANN_HLLC, dx, dy, bias, limiter, q1, q2, nullptr, 0, dts_dev,
+                                            n_steps);
+    if (!rc) rc = g.store(m1_out, m2_out, phi_out);
+    g.destroy();
+    return rc;
+}
+
+// Reverse mode through the HLLC RGFM solve, time steps frozen.  Forward with a
+// checkpoint (both materials and the level set) every K steps; then, segment by
+// segment from the end, the segment is recomputed from its checkpoint keeping
+// each step's starting state, and each step, from the last, has its inside
+// recomputed (fills, sweep inputs) and is reversed: the SLIC sweeps of both
+// materials, then the ghost fills (y, then x).
+int adjoint(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, const double *phi_in, double *m1_out,
+            double *m2_out, double *phi_out, const double *seed1, const double *seed2, double *grad1, double *grad2,
+            int ny, int nx, double dx, double dy, double cfl, double t_final, int max_steps, double bias, int limiter,
+            const vg_euler_params *p1, const vg_euler_params *p2, int checkpoint_every, double *dt_log, int dt_cap,
+            vg_run_stats *stats)
+{
+    vg::clear_error();
+    VG_CHECK(ctx != nullptr, "null context");
+    VG_CHECK(m1_in && m2_in && phi_in && m1_out && m2_out && phi_out && seed1 && seed2 && grad1 && grad2,
+             "null state pointer");
+    VG_CHECK(checkpoint_every >= 0, "checkpoint_every must be >= 0");
+    int rc = check_args(dim, ny, nx, dx, dy, cfl, t_final, max_steps, limiter, VG_RIEMANN_HLLC, p1, p2, VG_MODE_EXACT);
+    if (rc) return rc;
+    rc = vg::bind(ctx);
+    if (rc) return rc;
+    auto w0 = std::chrono::steady_clock::now();
+    const vg::EP q1 = to_params(p1), q2 = to_params(p2);
+    hipStream_t s = ctx->stream;
+    vg::RGFMGrid g;
+    rc = g.create(ctx, dim, ny, nx);
+    if (!rc) rc = g.load(m1_in, m2_in, phi_in);
+    if (rc) {
+        g.destroy();
+        return rc;
+    }
+    const size_t gb = sizeof(double) * 6 * g.plane, pb = sizeof(double) * g.cells;
+    const int K = checkpoint_every > 0 ? checkpoint_every
+                                       : (max_steps > 0 ? std::max(1, (int)std::ceil(std::sqrt((double)max_steps)))
+                                                        : 64);
+    // workspace: 0 dts, 1 clock, 2 ghost adjoints, 3-6 adjoint ping-pong (A[b][q]),
+    // 7-18 step scratch (P1, P2, Q1, Q2, T1, T2 per material), then K step
+    // states (m1, m2, phi) and the checkpoints (m1, m2, phi)
+    auto wsp = [&](size_t i, size_t bytes, double **p) {
+        void *v = nullptr;
+        int r = ctx->workspace_buffer(i, bytes, &v);
+        *p = static_cast<double *>(v);
+        return r;
+    };
+    const size_t st0 = 19, ck0 = st0 + 3 * (size_t)K;
+    // ---- forward with checkpoints ---------------------------------------------------
+    std::vector<double> dts;
+    std::vector<double *> ck;   // 3 per checkpoint
+    const int cap = max_steps > 0 ? max_steps : 1 << 22;
+    vg_run_stats fst{};
+    bool first = true;
+    for (;;) {
+        if ((int)dts.size() >= cap) break;
+        double *c1 = nullptr, *c2 = nullptr, *cp = nullptr;
+        const size_t base = ck0 + ck.size();
+        if ((rc = wsp(base, gb, &c1)) || (rc = wsp(base + 1, gb, &c2)) || (rc = wsp(base + 2, pb, &cp))) break;
+        VG_HIP(hipMemcpyAsync(c1, g.m1[g.cur], gb, hipMemcpyDeviceToDevice, s));
+        VG_HIP(hipMemcpyAsync(c2, g.m2[g.cur], gb, hipMemcpyDeviceToDevice, s));
+        VG_HIP(hipMemcpyAsync(cp, g.phi[g.pcur], pb, hipMemcpyDeviceToDevice, s));
+        ck.push_back(c1);
+        ck.push_back(c2);
+        ck.push_back(cp);
+        std::vector<double> part(K);
+        vg_run_stats st{};
+        const int want = std::min(K, cap - (int)dts.size());
+        rc = g.run(VG_RIEMANN_HLLC, VG_MODE_EXACT, dx, dy, cfl, t_final, want, bias, limiter, q1, q2, part.data(), K,
+                   &st, !first);
+        first = false;
+        if (rc) break;
+        dts.insert(dts.end(), part.begin(), part.begin() + st.steps);
+        fst = st;
+        if (st.steps < want) break;
+    }
+    if (rc) {
+        g.destroy();
+        return rc;
+    }
+    const int steps = (int)dts.size();
+    rc = g.store(m1_out, m2_out, phi_out);
+    // ---- buffers of the reverse pass --------------------------------------------------
+    double *dts_dev = nullptr, *ghost = nullptr, *A[2][2], *P1[2], *P2[2], *Q1[2], *Q2[2], *T1[2], *T2[2];
+    double *aclk_raw = nullptr;
+    if (!rc) rc = wsp(0, sizeof(double) * std::max(steps, 1), &dts_dev);
+    if (!rc) rc = wsp(1, sizeof(vg::Clock) + 64, &aclk_raw);
+    if (!rc) rc = wsp(2, sizeof(double) * 4 * 6 * (size_t)std::max(nx, ny), &ghost);
+    for (int b = 0; b < 2 && !rc; b++)
+        for (int q = 0; q < 2 && !rc; q++) rc = wsp(3 + 2 * b + q, gb, &A[b][q]);
+    double **scr[6] = {P1, P2, Q1, Q2, T1, T2};
+    for (int k = 0; k < 6 && !rc; k++)
+        for (int q = 0; q < 2 && !rc; q++) rc = wsp(7 + 2 * k + q, gb, &scr[k][q]);
+    std::vector<double *> sts(3 * (size_t)K, nullptr);
+    for (int k = 0; k < K && !rc; k++) {
+        if ((rc = wsp(st0 + 3 * k, gb, &sts[3 * k]))) break;
+        if ((rc = wsp(st0 + 3 * k + 1, gb, &sts[3 * k + 1]))) break;
+        rc = wsp(st0 + 3 * k + 2, pb, &sts[3 * k + 2]);
+    }
+    if (rc) {
+        g.destroy();
+        return rc;
+    }
+    vg::Clock *aclk = reinterpret_cast<vg::Clock *>(aclk_raw);
+    if (steps > 0) VG_HIP(hipMemcpyAsync(dts_dev, dts.data(), sizeof(double) * steps, hipMemcpyHostToDevice, s));
+    // seeds: arrays of structures -> planes
+    VG_HIP(hipMemcpyAsync(g.staging, seed1, gb / g.plane * g.cells, hipMemcpyHostToDevice, s));
+    vg::aos_to_soa_kernel<6><<<vg::RGFMGrid::blocks_for(g.cells), 256, 0, s>>>(g.staging, A[0][0], g.plane, g.cells);
+    VG_HIP(hipStreamSynchronize(s));
+    VG_HIP(hipMemcpyAsync(g.staging, seed2, gb / g.plane * g.cells, hipMemcpyHostToDevice, s));
+    vg::aos_to_soa_kernel<6><<<vg::RGFMGrid::blocks_for(g.cells), 256, 0, s>>>(g.staging, A[0][1], g.plane, g.cells);
+    int xv = vg::kAdjDefault[0], yv = vg::kAdjDefault[1];
+    if (steps > 0) {
+        const vg::AdjointArgs ta =
+            vg::adjoint_args(ck[0], A[0][0], A[1][0], ghost, g.plane, ny, nx, VG_BC_TRANSMISSIVE, dx, bias, limiter,
+                             dts[0], q1);
+        rc = vg::tune_adjoint(ctx, 0, ta, xv);
+        if (!rc && dim == 2) rc = vg::tune_adjoint(ctx, 1, ta, yv);
+    }
+    const vg::EP *qs[2] = {&q1, &q2};
+    const int cw = 0;
+    const int nseg = (int)ck.size() / 3;
+    for (int seg = nseg - 1; seg >= 0 && !rc; seg--) {
+        const int s0 = seg * K, s1 = std::min(steps, s0 + K);
+        if (s0 >= s1) continue;
+        // recompute the segment, keeping each step's starting state
+        g.cur = 0;
+        g.pcur = 0;
+        VG_HIP(hipMemcpyAsync(g.m1[0], ck[3 * seg], gb, hipMemcpyDeviceToDevice, s));
+        VG_HIP(hipMemcpyAsync(g.m2[0], ck[3 * seg + 1], gb, hipMemcpyDeviceToDevice, s));
+        VG_HIP(hipMemcpyAsync(g.phi[0], ck[3 * seg + 2], pb, hipMemcpyDeviceToDevice, s));
+        vg::Clock h{};
+        h.it = s0;
+        h.t_final = INFINITY;
+        h.max_steps = s1;
+        VG_HIP(hipMemcpyAsync(g.clk, &h, sizeof(h), hipMemcpyHostToDevice, s));
+        for (int st = s0; st < s1 && !rc; st++) {
+            const int k = st - s0;
+            VG_HIP(hipMemcpyAsync(sts[3 * k], g.m1[g.cur], gb, hipMemcpyDeviceToDevice, s));
+            VG_HIP(hipMemcpyAsync(sts[3 * k + 1], g.m2[g.cur], gb, hipMemcpyDeviceToDevice, s));
+            VG_HIP(hipMemcpyAsync(sts[3 * k + 2], g.phi[g.pcur], pb, hipMemcpyDeviceToDevice, s));
+            if (st + 1 < s1)
+                rc = g.enqueue_step<vg::MODE_EXACT>(VG_RIEMANN_HLLC, dx, dy, bias, limiter, q1, q2, nullptr, 0,
+                                                    dts_dev, s1);
+        }
+        // reverse the steps of the segment
+        for (int st = s1 - 1; st >= s0 && !rc; st--) {
+            const int k = st - s0;
+            double *S[2] = {sts[3 * k], sts[3 * k + 1]};
+            const double *phs = sts[3 * k + 2];
+            const double dt = dts[st];
+            vg::Clock c{};
+            c.dt = dt;
+            c.t_final = INFINITY;
+            VG_HIP(hipMemcpyAsync(aclk, &c, sizeof(c), hipMemcpyHostToDevice, s));
+            // inside of the step: fills, then the sweep inputs of both materials
+            for (int q = 0; q < 2; q++) VG_HIP(hipMemcpyAsync(P1[q], S[q], gb, hipMemcpyDeviceToDevice, s));
+            rc = g.fill_pass(0, P1[0], P1[1], phs, q1, q2, aclk);
+            double *post[2] = {P1[0], P1[1]};
+            if (!rc && dim == 2) {
+                for (int q = 0; q < 2; q++) VG_HIP(hipMemcpyAsync(P2[q], P1[q], gb, hipMemcpyDeviceToDevice, s));
+                rc = g.fill_pass(1, P2[0], P2[1], phs, q1, q2, aclk);
+                post[0] = P2[0];
+                post[1] = P2[1];
+                for (int q = 0; q < 2 && !rc; q++) {
+                    rc = g.sweep<vg::MODE_EXACT>(0, post[q], Q1[q], 0.5, dx, bias, limiter, *qs[q], aclk);
+                    if (!rc) rc = g.sweep<vg::MODE_EXACT>(1, Q1[q], Q2[q], 1.0, dy, bias, limiter, *qs[q], aclk);
+                }
+            }
+            // reverse: sweeps of each material
+            for (int q = 0; q < 2 && !rc; q++) {
+                if (dim == 1) {
+                    rc = vg::adj_sweep(ctx, 0, xv, post[q], A[cw][q], A[1 - cw][q], ghost, g.plane, ny, nx,
+                                       VG_BC_TRANSMISSIVE, dx, bias, limiter, dt, *qs[q]);
+                } else {
+                    rc = vg::adj_sweep(ctx, 0, xv, Q2[q], A[cw][q], T1[q], ghost, g.plane, ny, nx, VG_BC_TRANSMISSIVE,
+                                       dx, bias, limiter, 0.5 * dt, *qs[q]);
+                    if (!rc)
+                        rc = vg::adj_sweep(ctx, 1, yv, Q1[q], T1[q], T2[q], ghost, g.plane, ny, nx,
+                                           VG_BC_TRANSMISSIVE, dy, bias, limiter, dt, *qs[q]);
+                    if (!rc)
+                        rc = vg::adj_sweep(ctx, 0, xv, post[q], T2[q], A[1 - cw][q], ghost, g.plane, ny, nx,
+                                           VG_BC_TRANSMISSIVE, dx, bias, limiter, 0.5 * dt, *qs[q]);
+                }
+            }
+            if (rc) break;
+            // reverse: ghost fills (the y pass read P1, the x pass read S)
+            vg::FillAdjArgs fa;
+            fa.phi = phs;
+            fa.plane = g.plane;
+            fa.ny = ny;
+            fa.nx = nx;
+            fa.init_flag = (dim == 2) ? 1 : 0;
+            fa.p1 = q1;
+            fa.p2 = q2;
+            const double *xin[2] = {A[1 - cw][0], A[1 - cw][1]};
+            if (dim == 2) {
+                fa.m1 = P1[0];
+                fa.m2 = P1[1];
+                fa.bar1 = A[1 - cw][0];
+                fa.bar2 = A[1 - cw][1];
+                fa.out1 = T1[0];
+                fa.out2 = T1[1];
+                rc = ctx->timed("rgfm_fill_adjoint",
+                                [&] { vg::rgfm_fill_adjoint_kernel<1><<<nx, vg::kLineBS, 0, s>>>(fa); });
+                xin[0] = T1[0];
+                xin[1] = T1[1];
+            }
+            if (!rc) {
+                fa.m1 = S[0];
+                fa.m2 = S[1];
+                fa.bar1 = xin[0];
+                fa.bar2 = xin[1];
+                fa.out1 = A[cw][0];
+                fa.out2 = A[cw][1];
+                rc = ctx->timed("rgfm_fill_adjoint",
+                                [&] { vg::rgfm_fill_adjoint_kernel<0><<<ny, vg::kLineBS, 0, s>>>(fa); });
+            }
+        }
+    }
+    for (int q = 0; q < 2 && !rc; q++) {
+        vg::soa_to_aos_kernel<6><<<vg::RGFMGrid::blocks_for(g.cells), 256, 0, s>>>(A[cw][q], g.staging, g.plane,
+                                                                                    g.cells);
+        hipError_t e = hipMemcpyAsync(q == 0 ? grad1 : grad2, g.staging, sizeof(double) * 6 * g.cells,
+                                      hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            vg::set_error(std::string("rgfm adjoint copy: ") + hipGetErrorString(e));
+            rc = VG_ERR_HIP;
+        }
+    }
+    if (!rc && ctx->timing) rc = ctx->resolve_timing();
+    if (dt_log) std::memcpy(dt_log, dts.data(), sizeof(double) * std::min(steps, std::max(dt_cap, 0)));
+    if (stats) {
+        *stats = fst;
+        stats->steps = steps;
+        stats->total_steps = steps;
+        stats->dt_min = steps ? *std::min_element(dts.begin(), dts.end()) : 0.0;
+        stats->dt_max = steps ? *std::max_element(dts.begin(), dts.end()) : 0.0;
+        stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+    }
+    g.destroy();
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -534,6 +853,44 @@ int vg_rgfm_solve_batch_2d(vg_ctx *ctx, int n_inst, const double *m1_in, const d
     total.first_nonfinite = -1;
     if (stats) *stats = total;
     return rc ? rc : run_rc;
+}
+
+int vg_rgfm_adjoint_1d(vg_ctx *ctx, const double *m1_in, const double *m2_in, const double *phi_in, double *m1_out,
+                       double *m2_out, double *phi_out, const double *seed1, const double *seed2, double *grad1,
+                       double *grad2, int n, double dx, double cfl, double t_final, int max_steps, double bias,
+                       int limiter, const vg_euler_params *p1, const vg_euler_params *p2, int checkpoint_every,
+                       double *dt_log, int dt_log_cap, vg_run_stats *stats)
+{
+    return adjoint(ctx, 1, m1_in, m2_in, phi_in, m1_out, m2_out, phi_out, seed1, seed2, grad1, grad2, 1, n, dx, dx,
+                   cfl, t_final, max_steps, bias, limiter, p1, p2, checkpoint_every, dt_log, dt_log_cap, stats);
+}
+
+int vg_rgfm_adjoint_2d(vg_ctx *ctx, const double *m1_in, const double *m2_in, const double *phi_in, double *m1_out,
+                       double *m2_out, double *phi_out, const double *seed1, const double *seed2, double *grad1,
+                       double *grad2, int ny, int nx, double dx, double dy, double cfl, double t_final, int max_steps,
+                       double bias, int limiter, const vg_euler_params *p1, const vg_euler_params *p2,
+                       int checkpoint_every, double *dt_log, int dt_log_cap, vg_run_stats *stats)
+{
+    return adjoint(ctx, 2, m1_in, m2_in, phi_in, m1_out, m2_out, phi_out, seed1, seed2, grad1, grad2, ny, nx, dx, dy,
+                   cfl, t_final, max_steps, bias, limiter, p1, p2, checkpoint_every, dt_log, dt_log_cap, stats);
+}
+
+int vg_rgfm_solve_fixed_dt_1d(vg_ctx *ctx, const double *m1_in, const double *m2_in, const double *phi_in,
+                              double *m1_out, double *m2_out, double *phi_out, int n, double dx, const double *dts,
+                              int n_steps, double bias, int limiter, const vg_euler_params *p1,
+                              const vg_euler_params *p2)
+{
+    return solve_fixed(ctx, 1, m1_in, m2_in, phi_in, m1_out, m2_out, phi_out, 1, n, dx, dx, dts, n_steps, bias,
+                       limiter, p1, p2);
+}
+
+int vg_rgfm_solve_fixed_dt_2d(vg_ctx *ctx, const double *m1_in, const double *m2_in, const double *phi_in,
+                              double *m1_out, double *m2_out, double *phi_out, int ny, int nx, double dx, double dy,
+                              const double *dts, int n_steps, double bias, int limiter, const vg_euler_params *p1,
+                              const vg_euler_params *p2)
+{
+    return solve_fixed(ctx, 2, m1_in, m2_in, phi_in, m1_out, m2_out, phi_out, ny, nx, dx, dy, dts, n_steps, bias,
+                       limiter, p1, p2);
 }
 
 }  // extern "C"

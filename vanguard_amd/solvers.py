@@ -751,6 +751,72 @@ def rgfm_solve_batch_2d(m1, m2, phi, dx, dy, cfl, t_final, params1, params2, *, 
     return RGFMBatchResult(o1, o2, oph, steps, tend, circ, st, rc)
 
 
+@dataclass
+class RGFMAdjointResult:
+    system: MultimaterialSystem   # final states of the forward
+    grad1: np.ndarray             # d loss / d initial material 1
+    grad2: np.ndarray             # d loss / d initial material 2
+    dt: np.ndarray                # the time steps the forward chose (held fixed)
+    stats: _lib.RunStats
+
+
+def rgfm_adjoint(system: MultimaterialSystem, seed1: np.ndarray, seed2: np.ndarray, dx: float, cfl: float,
+                 t_final: float, params1, params2, *, dy: float | None = None, bias: float = 0.0, limiter: int = 0,
+                 max_steps: int = 0, checkpoint_every: int = 0, dt_cap: int = 200000,
+                 ctx: Context | None = None) -> RGFMAdjointResult:
+    """Reverse-mode derivative of the HLLC RGFM solve (vg_rgfm_adjoint_1d / _2d):
+    seeds are d loss / d final material states, the result holds d loss / d
+    initial material states, time steps frozen."""
+    ctx = ctx or default_context()
+    m1 = np.ascontiguousarray(system.material1, dtype=np.float64)
+    m2 = np.ascontiguousarray(system.material2, dtype=np.float64)
+    phi = np.ascontiguousarray(system.level_set, dtype=np.float64)
+    s1 = np.ascontiguousarray(seed1, dtype=np.float64)
+    s2 = np.ascontiguousarray(seed2, dtype=np.float64)
+    if m1.shape != m2.shape or m1.shape[:-1] != phi.shape or m1.shape[-1] != 6 or s1.shape != m1.shape \
+            or s2.shape != m1.shape:
+        raise ValueError("materials and seeds must be (..., 6) over the level set's shape")
+    o1, o2, oph = np.empty_like(m1), np.empty_like(m2), np.empty_like(phi)
+    g1, g2 = np.empty_like(m1), np.empty_like(m2)
+    dts = np.empty(dt_cap)
+    st = _lib.RunStats()
+    q1, q2 = _params(params1).c_struct(), _params(params2).c_struct()
+    P = _lib.as_dptr
+    if phi.ndim == 1:
+        rc = ctx.lib.vg_rgfm_adjoint_1d(ctx.handle, P(m1), P(m2), P(phi), P(o1), P(o2), P(oph), P(s1), P(s2), P(g1),
+                                        P(g2), phi.shape[0], dx, cfl, t_final, max_steps, bias, limiter,
+                                        C.byref(q1), C.byref(q2), checkpoint_every, P(dts), dt_cap, C.byref(st))
+    else:
+        rc = ctx.lib.vg_rgfm_adjoint_2d(ctx.handle, P(m1), P(m2), P(phi), P(o1), P(o2), P(oph), P(s1), P(s2), P(g1),
+                                        P(g2), phi.shape[0], phi.shape[1], dx, dx if dy is None else dy, cfl,
+                                        t_final, max_steps, bias, limiter, C.byref(q1), C.byref(q2),
+                                        checkpoint_every, P(dts), dt_cap, C.byref(st))
+    check(rc, "vg_rgfm_adjoint")
+    return RGFMAdjointResult(MultimaterialSystem(o1, o2, oph), g1, g2, dts[:min(st.steps, dt_cap)].copy(), st)
+
+
+def rgfm_solve_fixed_dt(system: MultimaterialSystem, dx: float, dts, params1, params2, *, dy: float | None = None,
+                        bias: float = 0.0, limiter: int = 0, ctx: Context | None = None) -> MultimaterialSystem:
+    """HLLC RGFM solve with prescribed time steps (vg_rgfm_solve_fixed_dt_1d / _2d)."""
+    ctx = ctx or default_context()
+    m1 = np.ascontiguousarray(system.material1, dtype=np.float64)
+    m2 = np.ascontiguousarray(system.material2, dtype=np.float64)
+    phi = np.ascontiguousarray(system.level_set, dtype=np.float64)
+    d = np.ascontiguousarray(dts, dtype=np.float64).ravel()
+    o1, o2, oph = np.empty_like(m1), np.empty_like(m2), np.empty_like(phi)
+    q1, q2 = _params(params1).c_struct(), _params(params2).c_struct()
+    P = _lib.as_dptr
+    if phi.ndim == 1:
+        rc = ctx.lib.vg_rgfm_solve_fixed_dt_1d(ctx.handle, P(m1), P(m2), P(phi), P(o1), P(o2), P(oph), phi.shape[0],
+                                               dx, P(d), d.size, bias, limiter, C.byref(q1), C.byref(q2))
+    else:
+        rc = ctx.lib.vg_rgfm_solve_fixed_dt_2d(ctx.handle, P(m1), P(m2), P(phi), P(o1), P(o2), P(oph), phi.shape[0],
+                                               phi.shape[1], dx, dx if dy is None else dy, P(d), d.size, bias,
+                                               limiter, C.byref(q1), C.byref(q2))
+    check(rc, "vg_rgfm_solve_fixed_dt")
+    return MultimaterialSystem(o1, o2, oph)
+
+
 class RGFMSolver:
     """RGFMSolver (RGFM/rgfmsolver.h:8-41): the whole-solve entry points."""
 
