@@ -27,6 +27,9 @@ Secondary workloads (one JSON line each, same contract):
     --workload adjoint     reverse-mode derivative of the same 2D Euler solve (EXACT
                            arithmetic, time steps frozen, sqrt(N) checkpoints): cost of
                            forward + adjoint over the plain EXACT forward of the same run
+    --workload rgfm_adjoint  d circulation / d Mach of shock-bubble instances (2048 x 1024,
+                           t = 170 us, time steps frozen): the RGFM adjoint's cost over the
+                           plain forward of the same instance
     --workload copy        a plain 1 GiB device copy (calibrates HBM byte counters)
 """
 
@@ -395,6 +398,74 @@ def bench_adjoint(args):
     print(json.dumps(line), flush=True)
 
 
+def circulation_seed(m1, phi, dx, dy):
+    """d Gamma / d (final material 1) for Gamma = sum over interior cells with
+    phi < 0 of (dv/dx - du/dy) dx dy (central differences)."""
+    mask = np.zeros(phi.shape)
+    mask[1:-1, 1:-1] = (phi[1:-1, 1:-1] < 0.0)
+    s = np.zeros_like(m1)
+    s[:, :-2, 2] -= 0.5 * dy * mask[:, 1:-1]
+    s[:, 2:, 2] += 0.5 * dy * mask[:, 1:-1]
+    s[:-2, :, 1] += 0.5 * dx * mask[1:-1, :]
+    s[2:, :, 1] -= 0.5 * dx * mask[1:-1, :]
+    return s
+
+
+def bench_rgfm_adjoint(args):
+    """dGamma/dM per shock-bubble instance: plain forward (for Gamma and its
+    seed), then vg_rgfm_adjoint_2d; value = adjoint wall (its own forward,
+    recomputation and reverse sweeps included) / plain forward wall, summed
+    over the instances."""
+    import torch
+    from vanguard_amd import problems
+    from vanguard_amd import solvers as S
+
+    nx, ny = args.rgfm_nx, args.rgfm_ny
+    radii, machs = problems.shock_bubble_batch(16, seed=6)
+    ctx = S.default_context()
+    fwd_total = adj_total = 0.0
+    per = []
+    for q in args.rgfm_instances:
+        r, M = float(radii[q]), float(machs[q])
+        m1, m2, phi, dx, dy = problems.shock_bubble_2d(nx, ny, radius=r, mach=M)
+        sysm = S.MultimaterialSystem(m1, m2, phi)
+        if args.warmup:
+            S.rgfm_solve(sysm, dx, 0.8, args.t_final, problems.HELIUM, problems.AIR, dy=dy, max_steps=args.warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f = S.rgfm_solve(sysm, dx, 0.8, args.t_final, problems.HELIUM, problems.AIR, dy=dy)
+        fwd = time.perf_counter() - t0
+        seed1 = circulation_seed(f.system.material1, f.system.level_set, dx, dy)
+        t0 = time.perf_counter()
+        a = S.rgfm_adjoint(sysm, seed1, np.zeros_like(m2), dx, 0.8, args.t_final, problems.HELIUM, problems.AIR,
+                           dy=dy, checkpoint_every=args.checkpoint_every)
+        adj = time.perf_counter() - t0
+        h = 1e-6
+        pa = problems.shock_bubble_2d(nx, ny, radius=r, mach=M + h)
+        pb = problems.shock_bubble_2d(nx, ny, radius=r, mach=M - h)
+        dm2 = (pa[1] - pb[1]) / (2 * h)   # d(initial air state)/dM (helium does not depend on M)
+        dgdm = float(np.sum(a.grad2 * dm2))
+        finite = bool(np.isfinite(a.grad1).all() and np.isfinite(a.grad2).all())
+        log(f"instance {q}: M={M:.4f} steps={f.stats.steps} forward {fwd:.2f} s, adjoint {adj:.2f} s, "
+            f"Gamma={f.circulation:.6g}, dGamma/dM={dgdm:.6g}, finite={finite}")
+        fwd_total += fwd
+        adj_total += adj
+        per.append({"instance": q, "mach": M, "radius": r, "steps": int(f.stats.steps), "forward_s": round(fwd, 3),
+                    "adjoint_s": round(adj, 3), "circulation": f.circulation, "dcirculation_dmach": dgdm,
+                    "finite": finite})
+    line = {
+        "metric": "adjoint/forward cost FP64 2D RGFM shock-bubble d circulation / d Mach",
+        "value": round(adj_total / fwd_total, 3), "unit": "x", "n_gpus": 1,
+        "steps": int(sum(p["steps"] for p in per)), "warmup": args.warmup,
+        "ms_per_step": round(adj_total * 1e3 / max(1, sum(p["steps"] for p in per)), 4),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"rgfm_shock_bubble_{nx}x{ny}_hllc_t{args.t_final:g}_dgamma_dmach",
+                   "instances": list(args.rgfm_instances), "checkpoint_every": int(args.checkpoint_every) or "auto"},
+        "instances": per, "target": 3.0, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_copy(args):
     """Plain device copy of 1 GiB, three times (torch's copy kernel): the known
     byte count that scales the FETCH_SIZE / WRITE_SIZE counters."""
@@ -415,8 +486,9 @@ def bench_copy(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["euler2d", "mhd_batch", "hpr2d", "rgfm", "adjoint", "copy"],
+    ap.add_argument("--workload", choices=["euler2d", "mhd_batch", "hpr2d", "rgfm", "adjoint", "rgfm_adjoint", "copy"],
                     default="euler2d")
+    ap.add_argument("--rgfm-instances", type=int, nargs="+", default=[0, 1])
     ap.add_argument("--adj-n", type=int, default=4096)
     ap.add_argument("--checkpoint-every", type=int, default=0)
     ap.add_argument("--rgfm-nx", type=int, default=2048)
@@ -460,6 +532,8 @@ def main():
         return bench_rgfm(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
+    if args.workload == "rgfm_adjoint":
+        return bench_rgfm_adjoint(args)
     if args.workload == "copy":
         return bench_copy(args)
     log(f"building initial condition {N}x{N} ...")
