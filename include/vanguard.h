@@ -383,6 +383,37 @@ int vg_hpr_state_run(vg_hpr_state *st, int order, double dx, double cfl, double 
  * 362-385) over the ghosted grid. */
 int vg_hpr_state_max_wave_speed(vg_hpr_state *st, int bc, const vg_hpr_params *mp, double *smax);
 
+/* Reverse-mode derivative of the second-order HPR solve (as vg_hpr_solve_*
+ * with order 2, EXACT arithmetic; stiffened gas, not plastic) with the time
+ * steps held at the values the forward run chose.  No reference counterpart:
+ * the reference has no derivatives.  Runs the forward (outputs as
+ * vg_hpr_solve_*, dt_log its steps), checkpointing every `checkpoint_every`
+ * steps (0: ceil(sqrt(max_steps)), 16 for runs to t_final), then the adjoint
+ * of every operation backwards: the RK4 relaxation sub-cycles
+ * (HPRForcingSolver::computeRungeKuttaTimeStep[2D], hprforcingsolver.cpp:341-356,
+ * 394-417) and the SLIC sweeps (hprsecondordersolver.cpp:7-17, 89-241).  seed:
+ * derivative of a scalar loss with respect to the final primitive state (17
+ * doubles per cell); grad receives its derivative with respect to the initial
+ * primitive state, grad_tau[0..1] (may be NULL) with respect to
+ * strain_dissipation_time and thermal_impulse_relaxation_time.  The slope
+ * limiter is differentiated piecewise as in vg_euler_adjoint_*.  Needs at
+ * least 3 cells per direction. */
+int vg_hpr_adjoint_1d(vg_ctx *ctx, const double *prim_in, double *prim_out, const double *seed, double *grad,
+                      double *grad_tau, int n, double dx, double cfl, double t_final, int max_steps, double bias,
+                      int limiter, int bc, int subcycles, const vg_hpr_params *mp, int checkpoint_every,
+                      double *dt_log, int dt_log_cap, vg_run_stats *stats);
+int vg_hpr_adjoint_2d(vg_ctx *ctx, const double *prim_in, double *prim_out, const double *seed, double *grad,
+                      double *grad_tau, int ny, int nx, double dx, double cfl, double t_final, int max_steps,
+                      double bias, int limiter, int bc, int subcycles, const vg_hpr_params *mp, int checkpoint_every,
+                      double *dt_log, int dt_log_cap, vg_run_stats *stats);
+/* The second-order HPR solve with prescribed time steps dts[0..n_steps): the
+ * map the adjoint differentiates, for finite-difference checks. */
+int vg_hpr_solve_fixed_dt_1d(vg_ctx *ctx, const double *prim_in, double *prim_out, int n, double dx, const double *dts,
+                             int n_steps, double bias, int limiter, int bc, int subcycles, const vg_hpr_params *mp);
+int vg_hpr_solve_fixed_dt_2d(vg_ctx *ctx, const double *prim_in, double *prim_out, int ny, int nx, double dx,
+                             const double *dts, int n_steps, double bias, int limiter, int bc, int subcycles,
+                             const vg_hpr_params *mp);
+
 #ifdef __cplusplus
 }
 #endif

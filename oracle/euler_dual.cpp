@@ -16,51 +16,14 @@
 #include <cstring>
 #include <vector>
 
+#include "dual.h"
 #include "oracle.h"
 
 namespace {
 
-struct Dual {
-    double v, d;
-    Dual(double a = 0.0, double b = 0.0) : v(a), d(b) {}
-};
-inline Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
-inline Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
-inline Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
-inline Dual operator*(Dual a, Dual b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
-inline Dual operator/(Dual a, Dual b) { return Dual(a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)); }
-inline Dual operator+(Dual a, double b) { return Dual(a.v + b, a.d); }
-inline Dual operator+(double a, Dual b) { return Dual(a + b.v, b.d); }
-inline Dual operator-(Dual a, double b) { return Dual(a.v - b, a.d); }
-inline Dual operator-(double a, Dual b) { return Dual(a - b.v, -b.d); }
-inline Dual operator*(Dual a, double b) { return Dual(a.v * b, a.d * b); }
-inline Dual operator*(double a, Dual b) { return Dual(a * b.v, a * b.d); }
-inline Dual operator/(Dual a, double b) { return Dual(a.v / b, a.d / b); }
-inline Dual operator/(double a, Dual b) { return Dual(a / b.v, -a * b.d / (b.v * b.v)); }
-inline double val(double x) { return x; }
-inline double val(Dual x) { return x.v; }
-inline double fabs_v(double x) { return std::fabs(x); }
-inline double fabs_v(Dual x) { return std::fabs(x.v); }
-inline double sqrt_t(double x) { return std::sqrt(x); }
-inline Dual sqrt_t(Dual x)
-{
-    const double r = std::sqrt(x.v);
-    return Dual(r, x.d / (2.0 * r));
-}
-inline double pow_t(double x, double e) { return std::pow(x, e); }
-inline Dual pow_t(Dual x, double e) { return Dual(std::pow(x.v, e), e * std::pow(x.v, e - 1.0) * x.d); }
+using namespace orc_dual;
 
 const int NF = 6;
-
-// Hash of every branch decision the step takes (slope clamps, limiter cases,
-// min selections): two runs with equal hashes evaluated the same smooth
-// pieces, so a finite difference between them involves no kink.
-uint64_t g_branches = 0;
-inline bool note(bool b)
-{
-    g_branches = (g_branches ^ (uint64_t)(b ? 0x9e37u : 0x51u)) * 0x100000001b3ull;
-    return b;
-}
 
 struct P {
     double gamma, p_inf, e_form;
@@ -542,7 +505,7 @@ void orc_euler_fixed_dt_jvp(int dim, int ny, int nx, double dx, const double *dt
 {
     const P m = {mp->gamma, mp->p_inf, mp->e_form};
     if (dim == 1) ny = 1;
-    g_branches = 0xcbf29ce484222325ull;
+    reset_branches();
     const size_t cells = (size_t)ny * nx;
     if (!tangent_in) {
         std::vector<double> a(prim_in, prim_in + NF * cells), b(NF * cells);
@@ -559,8 +522,8 @@ void orc_euler_fixed_dt_jvp(int dim, int ny, int nx, double dx, const double *dt
     }
 }
 
-/* Hash of the branch decisions of the last orc_euler_fixed_dt_jvp call. */
-uint64_t orc_euler_branch_hash(void) { return g_branches; }
+/* Hash of the branch decisions of the last dual-number oracle call. */
+uint64_t orc_euler_branch_hash(void) { return branch_state(); }
 
 /* HLLC RGFM solve with prescribed time steps (orc_rgfm_solve's step) and, when
  * t1_in is not NULL, the forward-mode derivative of both final material
@@ -575,7 +538,7 @@ void orc_rgfm_fixed_dt_jvp(int dim, int ny, int nx, double dx, double dy, const 
     const size_t cells = (size_t)ny * nx;
     const E e1 = {p1->gamma, p1->p_inf, p1->e_form}, e2 = {p2->gamma, p2->p_inf, p2->e_form};
     std::vector<double> phi(phi_in, phi_in + cells);
-    g_branches = 0xcbf29ce484222325ull;
+    reset_branches();
     if (!t1_in) {
         std::vector<double> a1(m1_in, m1_in + NF * cells), a2(m2_in, m2_in + NF * cells);
         rgfm_run(dim, ny, nx, dx, dy, dts, n_steps, bias, limiter, e1, e2, a1, a2, phi);

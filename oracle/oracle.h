@@ -96,6 +96,15 @@ int orc_hpr_solve(int order, int dim, int ny, int nx, double dx, double cfl, dou
 /* Per-function records of five states (85 doubles) -> 156 doubles. */
 void orc_hpr_units(int n, double bias, int limiter, double dx, double ts, const orc_hpr_params *mp,
                    const double *states, double *out);
+/* Dual-number restatement of the HPR second-order solve with prescribed time
+ * steps (hpr_dual.cpp): primal and, with tangent_in, the derivative along
+ * (tangent_in, dtau1, dtau2); sets the branch hash. */
+void orc_hpr_fixed_dt_jvp(int dim, int ny, int nx, double dx, const double *dts, int n_steps, double bias, int limiter,
+                          int bc, int subcycles, const orc_hpr_params *mp, const double *prim_in,
+                          const double *tangent_in, double dtau1, double dtau2, double *prim_out, double *tangent_out);
+/* Jacobian-vector products of p2c, c2p, Fx, Fy and S per record (5 x 17 doubles). */
+void orc_hpr_jvp_units(int n, const orc_hpr_params *mp, double dtau1, double dtau2, const double *w,
+                       const double *v, double *out);
 
 /* RGFMSolver::solveHLLC / solveHLLC2D (riemann 0) and solveExact / solveExact2D
  * (riemann 1).  Two full-domain material grids (primitive AoS, 6 doubles per

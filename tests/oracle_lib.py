@@ -81,6 +81,12 @@ def load():
     lib.orc_rgfm_fixed_dt_jvp.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, dptr, C.c_int, C.c_double,
                                           C.c_int, C.POINTER(OrcEulerParams), C.POINTER(OrcEulerParams), dptr, dptr,
                                           dptr, dptr, dptr, dptr, dptr, dptr, dptr, dptr]
+    HP = C.POINTER(OrcHPRParams)
+    lib.orc_hpr_fixed_dt_jvp.restype = None
+    lib.orc_hpr_fixed_dt_jvp.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, dptr, C.c_int, C.c_double, C.c_int,
+                                         C.c_int, C.c_int, HP, dptr, dptr, C.c_double, C.c_double, dptr, dptr]
+    lib.orc_hpr_jvp_units.restype = None
+    lib.orc_hpr_jvp_units.argtypes = [C.c_int, HP, C.c_double, C.c_double, dptr, dptr, dptr]
     lib.orc_euler_branch_hash.restype = C.c_uint64
     lib.orc_euler_branch_hash.argtypes = []
     lib.orc_rgfm_circulation.restype = C.c_double
@@ -172,6 +178,40 @@ def hpr_solve(prim, *, order, dim, dx, cfl, tfinal, max_steps, bias, limiter, su
     n = lib.orc_hpr_solve(order, dim, ny, nx, dx, cfl, tfinal, max_steps, bias, limiter, bc, subcycles, C.byref(mp),
                           _p(w), _p(out), _p(dts), dt_cap)
     return out, dts[:min(n, dt_cap)].copy()
+
+
+def hpr_fixed_dt(prim, dts, *, dim, dx, bias, limiter, subcycles, params, bc=0, tangent=None, dtau=(0.0, 0.0)):
+    """HPR second-order solve with prescribed time steps (dual-number
+    restatement); with `tangent` (and `dtau` = tangents of tau1, tau2), also the
+    forward-mode derivative of the final state.  Sets the branch hash."""
+    lib = load()
+    w = np.ascontiguousarray(prim, dtype=np.float64)
+    d = np.ascontiguousarray(dts, dtype=np.float64)
+    out = np.empty_like(w)
+    ny, nx = (1, w.shape[0]) if dim == 1 else w.shape[:2]
+    mp = _hpr_struct(params)
+    if tangent is None:
+        lib.orc_hpr_fixed_dt_jvp(dim, ny, nx, dx, _p(d), d.size, bias, limiter, bc, subcycles, C.byref(mp), _p(w),
+                                 None, 0.0, 0.0, _p(out), None)
+        return out
+    tv = np.ascontiguousarray(tangent, dtype=np.float64)
+    assert tv.shape == w.shape
+    tout = np.empty_like(w)
+    lib.orc_hpr_fixed_dt_jvp(dim, ny, nx, dx, _p(d), d.size, bias, limiter, bc, subcycles, C.byref(mp), _p(w),
+                             _p(tv), float(dtau[0]), float(dtau[1]), _p(out), _p(tout))
+    return out, tout
+
+
+def hpr_jvp_units(w, v, *, params, dtau=(0.0, 0.0)):
+    """Per record: JVPs of p2c at w, and of c2p, Fx, Fy, S at U = p2c(w) along
+    v (and dtau) -> (n, 5, 17)."""
+    lib = load()
+    a = np.ascontiguousarray(w, dtype=np.float64)
+    b = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.empty((a.shape[0], 5, 17))
+    mp = _hpr_struct(params)
+    lib.orc_hpr_jvp_units(a.shape[0], C.byref(mp), float(dtau[0]), float(dtau[1]), _p(a), _p(b), _p(out))
+    return out
 
 
 def hpr_units(states, *, bias, limiter, dx, ts, params):

@@ -158,6 +158,16 @@ SIGNATURES = {
                                    C.c_int, C.c_int, C.POINTER(HPRParams), C.c_int, dptr, C.c_int,
                                    C.POINTER(RunStats)]),
     "vg_hpr_state_max_wave_speed": (C.c_int, [vptr, C.c_int, C.POINTER(HPRParams), dptr]),
+    "vg_hpr_adjoint_1d": (C.c_int, [vptr, dptr, dptr, dptr, dptr, dptr, C.c_int, C.c_double, C.c_double, C.c_double,
+                                    C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(HPRParams), C.c_int, dptr,
+                                    C.c_int, C.POINTER(RunStats)]),
+    "vg_hpr_adjoint_2d": (C.c_int, [vptr, dptr, dptr, dptr, dptr, dptr, C.c_int, C.c_int, C.c_double, C.c_double,
+                                    C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(HPRParams),
+                                    C.c_int, dptr, C.c_int, C.POINTER(RunStats)]),
+    "vg_hpr_solve_fixed_dt_1d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_double, dptr, C.c_int, C.c_double, C.c_int,
+                                           C.c_int, C.c_int, C.POINTER(HPRParams)]),
+    "vg_hpr_solve_fixed_dt_2d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_int, C.c_double, dptr, C.c_int, C.c_double,
+                                           C.c_int, C.c_int, C.c_int, C.POINTER(HPRParams)]),
 }
 
 _lib = None

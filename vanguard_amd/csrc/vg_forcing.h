@@ -76,6 +76,103 @@ __device__ __forceinline__ void fractional_step(const typename P::Params &m, con
     P::cons_to_prim(m, N, w);
 }
 
+// Recorder of the stage states of forcing_cell (the forward records nothing;
+// the adjoint keeps the states it linearises at).
+struct NoStages {
+    template <int NF>
+    __device__ __forceinline__ void mid(int, int, const double (&)[NF])
+    {
+    }
+    template <int NF>
+    __device__ __forceinline__ void end(int, const double (&)[NF])
+    {
+    }
+};
+
+// One cell's RK4 sub-cycle: N (conserved) from its own conserved vector Um and
+// its neighbours' (UT, UB unused in 1D).  K[st][q] are the stage increments of
+// the non-zero slots.  rec.mid(st, phase, UM) sees the middle state entering
+// each fractional step of stage st, rec.end(st, Ue) the state whose source gives
+// the next increment.
+template <class P, int DIM, class R>
+__device__ __forceinline__ void forcing_cell(const typename P::Params &m, const double (&UL)[P::NF],
+                                             const double (&UR)[P::NF], const double (&UT)[P::NF],
+                                             const double (&UB)[P::NF], const double (&Um)[P::NF], double ts,
+                                             double dx, double bias, int limiter, double (&N)[P::NF],
+                                             double (&K)[4][ForcingTraits<P>::kMaxSlots], R &rec)
+{
+    constexpr int NF = P::NF;
+    using T = ForcingTraits<P>;
+    constexpr int NS = T::kMaxSlots;
+    const int nsrc = T::count(m);
+    double S[NF];
+    P::source(m, Um, S);
+#pragma unroll
+    for (int q = 0; q < NS; q++) K[0][q] = ts * S[T::slot(q)];
+    const double cx = ts / dx;
+#pragma unroll
+    for (int st = 0; st < 3; st++) {
+        double Us[NF];
+#pragma unroll
+        for (int k = 0; k < NF; k++) Us[k] = Um[k] + 0.0;
+#pragma unroll
+        for (int q = 0; q < NS; q++) {
+            if (q < nsrc) {
+                const double kp = K[st][q];
+                Us[T::slot(q)] = Um[T::slot(q)] + (st < 2 ? (0.5 * kp) : kp);
+            }
+        }
+        // The stage state enters the predictor as a primitive state
+        // (evolveConservedVariableVector takes states): c2p then p2c.
+        double wm[NF], UM[NF], e[NF];
+        P::cons_to_prim(m, Us, wm);
+        P::prim_to_cons(m, wm, UM);
+        rec.mid(st, 0, UM);
+        if (DIM == 1) {
+            fractional_step<P, 0>(m, UL, UM, UR, bias, limiter, (st < 2 ? 0.5 : 1.0) * cx, e);
+        } else {
+            const double fx = (st < 2) ? 0.25 : 0.5, fy = (st < 2) ? 0.5 : 1.0;
+            fractional_step<P, 0>(m, UL, UM, UR, bias, limiter, fx * cx, e);
+            P::prim_to_cons(m, e, UM);
+            rec.mid(st, 1, UM);
+            fractional_step<P, 1>(m, UT, UM, UB, bias, limiter, fy * cx, e);
+            P::prim_to_cons(m, e, UM);
+            rec.mid(st, 2, UM);
+            fractional_step<P, 0>(m, UL, UM, UR, bias, limiter, fx * cx, e);
+        }
+        double Ue[NF];
+        P::prim_to_cons(m, e, Ue);
+        rec.end(st, Ue);
+        P::source(m, Ue, S);
+#pragma unroll
+        for (int q = 0; q < NS; q++) K[st + 1][q] = ts * S[T::slot(q)];
+    }
+    // computeEvolvedConservedVariableVector (hprforcingsolver.cpp:332-339)
+#pragma unroll
+    for (int k = 0; k < NF; k++) N[k] = Um[k] + 0.0;
+#pragma unroll
+    for (int q = 0; q < NS; q++) {
+        if (q < nsrc) {
+            const int k = T::slot(q);
+            N[k] = Um[k] + ((1.0 / 6.0) * (K[0][q] + ((2.0 * K[1][q]) + ((2.0 * K[2][q]) + K[3][q]))));
+        }
+    }
+}
+
+// The neighbour states of forcing_cell: computeRungeKuttaTimeStep converts
+// every cell to conserved form; the neighbours re-enter the predictors as
+// primitive states, so a neighbour's conserved vector is p2c(c2p(p2c(w))).
+template <class P>
+__device__ __forceinline__ void forcing_neighbour(const typename P::Params &m, const double *in, long long plane,
+                                                  long long idx, double (&U)[P::NF])
+{
+    double w[P::NF], t[P::NF];
+    load_prim<P::NF>(in, plane, idx, w);
+    P::prim_to_cons(m, w, t);
+    P::cons_to_prim(m, t, w);
+    P::prim_to_cons(m, w, U);
+}
+
 template <class P, int DIM>
 __global__ __launch_bounds__(kForceBS) void forcing_kernel(ForcingArgs<P> a)
 {
@@ -91,85 +188,20 @@ __global__ __launch_bounds__(kForceBS) void forcing_kernel(ForcingArgs<P> a)
     unsigned long long key = 0;
     if (c < cells) {
         const int i = (int)(c / a.nx), j = (int)(c - (long long)i * a.nx);
-        auto cons_rt = [&](int ii, int jj, double (&U)[NF]) {
-            // computeRungeKuttaTimeStep converts every cell to conserved form;
-            // the neighbours re-enter the predictors as primitive states, so a
-            // neighbour's conserved vector is p2c(c2p(p2c(w))).
-            double w[NF], t[NF];
-            load_prim<NF>(a.in, a.plane, (long long)ii * a.nx + jj, w);
-            P::prim_to_cons(m, w, t);
-            P::cons_to_prim(m, t, w);
-            P::prim_to_cons(m, w, U);
-        };
         const int jl = ghost_index(j - 1, a.nx, a.bc), jr = ghost_index(j + 1, a.nx, a.bc);
         double UL[NF], UR[NF], UT[NF], UB[NF];
-        cons_rt(i, jl, UL);
-        cons_rt(i, jr, UR);
+        forcing_neighbour<P>(m, a.in, a.plane, (long long)i * a.nx + jl, UL);
+        forcing_neighbour<P>(m, a.in, a.plane, (long long)i * a.nx + jr, UR);
         if (DIM == 2) {
-            cons_rt(ghost_index(i - 1, a.ny, a.bc), j, UT);
-            cons_rt(ghost_index(i + 1, a.ny, a.bc), j, UB);
+            forcing_neighbour<P>(m, a.in, a.plane, (long long)ghost_index(i - 1, a.ny, a.bc) * a.nx + j, UT);
+            forcing_neighbour<P>(m, a.in, a.plane, (long long)ghost_index(i + 1, a.ny, a.bc) * a.nx + j, UB);
         }
         double w0[NF], Um[NF];
         load_prim<NF>(a.in, a.plane, c, w0);
         P::prim_to_cons(m, w0, Um);
-
-        const int nsrc = T::count(m);
-        double K[3][NS];
-        double S[NF];
-        P::source(m, Um, S);
-#pragma unroll
-        for (int q = 0; q < NS; q++) K[0][q] = ts * S[T::slot(q)];
-        double k4[NS];
-        const double cx = ts / a.dx;
-#pragma unroll
-        for (int st = 0; st < 3; st++) {
-            double Us[NF];
-#pragma unroll
-            for (int k = 0; k < NF; k++) Us[k] = Um[k] + 0.0;
-#pragma unroll
-            for (int q = 0; q < NS; q++) {
-                if (q < nsrc) {
-                    const double kp = K[st][q];
-                    Us[T::slot(q)] = Um[T::slot(q)] + (st < 2 ? (0.5 * kp) : kp);
-                }
-            }
-            // The stage state enters the predictor as a primitive state
-            // (evolveConservedVariableVector takes states): c2p then p2c.
-            double wm[NF], UM[NF], e[NF];
-            P::cons_to_prim(m, Us, wm);
-            P::prim_to_cons(m, wm, UM);
-            if (DIM == 1) {
-                fractional_step<P, 0>(m, UL, UM, UR, a.bias, a.limiter, (st < 2 ? 0.5 : 1.0) * cx, e);
-            } else {
-                const double fx = (st < 2) ? 0.25 : 0.5, fy = (st < 2) ? 0.5 : 1.0;
-                fractional_step<P, 0>(m, UL, UM, UR, a.bias, a.limiter, fx * cx, e);
-                P::prim_to_cons(m, e, UM);
-                fractional_step<P, 1>(m, UT, UM, UB, a.bias, a.limiter, fy * cx, e);
-                P::prim_to_cons(m, e, UM);
-                fractional_step<P, 0>(m, UL, UM, UR, a.bias, a.limiter, fx * cx, e);
-            }
-            double Ue[NF];
-            P::prim_to_cons(m, e, Ue);
-            P::source(m, Ue, S);
-            if (st < 2) {
-#pragma unroll
-                for (int q = 0; q < NS; q++) K[st + 1][q] = ts * S[T::slot(q)];
-            } else {
-#pragma unroll
-                for (int q = 0; q < NS; q++) k4[q] = ts * S[T::slot(q)];
-            }
-        }
-        // computeEvolvedConservedVariableVector (hprforcingsolver.cpp:332-339)
-        double N[NF];
-#pragma unroll
-        for (int k = 0; k < NF; k++) N[k] = Um[k] + 0.0;
-#pragma unroll
-        for (int q = 0; q < NS; q++) {
-            if (q < nsrc) {
-                const int k = T::slot(q);
-                N[k] = Um[k] + ((1.0 / 6.0) * (K[0][q] + ((2.0 * K[1][q]) + ((2.0 * K[2][q]) + k4[q]))));
-            }
-        }
+        double N[NF], K[4][NS];
+        NoStages rec;
+        forcing_cell<P, DIM>(m, UL, UR, UT, UB, Um, ts, a.dx, a.bias, a.limiter, N, K, rec);
         double wn[NF];
         P::cons_to_prim(m, N, wn);
         T::finish(m, w0, wn);

@@ -4,19 +4,9 @@
 #include <cstring>
 #include <string>
 
-#include "vg_grid.h"
-#include "vg_hpr_forcing.h"
-#include "vg_phys_hpr.h"
+#include "vg_hpr_host.h"
 
-namespace vg {
-
-// The relaxation sub-cycle launch for GridState<HPRPhys>::enqueue_step.
-template <>
-struct SourceStage<HPRPhys> : ForcingStage<HPRPhys> {};
-
-}  // namespace vg
-
-using HPRGrid = vg::GridState<vg::HPRPhys>;
+using HPRGrid = vg::HPRGrid;
 
 struct vg_hpr_state {
     HPRGrid g;
@@ -26,45 +16,9 @@ namespace {
 
 using vg::HPRPhys;
 
-HPRPhys::Params to_params(const vg_hpr_params *p)
-{
-    HPRPhys::Params m;
-    m.is_thermal = p->is_thermal ? 1 : 0;
-    m.is_plastic = p->is_plastic ? 1 : 0;
-    m.rho0 = p->reference_density;
-    m.t_ref = p->reference_temperature;
-    m.t0 = p->initial_temperature;
-    m.cv = p->specific_heat_capacity;
-    m.gamma = p->adiabatic_index;
-    m.p_inf = p->stiffening_parameter;
-    m.cs = p->transverse_wave_speed;
-    m.tau1 = p->strain_dissipation_time;
-    m.tau2 = p->thermal_impulse_relaxation_time;
-    m.alpha = p->heat_wave_speed;
-    // The same IEEE values the reference recomputes per call.
-    m.G = m.gamma - 1.0;
-    m.cs2 = m.cs * m.cs;
-    m.a2 = m.alpha * m.alpha;
-    return m;
-}
+HPRPhys::Params to_params(const vg_hpr_params *p) { return vg::hpr_to_params(p); }
 
-int check_params(const vg_hpr_params *mp, int subcycles)
-{
-    VG_CHECK(mp != nullptr, "null material parameters");
-    if (mp->eos != VG_HPR_EOS_STIFFENED_GAS) {
-        vg::set_error("HPR: only the StiffenedGas equation of state is implemented");
-        return VG_ERR_UNSUPPORTED;
-    }
-    if (mp->is_plastic) {
-        vg::set_error("HPR: the plastic (power-law) strain relaxation is not implemented");
-        return VG_ERR_UNSUPPORTED;
-    }
-    VG_CHECK(subcycles >= 0, "subcycles must be >= 0");
-    VG_CHECK(subcycles == 0 || (mp->strain_dissipation_time > 0.0 &&
-                                (!mp->is_thermal || mp->thermal_impulse_relaxation_time > 0.0)),
-             "relaxation times must be positive when sub-cycling the source");
-    return VG_OK;
-}
+int check_params(const vg_hpr_params *mp, int subcycles) { return vg::hpr_check_params(mp, subcycles); }
 
 int check_run_args(int dim, int ny, int nx, double dx, double cfl, double t_final, int max_steps, int order, int bc,
                    int subcycles, const vg_hpr_params *mp, int mode)

@@ -561,6 +561,72 @@ def hpr_solve(prim: np.ndarray, dim: int, dx: float, cfl: float, t_final: float,
     return SolveResult(out, dts[:min(st.steps, dt_cap)].copy(), st, rc)
 
 
+@dataclass
+class HPRAdjointResult:
+    cells: np.ndarray      # final primitive state of the forward solve
+    grad: np.ndarray       # d loss / d initial primitive state
+    grad_tau: np.ndarray   # d loss / d (strain_dissipation_time, thermal_impulse_relaxation_time)
+    dt: np.ndarray         # the time steps the forward chose (held fixed)
+    stats: _lib.RunStats
+
+
+def hpr_adjoint(prim: np.ndarray, seed: np.ndarray, dim: int, dx: float, cfl: float, t_final: float, *,
+                bias: float = 0.0, limiter: int = 0, subcycles: int = 0, params: HPRMaterialParameters | None = None,
+                max_steps: int = 0, bc: int = TRANSMISSIVE, checkpoint_every: int = 0, dt_cap: int = 100000,
+                ctx: Context | None = None) -> HPRAdjointResult:
+    """Reverse-mode derivative of the second-order HPR solve (vg_hpr_adjoint_1d /
+    _2d), the time steps held at the values the forward chose: `grad` is
+    d loss / d initial primitive state given `seed` = d loss / d final state,
+    `grad_tau` the derivative with respect to the two relaxation times."""
+    ctx = ctx or default_context()
+    params = params or HPRMaterialParameters()
+    w = np.ascontiguousarray(prim, dtype=np.float64)
+    sd = np.ascontiguousarray(seed, dtype=np.float64)
+    if sd.shape != w.shape:
+        raise ValueError("seed must have the shape of the state")
+    out, grad, gtau = np.empty_like(w), np.empty_like(w), np.zeros(2)
+    dts = np.empty(dt_cap)
+    st = _lib.RunStats()
+    mp = params.c_struct()
+    if dim == 1:
+        if w.ndim != 2 or w.shape[1] != 17:
+            raise ValueError("1D HPR state must be (cells, 17)")
+        rc = ctx.lib.vg_hpr_adjoint_1d(ctx.handle, _lib.as_dptr(w), _lib.as_dptr(out), _lib.as_dptr(sd),
+                                       _lib.as_dptr(grad), _lib.as_dptr(gtau), w.shape[0], dx, cfl, t_final,
+                                       max_steps, bias, limiter, bc, subcycles, C.byref(mp), checkpoint_every,
+                                       _lib.as_dptr(dts), dt_cap, C.byref(st))
+    else:
+        if w.ndim != 3 or w.shape[2] != 17:
+            raise ValueError("2D HPR state must be (rows, columns, 17)")
+        rc = ctx.lib.vg_hpr_adjoint_2d(ctx.handle, _lib.as_dptr(w), _lib.as_dptr(out), _lib.as_dptr(sd),
+                                       _lib.as_dptr(grad), _lib.as_dptr(gtau), w.shape[0], w.shape[1], dx, cfl,
+                                       t_final, max_steps, bias, limiter, bc, subcycles, C.byref(mp),
+                                       checkpoint_every, _lib.as_dptr(dts), dt_cap, C.byref(st))
+    check(rc, "vg_hpr_adjoint")
+    return HPRAdjointResult(out, grad, gtau, dts[:min(st.steps, dt_cap)].copy(), st)
+
+
+def hpr_solve_fixed_dt(prim: np.ndarray, dim: int, dx: float, dts, *, bias: float = 0.0, limiter: int = 0,
+                       subcycles: int = 0, params: HPRMaterialParameters | None = None, bc: int = TRANSMISSIVE,
+                       ctx: Context | None = None) -> np.ndarray:
+    """The second-order HPR solve with prescribed time steps (the map
+    hpr_adjoint differentiates)."""
+    ctx = ctx or default_context()
+    params = params or HPRMaterialParameters()
+    w = np.ascontiguousarray(prim, dtype=np.float64)
+    d = np.ascontiguousarray(dts, dtype=np.float64)
+    out = np.empty_like(w)
+    mp = params.c_struct()
+    if dim == 1:
+        rc = ctx.lib.vg_hpr_solve_fixed_dt_1d(ctx.handle, _lib.as_dptr(w), _lib.as_dptr(out), w.shape[0], dx,
+                                              _lib.as_dptr(d), d.size, bias, limiter, bc, subcycles, C.byref(mp))
+    else:
+        rc = ctx.lib.vg_hpr_solve_fixed_dt_2d(ctx.handle, _lib.as_dptr(w), _lib.as_dptr(out), w.shape[0], w.shape[1],
+                                              dx, _lib.as_dptr(d), d.size, bias, limiter, bc, subcycles, C.byref(mp))
+    check(rc, "vg_hpr_solve_fixed_dt")
+    return out
+
+
 class HPRSecondOrderSolver:
     """HPRSecondOrderSolver (Solvers/hprsecondordersolver.h:56, 67)."""
 
