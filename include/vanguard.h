@@ -37,8 +37,8 @@ enum { VG_BC_TRANSMISSIVE = 0, VG_BC_PERIODIC = 1 };
 
 /* VG_MODE_EXACT: the reference's operation order, bit-identical to its CPU
  * solver.  VG_MODE_FAST: algebraically reduced arithmetic (reciprocals reused,
- * no primitive/conserved round trips), same scheme; agrees with EXACT to about
- * 1e-13 relative and is not bit-identical. */
+ * no primitive/conserved round trips, multiply-adds fused), same scheme; agrees
+ * with EXACT to about 1e-13 relative and is not bit-identical. */
 enum { VG_MODE_EXACT = 0, VG_MODE_FAST = 1 };
 
 typedef struct vg_ctx vg_ctx;
@@ -86,12 +86,19 @@ int vg_ctx_set_graphs(vg_ctx *ctx, int enabled);
 /* Tile-shape autotuner (default on): the first run on a grid shape times the
  * candidate y-sweep shapes (threads per block x rows per strip) on that grid
  * and keeps the fastest for the context.  vg_ctx_tuned_shape returns the chosen
- * variant for a key "physics/order/mode/ny/nx" (forward Y sweeps, variants 0-3)
+ * variant for a key "physics/order/mode/ny/nx" (forward Y sweeps, variants 0-3;
+ * physics "euler", or "euler_planar" for grids swept with 4 fields)
  * or "euler_adjoint_x/ny/nx" / "euler_adjoint_y/ny/nx" (adjoint sweeps, variants
  * 0-3; a 1D grid of n cells has ny = 1, nx = n), -1 if not tuned
  * yet.  vg_ctx_set_tuned_shape pins a variant for a key (for example one saved
  * from an earlier process); -1 forgets it.  Shapes change speed, never results. */
 int vg_ctx_set_autotune(vg_ctx *ctx, int enabled);
+/* Constant-field specialisation (default on): an Euler grid whose out-of-plane
+ * velocity is +0 and reaction progress exactly 1 in every cell (checked at the
+ * start of each run) is swept with a 4-field kernel that carries rho, u, v, p
+ * and leaves those two planes as they are -- the same bits as the 6-field
+ * sweep, which keeps them at +0 and 1 exactly.  No reference counterpart. */
+int vg_ctx_set_reduced_fields(vg_ctx *ctx, int enabled);
 int vg_ctx_tuned_shape(vg_ctx *ctx, const char *key, int *variant);
 int vg_ctx_set_tuned_shape(vg_ctx *ctx, const char *key, int variant);
 /* Frees the device workspace the context keeps between calls (the adjoint's

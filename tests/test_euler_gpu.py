@@ -155,3 +155,42 @@ def test_invalid_arguments(vg_ctx):
     assert r.status == _lib.VG_ERR_ARG
     r = S.euler_solve(w[:1], 1, 0.1, 0.5, 0.1, raise_on_error=False)    # one cell: no mirror ghosts
     assert r.status == _lib.VG_ERR_ARG
+
+
+@pytest.mark.parametrize("dim,order,mode", [(2, 2, S.EXACT), (2, 1, S.EXACT), (1, 2, S.EXACT), (2, 2, S.FAST)])
+def test_reduced_field_sweeps_give_the_full_state_bits(vg_ctx, dim, order, mode):
+    """A grid at rest out of plane (w = +0) with lambda = 1 everywhere is swept
+    by the 4-field kernels; the 6-field kernels (the context switch off) give
+    the same bits in EXACT mode, and both keep w = +0 and lambda = 1 exactly
+    (FAST: within 1e-13).
+    One cell off the constants sends the run back to the 6-field kernels."""
+    w = problems.disc_2d(96)
+    if dim == 1:
+        w = np.ascontiguousarray(w[48])
+    dx = 2.0 / 96
+    full = S.Context()
+    try:
+        full.set_reduced_fields(False)
+        kw = dict(order=order, max_steps=30, mode=mode, bc=0)
+        a = S.euler_solve(w, dim, dx, 0.8, math.inf, ctx=full, **kw)
+        b = S.euler_solve(w, dim, dx, 0.8, math.inf, **kw)
+        if mode == S.EXACT:
+            O.assert_bitwise(b.dt, a.dt, "dt")
+            O.assert_bitwise(b.cells, a.cells, "state")
+        else:
+            assert rel_l1(cons(b.cells), cons(a.cells)) <= 1e-13
+        # FAST mode's 6-field lambda = (rho lambda) rcp(rho) is 1 to within an ulp
+        for r in ((a, b) if mode == S.EXACT else (b,)):
+            assert not np.any(r.cells[..., 3].view(np.uint64)), "w must stay +0"
+            assert np.all(r.cells[..., 5] == 1.0), "lambda must stay 1"
+        w2 = w.copy()
+        w2[(0,) * (w2.ndim - 1) + (5,)] = 0.999
+        c = S.euler_solve(w2, dim, dx, 0.8, math.inf, **kw)
+        d = S.euler_solve(w2, dim, dx, 0.8, math.inf, ctx=full, **kw)
+        if mode == S.EXACT:
+            O.assert_bitwise(c.cells, d.cells, "state with a reacting cell")
+            ref, dref = O.euler_solve(w2, order=order, dim=dim, dx=dx, cfl=0.8, tfinal=math.inf, max_steps=30,
+                                      bias=0.0, limiter=0, params=[1.4, 0.0, 25.0, 0.0, 0.25, 250.0], bc=0)
+            O.assert_bitwise(c.cells, ref, "state with a reacting cell vs the oracle")
+    finally:
+        full.close()

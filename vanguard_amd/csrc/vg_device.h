@@ -71,6 +71,36 @@ struct Tiles {
     static constexpr bool tune = true;
 };
 
+// Plane of field f of a physics' state in the grid's structure of arrays: the
+// identity, except for a physics that carries a subset of a wider state.
+template <class P>
+struct PlaneMap {
+    __device__ static constexpr int of(int f) { return f; }
+};
+
+template <class P>
+__device__ __forceinline__ void load_state(const double *__restrict__ in, long long plane, long long idx,
+                                           double (&w)[P::NF])
+{
+#pragma unroll
+    for (int f = 0; f < P::NF; f++) w[f] = in[PlaneMap<P>::of(f) * plane + idx];
+}
+
+template <class P>
+__device__ __forceinline__ void store_state(double *__restrict__ out, long long plane, long long idx,
+                                            const double (&w)[P::NF])
+{
+#pragma unroll
+    for (int f = 0; f < P::NF; f++) out[PlaneMap<P>::of(f) * plane + idx] = w[f];
+}
+
+// The narrower physics a grid of P may run with when some of its fields are
+// constant (void: none).  Chosen per run by an exact check of those fields.
+template <class P>
+struct ReducedPhys {
+    using type = void;
+};
+
 // Per-grid time-step clock kept in device memory so that a whole run can be
 // enqueued (or captured in a graph) without host round trips.  The step logic
 // is Solvers::computeStableTimeStep (Solvers/solvers.cpp:389-395, 413-419,

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "vg_runtime.h"
 #include "vg_sweep.h"
@@ -61,6 +62,21 @@ struct SourceStage {
         return 0;
     }
 };
+
+// Whether the planes a narrower physics drops hold their constant bit
+// patterns in every cell: any mismatch clears *ok (every writer stores 0).
+template <class R>
+__global__ __launch_bounds__(256) void constant_planes_kernel(const double *__restrict__ buf, long long plane,
+                                                              long long cells, int *ok)
+{
+    for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < cells; c += (long long)gridDim.x * 256) {
+        bool same = true;
+#pragma unroll
+        for (int i = 0; i < R::kConst; i++)
+            same = same && (__double_as_longlong(buf[R::plane(i) * plane + c]) == (long long)R::bits(i));
+        if (!same) *ok = 0;
+    }
+}
 
 // ---- y-sweep shapes and their autotuner --------------------------------------
 
@@ -156,6 +172,8 @@ struct GridState {
     typename P::Params last_params{};
     int last_bc = -1;
     int y_variant = 0;             // y-sweep shape (autotuned per grid shape)
+    bool reduced = false;          // this run sweeps with ReducedPhys<P> (constant planes checked)
+    int *flag = nullptr;           // device flag of the constant-plane check (lazy)
     Clock *tune_clk = nullptr;     // scratch clock for tuning launches (lazy)
     hipGraphExec_t graph = nullptr;  // captured chunk of steps
     std::string graph_key;           // arguments the graph was captured with
@@ -191,6 +209,7 @@ struct GridState {
         hipFree(scan);
         hipFree(partials);
         hipFree(tune_clk);
+        hipFree(flag);
         if (graph) hipGraphExecDestroy(graph);
     }
 
@@ -275,11 +294,12 @@ struct GridState {
         return VG_OK;
     }
 
-    template <int ORDER, int MODE>
-    int launch_sweep(int dir, const double *in, double *out, double dt_scale, int fuse, int bc, double dx,
-                     double bias, int limiter, const typename P::Params &m)
+    // One sweep with physics Q (P, or its reduced physics on the same planes).
+    template <class Q, int ORDER, int MODE>
+    int launch_sweep_as(int dir, const double *in, double *out, double dt_scale, int fuse, int bc, double dx,
+                        double bias, int limiter, const typename P::Params &m)
     {
-        SweepArgs<P> a;
+        SweepArgs<Q> a;
         a.in = in;
         a.out = out;
         a.plane = plane;
@@ -300,12 +320,53 @@ struct GridState {
             dim3 grid((nx + (kBSX - 2 * G) - 1) / (kBSX - 2 * G), (ny + kRX - 1) / kRX);
             if (fuse) n_partials = (int)(grid.x * grid.y);
             return ctx->timed(ORDER == 2 ? "sweep_x_slic" : "sweep_x_force", [&] {
-                sweep_x_kernel<P, ORDER, MODE, kBSX, kRX><<<grid, kBSX, 0, ctx->stream>>>(a);
+                sweep_x_kernel<Q, ORDER, MODE, kBSX, kRX><<<grid, kBSX, 0, ctx->stream>>>(a);
             });
         }
         // The y sweep never ends a step, so it never writes the CFL partials.
         a.fuse_max = 0;
-        return launch_y<P, ORDER, MODE>(ctx, y_variant, a);
+        return launch_y<Q, ORDER, MODE>(ctx, y_variant, a);
+    }
+
+    template <int ORDER, int MODE>
+    int launch_sweep(int dir, const double *in, double *out, double dt_scale, int fuse, int bc, double dx,
+                     double bias, int limiter, const typename P::Params &m)
+    {
+        using R = typename ReducedPhys<P>::type;
+        if constexpr (!std::is_void_v<R>) {
+            if (reduced) return launch_sweep_as<R, ORDER, MODE>(dir, in, out, dt_scale, fuse, bc, dx, bias, limiter, m);
+        }
+        return launch_sweep_as<P, ORDER, MODE>(dir, in, out, dt_scale, fuse, bc, dx, bias, limiter, m);
+    }
+
+    // Decides this run's physics: the reduced one when the context allows it and
+    // every cell of the current buffer holds the constant planes; the other
+    // buffer then receives those planes, so the sweeps may leave them alone.
+    int choose_physics()
+    {
+        reduced = false;
+        using R = typename ReducedPhys<P>::type;
+        if constexpr (!std::is_void_v<R>) {
+            if (!ctx->reduce_fields) return VG_OK;
+            if (!flag) VG_HIP(hipMalloc(&flag, sizeof(int)));
+            const int one = 1;
+            VG_HIP(hipMemcpyAsync(flag, &one, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+            constant_planes_kernel<ReducedPhys<P>><<<blocks_for(cells), 256, 0, ctx->stream>>>(buf[cur], plane, cells,
+                                                                                            flag);
+            int rc = launch_status("constant_planes");
+            if (rc) return rc;
+            int ok = 0;
+            VG_HIP(hipMemcpyAsync(&ok, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+            VG_HIP(hipStreamSynchronize(ctx->stream));
+            if (!ok) return VG_OK;
+            for (int i = 0; i < ReducedPhys<P>::kConst; i++) {
+                const long long off = (long long)ReducedPhys<P>::plane(i) * plane;
+                VG_HIP(hipMemcpyAsync(buf[1 - cur] + off, buf[cur] + off, sizeof(double) * cells,
+                                      hipMemcpyDeviceToDevice, ctx->stream));
+            }
+            reduced = true;
+        }
+        return VG_OK;
     }
 
     template <int ORDER, int MODE>
@@ -343,11 +404,38 @@ struct GridState {
         return VG_OK;
     }
 
+    // The y-sweep shape for physics Q on this grid (tune_y on the current buffer).
+    template <class Q, int ORDER, int MODE>
+    int tune_as(int bc, double dx, double bias, int limiter, const typename P::Params &m)
+    {
+        SweepArgs<Q> ta;
+        ta.in = buf[cur];
+        ta.out = buf[1 - cur];
+        ta.plane = plane;
+        ta.ny = ny;
+        ta.nx = nx;
+        ta.bc = bc;
+        ta.limiter = limiter;
+        ta.dx = dx;
+        ta.bias = bias;
+        ta.dt_scale = 1.0;
+        ta.dim = dim;
+        ta.fuse_max = 0;
+        ta.m = m;
+        ta.clk = tune_clk;
+        ta.partials = nullptr;
+        return tune_y<Q, ORDER, MODE>(ctx, tune_key(Q::name, ORDER, MODE, ny, nx), ta, y_variant);
+    }
+
     template <int ORDER, int MODE>
     int run_impl(double dx, double cfl, double t_final, int max_steps, double bias, int limiter, int bc,
                  const typename P::Params &m, double *dt_log, int dt_cap, vg_run_stats *stats)
     {
         auto w0 = std::chrono::steady_clock::now();
+        {
+            int rc = choose_physics();
+            if (rc) return rc;
+        }
         const int fused = fused_count(ORDER);
         if (!max_valid || last_bc != bc || std::memcmp(&last_params, &m, sizeof(m)) != 0 || n_partials != fused) {
             int rc = refresh_max(bc, m, fused);
@@ -358,23 +446,14 @@ struct GridState {
             Clock tc{};
             tc.dt = 1e-3 * dx;
             VG_HIP(hipMemcpyAsync(tune_clk, &tc, sizeof(Clock), hipMemcpyHostToDevice, ctx->stream));
-            SweepArgs<P> ta;
-            ta.in = buf[cur];
-            ta.out = buf[1 - cur];
-            ta.plane = plane;
-            ta.ny = ny;
-            ta.nx = nx;
-            ta.bc = bc;
-            ta.limiter = limiter;
-            ta.dx = dx;
-            ta.bias = bias;
-            ta.dt_scale = 1.0;
-            ta.dim = dim;
-            ta.fuse_max = 0;
-            ta.m = m;
-            ta.clk = tune_clk;
-            ta.partials = nullptr;
-            int rc = tune_y<P, ORDER, MODE>(ctx, tune_key(P::name, ORDER, MODE, ny, nx), ta, y_variant);
+            using R = typename ReducedPhys<P>::type;
+            int rc = VG_OK;
+            if constexpr (!std::is_void_v<R>) {
+                if (reduced) rc = tune_as<R, ORDER, MODE>(bc, dx, bias, limiter, m);
+                else rc = tune_as<P, ORDER, MODE>(bc, dx, bias, limiter, m);
+            } else {
+                rc = tune_as<P, ORDER, MODE>(bc, dx, bias, limiter, m);
+            }
             if (rc) return rc;
         }
         Clock h;
@@ -414,7 +493,7 @@ struct GridState {
             key += "|" + std::to_string(ORDER) + "|" + std::to_string(MODE) + "|" + std::to_string(bc) + "|" +
                    std::to_string(limiter) + "|" + std::to_string((unsigned long long)(uintptr_t)dlog) + "|" +
                    std::to_string(dt_cap) + "|" + std::to_string(subcycles) + "|" + std::to_string(y_variant) + "|" +
-                   std::to_string(cur) + "|" + std::to_string(n_partials);
+                   std::to_string(cur) + "|" + std::to_string(n_partials) + "|" + std::to_string((int)reduced);
             if (!graph || key != graph_key) {
                 if (graph) {
                     hipGraphExecDestroy(graph);

@@ -49,6 +49,9 @@ struct vg_ctx {
     // Time the candidate tile shapes on first use of a grid shape and keep the
     // fastest (key: physics/order/mode/ny/nx -> variant index).
     bool autotune = true;
+    // Run a grid with its narrower physics (ReducedPhys) when the fields that
+    // physics drops are constant; results are the same bits either way.
+    bool reduce_fields = true;
     std::map<std::string, int> tuned;
     std::map<std::string, vg::KernelTimer> timers;
     // Event pairs recorded around launches and not yet resolved.

@@ -164,6 +164,13 @@ int vg_ctx_set_autotune(vg_ctx *ctx, int enabled)
     return VG_OK;
 }
 
+int vg_ctx_set_reduced_fields(vg_ctx *ctx, int enabled)
+{
+    VG_CHECK(ctx != nullptr, "vg_ctx_set_reduced_fields: null context");
+    ctx->reduce_fields = enabled != 0;
+    return VG_OK;
+}
+
 int vg_ctx_tuned_shape(vg_ctx *ctx, const char *key, int *variant)
 {
     VG_CHECK(ctx != nullptr && key != nullptr && variant != nullptr, "vg_ctx_tuned_shape: null argument");

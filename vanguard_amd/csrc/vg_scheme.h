@@ -1,7 +1,9 @@
 // The SLIC / FORCE building blocks, generic over the physics P (Euler, MHD,
 // HPR) and the sweep direction, on register arrays.  EXACT mode follows the
-// reference's operation order (see the citations); FAST mode is the same scheme
-// with the primitive/conserved round trips removed.
+// reference's operation order (see the citations) with every product and sum
+// rounded separately; FAST mode is the same scheme with the primitive/conserved
+// round trips removed and multiply-adds fused (fp contract(fast) in its
+// branches only).
 #pragma once
 
 #include "vg_device.h"
@@ -33,6 +35,7 @@ __device__ __forceinline__ double limiter_value(double r, double bias, int limit
 // FAST-mode limiter: the same piecewise functions with Newton-refined reciprocals.
 __device__ __forceinline__ double limiter_value_fast(double r, double bias, int limiter)
 {
+#pragma clang fp contract(fast)
     if (r < 0.0) return 0.0;
     if (limiter == 0) {
         if (r < 0.5) return 2.0 * r;
@@ -100,27 +103,50 @@ struct Scheme {
     {
         double xl[NF], xr[NF];
         const double cl = 0.5 * (1.0 + bias), cr = 0.5 * (1.0 - bias);
+        if constexpr (MODE == MODE_EXACT) {
 #pragma unroll
-        for (int k = 0; k < NF; k++) {
-            const double dl = UM[k] - UL[k];
-            const double dr = UR[k] - UM[k];
-            double d = (cl * dl) + (cr * dr);
-            // |difference| < 1e-5 is replaced by +1e-5 (sign dropped) before the ratio.
-            const double num = (fabs(dl) < 1e-5) ? 1e-5 : dl;
-            const double den = (fabs(dr) < 1e-5) ? 1e-5 : dr;
-            if (MODE == MODE_EXACT) d *= limiter_value(num / den, bias, limiter);
-            else d *= limiter_value_fast(num * rcp_nr(den), bias, limiter);
-            xl[k] = UM[k] - (0.5 * d);
-            xr[k] = UM[k] + (0.5 * d);
-        }
-        double Fl[NF], Fr[NF];
-        flux_of_cons(m, xl, Fl);
-        flux_of_cons(m, xr, Fr);
+            for (int k = 0; k < NF; k++) {
+                const double dl = UM[k] - UL[k];
+                const double dr = UR[k] - UM[k];
+                double d = (cl * dl) + (cr * dr);
+                // |difference| < 1e-5 is replaced by +1e-5 (sign dropped) before the ratio.
+                const double num = (fabs(dl) < 1e-5) ? 1e-5 : dl;
+                const double den = (fabs(dr) < 1e-5) ? 1e-5 : dr;
+                d *= limiter_value(num / den, bias, limiter);
+                xl[k] = UM[k] - (0.5 * d);
+                xr[k] = UM[k] + (0.5 * d);
+            }
+            double Fl[NF], Fr[NF];
+            flux_of_cons(m, xl, Fl);
+            flux_of_cons(m, xr, Fr);
 #pragma unroll
-        for (int k = 0; k < NF; k++) {
-            const double evo = half_c * (Fl[k] - Fr[k]);
-            xl[k] = xl[k] + evo;
-            xr[k] = xr[k] + evo;
+            for (int k = 0; k < NF; k++) {
+                const double evo = half_c * (Fl[k] - Fr[k]);
+                xl[k] = xl[k] + evo;
+                xr[k] = xr[k] + evo;
+            }
+        } else {
+#pragma clang fp contract(fast)
+#pragma unroll
+            for (int k = 0; k < NF; k++) {
+                const double dl = UM[k] - UL[k];
+                const double dr = UR[k] - UM[k];
+                double d = (cl * dl) + (cr * dr);
+                const double num = (fabs(dl) < 1e-5) ? 1e-5 : dl;
+                const double den = (fabs(dr) < 1e-5) ? 1e-5 : dr;
+                d *= limiter_value_fast(num * rcp_nr(den), bias, limiter);
+                xl[k] = UM[k] - (0.5 * d);
+                xr[k] = UM[k] + (0.5 * d);
+            }
+            double Fl[NF], Fr[NF];
+            flux_of_cons(m, xl, Fl);
+            flux_of_cons(m, xr, Fr);
+#pragma unroll
+            for (int k = 0; k < NF; k++) {
+                const double evo = half_c * (Fl[k] - Fr[k]);
+                xl[k] = xl[k] + evo;
+                xr[k] = xr[k] + evo;
+            }
         }
         if (MODE == MODE_EXACT) {
             // setConservedVariableVector, then the FORCE flux converts the state
@@ -148,16 +174,27 @@ struct Scheme {
     __device__ static __forceinline__ void force_face(const Params &m, const ConsFlux<NF> &a, const ConsFlux<NF> &b,
                                                       double lf_c, double ri_c, double (&F)[NF])
     {
-        double us[NF], lf[NF];
+        double us[NF], lf[NF], ri[NF];
+        if constexpr (MODE == MODE_EXACT) {
 #pragma unroll
-        for (int k = 0; k < NF; k++) {
-            lf[k] = 0.5 * ((a.F[k] + b.F[k]) + (lf_c * (a.U[k] - b.U[k])));
-            us[k] = 0.5 * ((a.U[k] + b.U[k]) + (ri_c * (a.F[k] - b.F[k])));
+            for (int k = 0; k < NF; k++) {
+                lf[k] = 0.5 * ((a.F[k] + b.F[k]) + (lf_c * (a.U[k] - b.U[k])));
+                us[k] = 0.5 * ((a.U[k] + b.U[k]) + (ri_c * (a.F[k] - b.F[k])));
+            }
+            flux_of_cons(m, us, ri);
+#pragma unroll
+            for (int k = 0; k < NF; k++) F[k] = 0.5 * (lf[k] + ri[k]);
+        } else {
+#pragma clang fp contract(fast)
+#pragma unroll
+            for (int k = 0; k < NF; k++) {
+                lf[k] = 0.5 * ((a.F[k] + b.F[k]) + (lf_c * (a.U[k] - b.U[k])));
+                us[k] = 0.5 * ((a.U[k] + b.U[k]) + (ri_c * (a.F[k] - b.F[k])));
+            }
+            flux_of_cons(m, us, ri);
+#pragma unroll
+            for (int k = 0; k < NF; k++) F[k] = 0.5 * (lf[k] + ri[k]);
         }
-        double ri[NF];
-        flux_of_cons(m, us, ri);
-#pragma unroll
-        for (int k = 0; k < NF; k++) F[k] = 0.5 * (lf[k] + ri[k]);
     }
 
     // FirstOrderSolver::computeFORCEUpdate (firstordersolver.cpp:428-433) followed
@@ -166,10 +203,16 @@ struct Scheme {
                                                   const double (&Fp)[NF], double up_c, double (&w)[NF])
     {
         double N[NF];
+        if constexpr (MODE == MODE_EXACT) {
 #pragma unroll
-        for (int k = 0; k < NF; k++) N[k] = U[k] + (up_c * (Fm[k] - Fp[k]));
-        if (MODE == MODE_EXACT) P::cons_to_prim(m, N, w);
-        else P::cons_to_prim_fast(m, N, w);
+            for (int k = 0; k < NF; k++) N[k] = U[k] + (up_c * (Fm[k] - Fp[k]));
+            P::cons_to_prim(m, N, w);
+        } else {
+#pragma clang fp contract(fast)
+#pragma unroll
+            for (int k = 0; k < NF; k++) N[k] = U[k] + (up_c * (Fm[k] - Fp[k]));
+            P::cons_to_prim_fast(m, N, w);
+        }
     }
 };
 

@@ -159,7 +159,7 @@ __device__ __forceinline__ unsigned long long row_tile(const typename P::Params 
 #pragma unroll
         for (int f = 0; f < NF; f++) Fm[f] = s_a[f * BS + tid - 1];
         S::update(m, U, Fm, Fp, sc.up_c, wn);
-        store_prim<NF>(out, plane, out_idx, wn);
+        store_state<P>(out, plane, out_idx, wn);
         if (want_key) key = max_key(P::wave_speed(m, wn, dim));
     }
     __syncthreads();
@@ -192,13 +192,13 @@ __global__ __launch_bounds__(BS) void sweep_x_kernel(SweepArgs<P> a)
     // Each block walks RX rows; the next row's cells are loaded while the
     // current row is computed.
     double wnext[NF];
-    load_prim<NF>(a.in, a.plane, (long long)i_begin * nx + jj, wnext);
+    load_state<P>(a.in, a.plane, (long long)i_begin * nx + jj, wnext);
     for (int i = i_begin; i < i_end; i++) {
         const long long row = (long long)i * nx;
         double w[NF];
 #pragma unroll
         for (int f = 0; f < NF; f++) w[f] = wnext[f];
-        if (i + 1 < i_end) load_prim<NF>(a.in, a.plane, row + nx + jj, wnext);
+        if (i + 1 < i_end) load_state<P>(a.in, a.plane, row + nx + jj, wnext);
         const unsigned long long k2 = row_tile<P, ORDER, MODE, BS>(m, sc, a.bias, a.limiter, w, j < nx, a.out, a.plane,
                                                                    row + j, a.dim, a.fuse_max != 0, s_a, s_e);
         key = (k2 > key) ? k2 : key;
@@ -228,16 +228,16 @@ __global__ __launch_bounds__(BS) void sweep_y_kernel(SweepArgs<P> a)
         auto row_of = [&](int r) -> long long { return (long long)ghost_index(r, ny, a.bc) * nx + j; };
         if (ORDER == 2) {
             double A[NF], B[NF], C[NF], w[NF];
-            load_prim<NF>(a.in, a.plane, row_of(i0 - 2), w);
+            load_state<P>(a.in, a.plane, row_of(i0 - 2), w);
             S::cons(m, w, A);
-            load_prim<NF>(a.in, a.plane, row_of(i0 - 1), w);
+            load_state<P>(a.in, a.plane, row_of(i0 - 1), w);
             S::cons(m, w, B);
-            load_prim<NF>(a.in, a.plane, row_of(i0), w);   // row e + 1 of the first iteration
+            load_state<P>(a.in, a.plane, row_of(i0), w);   // row e + 1 of the first iteration
             ConsFlux<NF> er_prev;
             double Fprev[NF];
             for (int e = i0 - 1; e <= i1; e++) {
                 S::cons(m, w, C);
-                if (e + 2 <= i1 + 1) load_prim<NF>(a.in, a.plane, row_of(e + 2), w);   // prefetch
+                if (e + 2 <= i1 + 1) load_state<P>(a.in, a.plane, row_of(e + 2), w);   // prefetch
                 ConsFlux<NF> L, R;
                 S::evolve(m, A, B, C, a.bias, a.limiter, sc.half_c, L, R);
                 double Fnew[NF];
@@ -245,7 +245,7 @@ __global__ __launch_bounds__(BS) void sweep_y_kernel(SweepArgs<P> a)
                 if (e >= i0 + 1) {
                     double wn[NF];
                     S::update(m, A, Fprev, Fnew, sc.up_c, wn);                       // row e - 1
-                    store_prim<NF>(a.out, a.plane, (long long)(e - 1) * nx + j, wn);
+                    store_state<P>(a.out, a.plane, (long long)(e - 1) * nx + j, wn);
                     if (a.fuse_max) {
                         const unsigned long long k2 = max_key(P::wave_speed(m, wn, a.dim));
                         key = (k2 > key) ? k2 : key;
@@ -263,19 +263,19 @@ __global__ __launch_bounds__(BS) void sweep_y_kernel(SweepArgs<P> a)
         } else {
             double w[NF];
             ConsFlux<NF> prev;
-            load_prim<NF>(a.in, a.plane, row_of(i0 - 1), w);
+            load_state<P>(a.in, a.plane, row_of(i0 - 1), w);
             S::cons_flux_of_prim(m, w, prev);
             double Fprev[NF];
             for (int e = i0; e <= i1; e++) {
                 ConsFlux<NF> cur;
-                load_prim<NF>(a.in, a.plane, row_of(e), w);
+                load_state<P>(a.in, a.plane, row_of(e), w);
                 S::cons_flux_of_prim(m, w, cur);
                 double Fnew[NF];
                 S::force_face(m, prev, cur, sc.lf_c, sc.ri_c, Fnew);                 // face e - 1/2
                 if (e >= i0 + 1) {
                     double wn[NF];
                     S::update(m, prev.U, Fprev, Fnew, sc.up_c, wn);                  // row e - 1
-                    store_prim<NF>(a.out, a.plane, (long long)(e - 1) * nx + j, wn);
+                    store_state<P>(a.out, a.plane, (long long)(e - 1) * nx + j, wn);
                     if (a.fuse_max) {
                         const unsigned long long k2 = max_key(P::wave_speed(m, wn, a.dim));
                         key = (k2 > key) ? k2 : key;

@@ -63,6 +63,11 @@ class Context:
         """Time the candidate y-sweep shapes on first use of a grid shape (default on)."""
         check(self.lib.vg_ctx_set_autotune(self.handle, int(on)), "vg_ctx_set_autotune")
 
+    def set_reduced_fields(self, on: bool):
+        """Sweep Euler grids whose out-of-plane velocity and reaction progress are
+        constant with the 4-field kernels (default on; same bits either way)."""
+        check(self.lib.vg_ctx_set_reduced_fields(self.handle, int(on)), "vg_ctx_set_reduced_fields")
+
     def tuned_shape(self, physics: str, order: int, mode: int, ny: int, nx: int) -> int:
         v = C.c_int()
         key = f"{physics}/{order}/{mode}/{ny}/{nx}".encode()
