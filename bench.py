@@ -59,10 +59,10 @@ CFL = 0.8
 LIMITER = 0           # superbee (Solvers/slopelimiters.cpp:62-65)
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_SWEEP = 96   # read 6 + write 6 FP64 primitive fields
-# HBM bytes per launch of the 4096^2 sweeps, FETCH_SIZE x 2 + WRITE_SIZE x 1
-# (scales from a plain 1 GiB copy), measured with rocprofv3 --pmc on the FAST
-# kernels; profiles/euler2d_4096_slic_fast_hbm_traffic.md.
-MEASURED_TRAFFIC = {"sweep_x_slic": 1673988183, "sweep_y_slic": 1660324546}
+# HBM bytes per launch of the 4096^2 FAST sweeps (the 4-field kernels this
+# problem takes), FETCH_SIZE x 2 + WRITE_SIZE x 1 (scales from a plain 1 GiB
+# copy), measured with rocprofv3 --pmc; profiles/euler2d_4096_slic_fast_hbm_traffic.md.
+MEASURED_TRAFFIC = {"sweep_x_slic": 1116074640, "sweep_y_slic": 1106876032}
 METRIC = "Mcell-updates/s FP64 SLIC 2D Euler 4096²; %HBM peak; adjoint/forward cost"
 
 
@@ -123,6 +123,11 @@ def cpu_baseline(sample_n: int, steps: int):
 
 
 def run_mode(mode_name, w, steps, warmup, ctx):
+    """Times `steps` steps as a user runs them (captured HIP graphs replayed,
+    no per-launch events), then the same steps again with per-launch HIP events
+    on the solver's stream for the per-kernel durations (events split the graph
+    replay into single launches, so that run is slower and is not the value)."""
+    import torch
     from vanguard_amd import solvers as S
 
     mode = S.EXACT if mode_name == "exact" else S.FAST
@@ -132,27 +137,35 @@ def run_mode(mode_name, w, steps, warmup, ctx):
         g0.load(w)
         g0.run(dx, CFL, max_steps=warmup, limiter=LIMITER, mode=mode)
         g0.close()
-    g = S.EulerGrid(2, N, N, ctx=ctx)
-    g.load(w)
-    g.max_wave_speed()                    # the first step's reduction, outside the timed region
-    ctx.reset_timing()
-    ctx.set_timing(True)
-    import torch
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    st, dts = g.run(dx, CFL, max_steps=steps, limiter=LIMITER, mode=mode, dt_cap=steps)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ctx.set_timing(False)
+
+    def timed(events):
+        g = S.EulerGrid(2, N, N, ctx=ctx)
+        g.load(w)
+        g.max_wave_speed()                    # the first step's reduction, outside the timed region
+        ctx.reset_timing()
+        ctx.set_timing(events)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st, dts = g.run(dx, CFL, max_steps=steps, limiter=LIMITER, mode=mode, dt_cap=steps)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        ctx.set_timing(False)
+        out = g.store()
+        g.close()
+        return st, dts, elapsed, out
+
+    if warmup:
+        timed(False)                          # captures this configuration's graph
+    st, dts, elapsed, out = timed(False)
+    st2, dts2, elapsed2, out2 = timed(True)
     kernels = {}
     for k in ("sweep_x_slic", "sweep_y_slic"):
         n, ms = ctx.kernel_time(k)
         kernels[k] = (n, ms)
-    out = g.store()
+    same = bool(np.array_equal(out.view(np.uint64), out2.view(np.uint64)))
     nonfinite = int((~(out[..., 0] > 0)).sum() + (~np.isfinite(out)).sum())
-    g.close()
-    return dict(steps=st.steps, t=st.t, elapsed=elapsed, device_ms=st.device_ms, kernels=kernels, dts=dts,
-                nonfinite=nonfinite, out=out)
+    return dict(steps=st.steps, t=st.t, elapsed=elapsed, evented_elapsed=elapsed2, device_ms=st.device_ms,
+                kernels=kernels, dts=dts, nonfinite=nonfinite, out=out, evented_same=same)
 
 
 def conserved(w, gamma=1.4):
@@ -673,6 +686,8 @@ def main():
         "parity": parity,
         "modes": {m: {"Mcell_updates_per_s": round(cells * x["steps"] / x["elapsed"] / 1e6, 2),
                       "ms_per_step": round(x["elapsed"] * 1e3 / x["steps"], 4),
+                      "evented_ms_per_step": round(x["evented_elapsed"] * 1e3 / x["steps"], 4),
+                      "evented_run_same_bits": x["evented_same"],
                       "kernels_ms_avg": {k: round(v[1] / max(v[0], 1), 4) for k, v in x["kernels"].items()},
                       "t_final": x["t"], "nonfinite": x["nonfinite"]} for m, x in res.items()},
     }
