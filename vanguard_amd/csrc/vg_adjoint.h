@@ -43,10 +43,14 @@ namespace vg {
 using EPar = EulerPhys::Params;
 
 // ---- analytic vector-Jacobian products of the Euler primitives ------------------
+// These and the piece evaluations run at linearisation points only, so their
+// multiply-adds are fused; the branch decisions (limiter_piece and the
+// forward's conversions) keep the forward's separately rounded arithmetic.
 
 // w-bar = (d p2c / d w)^T U-bar  (p2c: Euler/eulerstatevector.cpp:78-91).
 __device__ __forceinline__ void vjp_p2c(const EPar &m, const double (&w)[6], const double (&Ub)[6], double (&wb)[6])
 {
+#pragma clang fp contract(fast)
     const double rho = w[0], u = w[1], v = w[2], z = w[3], lam = w[5];
     const double vsq = u * u + v * v + z * z;
     wb[0] = Ub[0] + u * Ub[1] + v * Ub[2] + z * Ub[3] + (0.5 * vsq + m.e_form * (lam - 1.0)) * Ub[4] + lam * Ub[5];
@@ -61,6 +65,7 @@ __device__ __forceinline__ void vjp_p2c(const EPar &m, const double (&w)[6], con
 // p = (gamma-1)(U4 - |m|^2/(2 U0) - E_f (U5 - U0)) - gamma p_inf.
 __device__ __forceinline__ void vjp_c2p(const EPar &m, const double (&U)[6], const double (&wb)[6], double (&Ub)[6])
 {
+#pragma clang fp contract(fast)
     const double q = rcp_nr(U[0]);
     const double u = U[1] * q, v = U[2] * q, z = U[3] * q, lam = U[5] * q;
     const double g1 = m.gm1;
@@ -82,6 +87,7 @@ __device__ __forceinline__ void vjp_c2p(const EPar &m, const double (&U)[6], con
 template <int DIR>
 __device__ __forceinline__ void vjp_flux(const EPar &m, const double (&U)[6], const double (&Fb)[6], double (&Ub)[6])
 {
+#pragma clang fp contract(fast)
     const double q = rcp_nr(U[0]);
     const double vel[3] = {U[1] * q, U[2] * q, U[3] * q};
     const double lam = U[5] * q;
@@ -134,6 +140,7 @@ __device__ __forceinline__ int limiter_piece(double r, double bias, int limiter)
 __device__ __forceinline__ void limiter_piece_eval(int piece, double r, double bias, int limiter, double &xi,
                                                    double &dxi)
 {
+#pragma clang fp contract(fast)
     const double iv = rcp_nr((1.0 - bias) + (1.0 + bias) * r);
     double x = 0.0, d = 0.0;
     x = (piece == kXiTwoR) ? 2.0 * r : x;
@@ -163,6 +170,7 @@ struct SlopeLin {
 __device__ __forceinline__ SlopeLin slope_lin(double dl, double dr, int piece, double cl, double cr, double bias,
                                               int limiter)
 {
+#pragma clang fp contract(fast)
     SlopeLin s;
     s.base = (cl * dl) + (cr * dr);
     s.cln = fabs(dl) < 1e-5;

@@ -57,37 +57,6 @@ int check_common(vg_ctx *ctx, int dim, int ny, int nx, double dx, int limiter, i
     return VG_OK;
 }
 
-// One forward sweep with the time step taken from the clock (dt x scale).
-template <int MODE>
-int fwd_sweep(vg_ctx *ctx, int dir, const double *in, double *out, long long plane, int ny, int nx, int dim, int bc,
-              double dx, double bias, int limiter, double scale, const EulerPhys::Params &m, vg::Clock *clk,
-              int y_variant)
-{
-    vg::SweepArgs<EulerPhys> a;
-    a.in = in;
-    a.out = out;
-    a.plane = plane;
-    a.ny = ny;
-    a.nx = nx;
-    a.bc = bc;
-    a.limiter = limiter;
-    a.dx = dx;
-    a.bias = bias;
-    a.dt_scale = scale;
-    a.dim = dim;
-    a.fuse_max = 0;
-    a.m = m;
-    a.clk = clk;
-    a.partials = nullptr;
-    if (dir == 0) {
-        constexpr int BS = vg::Tiles<EulerPhys>::bsx, RX = vg::Tiles<EulerPhys>::rx;
-        dim3 grid((nx + (BS - 4) - 1) / (BS - 4), (ny + RX - 1) / RX);
-        vg::sweep_x_kernel<EulerPhys, 2, MODE, BS, RX><<<grid, BS, 0, ctx->stream>>>(a);
-        return vg::launch_status("sweep_x");
-    }
-    return vg::launch_y<EulerPhys, 2, MODE>(ctx, y_variant, a);
-}
-
 struct DeviceBuffers {
     std::vector<double *> ptrs;
     ~DeviceBuffers()
@@ -123,19 +92,16 @@ int forward_fixed(vg_ctx *ctx, EGrid &g, const double *dts_dev, int s0, int s1, 
         const int base = (s - s0) * per;
         const bool last = skip_last && s == s1 - 1;
         int rc = VG_OK;
+        // the grid's own sweep launch: the 4-field kernels when its last run took them
+        auto sweep = [&](int dir, const double *in, double *out, double scale) {
+            return g.launch_sweep<2, vg::MODE_EXACT>(dir, in, out, scale, 0, bc, dx, bias, limiter, m);
+        };
         if (g.dim == 1) {
-            if (!last)
-                rc = fwd_sweep<vg::MODE_EXACT>(ctx, 0, slots[base], slots[base + 1], g.plane, g.ny, g.nx, 1, bc, dx,
-                                               bias, limiter, 1.0, m, g.clk, 0);
+            if (!last) rc = sweep(0, slots[base], slots[base + 1], 1.0);
         } else {
-            rc = fwd_sweep<vg::MODE_EXACT>(ctx, 0, slots[base], slots[base + 1], g.plane, g.ny, g.nx, 2, bc, dx, bias,
-                                           limiter, 0.5, m, g.clk, g.y_variant);
-            if (!rc)
-                rc = fwd_sweep<vg::MODE_EXACT>(ctx, 1, slots[base + 1], slots[base + 2], g.plane, g.ny, g.nx, 2, bc, dx,
-                                               bias, limiter, 1.0, m, g.clk, g.y_variant);
-            if (!rc && !last)
-                rc = fwd_sweep<vg::MODE_EXACT>(ctx, 0, slots[base + 2], slots[base + 3], g.plane, g.ny, g.nx, 2, bc, dx,
-                                               bias, limiter, 0.5, m, g.clk, g.y_variant);
+            rc = sweep(0, slots[base], slots[base + 1], 0.5);
+            if (!rc) rc = sweep(1, slots[base + 1], slots[base + 2], 1.0);
+            if (!rc && !last) rc = sweep(0, slots[base + 2], slots[base + 3], 0.5);
         }
         if (rc) return rc;
     }
@@ -236,6 +202,17 @@ int adjoint(vg_ctx *ctx, int dim, const double *prim_in, double *prim_out, const
         return rc;
     }
     if (steps > 0) VG_HIP(hipMemcpyAsync(dts_dev, dts.data(), sizeof(double) * steps, hipMemcpyHostToDevice, s));
+    // The 4-field sweeps leave the constant planes alone: give every recompute
+    // slot those planes once (the adjoint sweeps read all six).
+    if (g.reduced && !ckpt.empty()) {
+        using R = vg::ReducedPhys<EulerPhys>;
+        for (int k = 1; k < per * K; k++)
+            for (int i = 0; i < R::kConst; i++) {
+                const long long off = (long long)R::plane(i) * g.plane;
+                VG_HIP(hipMemcpyAsync(slots[k] + off, ckpt[0] + off, sizeof(double) * g.cells, hipMemcpyDeviceToDevice,
+                                      s));
+            }
+    }
     // seed: array of structures -> planes
     const double *seed_aos = seed;
     if (!on_device) {

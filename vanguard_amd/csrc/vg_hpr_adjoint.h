@@ -14,8 +14,9 @@
 //   * the thermal source -rho theta2^-1 alpha^2 J reduces to -(rho0 T / (tau2 T0)) J;
 //   * theta1^-1 = 3 det(A)^(5/3) / (c_s^2 tau1) has d/dA = (5/3) theta1^-1 cof(A) / det(A).
 // Branch decisions (slope clamps, limiter cases) are recomputed in the
-// forward's IEEE arithmetic; linearisation points use refined reciprocals, and
-// a conserved -> primitive -> conserved round trip counts as the identity.
+// forward's IEEE arithmetic; linearisation points use refined reciprocals and
+// fused multiply-adds, and a conserved -> primitive -> conserved round trip
+// counts as the identity.
 //
 // The sweep adjoint runs as four launches over a grid of cells or faces (one
 // thread each, SoA planes, coalesced along x in both directions):
@@ -56,6 +57,7 @@ struct ConsPoint {
 
 __device__ __forceinline__ void cons_point(const HPar &m, const double (&U)[NF], ConsPoint &c)
 {
+#pragma clang fp contract(fast)
     c.rho = U[0];
     c.ir = rcp_nr(c.rho);
 #pragma unroll
@@ -85,6 +87,7 @@ __device__ __forceinline__ void cons_point(const HPar &m, const double (&U)[NF],
 // Ab += A (Gb + Gb^T), Gb = devb - tr(devb)/3 I: the adjoint of dev(A^T A).
 __device__ __forceinline__ void dev_gram_vjp(const double (&A)[3][3], const double (&devb)[3][3], double (&Ab)[3][3])
 {
+#pragma clang fp contract(fast)
     const double t = (devb[0][0] + devb[1][1] + devb[2][2]) * (1.0 / 3.0);
     double S[3][3];
 #pragma unroll
@@ -100,6 +103,7 @@ __device__ __forceinline__ void dev_gram_vjp(const double (&A)[3][3], const doub
 // Adjoint of E_A = c_s^2 A dev(A^T A) given EAb: adds to Ab.
 __device__ __forceinline__ void ea_vjp(const HPar &m, const ConsPoint &c, const double (&EAb)[3][3], double (&Ab)[3][3])
 {
+#pragma clang fp contract(fast)
     double devb[3][3];
 #pragma unroll
     for (int l = 0; l < 3; l++)
@@ -119,6 +123,7 @@ __device__ __forceinline__ void ea_vjp(const HPar &m, const ConsPoint &c, const 
 // setConservedVariableVector: U-bar = c2p'(U)^T w-bar.
 __device__ __forceinline__ void vjp_c2p(const HPar &m, const ConsPoint &c, const double (&wb)[NF], double (&Ub)[NF])
 {
+#pragma clang fp contract(fast)
     double rb = wb[0] + wb[1] * m.G * c.e;
     const double eb = wb[1] * m.G * c.rho;
 #pragma unroll
@@ -146,6 +151,7 @@ __device__ __forceinline__ void vjp_c2p(const HPar &m, const ConsPoint &c, const
 // computeConservedVariableVector: w-bar = p2c'(w)^T U-bar.
 __device__ __forceinline__ void vjp_p2c(const HPar &m, const double (&w)[NF], const double (&Ub)[NF], double (&wb)[NF])
 {
+#pragma clang fp contract(fast)
     const double rho = w[0], p = w[1];
     double A[3][3], G[3][3], dev[3][3], Ad[3][3];
     HP::load_A(w, A);
@@ -186,6 +192,7 @@ __device__ __forceinline__ void vjp_p2c(const HPar &m, const double (&w)[NF], co
 template <int DIR>
 __device__ __forceinline__ void vjp_flux(const HPar &m, const ConsPoint &c, const double (&Fb)[NF], double (&Ub)[NF])
 {
+#pragma clang fp contract(fast)
     const double rho = c.rho, vn = c.vel[DIR];
     double Prow[3], sig[3];
 #pragma unroll
@@ -301,6 +308,7 @@ __device__ __forceinline__ void vjp_flux(const HPar &m, const ConsPoint &c, cons
 __device__ __forceinline__ void vjp_source(const HPar &m, const ConsPoint &c, const double (&Sb)[NF],
                                            double (&Ub)[NF], double &tb1, double &tb2)
 {
+#pragma clang fp contract(fast)
     const double d = HP::det3(c.A);
     const double th1 = 3.0 * (pow(d, (5.0 / 3.0)) / (m.cs2 * m.tau1));
     double EAb[3][3], Ab[3][3], th1b = 0.0;
@@ -385,6 +393,7 @@ __device__ __forceinline__ void slope_vjp(const double (&UL)[NF], const double (
                                           const double (&xrb)[NF], double (&own)[NF], double (&toL)[NF],
                                           double (&toR)[NF])
 {
+#pragma clang fp contract(fast)
     const double cl = 0.5 * (1.0 + bias), cr = 0.5 * (1.0 - bias);
 #pragma unroll
     for (int k = 0; k < NF; k++) {
