@@ -41,6 +41,14 @@ static __global__ void clock_check_kernel(Clock *c)
     }
 }
 
+// The inside of one step, kept for the adjoint: both materials after the x
+// ghost fill (P1) and after the y fill (P2, the first x sweep's input), after
+// the first x sweep (Q1, the y sweep's input) and after the y sweep (Q2, the
+// last x sweep's input).  1D steps keep P1 only.
+struct StepInternals {
+    double *P1[2], *P2[2], *Q1[2], *Q2[2];
+};
+
 struct RGFMGrid {
     static constexpr int BSX = 256, RX = 4, BSY = 256, HY = 64;
     vg_ctx *ctx = nullptr;
@@ -194,14 +202,25 @@ struct RGFMGrid {
     // One step; with fixed_dts the time step is fixed_dts[it] (the clock's
     // step counter) instead of the CFL formula.
     template <int MODE>
+    // keep (optional): where to leave the step's inside for the adjoint; the
+    // arithmetic is the same either way.
     int enqueue_step(int riemann, double dx, double dy, double bias, int limiter, const EP &p1, const EP &p2,
-                     double *dt_log, int dt_cap, const double *fixed_dts = nullptr, int n_fixed = 0)
+                     double *dt_log, int dt_cap, const double *fixed_dts = nullptr, int n_fixed = 0,
+                     const StepInternals *keep = nullptr)
     {
         hipStream_t s = ctx->stream;
         clock_check_kernel<<<1, 1, 0, s>>>(clk);
         LineArgs la = line_args(riemann, p1, p2);
+        const size_t gbytes = sizeof(double) * 6 * plane;
+        auto snapshot = [&](double *const (&dst)[2]) {
+            VG_HIP(hipMemcpyAsync(dst[0], m1[cur], gbytes, hipMemcpyDeviceToDevice, s));
+            VG_HIP(hipMemcpyAsync(dst[1], m2[cur], gbytes, hipMemcpyDeviceToDevice, s));
+            return VG_OK;
+        };
         int rc = ctx->timed("rgfm_fill_x", [&] { rgfm_fill_kernel<0><<<ny, kLineBS, 0, s>>>(la); });
+        if (!rc && keep) rc = snapshot(keep->P1);
         if (!rc && dim == 2) rc = ctx->timed("rgfm_fill_y", [&] { rgfm_fill_kernel<1><<<nx, kLineBS, 0, s>>>(la); });
+        if (!rc && keep && dim == 2) rc = snapshot(keep->P2);
         if (rc) return launch_status("rgfm_fill");
         if (fixed_dts) {
             clock_fixed_kernel<<<1, 1, 0, s>>>(clk, fixed_dts, n_fixed);
@@ -244,9 +263,11 @@ struct RGFMGrid {
             if (dim == 1) {
                 rc = sweep<MODE>(0, A, B, 1.0, dx, bias, limiter, *ps[q]);
             } else {
-                rc = sweep<MODE>(0, A, B, 0.5, dx, bias, limiter, *ps[q]);
-                if (!rc) rc = sweep<MODE>(1, B, A, 1.0, dy, bias, limiter, *ps[q]);
-                if (!rc) rc = sweep<MODE>(0, A, B, 0.5, dx, bias, limiter, *ps[q]);
+                // with keep, the two intermediate states go to the record
+                double *X1 = keep ? keep->Q1[q] : B, *X2 = keep ? keep->Q2[q] : A;
+                rc = sweep<MODE>(0, A, X1, 0.5, dx, bias, limiter, *ps[q]);
+                if (!rc) rc = sweep<MODE>(1, X1, X2, 1.0, dy, bias, limiter, *ps[q]);
+                if (!rc) rc = sweep<MODE>(0, X2, B, 0.5, dx, bias, limiter, *ps[q]);
             }
         }
         if (rc) return launch_status("rgfm sweep");
@@ -544,9 +565,11 @@ int solve_fixed(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, 
 // Reverse mode through the HLLC RGFM solve, time steps frozen.  Forward with a
 // checkpoint (both materials and the level set) every K steps; then, segment by
 // segment from the end, the segment is recomputed from its checkpoint keeping
-// each step's starting state, and each step, from the last, has its inside
-// recomputed (fills, sweep inputs) and is reversed: the SLIC sweeps of both
-// materials, then the ghost fills (y, then x).
+// each step's starting state and inside (StepInternals), and its steps, from
+// the last, are reversed: the SLIC sweeps of both materials, then the ghost
+// fills (y, then x).  K is ceil(sqrt(steps)) (64 for runs to t_final), or
+// less when K kept steps (10 material grids and a level set each in 2D) would
+// not fit in 60 % of free device memory.
 int adjoint(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, const double *phi_in, double *m1_out,
             double *m2_out, double *phi_out, const double *seed1, const double *seed2, double *grad1, double *grad2,
             int ny, int nx, double dx, double dy, double cfl, double t_final, int max_steps, double bias, int limiter,
@@ -573,19 +596,27 @@ int adjoint(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, cons
         return rc;
     }
     const size_t gb = sizeof(double) * 6 * g.plane, pb = sizeof(double) * g.cells;
-    const int K = checkpoint_every > 0 ? checkpoint_every
-                                       : (max_steps > 0 ? std::max(1, (int)std::ceil(std::sqrt((double)max_steps)))
-                                                        : 64);
-    // workspace: 0 dts, 1 clock, 2 ghost adjoints, 3-6 adjoint ping-pong (A[b][q]),
-    // 7-18 step scratch (P1, P2, Q1, Q2, T1, T2 per material), then K step
-    // states (m1, m2, phi) and the checkpoints (m1, m2, phi)
+    int K = checkpoint_every > 0 ? checkpoint_every
+                                 : (max_steps > 0 ? std::max(1, (int)std::ceil(std::sqrt((double)max_steps))) : 64);
+    constexpr int kKept = 11;   // per kept step: m1, m2, phi, P1 x2, P2 x2, Q1 x2, Q2 x2
+    if (checkpoint_every == 0) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+            const double per = 10.0 * (double)gb + (double)pb;
+            const double avail = 0.6 * (double)(free_b + ctx->workspace_bytes());
+            K = std::max(1, std::min(K, (int)(avail / per)));
+        }
+    }
+    // workspace: 0 dts, 2 ghost adjoints, 3-6 adjoint ping-pong (A[b][q]),
+    // 7-10 reverse temporaries (T1, T2 per material), then K kept steps
+    // (kKept buffers each) and the checkpoints (m1, m2, phi)
     auto wsp = [&](size_t i, size_t bytes, double **p) {
         void *v = nullptr;
         int r = ctx->workspace_buffer(i, bytes, &v);
         *p = static_cast<double *>(v);
         return r;
     };
-    const size_t st0 = 19, ck0 = st0 + 3 * (size_t)K;
+    const size_t st0 = 11, ck0 = st0 + kKept * (size_t)K;
     // ---- forward with checkpoints ---------------------------------------------------
     std::vector<double> dts;
     std::vector<double *> ck;   // 3 per checkpoint
@@ -621,27 +652,29 @@ int adjoint(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, cons
     const int steps = (int)dts.size();
     rc = g.store(m1_out, m2_out, phi_out);
     // ---- buffers of the reverse pass --------------------------------------------------
-    double *dts_dev = nullptr, *ghost = nullptr, *A[2][2], *P1[2], *P2[2], *Q1[2], *Q2[2], *T1[2], *T2[2];
-    double *aclk_raw = nullptr;
+    double *dts_dev = nullptr, *ghost = nullptr, *A[2][2], *T1[2], *T2[2];
     if (!rc) rc = wsp(0, sizeof(double) * std::max(steps, 1), &dts_dev);
-    if (!rc) rc = wsp(1, sizeof(vg::Clock) + 64, &aclk_raw);
     if (!rc) rc = wsp(2, sizeof(double) * 4 * 6 * (size_t)std::max(nx, ny), &ghost);
     for (int b = 0; b < 2 && !rc; b++)
         for (int q = 0; q < 2 && !rc; q++) rc = wsp(3 + 2 * b + q, gb, &A[b][q]);
-    double **scr[6] = {P1, P2, Q1, Q2, T1, T2};
-    for (int k = 0; k < 6 && !rc; k++)
-        for (int q = 0; q < 2 && !rc; q++) rc = wsp(7 + 2 * k + q, gb, &scr[k][q]);
+    for (int q = 0; q < 2 && !rc; q++) rc = wsp(7 + q, gb, &T1[q]);
+    for (int q = 0; q < 2 && !rc; q++) rc = wsp(9 + q, gb, &T2[q]);
+    // kept step k: sts[3k .. 3k+2] = starting m1, m2, phi; ins[k] = its inside
     std::vector<double *> sts(3 * (size_t)K, nullptr);
+    std::vector<vg::StepInternals> ins((size_t)K);
     for (int k = 0; k < K && !rc; k++) {
-        if ((rc = wsp(st0 + 3 * k, gb, &sts[3 * k]))) break;
-        if ((rc = wsp(st0 + 3 * k + 1, gb, &sts[3 * k + 1]))) break;
-        rc = wsp(st0 + 3 * k + 2, pb, &sts[3 * k + 2]);
+        const size_t b = st0 + (size_t)kKept * k;
+        if ((rc = wsp(b, gb, &sts[3 * k]))) break;
+        if ((rc = wsp(b + 1, gb, &sts[3 * k + 1]))) break;
+        if ((rc = wsp(b + 2, pb, &sts[3 * k + 2]))) break;
+        double **slots[4] = {ins[k].P1, ins[k].P2, ins[k].Q1, ins[k].Q2};
+        for (int j = 0; j < 4 && !rc; j++)
+            for (int q = 0; q < 2 && !rc; q++) rc = wsp(b + 3 + 2 * j + q, gb, &slots[j][q]);
     }
     if (rc) {
         g.destroy();
         return rc;
     }
-    vg::Clock *aclk = reinterpret_cast<vg::Clock *>(aclk_raw);
     if (steps > 0) VG_HIP(hipMemcpyAsync(dts_dev, dts.data(), sizeof(double) * steps, hipMemcpyHostToDevice, s));
     // seeds: arrays of structures -> planes
     VG_HIP(hipMemcpyAsync(g.staging, seed1, gb / g.plane * g.cells, hipMemcpyHostToDevice, s));
@@ -663,7 +696,7 @@ int adjoint(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, cons
     for (int seg = nseg - 1; seg >= 0 && !rc; seg--) {
         const int s0 = seg * K, s1 = std::min(steps, s0 + K);
         if (s0 >= s1) continue;
-        // recompute the segment, keeping each step's starting state
+        // recompute the segment, keeping each step's starting state and inside
         g.cur = 0;
         g.pcur = 0;
         VG_HIP(hipMemcpyAsync(g.m1[0], ck[3 * seg], gb, hipMemcpyDeviceToDevice, s));
@@ -679,9 +712,8 @@ int adjoint(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, cons
             VG_HIP(hipMemcpyAsync(sts[3 * k], g.m1[g.cur], gb, hipMemcpyDeviceToDevice, s));
             VG_HIP(hipMemcpyAsync(sts[3 * k + 1], g.m2[g.cur], gb, hipMemcpyDeviceToDevice, s));
             VG_HIP(hipMemcpyAsync(sts[3 * k + 2], g.phi[g.pcur], pb, hipMemcpyDeviceToDevice, s));
-            if (st + 1 < s1)
-                rc = g.enqueue_step<vg::MODE_EXACT>(VG_RIEMANN_HLLC, dx, dy, bias, limiter, q1, q2, nullptr, 0,
-                                                    dts_dev, s1);
+            rc = g.enqueue_step<vg::MODE_EXACT>(VG_RIEMANN_HLLC, dx, dy, bias, limiter, q1, q2, nullptr, 0, dts_dev, s1,
+                                                &ins[k]);
         }
         // reverse the steps of the segment
         for (int st = s1 - 1; st >= s0 && !rc; st--) {
@@ -689,24 +721,8 @@ int adjoint(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, cons
             double *S[2] = {sts[3 * k], sts[3 * k + 1]};
             const double *phs = sts[3 * k + 2];
             const double dt = dts[st];
-            vg::Clock c{};
-            c.dt = dt;
-            c.t_final = INFINITY;
-            VG_HIP(hipMemcpyAsync(aclk, &c, sizeof(c), hipMemcpyHostToDevice, s));
-            // inside of the step: fills, then the sweep inputs of both materials
-            for (int q = 0; q < 2; q++) VG_HIP(hipMemcpyAsync(P1[q], S[q], gb, hipMemcpyDeviceToDevice, s));
-            rc = g.fill_pass(0, P1[0], P1[1], phs, q1, q2, aclk);
-            double *post[2] = {P1[0], P1[1]};
-            if (!rc && dim == 2) {
-                for (int q = 0; q < 2; q++) VG_HIP(hipMemcpyAsync(P2[q], P1[q], gb, hipMemcpyDeviceToDevice, s));
-                rc = g.fill_pass(1, P2[0], P2[1], phs, q1, q2, aclk);
-                post[0] = P2[0];
-                post[1] = P2[1];
-                for (int q = 0; q < 2 && !rc; q++) {
-                    rc = g.sweep<vg::MODE_EXACT>(0, post[q], Q1[q], 0.5, dx, bias, limiter, *qs[q], aclk);
-                    if (!rc) rc = g.sweep<vg::MODE_EXACT>(1, Q1[q], Q2[q], 1.0, dy, bias, limiter, *qs[q], aclk);
-                }
-            }
+            double *const *P1 = ins[k].P1, *const *Q1 = ins[k].Q1, *const *Q2 = ins[k].Q2;
+            double *post[2] = {dim == 2 ? ins[k].P2[0] : P1[0], dim == 2 ? ins[k].P2[1] : P1[1]};
             // reverse: sweeps of each material
             for (int q = 0; q < 2 && !rc; q++) {
                 if (dim == 1) {
