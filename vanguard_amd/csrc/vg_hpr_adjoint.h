@@ -29,9 +29,10 @@
 //      the two neighbours;
 //   4. per cell: gather, then p2c'(w)^T.
 // The sub-cycle adjoint recomputes each cell's RK4 forward (forcing_cell, the
-// forward's own code) with its stage states recorded, reverses it, and writes
-// the cell's term and its four neighbour terms, gathered by a fifth launch.
-// The relaxation-time derivatives are summed per block into fixed slots.
+// forward's own code) with its stage states recorded in HBM, reverses it one
+// stage-sized launch at a time, and writes the cell's term and its four
+// neighbour terms, gathered by a last launch.  The relaxation-time derivatives
+// are summed per block into fixed slots.
 #pragma once
 
 #include "vg_adjoint.h"
@@ -694,11 +695,24 @@ __global__ __launch_bounds__(kBS) void hpr_adj_gather_kernel(SweepAdjArgs a)
 }
 
 // ---- sub-cycle adjoint -----------------------------------------------------------------
+// One sub-cycle's adjoint runs as stage-sized launches with the stage states in
+// HBM, so that no thread holds a whole RK4 reverse at once: a recording
+// forward, then per stage (last first) the source VJP and the VJPs of its
+// fractional steps (y between two x in 2D), then the source VJP at the cell's
+// own state and the conversions back to primitives; the gather kernel below
+// sums each cell's own and neighbour terms.
 struct ForcingAdjArgs {
     const double *w;        // the sub-cycle's input (primitive planes)
     const double *wb_out;   // adjoint of its output
     double *wb_in;          // adjoint of its input
-    double *own, *tl, *tr, *tt, *tb;   // per cell: its own term and its neighbours'
+    double *own, *tl, *tr, *tt, *tb;   // per cell: its own term and its neighbours' (primitive)
+    double *rt;             // per cell: the conserved state its neighbours' predictors see
+    double *um;             // per cell: the middle state entering fractional step (st, ph), 9 x 17 planes
+    double *ue;             // per cell: the state whose source gives increment st + 1, 3 x 17 planes
+    double *nb;             // per cell: the adjoint flowing back through the current stage
+    double *umb;            // per cell: adjoint of the cell's own conserved vector
+    double *kb;             // per cell: adjoints of the stage increments, 4 x kMaxSlots planes
+    double *ulb, *urb, *utb, *ubb;   // per cell: conserved adjoints of its neighbours' states
     double *tau_part;       // per block: (tau1, tau2) derivative sums, accumulated
     long long plane;
     int ny, nx, dim, bc, limiter;
@@ -706,153 +720,38 @@ struct ForcingAdjArgs {
     HPar m;
 };
 
-struct StageRecord {
-    double UM[3][3][NF];
-    double Ue[3][NF];
+// The stage states, written to their planes as forcing_cell produces them.
+struct GlobalStages {
+    double *um, *ue;
+    long long plane, c;
     template <int N>
     __device__ __forceinline__ void mid(int st, int ph, const double (&U)[N])
     {
 #pragma unroll
-        for (int k = 0; k < N; k++) UM[st][ph][k] = U[k];
+        for (int k = 0; k < N; k++) um[((st * 3 + ph) * N + k) * plane + c] = U[k];
     }
     template <int N>
     __device__ __forceinline__ void end(int st, const double (&U)[N])
     {
 #pragma unroll
-        for (int k = 0; k < N; k++) Ue[st][k] = U[k];
+        for (int k = 0; k < N; k++) ue[(st * N + k) * plane + c] = U[k];
     }
 };
 
-template <int DIM>
-__global__ __launch_bounds__(kBS) void hpr_adj_forcing_kernel(ForcingAdjArgs a)
+__device__ __forceinline__ void cell_neighbours(const ForcingAdjArgs &a, long long c, long long &cl, long long &cr,
+                                                long long &ct, long long &cb)
 {
-    using T = ForcingTraits<HP>;
-    constexpr int NS = T::kMaxSlots;
-    const HPar &m = a.m;
-    const long long cells = (long long)a.ny * a.nx;
-    const long long c = (long long)blockIdx.x * kBS + threadIdx.x;
-    double tb1 = 0.0, tb2 = 0.0;
-    if (c < cells) {
-        const int i = (int)(c / a.nx), j = (int)(c - (long long)i * a.nx);
-        const long long cl = (long long)i * a.nx + ghost_index(j - 1, a.nx, a.bc);
-        const long long cr = (long long)i * a.nx + ghost_index(j + 1, a.nx, a.bc);
-        const long long ct = (DIM == 2) ? (long long)ghost_index(i - 1, a.ny, a.bc) * a.nx + j : c;
-        const long long cb = (DIM == 2) ? (long long)ghost_index(i + 1, a.ny, a.bc) * a.nx + j : c;
-        double UL[NF], UR[NF], UT[NF], UB[NF], w0[NF], Um[NF];
-        forcing_neighbour<HP>(m, a.w, a.plane, cl, UL);
-        forcing_neighbour<HP>(m, a.w, a.plane, cr, UR);
-        if (DIM == 2) {
-            forcing_neighbour<HP>(m, a.w, a.plane, ct, UT);
-            forcing_neighbour<HP>(m, a.w, a.plane, cb, UB);
-        }
-        load_planes(a.w, a.plane, c, w0);
-        HP::prim_to_cons(m, w0, Um);
-        double N[NF], K[4][NS];
-        StageRecord rec;
-        forcing_cell<HP, DIM>(m, UL, UR, UT, UB, Um, a.ts, a.dx, a.bias, a.limiter, N, K, rec);
-        // ---- reverse ----
-        const int nsrc = T::count(m);
-        double wbo[NF], own[NF];
-        load_planes(a.wb_out, a.plane, c, wbo);
-#pragma unroll
-        for (int k = 0; k < NF; k++) own[k] = 0.0;
-        if (!m.is_thermal) {   // the cell keeps its own J
-#pragma unroll
-            for (int k = 14; k < 17; k++) {
-                own[k] = wbo[k];
-                wbo[k] = 0.0;
-            }
-        }
-        double Nb[NF], Umb[NF];
-        {
-            ConsPoint p;
-            cons_point(m, N, p);
-            vjp_c2p(m, p, wbo, Nb);
-        }
-        double Kb[4][NS];
-#pragma unroll
-        for (int k = 0; k < NF; k++) Umb[k] = Nb[k];
-#pragma unroll
-        for (int q = 0; q < NS; q++) {
-            const double g = (q < nsrc) ? (1.0 / 6.0) * Nb[T::slot(q)] : 0.0;
-            Kb[0][q] = g;
-            Kb[1][q] = 2.0 * g;
-            Kb[2][q] = 2.0 * g;
-            Kb[3][q] = g;
-        }
-        double ULb[NF], URb[NF], UTb[NF], UBb[NF];
-#pragma unroll
-        for (int k = 0; k < NF; k++) {
-            ULb[k] = 0.0;
-            URb[k] = 0.0;
-            UTb[k] = 0.0;
-            UBb[k] = 0.0;
-        }
-        const double cx = a.ts / a.dx;
-#pragma unroll
-        for (int st = 2; st >= 0; st--) {
-            double Sb[NF], Ueb[NF];
-#pragma unroll
-            for (int k = 0; k < NF; k++) Sb[k] = 0.0;
-#pragma unroll
-            for (int q = 0; q < NS; q++)
-                if (q < nsrc) Sb[T::slot(q)] = a.ts * Kb[st + 1][q];
-            {
-                ConsPoint p;
-                cons_point(m, rec.Ue[st], p);
-                vjp_source(m, p, Sb, Ueb, tb1, tb2);
-            }
-            double b0[NF];
-            if (DIM == 1) {
-                vjp_fractional<0>(m, UL, rec.UM[st][0], UR, a.bias, a.limiter, (st < 2 ? 0.5 : 1.0) * cx, Ueb, ULb, b0,
-                                  URb);
-            } else {
-                const double fx = (st < 2) ? 0.25 : 0.5, fy = (st < 2) ? 0.5 : 1.0;
-                double b2[NF], b1[NF];
-                vjp_fractional<0>(m, UL, rec.UM[st][2], UR, a.bias, a.limiter, fx * cx, Ueb, ULb, b2, URb);
-                vjp_fractional<1>(m, UT, rec.UM[st][1], UB, a.bias, a.limiter, fy * cx, b2, UTb, b1, UBb);
-                vjp_fractional<0>(m, UL, rec.UM[st][0], UR, a.bias, a.limiter, fx * cx, b1, ULb, b0, URb);
-            }
-#pragma unroll
-            for (int k = 0; k < NF; k++) Umb[k] += b0[k];
-#pragma unroll
-            for (int q = 0; q < NS; q++)
-                if (q < nsrc) Kb[st][q] += (st < 2 ? 0.5 : 1.0) * b0[T::slot(q)];
-        }
-        {
-            double Sb[NF], t[NF];
-#pragma unroll
-            for (int k = 0; k < NF; k++) Sb[k] = 0.0;
-#pragma unroll
-            for (int q = 0; q < NS; q++)
-                if (q < nsrc) Sb[T::slot(q)] = a.ts * Kb[0][q];
-            ConsPoint p;
-            cons_point(m, Um, p);
-            vjp_source(m, p, Sb, t, tb1, tb2);
-#pragma unroll
-            for (int k = 0; k < NF; k++) Umb[k] += t[k];
-        }
-        double wb[NF], wn[NF];
-        vjp_p2c(m, w0, Umb, wb);
-#pragma unroll
-        for (int k = 0; k < NF; k++) own[k] += wb[k];
-        store_planes(a.own, a.plane, c, own);
-        load_planes(a.w, a.plane, cl, wn);
-        vjp_p2c(m, wn, ULb, wb);
-        store_planes(a.tl, a.plane, c, wb);
-        load_planes(a.w, a.plane, cr, wn);
-        vjp_p2c(m, wn, URb, wb);
-        store_planes(a.tr, a.plane, c, wb);
-        if (DIM == 2) {
-            load_planes(a.w, a.plane, ct, wn);
-            vjp_p2c(m, wn, UTb, wb);
-            store_planes(a.tt, a.plane, c, wb);
-            load_planes(a.w, a.plane, cb, wn);
-            vjp_p2c(m, wn, UBb, wb);
-            store_planes(a.tb, a.plane, c, wb);
-        }
-    }
-    // per-block sums of the relaxation-time derivatives into this block's slots
+    const int i = (int)(c / a.nx), j = (int)(c - (long long)i * a.nx);
+    cl = (long long)i * a.nx + ghost_index(j - 1, a.nx, a.bc);
+    cr = (long long)i * a.nx + ghost_index(j + 1, a.nx, a.bc);
+    ct = (a.dim == 2) ? (long long)ghost_index(i - 1, a.ny, a.bc) * a.nx + j : c;
+    cb = (a.dim == 2) ? (long long)ghost_index(i + 1, a.ny, a.bc) * a.nx + j : c;
+}
+
+// Adds this block's relaxation-time derivative sums to its own slots (every
+// thread of the block calls it).
+__device__ __forceinline__ void tau_block_add(double *tau_part, double tb1, double tb2)
+{
     __shared__ double s1[kBS], s2[kBS];
     s1[threadIdx.x] = tb1;
     s2[threadIdx.x] = tb2;
@@ -865,9 +764,226 @@ __global__ __launch_bounds__(kBS) void hpr_adj_forcing_kernel(ForcingAdjArgs a)
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        a.tau_part[2 * blockIdx.x] += s1[0];
-        a.tau_part[2 * blockIdx.x + 1] += s2[0];
+        tau_part[2 * blockIdx.x] += s1[0];
+        tau_part[2 * blockIdx.x + 1] += s2[0];
     }
+}
+
+// 1. The forward sub-cycle of every cell with its stage states recorded, the
+// adjoint of its output conversion, and the first increment adjoints.
+template <int DIM>
+__global__ __launch_bounds__(kBS) void hpr_adj_force_record_kernel(ForcingAdjArgs a)
+{
+    using T = ForcingTraits<HP>;
+    constexpr int NS = T::kMaxSlots;
+    const HPar &m = a.m;
+    const long long cells = (long long)a.ny * a.nx;
+    const long long c = (long long)blockIdx.x * kBS + threadIdx.x;
+    if (c >= cells) return;
+    long long cl, cr, ct, cb;
+    cell_neighbours(a, c, cl, cr, ct, cb);
+    double N[NF], K[4][NS];
+    {
+        double UL[NF], UR[NF], UT[NF], UB[NF], w0[NF], Um[NF], RT[NF];
+        forcing_neighbour<HP>(m, a.w, a.plane, cl, UL);
+        forcing_neighbour<HP>(m, a.w, a.plane, cr, UR);
+        if (DIM == 2) {
+            forcing_neighbour<HP>(m, a.w, a.plane, ct, UT);
+            forcing_neighbour<HP>(m, a.w, a.plane, cb, UB);
+        }
+        forcing_neighbour<HP>(m, a.w, a.plane, c, RT);
+        store_planes(a.rt, a.plane, c, RT);
+        load_planes(a.w, a.plane, c, w0);
+        HP::prim_to_cons(m, w0, Um);
+        GlobalStages rec{a.um, a.ue, a.plane, c};
+        forcing_cell<HP, DIM>(m, UL, UR, UT, UB, Um, a.ts, a.dx, a.bias, a.limiter, N, K, rec);
+    }
+    double wbo[NF], own[NF], Nb[NF], zero[NF];
+    load_planes(a.wb_out, a.plane, c, wbo);
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+        own[k] = 0.0;
+        zero[k] = 0.0;
+    }
+    if (!m.is_thermal) {   // the cell keeps its own J
+#pragma unroll
+        for (int k = 14; k < 17; k++) {
+            own[k] = wbo[k];
+            wbo[k] = 0.0;
+        }
+    }
+    {
+        ConsPoint p;
+        cons_point(m, N, p);
+        vjp_c2p(m, p, wbo, Nb);
+    }
+    store_planes(a.own, a.plane, c, own);
+    store_planes(a.umb, a.plane, c, Nb);
+    const int nsrc = T::count(m);
+#pragma unroll
+    for (int q = 0; q < NS; q++) {
+        const double g = (q < nsrc) ? (1.0 / 6.0) * Nb[T::slot(q)] : 0.0;
+        a.kb[(0 * NS + q) * a.plane + c] = g;
+        a.kb[(1 * NS + q) * a.plane + c] = 2.0 * g;
+        a.kb[(2 * NS + q) * a.plane + c] = 2.0 * g;
+        a.kb[(3 * NS + q) * a.plane + c] = g;
+    }
+    store_planes(a.ulb, a.plane, c, zero);
+    store_planes(a.urb, a.plane, c, zero);
+    if (DIM == 2) {
+        store_planes(a.utb, a.plane, c, zero);
+        store_planes(a.ubb, a.plane, c, zero);
+    }
+}
+
+// 2. Stage st: the adjoint of its increment's source, S-bar = ts K-bar[st + 1].
+__global__ __launch_bounds__(kBS) void hpr_adj_force_source_kernel(ForcingAdjArgs a, int st)
+{
+    using T = ForcingTraits<HP>;
+    constexpr int NS = T::kMaxSlots;
+    const HPar &m = a.m;
+    const long long cells = (long long)a.ny * a.nx;
+    const long long c = (long long)blockIdx.x * kBS + threadIdx.x;
+    double tb1 = 0.0, tb2 = 0.0;
+    if (c < cells) {
+        const int nsrc = T::count(m);
+        double Sb[NF], Ue[NF], Ueb[NF];
+#pragma unroll
+        for (int k = 0; k < NF; k++) Sb[k] = 0.0;
+#pragma unroll
+        for (int q = 0; q < NS; q++)
+            if (q < nsrc) Sb[T::slot(q)] = a.ts * a.kb[((st + 1) * NS + q) * a.plane + c];
+#pragma unroll
+        for (int k = 0; k < NF; k++) Ue[k] = a.ue[(st * NF + k) * a.plane + c];
+        ConsPoint p;
+        cons_point(m, Ue, p);
+        vjp_source(m, p, Sb, Ueb, tb1, tb2);
+        store_planes(a.nb, a.plane, c, Ueb);
+    }
+    tau_block_add(a.tau_part, tb1, tb2);
+}
+
+// 3. Stage st, fractional step ph (DIR: its direction): N = UM + frac (F(xl) - F(xr))
+// reversed.  The neighbours' terms go to their accumulators; ph > 0 passes the
+// middle state's adjoint on (nb), ph = 0 ends the stage: the adjoint of
+// Us = Um + coef K[st] goes to Um and to K-bar[st].
+template <int DIR>
+__global__ __launch_bounds__(kBS) void hpr_adj_force_frac_kernel(ForcingAdjArgs a, int st, int ph, double frac,
+                                                                  double coef)
+{
+    using T = ForcingTraits<HP>;
+    constexpr int NS = T::kMaxSlots;
+    const HPar &m = a.m;
+    const long long cells = (long long)a.ny * a.nx;
+    const long long c = (long long)blockIdx.x * kBS + threadIdx.x;
+    if (c >= cells) return;
+    long long cl, cr, ct, cb;
+    cell_neighbours(a, c, cl, cr, ct, cb);
+    const long long nl = (DIR == 0) ? cl : ct, nr = (DIR == 0) ? cr : cb;
+    const double *um = a.um + (long long)(st * 3 + ph) * NF * a.plane;
+    unsigned long long pieces;
+    double xlb[NF], xrb[NF];
+    {
+        double UL[NF], UM[NF], UR[NF], xl[NF], xr[NF], s[NF];
+        load_planes(a.rt, a.plane, nl, UL);
+        load_planes(um, a.plane, c, UM);
+        load_planes(a.rt, a.plane, nr, UR);
+        slope_states(UL, UM, UR, a.bias, a.limiter, pieces, xl, xr);
+        load_planes(a.nb, a.plane, c, s);
+#pragma unroll
+        for (int k = 0; k < NF; k++) s[k] *= frac;
+        ConsPoint p;
+        cons_point(m, xl, p);
+        vjp_flux<DIR>(m, p, s, xlb);
+        cons_point(m, xr, p);
+        vjp_flux<DIR>(m, p, s, xrb);
+    }
+    // the inputs again, from memory rather than kept live across the VJPs
+    asm volatile("" ::: "memory");
+    double UL[NF], UM[NF], UR[NF], own[NF], toL[NF], toR[NF];
+    load_planes(a.rt, a.plane, nl, UL);
+    load_planes(um, a.plane, c, UM);
+    load_planes(a.rt, a.plane, nr, UR);
+#pragma unroll
+    for (int k = 0; k < NF; k++) xrb[k] = -xrb[k];
+    slope_vjp(UL, UM, UR, a.bias, a.limiter, pieces, xlb, xrb, own, toL, toR);
+    double *accL = (DIR == 0) ? a.ulb : a.utb, *accR = (DIR == 0) ? a.urb : a.ubb;
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+        accL[k * a.plane + c] += toL[k];
+        accR[k * a.plane + c] += toR[k];
+        own[k] += a.nb[k * a.plane + c];   // UM-bar = N-bar + the extrapolations' terms
+    }
+    if (ph > 0) {
+        store_planes(a.nb, a.plane, c, own);
+    } else {
+        const int nsrc = T::count(m);
+#pragma unroll
+        for (int k = 0; k < NF; k++) a.umb[k * a.plane + c] += own[k];
+#pragma unroll
+        for (int q = 0; q < NS; q++)
+            if (q < nsrc) a.kb[(st * NS + q) * a.plane + c] += coef * own[T::slot(q)];
+    }
+}
+
+// 4. The source at the cell's own state (K-bar[0]), then every adjoint back to
+// primitives: the cell's own term and its neighbours'.
+template <int DIM>
+__global__ __launch_bounds__(kBS) void hpr_adj_force_final_kernel(ForcingAdjArgs a)
+{
+    using T = ForcingTraits<HP>;
+    constexpr int NS = T::kMaxSlots;
+    const HPar &m = a.m;
+    const long long cells = (long long)a.ny * a.nx;
+    const long long c = (long long)blockIdx.x * kBS + threadIdx.x;
+    double tb1 = 0.0, tb2 = 0.0;
+    if (c < cells) {
+        long long cl, cr, ct, cb;
+        cell_neighbours(a, c, cl, cr, ct, cb);
+        const int nsrc = T::count(m);
+        double w0[NF], Umb[NF], wb[NF], wn[NF], own[NF];
+        load_planes(a.w, a.plane, c, w0);
+        load_planes(a.umb, a.plane, c, Umb);
+        {
+            double Sb[NF], Um[NF], t[NF];
+#pragma unroll
+            for (int k = 0; k < NF; k++) Sb[k] = 0.0;
+#pragma unroll
+            for (int q = 0; q < NS; q++)
+                if (q < nsrc) Sb[T::slot(q)] = a.ts * a.kb[q * a.plane + c];
+            HP::prim_to_cons(m, w0, Um);
+            ConsPoint p;
+            cons_point(m, Um, p);
+            vjp_source(m, p, Sb, t, tb1, tb2);
+#pragma unroll
+            for (int k = 0; k < NF; k++) Umb[k] += t[k];
+        }
+        vjp_p2c(m, w0, Umb, wb);
+        load_planes(a.own, a.plane, c, own);
+#pragma unroll
+        for (int k = 0; k < NF; k++) own[k] += wb[k];
+        store_planes(a.own, a.plane, c, own);
+        double ub[NF];
+        load_planes(a.w, a.plane, cl, wn);
+        load_planes(a.ulb, a.plane, c, ub);
+        vjp_p2c(m, wn, ub, wb);
+        store_planes(a.tl, a.plane, c, wb);
+        load_planes(a.w, a.plane, cr, wn);
+        load_planes(a.urb, a.plane, c, ub);
+        vjp_p2c(m, wn, ub, wb);
+        store_planes(a.tr, a.plane, c, wb);
+        if (DIM == 2) {
+            load_planes(a.w, a.plane, ct, wn);
+            load_planes(a.utb, a.plane, c, ub);
+            vjp_p2c(m, wn, ub, wb);
+            store_planes(a.tt, a.plane, c, wb);
+            load_planes(a.w, a.plane, cb, wn);
+            load_planes(a.ubb, a.plane, c, ub);
+            vjp_p2c(m, wn, ub, wb);
+            store_planes(a.tb, a.plane, c, wb);
+        }
+    }
+    tau_block_add(a.tau_part, tb1, tb2);
 }
 
 // Per cell: its own term plus the neighbour terms of the cells whose one-layer

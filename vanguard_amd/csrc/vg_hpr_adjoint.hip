@@ -92,10 +92,15 @@ int forward_fixed(vg_ctx *ctx, HPRGrid &g, const double *dts_dev, int s0, int s1
     return VG_OK;
 }
 
+// Planes of the sub-cycle adjoint's scratch: rt, um (9 x 17), ue (3 x 17), nb,
+// umb, kb (4 x kMaxSlots), four neighbour accumulators.
+constexpr int kForcePlanes = NF + 9 * NF + 3 * NF + NF + NF + 4 * vg::ForcingTraits<HPRPhys>::kMaxSlots + 4 * NF;
+
 struct Work {
     double *wb[2] = {nullptr, nullptr};
     double *nbar = nullptr;
     double *ext[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    double *fs = nullptr;   // sub-cycle adjoint scratch, kForcePlanes planes
     double *tau_part = nullptr;
     long long eplane = 0;
     int force_blocks = 0;
@@ -154,6 +159,7 @@ int sweep_adjoint_dir(vg_ctx *ctx, const ha::SweepAdjArgs &a)
 int forcing_adjoint(vg_ctx *ctx, const HPRGrid &g, const Work &w, const double *in, const double *wb_out,
                     double *wb_in, int bc, int limiter, double dx, double bias, double ts, const HPRPhys::Params &m)
 {
+    constexpr int NS = vg::ForcingTraits<HPRPhys>::kMaxSlots;
     ha::ForcingAdjArgs a;
     a.w = in;
     a.wb_out = wb_out;
@@ -163,6 +169,18 @@ int forcing_adjoint(vg_ctx *ctx, const HPRGrid &g, const Work &w, const double *
     a.tr = w.ext[2];
     a.tt = w.ext[3];
     a.tb = w.ext[4];
+    double *f = w.fs;
+    const long long P = g.plane;
+    a.rt = f;
+    a.um = f + NF * P;
+    a.ue = f + 10 * NF * P;
+    a.nb = f + 13 * NF * P;
+    a.umb = f + 14 * NF * P;
+    a.kb = f + 15 * NF * P;
+    a.ulb = a.kb + 4 * NS * P;
+    a.urb = a.ulb + NF * P;
+    a.utb = a.urb + NF * P;
+    a.ubb = a.utb + NF * P;
     a.tau_part = w.tau_part;
     a.plane = g.plane;
     a.ny = g.ny;
@@ -175,12 +193,26 @@ int forcing_adjoint(vg_ctx *ctx, const HPRGrid &g, const Work &w, const double *
     a.ts = ts;
     a.m = m;
     const unsigned nb = (unsigned)w.force_blocks;
+    hipStream_t s = ctx->stream;
+    const double cx = ts / dx;
     return ctx->timed("hpr_adjoint_source", [&] {
-        if (g.dim == 1)
-            ha::hpr_adj_forcing_kernel<1><<<nb, ha::kBS, 0, ctx->stream>>>(a);
-        else
-            ha::hpr_adj_forcing_kernel<2><<<nb, ha::kBS, 0, ctx->stream>>>(a);
-        ha::hpr_adj_forcing_gather_kernel<<<nb, ha::kBS, 0, ctx->stream>>>(a);
+        if (g.dim == 1) ha::hpr_adj_force_record_kernel<1><<<nb, ha::kBS, 0, s>>>(a);
+        else ha::hpr_adj_force_record_kernel<2><<<nb, ha::kBS, 0, s>>>(a);
+        for (int st = 2; st >= 0; st--) {
+            const double coef = (st < 2) ? 0.5 : 1.0;
+            ha::hpr_adj_force_source_kernel<<<nb, ha::kBS, 0, s>>>(a, st);
+            if (g.dim == 1) {
+                ha::hpr_adj_force_frac_kernel<0><<<nb, ha::kBS, 0, s>>>(a, st, 0, coef * cx, coef);
+            } else {
+                const double fx = (st < 2) ? 0.25 : 0.5, fy = (st < 2) ? 0.5 : 1.0;
+                ha::hpr_adj_force_frac_kernel<0><<<nb, ha::kBS, 0, s>>>(a, st, 2, fx * cx, coef);
+                ha::hpr_adj_force_frac_kernel<1><<<nb, ha::kBS, 0, s>>>(a, st, 1, fy * cx, coef);
+                ha::hpr_adj_force_frac_kernel<0><<<nb, ha::kBS, 0, s>>>(a, st, 0, fx * cx, coef);
+            }
+        }
+        if (g.dim == 1) ha::hpr_adj_force_final_kernel<1><<<nb, ha::kBS, 0, s>>>(a);
+        else ha::hpr_adj_force_final_kernel<2><<<nb, ha::kBS, 0, s>>>(a);
+        ha::hpr_adj_forcing_gather_kernel<<<nb, ha::kBS, 0, s>>>(a);
     });
 }
 
@@ -223,9 +255,10 @@ int adjoint(vg_ctx *ctx, int dim, const double *prim_in, double *prim_out, const
         return r;
     };
     // Workspace: 0 dts, 1-2 adjoint ping-pong, 3 N-bar, 4-8 extended work planes,
-    // 9 relaxation-time partial sums, 10 .. 10 + K * nops - 1 stored states (the
-    // segment's start is its checkpoint), then the checkpoints.
-    const size_t st0 = 10, ck0 = st0 + (size_t)K * nops;
+    // 9 relaxation-time partial sums, 10 sub-cycle adjoint scratch, 11 .. 11 +
+    // K * nops - 1 stored states (the segment's start is its checkpoint), then
+    // the checkpoints.
+    const size_t st0 = 11, ck0 = st0 + (size_t)K * nops;
     // ---- forward with checkpoints ---------------------------------------------------
     std::vector<double> dts;
     std::vector<double *> ckpt;
@@ -262,6 +295,7 @@ int adjoint(vg_ctx *ctx, int dim, const double *prim_in, double *prim_out, const
     if (!rc) rc = wsp(3, gbytes, &w.nbar);
     for (int i = 0; i < 5 && !rc; i++) rc = wsp(4 + i, sizeof(double) * NF * w.eplane, &w.ext[i]);
     if (!rc) rc = wsp(9, sizeof(double) * 2 * w.force_blocks, &w.tau_part);
+    if (!rc && subcycles > 0) rc = wsp(10, sizeof(double) * kForcePlanes * g.plane, &w.fs);
     std::vector<double *> states((size_t)K * nops + 1, nullptr);
     for (int k = 1; k <= K * nops && !rc; k++) rc = wsp(st0 + k - 1, gbytes, &states[k]);
     if (rc) {
