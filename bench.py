@@ -209,6 +209,7 @@ def bench_mhd_batch(args):
     ctx.reset_timing()
     ctx.set_timing(True)
     torch.cuda.synchronize()
+    log(f"one launch: {n_inst} instances, {int(cells.sum())} cells, t = 0.1 each ...")
     t0 = time.perf_counter()
     r = S.mhd_solve_batch(prim, off, dx, gamma, tf, 0.8, limiter=1, mode=S.EXACT if args.mode != "fast" else S.FAST,
                           ctx=ctx)
