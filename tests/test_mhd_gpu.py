@@ -146,3 +146,36 @@ def test_batch_rejects_bad_offsets(vg_ctx):
     prim[:, 4] = 1
     r = S.mhd_solve_batch(prim, [0, 1, 10], [0.1, 0.1], [2.0, 2.0], [0.1, 0.1], 0.8, raise_on_error=False)
     assert r.status == _lib.VG_ERR_ARG
+
+
+def test_mhd_batch_continuation_past_the_step_cap_gives_the_same_bits(vg_ctx):
+    """The launch stops every instance at 1.25x the largest step count its
+    initial wave speeds predict.  A strong blast (pressure ratio 1e5, 2048
+    cells) speeds up by more than that: 957 steps predicted, 1371 taken.  It
+    finishes on the grid path from where the launch stopped it, with the bits
+    and step count of solving it alone; the light instances match the oracle."""
+    prim, off, gamma, dx = problems.mhd_batch(200, seed=5, n_min=256, n_max=256)
+    n = 2048
+    blast = np.zeros((n, 9))
+    left = np.arange(n) <= 0.5 * n
+    blast[left] = (1.0, 0.0, 0.0, 0.0, 1000.0, 0.5, 0.0, 0.0, 0.0)
+    blast[~left] = (1.0, 0.0, 0.0, 0.0, 0.01, 0.5, 0.0, 0.0, 0.0)
+    prim = np.concatenate([prim[:off[100]], blast, prim[off[100]:]])
+    sizes = np.concatenate([np.diff(off)[:100], [n], np.diff(off)[100:]])
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    gamma = np.concatenate([gamma[:100], [1.4], gamma[100:]])
+    dx = 1.0 / sizes.astype(np.float64)
+    tf = np.full(201, 0.01)
+    r = S.mhd_solve_batch(prim, off, dx, gamma, tf, 0.8, limiter=1)
+    a, b = off[100], off[101]
+    alone = S.mhd_solve(prim[a:b], 1, float(dx[100]), 0.8, 0.01, limiter=1, params=S.MHDMaterialParameters(1.4))
+    print(f"blast: {r.steps[100]} steps", flush=True)
+    assert r.steps[100] == alone.stats.steps == 1371
+    assert r.t[100] == alone.stats.t
+    O.assert_bitwise(r.cells[a:b], alone.cells, "blast vs alone")
+    for q in (0, 57, 150, 200):
+        a, b = off[q], off[q + 1]
+        ref, dts = O.mhd_solve(prim[a:b], order=2, dim=1, dx=float(dx[q]), cfl=0.8, tfinal=0.01, max_steps=0,
+                               bias=0.0, limiter=1, gamma=float(gamma[q]))
+        assert r.steps[q] == len(dts)
+        O.assert_bitwise(r.cells[a:b], ref, f"instance {q}")

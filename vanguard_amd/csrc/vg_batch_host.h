@@ -1,7 +1,11 @@
 // Host side of the batched solve: uploads the instances, orders them by
 // decreasing estimated work (cells^2 x initial wave speed: the step count of an
 // instance grows like N * S / CFL), launches one workgroup per instance in that
-// order and collects results.
+// order and collects results.  The launch stops every instance at 1.25x the
+// largest step count the initial wave speeds predict; an instance still short
+// of its final time then continues on the grid path -- the same arithmetic
+// from the same state and clock, its sweeps spread over all CUs -- so that one
+// instance whose speeds grow during the run does not hold the whole launch.
 #pragma once
 
 #include <algorithm>
@@ -11,6 +15,7 @@
 #include <vector>
 
 #include "vg_batch.h"
+#include "vg_grid.h"
 #include "vg_runtime.h"
 
 namespace vg {
@@ -106,6 +111,25 @@ int run_batch(vg_ctx *ctx, int order, int mode, int n_inst, const int64_t *offse
         }
         std::stable_sort(order_h.begin(), order_h.end(), [&](int x, int y) { return est[x] > est[y]; });
     }
+    // The launch's step cap: 1.25x the largest predicted step count
+    // (t_final S / (CFL dx) from the initial wave speed), at least 1024.
+    int cap = max_steps;
+    if (n_inst > 1) {
+        std::vector<double> sp(n_inst);
+        VG_HIP(hipMemcpyAsync(sp.data(), b.speed, sizeof(double) * n_inst, hipMemcpyDeviceToHost, s));
+        VG_HIP(hipStreamSynchronize(s));
+        double most = 0.0;
+        bool known = true;
+        for (int q = 0; q < n_inst; q++) {
+            const double speed = (std::isfinite(sp[q]) && sp[q] > 0.0) ? sp[q] : 0.0;
+            if (!std::isfinite(t_final_h[q])) known = false;
+            else most = std::max(most, t_final_h[q] * speed / (cfl * dx_h[q]));
+        }
+        if (known) {
+            const double c = std::max(1024.0, std::ceil(1.25 * most));
+            if (c < (double)0x7fffffff && (max_steps == 0 || c < max_steps)) cap = (int)c;
+        }
+    }
     VG_HIP(hipMemcpyAsync(b.order, order_h.data(), sizeof(int) * n_inst, hipMemcpyHostToDevice, s));
 
     BatchArgs<P> a;
@@ -118,7 +142,7 @@ int run_batch(vg_ctx *ctx, int order, int mode, int n_inst, const int64_t *offse
     a.cfl = cfl;
     a.bias = bias;
     a.limiter = limiter;
-    a.max_steps = max_steps;
+    a.max_steps = cap;
     a.bc = bc;
     a.buf0 = b.buf0;
     a.buf1 = b.buf1;
@@ -149,6 +173,33 @@ int run_batch(vg_ctx *ctx, int order, int mode, int n_inst, const int64_t *offse
     VG_HIP(hipMemcpyAsync(stalled.data(), b.stalled, sizeof(int) * n_inst, hipMemcpyDeviceToHost, s));
     VG_HIP(hipMemcpyAsync(tend.data(), b.t_out, sizeof(double) * n_inst, hipMemcpyDeviceToHost, s));
     VG_HIP(hipStreamSynchronize(s));
+    // instances the cap stopped short of their final time: the rest of their
+    // run on the grid path, from their state, time and step count
+    for (int q = 0; q < n_inst && cap != max_steps; q++) {
+        if (stalled[q] || steps[q] < cap || !(tend[q] < t_final_h[q])) continue;
+        GridState<P> g;
+        rc = g.create(ctx, 1, 1, (int)(off[q + 1] - off[q]));
+        if (!rc) rc = g.load(prim_out + (size_t)NF * off[q], 0);
+        if (!rc) {
+            Clock h{};
+            h.t = tend[q];
+            h.it = steps[q];
+            hipError_t e = hipMemcpyAsync(g.clk, &h, sizeof(h), hipMemcpyHostToDevice, s);
+            if (e != hipSuccess) {
+                set_error(std::string("batch continuation clock: ") + hipGetErrorString(e));
+                rc = VG_ERR_HIP;
+            }
+        }
+        vg_run_stats st{};
+        if (!rc)
+            rc = g.run(order, mode, dx_h[q], cfl, t_final_h[q], max_steps > 0 ? max_steps - steps[q] : 0, bias,
+                       limiter, bc, params_h[q], nullptr, 0, &st);
+        if (!rc) rc = g.store(prim_out + (size_t)NF * off[q], 0);
+        g.destroy();
+        if (rc) return rc;
+        steps[q] += st.steps;
+        tend[q] = st.t;
+    }
     float dev_ms = 0.0f;
     VG_HIP(hipEventElapsedTime(&dev_ms, e0, e1));
     hipEventDestroy(e0);
