@@ -523,8 +523,17 @@ def bench_rgfm_adjoint(args):
             S.rgfm_solve(sysm, dx, 0.8, args.t_final, problems.HELIUM, problems.AIR, dy=dy, max_steps=args.warmup)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        f = S.rgfm_solve(sysm, dx, 0.8, args.t_final, problems.HELIUM, problems.AIR, dy=dy)
+        f = S.rgfm_solve(sysm, dx, 0.8, args.t_final, problems.HELIUM, problems.AIR, dy=dy, raise_on_error=False)
         fwd = time.perf_counter() - t0
+        if f.stats.nonfinite_cells > 0:
+            # the method loses positivity on this instance at this resolution
+            # (a state the oracle reproduces bit for bit): no derivative to take
+            log(f"instance {q}: M={M:.4f} steps={f.stats.steps} forward {fwd:.2f} s: "
+                f"{f.stats.nonfinite_cells} inadmissible cells, no adjoint")
+            per.append({"instance": q, "mach": M, "radius": r, "steps": int(f.stats.steps),
+                        "forward_s": round(fwd, 3), "adjoint_s": None, "circulation": None,
+                        "dcirculation_dmach": None, "finite": False})
+            continue
         seed1 = circulation_seed(f.system.material1, f.system.level_set, dx, dy)
         t0 = time.perf_counter()
         a = S.rgfm_adjoint(sysm, seed1, np.zeros_like(m2), dx, 0.8, args.t_final, problems.HELIUM, problems.AIR,
@@ -546,8 +555,8 @@ def bench_rgfm_adjoint(args):
     line = {
         "metric": "adjoint/forward cost FP64 2D RGFM shock-bubble d circulation / d Mach",
         "value": round(adj_total / fwd_total, 3), "unit": "x", "n_gpus": 1,
-        "steps": int(sum(p["steps"] for p in per)), "warmup": args.warmup,
-        "ms_per_step": round(adj_total * 1e3 / max(1, sum(p["steps"] for p in per)), 4),
+        "steps": int(sum(p["steps"] for p in per if p["adjoint_s"] is not None)), "warmup": args.warmup,
+        "ms_per_step": round(adj_total * 1e3 / max(1, sum(p["steps"] for p in per if p["adjoint_s"] is not None)), 4),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"rgfm_shock_bubble_{nx}x{ny}_hllc_t{args.t_final:g}_dgamma_dmach",
                    "instances": list(args.rgfm_instances), "checkpoint_every": int(args.checkpoint_every) or "auto"},
