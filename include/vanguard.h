@@ -99,6 +99,14 @@ int vg_ctx_set_autotune(vg_ctx *ctx, int enabled);
  * and leaves those two planes as they are -- the same bits as the 6-field
  * sweep, which keeps them at +0 and 1 exactly.  No reference counterpart. */
 int vg_ctx_set_reduced_fields(vg_ctx *ctx, int enabled);
+/* Streams a batch call may run its independent instances on at once (1..16,
+ * default 4; 1 runs them one after another on the context's stream).  The 2D
+ * RGFM batch (vg_rgfm_solve_batch_2d) hands instances, longest predicted run
+ * first, to that many worker streams, each replaying its own captured steps,
+ * so one instance's kernels fill the others' launch gaps.  Per-kernel timing
+ * (vg_ctx_set_timing) runs a batch on one stream.  Results are the same bits
+ * for every count.  No reference counterpart. */
+int vg_ctx_set_batch_streams(vg_ctx *ctx, int streams);
 int vg_ctx_tuned_shape(vg_ctx *ctx, const char *key, int *variant);
 int vg_ctx_set_tuned_shape(vg_ctx *ctx, const char *key, int variant);
 /* Frees the device workspace the context keeps between calls (the adjoint's

@@ -16,6 +16,7 @@ import pytest
 import oracle_lib as O
 from vanguard_amd import _lib, problems
 from vanguard_amd import solvers as S
+from vanguard_amd._lib import VanguardError
 
 pytestmark = pytest.mark.gpu
 
@@ -123,6 +124,43 @@ def test_rgfm_batch_matches_single_solves_and_circulation(vg_ctx):
         O.assert_bitwise(r.level_set[q], oph, f"instance {q} level set")
         ref = O.rgfm_circulation(o1, oph, dx, dy)
         assert r.circulation[q] == pytest.approx(ref, rel=1e-11, abs=1e-14)
+
+
+def test_rgfm_batch_streams_change_nothing():
+    """Five instances on 1, 2 and 4 worker streams (instances taken longest
+    predicted run first from a shared queue): every output is the same bits,
+    and the stats' step total matches."""
+    radii, machs = problems.shock_bubble_batch(5, seed=9)
+    ins = [problems.shock_bubble_2d(48, 20, length=(0.325, 0.325 * 20 / 48), center=(0.175, 0.325 * 10 / 48),
+                                    radius=float(r) * 1.5, mach=float(mc)) for r, mc in zip(radii, machs)]
+    dx, dy = ins[0][3], ins[0][4]
+    m1 = np.stack([a[0] for a in ins])
+    m2 = np.stack([a[1] for a in ins])
+    phi = np.stack([a[2] for a in ins])
+    tf = np.array([2e-5, 3e-5, 1e-5, 2.5e-5, 1.5e-5])
+    base = None
+    for streams in (1, 2, 4):
+        ctx = S.Context(0)
+        try:
+            ctx.set_batch_streams(streams)
+            print(f"batch of 5 on {streams} streams", flush=True)
+            r = S.rgfm_solve_batch_2d(m1, m2, phi, dx, dy, 0.8, tf, problems.HELIUM, problems.AIR, ctx=ctx)
+        finally:
+            ctx.close()
+        assert r.stats.steps == int(r.steps.sum()) and r.stats.nonfinite_cells == 0
+        if base is None:
+            base = r
+            continue
+        for name in ("material1", "material2", "level_set", "t", "circulation"):
+            O.assert_bitwise(getattr(r, name), getattr(base, name), f"{name} on {streams} streams")
+        assert np.array_equal(r.steps, base.steps)
+    ctx = S.Context(0)
+    try:
+        for bad in (0, 17):
+            with pytest.raises(VanguardError):
+                ctx.set_batch_streams(bad)
+    finally:
+        ctx.close()
 
 
 def test_rgfm_fast_mode_within_tolerance(vg_ctx):

@@ -73,6 +73,7 @@ SIGNATURES = {
     "vg_ctx_set_graphs": (C.c_int, [vptr, C.c_int]),
     "vg_ctx_set_autotune": (C.c_int, [vptr, C.c_int]),
     "vg_ctx_set_reduced_fields": (C.c_int, [vptr, C.c_int]),
+    "vg_ctx_set_batch_streams": (C.c_int, [vptr, C.c_int]),
     "vg_ctx_tuned_shape": (C.c_int, [vptr, C.c_char_p, C.POINTER(C.c_int)]),
     "vg_ctx_set_tuned_shape": (C.c_int, [vptr, C.c_char_p, C.c_int]),
     "vg_ctx_release_workspace": (C.c_int, [vptr]),

@@ -7,6 +7,8 @@
 //   for material 1, then material 2.
 // 1D is the same with one row, one level-set pass and one sweep per material.
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -390,7 +392,9 @@ struct RGFMGrid {
                 for (int r = 0; r < reps; r++) VG_HIP(hipGraphLaunch(graph, s));
                 VG_HIP(hipMemcpyAsync(&h, clk, sizeof(Clock), hipMemcpyDeviceToHost, s));
                 VG_HIP(hipStreamSynchronize(s));
-                if (h.done) break;
+                // a capped run is complete once its last step has started (the
+                // clock folds that step in at the next check, or at the finish)
+                if (h.done || (max_steps > 0 && h.steps_this_run >= max_steps)) break;
             }
         } else {
             for (;;) {
@@ -400,7 +404,7 @@ struct RGFMGrid {
                 }
                 VG_HIP(hipMemcpyAsync(&h, clk, sizeof(Clock), hipMemcpyDeviceToHost, s));
                 VG_HIP(hipStreamSynchronize(s));
-                if (h.done) break;
+                if (h.done || (max_steps > 0 && h.steps_this_run >= max_steps)) break;
             }
         }
         clock_finish_kernel<<<1, 1, 0, s>>>(clk);
@@ -489,6 +493,35 @@ int check_args(int dim, int ny, int nx, double dx, double dy, double cfl, double
     return VG_OK;
 }
 
+// Final-state check on the host copies: in every cell the material the level
+// set makes real there (material 1 where phi < 0, else material 2) must be
+// finite with rho > 0 and p + p_inf > 0, and phi finite.  Counts the cells that
+// fail and the first one's flat index (-1 if none).
+void scan_final(const double *m1, const double *m2, const double *phi, size_t cells, const vg_euler_params *p1,
+                const vg_euler_params *p2, int64_t &count, int64_t &first)
+{
+    count = 0;
+    first = -1;
+    for (size_t c = 0; c < cells; c++) {
+        const bool one = phi[c] < 0.0;
+        const double *w = (one ? m1 : m2) + 6 * c;
+        bool bad = !std::isfinite(phi[c]) || !(w[0] > 0.0) || !((w[4] + (one ? p1 : p2)->p_inf) > 0.0);
+        for (int f = 0; f < 6; f++) bad = bad || !std::isfinite(w[f]);
+        if (bad) {
+            if (first < 0) first = (int64_t)c;
+            count += 1;
+        }
+    }
+}
+
+int nonfinite_error(int64_t count, int64_t first)
+{
+    vg::set_error("final state holds " + std::to_string(count) +
+                  " non-finite or inadmissible cells (rho <= 0 or p + p_inf <= 0 in the material the level set "
+                  "selects), first at flat index " + std::to_string(first));
+    return VG_ERR_NONFINITE;
+}
+
 int solve(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, const double *phi_in, double *m1_out,
           double *m2_out, double *phi_out, int ny, int nx, double dx, double dy, double cfl, double t_final,
           int max_steps, double bias, int limiter, int riemann, const vg_euler_params *p1,
@@ -515,8 +548,10 @@ int solve(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, const 
     if (!rc) rc = g.store(m1_out, m2_out, phi_out);
     if (!rc && circulation) rc = g.circulation(dx, dy, circulation);
     g.destroy();
+    if (!rc) scan_final(m1_out, m2_out, phi_out, (size_t)ny * nx, p1, p2, local.nonfinite_cells, local.first_nonfinite);
     if (stats) *stats = local;
-    return rc ? rc : run_rc;
+    if (rc || run_rc) return rc ? rc : run_rc;
+    return local.nonfinite_cells > 0 ? nonfinite_error(local.nonfinite_cells, local.first_nonfinite) : VG_OK;
 }
 
 // ---- adjoint (no reference counterpart) ---------------------------------------------
@@ -837,38 +872,121 @@ int vg_rgfm_solve_batch_2d(vg_ctx *ctx, int n_inst, const double *m1_in, const d
     }
     int rc = vg::bind(ctx);
     if (rc) return rc;
-    vg::RGFMGrid g;
-    rc = g.create(ctx, 2, ny, nx);
     const size_t cells = (size_t)ny * nx;
-    vg_run_stats total{};
-    double dt_min = 0.0, dt_max = 0.0;
-    int run_rc = VG_OK;
-    for (int q = 0; q < n_inst && !rc; q++) {
-        rc = g.load(m1_in + 6 * cells * q, m2_in + 6 * cells * q, phi_in + cells * q);
-        vg_run_stats st{};
-        if (!rc) {
-            const int r = g.run(riemann, mode, dx, dy, cfl, t_final[q], max_steps, bias, limiter, to_params(p1),
-                                to_params(p2), nullptr, 0, &st);
-            if (r == VG_ERR_NONFINITE) run_rc = r;
-            else rc = r;
+    const vg::EP e1 = to_params(p1), e2 = to_params(p2);
+    std::vector<vg_run_stats> per(n_inst);
+    std::vector<char> nonfinite(n_inst, 0);
+    // One instance on grid g of context c: load, run, store, circulation.
+    auto solve_one = [&](vg::RGFMGrid &g, int q) {
+        int r = g.load(m1_in + 6 * cells * q, m2_in + 6 * cells * q, phi_in + cells * q);
+        if (!r) {
+            r = g.run(riemann, mode, dx, dy, cfl, t_final[q], max_steps, bias, limiter, e1, e2, nullptr, 0, &per[q]);
+            if (r == VG_ERR_NONFINITE) {
+                nonfinite[q] = 1;
+                r = VG_OK;
+            }
         }
-        if (!rc) rc = g.store(m1_out + 6 * cells * q, m2_out + 6 * cells * q, phi_out + cells * q);
-        if (!rc && circulation) rc = g.circulation(dx, dy, circulation + q);
+        if (!r) r = g.store(m1_out + 6 * cells * q, m2_out + 6 * cells * q, phi_out + cells * q);
+        if (!r && circulation) r = g.circulation(dx, dy, circulation + q);
+        if (!r) {
+            scan_final(m1_out + 6 * cells * q, m2_out + 6 * cells * q, phi_out + cells * q, cells, p1, p2,
+                       per[q].nonfinite_cells, per[q].first_nonfinite);
+            if (per[q].nonfinite_cells > 0) nonfinite[q] = 1;
+        }
+        return r;
+    };
+    const int nw = ctx->timing ? 1 : std::min(n_inst, ctx->batch_streams);
+    if (nw == 1) {
+        vg::RGFMGrid g;
+        rc = g.create(ctx, 2, ny, nx);
+        for (int q = 0; q < n_inst && !rc; q++) rc = solve_one(g, q);
+        g.destroy();
+    } else {
+        // Longest predicted run first: an instance's step count is about
+        // t_final times its largest signal speed max(|u|, |v|) + c (the time
+        // step's denominator), c^2 = gamma (p + p_inf) / rho.  The order only
+        // affects how well the streams stay busy.
+        std::vector<std::pair<double, int>> cost(n_inst);
+        for (int q = 0; q < n_inst; q++) {
+            double smax = 0.0;
+            const double *ms[2] = {m1_in + 6 * cells * q, m2_in + 6 * cells * q};
+            const vg_euler_params *ps[2] = {p1, p2};
+            for (int k = 0; k < 2; k++)
+                for (size_t c = 0; c < cells; c++) {
+                    const double *w = ms[k] + 6 * c;
+                    const double a2 = ps[k]->gamma * (w[4] + ps[k]->p_inf) / w[0];
+                    if (a2 > 0.0 && std::isfinite(a2))
+                        smax = std::max(smax, std::max(std::fabs(w[1]), std::fabs(w[2])) + std::sqrt(a2));
+                }
+            cost[q] = {-t_final[q] * smax, q};
+        }
+        std::stable_sort(cost.begin(), cost.end());
+        rc = vg::batch_workers(ctx, nw);
+        std::atomic<int> next{0};
+        std::atomic<bool> stop{false};
+        std::vector<int> wrc(nw, VG_OK);
+        std::vector<std::string> werr(nw);
+        // Each worker runs instances from the shared queue on its own stream
+        // and grid until the queue is empty or another worker has failed.
+        auto worker = [&](int k) {
+            vg_ctx *c = ctx->workers[k];
+            int r = vg::bind(c);
+            vg::RGFMGrid g;
+            if (!r) r = g.create(c, 2, ny, nx);
+            while (!r && !stop.load()) {
+                const int i = next.fetch_add(1);
+                if (i >= n_inst) break;
+                r = solve_one(g, cost[i].second);
+            }
+            if (c->stream) hipStreamSynchronize(c->stream);
+            g.destroy();
+            if (r) {
+                stop = true;
+                werr[k] = vg_last_error();
+            }
+            wrc[k] = r;
+        };
+        if (!rc) {
+            std::vector<std::thread> threads;
+            for (int k = 0; k < nw; k++) threads.emplace_back(worker, k);
+            for (auto &t : threads) t.join();
+            vg::merge_worker_tuning(ctx);
+            for (int k = 0; k < nw && !rc; k++)
+                if (wrc[k]) {
+                    rc = wrc[k];
+                    vg::set_error(werr[k]);
+                }
+        }
+    }
+    // Totals; first_nonfinite is a flat index over the concatenated instances.
+    vg_run_stats total{};
+    total.first_nonfinite = -1;
+    double dt_min = 0.0, dt_max = 0.0;
+    bool any_nonfinite = false;
+    for (int q = 0; q < n_inst; q++) {
+        const vg_run_stats &st = per[q];
         if (steps) steps[q] = st.steps;
         if (t_out) t_out[q] = st.t;
         total.steps += st.steps;
         total.device_ms += st.device_ms;
         total.wall_ms += st.wall_ms;
+        total.nonfinite_cells += st.nonfinite_cells;
+        if (total.first_nonfinite < 0 && st.nonfinite_cells > 0)
+            total.first_nonfinite = (int64_t)cells * q + st.first_nonfinite;
         if (q == 0 || st.dt_min < dt_min) dt_min = st.dt_min;
         if (q == 0 || st.dt_max > dt_max) dt_max = st.dt_max;
+        any_nonfinite = any_nonfinite || nonfinite[q];
     }
-    g.destroy();
     total.total_steps = total.steps;
     total.dt_min = dt_min;
     total.dt_max = dt_max;
-    total.first_nonfinite = -1;
     if (stats) *stats = total;
-    return rc ? rc : run_rc;
+    if (!rc && any_nonfinite) {
+        vg::set_error("RGFM batch: " + std::to_string(total.nonfinite_cells) +
+                      " non-finite or inadmissible final cells, or a stalled time step, in at least one instance");
+        return VG_ERR_NONFINITE;
+    }
+    return rc;
 }
 
 int vg_rgfm_adjoint_1d(vg_ctx *ctx, const double *m1_in, const double *m2_in, const double *phi_in, double *m1_out,

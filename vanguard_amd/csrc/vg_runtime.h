@@ -52,6 +52,11 @@ struct vg_ctx {
     // Run a grid with its narrower physics (ReducedPhys) when the fields that
     // physics drops are constant; results are the same bits either way.
     bool reduce_fields = true;
+    // Streams a batch call may run independent instances on at once (1: one
+    // after another on this context's stream), and the worker contexts (one
+    // stream each) that carry them, kept between calls.
+    int batch_streams = 4;
+    std::vector<vg_ctx *> workers;
     std::map<std::string, int> tuned;
     std::map<std::string, vg::KernelTimer> timers;
     // Event pairs recorded around launches and not yet resolved.
@@ -88,5 +93,10 @@ struct vg_ctx {
 namespace vg {
 // Makes the context's device current; returns VG_ERR_HIP on failure.
 int bind(vg_ctx *ctx);
+// Ensures n worker contexts on ctx's device and gives them ctx's switches and
+// tuned shapes.
+int batch_workers(vg_ctx *ctx, int n);
+// Copies the shapes the workers tuned into ctx's table (existing keys kept).
+void merge_worker_tuning(vg_ctx *ctx);
 int launch_status(const char *what);
 }  // namespace vg

@@ -20,6 +20,30 @@ int bind(vg_ctx *ctx)
     return VG_OK;
 }
 
+int batch_workers(vg_ctx *ctx, int n)
+{
+    while ((int)ctx->workers.size() < n) {
+        vg_ctx *w = nullptr;
+        const int rc = vg_ctx_create(&w, ctx->device);
+        if (rc) return rc;
+        ctx->workers.push_back(w);
+    }
+    for (vg_ctx *w : ctx->workers) {
+        w->graphs = ctx->graphs;
+        w->autotune = ctx->autotune;
+        w->reduce_fields = ctx->reduce_fields;
+        w->timing = false;
+        for (const auto &kv : ctx->tuned) w->tuned[kv.first] = kv.second;
+    }
+    return VG_OK;
+}
+
+void merge_worker_tuning(vg_ctx *ctx)
+{
+    for (vg_ctx *w : ctx->workers)
+        for (const auto &kv : w->tuned) ctx->tuned.insert(kv);
+}
+
 int launch_status(const char *what)
 {
     hipError_t e = hipGetLastError();
@@ -129,7 +153,16 @@ void vg_ctx_destroy(vg_ctx *ctx)
     }
     for (hipEvent_t e : ctx->event_pool) hipEventDestroy(e);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
+    for (vg_ctx *w : ctx->workers) vg_ctx_destroy(w);
     delete ctx;
+}
+
+int vg_ctx_set_batch_streams(vg_ctx *ctx, int streams)
+{
+    VG_CHECK(ctx != nullptr, "vg_ctx_set_batch_streams: null context");
+    VG_CHECK(streams >= 1 && streams <= 16, "vg_ctx_set_batch_streams: streams must be 1..16");
+    ctx->batch_streams = streams;
+    return VG_OK;
 }
 
 int vg_ctx_set_timing(vg_ctx *ctx, int enabled)

@@ -68,6 +68,11 @@ class Context:
         constant with the 4-field kernels (default on; same bits either way)."""
         check(self.lib.vg_ctx_set_reduced_fields(self.handle, int(on)), "vg_ctx_set_reduced_fields")
 
+    def set_batch_streams(self, streams: int):
+        """Streams a batch call may run its instances on at once (1..16);
+        results are the same bits for every count."""
+        check(self.lib.vg_ctx_set_batch_streams(self.handle, int(streams)), "vg_ctx_set_batch_streams")
+
     def tuned_shape(self, physics: str, order: int, mode: int, ny: int, nx: int) -> int:
         v = C.c_int()
         key = f"{physics}/{order}/{mode}/{ny}/{nx}".encode()
