@@ -163,6 +163,38 @@ def test_rgfm_batch_streams_change_nothing():
         ctx.close()
 
 
+def test_rgfm_reduced_fields_give_the_same_bits():
+    """Shock-bubble states hold w = +0 and lambda = 1 in both materials, so a
+    run takes the 4-field sweeps; with the switch off it takes the 6-field
+    ones.  Same bits in 1D and 2D (materials, level set, time steps), and the
+    two constant planes come back unchanged."""
+    m1, m2, phi, dx, dy = problems.shock_bubble_2d(64, 24, length=(0.325, 0.325 * 24 / 64),
+                                                   center=(0.175, 0.325 * 12 / 64), radius=0.03, mach=1.25)
+    for m in (m1, m2):
+        assert np.all(m[..., 3].view(np.uint64) == 0) and np.all(m[..., 5] == 1.0)
+    cases = {"2d": (m1, m2, phi, dy), "1d": (m1[12].copy(), m2[12].copy(), phi[12].copy(), None)}
+    out = {}
+    for on in (True, False):
+        ctx = S.Context(0)
+        try:
+            ctx.set_reduced_fields(on)
+            for name, (a, b, ph, ddy) in cases.items():
+                print(f"rgfm {name}, reduced fields {on}", flush=True)
+                out[name, on] = S.rgfm_solve(S.MultimaterialSystem(a, b, ph), dx, 0.8, 3e-5, problems.HELIUM,
+                                             problems.AIR, dy=ddy, ctx=ctx)
+        finally:
+            ctx.close()
+    for name in cases:
+        a, b = out[name, True], out[name, False]
+        assert a.stats.steps == b.stats.steps > 5
+        O.assert_bitwise(a.dt, b.dt, f"{name} time steps")
+        O.assert_bitwise(a.system.material1, b.system.material1, f"{name} material 1")
+        O.assert_bitwise(a.system.material2, b.system.material2, f"{name} material 2")
+        O.assert_bitwise(a.system.level_set, b.system.level_set, f"{name} level set")
+        for m in (a.system.material1, a.system.material2):
+            assert np.all(m[..., 3].view(np.uint64) == 0) and np.all(m[..., 5] == 1.0)
+
+
 def test_rgfm_fast_mode_within_tolerance(vg_ctx):
     g = O.golden("rgfm_2d_bubble96")
     a = gpu_golden(g)

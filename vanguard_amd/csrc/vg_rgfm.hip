@@ -64,6 +64,8 @@ struct RGFMGrid {
     unsigned long long *partials = nullptr;
     int max_blocks = 0, circ_blocks = 0;
     int y_variant = 0;                  // autotuned y-sweep shape
+    bool reduced = false;               // this run sweeps with the 4-field EulerPlanarPhys
+    int *flag = nullptr;                // constant-plane check result
     hipGraphExec_t graph = nullptr;     // captured chunk of steps
     std::string graph_key;
     static constexpr int kGraphSteps = 8;   // even: buffers return to their start
@@ -105,6 +107,7 @@ struct RGFMGrid {
         hipFree(circ);
         hipFree(dt_dev);
         hipFree(tune_clk);
+        hipFree(flag);
         if (graph) hipGraphExecDestroy(graph);
     }
 
@@ -172,11 +175,11 @@ struct RGFMGrid {
         return launch_status("rgfm fill (recomputation)");
     }
 
-    template <int MODE>
-    int sweep(int dir, const double *in, double *out, double scale, double h, double bias, int limiter, const EP &m,
-              Clock *use_clk = nullptr)
+    template <class Q, int MODE>
+    int sweep_as(int dir, const double *in, double *out, double scale, double h, double bias, int limiter,
+                 const EP &m, Clock *use_clk)
     {
-        SweepArgs<EulerPhys> a;
+        SweepArgs<Q> a;
         a.in = in;
         a.out = out;
         a.plane = plane;
@@ -195,10 +198,82 @@ struct RGFMGrid {
         if (dir == 0) {
             dim3 grid((nx + (BSX - 4) - 1) / (BSX - 4), (ny + RX - 1) / RX);
             return ctx->timed("rgfm_sweep_x", [&] {
-                sweep_x_kernel<EulerPhys, 2, MODE, BSX, RX><<<grid, BSX, 0, ctx->stream>>>(a);
+                sweep_x_kernel<Q, 2, MODE, BSX, RX><<<grid, BSX, 0, ctx->stream>>>(a);
             });
         }
-        return launch_y<EulerPhys, 2, MODE>(ctx, y_variant, a);
+        return launch_y<Q, 2, MODE>(ctx, y_variant, a);
+    }
+
+    template <int MODE>
+    int sweep(int dir, const double *in, double *out, double scale, double h, double bias, int limiter, const EP &m,
+              Clock *use_clk = nullptr)
+    {
+        if (reduced) return sweep_as<EulerPlanarPhys, MODE>(dir, in, out, scale, h, bias, limiter, m, use_clk);
+        return sweep_as<EulerPhys, MODE>(dir, in, out, scale, h, bias, limiter, m, use_clk);
+    }
+
+    // The 4-field sweeps for this run when both materials hold w = +0 and
+    // lambda = 1 in every cell of the current buffers.  The ghost fill writes
+    // only rho, the normal velocity and p, and the 6-field sweeps keep those two
+    // planes at their bits while computing the same rho, u, v, p as the 4-field
+    // ones, so the results are the same bits.  The other buffers receive the two
+    // planes here; the sweeps then leave them alone.
+    int choose_physics()
+    {
+        reduced = false;
+        if (!ctx->reduce_fields) return VG_OK;
+        using R = ReducedPhys<EulerPhys>;
+        hipStream_t s = ctx->stream;
+        if (!flag) VG_HIP(hipMalloc(&flag, sizeof(int)));
+        const int one = 1;
+        VG_HIP(hipMemcpyAsync(flag, &one, sizeof(int), hipMemcpyHostToDevice, s));
+        constant_planes_kernel<R><<<blocks_for(cells), 256, 0, s>>>(m1[cur], plane, cells, flag);
+        constant_planes_kernel<R><<<blocks_for(cells), 256, 0, s>>>(m2[cur], plane, cells, flag);
+        int rc = launch_status("constant_planes");
+        if (rc) return rc;
+        int ok = 0;
+        VG_HIP(hipMemcpyAsync(&ok, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+        VG_HIP(hipStreamSynchronize(s));
+        if (!ok) return VG_OK;
+        for (int i = 0; i < R::kConst; i++) {
+            const long long off = (long long)R::plane(i) * plane;
+            VG_HIP(hipMemcpyAsync(m1[1 - cur] + off, m1[cur] + off, sizeof(double) * cells, hipMemcpyDeviceToDevice,
+                                  s));
+            VG_HIP(hipMemcpyAsync(m2[1 - cur] + off, m2[cur] + off, sizeof(double) * cells, hipMemcpyDeviceToDevice,
+                                  s));
+        }
+        reduced = true;
+        return VG_OK;
+    }
+
+    // y-sweep shape of this run's physics (shared with Euler grids of the same
+    // shape and physics).
+    template <class Q>
+    int tune_as(int mode, double dx, double dy, double bias, int limiter, const EP &m)
+    {
+        if (!tune_clk) VG_HIP(hipMalloc(&tune_clk, sizeof(Clock)));
+        Clock tc{};
+        tc.dt = 1e-3 * std::min(dx, dy);
+        VG_HIP(hipMemcpyAsync(tune_clk, &tc, sizeof(Clock), hipMemcpyHostToDevice, ctx->stream));
+        SweepArgs<Q> ta;
+        ta.in = m1[cur];
+        ta.out = m1[1 - cur];
+        ta.plane = plane;
+        ta.ny = ny;
+        ta.nx = nx;
+        ta.bc = VG_BC_TRANSMISSIVE;
+        ta.limiter = limiter;
+        ta.dx = dy;
+        ta.bias = bias;
+        ta.dt_scale = 1.0;
+        ta.dim = 2;
+        ta.fuse_max = 0;
+        ta.m = m;
+        ta.clk = tune_clk;
+        ta.partials = nullptr;
+        const std::string key = tune_key(Q::name, 2, mode, ny, nx);
+        return (mode == VG_MODE_FAST) ? tune_y<Q, 2, MODE_FAST>(ctx, key, ta, y_variant)
+                                      : tune_y<Q, 2, MODE_EXACT>(ctx, key, ta, y_variant);
     }
 
     // One step; with fixed_dts the time step is fixed_dts[it] (the clock's
@@ -312,31 +387,18 @@ struct RGFMGrid {
             dt_dev_cap = dt_cap;
         }
         double *dlog = (dt_log && dt_cap > 0) ? dt_dev : nullptr;
+        int prc = choose_physics();
+        if (prc) return prc;
+        // every exit of this run returns to the 6-field sweeps: steps enqueued
+        // outside a run (the fixed-dt forward, the adjoint's recording
+        // recomputation) write all six planes of their outputs
+        struct Reset {
+            bool &r;
+            ~Reset() { r = false; }
+        } reset{reduced};
         if (dim == 2) {
-            // y-sweep shape for this grid (shared with Euler grids of the same shape).
-            if (!tune_clk) VG_HIP(hipMalloc(&tune_clk, sizeof(Clock)));
-            Clock tc{};
-            tc.dt = 1e-3 * std::min(dx, dy);
-            VG_HIP(hipMemcpyAsync(tune_clk, &tc, sizeof(Clock), hipMemcpyHostToDevice, s));
-            SweepArgs<EulerPhys> ta;
-            ta.in = m1[cur];
-            ta.out = m1[1 - cur];
-            ta.plane = plane;
-            ta.ny = ny;
-            ta.nx = nx;
-            ta.bc = VG_BC_TRANSMISSIVE;
-            ta.limiter = limiter;
-            ta.dx = dy;
-            ta.bias = bias;
-            ta.dt_scale = 1.0;
-            ta.dim = 2;
-            ta.fuse_max = 0;
-            ta.m = p1;
-            ta.clk = tune_clk;
-            ta.partials = nullptr;
-            int rc = (mode == VG_MODE_FAST)
-                         ? tune_y<EulerPhys, 2, MODE_FAST>(ctx, tune_key("euler", 2, mode, ny, nx), ta, y_variant)
-                         : tune_y<EulerPhys, 2, MODE_EXACT>(ctx, tune_key("euler", 2, mode, ny, nx), ta, y_variant);
+            int rc = reduced ? tune_as<EulerPlanarPhys>(mode, dx, dy, bias, limiter, p1)
+                             : tune_as<EulerPhys>(mode, dx, dy, bias, limiter, p1);
             if (rc) return rc;
         }
         hipEvent_t e0, e1;
@@ -359,7 +421,8 @@ struct RGFMGrid {
             key.append(reinterpret_cast<const char *>(num), sizeof(num));
             key += "|" + std::to_string(riemann) + "|" + std::to_string(mode) + "|" + std::to_string(limiter) + "|" +
                    std::to_string((unsigned long long)(uintptr_t)dlog) + "|" + std::to_string(dt_cap) + "|" +
-                   std::to_string(y_variant) + "|" + std::to_string(cur) + "|" + std::to_string(pcur);
+                   std::to_string(y_variant) + "|" + std::to_string(cur) + "|" + std::to_string(pcur) + "|" +
+                   std::to_string((int)reduced);
             if (!graph || key != graph_key) {
                 if (graph) {
                     hipGraphExecDestroy(graph);
