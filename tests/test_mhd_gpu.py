@@ -148,6 +148,53 @@ def test_batch_rejects_bad_offsets(vg_ctx):
     assert r.status == _lib.VG_ERR_ARG
 
 
+def test_mhd_batch_work_budget_gives_the_same_bits():
+    """With more instances than resident workgroups (the MHD batch kernel's
+    55 KB of LDS allows two per CU, so at most 512 on 256 CUs), each instance
+    also stops at the launch's work budget (predicted cells x steps over the
+    resident workgroups, at least 1024 steps).  One 4096-cell instance of about
+    1230 steps among 1300 light ones stops at 1024 and finishes on the grid path:
+    the bits and step count of solving it alone, on 1 or 4 continuation
+    streams; the light instances match the oracle."""
+    prim, off, gamma, dx = problems.mhd_batch(1300, seed=7, n_min=32, n_max=32)
+    n = 4096
+    heavy = np.zeros((n, 9))
+    left = np.arange(n) <= 0.5 * n
+    heavy[left] = (1.0, 0.0, 0.0, 0.0, 10.0, 0.5, 0.2, 0.0, 0.0)
+    heavy[~left] = (0.5, 0.0, 0.0, 0.0, 1.0, 0.5, -0.2, 0.0, 0.0)
+    prim = np.concatenate([heavy, prim])
+    sizes = np.concatenate([[n], np.diff(off)])
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    gamma = np.concatenate([[1.4], gamma])
+    dx = 1.0 / sizes.astype(np.float64)
+    tf = np.full(1301, 0.01)
+    tf[0] = 0.06
+    alone = S.mhd_solve(heavy, 1, float(dx[0]), 0.8, 0.06, limiter=1, params=S.MHDMaterialParameters(1.4))
+    assert alone.stats.steps > 1024
+    base = None
+    for streams in (1, 4):
+        ctx = S.Context(0)
+        try:
+            ctx.set_batch_streams(streams)
+            print(f"1301 instances, {streams} continuation streams", flush=True)
+            r = S.mhd_solve_batch(prim, off, dx, gamma, tf, 0.8, limiter=1, ctx=ctx)
+        finally:
+            ctx.close()
+        assert r.steps[0] == alone.stats.steps and r.t[0] == alone.stats.t
+        O.assert_bitwise(r.cells[:n], alone.cells, f"heavy instance vs alone, {streams} streams")
+        if base is None:
+            base = r
+        else:
+            O.assert_bitwise(r.cells, base.cells, "1 vs 4 continuation streams")
+            assert np.array_equal(r.steps, base.steps)
+    for q in (1, 650, 1300):
+        a, b = off[q], off[q + 1]
+        ref, dts = O.mhd_solve(prim[a:b], order=2, dim=1, dx=float(dx[q]), cfl=0.8, tfinal=0.01, max_steps=0,
+                               bias=0.0, limiter=1, gamma=float(gamma[q]))
+        assert base.steps[q] == len(dts)
+        O.assert_bitwise(base.cells[a:b], ref, f"instance {q}")
+
+
 def test_mhd_batch_continuation_past_the_step_cap_gives_the_same_bits(vg_ctx):
     """The launch stops every instance at 1.25x the largest step count its
     initial wave speeds predict.  A strong blast (pressure ratio 1e5, 2048

@@ -42,6 +42,7 @@ struct BatchArgs {
     int *stalled;
     double *dt_log;                   // single-instance runs: the dt sequence
     int dt_cap;
+    double work_budget;               // > 0: an instance of n cells stops after max(1024, ceil(budget / n)) steps
 };
 
 template <int BS>
@@ -89,10 +90,19 @@ __global__ __launch_bounds__(BS) void batch_solve_kernel(BatchArgs<P> a)
 
     // The solve loop (secondordersolver.cpp:295-311): dt from the maximum
     // wave speed, x0.2 for the first six steps, clipped to the final time.
+    // This instance's step cap: the launch's, lowered for a large instance by
+    // the work budget (cells x steps), so no single workgroup holds the launch
+    // much past the others; the host continues such an instance elsewhere.
+    int cap = a.max_steps;
+    if (a.work_budget > 0.0) {
+        const double c = ceil(a.work_budget / (double)n);
+        const int cq = (c < 1024.0) ? 1024 : (c > 2.0e9 ? 2000000000 : (int)c);
+        if (cap == 0 || cq < cap) cap = cq;
+    }
     double t = 0.0;
     int it = 0, stalled = 0;
     for (;;) {
-        if (!(t < T) || (a.max_steps > 0 && it >= a.max_steps)) break;
+        if (!(t < T) || (cap > 0 && it >= cap)) break;
         const double S = __longlong_as_double((long long)key);
         double dt = a.cfl * (dx / S);
         if (it <= 5) dt *= 0.2;

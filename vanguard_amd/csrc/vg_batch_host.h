@@ -2,16 +2,23 @@
 // decreasing estimated work (cells^2 x initial wave speed: the step count of an
 // instance grows like N * S / CFL), launches one workgroup per instance in that
 // order and collects results.  The launch stops every instance at 1.25x the
-// largest step count the initial wave speeds predict; an instance still short
-// of its final time then continues on the grid path -- the same arithmetic
-// from the same state and clock, its sweeps spread over all CUs -- so that one
-// instance whose speeds grow during the run does not hold the whole launch.
+// largest step count the initial wave speeds predict and, when there are more
+// instances than resident workgroups, at a work budget of cells x steps (the
+// predicted total over the resident workgroups); an instance still short of
+// its final time then continues on the grid path -- the same arithmetic from
+// the same state and clock, its sweeps spread over all CUs, several instances
+// at once on the context's worker streams -- so that no single instance, one
+// whose speeds grow during the run or one both large and long, holds the
+// whole launch on one CU.
 #pragma once
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <numeric>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "vg_batch.h"
@@ -34,6 +41,18 @@ struct BatchBuffers {
         hipFree(where); hipFree(stalled); hipFree(order);
     }
 };
+
+// Workgroups of the batch kernel resident at once on the device.
+template <class P, int ORDER, int MODE, int BS>
+int batch_slots(vg_ctx *ctx)
+{
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_solve_kernel<P, ORDER, MODE, BS>, BS, 0) !=
+        hipSuccess)
+        per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) cus = 1;
+    return std::max(1, per_cu) * std::max(1, cus);
+}
 
 template <class P, int ORDER, int MODE, int BS>
 int launch_batch(vg_ctx *ctx, const BatchArgs<P> &args, int n_inst)
@@ -130,6 +149,30 @@ int run_batch(vg_ctx *ctx, int order, int mode, int n_inst, const int64_t *offse
             if (c < (double)0x7fffffff && (max_steps == 0 || c < max_steps)) cap = (int)c;
         }
     }
+    // With more instances than resident workgroups, each instance also gets a
+    // work budget (cells x steps): the predicted total over the resident
+    // workgroups, so a large, long instance leaves the launch at about the
+    // time the others finish instead of holding it alone on one CU.
+    double budget = 0.0;
+    if (n_inst > 1) {
+        const int slots = (order == 2 && mode == 0)   ? batch_slots<P, 2, 0, BS>(ctx)
+                          : (order == 2)              ? batch_slots<P, 2, 1, BS>(ctx)
+                          : (mode == 0)               ? batch_slots<P, 1, 0, BS>(ctx)
+                                                      : batch_slots<P, 1, 1, BS>(ctx);
+        if (n_inst > slots) {
+            std::vector<double> sp(n_inst);
+            VG_HIP(hipMemcpyAsync(sp.data(), b.speed, sizeof(double) * n_inst, hipMemcpyDeviceToHost, s));
+            VG_HIP(hipStreamSynchronize(s));
+            double total = 0.0;
+            bool known = true;
+            for (int q = 0; q < n_inst; q++) {
+                const double speed = (std::isfinite(sp[q]) && sp[q] > 0.0) ? sp[q] : 0.0;
+                if (!std::isfinite(t_final_h[q])) known = false;
+                else total += (double)(off[q + 1] - off[q]) * (t_final_h[q] * speed / (cfl * dx_h[q]));
+            }
+            if (known && total > 0.0 && std::isfinite(total)) budget = total / slots;
+        }
+    }
     VG_HIP(hipMemcpyAsync(b.order, order_h.data(), sizeof(int) * n_inst, hipMemcpyHostToDevice, s));
 
     BatchArgs<P> a;
@@ -152,6 +195,7 @@ int run_batch(vg_ctx *ctx, int order, int mode, int n_inst, const int64_t *offse
     a.stalled = b.stalled;
     a.dt_log = (n_inst == 1) ? b.dt_log : nullptr;
     a.dt_cap = dt_cap;
+    a.work_budget = budget;
 
     hipEvent_t e0, e1;
     VG_HIP(hipEventCreate(&e0));
@@ -173,33 +217,93 @@ int run_batch(vg_ctx *ctx, int order, int mode, int n_inst, const int64_t *offse
     VG_HIP(hipMemcpyAsync(stalled.data(), b.stalled, sizeof(int) * n_inst, hipMemcpyDeviceToHost, s));
     VG_HIP(hipMemcpyAsync(tend.data(), b.t_out, sizeof(double) * n_inst, hipMemcpyDeviceToHost, s));
     VG_HIP(hipStreamSynchronize(s));
-    // instances the cap stopped short of their final time: the rest of their
-    // run on the grid path, from their state, time and step count
-    for (int q = 0; q < n_inst && cap != max_steps; q++) {
-        if (stalled[q] || steps[q] < cap || !(tend[q] < t_final_h[q])) continue;
-        GridState<P> g;
-        rc = g.create(ctx, 1, 1, (int)(off[q + 1] - off[q]));
-        if (!rc) rc = g.load(prim_out + (size_t)NF * off[q], 0);
-        if (!rc) {
+    // Instances a cap stopped short of their final time (and of the caller's
+    // max_steps): the rest of their run on the grid path, from their state,
+    // time and step count, largest remaining work first, on the context's
+    // worker streams so several continue at once.
+    std::vector<std::pair<double, int>> cont;
+    for (int q = 0; q < n_inst; q++) {
+        if (stalled[q] || !(tend[q] < t_final_h[q]) || (max_steps > 0 && steps[q] >= max_steps)) continue;
+        const double left = (tend[q] > 0.0) ? steps[q] * ((t_final_h[q] - tend[q]) / tend[q]) : 1.0;
+        cont.push_back({-(double)(off[q + 1] - off[q]) * left, q});
+    }
+    std::stable_sort(cont.begin(), cont.end());
+    // Every continuation grid is allocated here, before any worker starts, and
+    // freed after they all finish: no allocation or free overlaps a worker's
+    // graph capture.
+    std::vector<GridState<P>> grids(cont.size());
+    for (size_t k = 0; k < cont.size() && !rc; k++) {
+        const int q = cont[k].second;
+        rc = grids[k].create(ctx, 1, 1, (int)(off[q + 1] - off[q]));
+        if (!rc) rc = grids[k].ensure_staging();
+    }
+    auto continue_one = [&](vg_ctx *c, size_t k) {
+        const int q = cont[k].second;
+        GridState<P> &g = grids[k];
+        g.ctx = c;   // run on the worker's stream
+        int r = g.load(prim_out + (size_t)NF * off[q], 0);
+        if (!r) {
             Clock h{};
             h.t = tend[q];
             h.it = steps[q];
-            hipError_t e = hipMemcpyAsync(g.clk, &h, sizeof(h), hipMemcpyHostToDevice, s);
+            hipError_t e = hipMemcpyAsync(g.clk, &h, sizeof(h), hipMemcpyHostToDevice, c->stream);
             if (e != hipSuccess) {
                 set_error(std::string("batch continuation clock: ") + hipGetErrorString(e));
-                rc = VG_ERR_HIP;
+                r = VG_ERR_HIP;
             }
         }
         vg_run_stats st{};
-        if (!rc)
-            rc = g.run(order, mode, dx_h[q], cfl, t_final_h[q], max_steps > 0 ? max_steps - steps[q] : 0, bias,
-                       limiter, bc, params_h[q], nullptr, 0, &st);
-        if (!rc) rc = g.store(prim_out + (size_t)NF * off[q], 0);
-        g.destroy();
-        if (rc) return rc;
-        steps[q] += st.steps;
-        tend[q] = st.t;
+        if (!r)
+            r = g.run(order, mode, dx_h[q], cfl, t_final_h[q], max_steps > 0 ? max_steps - steps[q] : 0, bias,
+                      limiter, bc, params_h[q], nullptr, 0, &st);
+        if (!r) r = g.store(prim_out + (size_t)NF * off[q], 0);
+        if (c->stream) hipStreamSynchronize(c->stream);
+        g.ctx = ctx;
+        if (!r) {
+            steps[q] += st.steps;
+            tend[q] = st.t;
+        }
+        return r;
+    };
+    const int nw = ctx->timing ? 1 : std::min((int)cont.size(), ctx->batch_streams);
+    if (rc) {
+        // creation failed: nothing runs
+    } else if (nw <= 1) {
+        for (size_t k = 0; k < cont.size() && !rc; k++) rc = continue_one(ctx, k);
+    } else {
+        rc = batch_workers(ctx, nw);
+        std::atomic<int> next{0};
+        std::atomic<bool> stop{false};
+        std::vector<int> wrc(nw, VG_OK);
+        std::vector<std::string> werr(nw);
+        auto worker = [&](int k) {
+            vg_ctx *c = ctx->workers[k];
+            int r = bind(c);
+            while (!r && !stop.load()) {
+                const int i = next.fetch_add(1);
+                if (i >= (int)cont.size()) break;
+                r = continue_one(c, (size_t)i);
+            }
+            if (r) {
+                stop = true;
+                werr[k] = vg_last_error();
+            }
+            wrc[k] = r;
+        };
+        if (!rc) {
+            std::vector<std::thread> threads;
+            for (int k = 0; k < nw; k++) threads.emplace_back(worker, k);
+            for (auto &t : threads) t.join();
+            merge_worker_tuning(ctx);
+            for (int k = 0; k < nw && !rc; k++)
+                if (wrc[k]) {
+                    rc = wrc[k];
+                    set_error(werr[k]);
+                }
+        }
     }
+    for (auto &g : grids) g.destroy();
+    if (rc) return rc;
     float dev_ms = 0.0f;
     VG_HIP(hipEventElapsedTime(&dev_ms, e0, e1));
     hipEventDestroy(e0);
@@ -210,7 +314,8 @@ int run_batch(vg_ctx *ctx, int order, int mode, int n_inst, const int64_t *offse
     }
     if (b.dt_log && n_inst == 1 && steps[0] > 0) {
         const int n = std::min(steps[0], dt_cap);
-        VG_HIP(hipMemcpy(dt_log_h, b.dt_log, sizeof(double) * n, hipMemcpyDeviceToHost));
+        VG_HIP(hipMemcpyAsync(dt_log_h, b.dt_log, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        VG_HIP(hipStreamSynchronize(s));
     }
     long long total_steps = 0, stalled_count = 0;
     for (int q = 0; q < n_inst; q++) {

@@ -194,8 +194,11 @@ struct GridState {
         const long long gx = (nx + kBSX - 4 - 1) / (kBSX - 4) + 1;
         const long long slots = std::max<long long>(std::max<long long>(gx * ny, cells / 64 + 1), 4096);
         VG_HIP(hipMalloc(&partials, sizeof(unsigned long long) * slots));
+        // stream-ordered copies only: a synchronous one would join the legacy
+        // stream, which HIP refuses while another thread captures a graph
         Clock h{};
-        VG_HIP(hipMemcpy(clk, &h, sizeof(Clock), hipMemcpyHostToDevice));
+        VG_HIP(hipMemcpyAsync(clk, &h, sizeof(Clock), hipMemcpyHostToDevice, ctx->stream));
+        VG_HIP(hipStreamSynchronize(ctx->stream));
         return VG_OK;
     }
 
@@ -556,7 +559,10 @@ struct GridState {
         cur = start_cur ^ (h.steps_this_run & 1);
         if (dlog) {
             const int n = h.steps_this_run < dt_cap ? h.steps_this_run : dt_cap;
-            if (n > 0) VG_HIP(hipMemcpy(dt_log, dt_dev, sizeof(double) * n, hipMemcpyDeviceToHost));
+            if (n > 0) {
+                VG_HIP(hipMemcpyAsync(dt_log, dt_dev, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+                VG_HIP(hipStreamSynchronize(ctx->stream));
+            }
         }
         if (ctx->timing) {
             rc = ctx->resolve_timing();

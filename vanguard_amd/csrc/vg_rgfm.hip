@@ -482,7 +482,10 @@ struct RGFMGrid {
         pcur = start_pcur ^ (h.steps_this_run & 1);
         if (dlog) {
             const int n = std::min(h.steps_this_run, dt_cap);
-            if (n > 0) VG_HIP(hipMemcpy(dt_log, dt_dev, sizeof(double) * n, hipMemcpyDeviceToHost));
+            if (n > 0) {
+                VG_HIP(hipMemcpyAsync(dt_log, dt_dev, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+                VG_HIP(hipStreamSynchronize(s));
+            }
         }
         if (ctx->timing) {
             int rc = ctx->resolve_timing();

@@ -94,8 +94,9 @@ namespace vg {
 // Makes the context's device current; returns VG_ERR_HIP on failure.
 int bind(vg_ctx *ctx);
 // Ensures n worker contexts on ctx's device and gives them ctx's switches and
-// tuned shapes.
-int batch_workers(vg_ctx *ctx, int n);
+// tuned shapes; `graphs` false makes them launch eagerly (no stream capture,
+// for workers that allocate and free grids while others run).
+int batch_workers(vg_ctx *ctx, int n, bool graphs = true);
 // Copies the shapes the workers tuned into ctx's table (existing keys kept).
 void merge_worker_tuning(vg_ctx *ctx);
 int launch_status(const char *what);

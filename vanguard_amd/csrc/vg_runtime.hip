@@ -20,7 +20,7 @@ int bind(vg_ctx *ctx)
     return VG_OK;
 }
 
-int batch_workers(vg_ctx *ctx, int n)
+int batch_workers(vg_ctx *ctx, int n, bool graphs)
 {
     while ((int)ctx->workers.size() < n) {
         vg_ctx *w = nullptr;
@@ -29,7 +29,7 @@ int batch_workers(vg_ctx *ctx, int n)
         ctx->workers.push_back(w);
     }
     for (vg_ctx *w : ctx->workers) {
-        w->graphs = ctx->graphs;
+        w->graphs = ctx->graphs && graphs;
         w->autotune = ctx->autotune;
         w->reduce_fields = ctx->reduce_fields;
         w->timing = false;
