@@ -535,6 +535,65 @@ __global__ __launch_bounds__(kLineBS) void rgfm_level_set_kernel(LineArgs a)
     }
 }
 
+// The y pass of the level-set update with coalesced rows.  A workgroup takes
+// kLsCols adjacent columns, each split into kLsSegs segments of consecutive
+// rows, one thread per column segment, so each row read covers kLsCols
+// adjacent doubles (one workgroup per column, as in rgfm_level_set_kernel<1>,
+// reads one double per row at a stride of nx).  Pass 1 counts each segment's
+// sign changes; their prefix over the column's segments gives the count before
+// the segment; pass 2 walks the segment again, counting on, and updates phi.
+// The flag is that count's parity, so flags, velocities and the upwind
+// arithmetic are those of rgfm_level_set_kernel<1>, bit for bit.
+constexpr int kLsCols = 8, kLsSegs = 64;
+
+__global__ __launch_bounds__(kLsCols * kLsSegs) void rgfm_level_set_y_kernel(LineArgs a)
+{
+    __shared__ int s_cnt[kLsSegs][kLsCols];
+    if (a.clk->done) return;
+    const double dt = a.clk->dt * a.dt_scale;
+    const int tid = threadIdx.x;
+    const int cl = tid % kLsCols, seg = tid / kLsCols;
+    const int col = (int)blockIdx.x * kLsCols + cl;
+    const int n = a.ny, nx = a.nx;
+    const bool ok = col < nx;
+    const int len = (n + kLsSegs - 1) / kLsSegs;
+    const int k0 = seg * len, k1 = (k0 + len < n) ? k0 + len : n;
+    int cnt = 0;
+    if (ok) {
+#pragma unroll 4
+        for (int k = k0; k < k1; k++) {
+            const double c = a.phi[(long long)k * nx + col];
+            const double pv = a.phi[(long long)(k > 0 ? k - 1 : 0) * nx + col];
+            cnt += ((pv * c) <= 0.0) ? 1 : 0;
+        }
+    }
+    s_cnt[seg][cl] = cnt;
+    __syncthreads();
+    int run = 0;
+    for (int s2 = 0; s2 < seg; s2++) run += s_cnt[s2][cl];
+    if (!ok) return;
+#pragma unroll 4
+    for (int k = k0; k < k1; k++) {
+        const long long cell = (long long)k * nx + col;
+        const double c = a.phi[cell];
+        const double pv = a.phi[(long long)(k > 0 ? k - 1 : 0) * nx + col];
+        run += ((pv * c) <= 0.0) ? 1 : 0;
+        const int f = a.init_flag ^ (run & 1);
+        const double v = (f ? a.m2 : a.m1)[2 * a.plane + cell];
+        double up;
+        if (v < 0.0) {
+            const double p2 = a.phi[(long long)ghost_index(k + 2, n, 0) * nx + col];
+            const double p1 = a.phi[(long long)ghost_index(k + 1, n, 0) * nx + col];
+            up = (-p2 + (4.0 * p1) - (3.0 * c)) / (2.0 * a.h);
+        } else {
+            const double m1v = a.phi[(long long)ghost_index(k - 1, n, 0) * nx + col];
+            const double m2v = a.phi[(long long)ghost_index(k - 2, n, 0) * nx + col];
+            up = ((3.0 * c) - (4.0 * m1v) + m2v) / (2.0 * a.h);
+        }
+        a.phi_out[cell] = c - (dt * (v * up));
+    }
+}
+
 // Interface circulation partial sums: (dv/dx - du/dy) dx dy over interior cells
 // with phi < 0, from material 1's velocity; one partial per block.
 static __global__ __launch_bounds__(256) void rgfm_circulation_kernel(const double *m1, long long plane, const double *phi,

@@ -324,7 +324,10 @@ struct RGFMGrid {
             a.dt_scale = 1.0;
             a.phi = phi[1 - pcur];
             a.phi_out = phi[pcur];
-            if (!rc) rc = ctx->timed("rgfm_level_set", [&] { rgfm_level_set_kernel<1><<<nx, kLineBS, 0, s>>>(a); });
+            if (!rc)
+                rc = ctx->timed("rgfm_level_set", [&] {
+                    rgfm_level_set_y_kernel<<<(nx + kLsCols - 1) / kLsCols, kLsCols * kLsSegs, 0, s>>>(a);
+                });
             a.h = dx;
             a.dt_scale = 0.5;
             a.phi = phi[pcur];
@@ -987,7 +990,24 @@ int vg_rgfm_solve_batch_2d(vg_ctx *ctx, int n_inst, const double *m1_in, const d
             cost[q] = {-t_final[q] * smax, q};
         }
         std::stable_sort(cost.begin(), cost.end());
-        rc = vg::batch_workers(ctx, nw);
+        // The y-sweep shape is tuned here, alone on the device, before the
+        // workers start (they take it from this context): timings taken while
+        // other streams run are disturbed.
+        const bool tuned = ctx->tuned.count(vg::tune_key(vg::EulerPhys::name, 2, mode, ny, nx)) &&
+                           ctx->tuned.count(vg::tune_key(vg::EulerPlanarPhys::name, 2, mode, ny, nx));
+        if (ctx->autotune && !tuned) {
+            vg::RGFMGrid g;
+            const int q = cost[0].second;
+            rc = g.create(ctx, 2, ny, nx);
+            if (!rc) rc = g.load(m1_in + 6 * cells * q, m2_in + 6 * cells * q, phi_in + cells * q);
+            if (!rc) rc = g.choose_physics();
+            if (!rc)
+                rc = g.reduced ? g.tune_as<vg::EulerPlanarPhys>(mode, dx, dy, bias, limiter, e1)
+                               : g.tune_as<vg::EulerPhys>(mode, dx, dy, bias, limiter, e1);
+            if (ctx->stream) hipStreamSynchronize(ctx->stream);
+            g.destroy();
+        }
+        if (!rc) rc = vg::batch_workers(ctx, nw);
         std::atomic<int> next{0};
         std::atomic<bool> stop{false};
         std::vector<int> wrc(nw, VG_OK);
