@@ -150,13 +150,14 @@ def test_batch_rejects_bad_offsets(vg_ctx):
 
 def test_mhd_batch_work_budget_gives_the_same_bits():
     """With more instances than resident workgroups (the MHD batch kernel's
-    55 KB of LDS allows two per CU, so at most 512 on 256 CUs), each instance
-    also stops at the launch's work budget (predicted cells x steps over the
-    resident workgroups, at least 1024 steps).  One 4096-cell instance of about
-    1230 steps among 1300 light ones stops at 1024 and finishes on the grid path:
+    27 KB of LDS allows five per CU, 1280 on 256 CUs), each instance also stops
+    at the launch's work budget (predicted cells x steps over the resident
+    workgroups, at least 1024 steps).  One 4096-cell instance of about 1200
+    steps among 3000 light ones stops at 1024 and finishes on the grid path:
     the bits and step count of solving it alone, on 1 or 4 continuation
     streams; the light instances match the oracle."""
-    prim, off, gamma, dx = problems.mhd_batch(1300, seed=7, n_min=32, n_max=32)
+    n_light = 3000
+    prim, off, gamma, dx = problems.mhd_batch(n_light, seed=7, n_min=32, n_max=32)
     n = 4096
     heavy = np.zeros((n, 9))
     left = np.arange(n) <= 0.5 * n
@@ -167,7 +168,7 @@ def test_mhd_batch_work_budget_gives_the_same_bits():
     off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
     gamma = np.concatenate([[1.4], gamma])
     dx = 1.0 / sizes.astype(np.float64)
-    tf = np.full(1301, 0.01)
+    tf = np.full(n_light + 1, 0.01)
     tf[0] = 0.06
     alone = S.mhd_solve(heavy, 1, float(dx[0]), 0.8, 0.06, limiter=1, params=S.MHDMaterialParameters(1.4))
     assert alone.stats.steps > 1024
@@ -176,7 +177,7 @@ def test_mhd_batch_work_budget_gives_the_same_bits():
         ctx = S.Context(0)
         try:
             ctx.set_batch_streams(streams)
-            print(f"1301 instances, {streams} continuation streams", flush=True)
+            print(f"{n_light + 1} instances, {streams} continuation streams", flush=True)
             r = S.mhd_solve_batch(prim, off, dx, gamma, tf, 0.8, limiter=1, ctx=ctx)
         finally:
             ctx.close()
@@ -187,7 +188,7 @@ def test_mhd_batch_work_budget_gives_the_same_bits():
         else:
             O.assert_bitwise(r.cells, base.cells, "1 vs 4 continuation streams")
             assert np.array_equal(r.steps, base.steps)
-    for q in (1, 650, 1300):
+    for q in (1, 1500, n_light):
         a, b = off[q], off[q + 1]
         ref, dts = O.mhd_solve(prim[a:b], order=2, dim=1, dx=float(dx[q]), cfl=0.8, tfinal=0.01, max_steps=0,
                                bias=0.0, limiter=1, gamma=float(gamma[q]))
