@@ -103,7 +103,9 @@ int vg_ctx_set_reduced_fields(vg_ctx *ctx, int enabled);
  * default 4; 1 runs them one after another on the context's stream).  The 2D
  * RGFM batch (vg_rgfm_solve_batch_2d) hands instances, longest predicted run
  * first, to that many worker streams, each replaying its own captured steps,
- * so one instance's kernels fill the others' launch gaps.  Per-kernel timing
+ * so one instance's kernels fill the others' launch gaps.  The 1D batches
+ * (vg_euler_solve_batch, vg_mhd_solve_batch) finish there the instances their
+ * single launch stopped at its step cap or work budget.  Per-kernel timing
  * (vg_ctx_set_timing) runs a batch on one stream.  Results are the same bits
  * for every count.  No reference counterpart. */
 int vg_ctx_set_batch_streams(vg_ctx *ctx, int streams);
