@@ -318,14 +318,15 @@ def bench_rgfm(args):
     tf = np.full(n_inst, args.t_final)
     ctx = S.default_context()
     ctx.set_batch_streams(args.batch_streams)
+    mode = S.FAST if args.mode == "fast" else S.EXACT   # "auto" keeps the reference's arithmetic here
     if args.warmup:
         S.rgfm_solve_batch_2d(m1[:1], m2[:1], phi[:1], dx, dy, 0.8, tf[:1], problems.HELIUM, problems.AIR,
-                              max_steps=args.warmup, ctx=ctx)
+                              max_steps=args.warmup, ctx=ctx, mode=mode)
     log(f"rgfm batch: {n_inst} instances of {nx}x{ny} on {args.batch_streams} streams ...")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     r = S.rgfm_solve_batch_2d(m1, m2, phi, dx, dy, 0.8, tf, problems.HELIUM, problems.AIR, ctx=ctx,
-                              max_steps=args.max_steps, raise_on_error=False)
+                              max_steps=args.max_steps, raise_on_error=False, mode=mode)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     log(f"rgfm batch: {int(r.steps.sum())} steps in {elapsed:.2f} s, status {r.status}")
@@ -334,7 +335,7 @@ def bench_rgfm(args):
     ctx.reset_timing()
     ctx.set_timing(True)
     S.rgfm_solve_batch_2d(m1[:1], m2[:1], phi[:1], dx, dy, 0.8, tf[:1], problems.HELIUM, problems.AIR, ctx=ctx,
-                          max_steps=min(int(r.steps[0]), 200), raise_on_error=False)
+                          max_steps=min(int(r.steps[0]), 200), raise_on_error=False, mode=mode)
     ctx.set_timing(False)
     kt = kernel_table(ctx, ["rgfm_sweep_x", "sweep_y_slic", "rgfm_fill_x", "rgfm_fill_y", "rgfm_level_set"])
     # the material the level set makes real must be finite and admissible in every cell
@@ -357,7 +358,8 @@ def bench_rgfm(args):
         "steps": int(r.steps.sum()), "warmup": args.warmup, "ms_per_step": round(elapsed * 1e3 / r.steps.sum(), 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"rgfm_shock_bubble_{n_inst}inst_{nx}x{ny}_hllc_t{args.t_final:g}", "grid": [nx, ny],
-                   "dx": dx, "dy": dy, "instances": n_inst, "riemann": "hllc"},
+                   "dx": dx, "dy": dy, "instances": n_inst, "riemann": "hllc",
+                   "mode": "fast" if mode == S.FAST else "exact"},
         "algorithmic_bytes_per_cell_update": 672,
         "roofline": {"bound": "hbm", "achieved": None if achieved is None else round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
