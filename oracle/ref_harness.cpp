@@ -26,6 +26,10 @@
 #include "Solvers/mhdforcingsolver.h"
 #include "HPR/hpracoustictensor.h"
 #include "RGFM/rgfmsolver.h"
+#include "Euler/eulertests.h"
+#include "MHD/mhdtests.h"
+#include "HPR/hprtests.h"
+#include "RGFM/eulerrgfmtests.h"
 
 #include <cstdio>
 #include <cstdlib>
@@ -904,6 +908,72 @@ int hllcUnits(int argc, char **argv)
     return 0;
 }
 
+// The reference's own .dat writers (EulerTests::outputSolution[2D],
+// eulertests.cpp:54-108; MHDTests, mhdtests.cpp:123-160; HPRTests,
+// hprtests.cpp:219-277; EulerRGFMTests, eulerrgfmtests.cpp:215-334), called on
+// a state read from disk; they write their files into the working directory.
+//   dat <euler|mhd|hpr|rgfm> <dim> <ny> <nx> <in> [<in2> <phi>] [hpr params x12]
+int datOut(int argc, char **argv)
+{
+    if (argc < 7) { std::fprintf(stderr, "dat: too few arguments\n"); return 2; }
+    const std::string kind = argv[2];
+    const int dim = std::atoi(argv[3]), ny = std::atoi(argv[4]), nx = std::atoi(argv[5]);
+    const size_t cells = (size_t)ny * nx;
+    if (kind == "euler" || kind == "mhd") {
+        const int nf = (kind == "euler") ? 6 : 9;
+        std::vector<double> in = readDoubles(argv[6], cells * nf);
+        if (kind == "euler") {
+            std::vector<std::vector<EulerStateVector> > g(ny, std::vector<EulerStateVector>(nx));
+            for (int i = 0; i < ny; i++)
+                for (int j = 0; j < nx; j++) g[i][j] = stateFrom(&in[((size_t)i * nx + j) * nf]);
+            if (dim == 1) EulerTests::outputSolution(g[0]);
+            else EulerTests::outputSolution2D(g);
+        } else {
+            std::vector<std::vector<MHDStateVector> > g(ny, std::vector<MHDStateVector>(nx));
+            for (int i = 0; i < ny; i++)
+                for (int j = 0; j < nx; j++) g[i][j] = mhdStateFrom(&in[((size_t)i * nx + j) * nf]);
+            if (dim == 1) MHDTests::outputSolution(g[0]);
+            else MHDTests::outputSolution2D(g);
+        }
+        return 0;
+    }
+    if (kind == "hpr") {
+        if (argc < 19) { std::fprintf(stderr, "dat hpr: too few arguments\n"); return 2; }
+        std::vector<double> in = readDoubles(argv[6], cells * 17);
+        HPRMaterialParameters mp = hprParams(argv + 7);
+        std::vector<std::vector<HPRStateVector> > g(ny, std::vector<HPRStateVector>(nx));
+        for (int i = 0; i < ny; i++)
+            for (int j = 0; j < nx; j++) g[i][j] = hprStateFrom(&in[((size_t)i * nx + j) * 17]);
+        if (dim == 1) HPRTests::outputSolution(g[0], mp);
+        else HPRTests::outputSolution2D(g, mp);
+        return 0;
+    }
+    if (kind == "rgfm") {
+        if (argc < 9) { std::fprintf(stderr, "dat rgfm: too few arguments\n"); return 2; }
+        std::vector<double> in1 = readDoubles(argv[6], cells * 6), in2 = readDoubles(argv[7], cells * 6);
+        std::vector<double> phi = readDoubles(argv[8], cells);
+        if (dim == 1) {
+            std::vector<EulerStateVector> m1(nx), m2(nx);
+            for (int j = 0; j < nx; j++) { m1[j] = stateFrom(&in1[(size_t)j * 6]); m2[j] = stateFrom(&in2[(size_t)j * 6]); }
+            EulerRGFMTests::outputSolution(MultimaterialSystem(m1, m2, phi));
+        } else {
+            std::vector<std::vector<EulerStateVector> > m1(ny, std::vector<EulerStateVector>(nx)), m2 = m1;
+            std::vector<std::vector<double> > ph(ny, std::vector<double>(nx));
+            for (int i = 0; i < ny; i++)
+                for (int j = 0; j < nx; j++) {
+                    const size_t c = (size_t)i * nx + j;
+                    m1[i][j] = stateFrom(&in1[c * 6]);
+                    m2[i][j] = stateFrom(&in2[c * 6]);
+                    ph[i][j] = phi[c];
+                }
+            EulerRGFMTests::outputSolution2D(MultimaterialSystem(m1, m2, ph));
+        }
+        return 0;
+    }
+    std::fprintf(stderr, "dat: unknown kind %s\n", kind.c_str());
+    return 2;
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) { std::fprintf(stderr, "usage: ref_harness <mode> ...\n"); return 2; }
@@ -917,6 +987,7 @@ int main(int argc, char **argv)
     if (mode == "hpr_units") return hprUnits(argc, argv);
     if (mode == "rgfm_run") return rgfmRun(argc, argv);
     if (mode == "hllc_units") return hllcUnits(argc, argv);
+    if (mode == "dat") return datOut(argc, argv);
     std::fprintf(stderr, "unknown mode %s\n", mode.c_str());
     return 2;
 }
