@@ -29,7 +29,9 @@ enum {
     VG_ERR_ARG = -1,        /* invalid argument (sizes, pointers, options) */
     VG_ERR_HIP = -2,        /* HIP runtime failure */
     VG_ERR_NONFINITE = -3,  /* the final state holds NaN/Inf or inadmissible cells (Euler: rho <= 0 or
-                               p + p_inf <= 0); outputs are still written */
+                               p + p_inf <= 0; RGFM: in the material the level set makes real in
+                               each cell, or a non-finite level set), or a time step could not
+                               advance t; outputs are still written */
     VG_ERR_UNSUPPORTED = -4 /* option combination not implemented */
 };
 
