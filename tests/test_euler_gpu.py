@@ -194,3 +194,51 @@ def test_reduced_field_sweeps_give_the_full_state_bits(vg_ctx, dim, order, mode)
             O.assert_bitwise(c.cells, ref, "state with a reacting cell vs the oracle")
     finally:
         full.close()
+
+
+def _write_plane(ptr, plane, f, values):
+    """Host values into plane f of a grid's device planes (synchronous copy)."""
+    import ctypes
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    src = np.ascontiguousarray(values, dtype=np.float64).ravel()
+    dst = ctypes.c_void_p(ptr + 8 * f * plane)
+    rc = hip.hipMemcpy(dst, src.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(src.nbytes), 1)
+    assert rc == 0, rc
+
+
+def test_planar_check_is_redone_after_a_load_or_a_planes_handout():
+    """A grid remembers that its constant planes hold w = +0 and lambda = 1 and
+    skips the check on later runs.  Loading a state with w != 0, or writing w
+    through the planes pointer between runs, must send the next run back to
+    the 6-field sweeps: the same bits as a context that never takes the 4-field
+    ones."""
+    n, dx = 64, 2.0 / 64
+    w = problems.disc_2d(n)
+    w2 = w.copy()
+    w2[..., 3] = 0.05 * np.sin(2 * np.pi * (np.arange(n) + 0.5) / n)[None, :]
+    full = S.Context()
+    try:
+        full.set_reduced_fields(False)
+        grids = [S.EulerGrid(2, n, n), S.EulerGrid(2, n, n, ctx=full)]
+        out = []
+        for g in grids:
+            g.load(w)
+            g.run(dx, 0.8, max_steps=3)
+            g.load(w2)
+            g.run(dx, 0.8, max_steps=3)
+            first = g.store()
+            g.load(w)
+            g.run(dx, 0.8, max_steps=3)
+            ptr, plane = g.planes()
+            _write_plane(ptr, plane, 3, w2[..., 3])
+            g.run(dx, 0.8, max_steps=3)
+            out.append((first, g.store()))
+            g.close()
+        O.assert_bitwise(out[0][0], out[1][0], "after loading a state with w != 0")
+        O.assert_bitwise(out[0][1], out[1][1], "after writing w through the planes pointer")
+        assert np.any(out[0][1][..., 3] != 0.0)
+    finally:
+        full.close()

@@ -182,6 +182,7 @@ int vg_euler_state_planes(vg_euler_state *st, double **planes, int64_t *plane)
     VG_CHECK(st != nullptr && planes != nullptr && plane != nullptr, "null argument");
     *planes = st->g.buf[st->g.cur];
     *plane = st->g.plane;
+    st->g.planar_known = false;   // the caller may write any plane
     return VG_OK;
 }
 

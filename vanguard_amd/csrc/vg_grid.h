@@ -173,6 +173,7 @@ struct GridState {
     int last_bc = -1;
     int y_variant = 0;             // y-sweep shape (autotuned per grid shape)
     bool reduced = false;          // this run sweeps with ReducedPhys<P> (constant planes checked)
+    bool planar_known = false;     // both buffers hold the constant planes (checked since the last load)
     int *flag = nullptr;           // device flag of the constant-plane check (lazy)
     Clock *tune_clk = nullptr;     // scratch clock for tuning launches (lazy)
     hipGraphExec_t graph = nullptr;  // captured chunk of steps
@@ -237,6 +238,7 @@ struct GridState {
         int rc = launch_status("aos_to_soa");
         if (rc) return rc;
         cur = 0;
+        planar_known = false;
         Clock h{};
         VG_HIP(hipMemcpyAsync(clk, &h, sizeof(Clock), hipMemcpyHostToDevice, ctx->stream));
         VG_HIP(hipStreamSynchronize(ctx->stream));
@@ -351,6 +353,13 @@ struct GridState {
         using R = typename ReducedPhys<P>::type;
         if constexpr (!std::is_void_v<R>) {
             if (!ctx->reduce_fields) return VG_OK;
+            // Once checked, the planes stay constant until the grid is loaded
+            // again or its planes are handed out: the reduced sweeps leave them
+            // alone and the full sweeps keep them at the same bits.
+            if (planar_known) {
+                reduced = true;
+                return VG_OK;
+            }
             if (!flag) VG_HIP(hipMalloc(&flag, sizeof(int)));
             const int one = 1;
             VG_HIP(hipMemcpyAsync(flag, &one, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
@@ -368,6 +377,7 @@ struct GridState {
                                       hipMemcpyDeviceToDevice, ctx->stream));
             }
             reduced = true;
+            planar_known = true;
         }
         return VG_OK;
     }

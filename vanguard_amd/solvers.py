@@ -340,6 +340,15 @@ class EulerGrid:
               "vg_euler_state_max_wave_speed")
         return s.value
 
+    def planes(self) -> tuple[int, int]:
+        """vg_euler_state_planes: the device address of the current
+        structure-of-arrays planes (field f of cell c at element f * plane + c)
+        and the plane stride in doubles."""
+        ptr = C.c_void_p()
+        plane = C.c_int64()
+        check(self.lib.vg_euler_state_planes(self.handle, C.byref(ptr), C.byref(plane)), "vg_euler_state_planes")
+        return ptr.value, plane.value
+
 
 # ---- batched 1D solves ---------------------------------------------------------
 
