@@ -39,8 +39,12 @@ enum { VG_BC_TRANSMISSIVE = 0, VG_BC_PERIODIC = 1 };
 
 /* VG_MODE_EXACT: the reference's operation order, bit-identical to its CPU
  * solver.  VG_MODE_FAST: algebraically reduced arithmetic (reciprocals reused,
- * no primitive/conserved round trips, multiply-adds fused), same scheme; agrees
- * with EXACT to about 1e-13 relative and is not bit-identical. */
+ * no primitive/conserved round trips, multiply-adds fused), same scheme; not
+ * bit-identical.  Over runs of hundreds of steps it agrees with EXACT to about
+ * 1e-13 relative; over thousands, a last-bit difference can flip a limiter
+ * branch or a level-set decision and grow (1e-3 relative and more seen on
+ * batched MHD and RGFM runs), so use EXACT where the reference's results
+ * matter. */
 enum { VG_MODE_EXACT = 0, VG_MODE_FAST = 1 };
 
 typedef struct vg_ctx vg_ctx;
