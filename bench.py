@@ -1,6 +1,6 @@
 """Benchmark: FP64 SLIC 2D Euler on a 4096 x 4096 grid (one MI355X).
 
-Workload: ideal gas gamma 1.4, dimensionally split SLIC
+Workload (the 2D Euler headline configuration): ideal gas gamma 1.4, dimensionally split SLIC
 with the superbee limiter, 4096^2 cells on [0,2]^2, transmissive boundaries,
 cylindrical explosion (radius 0.4) plus 64 seeded Gaussian density bumps
 (amplitude 1e-3, width 0.02), CFL 0.8, 200 steps, FP64.

@@ -2,7 +2,7 @@
 
 EXACT mode must be bit-identical to the golden vectors of the compiled
 reference and to the oracle (time steps and final states); FAST mode must agree
-within 1e-10 relative L1 (BASELINE's parity bar for conserved fields).
+within 1e-10 relative L1 (the parity bar for conserved fields in FAST mode).
 """
 
 import math
