@@ -501,7 +501,7 @@ JOBS = {
     "mhd_batch12": lambda: mhd_batch_fixture("mhd_batch12", 12, 3, 64, 256, 0.02),
     "mhd_units": lambda: mhd_units("mhd_units", 500, 21, 0.0, 1, 2.0, 1.7),
 
-    # config C1 exactly (1000 cells, minbee, CFL 0.9, T 0.25); also checks the
+    # the Sod benchmark exactly (1000 cells, minbee, CFL 0.9, T 0.25); also checks the
     # reference's own SecondOrderSolver::solve against the mirror loop.
     "euler_c1_sod1000": lambda: euler_run("euler_c1_sod1000", problems.sod_1d(1000), order=2, dim=1, dx=1e-3,
                                           cfl=0.9, tfinal=0.25, max_steps=0, bias=0.0, limiter=2,

@@ -42,7 +42,7 @@ def test_mhd_batch_bit_identical_to_reference(vg_ctx):
 
 def test_mhd_batch_uneven_sizes_match_oracle(vg_ctx):
     """Sizes from 2 cells to a few thousand, mixed in one launch, in the
-    config-3 recipe; more instances than one wave of workgroups."""
+    batched-MHD benchmark recipe; more instances than one wave of workgroups."""
     prim, off, gamma, dx = problems.mhd_batch(600, seed=11, n_min=2, n_max=3000)
     tf = np.full(600, 2e-3)
     r = S.mhd_solve_batch(prim, off, dx, gamma, tf, 0.8, limiter=1)

@@ -108,7 +108,7 @@ def brio_wu(n: int) -> np.ndarray:
 
 
 def mhd_batch(n_inst: int, seed: int = 3, n_min: int = 256, n_max: int = 16384):
-    """Independent 1D ideal-MHD Riemann problems (benchmark config 3 recipe).
+    """Independent 1D ideal-MHD Riemann problems (the batched-MHD benchmark recipe).
 
     Per instance, 17 draws u_k in [0, 1) from splitmix64(seed) in this order:
     N = n_min + floor(u (n_max - n_min + 1)); gamma = 1.3 + 0.7 u; Bx = 2u - 1
