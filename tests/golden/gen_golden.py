@@ -525,7 +525,7 @@ JOBS = {
     "euler_2d_disc64_l2": lambda: euler_run("euler_2d_disc64_l2", problems.disc_2d(64, gaussians=0, length=1.0),
                                             order=2, dim=2, dx=1.0 / 64, cfl=0.8, tfinal=np.inf, max_steps=120,
                                             bias=0.0, limiter=2, params=IDEAL, check_api=False),
-    # config C2's initial condition (disc + 64 Gaussians, [0,2]^2) at 64^2.
+    # the 2D Euler headline's initial condition (disc + 64 Gaussians, [0,2]^2) at 64^2.
     "euler_2d_c2ic64": lambda: euler_run("euler_2d_c2ic64", problems.disc_2d(64), order=2, dim=2, dx=2.0 / 64,
                                          cfl=0.8, tfinal=np.inf, max_steps=200, bias=0.0, limiter=0,
                                          params=IDEAL, check_api=False),
