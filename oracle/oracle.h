@@ -121,11 +121,12 @@ void orc_rgfm_ghost_fill(int riemann, int dim, int ny, int nx, const orc_euler_p
 /* updateLevelSetFunction (dim 1) / update2DLevelSetFunctionX (dir 0) / Y (dir 1). */
 void orc_rgfm_level_set(int dim, int dir, int ny, int nx, double h, double dt, const double *m1, const double *m2,
                         const double *phi_in, double *phi_out);
-/* HLLC records (12 doubles -> 24) and exact-solver records (12 -> 10). */
+/* HLLC records (12 doubles -> 24) and exact-solver records (12 -> 10: p*, u*,
+ * rho*_a, rho*_b and the state sampled at x/t = s in direction dir). */
 void orc_hllc_units(int n, const orc_euler_params *p1, const orc_euler_params *p2, const double *states,
                     double *out);
-void orc_exact_units(int n, double s, const orc_euler_params *p1, const orc_euler_params *p2, const double *states,
-                     double *out);
+void orc_exact_units(int n, int dir, double s, const orc_euler_params *p1, const orc_euler_params *p2,
+                     const double *states, double *out);
 /* HLLC RGFM solve with prescribed time steps and (t1_in not NULL) the
  * forward-mode derivative of both final material states along (t1_in, t2_in),
  * by dual numbers (euler_dual.cpp); the level set is not differentiated.

@@ -533,17 +533,17 @@ void orc_hllc_units(int n, const orc_euler_params *p1, const orc_euler_params *p
 
 /* Exact-solver records: two states -> p*, u*, rho*_left, rho*_right, and the
  * sampled state at x/t = s (6). */
-void orc_exact_units(int n, double s, const orc_euler_params *p1, const orc_euler_params *p2, const double *states,
-                     double *out)
+void orc_exact_units(int n, int dir, double s, const orc_euler_params *p1, const orc_euler_params *p2,
+                     const double *states, double *out)
 {
     for (int r = 0; r < n; r++) {
         const double *a = states + 12 * (size_t)r, *b = a + 6;
         double *o = out + 10 * (size_t)r;
-        o[0] = orc_exact_star_pres(0, a, b, p1, p2);
-        o[1] = orc_exact_star_vel(0, o[0], a, b, p1, p2);
+        o[0] = orc_exact_star_pres(dir, a, b, p1, p2);
+        o[1] = orc_exact_star_vel(dir, o[0], a, b, p1, p2);
         o[2] = orc_exact_star_rho(o[0], a, p1);
         o[3] = orc_exact_star_rho(o[0], b, p2);
-        orc_exact_solve(0, s, a, b, p1, p2, o + 4);
+        orc_exact_solve(dir, s, a, b, p1, p2, o + 4);
     }
 }
 

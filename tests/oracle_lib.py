@@ -73,7 +73,7 @@ def load():
     lib.orc_hllc_units.restype = None
     lib.orc_hllc_units.argtypes = [C.c_int, P, P, dptr, dptr]
     lib.orc_exact_units.restype = None
-    lib.orc_exact_units.argtypes = [C.c_int, C.c_double, P, P, dptr, dptr]
+    lib.orc_exact_units.argtypes = [C.c_int, C.c_int, C.c_double, P, P, dptr, dptr]
     lib.orc_euler_fixed_dt_jvp.restype = None
     lib.orc_euler_fixed_dt_jvp.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, dptr, C.c_int, C.c_double, C.c_int,
                                            C.c_int, C.POINTER(OrcEulerParams), dptr, dptr, dptr, dptr]
@@ -119,12 +119,12 @@ def hllc_units(states, p1, p2):
     return out
 
 
-def exact_units(states, p1, p2, s=0.0):
+def exact_units(states, p1, p2, s=0.0, direction=0):
     lib = load()
     st = np.ascontiguousarray(states, dtype=np.float64)
     out = np.empty((st.shape[0], 10))
     q1, q2 = OrcEulerParams(*p1), OrcEulerParams(*p2)
-    lib.orc_exact_units(st.shape[0], s, C.byref(q1), C.byref(q2), _p(st), _p(out))
+    lib.orc_exact_units(st.shape[0], direction, s, C.byref(q1), C.byref(q2), _p(st), _p(out))
     return out
 
 
