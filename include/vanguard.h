@@ -162,7 +162,11 @@ int vg_euler_state_load(vg_euler_state *st, const double *prim, int on_device);
 /* Writes the current grid as array-of-structures primitives. */
 int vg_euler_state_store(vg_euler_state *st, double *prim, int on_device);
 /* Device pointer of the current structure-of-arrays grid: 6 planes of
- * ny*nx doubles at a stride of *plane doubles. */
+ * ny*nx doubles at a stride of *plane doubles.  The pointer, and writes through
+ * it, are valid only until the next vg_euler_state_run, _sweep or _load on this
+ * state (the grid alternates between two buffers): call this again before each
+ * write.  Each call marks the planes as possibly modified, so the next run
+ * checks again whether the 4-field sweeps apply (w = +0, lambda = 1). */
 int vg_euler_state_planes(vg_euler_state *st, double **planes, int64_t *plane);
 /* Advances up to max_steps steps (0 = until t >= t_final) continuing the clock. */
 int vg_euler_state_run(vg_euler_state *st, int order, double dx, double cfl, double t_final, int max_steps,
@@ -172,6 +176,29 @@ int vg_euler_state_run(vg_euler_state *st, int order, double dx, double cfl, dou
  * ghosted grid (2D transmissive: default-state corners included); NaN speeds
  * are skipped as in the reference.  Order-independent, hence deterministic. */
 int vg_euler_state_max_wave_speed(vg_euler_state *st, int bc, const vg_euler_params *mp, double *smax);
+/* One sweep of the device-resident grid with the caller's time step dt:
+ * SecondOrderSolver::computeSLICTimeStep (1D, secondordersolver.h:32,
+ * secondordersolver.cpp:80-98) or computeX|YSLICTimeStep2D (dir 0 / 1,
+ * secondordersolver.h:39-48, secondordersolver.cpp:140-162, 214-236) when
+ * order == 2; FirstOrderSolver::computeFORCETimeStep / computeX|YFORCETimeStep2D
+ * (firstordersolver.h:63-81, firstordersolver.cpp:232-378) when order == 1.
+ * Ghost cells come from `bc` (transmissive: the reference's insertBoundaryCells
+ * mirror); the clock of vg_euler_state_run is not advanced. */
+int vg_euler_state_sweep(vg_euler_state *st, int dir, int order, double dx, double dt, double bias, int limiter, int bc,
+                         const vg_euler_params *mp, int mode);
+
+/* The same per-sweep calls on host arrays, as the reference takes them:
+ * `cells_with_boundary` is the ghosted snapshot the caller built (read as
+ * given, corners included): n + 2G cells (1D) or (ny + 2G) x (nx + 2G) (2D),
+ * G = 2 for SLIC (order 2), 1 for FORCE (order 1), 6 doubles per cell;
+ * `cells` (n or ny x nx cells) receives the updated states, as the reference's
+ * currentCells.  dir selects computeXSLICTimeStep2D (0) or computeYSLICTimeStep2D
+ * (1).  Synchronous. */
+int vg_euler_time_step_1d(vg_ctx *ctx, double *cells, const double *cells_with_boundary, int n, double dx, double dt,
+                          double bias, int limiter, int order, const vg_euler_params *mp, int mode);
+int vg_euler_time_step_2d(vg_ctx *ctx, double *cells, const double *cells_with_boundary, int ny, int nx, int dir,
+                          double dx, double dt, double bias, int limiter, int order, const vg_euler_params *mp,
+                          int mode);
 
 /* Batched 1D Euler solves in one persistent launch: n_inst independent
  * SecondOrderSolver::solve (order 2) / FirstOrderSolver::solve (order 1) calls
@@ -271,6 +298,17 @@ int vg_mhd_solve_batch(vg_ctx *ctx, int n_inst, const int64_t *offsets, const do
                        const double *dx, const vg_mhd_params *mp, const double *t_final, double cfl, int max_steps,
                        double bias, int limiter, int order, int mode, int32_t *steps, double *t_out,
                        vg_run_stats *stats);
+
+/* MHDSecondOrderSolver::computeSLICTimeStep / computeX|YSLICTimeStep2D
+ * (mhdsecondordersolver.h:33-56, mhdsecondordersolver.cpp:91-108) when order
+ * == 2 and MHDFirstOrderSolver's FORCE twins when order == 1, on host arrays as
+ * vg_euler_time_step_*: 9 doubles per cell.  The GLM constants c_h, c_p are
+ * used as given (the reference's solve sets them per step before calling these,
+ * mhdsecondordersolver.cpp:330-332). */
+int vg_mhd_time_step_1d(vg_ctx *ctx, double *cells, const double *cells_with_boundary, int n, double dx, double dt,
+                        double bias, int limiter, int order, const vg_mhd_params *mp, int mode);
+int vg_mhd_time_step_2d(vg_ctx *ctx, double *cells, const double *cells_with_boundary, int ny, int nx, int dir,
+                        double dx, double dt, double bias, int limiter, int order, const vg_mhd_params *mp, int mode);
 
 /* ---- two-material Euler: real ghost fluid method (RGFM) ----------------- */
 
@@ -405,6 +443,18 @@ int vg_hpr_state_run(vg_hpr_state *st, int order, double dx, double cfl, double 
 /* Maximum wave speed (HPRSolvers::computeMaximumWaveSpeed[2D], hprsolvers.cpp:286-303,
  * 362-385) over the ghosted grid. */
 int vg_hpr_state_max_wave_speed(vg_hpr_state *st, int bc, const vg_hpr_params *mp, double *smax);
+/* One sweep with the caller's time step (HPRSecondOrderSolver::computeSLICTimeStep /
+ * computeX|YSLICTimeStep2D, hprsecondordersolver.h:33-54, hprsecondordersolver.cpp:89-241,
+ * when order == 2; HPRFirstOrderSolver's FORCE twins when order == 1), as
+ * vg_euler_state_sweep; no relaxation source. */
+int vg_hpr_state_sweep(vg_hpr_state *st, int dir, int order, double dx, double dt, double bias, int limiter, int bc,
+                       const vg_hpr_params *mp);
+/* The same on host arrays with the caller's ghosted snapshot, as
+ * vg_euler_time_step_*: 17 doubles per cell. */
+int vg_hpr_time_step_1d(vg_ctx *ctx, double *cells, const double *cells_with_boundary, int n, double dx, double dt,
+                        double bias, int limiter, int order, const vg_hpr_params *mp);
+int vg_hpr_time_step_2d(vg_ctx *ctx, double *cells, const double *cells_with_boundary, int ny, int nx, int dir,
+                        double dx, double dt, double bias, int limiter, int order, const vg_hpr_params *mp);
 
 /* Reverse-mode derivative of the second-order HPR solve (as vg_hpr_solve_*
  * with order 2, EXACT arithmetic; stiffened gas, not plastic) with the time

@@ -196,6 +196,18 @@ int orc_mhd_solve(int order, int dim, int ny, int nx, double dx, double cfl, dou
                          prim_out, dt_log, dt_cap);
 }
 
+/* One sweep with the parameters as given (the per-sweep calls
+ * MHDSecondOrderSolver::computeSLICTimeStep / computeX|YSLICTimeStep2D,
+ * mhdsecondordersolver.cpp:91-108, and the FORCE twins: c_h and c_p are the
+ * caller's, not the step's maximum). */
+void orc_mhd_sweep(int order, int dim, int dir, int ny, int nx, double dx, double ts, double bias, int limiter, int bc,
+                   const orc_mhd_params *mp, const double *prim_in, double *prim_out)
+{
+    orc_mhd_params local = *mp;
+    orc_phys ph = mhd_phys(&local);
+    orc_sweep(&ph, order, dim, dir, ny, nx, dx, ts, bias, limiter, bc, prim_in, prim_out);
+}
+
 void orc_mhd_units(int n, double bias, int limiter, double dx, double ts, const orc_mhd_params *mp,
                    const double *states, double *out)
 {

@@ -76,6 +76,8 @@ int orc_mhd_solve(int order, int dim, int ny, int nx, double dx, double cfl, dou
                   double bias, int limiter, int bc, int subcycles, const orc_mhd_params *mp, const double *prim_in,
                   double *prim_out, double *dt_log, int dt_cap);
 /* Per-function records of five input states (45 doubles) -> 85 doubles. */
+void orc_mhd_sweep(int order, int dim, int dir, int ny, int nx, double dx, double ts, double bias, int limiter, int bc,
+                   const orc_mhd_params *mp, const double *prim_in, double *prim_out);
 void orc_mhd_units(int n, double bias, int limiter, double dx, double ts, const orc_mhd_params *mp,
                    const double *states, double *out);
 

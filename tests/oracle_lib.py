@@ -41,6 +41,9 @@ def load():
                                     dptr, dptr]
     lib.orc_limiter_units.restype = None
     lib.orc_limiter_units.argtypes = [C.c_int, C.c_double, dptr, dptr]
+    lib.orc_mhd_sweep.restype = None
+    lib.orc_mhd_sweep.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                  C.c_int, C.c_int, C.POINTER(OrcMHDParams), dptr, dptr]
     lib.orc_euler_sweep.restype = None
     lib.orc_euler_sweep.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                     C.c_int, C.c_int, C.POINTER(OrcEulerParams), dptr, dptr]
@@ -238,6 +241,28 @@ def mhd_solve(prim, *, order, dim, dx, cfl, tfinal, max_steps, bias, limiter, ga
     n = lib.orc_mhd_solve(order, dim, ny, nx, dx, cfl, tfinal, max_steps, bias, limiter, bc, subcycles, C.byref(mp),
                           _p(w), _p(out), _p(dts), dt_cap)
     return out, dts[:min(n, dt_cap)].copy()
+
+
+def mhd_sweep(prim, *, order, dim, direction, dx, ts, bias, limiter, gamma, p_inf, ch, cp, bc=0):
+    """One MHD sweep with c_h, c_p as given (AoS in, AoS out)."""
+    lib = load()
+    w = np.ascontiguousarray(prim, dtype=np.float64)
+    out = np.empty_like(w)
+    ny, nx = (1, w.shape[0]) if dim == 1 else w.shape[:2]
+    mp = OrcMHDParams(gamma, p_inf, ch, cp)
+    lib.orc_mhd_sweep(order, dim, direction, ny, nx, dx, ts, bias, limiter, bc, C.byref(mp), _p(w), _p(out))
+    return out
+
+
+def euler_sweep(prim, *, order, dim, direction, dx, ts, bias, limiter, params, bc=0):
+    """One Euler sweep (AoS in, AoS out)."""
+    lib = load()
+    w = np.ascontiguousarray(prim, dtype=np.float64)
+    out = np.empty_like(w)
+    ny, nx = (1, w.shape[0]) if dim == 1 else w.shape[:2]
+    mp = OrcEulerParams(*params)
+    lib.orc_euler_sweep(order, dim, direction, ny, nx, dx, ts, bias, limiter, bc, C.byref(mp), _p(w), _p(out))
+    return out
 
 
 def mhd_units(states, *, bias, limiter, dx, ts, gamma, p_inf, ch):

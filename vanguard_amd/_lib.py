@@ -108,6 +108,16 @@ SIGNATURES = {
                                      C.c_int, C.c_int, C.POINTER(EulerParams), C.c_int, dptr, C.c_int,
                                      C.POINTER(RunStats)]),
     "vg_euler_state_max_wave_speed": (C.c_int, [vptr, C.c_int, C.POINTER(EulerParams), dptr]),
+    "vg_euler_state_sweep": (C.c_int, [vptr, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                       C.POINTER(EulerParams), C.c_int]),
+    "vg_euler_time_step_1d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                        C.c_int, C.POINTER(EulerParams), C.c_int]),
+    "vg_euler_time_step_2d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                        C.c_double, C.c_int, C.c_int, C.POINTER(EulerParams), C.c_int]),
+    "vg_mhd_time_step_1d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                      C.c_int, C.POINTER(MHDParams), C.c_int]),
+    "vg_mhd_time_step_2d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                      C.c_double, C.c_int, C.c_int, C.POINTER(MHDParams), C.c_int]),
     "vg_euler_solve_batch": (C.c_int, [vptr, C.c_int, C.POINTER(C.c_int64), dptr, dptr, dptr,
                                        C.POINTER(EulerParams), dptr, C.c_double, C.c_int, C.c_double, C.c_int,
                                        C.c_int, C.c_int, C.POINTER(C.c_int32), dptr, C.POINTER(RunStats)]),
@@ -156,6 +166,12 @@ SIGNATURES = {
     "vg_hpr_state_load": (C.c_int, [vptr, vptr, C.c_int]),
     "vg_hpr_state_store": (C.c_int, [vptr, vptr, C.c_int]),
     "vg_hpr_state_planes": (C.c_int, [vptr, C.POINTER(vptr), C.POINTER(C.c_int64)]),
+    "vg_hpr_state_sweep": (C.c_int, [vptr, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                     C.POINTER(HPRParams)]),
+    "vg_hpr_time_step_1d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                      C.c_int, C.POINTER(HPRParams)]),
+    "vg_hpr_time_step_2d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                      C.c_double, C.c_int, C.c_int, C.POINTER(HPRParams)]),
     "vg_hpr_state_run": (C.c_int, [vptr, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int,
                                    C.c_int, C.c_int, C.POINTER(HPRParams), C.c_int, dptr, C.c_int,
                                    C.POINTER(RunStats)]),

@@ -247,6 +247,26 @@ __device__ __forceinline__ void opaque(double (&v)[K])
 // WPE is the occupancy the register allocation must allow (waves per SIMD): 2
 // keeps every value in registers (about 234 live at the peak), 3 spills some
 // to scratch for half as many waves again in flight.  The arithmetic is the same.
+// The state a position is linearised at.  A forward state that is not finite
+// (or has rho <= 0) influences no finite result downstream -- every output that
+// reads it is itself non-finite -- so its adjoint is zero in exact arithmetic;
+// IEEE arithmetic would still form 0 x NaN = NaN in the products of its
+// Jacobian and pass NaN to its finite neighbours.  Such a position is
+// linearised at the default state (1, 0, 0, 0, 1, 1) instead, which makes its
+// zero contributions exact zeros.  Finite states with rho > 0 -- every state of
+// a solve whose grid stays admissible, such as the plain Euler adjoint's -- are
+// left as they are.  In the two-material adjoint such states arise in a
+// material's grid far from the region the level set makes real.
+__device__ __forceinline__ void linearisation_state(double (&w)[6])
+{
+    bool ok = w[0] > 0.0;
+#pragma unroll
+    for (int f = 0; f < 6; f++) ok = ok && isfinite(w[f]);
+    if (!ok) {
+        w[0] = 1.0; w[1] = 0.0; w[2] = 0.0; w[3] = 0.0; w[4] = 1.0; w[5] = 1.0;
+    }
+}
+
 template <int DIR, int BS, int LW, int WPE>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void euler_adjoint_sweep_kernel(
     AdjointArgs a)
@@ -275,6 +295,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         const long long c = adj_cell<DIR>(line, k, a.nx);
 #pragma unroll
         for (int f = 0; f < NF; f++) w[f] = a.w[f * a.plane + c];
+        linearisation_state(w);
         double U[NF];
         EulerPhys::prim_to_cons(m, w, U);
 #pragma unroll
@@ -449,6 +470,20 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void 
             for (int f = 0; f < NF; f++) dst[f] = wb[f];
         }
     }
+}
+
+// Marks the latest reverse step (forward step index) after which an adjoint
+// field holds a non-finite value: *first = max(*first, step) when any field of
+// any cell is not finite.
+static __global__ __launch_bounds__(256) void adjoint_nonfinite_kernel(const double *__restrict__ a, long long plane,
+                                                                       long long cells, int step, int *first)
+{
+    bool bad = false;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int f = 0; f < 6; f++) bad = bad || !isfinite(a[f * plane + c]);
+    }
+    if (bad) atomicMax(first, step);
 }
 
 // Adds the ghost positions' adjoint to the cells they copy: transmissive

@@ -218,6 +218,48 @@ int vg_euler_state_max_wave_speed(vg_euler_state *st, int bc, const vg_euler_par
     return VG_OK;
 }
 
+int vg_euler_state_sweep(vg_euler_state *st, int dir, int order, double dx, double dt, double bias, int limiter, int bc,
+                         const vg_euler_params *mp, int mode)
+{
+    vg::clear_error();
+    VG_CHECK(st != nullptr && mp != nullptr, "null argument");
+    VG_CHECK(order == 1 || order == 2, "order must be 1 (FORCE) or 2 (SLIC)");
+    VG_CHECK(mode == VG_MODE_EXACT || mode == VG_MODE_FAST, "mode must be VG_MODE_EXACT or VG_MODE_FAST");
+    VG_CHECK(bc == VG_BC_TRANSMISSIVE || bc == VG_BC_PERIODIC, "unknown boundary condition");
+    VG_CHECK(dir == 0 || (dir == 1 && st->g.dim == 2), "dir must be 0 (x) or 1 (y; 2D only)");
+    VG_CHECK(dx > 0.0 && std::isfinite(dx) && std::isfinite(dt), "cell spacing must be positive, dt finite");
+    int rc = vg::bind(st->g.ctx);
+    if (rc) return rc;
+    return st->g.sweep_once(order, mode, dir, dt, bc, dx, bias, limiter, to_params(mp));
+}
+
+int vg_euler_time_step_1d(vg_ctx *ctx, double *cells, const double *cells_with_boundary, int n, double dx, double dt,
+                          double bias, int limiter, int order, const vg_euler_params *mp, int mode)
+{
+    vg::clear_error();
+    VG_CHECK(ctx != nullptr, "null context");
+    int rc = vg::check_time_step_args(1, 1, n, 0, dx, dt, order, mode, cells, cells_with_boundary, mp);
+    if (rc) return rc;
+    rc = vg::bind(ctx);
+    if (rc) return rc;
+    return vg::ghosted_time_step<vg::EulerPhys>(ctx, 1, cells, cells_with_boundary, 1, n, 0, dx, dt, bias, limiter,
+                                                order, mode, to_params(mp));
+}
+
+int vg_euler_time_step_2d(vg_ctx *ctx, double *cells, const double *cells_with_boundary, int ny, int nx, int dir,
+                          double dx, double dt, double bias, int limiter, int order, const vg_euler_params *mp,
+                          int mode)
+{
+    vg::clear_error();
+    VG_CHECK(ctx != nullptr, "null context");
+    int rc = vg::check_time_step_args(2, ny, nx, dir, dx, dt, order, mode, cells, cells_with_boundary, mp);
+    if (rc) return rc;
+    rc = vg::bind(ctx);
+    if (rc) return rc;
+    return vg::ghosted_time_step<vg::EulerPhys>(ctx, 2, cells, cells_with_boundary, ny, nx, dir, dx, dt, bias, limiter,
+                                                order, mode, to_params(mp));
+}
+
 }  // extern "C"
 
 extern "C" int vg_euler_solve_batch(vg_ctx *ctx, int n_inst, const int64_t *offsets, const double *prim_in,

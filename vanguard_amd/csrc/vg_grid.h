@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstring>
 #include <type_traits>
+#include <vector>
 
 #include "vg_runtime.h"
 #include "vg_sweep.h"
@@ -176,6 +177,7 @@ struct GridState {
     bool planar_known = false;     // both buffers hold the constant planes (checked since the last load)
     int *flag = nullptr;           // device flag of the constant-plane check (lazy)
     Clock *tune_clk = nullptr;     // scratch clock for tuning launches (lazy)
+    Clock *sweep_clk = nullptr;    // scratch clock of the per-sweep entry points (lazy)
     hipGraphExec_t graph = nullptr;  // captured chunk of steps
     std::string graph_key;           // arguments the graph was captured with
     static constexpr int kGraphSteps = 8;   // steps per captured chunk (even: buffers return to their start)
@@ -213,6 +215,7 @@ struct GridState {
         hipFree(scan);
         hipFree(partials);
         hipFree(tune_clk);
+        hipFree(sweep_clk);
         hipFree(flag);
         if (graph) hipGraphExecDestroy(graph);
     }
@@ -302,7 +305,8 @@ struct GridState {
     // One sweep with physics Q (P, or its reduced physics on the same planes).
     template <class Q, int ORDER, int MODE>
     int launch_sweep_as(int dir, const double *in, double *out, double dt_scale, int fuse, int bc, double dx,
-                        double bias, int limiter, const typename P::Params &m)
+                        double bias, int limiter, const typename P::Params &m, Clock *use_clk = nullptr,
+                        int fixed_params = 0)
     {
         SweepArgs<Q> a;
         a.in = in;
@@ -318,8 +322,9 @@ struct GridState {
         a.dim = dim;
         a.fuse_max = fuse;
         a.m = m;
-        a.clk = clk;
+        a.clk = use_clk ? use_clk : clk;
         a.partials = partials;
+        a.fixed_params = fixed_params;
         if (dir == 0) {
             constexpr int G = (ORDER == 2) ? 2 : 1;
             dim3 grid((nx + (kBSX - 2 * G) - 1) / (kBSX - 2 * G), (ny + kRX - 1) / kRX);
@@ -335,13 +340,47 @@ struct GridState {
 
     template <int ORDER, int MODE>
     int launch_sweep(int dir, const double *in, double *out, double dt_scale, int fuse, int bc, double dx,
-                     double bias, int limiter, const typename P::Params &m)
+                     double bias, int limiter, const typename P::Params &m, Clock *use_clk = nullptr,
+                     int fixed_params = 0)
     {
         using R = typename ReducedPhys<P>::type;
         if constexpr (!std::is_void_v<R>) {
-            if (reduced) return launch_sweep_as<R, ORDER, MODE>(dir, in, out, dt_scale, fuse, bc, dx, bias, limiter, m);
+            if (reduced)
+                return launch_sweep_as<R, ORDER, MODE>(dir, in, out, dt_scale, fuse, bc, dx, bias, limiter, m, use_clk,
+                                                       fixed_params);
         }
-        return launch_sweep_as<P, ORDER, MODE>(dir, in, out, dt_scale, fuse, bc, dx, bias, limiter, m);
+        return launch_sweep_as<P, ORDER, MODE>(dir, in, out, dt_scale, fuse, bc, dx, bias, limiter, m, use_clk,
+                                               fixed_params);
+    }
+
+    // One sweep of the current grid in direction dir with the caller's time step
+    // dt and parameters as given (the per-sweep entry points: the reference's
+    // computeSLICTimeStep / computeX|YSLICTimeStep2D and FORCE twins).  The
+    // grid's own clock is left alone: dt goes into a scratch clock.
+    template <int ORDER, int MODE>
+    int sweep_once(int dir, double dt, int bc, double dx, double bias, int limiter, const typename P::Params &m)
+    {
+        if (!sweep_clk) VG_HIP(hipMalloc(&sweep_clk, sizeof(Clock)));
+        Clock h{};
+        h.dt = dt;
+        VG_HIP(hipMemcpyAsync(sweep_clk, &h, sizeof(Clock), hipMemcpyHostToDevice, ctx->stream));
+        int rc = choose_physics();
+        if (rc) return rc;
+        rc = launch_sweep<ORDER, MODE>(dir, buf[cur], buf[1 - cur], 1.0, 0, bc, dx, bias, limiter, m, sweep_clk, 1);
+        if (rc) return launch_status("sweep");
+        cur = 1 - cur;
+        max_valid = false;
+        VG_HIP(hipStreamSynchronize(ctx->stream));
+        return VG_OK;
+    }
+    int sweep_once(int order, int mode, int dir, double dt, int bc, double dx, double bias, int limiter,
+                   const typename P::Params &m)
+    {
+        if (order == 2)
+            return mode == VG_MODE_EXACT ? sweep_once<2, 0>(dir, dt, bc, dx, bias, limiter, m)
+                                         : sweep_once<2, 1>(dir, dt, bc, dx, bias, limiter, m);
+        return mode == VG_MODE_EXACT ? sweep_once<1, 0>(dir, dt, bc, dx, bias, limiter, m)
+                                     : sweep_once<1, 1>(dir, dt, bc, dx, bias, limiter, m);
     }
 
     // Decides this run's physics: the reduced one when the context allows it and
@@ -619,5 +658,56 @@ struct GridState {
         return VG_OK;
     }
 };
+
+
+// The reference's per-sweep calls on a grid the caller holds on the host with
+// its ghost cells (computeSLICTimeStep(currentCells, currentCellsWithBoundary,
+// ...), secondordersolver.cpp:80-98, 140-162, 214-236; computeFORCETimeStep and
+// computeX|YFORCETimeStep2D, firstordersolver.cpp:232-378): `with_boundary` is
+// the ghosted snapshot as given ((n + 2G) cells in 1D, (ny + 2G) x (nx + 2G) in
+// 2D, G = 2 for SLIC, 1 for FORCE), `cells` receives the ny x nx updated
+// interior.  The whole ghosted array is swept on the device (its ghost rows and
+// columns with index-mapped neighbours, then discarded); every interior cell
+// reads exactly the ghosted neighbours the reference's loop reads.
+template <class P>
+int ghosted_time_step(vg_ctx *ctx, int dim, double *cells, const double *with_boundary, int ny, int nx, int dir,
+                      double dx, double dt, double bias, int limiter, int order, int mode,
+                      const typename P::Params &m)
+{
+    constexpr int NF = P::NF;
+    const int G = (order == 2) ? 2 : 1;
+    const int gy = (dim == 1) ? 1 : ny + 2 * G, gx = nx + 2 * G;
+    GridState<P> g;
+    int rc = g.create(ctx, dim, gy, gx);
+    if (!rc) rc = g.load(with_boundary, 0);
+    if (!rc) rc = g.sweep_once(order, mode, dir, dt, VG_BC_TRANSMISSIVE, dx, bias, limiter, m);
+    std::vector<double> out;
+    if (!rc) {
+        out.resize((size_t)gy * gx * NF);
+        rc = g.store(out.data(), 0);
+    }
+    g.destroy();
+    if (rc) return rc;
+    const int oy = (dim == 1) ? 0 : G;
+    for (int i = 0; i < ny; i++)
+        std::memcpy(cells + (size_t)i * nx * NF, out.data() + ((size_t)(i + oy) * gx + G) * NF, sizeof(double) * nx * NF);
+    return VG_OK;
+}
+
+// Argument checks shared by the per-sweep entry points.
+inline int check_time_step_args(int dim, int ny, int nx, int dir, double dx, double dt, int order, int mode,
+                                const void *cells, const void *with_boundary, const void *mp)
+{
+    VG_CHECK(cells != nullptr && with_boundary != nullptr && mp != nullptr, "null argument");
+    VG_CHECK(order == 1 || order == 2, "order must be 1 (FORCE) or 2 (SLIC)");
+    VG_CHECK(mode == VG_MODE_EXACT || mode == VG_MODE_FAST, "mode must be VG_MODE_EXACT or VG_MODE_FAST");
+    VG_CHECK(dim == 1 ? (ny == 1 && dir == 0) : (ny >= 1 && (dir == 0 || dir == 1)),
+             "dir must be 0 (x) or 1 (y; 2D only)");
+    VG_CHECK(nx >= 1, "grid needs at least 1 cell");
+    VG_CHECK((long long)(ny + 4) * (nx + 4) <= (1ll << 31), "grid too large");
+    VG_CHECK(dx > 0.0 && std::isfinite(dx), "cell spacing must be positive");
+    VG_CHECK(std::isfinite(dt), "time step must be finite");
+    return VG_OK;
+}
 
 }  // namespace vg

@@ -188,4 +188,50 @@ int vg_hpr_state_max_wave_speed(vg_hpr_state *st, int bc, const vg_hpr_params *m
     return VG_OK;
 }
 
+int vg_hpr_state_sweep(vg_hpr_state *st, int dir, int order, double dx, double dt, double bias, int limiter, int bc,
+                       const vg_hpr_params *mp)
+{
+    vg::clear_error();
+    VG_CHECK(st != nullptr, "null state");
+    int rc = check_params(mp, 0);
+    if (rc) return rc;
+    VG_CHECK(order == 1 || order == 2, "order must be 1 (FORCE) or 2 (SLIC)");
+    VG_CHECK(bc == VG_BC_TRANSMISSIVE || bc == VG_BC_PERIODIC, "unknown boundary condition");
+    VG_CHECK(dir == 0 || (dir == 1 && st->g.dim == 2), "dir must be 0 (x) or 1 (y; 2D only)");
+    VG_CHECK(dx > 0.0 && std::isfinite(dx) && std::isfinite(dt), "cell spacing must be positive, dt finite");
+    rc = vg::bind(st->g.ctx);
+    if (rc) return rc;
+    return st->g.sweep_once(order, VG_MODE_EXACT, dir, dt, bc, dx, bias, limiter, to_params(mp));
+}
+
+int vg_hpr_time_step_1d(vg_ctx *ctx, double *cells, const double *cells_with_boundary, int n, double dx, double dt,
+                        double bias, int limiter, int order, const vg_hpr_params *mp)
+{
+    vg::clear_error();
+    VG_CHECK(ctx != nullptr, "null context");
+    int rc = check_params(mp, 0);
+    if (rc) return rc;
+    rc = vg::check_time_step_args(1, 1, n, 0, dx, dt, order, VG_MODE_EXACT, cells, cells_with_boundary, mp);
+    if (rc) return rc;
+    rc = vg::bind(ctx);
+    if (rc) return rc;
+    return vg::ghosted_time_step<HPRPhys>(ctx, 1, cells, cells_with_boundary, 1, n, 0, dx, dt, bias, limiter, order,
+                                          VG_MODE_EXACT, to_params(mp));
+}
+
+int vg_hpr_time_step_2d(vg_ctx *ctx, double *cells, const double *cells_with_boundary, int ny, int nx, int dir,
+                        double dx, double dt, double bias, int limiter, int order, const vg_hpr_params *mp)
+{
+    vg::clear_error();
+    VG_CHECK(ctx != nullptr, "null context");
+    int rc = check_params(mp, 0);
+    if (rc) return rc;
+    rc = vg::check_time_step_args(2, ny, nx, dir, dx, dt, order, VG_MODE_EXACT, cells, cells_with_boundary, mp);
+    if (rc) return rc;
+    rc = vg::bind(ctx);
+    if (rc) return rc;
+    return vg::ghosted_time_step<HPRPhys>(ctx, 2, cells, cells_with_boundary, ny, nx, dir, dx, dt, bias, limiter,
+                                          order, VG_MODE_EXACT, to_params(mp));
+}
+
 }  // extern "C"

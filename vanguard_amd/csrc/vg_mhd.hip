@@ -141,4 +141,30 @@ int vg_mhd_solve_batch(vg_ctx *ctx, int n_inst, const int64_t *offsets, const do
                                             t_out, stats);
 }
 
+int vg_mhd_time_step_1d(vg_ctx *ctx, double *cells, const double *cells_with_boundary, int n, double dx, double dt,
+                        double bias, int limiter, int order, const vg_mhd_params *mp, int mode)
+{
+    vg::clear_error();
+    VG_CHECK(ctx != nullptr, "null context");
+    int rc = vg::check_time_step_args(1, 1, n, 0, dx, dt, order, mode, cells, cells_with_boundary, mp);
+    if (rc) return rc;
+    rc = vg::bind(ctx);
+    if (rc) return rc;
+    return vg::ghosted_time_step<MHDPhys>(ctx, 1, cells, cells_with_boundary, 1, n, 0, dx, dt, bias, limiter, order,
+                                          mode, to_params(mp));
+}
+
+int vg_mhd_time_step_2d(vg_ctx *ctx, double *cells, const double *cells_with_boundary, int ny, int nx, int dir,
+                        double dx, double dt, double bias, int limiter, int order, const vg_mhd_params *mp, int mode)
+{
+    vg::clear_error();
+    VG_CHECK(ctx != nullptr, "null context");
+    int rc = vg::check_time_step_args(2, ny, nx, dir, dx, dt, order, mode, cells, cells_with_boundary, mp);
+    if (rc) return rc;
+    rc = vg::bind(ctx);
+    if (rc) return rc;
+    return vg::ghosted_time_step<MHDPhys>(ctx, 2, cells, cells_with_boundary, ny, nx, dir, dx, dt, bias, limiter,
+                                          order, mode, to_params(mp));
+}
+
 }  // extern "C"

@@ -5,6 +5,7 @@
 #pragma once
 
 #include "vg_device.h"
+#include "vg_quot.h"
 
 namespace vg {
 
@@ -53,32 +54,53 @@ struct EulerPhys {
         U[5] = rho * lam;
     }
 
-    // setConservedVariableVector (Euler/eulerstatevector.cpp:47-61).
-    __device__ static __forceinline__ void cons_to_prim(const Params &m, const double (&U)[NF], double (&w)[NF])
+    // The quotients by rho of a conserved vector (u, v, w, lambda, E / rho), each
+    // the correctly rounded IEEE quotient, from one shared reciprocal of rho.
+    __device__ static __forceinline__ void by_rho(const double (&U)[NF], const SharedDivisor &d, double (&q)[5])
+    {
+        const double a[5] = {U[1], U[2], U[3], U[5], U[4]};
+        quot_group<5>(a, d, q);
+    }
+
+    // setConservedVariableVector (Euler/eulerstatevector.cpp:47-61); returns
+    // the shared divisor of rho for a flux of the same density.
+    __device__ static __forceinline__ SharedDivisor cons_to_prim_d(const Params &m, const double (&U)[NF], double (&w)[NF])
     {
         const double rho = U[0];
-        const double u = U[1] / rho, v = U[2] / rho, z = U[3] / rho;
-        const double lam = U[5] / rho;
+        const SharedDivisor d = shared_divisor(rho);
+        double q[5];
+        by_rho(U, d, q);
+        const double u = q[0], v = q[1], z = q[2];
+        const double lam = q[3];
         const double vsq = (u * u) + (v * v) + (z * z);
-        const double e = (U[4] / rho) - (0.5 * vsq) - (m.e_form * (lam - 1.0));
+        const double e = q[4] - (0.5 * vsq) - (m.e_form * (lam - 1.0));
         w[0] = rho;
         w[1] = u;
         w[2] = v;
         w[3] = z;
         w[4] = eos_p(m, rho, e);
         w[5] = lam;
+        return d;
+    }
+    __device__ static __forceinline__ void cons_to_prim(const Params &m, const double (&U)[NF], double (&w)[NF])
+    {
+        (void)cons_to_prim_d(m, U, w);
     }
 
     // computeXFluxVector / computeYFluxVector of a conserved vector
-    // (Euler/eulerstatevector.cpp:93-120, 127-154).
+    // (Euler/eulerstatevector.cpp:93-120, 127-154), given the shared divisor
+    // of its density U[0].
     template <int DIR>
-    __device__ static __forceinline__ void flux(const Params &m, const double (&U)[NF], double (&F)[NF])
+    __device__ static __forceinline__ void flux_d(const Params &m, const double (&U)[NF], const SharedDivisor &d,
+                                                  double (&F)[NF])
     {
         const double rho = U[0];
-        const double u = U[1] / rho, v = U[2] / rho, z = U[3] / rho;
-        const double lam = U[5] / rho;
+        double q[5];
+        by_rho(U, d, q);
+        const double u = q[0], v = q[1], z = q[2];
+        const double lam = q[3];
         const double vsq = (u * u) + (v * v) + (z * z);
-        const double et = U[4] / rho;
+        const double et = q[4];
         const double e = et - (0.5 * vsq) - (m.e_form * (lam - 1.0));
         const double p = eos_p(m, rho, e);
         if (DIR == 0) {
@@ -96,6 +118,11 @@ struct EulerPhys {
             F[4] = (rho * (v * et)) + (v * p);
             F[5] = rho * (v * lam);
         }
+    }
+    template <int DIR>
+    __device__ static __forceinline__ void flux(const Params &m, const double (&U)[NF], double (&F)[NF])
+    {
+        flux_d<DIR>(m, U, shared_divisor(U[0]), F);
     }
 
     // Wave speed of Solvers::computeMaximumWaveSpeed (1D, |u| + c) and
@@ -190,24 +217,40 @@ struct EulerPlanarPhys {
         U[2] = rho * v;
         U[3] = rho * ((0.5 * vsq) + EulerPhys::eos_e(m, rho, p));
     }
-    __device__ static __forceinline__ void cons_to_prim(const Params &m, const double (&U)[NF], double (&w)[NF])
+    __device__ static __forceinline__ void by_rho(const double (&U)[NF], const SharedDivisor &d, double (&q)[3])
+    {
+        const double a[3] = {U[1], U[2], U[3]};
+        quot_group<3>(a, d, q);
+    }
+    __device__ static __forceinline__ SharedDivisor cons_to_prim_d(const Params &m, const double (&U)[NF], double (&w)[NF])
     {
         const double rho = U[0];
-        const double u = U[1] / rho, v = U[2] / rho;
+        const SharedDivisor d = shared_divisor(rho);
+        double q[3];
+        by_rho(U, d, q);
+        const double u = q[0], v = q[1];
         const double vsq = (u * u) + (v * v);
-        const double e = (U[3] / rho) - (0.5 * vsq);
+        const double e = q[2] - (0.5 * vsq);
         w[0] = rho;
         w[1] = u;
         w[2] = v;
         w[3] = EulerPhys::eos_p(m, rho, e);
+        return d;
+    }
+    __device__ static __forceinline__ void cons_to_prim(const Params &m, const double (&U)[NF], double (&w)[NF])
+    {
+        (void)cons_to_prim_d(m, U, w);
     }
     template <int DIR>
-    __device__ static __forceinline__ void flux(const Params &m, const double (&U)[NF], double (&F)[NF])
+    __device__ static __forceinline__ void flux_d(const Params &m, const double (&U)[NF], const SharedDivisor &d,
+                                                  double (&F)[NF])
     {
         const double rho = U[0];
-        const double u = U[1] / rho, v = U[2] / rho;
+        double q[3];
+        by_rho(U, d, q);
+        const double u = q[0], v = q[1];
         const double vsq = (u * u) + (v * v);
-        const double et = U[3] / rho;
+        const double et = q[2];
         const double e = et - (0.5 * vsq);
         const double p = EulerPhys::eos_p(m, rho, e);
         if (DIR == 0) {
@@ -221,6 +264,11 @@ struct EulerPlanarPhys {
             F[2] = (rho * (v * v)) + p;
             F[3] = (rho * (v * et)) + (v * p);
         }
+    }
+    template <int DIR>
+    __device__ static __forceinline__ void flux(const Params &m, const double (&U)[NF], double (&F)[NF])
+    {
+        flux_d<DIR>(m, U, shared_divisor(U[0]), F);
     }
     __device__ static __forceinline__ double wave_speed(const Params &m, const double (&w)[NF], int dim)
     {

@@ -755,9 +755,32 @@ int adjoint(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, cons
     }
     const int steps = (int)dts.size();
     rc = g.store(m1_out, m2_out, phi_out);
+    if (!rc) {
+        // A forward that ends with inadmissible real cells has no derivative:
+        // report it as solve() does; the gradients are set to NaN.
+        int64_t bad = 0, first_bad = -1;
+        scan_final(m1_out, m2_out, phi_out, (size_t)ny * nx, p1, p2, bad, first_bad);
+        if (bad > 0) {
+            const size_t n6 = 6 * (size_t)ny * nx;
+            std::fill(grad1, grad1 + n6, std::nan(""));
+            std::fill(grad2, grad2 + n6, std::nan(""));
+            if (dt_log) std::memcpy(dt_log, dts.data(), sizeof(double) * std::min(steps, std::max(dt_cap, 0)));
+            if (stats) {
+                *stats = fst;
+                stats->steps = stats->total_steps = steps;
+                stats->nonfinite_cells = bad;
+                stats->first_nonfinite = first_bad;
+            }
+            g.destroy();
+            return nonfinite_error(bad, first_bad);
+        }
+    }
     // ---- buffers of the reverse pass --------------------------------------------------
-    double *dts_dev = nullptr, *ghost = nullptr, *A[2][2], *T1[2], *T2[2];
+    double *dts_dev = nullptr, *ghost = nullptr, *A[2][2], *T1[2], *T2[2], *flag_buf = nullptr;
     if (!rc) rc = wsp(0, sizeof(double) * std::max(steps, 1), &dts_dev);
+    if (!rc) rc = wsp(1, sizeof(double), &flag_buf);
+    int *bad_step = reinterpret_cast<int *>(flag_buf);   // latest reversed step after which the adjoint is non-finite
+    if (!rc) VG_HIP(hipMemsetAsync(bad_step, 0xFF, sizeof(int), s));
     if (!rc) rc = wsp(2, sizeof(double) * 4 * 6 * (size_t)std::max(nx, ny), &ghost);
     for (int b = 0; b < 2 && !rc; b++)
         for (int q = 0; q < 2 && !rc; q++) rc = wsp(3 + 2 * b + q, gb, &A[b][q]);
@@ -876,6 +899,11 @@ int adjoint(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, cons
                 rc = ctx->timed("rgfm_fill_adjoint",
                                 [&] { vg::rgfm_fill_adjoint_kernel<0><<<ny, vg::kLineBS, 0, s>>>(fa); });
             }
+            for (int q = 0; q < 2 && !rc; q++) {
+                vg::adjoint_nonfinite_kernel<<<vg::RGFMGrid::blocks_for(g.cells), 256, 0, s>>>(A[cw][q], g.plane,
+                                                                                              g.cells, st, bad_step);
+                rc = vg::launch_status("adjoint_nonfinite");
+            }
         }
     }
     for (int q = 0; q < 2 && !rc; q++) {
@@ -890,6 +918,11 @@ int adjoint(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, cons
         }
     }
     if (!rc && ctx->timing) rc = ctx->resolve_timing();
+    int first_bad_step = -1;
+    if (!rc) {
+        VG_HIP(hipMemcpyAsync(&first_bad_step, bad_step, sizeof(int), hipMemcpyDeviceToHost, s));
+        VG_HIP(hipStreamSynchronize(s));
+    }
     if (dt_log) std::memcpy(dt_log, dts.data(), sizeof(double) * std::min(steps, std::max(dt_cap, 0)));
     if (stats) {
         *stats = fst;
@@ -900,6 +933,29 @@ int adjoint(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, cons
         stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
     }
     g.destroy();
+    if (!rc) {
+        // A gradient that is not finite is reported, with the forward step whose
+        // reversal first produced a non-finite adjoint.
+        const size_t n6 = 6 * (size_t)ny * nx;
+        int64_t bad = 0, first = -1;
+        for (size_t k = 0; k < 2 * n6; k++) {
+            const double v = k < n6 ? grad1[k] : grad2[k - n6];
+            if (!std::isfinite(v)) {
+                if (first < 0) first = (int64_t)(k % n6) / 6;
+                bad += 1;
+            }
+        }
+        if (stats) {
+            stats->nonfinite_cells = bad;
+            stats->first_nonfinite = first;
+        }
+        if (bad > 0) {
+            vg::set_error("gradient holds " + std::to_string(bad) + " non-finite values (first at cell " +
+                          std::to_string(first) + "); the adjoint first became non-finite in the reverse of step " +
+                          std::to_string(first_bad_step) + " of " + std::to_string(steps));
+            return VG_ERR_NONFINITE;
+        }
+    }
     return rc;
 }
 

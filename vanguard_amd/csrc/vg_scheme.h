@@ -7,10 +7,22 @@
 #pragma once
 
 #include "vg_device.h"
+#include "vg_quot.h"
 
 namespace vg {
 
 enum { MODE_EXACT = 0, MODE_FAST = 1 };
+
+// Physics whose EXACT conversions can hand the shared divisor of rho from a
+// primitive conversion to the flux of the same density (cons_to_prim_d, flux_d).
+template <class P, class = void>
+struct HasSharedDivisor {
+    static constexpr bool value = false;
+};
+template <class P>
+struct HasSharedDivisor<P, decltype((void)&P::template flux_d<0>)> {
+    static constexpr bool value = true;
+};
 
 // SlopeLimiters::computeRCoefficient and computeSlopeLimiter
 // (Solvers/slopelimiters.cpp:7-74): 0 superbee, 1 van Leer, anything else minbee.
@@ -150,12 +162,23 @@ struct Scheme {
         }
         if (MODE == MODE_EXACT) {
             // setConservedVariableVector, then the FORCE flux converts the state
-            // back: computeConservedVariableVector + state-overload flux.
+            // back: computeConservedVariableVector + state-overload flux.  The
+            // round trip returns rho unchanged, so the flux divides by the
+            // density whose reciprocal the conversion already holds.
             double w[NF];
-            P::cons_to_prim(m, xl, w);
-            cons_flux_of_prim(m, w, left);
-            P::cons_to_prim(m, xr, w);
-            cons_flux_of_prim(m, w, right);
+            if constexpr (HasSharedDivisor<P>::value) {
+                const SharedDivisor dl = P::cons_to_prim_d(m, xl, w);
+                cons(m, w, left.U);
+                P::template flux_d<DIR>(m, left.U, dl, left.F);
+                const SharedDivisor dr = P::cons_to_prim_d(m, xr, w);
+                cons(m, w, right.U);
+                P::template flux_d<DIR>(m, right.U, dr, right.F);
+            } else {
+                P::cons_to_prim(m, xl, w);
+                cons_flux_of_prim(m, w, left);
+                P::cons_to_prim(m, xr, w);
+                cons_flux_of_prim(m, w, right);
+            }
         } else {
             double w[NF];
 #pragma unroll

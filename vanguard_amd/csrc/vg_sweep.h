@@ -35,6 +35,7 @@ struct SweepArgs {
     typename P::Params m;
     Clock *clk;
     unsigned long long *partials;   // one wave-speed key per block when fuse_max
+    int fixed_params = 0;           // keep m as given (a per-sweep call): no per-step parameters from the clock
 };
 
 // Per-step scalars, computed identically in every thread.
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(BS) void sweep_x_kernel(SweepArgs<P> a)
     const double ts = a.clk->dt * a.dt_scale;
     const StepScalars sc(ts, a.dx);
     typename P::Params m = a.m;
-    if (ORDER == 2) P::begin_step(m, a.clk->S);   // first-order solvers never set c_h
+    if (ORDER == 2 && !a.fixed_params) P::begin_step(m, a.clk->S);   // first-order solvers never set c_h
 
     const int tid = threadIdx.x;
     const int j = blockIdx.x * OUT - G + tid;
@@ -216,7 +217,7 @@ __global__ __launch_bounds__(BS) void sweep_y_kernel(SweepArgs<P> a)
     const double ts = a.clk->dt * a.dt_scale;
     const StepScalars sc(ts, a.dx);
     typename P::Params m = a.m;
-    if (ORDER == 2) P::begin_step(m, a.clk->S);   // first-order solvers never set c_h
+    if (ORDER == 2 && !a.fixed_params) P::begin_step(m, a.clk->S);   // first-order solvers never set c_h
 
     const int j = blockIdx.x * BS + threadIdx.x;
     const int ny = a.ny, nx = a.nx;

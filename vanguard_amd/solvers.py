@@ -253,6 +253,84 @@ def euler_solve_fixed_dt(prim: np.ndarray, dim: int, dx: float, dts, *, bias: fl
     return out
 
 
+# ---- ghost cells and per-sweep calls ---------------------------------------------
+
+EULER_DEFAULT_STATE = (1.0, 0.0, 0.0, 0.0, 1.0, 1.0)                 # Euler/eulerstatevector.cpp:3-12
+MHD_DEFAULT_STATE = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0)    # MHD/mhdstatevector.cpp:3-16
+HPR_DEFAULT_STATE = (1.0, 1.0 / 1.4, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0,
+                     0.0)                                             # HPR/hprstatevector.cpp:3-16
+
+
+class Solvers:
+    """Host-side helpers of Solvers/solvers.cpp that build the ghosted grids the
+    per-sweep calls read."""
+
+    @staticmethod
+    def insertBoundaryCells(current_cells, boundary_size, default_state=EULER_DEFAULT_STATE):
+        """Solvers::insertBoundaryCells (solvers.cpp:7-32): transmissive ghosts,
+        one layer copied, two layers mirror-ordered (g[-1] = c[0], g[-2] = c[1])."""
+        c = np.asarray(current_cells, dtype=np.float64)
+        n, b = c.shape[0], boundary_size
+        out = np.empty((n + 2 * b, c.shape[1]))
+        out[:] = default_state
+        if b == 1:
+            out[0], out[n + 1] = c[0], c[n - 1]
+        elif b == 2:
+            out[0], out[1], out[n + 2], out[n + 3] = c[1], c[0], c[n - 1], c[n - 2]
+        out[b:b + n] = c
+        return out
+
+    @staticmethod
+    def insertBoundaryCells2D(current_cells, boundary_size, default_state=EULER_DEFAULT_STATE):
+        """Solvers::insertBoundaryCells2D (solvers.cpp:88-143): the same per row
+        and per column; the corner ghosts keep the default state."""
+        c = np.asarray(current_cells, dtype=np.float64)
+        ny, nx, b = c.shape[0], c.shape[1], boundary_size
+        out = np.empty((ny + 2 * b, nx + 2 * b, c.shape[2]))
+        out[:] = default_state
+        if b == 1:
+            out[1:ny + 1, 0], out[1:ny + 1, nx + 1] = c[:, 0], c[:, nx - 1]
+            out[0, 1:nx + 1], out[ny + 1, 1:nx + 1] = c[0], c[ny - 1]
+        elif b == 2:
+            out[2:ny + 2, 0], out[2:ny + 2, 1] = c[:, 1], c[:, 0]
+            out[2:ny + 2, nx + 2], out[2:ny + 2, nx + 3] = c[:, nx - 1], c[:, nx - 2]
+            out[0, 2:nx + 2], out[1, 2:nx + 2] = c[1], c[0]
+            out[ny + 2, 2:nx + 2], out[ny + 3, 2:nx + 2] = c[ny - 1], c[ny - 2]
+        out[b:b + ny, b:b + nx] = c
+        return out
+
+
+def _time_step(fn_name, params_struct, current_cells, with_boundary, dim, direction, cell_spacing, time_step, bias,
+               limiter, order, nf, mode=None, ctx: Context | None = None):
+    """One per-sweep call: current_cells (an array of the reference's layout) is
+    overwritten in place with the updated states, as the reference's
+    computeSLICTimeStep mutates currentCells."""
+    ctx = ctx or default_context()
+    cells = current_cells
+    if not (isinstance(cells, np.ndarray) and cells.dtype == np.float64 and cells.flags.c_contiguous):
+        raise TypeError("current_cells must be a C-contiguous float64 array (updated in place)")
+    g = 2 if order == 2 else 1
+    wb = np.ascontiguousarray(with_boundary, dtype=np.float64)
+    if dim == 1:
+        n = cells.shape[0]
+        if cells.shape != (n, nf) or wb.shape != (n + 2 * g, nf):
+            raise ValueError(f"expected cells ({n}, {nf}) and a ghosted grid ({n + 2 * g}, {nf})")
+        args = [n]
+    else:
+        ny, nx = cells.shape[:2]
+        if cells.shape != (ny, nx, nf) or wb.shape != (ny + 2 * g, nx + 2 * g, nf):
+            raise ValueError(f"expected cells ({ny}, {nx}, {nf}) and a ghosted grid "
+                             f"({ny + 2 * g}, {nx + 2 * g}, {nf})")
+        args = [ny, nx, direction]
+    extra = [] if mode is None else [mode]
+    rc = getattr(ctx.lib, fn_name)(ctx.handle, _lib.as_dptr(cells), _lib.as_dptr(wb), *args, cell_spacing, time_step,
+                                   bias, limiter, order, C.byref(params_struct), *extra)
+    check(rc, fn_name)
+    return cells
+
+
+
+
 class SecondOrderSolver:
     """SLIC (MUSCL-Hancock + FORCE), Solvers/secondordersolver.h:7-72."""
 
@@ -269,6 +347,24 @@ class SecondOrderSolver:
         return euler_solve(initial_cells, 2, cell_spacing, cfl_coefficient, final_time, order=2, bias=bias,
                            limiter=slope_limiter, params=material_parameters, **kw).cells
 
+    @staticmethod
+    def computeSLICTimeStep(current_cells, current_cells_with_boundary, cell_spacing, time_step, bias, slope_limiter,
+                            material_parameters, mode=EXACT):
+        return _time_step("vg_euler_time_step_1d", material_parameters.c_struct(), current_cells,
+                          current_cells_with_boundary, 1, 0, cell_spacing, time_step, bias, slope_limiter, 2, 6, mode)
+
+    @staticmethod
+    def computeXSLICTimeStep2D(current_cells, current_cells_with_boundary, cell_spacing, time_step, bias,
+                               slope_limiter, material_parameters, mode=EXACT):
+        return _time_step("vg_euler_time_step_2d", material_parameters.c_struct(), current_cells,
+                          current_cells_with_boundary, 2, 0, cell_spacing, time_step, bias, slope_limiter, 2, 6, mode)
+
+    @staticmethod
+    def computeYSLICTimeStep2D(current_cells, current_cells_with_boundary, cell_spacing, time_step, bias,
+                               slope_limiter, material_parameters, mode=EXACT):
+        return _time_step("vg_euler_time_step_2d", material_parameters.c_struct(), current_cells,
+                          current_cells_with_boundary, 2, 1, cell_spacing, time_step, bias, slope_limiter, 2, 6, mode)
+
 
 class FirstOrderSolver:
     """FORCE, Solvers/firstordersolver.h:7-101."""
@@ -284,6 +380,24 @@ class FirstOrderSolver:
                 material_parameters, **kw):
         return euler_solve(initial_cells, 2, cell_spacing, cfl_coefficient, final_time, order=1,
                            params=material_parameters, **kw).cells
+
+    @staticmethod
+    def computeFORCETimeStep(current_cells, current_cells_with_boundary, cell_spacing, time_step,
+                             material_parameters, mode=EXACT):
+        return _time_step("vg_euler_time_step_1d", material_parameters.c_struct(), current_cells,
+                          current_cells_with_boundary, 1, 0, cell_spacing, time_step, 0.0, 0, 1, 6, mode)
+
+    @staticmethod
+    def computeXFORCETimeStep2D(current_cells, current_cells_with_boundary, cell_spacing, time_step,
+                                material_parameters, mode=EXACT):
+        return _time_step("vg_euler_time_step_2d", material_parameters.c_struct(), current_cells,
+                          current_cells_with_boundary, 2, 0, cell_spacing, time_step, 0.0, 0, 1, 6, mode)
+
+    @staticmethod
+    def computeYFORCETimeStep2D(current_cells, current_cells_with_boundary, cell_spacing, time_step,
+                                material_parameters, mode=EXACT):
+        return _time_step("vg_euler_time_step_2d", material_parameters.c_struct(), current_cells,
+                          current_cells_with_boundary, 2, 1, cell_spacing, time_step, 0.0, 0, 1, 6, mode)
 
 
 class EulerGrid:
@@ -331,6 +445,14 @@ class EulerGrid:
                                           C.byref(mp), mode, _lib.as_dptr(dts) if dt_cap else None, dt_cap,
                                           C.byref(st)), "vg_euler_state_run")
         return st, dts[:min(st.steps, dt_cap)].copy()
+
+    def sweep(self, direction, dx, dt, *, order=2, bias=0.0, limiter=0, bc=TRANSMISSIVE,
+              params: EulerMaterialParameters | None = None, mode=EXACT):
+        """vg_euler_state_sweep: one sweep with the caller's time step."""
+        params = params or EulerMaterialParameters(1.4)
+        mp = params.c_struct()
+        check(self.lib.vg_euler_state_sweep(self.handle, direction, order, dx, dt, bias, limiter, bc, C.byref(mp),
+                                            mode), "vg_euler_state_sweep")
 
     def max_wave_speed(self, params: EulerMaterialParameters | None = None, bc=TRANSMISSIVE) -> float:
         params = params or EulerMaterialParameters(1.4)
@@ -477,6 +599,24 @@ class MHDSecondOrderSolver:
                          limiter=slope_limiter, params=material_parameters, subcycles=subcycling_iterations,
                          **kw).cells
 
+    @staticmethod
+    def computeSLICTimeStep(current_cells, current_cells_with_boundary, cell_spacing, time_step, bias, slope_limiter,
+                            material_parameters, mode=EXACT):
+        return _time_step("vg_mhd_time_step_1d", material_parameters.c_struct(), current_cells,
+                          current_cells_with_boundary, 1, 0, cell_spacing, time_step, bias, slope_limiter, 2, 9, mode)
+
+    @staticmethod
+    def computeXSLICTimeStep2D(current_cells, current_cells_with_boundary, cell_spacing, time_step, bias,
+                               slope_limiter, material_parameters, mode=EXACT):
+        return _time_step("vg_mhd_time_step_2d", material_parameters.c_struct(), current_cells,
+                          current_cells_with_boundary, 2, 0, cell_spacing, time_step, bias, slope_limiter, 2, 9, mode)
+
+    @staticmethod
+    def computeYSLICTimeStep2D(current_cells, current_cells_with_boundary, cell_spacing, time_step, bias,
+                               slope_limiter, material_parameters, mode=EXACT):
+        return _time_step("vg_mhd_time_step_2d", material_parameters.c_struct(), current_cells,
+                          current_cells_with_boundary, 2, 1, cell_spacing, time_step, bias, slope_limiter, 2, 9, mode)
+
 
 class MHDFirstOrderSolver:
     """MHDFirstOrderSolver (Solvers/mhdfirstordersolver.h): FORCE; never
@@ -494,10 +634,30 @@ class MHDFirstOrderSolver:
         return mhd_solve(initial_cells, 2, cell_spacing, cfl_coefficient, final_time, order=1,
                          params=material_parameters, **kw).cells
 
+    @staticmethod
+    def computeFORCETimeStep(current_cells, current_cells_with_boundary, cell_spacing, time_step,
+                             material_parameters, mode=EXACT):
+        return _time_step("vg_mhd_time_step_1d", material_parameters.c_struct(), current_cells,
+                          current_cells_with_boundary, 1, 0, cell_spacing, time_step, 0.0, 0, 1, 9, mode)
+
+    @staticmethod
+    def computeXFORCETimeStep2D(current_cells, current_cells_with_boundary, cell_spacing, time_step,
+                                material_parameters, mode=EXACT):
+        return _time_step("vg_mhd_time_step_2d", material_parameters.c_struct(), current_cells,
+                          current_cells_with_boundary, 2, 0, cell_spacing, time_step, 0.0, 0, 1, 9, mode)
+
+    @staticmethod
+    def computeYFORCETimeStep2D(current_cells, current_cells_with_boundary, cell_spacing, time_step,
+                                material_parameters, mode=EXACT):
+        return _time_step("vg_mhd_time_step_2d", material_parameters.c_struct(), current_cells,
+                          current_cells_with_boundary, 2, 1, cell_spacing, time_step, 0.0, 0, 1, 9, mode)
+
 
 # ---- Godunov-Peshkov-Romenski (HPR) ----------------------------------------------
 
 @dataclass
+
+
 class HPRMaterialParameters:
     """HPRMaterialParameters (HPR/hprmaterialparameters.h:10-15): the
     constructor's arguments in order, the EOS name as a string."""
@@ -663,6 +823,24 @@ class HPRSecondOrderSolver:
                          limiter=slope_limiter, subcycles=subcycling_iterations, params=material_parameters,
                          **kw).cells
 
+    @staticmethod
+    def computeSLICTimeStep(current_cells, current_cells_with_boundary, cell_spacing, time_step, bias, slope_limiter,
+                            material_parameters):
+        return _time_step("vg_hpr_time_step_1d", material_parameters.c_struct(), current_cells,
+                          current_cells_with_boundary, 1, 0, cell_spacing, time_step, bias, slope_limiter, 2, 17)
+
+    @staticmethod
+    def computeXSLICTimeStep2D(current_cells, current_cells_with_boundary, cell_spacing, time_step, bias,
+                               slope_limiter, material_parameters):
+        return _time_step("vg_hpr_time_step_2d", material_parameters.c_struct(), current_cells,
+                          current_cells_with_boundary, 2, 0, cell_spacing, time_step, bias, slope_limiter, 2, 17)
+
+    @staticmethod
+    def computeYSLICTimeStep2D(current_cells, current_cells_with_boundary, cell_spacing, time_step, bias,
+                               slope_limiter, material_parameters):
+        return _time_step("vg_hpr_time_step_2d", material_parameters.c_struct(), current_cells,
+                          current_cells_with_boundary, 2, 1, cell_spacing, time_step, bias, slope_limiter, 2, 17)
+
 
 class HPRFirstOrderSolver:
     """HPRFirstOrderSolver (Solvers/hprfirstordersolver.h): FORCE."""
@@ -678,6 +856,26 @@ class HPRFirstOrderSolver:
                 **kw):
         return hpr_solve(initial_cells, 2, cell_spacing, cfl_coefficient, final_time, order=1,
                          subcycles=subcycling_iterations, params=material_parameters, **kw).cells
+
+    @staticmethod
+    def computeFORCETimeStep(current_cells, current_cells_with_boundary, cell_spacing, time_step,
+                             material_parameters):
+        return _time_step("vg_hpr_time_step_1d", material_parameters.c_struct(), current_cells,
+                          current_cells_with_boundary, 1, 0, cell_spacing, time_step, 0.0, 0, 1, 17)
+
+    @staticmethod
+    def computeXFORCETimeStep2D(current_cells, current_cells_with_boundary, cell_spacing, time_step,
+                                material_parameters):
+        return _time_step("vg_hpr_time_step_2d", material_parameters.c_struct(), current_cells,
+                          current_cells_with_boundary, 2, 0, cell_spacing, time_step, 0.0, 0, 1, 17)
+
+    @staticmethod
+    def computeYFORCETimeStep2D(current_cells, current_cells_with_boundary, cell_spacing, time_step,
+                                material_parameters):
+        return _time_step("vg_hpr_time_step_2d", material_parameters.c_struct(), current_cells,
+                          current_cells_with_boundary, 2, 1, cell_spacing, time_step, 0.0, 0, 1, 17)
+
+
 
 
 class HPRGrid:
@@ -727,6 +925,14 @@ class HPRGrid:
                                         C.byref(mp), EXACT, _lib.as_dptr(dts) if dt_cap else None, dt_cap,
                                         C.byref(st)), "vg_hpr_state_run")
         return st, dts[:min(st.steps, dt_cap)].copy()
+
+    def sweep(self, direction, dx, dt, *, order=2, bias=0.0, limiter=0, bc=TRANSMISSIVE,
+              params: HPRMaterialParameters | None = None):
+        """vg_hpr_state_sweep: one sweep with the caller's time step (no source)."""
+        params = params or HPRMaterialParameters()
+        mp = params.c_struct()
+        check(self.lib.vg_hpr_state_sweep(self.handle, direction, order, dx, dt, bias, limiter, bc, C.byref(mp)),
+              "vg_hpr_state_sweep")
 
     def max_wave_speed(self, params: HPRMaterialParameters | None = None, bc=TRANSMISSIVE) -> float:
         params = params or HPRMaterialParameters()
@@ -843,15 +1049,19 @@ class RGFMAdjointResult:
     grad2: np.ndarray             # d loss / d initial material 2
     dt: np.ndarray                # the time steps the forward chose (held fixed)
     stats: _lib.RunStats
+    status: int = 0               # VG_OK, or VG_ERR_NONFINITE (see rgfm_adjoint)
 
 
 def rgfm_adjoint(system: MultimaterialSystem, seed1: np.ndarray, seed2: np.ndarray, dx: float, cfl: float,
                  t_final: float, params1, params2, *, dy: float | None = None, bias: float = 0.0, limiter: int = 0,
                  max_steps: int = 0, checkpoint_every: int = 0, dt_cap: int = 200000,
-                 ctx: Context | None = None) -> RGFMAdjointResult:
+                 ctx: Context | None = None, raise_on_error: bool = True) -> RGFMAdjointResult:
     """Reverse-mode derivative of the HLLC RGFM solve (vg_rgfm_adjoint_1d / _2d):
     seeds are d loss / d final material states, the result holds d loss / d
-    initial material states, time steps frozen."""
+    initial material states, time steps frozen.  A forward that ends with
+    inadmissible real cells, or a gradient that is not finite, gives
+    VG_ERR_NONFINITE (raised unless raise_on_error is False; the status is in
+    the result's `status`)."""
     ctx = ctx or default_context()
     m1 = np.ascontiguousarray(system.material1, dtype=np.float64)
     m2 = np.ascontiguousarray(system.material2, dtype=np.float64)
@@ -876,8 +1086,9 @@ def rgfm_adjoint(system: MultimaterialSystem, seed1: np.ndarray, seed2: np.ndarr
                                         P(g2), phi.shape[0], phi.shape[1], dx, dx if dy is None else dy, cfl,
                                         t_final, max_steps, bias, limiter, C.byref(q1), C.byref(q2),
                                         checkpoint_every, P(dts), dt_cap, C.byref(st))
-    check(rc, "vg_rgfm_adjoint")
-    return RGFMAdjointResult(MultimaterialSystem(o1, o2, oph), g1, g2, dts[:min(st.steps, dt_cap)].copy(), st)
+    if raise_on_error and rc != _lib.VG_OK:
+        check(rc, "vg_rgfm_adjoint")
+    return RGFMAdjointResult(MultimaterialSystem(o1, o2, oph), g1, g2, dts[:min(st.steps, dt_cap)].copy(), st, rc)
 
 
 def rgfm_solve_fixed_dt(system: MultimaterialSystem, dx: float, dts, params1, params2, *, dy: float | None = None,
