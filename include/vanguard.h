@@ -299,6 +299,20 @@ int vg_mhd_solve_batch(vg_ctx *ctx, int n_inst, const int64_t *offsets, const do
                        double bias, int limiter, int order, int mode, int32_t *steps, double *t_out,
                        vg_run_stats *stats);
 
+/* vg_euler_time_step_2d without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * `cells` and `cells_with_boundary` are DEVICE arrays (same layouts as
+ * vg_euler_time_step_2d); the intermediate planes and the clock live in a fixed
+ * device array of the library (96 MiB per device, held until the process
+ * ends), so (ny + 2G) * (nx + 2G) must not exceed 1048576 cells (VG_ERR_ARG
+ * otherwise).  Calls that overlap on one device share that array and race.
+ * Stream-ordered on the context's stream; returns after the work completes
+ * unless the stream is being captured. */
+int vg_euler_time_step_2d_noalloc(vg_ctx *ctx, double *cells, const double *cells_with_boundary, int ny, int nx,
+                                  int dir, double dx, double dt, double bias, int limiter, int order,
+                                  const vg_euler_params *mp, int mode);
+
 /* MHDSecondOrderSolver::computeSLICTimeStep / computeX|YSLICTimeStep2D
  * (mhdsecondordersolver.h:33-56, mhdsecondordersolver.cpp:91-108) when order
  * == 2 and MHDFirstOrderSolver's FORCE twins when order == 1, on host arrays as

@@ -114,6 +114,9 @@ SIGNATURES = {
                                         C.c_int, C.POINTER(EulerParams), C.c_int]),
     "vg_euler_time_step_2d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                         C.c_double, C.c_int, C.c_int, C.POINTER(EulerParams), C.c_int]),
+    "vg_euler_time_step_2d_noalloc": (C.c_int, [vptr, vptr, vptr, C.c_int, C.c_int, C.c_int, C.c_double,
+                                                C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(EulerParams),
+                                                C.c_int]),
     "vg_mhd_time_step_1d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                                       C.c_int, C.POINTER(MHDParams), C.c_int]),
     "vg_mhd_time_step_2d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
