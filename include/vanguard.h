@@ -324,6 +324,63 @@ int vg_mhd_time_step_1d(vg_ctx *ctx, double *cells, const double *cells_with_bou
 int vg_mhd_time_step_2d(vg_ctx *ctx, double *cells, const double *cells_with_boundary, int ny, int nx, int dir,
                         double dx, double dt, double bias, int limiter, int order, const vg_mhd_params *mp, int mode);
 
+/* ---- two-material Euler: diffuse-interface models ------------------------ */
+
+/* The reference's EulerMultiphysicsStateVector (Euler/Multiphysics/
+ * eulermultiphysicsstatevector.h:8-76): 8 primitives per cell, in the order of
+ * its primitive vector (alpha1, u, v, w, rho1, p1, rho2, p2); and
+ * EulerReducedStateVector (eulerreducedstatevector.h:8-71): 7 (alpha1, u, v, w,
+ * rho1, rho2, p).  p1 / p2 are the two EulerMaterialParameters.  Both compute
+ * in EXACT arithmetic, bit-identical to the reference. */
+
+/* SecondOrderSolver::solve / solve2D for these states
+ * (Solvers/secondordersolver.h:57-70, secondordersolver.cpp:316-517) when order
+ * == 2, with MultiphysicsSolvers::reinitialiseVolumeFraction after every
+ * reinit_every-th step (0: never); FirstOrderSolver::solve / solve2D
+ * (firstordersolver.h:89-100, firstordersolver.cpp:463-640) when order == 1,
+ * which never reinitialises.  The remaining arguments are vg_euler_solve_*'s. */
+int vg_euler_multiphysics_solve_1d(vg_ctx *ctx, const double *prim_in, double *prim_out, int n, double dx, double cfl,
+                                   double t_final, int max_steps, double bias, int limiter, int order, int bc,
+                                   int reinit_every, const vg_euler_params *p1, const vg_euler_params *p2,
+                                   double *dt_log, int dt_log_cap, vg_run_stats *stats);
+int vg_euler_multiphysics_solve_2d(vg_ctx *ctx, const double *prim_in, double *prim_out, int ny, int nx, double dx,
+                                   double cfl, double t_final, int max_steps, double bias, int limiter, int order,
+                                   int bc, int reinit_every, const vg_euler_params *p1, const vg_euler_params *p2,
+                                   double *dt_log, int dt_log_cap, vg_run_stats *stats);
+int vg_euler_reduced_solve_1d(vg_ctx *ctx, const double *prim_in, double *prim_out, int n, double dx, double cfl,
+                              double t_final, int max_steps, double bias, int limiter, int order, int bc,
+                              int reinit_every, const vg_euler_params *p1, const vg_euler_params *p2, double *dt_log,
+                              int dt_log_cap, vg_run_stats *stats);
+int vg_euler_reduced_solve_2d(vg_ctx *ctx, const double *prim_in, double *prim_out, int ny, int nx, double dx,
+                              double cfl, double t_final, int max_steps, double bias, int limiter, int order, int bc,
+                              int reinit_every, const vg_euler_params *p1, const vg_euler_params *p2, double *dt_log,
+                              int dt_log_cap, vg_run_stats *stats);
+
+/* computeSLICTimeStep / computeX|YSLICTimeStep2D for these states
+ * (secondordersolver.h:34-36, 41-44, 49-52) when order == 2 and the FORCE twins
+ * (firstordersolver.h:65-82) when order == 1, on host arrays with the caller's
+ * ghosted snapshot, as vg_euler_time_step_*. */
+int vg_euler_multiphysics_time_step_1d(vg_ctx *ctx, double *cells, const double *cells_with_boundary, int n,
+                                       double dx, double dt, double bias, int limiter, int order,
+                                       const vg_euler_params *p1, const vg_euler_params *p2);
+int vg_euler_multiphysics_time_step_2d(vg_ctx *ctx, double *cells, const double *cells_with_boundary, int ny, int nx,
+                                       int dir, double dx, double dt, double bias, int limiter, int order,
+                                       const vg_euler_params *p1, const vg_euler_params *p2);
+int vg_euler_reduced_time_step_1d(vg_ctx *ctx, double *cells, const double *cells_with_boundary, int n, double dx,
+                                  double dt, double bias, int limiter, int order, const vg_euler_params *p1,
+                                  const vg_euler_params *p2);
+int vg_euler_reduced_time_step_2d(vg_ctx *ctx, double *cells, const double *cells_with_boundary, int ny, int nx,
+                                  int dir, double dx, double dt, double bias, int limiter, int order,
+                                  const vg_euler_params *p1, const vg_euler_params *p2);
+
+/* MultiphysicsSolvers::reinitialiseVolumeFraction (multiphysicssolvers.h:12-17)
+ * on a host array in place: 1D resets the four cells around the last falling
+ * crossing of alpha1 = 0.5, 2D every crossing of every row, then of every
+ * column.  Where the reference would index outside the grid (a crossing within
+ * two cells of an edge, or no crossing in 1D) the outside cells are skipped. */
+int vg_euler_multiphysics_reinitialise(vg_ctx *ctx, double *prim, int dim, int ny, int nx);
+int vg_euler_reduced_reinitialise(vg_ctx *ctx, double *prim, int dim, int ny, int nx);
+
 /* ---- two-material Euler: real ghost fluid method (RGFM) ----------------- */
 
 enum {
@@ -404,17 +461,23 @@ int vg_rgfm_solve_fixed_dt_2d(vg_ctx *ctx, const double *m1_in, const double *m2
 
 enum {
     VG_HPR_EOS_STIFFENED_GAS = 0,       /* "StiffenedGas" */
-    VG_HPR_EOS_SHOCK_MIE_GRUNEISEN = 1, /* "ShockMieGruneisen" (not supported: VG_ERR_UNSUPPORTED) */
-    VG_HPR_EOS_GODUNOV_ROMENSKI = 2     /* "GodunovRomenski"   (not supported: VG_ERR_UNSUPPORTED) */
+    VG_HPR_EOS_SHOCK_MIE_GRUNEISEN = 1, /* "ShockMieGruneisen" */
+    VG_HPR_EOS_GODUNOV_ROMENSKI = 2     /* "GodunovRomenski" */
 };
 
 /* HPRMaterialParameters (HPR/hprmaterialparameters.h:10-15), the full
- * constructor argument list in its order, the EOS name as an enum.  The
- * stiffened-gas path reads eos, is_thermal, is_plastic (must be 0),
- * reference_density, reference_temperature, initial_temperature,
- * specific_heat_capacity, adiabatic_index, stiffening_parameter,
- * transverse_wave_speed, strain_dissipation_time,
- * thermal_impulse_relaxation_time and heat_wave_speed.  As in the reference,
+ * constructor argument list in its order, the EOS name as an enum.  Every
+ * path reads eos, is_thermal, is_plastic, reference_density,
+ * reference_temperature, initial_temperature, specific_heat_capacity,
+ * adiabatic_index, transverse_wave_speed, strain_dissipation_time,
+ * thermal_impulse_relaxation_time and heat_wave_speed; the stiffened gas adds
+ * stiffening_parameter; ShockMieGruneisen reference_pressure,
+ * reference_sound_speed, reference_gruneisen_coefficient and
+ * hugoniot_slope_coefficient; GodunovRomenski reference_sound_speed and
+ * alpha_parameter (its Grueneisen coefficient is adiabatic_index); the plastic
+ * relaxation (is_plastic != 0) elastic_plastic_transition_parameter (> 0) and
+ * power_law_index (hprsourceterms.cpp:7-39).  The adjoints take the stiffened
+ * gas without plastic relaxation only (VG_ERR_UNSUPPORTED otherwise).  As in the reference,
  * the two relaxation times are taken as given (the reference's
  * configureStrainDissipationTime / configureThermalImpulseRelaxationTime,
  * hprmaterialparameters.cpp:101-119, are helpers the caller applies). */

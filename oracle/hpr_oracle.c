@@ -72,31 +72,106 @@ static void gram_dev(const mat3 A, mat3 dev)
 
 static const orc_hpr_params *P_(const orc_phys *ph) { return (const orc_hpr_params *)ph->params; }
 
-/* HPRMieGruneisen, StiffenedGas branch (hprmiegruneisen.cpp:7-263):
- * Gamma = gamma - 1, p_ref = -p_inf, e_ref = p_inf / rho. */
-static double mg_gamma(const orc_hpr_params *m) { return m->gamma - 1.0; }
+/* HPRMieGruneisen (hprmiegruneisen.cpp:7-263): the Grueneisen coefficient, the
+ * reference pressure and reference internal energy of the three equations of
+ * state, and their density derivatives.  StiffenedGas: Gamma = gamma - 1,
+ * p_ref = -p_inf, e_ref = p_inf / rho.  The reference's
+ * computeGruneisenCoefficientDerivative has no GodunovRomenski branch (it ends
+ * without a return, :110-125); Gamma = gamma is constant there, so its
+ * derivative is 0 here. */
+static double mg_gamma_of(const orc_hpr_params *m, double rho)
+{
+    if (m->eos == 1) return m->gamma0 * (m->rho0 / rho);
+    if (m->eos == 2) return m->gamma;
+    return m->gamma - 1.0;
+}
+static double mg_pref(const orc_hpr_params *m, double rho)
+{
+    if (m->eos == 1) {
+        double c2 = m->c0 * m->c0;
+        if (rho > m->rho0)
+            return c2 * (((1.0 / m->rho0) - (1.0 / rho)) /
+                         pow(((1.0 / m->rho0) - (m->s_h * ((1.0 / m->rho0) - (1.0 / rho)))), 2.0));
+        return c2 * (rho - m->rho0);
+    }
+    if (m->eos == 2) {
+        double c2 = m->c0 * m->c0, co = pow((rho / m->rho0), m->alpha_gr);
+        return c2 * (rho / m->alpha_gr) * (co - 1.0) * co;
+    }
+    return -m->p_inf;
+}
+static double mg_eref(const orc_hpr_params *m, double rho)
+{
+    if (m->eos == 1) {
+        if (rho > m->rho0) return 0.5 * m->p_ref * ((1.0 / m->rho0) - (1.0 / rho));
+        return 0.0;
+    }
+    if (m->eos == 2) {
+        double c2 = m->c0 * m->c0, co = pow((rho / m->rho0), m->alpha_gr);
+        return (c2 / (2.0 * (m->alpha_gr * m->alpha_gr))) * ((co - 1.0) * (co - 1.0));
+    }
+    return m->p_inf / rho;
+}
+static double mg_gamma_d(const orc_hpr_params *m, double rho)
+{
+    if (m->eos == 1) return -m->gamma0 * (m->rho0 / (rho * rho));
+    return 0.0;
+}
+static double mg_pref_d(const orc_hpr_params *m, double rho)
+{
+    if (m->eos == 1) {
+        double c2 = m->c0 * m->c0;
+        if (rho > m->rho0)
+            return c2 * (m->rho0 * m->rho0) * (((m->s_h * (m->rho0 - rho)) - rho) / pow((m->s_h * (rho - m->rho0)) - rho, 3.0));
+        return c2;
+    }
+    if (m->eos == 2) {
+        double c2 = m->c0 * m->c0, co = pow((rho / m->rho0), m->alpha_gr);
+        return (c2 / m->alpha_gr) * co * (((1.0 + m->alpha_gr) * (co - 1.0)) + (m->alpha_gr * co));
+    }
+    return 0.0;
+}
+static double mg_eref_d(const orc_hpr_params *m, double rho)
+{
+    if (m->eos == 1) {
+        double c2 = m->c0 * m->c0;
+        if (rho > m->rho0) return -((rho - m->rho0) * m->rho0 * c2) / pow((m->s_h * (rho - m->rho0)) - rho, 3.0);
+        return 0.0;
+    }
+    if (m->eos == 2) {
+        double c2 = m->c0 * m->c0, co = pow((rho / m->rho0), m->alpha_gr);
+        return (c2 / (rho * m->alpha_gr)) * (co - 1.0) * co;
+    }
+    return -(m->p_inf / (rho * rho));
+}
+/* computeInternalEnergy, computePressure, computeTemperature and the derivatives
+ * (hprmiegruneisen.cpp:192-263). */
 static double mg_e(const orc_hpr_params *m, double rho, double p)
 {
-    return (m->p_inf / rho) + ((p - (-m->p_inf)) / (rho * mg_gamma(m)));
+    double G = mg_gamma_of(m, rho), pr = mg_pref(m, rho), er = mg_eref(m, rho);
+    return er + ((p - pr) / (rho * G));
 }
 static double mg_p(const orc_hpr_params *m, double rho, double e)
 {
-    return ((e - (m->p_inf / rho)) * rho * mg_gamma(m)) + (-m->p_inf);
+    double G = mg_gamma_of(m, rho), pr = mg_pref(m, rho), er = mg_eref(m, rho);
+    return ((e - er) * rho * G) + pr;
 }
 static double mg_T(const orc_hpr_params *m, double rho, double p)
 {
-    return m->t_ref + ((p - (-m->p_inf)) / (rho * mg_gamma(m) * m->cv));
+    double G = mg_gamma_of(m, rho), pr = mg_pref(m, rho);
+    return m->t_ref + ((p - pr) / (rho * G * m->cv));
 }
 static double mg_e_rho(const orc_hpr_params *m, double rho, double p)
 {
-    double G = mg_gamma(m), Gd = 0.0, prd = 0.0, erd = -(m->p_inf / (rho * rho));
-    return erd - (((prd * rho * G) + ((G + (rho * Gd)) * (p - (-m->p_inf)))) / ((rho * G) * (rho * G)));
+    double G = mg_gamma_of(m, rho), Gd = mg_gamma_d(m, rho), pr = mg_pref(m, rho), prd = mg_pref_d(m, rho),
+           erd = mg_eref_d(m, rho);
+    return erd - (((prd * rho * G) + ((G + (rho * Gd)) * (p - pr))) / ((rho * G) * (rho * G)));
 }
-static double mg_e_p(const orc_hpr_params *m, double rho) { return 1.0 / (rho * mg_gamma(m)); }
+static double mg_e_p(const orc_hpr_params *m, double rho) { return 1.0 / (rho * mg_gamma_of(m, rho)); }
 static double mg_T_rho(const orc_hpr_params *m, double rho, double p)
 {
-    double G = mg_gamma(m), Gd = 0.0, prd = 0.0;
-    return -((prd * rho * G) + ((G + (rho * Gd)) * (p - (-m->p_inf)))) / (((rho * G) * (rho * G)) / m->cv);
+    double G = mg_gamma_of(m, rho), Gd = mg_gamma_d(m, rho), pr = mg_pref(m, rho), prd = mg_pref_d(m, rho);
+    return -((prd * rho * G) + ((G + (rho * Gd)) * (p - pr))) / (((rho * G) * (rho * G)) / m->cv);
 }
 static double mg_T_p(const orc_hpr_params *m, double rho) { return mg_e_p(m, rho) / m->cv; }
 
@@ -359,6 +434,25 @@ static const double DEFAULT_STATE[NF] = {1.0, 1.0 / 1.4, 1, 0, 0, 0, 1, 0, 0, 0,
 /* HPRStateVector::computeSourceTermVector (hprstatevector.cpp:312-366) with
  * HPRSourceTerms::computeTheta1Reciprocal / computeTheta2Reciprocal
  * (hprsourceterms.cpp:7-39), non-plastic branch. */
+/* HPRSourceTerms::computeTheta1Reciprocal (hprsourceterms.cpp:7-30): 3 det(A)^(5/3)
+ * / (c_s^2 tau1), times (min(|sigma|, 1e8) / sigma_Y)^n when plastic, with
+ * |sigma| TensorAlgebra::computeSigmaNorm (tensoralgebra.cpp:276-283) of the shear
+ * stress -rho A^T E_A. */
+static double theta1_rcp(const orc_hpr_params *m, double rho, const mat3 A)
+{
+    double cs2 = m->cs * m->cs;
+    double base = 3.0 * (pow(det3(A), (5.0 / 3.0)) / (cs2 * m->tau1));
+    if (!m->is_plastic) return base;
+    mat3 sig;
+    shear(m, rho, A, sig, NULL);
+    double c1 = ((sig[0][0] - sig[1][1]) * (sig[0][0] - sig[1][1])) + ((sig[1][1] - sig[2][2]) * (sig[1][1] - sig[2][2])) +
+                ((sig[2][2] - sig[0][0]) * (sig[2][2] - sig[0][0]));
+    double c2 = (sig[0][1] * sig[0][1]) + (sig[1][2] * sig[1][2]) + (sig[2][0] * sig[2][0]);
+    double norm = sqrt((0.5 * c1) + (3.0 * c2));
+    norm = orc_smin(norm, pow(10.0, 8.0));
+    return base * pow(norm / m->sigma_y, m->n_pow);
+}
+
 static void source(const orc_hpr_params *m, const double *U, double *S)
 {
     mat3 A, EA;
@@ -366,8 +460,7 @@ static void source(const orc_hpr_params *m, const double *U, double *S)
     double rho = U[0];
     for (int k = 0; k < NF; k++) S[k] = 0.0;
     e_A(m, A, EA);
-    double cs2 = m->cs * m->cs;
-    double th1 = 3.0 * (pow(det3(A), (5.0 / 3.0)) / (cs2 * m->tau1));
+    double th1 = theta1_rcp(m, rho, A);
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) S[2 + 3 * i + j] = -EA[i][j] * th1;
     if (m->is_thermal) {
@@ -454,5 +547,25 @@ void orc_hpr_units(int n, double bias, int limiter, double dx, double ts, const 
         *o++ = acoustic_speed(mp, b, 0);
         *o++ = acoustic_speed(mp, b, 1);
         *o++ = wave_speed(&ph, b, 2);
+    }
+}
+
+void orc_hpr_eos_units(int n, const orc_hpr_params *mp, const double *states, double *out)
+{
+    for (int r = 0; r < n; r++) {
+        const double *st = states + 11 * (size_t)r;
+        double *o = out + 8 * (size_t)r;
+        double rho = st[0], p = st[1];
+        mat3 A;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) A[i][j] = st[2 + 3 * i + j];
+        o[0] = mg_e(mp, rho, p);
+        o[1] = mg_p(mp, rho, o[0]);
+        o[2] = mg_T(mp, rho, p);
+        o[3] = mg_e_rho(mp, rho, p);
+        o[4] = mg_e_p(mp, rho);
+        o[5] = mg_T_rho(mp, rho, p);
+        o[6] = mg_T_p(mp, rho);
+        o[7] = theta1_rcp(mp, rho, A);
     }
 }

@@ -81,14 +81,24 @@ void orc_mhd_sweep(int order, int dim, int dir, int ny, int nx, double dx, doubl
 void orc_mhd_units(int n, double bias, int limiter, double dx, double ts, const orc_mhd_params *mp,
                    const double *states, double *out);
 
-/* HPRMaterialParameters (HPR/hprmaterialparameters.h), stiffened-gas EOS: the
- * fields the solver reads.  tau1/tau2 are the strain-dissipation and thermal
- * relaxation times as configureStrainDissipationTime /
- * configureThermalImpulseRelaxationTime set them (hprmaterialparameters.cpp:101-119). */
+/* HPRMaterialParameters (HPR/hprmaterialparameters.h): the fields the solver
+ * reads.  tau1/tau2 are the strain-dissipation and thermal relaxation times as
+ * configureStrainDissipationTime / configureThermalImpulseRelaxationTime set
+ * them (hprmaterialparameters.cpp:101-119).  eos: 0 StiffenedGas, 1
+ * ShockMieGruneisen, 2 GodunovRomenski; p_ref, c0, gamma0, s_h, alpha_gr are
+ * the reference pressure, reference sound speed, reference Grueneisen
+ * coefficient, Hugoniot slope and alpha parameter those branches read; sigma_y
+ * and n_pow the elastic-plastic transition parameter and power-law index of the
+ * plastic relaxation (hprsourceterms.cpp:12-22). */
 typedef struct {
     int32_t is_thermal, is_plastic;
     double rho0, t_ref, t0, cv, gamma, p_inf, cs, tau1, tau2, alpha;
+    int32_t eos, reserved;
+    double p_ref, c0, gamma0, s_h, alpha_gr, sigma_y, n_pow;
 } orc_hpr_params;
+/* HPRMieGruneisen records (rho, p, A: 11 doubles) -> e, p(rho, e), T, de/drho,
+ * de/dp, dT/drho, dT/dp, theta1^-1 (8 doubles). */
+void orc_hpr_eos_units(int n, const orc_hpr_params *mp, const double *states, double *out);
 
 /* HPRSecondOrderSolver::solve / solve2D with `subcycles` RK4 source sub-cycles on
  * each side of the SLIC step.  Primitive order (rho, p, A row-major, u, v, w, J). */
@@ -143,4 +153,19 @@ double orc_rgfm_circulation(int ny, int nx, double dx, double dy, const double *
 #ifdef __cplusplus
 }
 #endif
+/* Two-material diffuse-interface Euler models (multiphysics_oracle.c).  kind 0:
+ * EulerMultiphysicsStateVector, 8 primitives per cell (alpha1, u, v, w, rho1,
+ * p1, rho2, p2); kind 1: EulerReducedStateVector, 7 (alpha1, u, v, w, rho1,
+ * rho2, p).  order 2 reinitialises the volume fraction after every
+ * reinit_every-th step (0: never); order 1 never does. */
+int orc_mp_solve(int kind, int order, int dim, int ny, int nx, double dx, double cfl, double t_final, int max_steps,
+                 double bias, int limiter, int bc, int reinit_every, const orc_euler_params *m1,
+                 const orc_euler_params *m2, const double *prim_in, double *prim_out, double *dt_log, int dt_cap);
+void orc_mp_sweep(int kind, int order, int dim, int dir, int ny, int nx, double dx, double ts, double bias,
+                  int limiter, int bc, const orc_euler_params *m1, const orc_euler_params *m2, const double *prim_in,
+                  double *prim_out);
+void orc_mp_reinit(int kind, int dim, int ny, int nx, double *prim);
+void orc_mp_units(int kind, int n, const orc_euler_params *m1, const orc_euler_params *m2, double bias, int limiter,
+                  double dx, double ts, const double *states, double *out);
+
 #endif

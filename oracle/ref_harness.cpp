@@ -30,6 +30,12 @@
 #include "MHD/mhdtests.h"
 #include "HPR/hprtests.h"
 #include "RGFM/eulerrgfmtests.h"
+#include "HPR/hprmiegruneisen.h"
+#include "HPR/hprsourceterms.h"
+#include "Elasticity/hyperelasticmaterialparameters.h"
+#include "Solvers/multiphysicssolvers.h"
+#include "Euler/Multiphysics/eulermultiphysicstests.h"
+#include "Euler/Multiphysics/eulerreducedtests.h"
 
 #include <cstdio>
 #include <cstdlib>
@@ -530,6 +536,18 @@ HPRMaterialParameters hprParams(char **a)
     return mp;
 }
 
+// HPR material parameters as 26 doubles: the EOS code (0 StiffenedGas,
+// 1 ShockMieGruneisen, 2 GodunovRomenski), is_thermal, is_plastic, then the 23
+// remaining arguments of the full constructor in order (hprmaterialparameters.h:12-16).
+static const char *kHprEosNames[3] = {"StiffenedGas", "ShockMieGruneisen", "GodunovRomenski"};
+
+HPRMaterialParameters hprParamsVec(const std::vector<double> &v)
+{
+    return HPRMaterialParameters(kHprEosNames[(int)v[0]], v[1] != 0.0, v[2] != 0.0, v[3], v[4], v[5], v[6], v[7], v[8],
+                                 v[9], v[10], v[11], v[12], v[13], v[14], v[15], v[16], v[17], v[18], v[19], v[20],
+                                 v[21], v[22], v[23], v[24], v[25]);
+}
+
 HPRStateVector hprStateFrom(const double *p)
 {
     std::vector<std::vector<double> > A(3, std::vector<double>(3));
@@ -555,9 +573,25 @@ void putHprState(HPRStateVector s, double *p)
 
 // hpr_run order dim ny nx dx cfl tfinal max_steps bias limiter subcycles <12 params> in out_state out_dt check_api
 // Mirrors HPRSecondOrderSolver::solve / solve2D (hprsecondordersolver.cpp:314-352, 502-546).
+int hprRunWith(HPRMaterialParameters mp, int argc, char **argv, int rest);
+
 int hprRun(int argc, char **argv)
 {
     if (argc < 29) { std::fprintf(stderr, "hpr_run: too few arguments\n"); return 2; }
+    return hprRunWith(hprParams(argv + 13), argc, argv, 25);
+}
+
+// hpr_run_p: as hpr_run with the 12 parameters replaced by a file of 26 (hprParamsVec).
+int hprRunP(int argc, char **argv)
+{
+    if (argc < 18) { std::fprintf(stderr, "hpr_run_p: too few arguments\n"); return 2; }
+    return hprRunWith(hprParamsVec(readDoubles(argv[13], 26)), argc, argv, 14);
+}
+
+// argv[rest .. rest + 3]: in out_state out_dt check_api
+int hprRunWith(HPRMaterialParameters mp, int argc, char **argv, int rest)
+{
+    (void)argc;
     int order = std::atoi(argv[2]);
     int dim = std::atoi(argv[3]);
     int ny = std::atoi(argv[4]);
@@ -569,11 +603,10 @@ int hprRun(int argc, char **argv)
     double bias = std::atof(argv[10]);
     int limiter = std::atoi(argv[11]);
     int sub = std::atoi(argv[12]);
-    HPRMaterialParameters mp = hprParams(argv + 13);
-    const char *inPath = argv[25];
-    const char *outState = argv[26];
-    const char *outDt = argv[27];
-    int checkApi = std::atoi(argv[28]);
+    const char *inPath = argv[rest];
+    const char *outState = argv[rest + 1];
+    const char *outDt = argv[rest + 2];
+    int checkApi = std::atoi(argv[rest + 3]);
     const int nf = 17;
     std::vector<double> in = readDoubles(inPath, (size_t)ny * nx * nf);
     std::vector<double> dts;
@@ -671,16 +704,31 @@ int hprRun(int argc, char **argv)
 
 // hpr_units n bias limiter dx ts <12 params> in out: records of five states (85 doubles) -> 156 doubles,
 // the layout of orc_hpr_units.
+int hprUnitsWith(HPRMaterialParameters mp, int argc, char **argv, int rest);
+
 int hprUnits(int argc, char **argv)
 {
     if (argc < 20) { std::fprintf(stderr, "hpr_units: too few arguments\n"); return 2; }
+    return hprUnitsWith(hprParams(argv + 7), argc, argv, 19);
+}
+
+// hpr_units_p: as hpr_units with a parameter file of 26 doubles.
+int hprUnitsP(int argc, char **argv)
+{
+    if (argc < 10) { std::fprintf(stderr, "hpr_units_p: too few arguments\n"); return 2; }
+    return hprUnitsWith(hprParamsVec(readDoubles(argv[7], 26)), argc, argv, 8);
+}
+
+// argv[rest], argv[rest + 1]: in out
+int hprUnitsWith(HPRMaterialParameters mp, int argc, char **argv, int rest)
+{
+    (void)argc;
     int n = std::atoi(argv[2]);
     double bias = std::atof(argv[3]);
     int limiter = std::atoi(argv[4]);
     double dx = std::atof(argv[5]);
     double ts = std::atof(argv[6]);
-    HPRMaterialParameters mp = hprParams(argv + 7);
-    std::vector<double> in = readDoubles(argv[19], (size_t)n * 85);
+    std::vector<double> in = readDoubles(argv[rest], (size_t)n * 85);
     const int rec = 156, nf = 17;
     std::vector<double> out((size_t)n * rec);
     for (int r = 0; r < n; r++)
@@ -719,8 +767,119 @@ int hprUnits(int argc, char **argv)
             *o++ = HPRSolvers::computeMaximumWaveSpeed2D(one2, mp);
         }
     }
-    writeDoubles(argv[20], out);
+    writeDoubles(argv[rest + 1], out);
     return 0;
+}
+
+// hpr_eos_units n params in out: per record (rho, p, A row-major: 11 doubles) the
+// HPRMieGruneisen functions (hprmiegruneisen.cpp:192-263) -- e(rho, p), p(rho, e),
+// T(rho, p), de/drho, de/dp, dT/drho, dT/dp -- and computeTheta1Reciprocal
+// (hprsourceterms.cpp:7-30): 8 doubles.  For GodunovRomenski de/drho and dT/drho
+// are NaN: they call computeGruneisenCoefficientDerivative, which has no
+// GodunovRomenski branch and ends without a return (hprmiegruneisen.cpp:110-125),
+// undefined behaviour that crashes this build.
+int hprEosUnits(int argc, char **argv)
+{
+    if (argc < 6) { std::fprintf(stderr, "hpr_eos_units: too few arguments\n"); return 2; }
+    int n = std::atoi(argv[2]);
+    HPRMaterialParameters mp = hprParamsVec(readDoubles(argv[3], 26));
+    std::vector<double> in = readDoubles(argv[4], (size_t)n * 11);
+    std::vector<double> out((size_t)n * 8);
+    for (int r = 0; r < n; r++)
+    {
+        const double *s = &in[(size_t)r * 11];
+        double rho = s[0], p = s[1];
+        std::vector<std::vector<double> > A(3, std::vector<double>(3));
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) A[i][j] = s[2 + 3 * i + j];
+        double *o = &out[(size_t)r * 8];
+        o[0] = HPRMieGruneisen::computeInternalEnergy(rho, p, mp);
+        o[1] = HPRMieGruneisen::computePressure(rho, o[0], mp);
+        o[2] = HPRMieGruneisen::computeTemperature(rho, p, mp);
+        const bool gr = mp.getEquationOfState() == "GodunovRomenski";
+        o[3] = gr ? std::nan("") : HPRMieGruneisen::computeInternalEnergyDerivativeDensity(rho, p, mp);
+        o[4] = HPRMieGruneisen::computeInternalEnergyDerivativePressure(rho, mp);
+        o[5] = gr ? std::nan("") : HPRMieGruneisen::computeTemperatureDerivativeDensity(rho, p, mp);
+        o[6] = HPRMieGruneisen::computeTemperatureDerivativePressure(rho, mp);
+        o[7] = HPRSourceTerms::computeTheta1Reciprocal(rho, A, mp);
+    }
+    writeDoubles(argv[5], out);
+    return 0;
+}
+
+// hpr_driver <barton|zhang> cellCount subcycles max_steps out_params out_in out_state out_dt:
+// the initial condition and material parameters of HPRTests::solveBartonTest1 /
+// solveZhangElasticPlasticTest (hprtests.cpp:59-134), built with the reference's
+// own constructors (the hyperelastic entropy constructor,
+// hprstatevector.cpp:34-51), then the reference's solve loop for max_steps
+// steps (0: to the driver's final time).
+int hprDriver(int argc, char **argv)
+{
+    if (argc < 10) { std::fprintf(stderr, "hpr_driver: too few arguments\n"); return 2; }
+    const std::string which = argv[2];
+    const int cellCount = std::atoi(argv[3]);
+    std::vector<HPRStateVector> cells(cellCount);
+    HPRMaterialParameters mp;
+    std::vector<double> pv;
+    double cellSpacing, finalTime;
+    if (which == "barton")
+    {
+        HyperelasticMaterialParameters hp(8.9, 4.6, 2.1, 3.94 * pow(10.0, -4.0), 300.0, 1.0, 3.0, 2.0);
+        pv = {2.0, 0.0, 0.0, 8.31445985, 8.9, 0.0, 300.0, 300.0, 3.94 * pow(10.0, -4.0), 1.0, 0.0, 3.909, 0.0, 0.0, 0.0,
+              2.1, pow(10.0, 8.0), 0.0, 0.0, 0.75, 0.0, 1.0, 8.9 * 3.909 * sqrt(4.0 * pow(10.0, -4.0) / 300.0), 1.0, 3.0,
+              2.0};
+        mp = hprParamsVec(pv);
+        mp.configureThermalImpulseRelaxationTime(4.0 * pow(10.0, -8.0));
+        pv[17] = mp.getThermalImpulseRelaxationTime();
+        cellSpacing = 1.0 / cellCount;
+        finalTime = 0.06;
+        std::vector<std::vector<double> > L(3, std::vector<double>(3, 0.0)), R(3, std::vector<double>(3, 0.0));
+        L[0][0] = 0.98; L[1][0] = 0.02; L[1][1] = 1.0; L[1][2] = 0.1; L[2][2] = 1.0;
+        R[0][0] = 1.0; R[1][1] = 1.0; R[1][2] = 0.1; R[2][2] = 1.0;
+        std::vector<std::vector<double> > AL = MatrixAlgebra::computeInverseMatrix(L), AR = MatrixAlgebra::computeInverseMatrix(R);
+        for (int i = 0; i < cellCount; i++)
+            cells[i] = (i <= (0.5 * cellCount)) ? HPRStateVector(0.0, 0.5, 1.0, AL, 0.001, hp, mp)
+                                                : HPRStateVector(0.0, 0.0, 0.0, AR, 0.0, hp, mp);
+    }
+    else if (which == "zhang")
+    {
+        HyperelasticMaterialParameters hp(2.71, 6.22, 3.16, 9.0 * pow(10.0, -4.0), 300.0, 1.0, 3.577, 2.088);
+        pv = {2.0, 0.0, 1.0, 8.31445985, 2.71, 73.0, 300.0, 300.0, 9.0 * pow(10.0, -4.0), 1.0, 0.0, 5.037, 1.0, 1.338,
+              0.0, 3.16, pow(10.0, -9.0), 0.0, 24.8 * pow(10.0, 9.0), 0.75, 0.2976 * pow(10.0, 9.0), 1.338, 2.0, 1.0,
+              3.577, 2.088};
+        mp = hprParamsVec(pv);
+        cellSpacing = 10.0 / cellCount;
+        finalTime = 0.6;
+        std::vector<std::vector<double> > I = MatrixAlgebra::computeIdentityMatrix(3);
+        for (int i = 0; i < cellCount; i++)
+            cells[i] = (i <= (0.5 * cellCount)) ? HPRStateVector(0.75, 0.0, 0.0, I, 0.0, hp, mp)
+                                                : HPRStateVector(-0.75, 0.0, 0.0, I, 0.0, hp, mp);
+    }
+    else
+    {
+        std::fprintf(stderr, "hpr_driver: unknown problem %s\n", which.c_str());
+        return 2;
+    }
+    const int nf = 17;
+    std::vector<double> in((size_t)cellCount * nf);
+    for (int i = 0; i < cellCount; i++) putHprState(cells[i], &in[(size_t)i * nf]);
+    writeDoubles(argv[6], pv);
+    writeDoubles(argv[7], in);
+    std::fprintf(stderr, "DX %.17g TFINAL %.17g\n", cellSpacing, finalTime);
+    // max_steps < 0: the initial condition only.  The GodunovRomenski wave speed
+    // reaches the undefined computeGruneisenCoefficientDerivative branch (see
+    // hpr_eos_units), so the reference cannot run these two drivers.
+    if (std::atoi(argv[5]) < 0) return 0;
+    // the driver's solve (CFL 0.8, superbee, no bias) as hpr_run's 1D loop
+    char dxs[64], ts[64], ms[32];
+    std::snprintf(dxs, sizeof(dxs), "%.17g", cellSpacing);
+    std::snprintf(ts, sizeof(ts), "%.17g", finalTime);
+    std::snprintf(ms, sizeof(ms), "%d", std::atoi(argv[5]));
+    char a0[] = "hpr_run", a1[] = "2", a2[] = "1", a3[] = "1", cfl[] = "0.8", bias[] = "0", lim[] = "0", chk[] = "0";
+    char ncell[32];
+    std::snprintf(ncell, sizeof(ncell), "%d", cellCount);
+    char *args[] = {argv[0], a0, a1, a2, a3, ncell, dxs, cfl, ts, ms, bias, lim, argv[4], argv[7], argv[8], argv[9], chk};
+    return hprRunWith(mp, 17, args, 13);
 }
 
 } // namespace
@@ -974,6 +1133,251 @@ int datOut(int argc, char **argv)
     return 2;
 }
 
+
+// ---- two-material diffuse-interface Euler models ------------------------------
+// State vectors are built from and read back to primitive records through the
+// reference's own setPrimitiveVariableVector / computePrimitiveVariableVector.
+template <class S>
+S mpFrom(const double *p, int np)
+{
+    S s;
+    s.setPrimitiveVariableVector(std::vector<double>(p, p + np));
+    return s;
+}
+
+template <class S>
+void mpPut(S s, double *p)
+{
+    putVec(s.computePrimitiveVariableVector(), p);
+}
+
+// mp_run kind order dim ny nx dx cfl tfinal max_steps bias limiter reinit
+//        m1(g pinf cv ef tign krate) m2(6) in out_state out_dt check_api
+// The loops mirror SecondOrderSolver::solve[2D] for these states
+// (secondordersolver.cpp:316-390, 426-517: reinitialisation after every
+// reinit-th step) and FirstOrderSolver::solve[2D] (firstordersolver.cpp:463-640:
+// none), with a step cap; check_api compares with the reference's own solve.
+template <class S>
+int mpRunT(int argc, char **argv, int np)
+{
+    int order = std::atoi(argv[3]);
+    int dim = std::atoi(argv[4]);
+    int ny = std::atoi(argv[5]);
+    int nx = std::atoi(argv[6]);
+    double dx = std::atof(argv[7]);
+    double cfl = std::atof(argv[8]);
+    double tfinal = std::strcmp(argv[9], "inf") == 0 ? std::numeric_limits<double>::infinity() : std::atof(argv[9]);
+    int maxSteps = std::atoi(argv[10]);
+    double bias = std::atof(argv[11]);
+    int limiter = std::atoi(argv[12]);
+    int reinit = std::atoi(argv[13]);
+    EulerMaterialParameters m1 = eulerParams(argv + 14);
+    EulerMaterialParameters m2 = eulerParams(argv + 20);
+    const char *inPath = argv[26];
+    const char *outState = argv[27];
+    const char *outDt = argv[28];
+    int checkApi = std::atoi(argv[29]);
+    int ghost = (order == 1) ? 1 : 2;
+    std::vector<double> in = readDoubles(inPath, (size_t)ny * nx * np);
+    std::vector<double> out((size_t)ny * nx * np);
+    std::vector<double> dts;
+    MuteCout mute;
+    if (dim == 1)
+    {
+        std::vector<S> cells(nx);
+        for (int j = 0; j < nx; j++) cells[j] = mpFrom<S>(&in[(size_t)j * np], np);
+        std::vector<S> initial = cells;
+        double t = 0.0;
+        int it = 0;
+        while (t < tfinal && (maxSteps == 0 || it < maxSteps))
+        {
+            std::vector<S> wb = Solvers::insertBoundaryCells(cells, ghost);
+            double dt = Solvers::computeStableTimeStep(wb, dx, cfl, t, tfinal, it, m1, m2);
+            if (order == 1)
+                FirstOrderSolver::computeFORCETimeStep(cells, wb, dx, dt, m1, m2);
+            else
+                SecondOrderSolver::computeSLICTimeStep(cells, wb, dx, dt, bias, limiter, m1, m2);
+            t += dt;
+            it += 1;
+            dts.push_back(dt);
+            if (order == 2 && reinit != 0 && (it % reinit) == 0) MultiphysicsSolvers::reinitialiseVolumeFraction(cells, m1, m2);
+        }
+        for (int j = 0; j < nx; j++) mpPut(cells[j], &out[(size_t)j * np]);
+        if (checkApi && maxSteps == 0)
+        {
+            std::vector<S> api = (order == 1) ? FirstOrderSolver::solve(initial, dx, cfl, tfinal, 0, m1, m2)
+                                              : SecondOrderSolver::solve(initial, dx, cfl, tfinal, bias, limiter, 0, reinit, m1, m2);
+            bool match = true;
+            std::vector<double> p(np);
+            for (int j = 0; j < nx; j++)
+            {
+                mpPut(api[j], p.data());
+                for (int f = 0; f < np; f++) match = match && sameBits(p[f], out[(size_t)j * np + f]);
+            }
+            std::fprintf(stderr, "API_MATCH %d\n", match ? 1 : 0);
+        }
+    }
+    else
+    {
+        std::vector<std::vector<S> > cells(ny, std::vector<S>(nx));
+        for (int i = 0; i < ny; i++)
+            for (int j = 0; j < nx; j++) cells[i][j] = mpFrom<S>(&in[((size_t)i * nx + j) * np], np);
+        std::vector<std::vector<S> > initial = cells;
+        double t = 0.0;
+        int it = 0;
+        while (t < tfinal && (maxSteps == 0 || it < maxSteps))
+        {
+            std::vector<std::vector<S> > wb = Solvers::insertBoundaryCells2D(cells, ghost);
+            double dt = Solvers::computeStableTimeStep2D(wb, dx, cfl, t, tfinal, it, m1, m2);
+            if (order == 1)
+            {
+                FirstOrderSolver::computeXFORCETimeStep2D(cells, wb, dx, 0.5 * dt, m1, m2);
+                wb = Solvers::insertBoundaryCells2D(cells, ghost);
+                FirstOrderSolver::computeYFORCETimeStep2D(cells, wb, dx, dt, m1, m2);
+                wb = Solvers::insertBoundaryCells2D(cells, ghost);
+                FirstOrderSolver::computeXFORCETimeStep2D(cells, wb, dx, 0.5 * dt, m1, m2);
+            }
+            else
+            {
+                SecondOrderSolver::computeXSLICTimeStep2D(cells, wb, dx, 0.5 * dt, bias, limiter, m1, m2);
+                wb = Solvers::insertBoundaryCells2D(cells, ghost);
+                SecondOrderSolver::computeYSLICTimeStep2D(cells, wb, dx, dt, bias, limiter, m1, m2);
+                wb = Solvers::insertBoundaryCells2D(cells, ghost);
+                SecondOrderSolver::computeXSLICTimeStep2D(cells, wb, dx, 0.5 * dt, bias, limiter, m1, m2);
+            }
+            t += dt;
+            it += 1;
+            dts.push_back(dt);
+            if (order == 2 && reinit != 0 && (it % reinit) == 0) MultiphysicsSolvers::reinitialiseVolumeFraction(cells, m1, m2);
+        }
+        for (int i = 0; i < ny; i++)
+            for (int j = 0; j < nx; j++) mpPut(cells[i][j], &out[((size_t)i * nx + j) * np]);
+        if (checkApi && maxSteps == 0)
+        {
+            std::vector<std::vector<S> > api = (order == 1) ? FirstOrderSolver::solve2D(initial, dx, cfl, tfinal, 0, m1, m2)
+                                                            : SecondOrderSolver::solve2D(initial, dx, cfl, tfinal, bias, limiter, 0, reinit, m1, m2);
+            bool match = true;
+            std::vector<double> p(np);
+            for (int i = 0; i < ny; i++)
+                for (int j = 0; j < nx; j++)
+                {
+                    mpPut(api[i][j], p.data());
+                    for (int f = 0; f < np; f++) match = match && sameBits(p[f], out[((size_t)i * nx + j) * np + f]);
+                }
+            std::fprintf(stderr, "API_MATCH %d\n", match ? 1 : 0);
+        }
+    }
+    std::fprintf(stderr, "STEPS %zu\n", dts.size());
+    writeDoubles(outState, out);
+    writeDoubles(outDt, dts);
+    return 0;
+}
+
+int mpRun(int argc, char **argv)
+{
+    if (argc < 30) { std::fprintf(stderr, "mp_run: too few arguments\n"); return 2; }
+    return std::atoi(argv[2]) == 0 ? mpRunT<EulerMultiphysicsStateVector>(argc, argv, 8)
+                                   : mpRunT<EulerReducedStateVector>(argc, argv, 7);
+}
+
+// mp_units kind n bias limiter dx ts m1(6) m2(6) in out: the record layout of
+// orc_mp_units (four primitive states in; cons, prim(cons), Fx, Fy, wave speeds
+// 1D/2D, slope, evolveX sides 0/1, FORCEx, SLICx, SLICy out).
+template <class S>
+int mpUnitsT(int argc, char **argv, int np, int nf)
+{
+    int n = std::atoi(argv[3]);
+    double bias = std::atof(argv[4]);
+    int limiter = std::atoi(argv[5]);
+    double dx = std::atof(argv[6]);
+    double ts = std::atof(argv[7]);
+    EulerMaterialParameters m1 = eulerParams(argv + 8);
+    EulerMaterialParameters m2 = eulerParams(argv + 14);
+    std::vector<double> in = readDoubles(argv[20], (size_t)n * 4 * np);
+    const int rec = 7 * nf + 3 * np + 2;
+    std::vector<double> out((size_t)n * rec);
+    for (int r = 0; r < n; r++)
+    {
+        const double *q = &in[(size_t)r * 4 * np];
+        S a = mpFrom<S>(q, np), b = mpFrom<S>(q + np, np), c = mpFrom<S>(q + 2 * np, np), d = mpFrom<S>(q + 3 * np, np);
+        double *o = &out[(size_t)r * rec];
+        std::vector<double> U = S(b).computeConservedVariableVector(m1, m2);
+        putVec(U, o);
+        o += nf;
+        S back;
+        back.setConservedVariableVector(U, m1, m2);
+        mpPut(back, o);
+        o += np;
+        putVec(S::computeXFluxVector(U, m1, m2), o);
+        o += nf;
+        putVec(S::computeYFluxVector(U, m1, m2), o);
+        o += nf;
+        std::vector<S> one(1, b);
+        std::vector<std::vector<S> > one2(1, std::vector<S>(1, b));
+        *o++ = Solvers::computeMaximumWaveSpeed(one, m1, m2);
+        *o++ = Solvers::computeMaximumWaveSpeed2D(one2, m1, m2);
+        putVec(SlopeLimiters::computeSlopeVector(a, b, c, bias, limiter, m1, m2), o);
+        o += nf;
+        mpPut(Solvers::evolveStateByHalfXTimeStep(a, b, c, dx, ts, bias, limiter, 0, m1, m2), o);
+        o += np;
+        mpPut(Solvers::evolveStateByHalfXTimeStep(a, b, c, dx, ts, bias, limiter, 1, m1, m2), o);
+        o += np;
+        putVec(FirstOrderSolver::computeXFORCEFlux(b, c, dx, ts, m1, m2), o);
+        o += nf;
+        putVec(SecondOrderSolver::computeXSLICFlux(a, b, c, d, dx, ts, bias, limiter, m1, m2), o);
+        o += nf;
+        putVec(SecondOrderSolver::computeYSLICFlux(a, b, c, d, dx, ts, bias, limiter, m1, m2), o);
+    }
+    writeDoubles(argv[21], out);
+    return 0;
+}
+
+int mpUnits(int argc, char **argv)
+{
+    if (argc < 22) { std::fprintf(stderr, "mp_units: too few arguments\n"); return 2; }
+    return std::atoi(argv[2]) == 0 ? mpUnitsT<EulerMultiphysicsStateVector>(argc, argv, 8, 9)
+                                   : mpUnitsT<EulerReducedStateVector>(argc, argv, 7, 8);
+}
+
+// mp_reinit kind dim ny nx m1(6) m2(6) in out: one
+// MultiphysicsSolvers::reinitialiseVolumeFraction on the grid.
+template <class S>
+int mpReinitT(int argc, char **argv, int np)
+{
+    int dim = std::atoi(argv[3]);
+    int ny = std::atoi(argv[4]);
+    int nx = std::atoi(argv[5]);
+    EulerMaterialParameters m1 = eulerParams(argv + 6);
+    EulerMaterialParameters m2 = eulerParams(argv + 12);
+    std::vector<double> in = readDoubles(argv[18], (size_t)ny * nx * np);
+    std::vector<double> out((size_t)ny * nx * np);
+    if (dim == 1)
+    {
+        std::vector<S> cells(nx);
+        for (int j = 0; j < nx; j++) cells[j] = mpFrom<S>(&in[(size_t)j * np], np);
+        MultiphysicsSolvers::reinitialiseVolumeFraction(cells, m1, m2);
+        for (int j = 0; j < nx; j++) mpPut(cells[j], &out[(size_t)j * np]);
+    }
+    else
+    {
+        std::vector<std::vector<S> > cells(ny, std::vector<S>(nx));
+        for (int i = 0; i < ny; i++)
+            for (int j = 0; j < nx; j++) cells[i][j] = mpFrom<S>(&in[((size_t)i * nx + j) * np], np);
+        MultiphysicsSolvers::reinitialiseVolumeFraction(cells, m1, m2);
+        for (int i = 0; i < ny; i++)
+            for (int j = 0; j < nx; j++) mpPut(cells[i][j], &out[((size_t)i * nx + j) * np]);
+    }
+    writeDoubles(argv[19], out);
+    return 0;
+}
+
+int mpReinit(int argc, char **argv)
+{
+    if (argc < 20) { std::fprintf(stderr, "mp_reinit: too few arguments\n"); return 2; }
+    return std::atoi(argv[2]) == 0 ? mpReinitT<EulerMultiphysicsStateVector>(argc, argv, 8)
+                                   : mpReinitT<EulerReducedStateVector>(argc, argv, 7);
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) { std::fprintf(stderr, "usage: ref_harness <mode> ...\n"); return 2; }
@@ -985,9 +1389,16 @@ int main(int argc, char **argv)
     if (mode == "mhd_units") return mhdUnits(argc, argv);
     if (mode == "hpr_run") return hprRun(argc, argv);
     if (mode == "hpr_units") return hprUnits(argc, argv);
+    if (mode == "hpr_run_p") return hprRunP(argc, argv);
+    if (mode == "hpr_units_p") return hprUnitsP(argc, argv);
+    if (mode == "hpr_eos_units") return hprEosUnits(argc, argv);
+    if (mode == "hpr_driver") return hprDriver(argc, argv);
     if (mode == "rgfm_run") return rgfmRun(argc, argv);
     if (mode == "hllc_units") return hllcUnits(argc, argv);
     if (mode == "dat") return datOut(argc, argv);
+    if (mode == "mp_run") return mpRun(argc, argv);
+    if (mode == "mp_units") return mpUnits(argc, argv);
+    if (mode == "mp_reinit") return mpReinit(argc, argv);
     std::fprintf(stderr, "unknown mode %s\n", mode.c_str());
     return 2;
 }

@@ -153,7 +153,7 @@ static int ghost_index(int k, int n, int bc)
 
 static double *ghosted_2d(const orc_phys *ph, int ny, int nx, int g, int bc, const double *w)
 {
-    const int nf = ph->nf;
+    const int nf = orc_np(ph);
     int GY = ny + 2 * g, GX = nx + 2 * g;
     double *out = (double *)malloc(sizeof(double) * nf * (size_t)GY * GX);
     for (int i = 0; i < GY; i++) {
@@ -174,7 +174,7 @@ static double *ghosted_2d(const orc_phys *ph, int ny, int nx, int g, int bc, con
 
 static double *ghosted_1d(const orc_phys *ph, int nx, int g, int bc, const double *w)
 {
-    const int nf = ph->nf;
+    const int nf = orc_np(ph);
     double *out = (double *)malloc(sizeof(double) * nf * (size_t)(nx + 2 * g));
     for (int j = 0; j < nx + 2 * g; j++)
         memcpy(out + nf * (size_t)j, w + nf * (size_t)ghost_index(j - g, nx, bc), sizeof(double) * nf);
@@ -186,7 +186,7 @@ static double *ghosted_1d(const orc_phys *ph, int nx, int g, int bc, const doubl
  * repeat interior cells, so only the 2D transmissive corners add a candidate. */
 double orc_ghosted_max(const orc_phys *ph, int dim, int ny, int nx, int bc, const double *prim)
 {
-    const int nf = ph->nf;
+    const int nf = orc_np(ph);
     double m = 0.0;
     size_t cells = (size_t)(dim == 1 ? 1 : ny) * nx;
     for (size_t c = 0; c < cells; c++) {
@@ -203,7 +203,7 @@ double orc_ghosted_max(const orc_phys *ph, int dim, int ny, int nx, int bc, cons
 void orc_sweep(const orc_phys *ph, int order, int dim, int dir, int ny, int nx, double dx, double ts, double bias,
                int limiter, int bc, const double *prim_in, double *prim_out)
 {
-    const int nf = ph->nf;
+    const int nf = orc_np(ph);   /* primitive stride */
     int g = (order == 1) ? 1 : 2;
     if (dim == 1) {
         double *wb = ghosted_1d(ph, nx, g, bc, prim_in);
@@ -364,7 +364,7 @@ int orc_solve_sub(orc_phys *ph, int order, int dim, int ny, int nx, double dx, d
                   int max_steps, double bias, int limiter, int bc, int subcycles, const double *prim_in,
                   double *prim_out, double *dt_log, int dt_cap)
 {
-    const int nf = ph->nf;
+    const int nf = orc_np(ph);   /* primitive stride */
     const int nsub = (order == 2 && ph->source) ? subcycles : 0;
     size_t cells = (size_t)(dim == 1 ? 1 : ny) * nx;
     double *a = (double *)malloc(sizeof(double) * nf * cells);
@@ -396,6 +396,7 @@ int orc_solve_sub(orc_phys *ph, int order, int dim, int ny, int nx, double dx, d
         if (dt_log && it < dt_cap) dt_log[it] = dt;
         t += dt;
         it += 1;
+        if (ph->after_step) ph->after_step(ph, order, dim, ny, nx, it, a);
     }
     memcpy(prim_out, a, sizeof(double) * nf * cells);
     free(a);

@@ -26,7 +26,18 @@ struct orc_phys {
      * given the old one w_in (HPR without heat conduction keeps its own J) */
     void (*forcing_finish)(const orc_phys *ph, const double *w_in, double *w_out);
     void *params;
+    /* primitive fields per cell when fewer than the conserved ones (0: nf);
+     * the two-material Euler models carry 8 or 7 primitives for 9 or 8
+     * conserved variables */
+    int np;
+    /* called after each step `it` (counted from 1) of a solve with the new
+     * primitive grid (NULL: nothing); the two-material models reinitialise their
+     * volume fraction here */
+    void (*after_step)(const orc_phys *ph, int order, int dim, int ny, int nx, int it, double *prim);
 };
+
+/* Primitive fields per cell of a physics table. */
+static inline int orc_np(const orc_phys *ph) { return ph->np ? ph->np : ph->nf; }
 
 double orc_smin(double a, double b);
 double orc_smax(double a, double b);

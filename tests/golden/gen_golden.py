@@ -340,6 +340,94 @@ def hpr_units(name, n, seed, thermal):
     print(f"{name}: {n} records")
 
 
+# ---- HPR with the full material parameters (all three EOS, plastic relaxation) ----
+
+def hpr_full_vec(eos, *, thermal=0, plastic=0, rho0=1.0, p_ref=0.0, t_ref=0.0, t0=1.0, cv=2.5, gamma=1.4, p_inf=0.0,
+                 c0=0.0, gamma0=0.0, s_h=0.0, cs=1.0, tau1=8.4e-3, tau2=0.0, sigma_y=0.0, n_pow=1.0, alpha=2.0,
+                 alpha_gr=1.0):
+    """The 26 values of the reference's full HPRMaterialParameters constructor as
+    the harness reads them (EOS code, is_thermal, is_plastic, then its 23 doubles)."""
+    return [float(eos), float(thermal), float(plastic), 8.31445985, rho0, p_ref, t_ref, t0, cv, gamma, p_inf, c0, gamma0,
+            s_h, 0.0, cs, tau1, tau2, 0.0, 0.75, sigma_y, n_pow, alpha, alpha_gr, 3.0, 2.0]
+
+
+def _oracle_params(pv):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    return oracle_lib.hpr_params_from_full(pv)
+
+
+def hpr_run_full(name, w, pv, *, order, dim, dx, cfl, tfinal, max_steps, bias, limiter, subcycles):
+    with tempfile.TemporaryDirectory() as tmp:
+        ny, nx = (1, w.shape[0]) if dim == 1 else w.shape[:2]
+        np.ascontiguousarray(w, dtype=np.float64).tofile(os.path.join(tmp, "in.f64"))
+        np.asarray(pv, dtype=np.float64).tofile(os.path.join(tmp, "params.f64"))
+        t = "inf" if np.isinf(tfinal) else repr(float(tfinal))
+        run(["hpr_run_p", order, dim, ny, nx, repr(dx), repr(cfl), t, max_steps, repr(bias), limiter, subcycles,
+             "params.f64", "in.f64", "state.f64", "dt.f64", 0], tmp)
+        state = np.fromfile(os.path.join(tmp, "state.f64")).reshape(w.shape)
+        dt = np.fromfile(os.path.join(tmp, "dt.f64"))
+    if not np.isfinite(state).all():
+        raise RuntimeError(f"{name}: non-finite reference state")
+    meta = dict(order=order, dim=dim, dx=dx, cfl=cfl, tfinal=tfinal, max_steps=max_steps, bias=bias, limiter=limiter,
+                subcycles=subcycles, params=_oracle_params(pv), params_full=list(pv))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), prim_in=w, prim_out=state, dt=dt,
+                        meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
+    print(f"{name}: {len(dt)} steps")
+
+
+def hpr_eos_units_fixture(name, pv, n, seed, rho_lo, rho_hi):
+    rng = np.random.default_rng(seed)
+    st = np.zeros((n, 11))
+    st[:, 0] = rng.uniform(rho_lo, rho_hi, n)
+    st[:, 1] = rng.uniform(0.05, 2.0, n)
+    st[:, 2:11] = np.eye(3).ravel()[None] + 0.1 * rng.standard_normal((n, 9))
+    with tempfile.TemporaryDirectory() as tmp:
+        st.tofile(os.path.join(tmp, "in.f64"))
+        np.asarray(pv, dtype=np.float64).tofile(os.path.join(tmp, "params.f64"))
+        run(["hpr_eos_units", n, "params.f64", "in.f64", "out.f64"], tmp)
+        out = np.fromfile(os.path.join(tmp, "out.f64")).reshape(n, 8)
+    meta = dict(params=_oracle_params(pv), params_full=list(pv))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), states=st, out=out,
+                        meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
+    print(f"{name}: {n} records")
+
+
+def hpr_driver_ic(name, which, n):
+    """The initial condition and parameters of HPRTests::solveBartonTest1 /
+    solveZhangElasticPlasticTest, built by the reference's own constructors."""
+    with tempfile.TemporaryDirectory() as tmp:
+        log = run(["hpr_driver", which, n, 0, -1, "params.f64", "in.f64", "state.f64", "dt.f64"], tmp)
+        pv = np.fromfile(os.path.join(tmp, "params.f64"))
+        w = np.fromfile(os.path.join(tmp, "in.f64")).reshape(n, 17)
+    dx = tfinal = None
+    for line in log.splitlines():
+        if line.startswith("DX"):
+            f = line.split()
+            dx, tfinal = float(f[1]), float(f[3])
+    meta = dict(params=_oracle_params(pv), params_full=[float(x) for x in pv], dx=dx, tfinal=tfinal, cfl=0.8,
+                limiter=0, bias=0.0)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), prim_in=w,
+                        meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
+    print(f"{name}: initial condition of {n} cells")
+
+
+def hpr_tube(n, left, right, shear=0.0):
+    """A 1D tube: (rho, p, u) left / right of the middle, distortion A = I plus an
+    off-diagonal shear (which loads the plastic relaxation)."""
+    rows = []
+    for i in range(n):
+        rho, p, u = left if i <= 0.5 * n else right
+        A = np.eye(3)
+        A[0, 1] = shear
+        rows.append(hpr_state(rho, p, u, 0.0, A=A))
+    return np.stack(rows)
+
+
+SMG = dict(rho0=1.0, p_ref=0.1, t_ref=0.0, cv=1.0, c0=1.5, gamma0=2.0, s_h=1.4, cs=0.5)
+GR = dict(rho0=1.0, t_ref=300.0, t0=300.0, cv=1e-3, gamma=2.0, c0=3.0, cs=1.0, alpha_gr=1.0)
+
+
 def rgfm_call(m1, m2, phi, *, dim, dx, cfl, tfinal, max_steps, bias, limiter, p1, p2, check_api):
     with tempfile.TemporaryDirectory() as tmp:
         ny, nx = (1, phi.shape[0]) if dim == 1 else phi.shape[:2]
@@ -435,6 +523,143 @@ def rgfm_bubble_small():
     return m1, m2, phi, dx
 
 
+# ---- two-material diffuse-interface Euler (kind 0 multiphysics, 1 reduced) ----
+MP_NP = (8, 7)
+MP_NF = (9, 8)
+AIR = (1.4, 0.0, 25.0, 0.0, 0.25, 250.0)       # EulerMaterialParameters(1.4)
+HELIUM = (1.67, 0.0, 25.0, 0.0, 0.25, 250.0)   # EulerMaterialParameters(1.67)
+STIFF_B = (4.4, 0.6, 25.0, 0.0, 0.25, 250.0)
+
+
+def mp_run(name, kind, w, *, order, dim, dx, cfl, tfinal, max_steps, bias, limiter, reinit, p1, p2, check_api):
+    with tempfile.TemporaryDirectory() as tmp:
+        ny, nx = (1, w.shape[0]) if dim == 1 else w.shape[:2]
+        np.ascontiguousarray(w, dtype=np.float64).tofile(os.path.join(tmp, "in.f64"))
+        t = "inf" if np.isinf(tfinal) else repr(float(tfinal))
+        log = run(["mp_run", kind, order, dim, ny, nx, repr(dx), repr(cfl), t, max_steps, repr(bias), limiter, reinit,
+                   *[repr(p) for p in p1], *[repr(p) for p in p2], "in.f64", "state.f64", "dt.f64", int(check_api)],
+                  tmp)
+        state = np.fromfile(os.path.join(tmp, "state.f64")).reshape(w.shape)
+        dt = np.fromfile(os.path.join(tmp, "dt.f64"))
+    api = None
+    for line in log.splitlines():
+        if line.startswith("API_MATCH"):
+            api = int(line.split()[1])
+    if check_api and max_steps == 0 and api != 1:
+        raise RuntimeError(f"{name}: mirror loop differs from the reference's own solve")
+    meta = dict(kind=kind, order=order, dim=dim, dx=dx, cfl=cfl, tfinal=tfinal, max_steps=max_steps, bias=bias,
+                limiter=limiter, reinit=reinit, p1=list(p1), p2=list(p2),
+                api_checked=bool(check_api and max_steps == 0))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), prim_in=w, prim_out=state, dt=dt,
+                        meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
+    print(f"{name}: {len(dt)} steps")
+
+
+def mp_random_states(rng, kind, n):
+    """Records of four primitive states: volume fractions across the clamps,
+    non-zero w (the multiphysics conversion adds w + w), moderate jumps."""
+    npf = MP_NP[kind]
+    st = np.empty((n, 4, npf))
+    base = np.empty((n, npf))
+    base[:, 0] = rng.choice([0.0004, 0.001, 0.3, 0.5, 0.7, 0.999, 0.9996], n)
+    base[:, 1:4] = rng.uniform(-0.8, 0.8, (n, 3))
+    base[:, 4:] = rng.uniform(0.4, 2.0, (n, npf - 4))
+    for k in range(4):
+        st[:, k] = base * (1.0 + rng.uniform(-0.3, 0.3, (n, npf)))
+        st[:, k, 0] = np.clip(base[:, 0] + rng.uniform(-0.2, 0.2, n), 0.0002, 0.9998)
+    return st.reshape(n, 4 * npf)
+
+
+def mp_units_fixture(name, kind, n, seed, bias, limiter, p1, p2):
+    rng = np.random.default_rng(seed)
+    states = mp_random_states(rng, kind, n)
+    dx, ts = 0.01, 0.002
+    npf, nf = MP_NP[kind], MP_NF[kind]
+    with tempfile.TemporaryDirectory() as tmp:
+        states.tofile(os.path.join(tmp, "in.f64"))
+        run(["mp_units", kind, n, repr(bias), limiter, repr(dx), repr(ts), *[repr(p) for p in p1],
+             *[repr(p) for p in p2], "in.f64", "out.f64"], tmp)
+        out = np.fromfile(os.path.join(tmp, "out.f64")).reshape(n, 7 * nf + 3 * npf + 2)
+    meta = dict(kind=kind, bias=bias, limiter=limiter, dx=dx, ts=ts, p1=list(p1), p2=list(p2))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), states=states, out=out,
+                        meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
+    print(f"{name}: {n} records")
+
+
+def mp_reinit_field(kind, ny, nx, seed):
+    """A volume-fraction field with crossings of both kinds along rows and
+    columns, none within two cells of an edge (where the reference indexes
+    outside its vectors), on varied densities and pressures."""
+    rng = np.random.default_rng(seed)
+    npf = MP_NP[kind]
+    w = np.empty((ny, nx, npf))
+    w[..., 1:4] = rng.uniform(-0.5, 0.5, (ny, nx, 3))
+    w[..., 4:] = rng.uniform(0.3, 2.0, (ny, nx, npf - 4))
+    i = np.arange(ny)[:, None]
+    j = np.arange(nx)[None, :]
+    blob = ((i - ny / 2) ** 2 / (0.3 * ny) ** 2 + (j - nx / 2) ** 2 / (0.3 * nx) ** 2) <= 1.0
+    stripe = (j >= 3) & (j <= 4) & (i >= 3) & (i < ny - 3)   # two crossings three cells apart
+    inside = blob | stripe
+    w[..., 0] = np.where(inside, rng.uniform(0.6, 0.999, (ny, nx)), rng.uniform(0.001, 0.4, (ny, nx)))
+    w[3:, :, 0] = np.where(np.abs(w[3:, :, 0] - 0.5) < 1e-3, 0.6, w[3:, :, 0])
+    w[:, :2, 0] = 0.9          # no crossing within two cells of the row ends
+    w[:, -2:, 0] = 0.9
+    w[:2, :, 0] = 0.9
+    w[-2:, :, 0] = 0.9
+    return w
+
+
+def mp_reinit_fixture(name, kind, w, dim, p1, p2):
+    ny, nx = (1, w.shape[0]) if dim == 1 else w.shape[:2]
+    with tempfile.TemporaryDirectory() as tmp:
+        np.ascontiguousarray(w, dtype=np.float64).tofile(os.path.join(tmp, "in.f64"))
+        run(["mp_reinit", kind, dim, ny, nx, *[repr(p) for p in p1], *[repr(p) for p in p2], "in.f64", "out.f64"], tmp)
+        out = np.fromfile(os.path.join(tmp, "out.f64")).reshape(w.shape)
+    meta = dict(kind=kind, dim=dim, p1=list(p1), p2=list(p2))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), prim_in=w, prim_out=out,
+                        meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
+    print(f"{name}: reinitialised")
+
+
+def mp_job(name, kind, ic, n, *, order=2, dim=1, tfinal, max_steps=0, reinit=0, p1=AIR, p2=AIR, limiter=0,
+           bias=0.0, cfl=0.8):
+    return lambda: mp_run(name, kind, ic(n, kind), order=order, dim=dim, dx=1.0 / n, cfl=cfl, tfinal=tfinal,
+                          max_steps=max_steps, bias=bias, limiter=limiter, reinit=reinit, p1=p1, p2=p2,
+                          check_api=True)
+
+
+MP_JOBS = {
+    "mp_toro100": mp_job("mp_toro100", 0, problems.two_material_toro_1d, 100, tfinal=0.25),
+    "mp_toro100_r5": mp_job("mp_toro100_r5", 0, problems.two_material_toro_1d, 100, tfinal=0.25, reinit=5),
+    "mp_fedkiw100_r10": mp_job("mp_fedkiw100_r10", 0, problems.two_material_fedkiw_1d, 100, tfinal=0.0012,
+                               reinit=10, p2=HELIUM, limiter=1),
+    "mp_force_fedkiw100": mp_job("mp_force_fedkiw100", 0, problems.two_material_fedkiw_1d, 100, order=1,
+                                 tfinal=0.0012, p2=HELIUM),
+    "rd_toro100_r5": mp_job("rd_toro100_r5", 1, problems.two_material_toro_1d, 100, tfinal=0.25, reinit=5),
+    "rd_fedkiw100_r10": mp_job("rd_fedkiw100_r10", 1, problems.two_material_fedkiw_1d, 100, tfinal=0.0012,
+                               reinit=10, p2=HELIUM, limiter=2, bias=0.3, max_steps=40),
+    "rd_force_toro100": mp_job("rd_force_toro100", 1, problems.two_material_toro_1d, 100, order=1, tfinal=0.25),
+    "mp_2d_toro32_r4": mp_job("mp_2d_toro32_r4", 0, problems.two_material_toro_2d, 32, dim=2, tfinal=0.125,
+                              reinit=4),
+    "mp_2d_fedkiw32_r3": mp_job("mp_2d_fedkiw32_r3", 0, problems.two_material_fedkiw_2d, 32, dim=2, tfinal=0.0006,
+                                reinit=3, p2=HELIUM),
+    "rd_2d_toro32_r4": mp_job("rd_2d_toro32_r4", 1, problems.two_material_toro_2d, 32, dim=2, tfinal=0.125,
+                              reinit=4, limiter=1),
+    "rd_2d_fedkiw32_r3": mp_job("rd_2d_fedkiw32_r3", 1, problems.two_material_fedkiw_2d, 32, dim=2, tfinal=0.0006,
+                                reinit=3, p2=HELIUM),
+    "mp_force_2d_toro24": mp_job("mp_force_2d_toro24", 0, problems.two_material_toro_2d, 24, order=1, dim=2,
+                                 tfinal=0.125),
+    "mp_units": lambda: mp_units_fixture("mp_units", 0, 400, 41, 0.0, 0, AIR, STIFF_B),
+    "mp_units_vl_bias": lambda: mp_units_fixture("mp_units_vl_bias", 0, 300, 42, 0.4, 1, STIFF_B, HELIUM),
+    "rd_units": lambda: mp_units_fixture("rd_units", 1, 400, 43, 0.0, 0, AIR, STIFF_B),
+    "rd_units_mb_bias": lambda: mp_units_fixture("rd_units_mb_bias", 1, 300, 44, -0.3, 2, HELIUM, STIFF_B),
+    "mp_reinit_2d": lambda: mp_reinit_fixture("mp_reinit_2d", 0, mp_reinit_field(0, 20, 27, 5), 2, AIR, HELIUM),
+    "rd_reinit_2d": lambda: mp_reinit_fixture("rd_reinit_2d", 1, mp_reinit_field(1, 23, 18, 6), 2, AIR, HELIUM),
+    "mp_reinit_1d": lambda: mp_reinit_fixture("mp_reinit_1d", 0, mp_reinit_field(0, 7, 40, 7)[3], 1, AIR, HELIUM),
+    "rd_reinit_1d": lambda: mp_reinit_fixture("rd_reinit_1d", 1, mp_reinit_field(1, 7, 40, 8)[3], 1, AIR, HELIUM),
+}
+
+
 JOBS = {
     "rgfm_toro1d_hllc": lambda: rgfm_run("rgfm_toro1d_hllc", *rgfm_toro_1d(200), dim=1, dx=1 / 200, cfl=0.8,
                                          tfinal=0.25, max_steps=0, bias=0.0, limiter=0, p1=IDEAL, p2=IDEAL,
@@ -477,6 +702,28 @@ JOBS = {
                                              cfl=0.8, tfinal=np.inf, max_steps=5, bias=0.0, limiter=1, subcycles=1,
                                              params=hpr_params(True), check_api=False),
     "hpr_units": lambda: hpr_units("hpr_units", 300, 31, False),
+    "hpr_eos_units_sg_plastic": lambda: hpr_eos_units_fixture(
+        "hpr_eos_units_sg_plastic", hpr_full_vec(0, plastic=1, p_inf=0.3, sigma_y=0.05, n_pow=1.7), 200, 41, 0.5, 2.0),
+    "hpr_eos_units_smg": lambda: hpr_eos_units_fixture(
+        "hpr_eos_units_smg", hpr_full_vec(1, plastic=1, sigma_y=0.2, n_pow=2.0, **SMG), 300, 42, 0.7, 1.4),
+    "hpr_eos_units_gr": lambda: hpr_eos_units_fixture(
+        "hpr_eos_units_gr", hpr_full_vec(2, plastic=1, sigma_y=0.3, n_pow=1.338, **GR), 300, 43, 0.8, 1.25),
+    "hpr_smg_tube100": lambda: hpr_run_full(
+        "hpr_smg_tube100", hpr_tube(100, (1.15, 1.0, 0.0), (0.92, 0.1, 0.0)), hpr_full_vec(1, **SMG), order=2, dim=1,
+        dx=0.01, cfl=0.8, tfinal=0.1, max_steps=0, bias=0.0, limiter=0, subcycles=0),
+    "hpr_smg_plastic_sub1": lambda: hpr_run_full(
+        "hpr_smg_plastic_sub1", hpr_tube(100, (1.1, 1.0, 0.2), (0.95, 0.3, -0.2), shear=0.05),
+        hpr_full_vec(1, plastic=1, sigma_y=0.2, n_pow=2.0, tau1=0.05, **SMG), order=2, dim=1, dx=0.01, cfl=0.8,
+        tfinal=0.05, max_steps=0, bias=0.0, limiter=1, subcycles=1),
+    "hpr_sg_plastic_sub1": lambda: hpr_run_full(
+        "hpr_sg_plastic_sub1", hpr_tube(100, (1.0, 1.0 / 1.4, 0.1), (1.0, 1.0 / 1.4, -0.1), shear=0.08),
+        hpr_full_vec(0, plastic=1, sigma_y=0.05, n_pow=1.5, tau1=0.02), order=2, dim=1, dx=0.01, cfl=0.8,
+        tfinal=0.05, max_steps=0, bias=0.0, limiter=0, subcycles=1),
+    "hpr_2d16_plastic_sub1": lambda: hpr_run_full(
+        "hpr_2d16_plastic_sub1", hpr_2d(16, 2), hpr_full_vec(0, plastic=1, sigma_y=0.5, n_pow=1.5, tau1=0.05),
+        order=2, dim=2, dx=1 / 16, cfl=0.8, tfinal=np.inf, max_steps=6, bias=0.0, limiter=0, subcycles=1),
+    "hpr_driver_barton100": lambda: hpr_driver_ic("hpr_driver_barton100", "barton", 100),
+    "hpr_driver_zhang100": lambda: hpr_driver_ic("hpr_driver_zhang100", "zhang", 100),
     "hpr_units_thermal": lambda: hpr_units("hpr_units_thermal", 300, 32, True),
     "mhd_briowu256": lambda: mhd_run("mhd_briowu256", problems.brio_wu(256), order=2, dim=1, dx=1.0 / 256, cfl=0.8,
                                      tfinal=0.1, max_steps=0, bias=0.0, limiter=1, gamma=2.0, p_inf=0.0,
@@ -545,6 +792,9 @@ JOBS = {
     "limiter_units_b0": lambda: limiter_units("limiter_units_b0", 0.0),
     "limiter_units_b03": lambda: limiter_units("limiter_units_b03", 0.3),
 }
+
+
+JOBS.update(MP_JOBS)
 
 
 def main(names):

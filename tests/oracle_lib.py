@@ -60,6 +60,8 @@ def load():
     lib.orc_hpr_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                                   C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(OrcHPRParams), dptr, dptr, dptr,
                                   C.c_int]
+    lib.orc_hpr_eos_units.restype = None
+    lib.orc_hpr_eos_units.argtypes = [C.c_int, C.POINTER(OrcHPRParams), dptr, dptr]
     lib.orc_hpr_units.restype = None
     lib.orc_hpr_units.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double, C.POINTER(OrcHPRParams),
                                   dptr, dptr]
@@ -94,6 +96,17 @@ def load():
     lib.orc_euler_branch_hash.argtypes = []
     lib.orc_rgfm_circulation.restype = C.c_double
     lib.orc_rgfm_circulation.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, dptr, dptr]
+    ep = C.POINTER(OrcEulerParams)
+    lib.orc_mp_solve.restype = C.c_int
+    lib.orc_mp_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                 C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, ep, ep, dptr, dptr, dptr, C.c_int]
+    lib.orc_mp_sweep.restype = None
+    lib.orc_mp_sweep.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                 C.c_double, C.c_int, C.c_int, ep, ep, dptr, dptr]
+    lib.orc_mp_reinit.restype = None
+    lib.orc_mp_reinit.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, dptr]
+    lib.orc_mp_units.restype = None
+    lib.orc_mp_units.argtypes = [C.c_int, C.c_int, ep, ep, C.c_double, C.c_int, C.c_double, C.c_double, dptr, dptr]
     _lib = lib
     return lib
 
@@ -150,7 +163,9 @@ def rgfm_circulation(m1, phi, dx, dy):
 
 class OrcHPRParams(C.Structure):
     _fields_ = [("is_thermal", C.c_int32), ("is_plastic", C.c_int32)] + [
-        (n, C.c_double) for n in ("rho0", "t_ref", "t0", "cv", "gamma", "p_inf", "cs", "tau1", "tau2", "alpha")]
+        (n, C.c_double) for n in ("rho0", "t_ref", "t0", "cv", "gamma", "p_inf", "cs", "tau1", "tau2", "alpha")] + [
+        ("eos", C.c_int32), ("reserved", C.c_int32)] + [
+        (n, C.c_double) for n in ("p_ref", "c0", "gamma0", "s_h", "alpha_gr", "sigma_y", "n_pow")]
 
 
 def hpr_params(thermal, *, rho0=1.0, p0=1.0 / 1.4, t_ref=0.0, t0=1.0, cv=2.5, gamma=1.4, p_inf=0.0, cs=1.0, mu=1e-2,
@@ -166,8 +181,34 @@ def hpr_params(thermal, *, rho0=1.0, p0=1.0 / 1.4, t_ref=0.0, t0=1.0, cv=2.5, ga
 
 
 def _hpr_struct(p):
-    return OrcHPRParams(int(p["is_thermal"]), 0, p["rho0"], p["t_ref"], p["t0"], p["cv"], p["gamma"], p["p_inf"],
-                        p["cs"], p["tau1"], p["tau2"], p["alpha"])
+    return OrcHPRParams(int(p["is_thermal"]), int(p.get("is_plastic", 0)), p["rho0"], p["t_ref"], p["t0"], p["cv"],
+                        p["gamma"], p["p_inf"], p["cs"], p["tau1"], p["tau2"], p["alpha"], int(p.get("eos", 0)), 0,
+                        p.get("p_ref", 0.0), p.get("c0", 0.0), p.get("gamma0", 0.0), p.get("s_h", 0.0),
+                        p.get("alpha_gr", 0.0), p.get("sigma_y", 0.0), p.get("n_pow", 0.0))
+
+
+# The 26 values of the reference's full HPRMaterialParameters constructor as the
+# harness takes them (EOS code, is_thermal, is_plastic, then the 23 doubles in
+# order, hprmaterialparameters.h:12-16) -> the oracle's parameter dictionary.
+HPR_FULL_FIELDS = ("eos", "is_thermal", "is_plastic", "ideal_gas_constant", "rho0", "p_ref", "t_ref", "t0", "cv",
+                   "gamma", "p_inf", "c0", "gamma0", "s_h", "e0", "cs", "tau1", "tau2", "mu", "prandtl", "sigma_y",
+                   "n_pow", "alpha", "alpha_gr", "beta", "gamma_gr")
+
+
+def hpr_params_from_full(v):
+    d = dict(zip(HPR_FULL_FIELDS, [float(x) for x in v]))
+    for k in ("eos", "is_thermal", "is_plastic"):
+        d[k] = int(d[k])
+    return d
+
+
+def hpr_eos_units(states, params):
+    lib = load()
+    st = np.ascontiguousarray(states, dtype=np.float64)
+    out = np.empty((st.shape[0], 8))
+    q = _hpr_struct(params)
+    lib.orc_hpr_eos_units(st.shape[0], C.byref(q), _p(st), _p(out))
+    return out
 
 
 def hpr_solve(prim, *, order, dim, dx, cfl, tfinal, max_steps, bias, limiter, subcycles, params, bc=0,
@@ -369,6 +410,54 @@ def limiter_units(r, bias):
     r = np.ascontiguousarray(r, dtype=np.float64)
     out = np.empty((r.size, 5))
     lib.orc_limiter_units(r.size, bias, _p(r), _p(out))
+    return out
+
+
+# The two-material Euler models: kind 0 multiphysics (8 primitives: alpha1, u,
+# v, w, rho1, p1, rho2, p2; 9 conserved), kind 1 reduced (7: alpha1, u, v, w,
+# rho1, rho2, p; 8 conserved).
+MP_NP = (8, 7)
+MP_NF = (9, 8)
+
+
+def mp_solve(kind, prim, *, order, dim, dx, cfl, tfinal, max_steps, bias, limiter, reinit, p1, p2, bc=0,
+             dt_cap=100000):
+    lib = load()
+    w = np.ascontiguousarray(prim, dtype=np.float64)
+    out = np.empty_like(w)
+    dts = np.empty(dt_cap)
+    ny, nx = (1, w.shape[0]) if dim == 1 else w.shape[:2]
+    n = lib.orc_mp_solve(kind, order, dim, ny, nx, dx, cfl, tfinal, max_steps, bias, limiter, bc, reinit,
+                         C.byref(OrcEulerParams(*p1)), C.byref(OrcEulerParams(*p2)), _p(w), _p(out), _p(dts), dt_cap)
+    return out, dts[:min(n, dt_cap)].copy()
+
+
+def mp_sweep(kind, prim, *, order, dim, direction, dx, ts, bias, limiter, p1, p2, bc=0):
+    lib = load()
+    w = np.ascontiguousarray(prim, dtype=np.float64)
+    out = np.empty_like(w)
+    ny, nx = (1, w.shape[0]) if dim == 1 else w.shape[:2]
+    lib.orc_mp_sweep(kind, order, dim, direction, ny, nx, dx, ts, bias, limiter, bc, C.byref(OrcEulerParams(*p1)),
+                     C.byref(OrcEulerParams(*p2)), _p(w), _p(out))
+    return out
+
+
+def mp_reinit(kind, prim, dim):
+    lib = load()
+    w = np.array(prim, dtype=np.float64, order="C")
+    ny, nx = (1, w.shape[0]) if dim == 1 else w.shape[:2]
+    lib.orc_mp_reinit(kind, dim, ny, nx, _p(w))
+    return w
+
+
+def mp_units(kind, states, *, bias, limiter, dx, ts, p1, p2):
+    lib = load()
+    st = np.ascontiguousarray(states, dtype=np.float64)
+    npf, nf = MP_NP[kind], MP_NF[kind]
+    n = st.shape[0]
+    out = np.empty((n, 7 * nf + 3 * npf + 2))
+    lib.orc_mp_units(kind, n, C.byref(OrcEulerParams(*p1)), C.byref(OrcEulerParams(*p2)), bias, limiter, dx, ts,
+                     _p(st), _p(out))
     return out
 
 

@@ -242,3 +242,18 @@ def test_planar_check_is_redone_after_a_load_or_a_planes_handout():
         assert np.any(out[0][1][..., 3] != 0.0)
     finally:
         full.close()
+
+
+def test_headline_grid_exact_mode_matches_oracle(vg_ctx):
+    """The 4096 x 4096 cylindrical-explosion workload (64 Gaussian bumps, superbee,
+    CFL 0.8 on [0, 2]^2), two steps in EXACT mode through the whole-solve entry
+    point, against the oracle on the host: time steps and state bit for bit."""
+    n = 4096
+    w = problems.disc_2d(n)
+    kw = dict(order=2, dim=2, dx=2.0 / n, cfl=0.8, tfinal=math.inf, max_steps=2, bias=0.0, limiter=0,
+              params=[1.4, 0.0, 25.0, 0.0, 0.25, 250.0])   # EulerMaterialParameters(1.4)
+    r = S.euler_solve(w, 2, 2.0 / n, 0.8, math.inf, limiter=0, max_steps=2, mode=S.EXACT)
+    print(f"GPU done: dt {r.dt}", flush=True)
+    ref, dref = O.euler_solve(w, **kw)
+    O.assert_bitwise(r.dt, dref, "dt")
+    O.assert_bitwise(r.cells, ref, "state")

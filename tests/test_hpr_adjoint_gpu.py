@@ -22,7 +22,7 @@ import pytest
 import oracle_lib as O
 from test_hpr_gpu import mat_params, rel_l1
 from test_oracle_hpr_adjoint import CASES, hpr_adj_case, hpr_adj_params
-from vanguard_amd import problems
+from vanguard_amd import _lib, problems
 from vanguard_amd import solvers as S
 from vanguard_amd._lib import VanguardError
 
@@ -154,7 +154,12 @@ def test_adjoint_rejects_bad_arguments():
                       max_steps=2)
     plastic = mat_params(p)
     plastic.is_plastic = True
-    with pytest.raises(VanguardError):
-        S.hpr_adjoint(w, np.zeros_like(w), 2, 1 / 12, 0.8, math.inf, subcycles=1, params=plastic, max_steps=2)
+    plastic.elastic_plastic_transition_parameter = 1.0
+    romenski = mat_params(p)
+    romenski.equation_of_state = "GodunovRomenski"
+    for q in (plastic, romenski):   # the forward solver runs these; the adjoint differentiates neither
+        with pytest.raises(VanguardError) as e:
+            S.hpr_adjoint(w, np.zeros_like(w), 2, 1 / 12, 0.8, math.inf, subcycles=1, params=q, max_steps=2)
+        assert e.value.status == _lib.VG_ERR_UNSUPPORTED
     with pytest.raises(ValueError):
         S.hpr_adjoint(w, np.zeros((12, 12, 6)), 2, 1 / 12, 0.8, math.inf, params=mat_params(p), max_steps=2)

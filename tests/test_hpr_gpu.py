@@ -22,13 +22,26 @@ pytestmark = pytest.mark.gpu
 REL_L1 = 1e-10
 
 
+EOS_NAMES = ("StiffenedGas", "ShockMieGruneisen", "GodunovRomenski")
+
+
 def mat_params(p):
-    """Oracle/golden parameter dict -> HPRMaterialParameters."""
-    return S.HPRMaterialParameters(
-        is_thermal=bool(p["is_thermal"]), reference_density=p["rho0"], reference_pressure=p["p0"],
-        reference_temperature=p["t_ref"], initial_temperature=p["t0"], specific_heat_capacity=p["cv"],
-        adiabatic_index=p["gamma"], stiffening_parameter=p["p_inf"], transverse_wave_speed=p["cs"],
-        strain_dissipation_time=p["tau1"], thermal_impulse_relaxation_time=p["tau2"], heat_wave_speed=p["alpha"])
+    """Oracle/golden parameter dict -> HPRMaterialParameters (keys the dict
+    lacks keep the class defaults)."""
+    keys = dict(ideal_gas_constant="ideal_gas_constant", reference_density="rho0", reference_pressure="p0",
+                reference_temperature="t_ref", initial_temperature="t0", specific_heat_capacity="cv",
+                adiabatic_index="gamma", stiffening_parameter="p_inf", reference_sound_speed="c0",
+                reference_gruneisen_coefficient="gamma0", hugoniot_slope_coefficient="s_h",
+                reference_internal_energy="e0", transverse_wave_speed="cs", strain_dissipation_time="tau1",
+                thermal_impulse_relaxation_time="tau2", dynamic_viscosity_coefficient="mu",
+                prandtl_number="prandtl", elastic_plastic_transition_parameter="sigma_y", power_law_index="n_pow",
+                heat_wave_speed="alpha", alpha_parameter="alpha_gr", beta_parameter="beta",
+                gamma_parameter="gamma_gr")
+    kw = {k: float(p[v]) for k, v in keys.items() if v in p}
+    if "p_ref" in p:
+        kw["reference_pressure"] = float(p["p_ref"])
+    return S.HPRMaterialParameters(equation_of_state=EOS_NAMES[int(p.get("eos", 0))],
+                                   is_thermal=bool(p["is_thermal"]), is_plastic=bool(p.get("is_plastic", 0)), **kw)
 
 
 def rel_l1(a, b):
@@ -123,10 +136,65 @@ def test_hpr_max_wave_speed_matches_oracle_units(vg_ctx):
     assert s == speeds.max()
 
 
-def test_hpr_unsupported_configurations(vg_ctx):
+@pytest.mark.parametrize("name", ["hpr_smg_tube100", "hpr_smg_plastic_sub1", "hpr_sg_plastic_sub1",
+                                  "hpr_2d16_plastic_sub1"])
+def test_hpr_eos_family_and_plastic_match_reference(vg_ctx, name):
+    """ShockMieGruneisen and the plastic power law against the compiled
+    reference.  Both evaluate pow (the Hugoniot reference curve's powers,
+    hprmiegruneisen.cpp:27-56; (|sigma| / sigma_Y)^n, hprsourceterms.cpp:26-33),
+    where the GPU's pow and glibc's may differ in the last bit."""
+    g, r = run_golden(name)
+    assert len(r.dt) == len(g["dt"])
+    np.testing.assert_allclose(r.dt, g["dt"], rtol=1e-12)
+    err = rel_l1(r.cells, g["prim_out"])
+    same = np.array_equal(np.asarray(r.cells).view(np.uint64), g["prim_out"].view(np.uint64))
+    print(f"{name}: rel L1 {err:.3e}, bit-identical {same}", flush=True)
+    assert err <= REL_L1
+
+
+@pytest.mark.parametrize("name,sub,steps", [("hpr_driver_barton100", 0, 0), ("hpr_driver_barton100", 1, 0),
+                                            ("hpr_driver_zhang100", 1, 20)])
+def test_hpr_godunov_romenski_matches_oracle(vg_ctx, name, sub, steps):
+    """GodunovRomenski on the reference driver's Barton and Zhang initial
+    conditions (the compiled reference's initial state, from its own driver).
+    The reference cannot run this EOS: computeGruneisenCoefficientDerivative
+    has no GodunovRomenski branch and falls off its end
+    (hprmiegruneisen.cpp:110-125); Gamma is constant there, so the oracle and
+    the GPU take the derivative as 0."""
+    g = O.golden(name)
+    m = g["meta"]
+    w = g["prim_in"]
+    ref, dts = O.hpr_solve(w, order=2, dim=1, dx=m["dx"], cfl=m["cfl"], tfinal=m["tfinal"], max_steps=steps,
+                           bias=0.0, limiter=0, subcycles=sub, params=m["params"])
+    r = S.hpr_solve(w, 1, m["dx"], m["cfl"], m["tfinal"], order=2, limiter=0, subcycles=sub,
+                    params=mat_params(m["params"]), max_steps=steps)
+    assert len(r.dt) == len(dts)
+    np.testing.assert_allclose(r.dt, dts, rtol=1e-12)
+    err = rel_l1(r.cells, ref)
+    print(f"{name} sub {sub}: {len(dts)} steps, rel L1 {err:.3e}", flush=True)
+    assert err <= REL_L1
+
+
+@pytest.mark.parametrize("eos,plastic", [(1, True), (2, False)])
+def test_hpr_eos_family_periodic_2d_matches_oracle(vg_ctx, eos, plastic):
+    """The 2D sweeps with ShockMieGruneisen (plastic) and GodunovRomenski on the
+    periodic Fourier-mode IC, against the oracle."""
+    p = O.hpr_params(False, mu=1e-3)
+    p.update(eos=eos, is_plastic=int(plastic), sigma_y=0.5, n_pow=1.5, c0=1.5, gamma0=2.0, s_h=1.4, p_ref=0.1,
+             alpha_gr=1.0, gamma=1.4)
+    w = problems.hpr_fourier_2d(20, seed=4, ny=12)
+    ref, dts = O.hpr_solve(w, order=2, dim=2, dx=1 / 20, cfl=0.8, tfinal=math.inf, max_steps=5, bias=0.0, limiter=0,
+                           subcycles=1, params=p, bc=1)
+    r = S.hpr_solve(w, 2, 1 / 20, 0.8, math.inf, order=2, limiter=0, subcycles=1, params=mat_params(p), max_steps=5,
+                    bc=S.PERIODIC)
+    assert np.isfinite(ref).all() and r.stats.steps == 5
+    np.testing.assert_allclose(r.dt, dts, rtol=1e-12)
+    assert rel_l1(r.cells, ref) <= REL_L1
+
+
+def test_hpr_rejected_configurations(vg_ctx):
     w = problems.hpr_stokes_1d(16)
-    for kw in (dict(equation_of_state="GodunovRomenski"), dict(is_plastic=True)):
-        r = S.hpr_solve(w, 1, 1 / 16, 0.8, 0.01, params=S.HPRMaterialParameters(**kw), raise_on_error=False)
-        assert r.status == _lib.VG_ERR_UNSUPPORTED
+    r = S.hpr_solve(w, 1, 1 / 16, 0.8, 0.01, params=S.HPRMaterialParameters(is_plastic=True), raise_on_error=False)
+    assert r.status == _lib.VG_ERR_ARG   # the plastic power law needs sigma_Y > 0
     r = S.hpr_solve(w, 1, 1 / 16, 0.8, 0.01, subcycles=1, params=S.HPRMaterialParameters(), raise_on_error=False)
     assert r.status == _lib.VG_ERR_ARG   # sub-cycles need a positive relaxation time

@@ -83,6 +83,32 @@ SIGNATURES = {
     "vg_euler_solve_2d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                     C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(EulerParams),
                                     C.c_int, dptr, C.c_int, C.POINTER(RunStats)]),
+    "vg_euler_multiphysics_solve_1d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_double, C.c_double, C.c_double,
+                                                 C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                 C.POINTER(EulerParams), C.POINTER(EulerParams), dptr, C.c_int,
+                                                 C.POINTER(RunStats)]),
+    "vg_euler_multiphysics_solve_2d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_int, C.c_double, C.c_double,
+                                                 C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                 C.POINTER(EulerParams), C.POINTER(EulerParams), dptr, C.c_int,
+                                                 C.POINTER(RunStats)]),
+    "vg_euler_reduced_solve_1d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                            C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(EulerParams),
+                                            C.POINTER(EulerParams), dptr, C.c_int, C.POINTER(RunStats)]),
+    "vg_euler_reduced_solve_2d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                            C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
+                                            C.POINTER(EulerParams), C.POINTER(EulerParams), dptr, C.c_int,
+                                            C.POINTER(RunStats)]),
+    "vg_euler_multiphysics_time_step_1d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_double, C.c_double, C.c_double,
+                                                     C.c_int, C.c_int, C.POINTER(EulerParams), C.POINTER(EulerParams)]),
+    "vg_euler_multiphysics_time_step_2d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_int, C.c_int, C.c_double,
+                                                     C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(EulerParams),
+                                                     C.POINTER(EulerParams)]),
+    "vg_euler_reduced_time_step_1d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_double, C.c_double, C.c_double,
+                                                C.c_int, C.c_int, C.POINTER(EulerParams), C.POINTER(EulerParams)]),
+    "vg_euler_reduced_time_step_2d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                                C.c_double, C.c_int, C.c_int, C.POINTER(EulerParams), C.POINTER(EulerParams)]),
+    "vg_euler_multiphysics_reinitialise": (C.c_int, [vptr, dptr, C.c_int, C.c_int, C.c_int]),
+    "vg_euler_reduced_reinitialise": (C.c_int, [vptr, dptr, C.c_int, C.c_int, C.c_int]),
     "vg_euler_adjoint_1d": (C.c_int, [vptr, dptr, dptr, dptr, dptr, C.c_int, C.c_double, C.c_double, C.c_double,
                                       C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(EulerParams), C.c_int, dptr,
                                       C.c_int, C.POINTER(RunStats)]),

@@ -62,6 +62,7 @@ template <class P, int ORDER, int MODE, int BS>
 __global__ __launch_bounds__(BS) void batch_solve_kernel(BatchArgs<P> a)
 {
     constexpr int NF = P::NF;
+    static_assert(StoredFields<P>::n == NF, "this kernel reads and writes all NF primitive planes");
     constexpr int G = (ORDER == 2) ? 2 : 1;
     constexpr int OUT = BS - 2 * G;
     __shared__ double s_a[NF * BS];
@@ -148,6 +149,7 @@ __global__ __launch_bounds__(256) void batch_speed_kernel(const long long *offse
                                                           const double *buf, double *speed)
 {
     constexpr int NF = P::NF;
+    static_assert(StoredFields<P>::n == NF, "this kernel reads and writes all NF primitive planes");
     __shared__ unsigned long long s_red[256 / kWave];
     const int inst = blockIdx.x;
     const long long off = offsets[inst];

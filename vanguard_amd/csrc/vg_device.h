@@ -6,6 +6,8 @@
 // EXACT mode bit-identical to the reference's x86-64 SSE2 arithmetic.
 #pragma once
 
+#include <type_traits>
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -78,20 +80,35 @@ struct PlaneMap {
     __device__ static constexpr int of(int f) { return f; }
 };
 
+// Fields a physics stores per cell: P::NF, or P::NS when it keeps fewer
+// primitive values than it has conserved variables (the two-material Euler
+// models: 8 or 7 primitives for 9 or 8 conserved); the remaining slots of its
+// register state are zero.
+template <class P, class = void>
+struct StoredFields {
+    static constexpr int n = P::NF;
+};
+template <class P>
+struct StoredFields<P, std::void_t<decltype(P::NS)>> {
+    static constexpr int n = P::NS;
+};
+
 template <class P>
 __device__ __forceinline__ void load_state(const double *__restrict__ in, long long plane, long long idx,
                                            double (&w)[P::NF])
 {
+    constexpr int NS = StoredFields<P>::n;
 #pragma unroll
-    for (int f = 0; f < P::NF; f++) w[f] = in[PlaneMap<P>::of(f) * plane + idx];
+    for (int f = 0; f < P::NF; f++) w[f] = (f < NS) ? in[PlaneMap<P>::of(f) * plane + idx] : 0.0;
 }
 
 template <class P>
 __device__ __forceinline__ void store_state(double *__restrict__ out, long long plane, long long idx,
                                             const double (&w)[P::NF])
 {
+    constexpr int NS = StoredFields<P>::n;
 #pragma unroll
-    for (int f = 0; f < P::NF; f++) out[PlaneMap<P>::of(f) * plane + idx] = w[f];
+    for (int f = 0; f < NS; f++) out[PlaneMap<P>::of(f) * plane + idx] = w[f];
 }
 
 // The narrower physics a grid of P may run with when some of its fields are

@@ -52,6 +52,7 @@ __device__ __forceinline__ void fractional_step(const typename P::Params &m, con
                                                 int limiter, double c, double (&w)[P::NF])
 {
     constexpr int NF = P::NF;
+    static_assert(StoredFields<P>::n == NF, "this kernel reads and writes all NF primitive planes");
     double xl[NF], xr[NF];
     const double cl = 0.5 * (1.0 + bias), cr = 0.5 * (1.0 - bias);
 #pragma unroll
@@ -102,6 +103,7 @@ __device__ __forceinline__ void forcing_cell(const typename P::Params &m, const 
                                              double (&K)[4][ForcingTraits<P>::kMaxSlots], R &rec)
 {
     constexpr int NF = P::NF;
+    static_assert(StoredFields<P>::n == NF, "this kernel reads and writes all NF primitive planes");
     using T = ForcingTraits<P>;
     constexpr int NS = T::kMaxSlots;
     const int nsrc = T::count(m);
@@ -177,6 +179,7 @@ template <class P, int DIM>
 __global__ __launch_bounds__(kForceBS) void forcing_kernel(ForcingArgs<P> a)
 {
     constexpr int NF = P::NF;
+    static_assert(StoredFields<P>::n == NF, "this kernel reads and writes all NF primitive planes");
     using T = ForcingTraits<P>;
     constexpr int NS = T::kMaxSlots;
     if (a.clk->done) return;

@@ -64,6 +64,27 @@ struct SourceStage {
     }
 };
 
+// Optional per-step stage after the sweeps of a second-order step, in place on
+// the step's result (the two-material models' volume-fraction
+// reinitialisation).  When active the step's last sweep does not fold in the
+// CFL maximum; the grid recomputes it after this stage.  `prepare` runs before a
+// run (outside any graph capture) and may allocate what `launch` needs in
+// GridState::post_buf.  The default has none.
+template <class P>
+struct PostStep {
+    static bool active(const typename P::Params &) { return false; }
+    template <class GS>
+    static int prepare(GS &, const typename P::Params &)
+    {
+        return VG_OK;
+    }
+    template <class GS>
+    static int launch(GS &, double *, const typename P::Params &)
+    {
+        return VG_OK;
+    }
+};
+
 // Whether the planes a narrower physics drops hold their constant bit
 // patterns in every cell: any mismatch clears *ok (every writer stores 0).
 template <class R>
@@ -155,6 +176,7 @@ int tune_y(vg_ctx *ctx, const std::string &key, SweepArgs<P> a, int &variant)
 template <class P>
 struct GridState {
     static constexpr int NF = P::NF;
+    static constexpr int NS = StoredFields<P>::n;   // planes per buffer (stored fields per cell)
     static constexpr int kBSX = Tiles<P>::bsx, kRX = Tiles<P>::rx, kBSY = Tiles<P>::bsy, kHY = Tiles<P>::hy;
     vg_ctx *ctx = nullptr;
     int dim = 2, ny = 1, nx = 1;
@@ -178,6 +200,7 @@ struct GridState {
     int *flag = nullptr;           // device flag of the constant-plane check (lazy)
     Clock *tune_clk = nullptr;     // scratch clock for tuning launches (lazy)
     Clock *sweep_clk = nullptr;    // scratch clock of the per-sweep entry points (lazy)
+    void *post_buf = nullptr;      // scratch of the PostStep stage (lazy)
     hipGraphExec_t graph = nullptr;  // captured chunk of steps
     std::string graph_key;           // arguments the graph was captured with
     static constexpr int kGraphSteps = 8;   // steps per captured chunk (even: buffers return to their start)
@@ -190,8 +213,8 @@ struct GridState {
         nx = x;
         cells = (long long)ny * nx;
         plane = (cells + 63) / 64 * 64;
-        VG_HIP(hipMalloc(&buf[0], sizeof(double) * NF * plane));
-        VG_HIP(hipMalloc(&buf[1], sizeof(double) * NF * plane));
+        VG_HIP(hipMalloc(&buf[0], sizeof(double) * NS * plane));
+        VG_HIP(hipMalloc(&buf[1], sizeof(double) * NS * plane));
         VG_HIP(hipMalloc(&clk, sizeof(Clock)));
         VG_HIP(hipMalloc(&scan, 2 * sizeof(unsigned long long)));
         const long long gx = (nx + kBSX - 4 - 1) / (kBSX - 4) + 1;
@@ -217,12 +240,55 @@ struct GridState {
         hipFree(tune_clk);
         hipFree(sweep_clk);
         hipFree(flag);
+        hipFree(post_buf);
         if (graph) hipGraphExecDestroy(graph);
+    }
+
+    // Takes over the device storage of an empty-or-not grid of another physics
+    // with the same fields (a variant of P's equations): buffers, clock and
+    // scratch move here and `o` is left empty; what depends on the physics (the
+    // captured graph, the tuned y shape, the cached wave-speed maximum) starts
+    // afresh.  *this must hold no storage.
+    template <class Q>
+    void adopt(GridState<Q> &o)
+    {
+        static_assert(Q::NF == NF, "adopt needs the same state layout");
+        ctx = o.ctx;
+        dim = o.dim;
+        ny = o.ny;
+        nx = o.nx;
+        plane = o.plane;
+        cells = o.cells;
+        buf[0] = o.buf[0];
+        buf[1] = o.buf[1];
+        cur = o.cur;
+        clk = o.clk;
+        staging = o.staging;
+        dt_dev = o.dt_dev;
+        dt_dev_cap = o.dt_dev_cap;
+        scan = o.scan;
+        partials = o.partials;
+        n_partials = 0;
+        max_valid = false;
+        subcycles = o.subcycles;
+        last_params = typename P::Params{};
+        last_bc = -1;
+        y_variant = 0;
+        reduced = false;
+        planar_known = false;
+        flag = o.flag;
+        tune_clk = o.tune_clk;
+        sweep_clk = o.sweep_clk;
+        post_buf = o.post_buf;
+        graph = nullptr;
+        graph_key.clear();
+        if (o.graph) hipGraphExecDestroy(o.graph);
+        o = GridState<Q>{};
     }
 
     int ensure_staging()
     {
-        if (!staging) VG_HIP(hipMalloc(&staging, sizeof(double) * NF * cells));
+        if (!staging) VG_HIP(hipMalloc(&staging, sizeof(double) * NS * cells));
         return VG_OK;
     }
 
@@ -234,10 +300,10 @@ struct GridState {
         if (!on_device) {
             int rc = ensure_staging();
             if (rc) return rc;
-            VG_HIP(hipMemcpyAsync(staging, prim, sizeof(double) * NF * cells, hipMemcpyHostToDevice, ctx->stream));
+            VG_HIP(hipMemcpyAsync(staging, prim, sizeof(double) * NS * cells, hipMemcpyHostToDevice, ctx->stream));
             src = staging;
         }
-        aos_to_soa_kernel<NF><<<blocks_for(cells), 256, 0, ctx->stream>>>(src, buf[0], plane, cells);
+        aos_to_soa_kernel<NS><<<blocks_for(cells), 256, 0, ctx->stream>>>(src, buf[0], plane, cells);
         int rc = launch_status("aos_to_soa");
         if (rc) return rc;
         cur = 0;
@@ -257,11 +323,11 @@ struct GridState {
             if (rc) return rc;
             dst = staging;
         }
-        soa_to_aos_kernel<NF><<<blocks_for(cells), 256, 0, ctx->stream>>>(buf[cur], dst, plane, cells);
+        soa_to_aos_kernel<NS><<<blocks_for(cells), 256, 0, ctx->stream>>>(buf[cur], dst, plane, cells);
         int rc = launch_status("soa_to_aos");
         if (rc) return rc;
         if (!on_device)
-            VG_HIP(hipMemcpyAsync(prim, staging, sizeof(double) * NF * cells, hipMemcpyDeviceToHost, ctx->stream));
+            VG_HIP(hipMemcpyAsync(prim, staging, sizeof(double) * NS * cells, hipMemcpyDeviceToHost, ctx->stream));
         VG_HIP(hipStreamSynchronize(ctx->stream));
         return VG_OK;
     }
@@ -435,7 +501,8 @@ struct GridState {
             if (rc) return rc;
             std::swap(a, b);
         }
-        const int fuse = (nsub == 0) ? 1 : 0;
+        const bool post = ORDER == 2 && PostStep<P>::active(m);
+        const int fuse = (nsub == 0 && !post) ? 1 : 0;
         if (dim == 1) {
             rc = launch_sweep<ORDER, MODE>(0, a, b, 1.0, fuse, bc, dx, bias, limiter, m);
         } else {
@@ -450,6 +517,15 @@ struct GridState {
             rc = SourceStage<P>::launch(*this, a, b, s == nsub - 1, bc, dx, bias, limiter, m);
             if (rc) return rc;
             std::swap(a, b);
+        }
+        if (post) {
+            // the stage, then the next step's CFL maximum of its result, written as
+            // the same number of partials a fused sweep would write
+            rc = PostStep<P>::launch(*this, a, m);
+            if (rc) return rc;
+            wave_max_kernel<P><<<fused_count(ORDER), 256, 0, ctx->stream>>>(a, plane, cells, dim, m, partials);
+            rc = launch_status("wave_max");
+            if (rc) return rc;
         }
         // Every step moves the grid an odd number of times (1 + 2 nsub).
         cur = (a == buf[0]) ? 0 : 1;
@@ -491,6 +567,10 @@ struct GridState {
         const int fused = fused_count(ORDER);
         if (!max_valid || last_bc != bc || std::memcmp(&last_params, &m, sizeof(m)) != 0 || n_partials != fused) {
             int rc = refresh_max(bc, m, fused);
+            if (rc) return rc;
+        }
+        if (ORDER == 2 && PostStep<P>::active(m)) {
+            int rc = PostStep<P>::prepare(*this, m);
             if (rc) return rc;
         }
         if (dim == 2) {
@@ -674,7 +754,7 @@ int ghosted_time_step(vg_ctx *ctx, int dim, double *cells, const double *with_bo
                       double dx, double dt, double bias, int limiter, int order, int mode,
                       const typename P::Params &m)
 {
-    constexpr int NF = P::NF;
+    constexpr int NF = StoredFields<P>::n;   // fields per cell of the caller's arrays
     const int G = (order == 2) ? 2 : 1;
     const int gy = (dim == 1) ? 1 : ny + 2 * G, gx = nx + 2 * G;
     GridState<P> g;

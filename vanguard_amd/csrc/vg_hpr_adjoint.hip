@@ -31,6 +31,8 @@ int check_adjoint_args(vg_ctx *ctx, int dim, int ny, int nx, double dx, int bc, 
     VG_CHECK(ctx != nullptr, "null context");
     int rc = vg::hpr_check_params(mp, subcycles);
     if (rc) return rc;
+    rc = vg::hpr_check_adjoint_params(mp);
+    if (rc) return rc;
     VG_CHECK(bc == VG_BC_TRANSMISSIVE || bc == VG_BC_PERIODIC, "unknown boundary condition");
     VG_CHECK(dim == 1 || dim == 2, "dim must be 1 or 2");
     VG_CHECK(nx >= 3 && (dim == 1 ? ny == 1 : ny >= 3), "the adjoint needs at least 3 cells per direction");

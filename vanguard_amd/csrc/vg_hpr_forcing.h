@@ -11,13 +11,13 @@
 
 namespace vg {
 
-template <>
-struct ForcingTraits<HPRPhys> {
+template <int EOS>
+struct ForcingTraits<HPRPhysT<EOS>> {
     static constexpr int kMaxSlots = 12;
     static constexpr const char *kTimer = "hpr_source_rk4";
     __device__ static __forceinline__ constexpr int slot(int q) { return q < 9 ? 2 + q : 14 + (q - 9); }
-    __device__ static __forceinline__ int count(const HPRPhys::Params &m) { return m.is_thermal ? 12 : 9; }
-    __device__ static __forceinline__ void finish(const HPRPhys::Params &m, const double (&w0)[17], double (&wn)[17])
+    __device__ static __forceinline__ int count(const HPRParams &m) { return m.is_thermal ? 12 : 9; }
+    __device__ static __forceinline__ void finish(const HPRParams &m, const double (&w0)[17], double (&wn)[17])
     {
         if (!m.is_thermal) {
             wn[14] = w0[14];

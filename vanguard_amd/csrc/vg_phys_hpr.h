@@ -1,4 +1,4 @@
-// Godunov-Peshkov-Romenski (HPR) continuum, stiffened-gas EOS: the physics of
+// Godunov-Peshkov-Romenski (HPR) continuum with the Mie-Grueneisen EOS family: the physics of
 // the reference's HPRStateVector / HPREquationOfState / HPRMieGruneisen /
 // HPRAcousticTensor / HPRSourceTerms on register arrays, in the reference's
 // operation order (every matrix product sums from 0.0 as MatrixAlgebra does).
@@ -14,44 +14,130 @@
 
 namespace vg {
 
-struct HPRPhys {
-    static constexpr int NF = 17;
-    static constexpr const char *name = "hpr";
+// HPRMaterialParameters (HPR/hprmaterialparameters.h): the fields the solver
+// reads, plus derived constants filled on the host (G = gamma - 1.0,
+// cs2 = cs^2, a2 = alpha^2, c02 = c0^2: the same IEEE values the reference
+// recomputes).  eos: 0 StiffenedGas, 1 ShockMieGruneisen, 2 GodunovRomenski;
+// p_ref, c0, gamma0, s_h, alpha_gr are the reference pressure, reference sound
+// speed, reference Grueneisen coefficient, Hugoniot slope and alpha parameter
+// those branches read; sigma_y, n_pow the plastic relaxation's
+// elastic-plastic transition parameter and power-law index.
+struct HPRParams {
+    int is_thermal, is_plastic;
+    double rho0, t_ref, t0, cv, gamma, p_inf, cs, tau1, tau2, alpha;
+    double G, cs2, a2;
+    int eos;
+    double p_ref, c0, gamma0, s_h, alpha_gr, sigma_y, n_pow, c02;
+};
 
-    // HPRMaterialParameters (HPR/hprmaterialparameters.h): the fields the solver
-    // reads, plus derived constants filled on the host (G = gamma - 1.0,
-    // cs2 = cs^2, a2 = alpha^2: the same IEEE values the reference recomputes).
-    struct Params {
-        int is_thermal, is_plastic;
-        double rho0, t_ref, t0, cv, gamma, p_inf, cs, tau1, tau2, alpha;
-        double G, cs2, a2;
-    };
+// EOS: the equation of state, fixed per kernel (0 StiffenedGas, 1
+// ShockMieGruneisen, 2 GodunovRomenski) so that the stiffened-gas kernels carry
+// none of the other curves' code; Params::eos names the same choice at run time.
+template <int EOS>
+struct HPRPhysT {
+    static constexpr int NF = 17;
+    static constexpr const char *name = EOS == 0 ? "hpr" : (EOS == 1 ? "hpr_smg" : "hpr_gr");
+    using Params = HPRParams;
 
     __device__ static __forceinline__ void begin_step(Params &, double) {}
 
-    // ---- Mie-Grueneisen, StiffenedGas branch (hprmiegruneisen.cpp:7-263) --------
+    // ---- HPRMieGruneisen (hprmiegruneisen.cpp:7-263) ------------------------------
+    // The Grueneisen coefficient, reference pressure and reference internal energy
+    // of the three equations of state and their density derivatives.  The
+    // reference's computeGruneisenCoefficientDerivative has no GodunovRomenski
+    // branch (it ends without a return, :110-125); Gamma = gamma is constant
+    // there, so its derivative is 0.
+    __device__ static __forceinline__ double mg_gamma_of(const Params &m, double rho)
+    {
+        if constexpr (EOS == 1) return m.gamma0 * (m.rho0 / rho);
+        else if constexpr (EOS == 2) return m.gamma;
+        else return m.G;
+    }
+    __device__ static __forceinline__ double mg_pref(const Params &m, double rho)
+    {
+        if constexpr (EOS == 1) {
+            if (rho > m.rho0)
+                return m.c02 * (((1.0 / m.rho0) - (1.0 / rho)) /
+                                pow(((1.0 / m.rho0) - (m.s_h * ((1.0 / m.rho0) - (1.0 / rho)))), 2.0));
+            return m.c02 * (rho - m.rho0);
+        } else if constexpr (EOS == 2) {
+            const double co = pow((rho / m.rho0), m.alpha_gr);
+            return m.c02 * (rho / m.alpha_gr) * (co - 1.0) * co;
+        } else {
+            return -m.p_inf;
+        }
+    }
+    __device__ static __forceinline__ double mg_eref(const Params &m, double rho)
+    {
+        if constexpr (EOS == 1) {
+            return (rho > m.rho0) ? 0.5 * m.p_ref * ((1.0 / m.rho0) - (1.0 / rho)) : 0.0;
+        } else if constexpr (EOS == 2) {
+            const double co = pow((rho / m.rho0), m.alpha_gr);
+            return (m.c02 / (2.0 * (m.alpha_gr * m.alpha_gr))) * ((co - 1.0) * (co - 1.0));
+        } else {
+            return m.p_inf / rho;
+        }
+    }
+    __device__ static __forceinline__ double mg_gamma_d(const Params &m, double rho)
+    {
+        if constexpr (EOS == 1) return -m.gamma0 * (m.rho0 / (rho * rho));
+        else return 0.0;
+    }
+    __device__ static __forceinline__ double mg_pref_d(const Params &m, double rho)
+    {
+        if constexpr (EOS == 1) {
+            if (rho > m.rho0)
+                return m.c02 * (m.rho0 * m.rho0) *
+                       (((m.s_h * (m.rho0 - rho)) - rho) / pow((m.s_h * (rho - m.rho0)) - rho, 3.0));
+            return m.c02;
+        } else if constexpr (EOS == 2) {
+            const double co = pow((rho / m.rho0), m.alpha_gr);
+            return (m.c02 / m.alpha_gr) * co * (((1.0 + m.alpha_gr) * (co - 1.0)) + (m.alpha_gr * co));
+        } else {
+            return 0.0;
+        }
+    }
+    __device__ static __forceinline__ double mg_eref_d(const Params &m, double rho)
+    {
+        if constexpr (EOS == 1) {
+            return (rho > m.rho0) ? -((rho - m.rho0) * m.rho0 * m.c02) / pow((m.s_h * (rho - m.rho0)) - rho, 3.0)
+                                  : 0.0;
+        } else if constexpr (EOS == 2) {
+            const double co = pow((rho / m.rho0), m.alpha_gr);
+            return (m.c02 / (rho * m.alpha_gr)) * (co - 1.0) * co;
+        } else {
+            return -(m.p_inf / (rho * rho));
+        }
+    }
+    // computeInternalEnergy / computePressure / computeTemperature and derivatives
+    // (hprmiegruneisen.cpp:192-263).  For the stiffened gas these are
+    // e_ref + (p - p_ref) / (rho Gamma) with e_ref = p_inf / rho, p_ref = -p_inf.
     __device__ static __forceinline__ double mg_e(const Params &m, double rho, double p)
     {
-        return (m.p_inf / rho) + ((p - (-m.p_inf)) / (rho * m.G));
+        return mg_eref(m, rho) + ((p - mg_pref(m, rho)) / (rho * mg_gamma_of(m, rho)));
     }
     __device__ static __forceinline__ double mg_p(const Params &m, double rho, double e)
     {
-        return ((e - (m.p_inf / rho)) * rho * m.G) + (-m.p_inf);
+        return ((e - mg_eref(m, rho)) * rho * mg_gamma_of(m, rho)) + mg_pref(m, rho);
     }
     __device__ static __forceinline__ double mg_T(const Params &m, double rho, double p)
     {
-        return m.t_ref + ((p - (-m.p_inf)) / (rho * m.G * m.cv));
+        return m.t_ref + ((p - mg_pref(m, rho)) / (rho * mg_gamma_of(m, rho) * m.cv));
     }
     __device__ static __forceinline__ double mg_e_rho(const Params &m, double rho, double p)
     {
-        const double G = m.G, Gd = 0.0, prd = 0.0, erd = -(m.p_inf / (rho * rho));
-        return erd - (((prd * rho * G) + ((G + (rho * Gd)) * (p - (-m.p_inf)))) / ((rho * G) * (rho * G)));
+        const double G = mg_gamma_of(m, rho), Gd = mg_gamma_d(m, rho), pr = mg_pref(m, rho), prd = mg_pref_d(m, rho),
+                     erd = mg_eref_d(m, rho);
+        return erd - (((prd * rho * G) + ((G + (rho * Gd)) * (p - pr))) / ((rho * G) * (rho * G)));
     }
-    __device__ static __forceinline__ double mg_e_p(const Params &m, double rho) { return 1.0 / (rho * m.G); }
+    __device__ static __forceinline__ double mg_e_p(const Params &m, double rho)
+    {
+        return 1.0 / (rho * mg_gamma_of(m, rho));
+    }
     __device__ static __forceinline__ double mg_T_rho(const Params &m, double rho, double p)
     {
-        const double G = m.G, Gd = 0.0, prd = 0.0;
-        return -((prd * rho * G) + ((G + (rho * Gd)) * (p - (-m.p_inf)))) / (((rho * G) * (rho * G)) / m.cv);
+        const double G = mg_gamma_of(m, rho), Gd = mg_gamma_d(m, rho), pr = mg_pref(m, rho), prd = mg_pref_d(m, rho);
+        return -((prd * rho * G) + ((G + (rho * Gd)) * (p - pr))) / (((rho * G) * (rho * G)) / m.cv);
     }
     __device__ static __forceinline__ double mg_T_p(const Params &m, double rho) { return mg_e_p(m, rho) / m.cv; }
 
@@ -367,9 +453,12 @@ struct HPRPhys {
         return smax(fabs(w[11]), fabs(w[12])) + smax(acoustic_speed(m, w, 0), acoustic_speed(m, w, 1));
     }
 
+    // The stiffened gas needs p + p_inf > 0 for a real sound speed; the
+    // Mie-Grueneisen solids may carry tension (p < 0) and only need rho > 0.
     __device__ static __forceinline__ bool admissible(const Params &m, const double (&w)[NF])
     {
-        return w[0] > 0.0 && (w[1] + m.p_inf) > 0.0;
+        if constexpr (EOS == 0) return w[0] > 0.0 && (w[1] + m.p_inf) > 0.0;
+        else return w[0] > 0.0;
     }
 
     // HPRStateVector() (hprstatevector.cpp:3-16): rho 1, p 1/1.4, A = I.
@@ -384,8 +473,10 @@ struct HPRPhys {
         w[10] = 1.0;
     }
 
-    // Stiff relaxation source (hprstatevector.cpp:312-366, hprsourceterms.cpp:7-39),
-    // non-plastic: theta1^-1 = 3 det(A)^(5/3) / (c_s^2 tau1).
+    // Stiff relaxation source (hprstatevector.cpp:312-366, hprsourceterms.cpp:7-39):
+    // theta1^-1 = 3 det(A)^(5/3) / (c_s^2 tau1), times (min(|sigma|, 1e8) /
+    // sigma_Y)^n when plastic, |sigma| the sigma norm (tensoralgebra.cpp:276-283)
+    // of the shear stress -rho A^T E_A.
     __device__ static __forceinline__ void source(const Params &m, const double (&U)[NF], double (&S)[NF])
     {
         double A[3][3], G[3][3], dev[3][3], Ad[3][3];
@@ -395,7 +486,24 @@ struct HPRPhys {
         for (int k = 0; k < NF; k++) S[k] = 0.0;
         gram_dev(A, G, dev);
         mul_nn(A, dev, Ad);
-        const double th1 = 3.0 * (pow(det3(A), (5.0 / 3.0)) / (m.cs2 * m.tau1));
+        double th1 = 3.0 * (pow(det3(A), (5.0 / 3.0)) / (m.cs2 * m.tau1));
+        if (m.is_plastic) {
+            double EA[3][3], P[3][3], sg[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) EA[i][j] = m.cs2 * Ad[i][j];
+            mul_tn(A, EA, P);
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) sg[i][j] = (-rho) * P[i][j];
+            const double c1 = ((sg[0][0] - sg[1][1]) * (sg[0][0] - sg[1][1])) + ((sg[1][1] - sg[2][2]) * (sg[1][1] - sg[2][2])) +
+                              ((sg[2][2] - sg[0][0]) * (sg[2][2] - sg[0][0]));
+            const double c2 = (sg[0][1] * sg[0][1]) + (sg[1][2] * sg[1][2]) + (sg[2][0] * sg[2][0]);
+            const double norm = smin(sqrt((0.5 * c1) + (3.0 * c2)), 1e8);
+            th1 = th1 * pow(norm / m.sigma_y, m.n_pow);
+        }
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
@@ -431,9 +539,11 @@ struct HPRPhys {
     }
 };
 
+using HPRPhys = HPRPhysT<0>;
+
 // 17 fields: 128-cell x tiles keep the LDS tile at 52 KB.
-template <>
-struct Tiles<HPRPhys> {
+template <int EOS>
+struct Tiles<HPRPhysT<EOS>> {
     static constexpr int bsx = 128, rx = 4, bsy = 128, hy = 32;
     static constexpr bool tune = false;
 };

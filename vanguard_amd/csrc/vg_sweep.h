@@ -430,7 +430,7 @@ __global__ __launch_bounds__(256) void nonfinite_kernel(const double *__restrict
         bool bad = false;
 #pragma unroll
         for (int f = 0; f < NF; f++) {
-            w[f] = soa[f * plane + c];
+            w[f] = (f < StoredFields<P>::n) ? soa[f * plane + c] : 0.0;
             bad = bad || !isfinite(w[f]);
         }
         bad = bad || !P::admissible(m, w);

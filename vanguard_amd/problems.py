@@ -248,3 +248,88 @@ def shock_bubble_batch(n_inst: int = 16, seed: int = 6):
     in [1.1, 1.5], two splitmix64 draws per instance (radius first)."""
     u = uniform01(seed, 2 * n_inst).reshape(n_inst, 2)
     return 0.015 + 0.015 * u[:, 0], 1.1 + 0.4 * u[:, 1]
+
+
+# ---- two-material diffuse-interface Euler ICs ---------------------------------
+# kind 0: EulerMultiphysicsStateVector primitives (alpha1, u, v, w, rho1, p1, rho2,
+# p2); kind 1: EulerReducedStateVector primitives (alpha1, u, v, w, rho1, rho2, p).
+
+def _two_material(kind: int, alpha: float, u: float, rho1: float, rho2: float, p1: float, p2: float,
+                  p_reduced: float) -> list:
+    if kind == 0:
+        return [alpha, u, 0.0, 0.0, rho1, p1, rho2, p2]
+    return [alpha, u, 0.0, 0.0, rho1, rho2, p_reduced]
+
+
+def two_material_toro_1d(n: int, kind: int = 0) -> np.ndarray:
+    """Toro's test 1 as a two-material problem, EulerMultiphysicsTests::solveToroTest1
+    (Euler/Multiphysics/eulermultiphysicstests.cpp:7-27) and
+    EulerReducedTests::solveToroTest1 (eulerreducedtests.cpp:7-27): material 1
+    (alpha1 = 0.999) where i <= n/2, material 2 (0.001) elsewhere; gamma 1.4 both,
+    t = 0.25, dx = 1/n."""
+    rows = []
+    for i in range(n):
+        left = i <= 0.5 * n
+        a = 0.999 if left else 0.001
+        if kind == 0:
+            rows.append(_two_material(0, a, 0.0, 1.0, 0.125, 1.0, 0.1, 0.0))
+        else:
+            rows.append(_two_material(1, a, 0.0, 1.0, 0.125, 0.0, 0.0, 1.0 if left else 0.1))
+    return np.array(rows)
+
+
+def two_material_fedkiw_1d(n: int, kind: int = 0) -> np.ndarray:
+    """Fedkiw's shocked air/helium tube, EulerMultiphysicsTests::solveFedkiwTest
+    (eulermultiphysicstests.cpp:29-59) and the reduced twin
+    (eulerreducedtests.cpp:29-57): gamma 1.4 / 1.67, t = 0.0012, dx = 1/n."""
+    us = 0.3535 * np.sqrt(10.0 ** 5)
+    rows = []
+    for i in range(n):
+        if i <= 0.5 * n:
+            if i <= 0.05 * n:
+                rows.append(_two_material(kind, 0.999, us, 1.3333, 0.1379, 1.5e5, 1.0e5, 1.5e5))
+            else:
+                rows.append(_two_material(kind, 0.999, 0.0, 1.0, 0.1379, 1.0e5, 1.0e5, 1.0e5))
+        else:
+            rows.append(_two_material(kind, 0.001, 0.0, 1.0, 0.1379, 1.0e5, 1.0e5, 1.0e5))
+    return np.array(rows)
+
+
+def _radius(n: int) -> np.ndarray:
+    i = np.arange(n, dtype=np.float64)[:, None] - (0.5 * n)
+    j = np.arange(n, dtype=np.float64)[None, :] - (0.5 * n)
+    return np.sqrt((i * i) + (j * j))
+
+
+def two_material_toro_2d(n: int, kind: int = 0) -> np.ndarray:
+    """The cylindrical Toro test, solve2DToroTest1 (eulermultiphysicstests.cpp:61-81,
+    eulerreducedtests.cpp:59-82): material 1 inside radius 0.2 n (in cells) of the
+    centre, t = 0.125."""
+    inside = _radius(n) <= (0.2 * n)
+    a = np.where(inside, 0.999, 0.001)
+    w = np.zeros((n, n, 8 if kind == 0 else 7))
+    w[..., 0] = a
+    if kind == 0:
+        w[..., 4], w[..., 5], w[..., 6], w[..., 7] = 1.0, 1.0, 0.125, 0.1
+    else:
+        w[..., 4], w[..., 5], w[..., 6] = 1.0, 0.125, np.where(inside, 1.0, 0.1)
+    return w
+
+
+def two_material_fedkiw_2d(n: int, kind: int = 0) -> np.ndarray:
+    """The cylindrical Fedkiw test, solve2DFedkiwTest (eulermultiphysicstests.cpp:83-113,
+    eulerreducedtests.cpp:84-115): a shocked core within 0.02 n of the centre, air
+    to 0.2 n, helium outside; gamma 1.4 / 1.67, t = 0.0006."""
+    r = _radius(n)
+    w = np.zeros((n, n, 8 if kind == 0 else 7))
+    for i in range(n):
+        for j in range(n):
+            if r[i, j] <= (0.2 * n):
+                if r[i, j] <= (0.02 * n):
+                    w[i, j] = _two_material(kind, 0.999, 0.3535 * np.sqrt(10.0 ** 5), 1.3333, 0.1379, 1.5e5, 1.0e5,
+                                            1.5e5)
+                else:
+                    w[i, j] = _two_material(kind, 0.999, 0.0, 1.0, 0.1379, 1.0e5, 1.0e5, 1.0e5)
+            else:
+                w[i, j] = _two_material(kind, 0.001, 0.0, 1.0, 0.1379, 1.0e5, 1.0e5, 1.0e5)
+    return w

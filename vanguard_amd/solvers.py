@@ -256,6 +256,17 @@ def euler_solve_fixed_dt(prim: np.ndarray, dim: int, dx: float, dts, *, bias: fl
 # ---- ghost cells and per-sweep calls ---------------------------------------------
 
 EULER_DEFAULT_STATE = (1.0, 0.0, 0.0, 0.0, 1.0, 1.0)                 # Euler/eulerstatevector.cpp:3-12
+# EulerMultiphysicsStateVector() / EulerReducedStateVector()
+# (eulermultiphysicsstatevector.cpp:3-15, eulerreducedstatevector.cpp:3-14).
+MULTIPHYSICS_DEFAULT_STATE = np.array([0.999, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0, 1.0])
+REDUCED_DEFAULT_STATE = np.array([0.999, 0.0, 0.0, 0.0, 1.0, 1.0, 1.0])
+
+
+def _default_state(nf: int) -> np.ndarray:
+    table = {6: EULER_DEFAULT_STATE, 8: MULTIPHYSICS_DEFAULT_STATE, 7: REDUCED_DEFAULT_STATE}
+    if nf not in table:
+        raise ValueError(f"no default state for {nf} fields per cell; pass default_state")
+    return table[nf]
 MHD_DEFAULT_STATE = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0)    # MHD/mhdstatevector.cpp:3-16
 HPR_DEFAULT_STATE = (1.0, 1.0 / 1.4, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0,
                      0.0)                                             # HPR/hprstatevector.cpp:3-16
@@ -266,10 +277,12 @@ class Solvers:
     per-sweep calls read."""
 
     @staticmethod
-    def insertBoundaryCells(current_cells, boundary_size, default_state=EULER_DEFAULT_STATE):
-        """Solvers::insertBoundaryCells (solvers.cpp:7-32): transmissive ghosts,
-        one layer copied, two layers mirror-ordered (g[-1] = c[0], g[-2] = c[1])."""
+    def insertBoundaryCells(current_cells, boundary_size, default_state=None):
+        """Solvers::insertBoundaryCells (solvers.cpp:7-32, and the overloads of
+        the two-material states, :34-86): transmissive ghosts, one layer copied,
+        two layers mirror-ordered (g[-1] = c[0], g[-2] = c[1])."""
         c = np.asarray(current_cells, dtype=np.float64)
+        default_state = _default_state(c.shape[-1]) if default_state is None else default_state
         n, b = c.shape[0], boundary_size
         out = np.empty((n + 2 * b, c.shape[1]))
         out[:] = default_state
@@ -281,10 +294,12 @@ class Solvers:
         return out
 
     @staticmethod
-    def insertBoundaryCells2D(current_cells, boundary_size, default_state=EULER_DEFAULT_STATE):
-        """Solvers::insertBoundaryCells2D (solvers.cpp:88-143): the same per row
-        and per column; the corner ghosts keep the default state."""
+    def insertBoundaryCells2D(current_cells, boundary_size, default_state=None):
+        """Solvers::insertBoundaryCells2D (solvers.cpp:88-258): the same per row
+        and per column; the corner ghosts keep the state's default (by default
+        the one of the state with this many fields)."""
         c = np.asarray(current_cells, dtype=np.float64)
+        default_state = _default_state(c.shape[-1]) if default_state is None else default_state
         ny, nx, b = c.shape[0], c.shape[1], boundary_size
         out = np.empty((ny + 2 * b, nx + 2 * b, c.shape[2]))
         out[:] = default_state
@@ -304,7 +319,8 @@ def _time_step(fn_name, params_struct, current_cells, with_boundary, dim, direct
                limiter, order, nf, mode=None, ctx: Context | None = None):
     """One per-sweep call: current_cells (an array of the reference's layout) is
     overwritten in place with the updated states, as the reference's
-    computeSLICTimeStep mutates currentCells."""
+    computeSLICTimeStep mutates currentCells.  params_struct: one parameter
+    struct, or a tuple of them (the two-material models take two)."""
     ctx = ctx or default_context()
     cells = current_cells
     if not (isinstance(cells, np.ndarray) and cells.dtype == np.float64 and cells.flags.c_contiguous):
@@ -323,81 +339,203 @@ def _time_step(fn_name, params_struct, current_cells, with_boundary, dim, direct
                              f"({ny + 2 * g}, {nx + 2 * g}, {nf})")
         args = [ny, nx, direction]
     extra = [] if mode is None else [mode]
+    structs = params_struct if isinstance(params_struct, tuple) else (params_struct,)
     rc = getattr(ctx.lib, fn_name)(ctx.handle, _lib.as_dptr(cells), _lib.as_dptr(wb), *args, cell_spacing, time_step,
-                                   bias, limiter, order, C.byref(params_struct), *extra)
+                                   bias, limiter, order, *[C.byref(p) for p in structs], *extra)
     check(rc, fn_name)
     return cells
 
 
 
 
+# ---- two-material diffuse-interface Euler -------------------------------------
+# "multiphysics": EulerMultiphysicsStateVector, 8 primitives per cell (alpha1, u,
+# v, w, rho1, p1, rho2, p2); "reduced": EulerReducedStateVector, 7 (alpha1, u, v,
+# w, rho1, rho2, p).
+TWO_MATERIAL_FIELDS = {"multiphysics": 8, "reduced": 7}
+
+
+def _two_params(params) -> bool:
+    """Whether an overload's trailing arguments are two EulerMaterialParameters
+    (the two-material states) rather than one (and an optional mode)."""
+    return len(params) >= 2 and all(isinstance(p, EulerMaterialParameters) for p in params[-2:])
+
+
+def _two_material_model(cells) -> str:
+    nf = np.shape(cells)[-1]
+    for model, n in TWO_MATERIAL_FIELDS.items():
+        if n == nf:
+            return model
+    raise ValueError(f"a two-material state has 8 (multiphysics) or 7 (reduced) fields per cell, not {nf}")
+
+
+def two_material_solve(model: str, prim: np.ndarray, dim: int, dx: float, cfl: float, t_final: float, *,
+                       order: int = 2, bias: float = 0.0, limiter: int = 0, reinit_every: int = 0,
+                       params1: EulerMaterialParameters | None = None, params2: EulerMaterialParameters | None = None,
+                       max_steps: int = 0, bc: int = TRANSMISSIVE, dt_cap: int = 100000, ctx: Context | None = None,
+                       raise_on_error: bool = True) -> SolveResult:
+    """Whole solve through vg_euler_{multiphysics,reduced}_solve_{1d,2d}
+    (SecondOrderSolver / FirstOrderSolver::solve[2D] for these states)."""
+    ctx = ctx or default_context()
+    nf = TWO_MATERIAL_FIELDS[model]
+    m1 = (params1 or EulerMaterialParameters(1.4)).c_struct()
+    m2 = (params2 or EulerMaterialParameters(1.4)).c_struct()
+    w = np.ascontiguousarray(prim, dtype=np.float64)
+    if w.ndim != dim + 1 or w.shape[-1] != nf:
+        raise ValueError(f"{dim}D {model} state must have shape {'(cells' if dim == 1 else '(rows, columns'}, {nf})")
+    out = np.empty_like(w)
+    dts = np.empty(dt_cap)
+    st = _lib.RunStats()
+    fn = getattr(ctx.lib, f"vg_euler_{model}_solve_{dim}d")
+    grid = [w.shape[0]] if dim == 1 else [w.shape[0], w.shape[1]]
+    rc = fn(ctx.handle, _lib.as_dptr(w), _lib.as_dptr(out), *grid, dx, cfl, t_final, max_steps, bias, limiter, order,
+            bc, reinit_every, C.byref(m1), C.byref(m2), _lib.as_dptr(dts), dt_cap, C.byref(st))
+    if raise_on_error and rc != _lib.VG_OK:
+        check(rc, f"vg_euler_{model}_solve_{dim}d")
+    return SolveResult(out, dts[:min(st.steps, dt_cap)].copy(), st, rc)
+
+
+class MultiphysicsSolvers:
+    """Solvers/multiphysicssolvers.h:8-39, the Euler overloads."""
+
+    @staticmethod
+    def reinitialiseVolumeFraction(current_cells, material1_parameters=None, material2_parameters=None,
+                                   ctx: Context | None = None):
+        """In place, as the reference mutates currentCells; the material
+        parameters do not enter (the reference's unused mixture energy)."""
+        ctx = ctx or default_context()
+        cells = current_cells
+        if not (isinstance(cells, np.ndarray) and cells.dtype == np.float64 and cells.flags.c_contiguous):
+            raise TypeError("current_cells must be a C-contiguous float64 array (updated in place)")
+        model = _two_material_model(cells)
+        dim = cells.ndim - 1
+        ny, nx = (1, cells.shape[0]) if dim == 1 else cells.shape[:2]
+        fn = f"vg_euler_{model}_reinitialise"
+        check(getattr(ctx.lib, fn)(ctx.handle, _lib.as_dptr(cells), dim, ny, nx), fn)
+        return cells
+
+
 class SecondOrderSolver:
     """SLIC (MUSCL-Hancock + FORCE), Solvers/secondordersolver.h:7-72."""
 
+    # The reference overloads these on the state type: EulerStateVector takes one
+    # EulerMaterialParameters; EulerMultiphysicsStateVector and
+    # EulerReducedStateVector (8 or 7 fields per cell here) take a
+    # reinitialisation frequency (solve only) and two
+    # (secondordersolver.h:34-70).
+
     @staticmethod
     def solve(initial_cells, cell_spacing, cfl_coefficient, final_time, bias, slope_limiter,
-              subcycling_iterations, material_parameters, **kw):
+              subcycling_iterations, *params, **kw):
+        if _two_params(params):
+            reinit, m1, m2 = params
+            return two_material_solve(_two_material_model(initial_cells), initial_cells, 1, cell_spacing,
+                                      cfl_coefficient, final_time, order=2, bias=bias, limiter=slope_limiter,
+                                      reinit_every=reinit, params1=m1, params2=m2, **kw).cells
         # subcyclingIterations is an empty loop for Euler (secondordersolver.cpp:302-305).
+        (material_parameters,) = params
         return euler_solve(initial_cells, 1, cell_spacing, cfl_coefficient, final_time, order=2, bias=bias,
                            limiter=slope_limiter, params=material_parameters, **kw).cells
 
     @staticmethod
     def solve2D(initial_cells, cell_spacing, cfl_coefficient, final_time, bias, slope_limiter,
-                subcycling_iterations, material_parameters, **kw):
+                subcycling_iterations, *params, **kw):
+        if _two_params(params):
+            reinit, m1, m2 = params
+            return two_material_solve(_two_material_model(initial_cells), initial_cells, 2, cell_spacing,
+                                      cfl_coefficient, final_time, order=2, bias=bias, limiter=slope_limiter,
+                                      reinit_every=reinit, params1=m1, params2=m2, **kw).cells
+        (material_parameters,) = params
         return euler_solve(initial_cells, 2, cell_spacing, cfl_coefficient, final_time, order=2, bias=bias,
                            limiter=slope_limiter, params=material_parameters, **kw).cells
 
     @staticmethod
+    def _sweep(dim, direction, current_cells, current_cells_with_boundary, cell_spacing, time_step, bias,
+               slope_limiter, params, mode):
+        if _two_params(params):
+            model = _two_material_model(current_cells)
+            return _time_step(f"vg_euler_{model}_time_step_{dim}d", tuple(p.c_struct() for p in params),
+                              current_cells, current_cells_with_boundary, dim, direction, cell_spacing, time_step,
+                              bias, slope_limiter, 2, TWO_MATERIAL_FIELDS[model])
+        if len(params) == 2:
+            mode = params[1]
+        return _time_step(f"vg_euler_time_step_{dim}d", params[0].c_struct(), current_cells,
+                          current_cells_with_boundary, dim, direction, cell_spacing, time_step, bias, slope_limiter,
+                          2, 6, mode)
+
+    @staticmethod
     def computeSLICTimeStep(current_cells, current_cells_with_boundary, cell_spacing, time_step, bias, slope_limiter,
-                            material_parameters, mode=EXACT):
-        return _time_step("vg_euler_time_step_1d", material_parameters.c_struct(), current_cells,
-                          current_cells_with_boundary, 1, 0, cell_spacing, time_step, bias, slope_limiter, 2, 6, mode)
+                            *params, mode=EXACT):
+        return SecondOrderSolver._sweep(1, 0, current_cells, current_cells_with_boundary, cell_spacing, time_step,
+                                        bias, slope_limiter, params, mode)
 
     @staticmethod
     def computeXSLICTimeStep2D(current_cells, current_cells_with_boundary, cell_spacing, time_step, bias,
-                               slope_limiter, material_parameters, mode=EXACT):
-        return _time_step("vg_euler_time_step_2d", material_parameters.c_struct(), current_cells,
-                          current_cells_with_boundary, 2, 0, cell_spacing, time_step, bias, slope_limiter, 2, 6, mode)
+                               slope_limiter, *params, mode=EXACT):
+        return SecondOrderSolver._sweep(2, 0, current_cells, current_cells_with_boundary, cell_spacing, time_step,
+                                        bias, slope_limiter, params, mode)
 
     @staticmethod
     def computeYSLICTimeStep2D(current_cells, current_cells_with_boundary, cell_spacing, time_step, bias,
-                               slope_limiter, material_parameters, mode=EXACT):
-        return _time_step("vg_euler_time_step_2d", material_parameters.c_struct(), current_cells,
-                          current_cells_with_boundary, 2, 1, cell_spacing, time_step, bias, slope_limiter, 2, 6, mode)
+                               slope_limiter, *params, mode=EXACT):
+        return SecondOrderSolver._sweep(2, 1, current_cells, current_cells_with_boundary, cell_spacing, time_step,
+                                        bias, slope_limiter, params, mode)
 
 
 class FirstOrderSolver:
     """FORCE, Solvers/firstordersolver.h:7-101."""
 
+    # Overloaded on the state type as SecondOrderSolver (firstordersolver.h:65-100).
+
     @staticmethod
-    def solve(initial_cells, cell_spacing, cfl_coefficient, final_time, subcycling_iterations,
-              material_parameters, **kw):
+    def solve(initial_cells, cell_spacing, cfl_coefficient, final_time, subcycling_iterations, *params, **kw):
+        if _two_params(params):
+            return two_material_solve(_two_material_model(initial_cells), initial_cells, 1, cell_spacing,
+                                      cfl_coefficient, final_time, order=1, params1=params[0], params2=params[1],
+                                      **kw).cells
+        (material_parameters,) = params
         return euler_solve(initial_cells, 1, cell_spacing, cfl_coefficient, final_time, order=1,
                            params=material_parameters, **kw).cells
 
     @staticmethod
-    def solve2D(initial_cells, cell_spacing, cfl_coefficient, final_time, subcycling_iterations,
-                material_parameters, **kw):
+    def solve2D(initial_cells, cell_spacing, cfl_coefficient, final_time, subcycling_iterations, *params, **kw):
+        if _two_params(params):
+            return two_material_solve(_two_material_model(initial_cells), initial_cells, 2, cell_spacing,
+                                      cfl_coefficient, final_time, order=1, params1=params[0], params2=params[1],
+                                      **kw).cells
+        (material_parameters,) = params
         return euler_solve(initial_cells, 2, cell_spacing, cfl_coefficient, final_time, order=1,
                            params=material_parameters, **kw).cells
 
     @staticmethod
-    def computeFORCETimeStep(current_cells, current_cells_with_boundary, cell_spacing, time_step,
-                             material_parameters, mode=EXACT):
-        return _time_step("vg_euler_time_step_1d", material_parameters.c_struct(), current_cells,
-                          current_cells_with_boundary, 1, 0, cell_spacing, time_step, 0.0, 0, 1, 6, mode)
+    def _sweep(dim, direction, current_cells, current_cells_with_boundary, cell_spacing, time_step, params, mode):
+        if _two_params(params):
+            model = _two_material_model(current_cells)
+            return _time_step(f"vg_euler_{model}_time_step_{dim}d", tuple(p.c_struct() for p in params),
+                              current_cells, current_cells_with_boundary, dim, direction, cell_spacing, time_step,
+                              0.0, 0, 1, TWO_MATERIAL_FIELDS[model])
+        if len(params) == 2:
+            mode = params[1]
+        return _time_step(f"vg_euler_time_step_{dim}d", params[0].c_struct(), current_cells,
+                          current_cells_with_boundary, dim, direction, cell_spacing, time_step, 0.0, 0, 1, 6, mode)
 
     @staticmethod
-    def computeXFORCETimeStep2D(current_cells, current_cells_with_boundary, cell_spacing, time_step,
-                                material_parameters, mode=EXACT):
-        return _time_step("vg_euler_time_step_2d", material_parameters.c_struct(), current_cells,
-                          current_cells_with_boundary, 2, 0, cell_spacing, time_step, 0.0, 0, 1, 6, mode)
+    def computeFORCETimeStep(current_cells, current_cells_with_boundary, cell_spacing, time_step, *params,
+                             mode=EXACT):
+        return FirstOrderSolver._sweep(1, 0, current_cells, current_cells_with_boundary, cell_spacing, time_step,
+                                       params, mode)
 
     @staticmethod
-    def computeYFORCETimeStep2D(current_cells, current_cells_with_boundary, cell_spacing, time_step,
-                                material_parameters, mode=EXACT):
-        return _time_step("vg_euler_time_step_2d", material_parameters.c_struct(), current_cells,
-                          current_cells_with_boundary, 2, 1, cell_spacing, time_step, 0.0, 0, 1, 6, mode)
+    def computeXFORCETimeStep2D(current_cells, current_cells_with_boundary, cell_spacing, time_step, *params,
+                                mode=EXACT):
+        return FirstOrderSolver._sweep(2, 0, current_cells, current_cells_with_boundary, cell_spacing, time_step,
+                                       params, mode)
+
+    @staticmethod
+    def computeYFORCETimeStep2D(current_cells, current_cells_with_boundary, cell_spacing, time_step, *params,
+                                mode=EXACT):
+        return FirstOrderSolver._sweep(2, 1, current_cells, current_cells_with_boundary, cell_spacing, time_step,
+                                       params, mode)
 
 
 class EulerGrid:
