@@ -238,3 +238,7 @@ void orc_mhd_units(int n, double bias, int limiter, double dx, double ts, const 
         *o++ = alfven(b[0], b[5], b[6], b[7]);
     }
 }
+
+/* Exposed for the MHD RGFM oracle (mhd_rgfm_oracle.c). */
+orc_phys orc_mhd_phys_table(orc_mhd_params *mp) { return mhd_phys(mp); }
+double orc_mhd_fast_speed(const orc_mhd_params *mp, const double *w, int dir) { return fast_speed(mp, w, dir); }

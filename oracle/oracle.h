@@ -168,4 +168,19 @@ void orc_mp_reinit(int kind, int dim, int ny, int nx, double *prim);
 void orc_mp_units(int kind, int n, const orc_euler_params *m1, const orc_euler_params *m2, double bias, int limiter,
                   double dx, double ts, const double *states, double *out);
 
+/* Two-material ideal MHD with the real ghost fluid method (mhd_rgfm_oracle.c):
+ * MHDRGFMSolver::solve (dim 1; the parameters' c_h as given, no sub-cycles)
+ * and solve2D (dim 2; c_h = 4 and c_p = sqrt(0.18 c_h) from the first step,
+ * `subcycles` RK4 GLM sub-cycles around each material's sweeps).  States are
+ * 9 doubles per cell (rho, u, v, w, p, Bx, By, Bz, psi). */
+int orc_mhd_rgfm_solve(int dim, int ny, int nx, double dx, double cfl, double t_final, int max_steps, double bias,
+                       int limiter, int subcycles, const orc_mhd_params *p1, const orc_mhd_params *p2,
+                       const double *m1_in, const double *m2_in, const double *phi_in, double *m1_out,
+                       double *m2_out, double *phi_out, double *dt_log, int dt_cap);
+void orc_mhd_rgfm_ghost_fill(int dim, int ny, int nx, const orc_mhd_params *p1, const orc_mhd_params *p2, double *m1,
+                             double *m2, const double *phi);
+void orc_mhd_hllc_solve(int d, const double *a, const double *b, const orc_mhd_params *m1,
+                        const orc_mhd_params *m2, double *out);
+void orc_mhd_hllc_units(int n, const orc_mhd_params *p1, const orc_mhd_params *p2, const double *states, double *out);
+
 #endif

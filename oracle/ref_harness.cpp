@@ -26,6 +26,7 @@
 #include "Solvers/mhdforcingsolver.h"
 #include "HPR/hpracoustictensor.h"
 #include "RGFM/rgfmsolver.h"
+#include "RGFM/mhdrgfmsolver.h"
 #include "Euler/eulertests.h"
 #include "MHD/mhdtests.h"
 #include "HPR/hprtests.h"
@@ -1378,6 +1379,207 @@ int mpReinit(int argc, char **argv)
                                    : mpReinitT<EulerReducedStateVector>(argc, argv, 7);
 }
 
+
+// ---- two-material MHD RGFM -----------------------------------------------------
+MHDMaterialParameters mhdParams4(char **a)
+{
+    MHDMaterialParameters m(std::atof(a[0]));
+    m.setStiffeningParameter(std::atof(a[1]));
+    m.setHyperbolicWaveSpeed(std::atof(a[2]));
+    m.setParabolicDamping(std::atof(a[3]));
+    return m;
+}
+
+// mhd_rgfm_run dim ny nx dx cfl tfinal max_steps bias limiter subcycles
+//              p1(g pinf ch cp) p2(4) m1_in m2_in phi_in m1_out m2_out phi_out dt_out check_api
+// The loops mirror MHDRGFMSolver::solve (mhdrgfmsolver.cpp:522-563) and
+// solve2D (:565-693) through the reference's own public static steps, with a
+// step cap; check_api compares with solve / solve2D themselves.
+int mhdRgfmRun(int argc, char **argv)
+{
+    if (argc < 28) { std::fprintf(stderr, "mhd_rgfm_run: too few arguments\n"); return 2; }
+    int dim = std::atoi(argv[2]);
+    int ny = std::atoi(argv[3]);
+    int nx = std::atoi(argv[4]);
+    double dx = std::atof(argv[5]);
+    double cfl = std::atof(argv[6]);
+    double tfinal = std::strcmp(argv[7], "inf") == 0 ? std::numeric_limits<double>::infinity() : std::atof(argv[7]);
+    int maxSteps = std::atoi(argv[8]);
+    double bias = std::atof(argv[9]);
+    int limiter = std::atoi(argv[10]);
+    int sub = std::atoi(argv[11]);
+    MHDMaterialParameters p1 = mhdParams4(argv + 12);
+    MHDMaterialParameters p2 = mhdParams4(argv + 16);
+    const size_t cells = (size_t)ny * nx;
+    std::vector<double> in1 = readDoubles(argv[20], cells * 9);
+    std::vector<double> in2 = readDoubles(argv[21], cells * 9);
+    std::vector<double> phiIn = readDoubles(argv[22], cells);
+    int checkApi = std::atoi(argv[27]);
+    std::vector<double> out1(cells * 9), out2(cells * 9), outPhi(cells), dts;
+    MuteCout mute;
+    bool match = true;
+    if (dim == 1)
+    {
+        std::vector<MHDStateVector> m1(nx), m2(nx);
+        std::vector<double> phi(phiIn.begin(), phiIn.end());
+        for (int j = 0; j < nx; j++) { m1[j] = mhdStateFrom(&in1[(size_t)j * 9]); m2[j] = mhdStateFrom(&in2[(size_t)j * 9]); }
+        MHDMultimaterialSystem initial(m1, m2, phi);
+        double t = 0.0;
+        int it = 0;
+        while (t < tfinal && (maxSteps == 0 || it < maxSteps))
+        {
+            MHDMultimaterialSystem g = MHDRGFMSolver::applyRGFMBoundaryConditions(MHDMultimaterialSystem(m1, m2, phi), p1, p2);
+            m1 = g.getMaterial1Cells();
+            m2 = g.getMaterial2Cells();
+            std::vector<MHDStateVector> b1 = MHDSolvers::insertBoundaryCells(m1, 2);
+            std::vector<MHDStateVector> b2 = MHDSolvers::insertBoundaryCells(m2, 2);
+            double dt = std::min(MHDSolvers::computeStableTimeStep(b1, dx, cfl, t, tfinal, it, p1),
+                                 MHDSolvers::computeStableTimeStep(b2, dx, cfl, t, tfinal, it, p2));
+            phi = MHDRGFMSolver::updateLevelSetFunction(phi, dx, dt, m1, m2);
+            MHDSecondOrderSolver::computeSLICTimeStep(m1, b1, dx, dt, bias, limiter, p1);
+            MHDSecondOrderSolver::computeSLICTimeStep(m2, b2, dx, dt, bias, limiter, p2);
+            t += dt;
+            it += 1;
+            dts.push_back(dt);
+        }
+        for (int j = 0; j < nx; j++) { putMhdState(m1[j], &out1[(size_t)j * 9]); putMhdState(m2[j], &out2[(size_t)j * 9]); }
+        for (int j = 0; j < nx; j++) outPhi[j] = phi[j];
+        if (checkApi && maxSteps == 0)
+        {
+            MHDMultimaterialSystem api = MHDRGFMSolver::solve(initial, dx, cfl, tfinal, bias, limiter, sub, 0, p1, p2);
+            std::vector<MHDStateVector> a1 = api.getMaterial1Cells(), a2 = api.getMaterial2Cells();
+            std::vector<double> ap = api.getLevelSetFunction();
+            for (int j = 0; j < nx; j++)
+            {
+                double q1[9], q2[9];
+                putMhdState(a1[j], q1);
+                putMhdState(a2[j], q2);
+                for (int f = 0; f < 9; f++)
+                    match = match && sameBits(q1[f], out1[(size_t)j * 9 + f]) && sameBits(q2[f], out2[(size_t)j * 9 + f]);
+                match = match && sameBits(ap[j], outPhi[j]);
+            }
+        }
+    }
+    else
+    {
+        typedef std::vector<std::vector<MHDStateVector> > Grid;
+        Grid m1(ny, std::vector<MHDStateVector>(nx)), m2(ny, std::vector<MHDStateVector>(nx));
+        std::vector<std::vector<double> > phi(ny, std::vector<double>(nx));
+        for (int i = 0; i < ny; i++)
+            for (int j = 0; j < nx; j++)
+            {
+                m1[i][j] = mhdStateFrom(&in1[((size_t)i * nx + j) * 9]);
+                m2[i][j] = mhdStateFrom(&in2[((size_t)i * nx + j) * 9]);
+                phi[i][j] = phiIn[(size_t)i * nx + j];
+            }
+        MHDMultimaterialSystem initial(m1, m2, phi);
+        MHDMaterialParameters q1 = p1, q2 = p2;
+        double t = 0.0;
+        int it = 0;
+        while (t < tfinal && (maxSteps == 0 || it < maxSteps))
+        {
+            MHDMultimaterialSystem g = MHDRGFMSolver::applyRGFMBoundaryConditions2D(MHDMultimaterialSystem(m1, m2, phi), q1, q2);
+            m1 = g.getMaterial1Cells2D();
+            m2 = g.getMaterial2Cells2D();
+            Grid b1 = MHDSolvers::insertBoundaryCells2D(m1, 2);
+            Grid b2 = MHDSolvers::insertBoundaryCells2D(m2, 2);
+            double dt = std::min(MHDSolvers::computeStableTimeStep2D(b1, dx, cfl, t, tfinal, it, q1),
+                                 MHDSolvers::computeStableTimeStep2D(b2, dx, cfl, t, tfinal, it, q2));
+            phi = MHDRGFMSolver::update2DLevelSetFunctionX(phi, dx, 0.5 * dt, m1, m2);
+            phi = MHDRGFMSolver::update2DLevelSetFunctionY(phi, dx, dt, m1, m2);
+            phi = MHDRGFMSolver::update2DLevelSetFunctionX(phi, dx, 0.5 * dt, m1, m2);
+            q1.setHyperbolicWaveSpeed(4.0);
+            q2.setHyperbolicWaveSpeed(4.0);
+            q1.configureParabolicDamping();
+            q2.configureParabolicDamping();
+            Grid *mats[2] = {&m1, &m2};
+            MHDMaterialParameters *qs[2] = {&q1, &q2};
+            for (int k = 0; k < 2; k++)
+            {
+                Grid &m = *mats[k];
+                for (int s = 0; s < sub; s++)
+                {
+                    Grid b = MHDSolvers::insertBoundaryCells2D(m, 1);
+                    MHDForcingSolver::computeRungeKuttaTimeStep2D(m, b, dx, 0.5 * (dt / sub), bias, limiter, *qs[k]);
+                }
+                Grid b = MHDSolvers::insertBoundaryCells2D(m, 2);
+                MHDSecondOrderSolver::computeXSLICTimeStep2D(m, b, dx, 0.5 * dt, bias, limiter, *qs[k]);
+                b = MHDSolvers::insertBoundaryCells2D(m, 2);
+                MHDSecondOrderSolver::computeYSLICTimeStep2D(m, b, dx, dt, bias, limiter, *qs[k]);
+                b = MHDSolvers::insertBoundaryCells2D(m, 2);
+                MHDSecondOrderSolver::computeXSLICTimeStep2D(m, b, dx, 0.5 * dt, bias, limiter, *qs[k]);
+                for (int s = 0; s < sub; s++)
+                {
+                    Grid b2g = MHDSolvers::insertBoundaryCells2D(m, 1);
+                    MHDForcingSolver::computeRungeKuttaTimeStep2D(m, b2g, dx, 0.5 * (dt / sub), bias, limiter, *qs[k]);
+                }
+            }
+            t += dt;
+            it += 1;
+            dts.push_back(dt);
+        }
+        for (int i = 0; i < ny; i++)
+            for (int j = 0; j < nx; j++)
+            {
+                putMhdState(m1[i][j], &out1[((size_t)i * nx + j) * 9]);
+                putMhdState(m2[i][j], &out2[((size_t)i * nx + j) * 9]);
+                outPhi[(size_t)i * nx + j] = phi[i][j];
+            }
+        if (checkApi && maxSteps == 0)
+        {
+            MHDMultimaterialSystem api = MHDRGFMSolver::solve2D(initial, dx, cfl, tfinal, bias, limiter, sub, 0, p1, p2);
+            Grid a1 = api.getMaterial1Cells2D(), a2 = api.getMaterial2Cells2D();
+            std::vector<std::vector<double> > ap = api.getLevelSetFunction2D();
+            for (int i = 0; i < ny; i++)
+                for (int j = 0; j < nx; j++)
+                {
+                    double r1[9], r2[9];
+                    putMhdState(a1[i][j], r1);
+                    putMhdState(a2[i][j], r2);
+                    for (int f = 0; f < 9; f++)
+                        match = match && sameBits(r1[f], out1[((size_t)i * nx + j) * 9 + f]) &&
+                                sameBits(r2[f], out2[((size_t)i * nx + j) * 9 + f]);
+                    match = match && sameBits(ap[i][j], outPhi[(size_t)i * nx + j]);
+                }
+        }
+    }
+    if (checkApi && maxSteps == 0) std::fprintf(stderr, "API_MATCH %d\n", match ? 1 : 0);
+    std::fprintf(stderr, "STEPS %zu\n", dts.size());
+    writeDoubles(argv[23], out1);
+    writeDoubles(argv[24], out2);
+    writeDoubles(argv[25], outPhi);
+    writeDoubles(argv[26], dts);
+    return 0;
+}
+
+// mhd_hllc_units n p1(4) p2(4) in out: per record of two states a, b:
+// solveX(a, b) [9] | solveY(a, b) [9] | left, right, top, bottom star densities
+// [4] | star x, y speeds [2].
+int mhdHllcUnits(int argc, char **argv)
+{
+    if (argc < 13) { std::fprintf(stderr, "mhd_hllc_units: too few arguments\n"); return 2; }
+    int n = std::atoi(argv[2]);
+    MHDMaterialParameters p1 = mhdParams4(argv + 3);
+    MHDMaterialParameters p2 = mhdParams4(argv + 7);
+    std::vector<double> in = readDoubles(argv[11], (size_t)n * 18);
+    std::vector<double> out((size_t)n * 24);
+    for (int r = 0; r < n; r++)
+    {
+        MHDStateVector a = mhdStateFrom(&in[(size_t)r * 18]), b = mhdStateFrom(&in[(size_t)r * 18 + 9]);
+        double *o = &out[(size_t)r * 24];
+        putMhdState(MHDHLLCSolver::solveX(a, b, p1, p2), o);
+        putMhdState(MHDHLLCSolver::solveY(a, b, p1, p2), o + 9);
+        o[18] = MHDHLLCSolver::computeLeftStarRegionDensity(a, b, p1, p2);
+        o[19] = MHDHLLCSolver::computeRightStarRegionDensity(a, b, p1, p2);
+        o[20] = MHDHLLCSolver::computeTopStarRegionDensity(a, b, p1, p2);
+        o[21] = MHDHLLCSolver::computeBottomStarRegionDensity(a, b, p1, p2);
+        o[22] = MHDHLLCSolver::computeStarRegionXVelocity(a, b, p1, p2);
+        o[23] = MHDHLLCSolver::computeStarRegionYVelocity(a, b, p1, p2);
+    }
+    writeDoubles(argv[12], out);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) { std::fprintf(stderr, "usage: ref_harness <mode> ...\n"); return 2; }
@@ -1397,6 +1599,8 @@ int main(int argc, char **argv)
     if (mode == "hllc_units") return hllcUnits(argc, argv);
     if (mode == "dat") return datOut(argc, argv);
     if (mode == "mp_run") return mpRun(argc, argv);
+    if (mode == "mhd_rgfm_run") return mhdRgfmRun(argc, argv);
+    if (mode == "mhd_hllc_units") return mhdHllcUnits(argc, argv);
     if (mode == "mp_units") return mpUnits(argc, argv);
     if (mode == "mp_reinit") return mpReinit(argc, argv);
     std::fprintf(stderr, "unknown mode %s\n", mode.c_str());

@@ -797,6 +797,91 @@ JOBS = {
 JOBS.update(MP_JOBS)
 
 
+# ---- two-material MHD RGFM ------------------------------------------------------
+def mhd_rgfm_call(m1, m2, phi, *, dim, dx, cfl, tfinal, max_steps, bias, limiter, subcycles, p1, p2, check_api):
+    with tempfile.TemporaryDirectory() as tmp:
+        ny, nx = (1, phi.shape[0]) if dim == 1 else phi.shape[:2]
+        for nm, a in (("in1.f64", m1), ("in2.f64", m2), ("phi.f64", phi)):
+            np.ascontiguousarray(a, dtype=np.float64).tofile(os.path.join(tmp, nm))
+        t = "inf" if np.isinf(tfinal) else repr(float(tfinal))
+        log = run(["mhd_rgfm_run", dim, ny, nx, repr(dx), repr(cfl), t, max_steps, repr(bias), limiter, subcycles,
+                   *[repr(v) for v in p1], *[repr(v) for v in p2], "in1.f64", "in2.f64", "phi.f64", "o1.f64",
+                   "o2.f64", "ophi.f64", "dt.f64", int(check_api)], tmp)
+        o1 = np.fromfile(os.path.join(tmp, "o1.f64")).reshape(m1.shape)
+        o2 = np.fromfile(os.path.join(tmp, "o2.f64")).reshape(m2.shape)
+        ophi = np.fromfile(os.path.join(tmp, "ophi.f64")).reshape(phi.shape)
+        dt = np.fromfile(os.path.join(tmp, "dt.f64"))
+    if check_api and max_steps == 0 and "API_MATCH 1" not in log:
+        raise RuntimeError("MHD RGFM mirror loop differs from the reference's own solve")
+    return o1, o2, ophi, dt
+
+
+def mhd_rgfm_run(name, m1, m2, phi, **kw):
+    o1, o2, ophi, dt = mhd_rgfm_call(m1, m2, phi, **kw)
+    meta = dict(kw)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), m1_in=m1, m2_in=m2, phi_in=phi, m1_out=o1, m2_out=o2,
+                        phi_out=ophi, dt=dt, meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
+    print(f"{name}: {len(dt)} steps, finite {bool(np.isfinite(o1).all() and np.isfinite(o2).all())}")
+
+
+def mhd_hllc_units_fixture(name, n, seed):
+    """Random state pairs across the HLLC branches (supersonic both ways, star
+    left and right), for three material pairs with GLM speeds."""
+    rng = np.random.default_rng(seed)
+    sets = [((2.0, 0.0, 0.0, 0.0), (2.0, 0.0, 0.0, 0.0)), ((1.4, 0.0, 1.3, 0.5), (1.67, 0.0, 0.7, 0.4)),
+            ((3.0, 0.4, 2.0, 0.6), (1.4, 0.0, 2.5, 0.7))]
+    out = {}
+    for k, (p1, p2) in enumerate(sets):
+        st = np.empty((n, 18))
+        for s in (0, 9):
+            st[:, s + 0] = 10.0 ** rng.uniform(-1, 0.5, n)
+            st[:, s + 1:s + 4] = rng.uniform(-1.5, 1.5, (n, 3))
+            st[:, s + 4] = 10.0 ** rng.uniform(-1, 1, n)
+            st[:, s + 5:s + 8] = rng.uniform(-1.2, 1.2, (n, 3))
+            st[:, s + 8] = rng.uniform(-0.3, 0.3, n)
+        fast = rng.random(n) < 0.2      # a fifth with a strong common drift: supersonic cases
+        st[fast, 1] += 6.0 * np.sign(rng.uniform(-1, 1, fast.sum()))
+        st[fast, 10] = st[fast, 1]
+        st[fast, 2] += 6.0 * np.sign(rng.uniform(-1, 1, fast.sum()))
+        st[fast, 11] = st[fast, 2]
+        with tempfile.TemporaryDirectory() as tmp:
+            st.tofile(os.path.join(tmp, "in.f64"))
+            run(["mhd_hllc_units", n, *[repr(v) for v in p1], *[repr(v) for v in p2], "in.f64", "out.f64"], tmp)
+            out[f"out{k}"] = np.fromfile(os.path.join(tmp, "out.f64")).reshape(n, 24)
+        out[f"states{k}"] = st
+        out[f"p1_{k}"] = np.array(p1)
+        out[f"p2_{k}"] = np.array(p2)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print(f"{name}: {n} x {len(sets)} records")
+
+
+MHD_RGFM_JOBS = {
+    "mhd_rgfm_dumbser200": lambda: mhd_rgfm_run(
+        "mhd_rgfm_dumbser200", *problems.mhd_rgfm_dumbser_1d(200), dim=1, dx=1 / 200, cfl=0.8, tfinal=0.1,
+        max_steps=0, bias=0.0, limiter=0, subcycles=0, p1=(2.0, 0.0, 0.0, 0.0), p2=(2.0, 0.0, 0.0, 0.0),
+        check_api=True),
+    "mhd_rgfm_dumbser_multi200": lambda: mhd_rgfm_run(
+        "mhd_rgfm_dumbser_multi200", *problems.mhd_rgfm_dumbser_1d(200), dim=1, dx=1 / 200, cfl=0.8, tfinal=0.2,
+        max_steps=0, bias=0.0, limiter=0, subcycles=0, p1=(1.4, 0.0, 0.0, 0.0), p2=(1.67, 0.0, 0.0, 0.0),
+        check_api=True),
+    "mhd_rgfm_vl_glm100": lambda: mhd_rgfm_run(
+        "mhd_rgfm_vl_glm100", *problems.mhd_rgfm_dumbser_1d(100), dim=1, dx=1 / 100, cfl=0.7, tfinal=0.1,
+        max_steps=0, bias=0.2, limiter=1, subcycles=0, p1=(1.4, 0.0, 1.5, 0.5), p2=(1.67, 0.2, 1.1, 0.45),
+        check_api=True),
+    "mhd_rgfm_2d_dumbser32": lambda: mhd_rgfm_run(
+        "mhd_rgfm_2d_dumbser32", *problems.mhd_rgfm_dumbser_2d(32), dim=2, dx=1 / 32, cfl=0.8, tfinal=0.05,
+        max_steps=0, bias=0.0, limiter=0, subcycles=0, p1=(2.0, 0.0, 0.0, 0.0), p2=(2.0, 0.0, 0.0, 0.0),
+        check_api=True),
+    "mhd_rgfm_2d_multi32_sub1": lambda: mhd_rgfm_run(
+        "mhd_rgfm_2d_multi32_sub1", *problems.mhd_rgfm_dumbser_2d(32), dim=2, dx=1 / 32, cfl=0.8, tfinal=0.05,
+        max_steps=0, bias=0.0, limiter=0, subcycles=1, p1=(1.4, 0.0, 0.0, 0.0), p2=(1.67, 0.0, 0.0, 0.0),
+        check_api=True),
+    "mhd_hllc_units": lambda: mhd_hllc_units_fixture("mhd_hllc_units", 300, 51),
+}
+JOBS.update(MHD_RGFM_JOBS)
+
+
+
 def main(names):
     if not os.path.exists(HARNESS):
         raise SystemExit("build the reference harness first: make -C oracle ref")

@@ -96,6 +96,13 @@ def load():
     lib.orc_euler_branch_hash.argtypes = []
     lib.orc_rgfm_circulation.restype = C.c_double
     lib.orc_rgfm_circulation.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, dptr, dptr]
+    mq = C.POINTER(OrcMHDParams)
+    lib.orc_mhd_rgfm_solve.restype = C.c_int
+    lib.orc_mhd_rgfm_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                       C.c_double, C.c_int, C.c_int, mq, mq, dptr, dptr, dptr, dptr, dptr, dptr, dptr,
+                                       C.c_int]
+    lib.orc_mhd_hllc_units.restype = None
+    lib.orc_mhd_hllc_units.argtypes = [C.c_int, mq, mq, dptr, dptr]
     ep = C.POINTER(OrcEulerParams)
     lib.orc_mp_solve.restype = C.c_int
     lib.orc_mp_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
@@ -410,6 +417,30 @@ def limiter_units(r, bias):
     r = np.ascontiguousarray(r, dtype=np.float64)
     out = np.empty((r.size, 5))
     lib.orc_limiter_units(r.size, bias, _p(r), _p(out))
+    return out
+
+
+def mhd_rgfm_solve(m1, m2, phi, *, dim, dx, cfl, tfinal, max_steps, bias, limiter, subcycles, p1, p2,
+                   dt_cap=200000):
+    """p1, p2: (gamma, p_inf, c_h, c_p) of each material."""
+    lib = load()
+    a1 = np.ascontiguousarray(m1, dtype=np.float64)
+    a2 = np.ascontiguousarray(m2, dtype=np.float64)
+    ph = np.ascontiguousarray(phi, dtype=np.float64)
+    o1, o2, oph = np.empty_like(a1), np.empty_like(a2), np.empty_like(ph)
+    dts = np.empty(dt_cap)
+    ny, nx = (1, ph.shape[0]) if dim == 1 else ph.shape
+    n = lib.orc_mhd_rgfm_solve(dim, ny, nx, dx, cfl, tfinal, max_steps, bias, limiter, subcycles,
+                               C.byref(OrcMHDParams(*p1)), C.byref(OrcMHDParams(*p2)), _p(a1), _p(a2), _p(ph),
+                               _p(o1), _p(o2), _p(oph), _p(dts), dt_cap)
+    return o1, o2, oph, dts[:min(n, dt_cap)].copy()
+
+
+def mhd_hllc_units(states, p1, p2):
+    lib = load()
+    st = np.ascontiguousarray(states, dtype=np.float64)
+    out = np.empty((st.shape[0], 24))
+    lib.orc_mhd_hllc_units(st.shape[0], C.byref(OrcMHDParams(*p1)), C.byref(OrcMHDParams(*p2)), _p(st), _p(out))
     return out
 
 

@@ -333,3 +333,29 @@ def two_material_fedkiw_2d(n: int, kind: int = 0) -> np.ndarray:
             else:
                 w[i, j] = _two_material(kind, 0.001, 0.0, 1.0, 0.1379, 1.0e5, 1.0e5, 1.0e5)
     return w
+
+
+# ---- two-material MHD (RGFM) ICs -------------------------------------------------
+# States (rho, u, v, w, p, Bx, By, Bz, psi).
+
+def mhd_rgfm_dumbser_1d(n: int):
+    """The Brio-Wu-like two-material tube of MHDRGFMTests::solveDumbserTest1 and
+    solveDumbserMultimaterialTest1 (RGFM/mhdrgfmtests.cpp:7-43): material 1
+    (rho 1, p 1, B = (0.75, 1, 0)) and material 2 (rho 0.125, p 0.1,
+    B = (0.75, -1, 0)) everywhere, phi_i = i dx - 0.5, dx = 1/n."""
+    m1 = np.tile([1.0, 0.0, 0.0, 0.0, 1.0, 0.75, 1.0, 0.0, 0.0], (n, 1))
+    m2 = np.tile([0.125, 0.0, 0.0, 0.0, 0.1, 0.75, -1.0, 0.0, 0.0], (n, 1))
+    phi = ((1.0 / n) * np.arange(n, dtype=np.float64)) - 0.5
+    return m1, m2, phi
+
+
+def mhd_rgfm_dumbser_2d(n: int):
+    """The cylindrical version, solve2DDumbserTest1 / solve2DDumbserMultimaterialTest1
+    (mhdrgfmtests.cpp:45-97): the same materials, phi = r / n - 0.2 with r the
+    distance in cells from (n/2, n/2)."""
+    m1 = np.tile([1.0, 0.0, 0.0, 0.0, 1.0, 0.75, 1.0, 0.0, 0.0], (n, n, 1))
+    m2 = np.tile([0.125, 0.0, 0.0, 0.0, 0.1, 0.75, -1.0, 0.0, 0.0], (n, n, 1))
+    i = np.arange(n, dtype=np.float64)[:, None] - (0.5 * n)
+    j = np.arange(n, dtype=np.float64)[None, :] - (0.5 * n)
+    phi = (np.sqrt((i * i) + (j * j)) / n) - 0.2
+    return m1, m2, phi
