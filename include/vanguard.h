@@ -457,6 +457,38 @@ int vg_rgfm_solve_fixed_dt_2d(vg_ctx *ctx, const double *m1_in, const double *m2
                               const double *dts, int n_steps, double bias, int limiter, const vg_euler_params *p1,
                               const vg_euler_params *p2);
 
+/* ---- two-material ideal MHD: real ghost fluid method ------------------- */
+
+/* MHDRGFMSolver::solve (mhdrgfmsolver.h:27-28, mhdrgfmsolver.cpp:543-581) and
+ * solve2D (:29-30, :584-693).  MHDMultimaterialSystem is passed as its three
+ * arrays: both materials' primitive states (rho, u, v, w, p, Bx, By, Bz, psi;
+ * 9 doubles per cell, rows contiguous in 2D) and the level set.  Interface
+ * states by MHDHLLCSolver (RGFM/mhdhllcsolver.cpp), ghost pressures floored at
+ * 1e-8; dt = the smaller of the two materials' stable steps.  The 1D solve
+ * uses the caller's c_h, c_p and ignores `subcycles`, as the reference does;
+ * solve2D sets c_h = 4, c_p = sqrt(0.18 c_h) after the first step's ghost fill
+ * and level set, and runs `subcycles` RK4 GLM sub-cycles before and after each
+ * material's X(dt/2) Y(dt) X(dt/2) sweeps.  The reference's
+ * reinitialisationFrequency is unused there and has no argument here.
+ * Transmissive boundaries; lines of at most 16384 cells. */
+int vg_mhd_rgfm_solve_1d(vg_ctx *ctx, const double *m1_in, const double *m2_in, const double *phi_in,
+                         double *m1_out, double *m2_out, double *phi_out, int n, double dx, double cfl,
+                         double t_final, int max_steps, double bias, int limiter, int subcycles,
+                         const vg_mhd_params *p1, const vg_mhd_params *p2, int mode, double *dt_log, int dt_log_cap,
+                         vg_run_stats *stats);
+int vg_mhd_rgfm_solve_2d(vg_ctx *ctx, const double *m1_in, const double *m2_in, const double *phi_in,
+                         double *m1_out, double *m2_out, double *phi_out, int ny, int nx, double dx, double cfl,
+                         double t_final, int max_steps, double bias, int limiter, int subcycles,
+                         const vg_mhd_params *p1, const vg_mhd_params *p2, int mode, double *dt_log, int dt_log_cap,
+                         vg_run_stats *stats);
+
+/* MHDRGFMSolver::applyRGFMBoundaryConditions (dim 1, mhdrgfmsolver.cpp:7-127:
+ * psi left alone) and applyRGFMBoundaryConditions2D (dim 2, :129-378: rows,
+ * then columns, psi written), in place on host arrays m1, m2 (ny x nx x 9; ny
+ * ignored in 1D). */
+int vg_mhd_rgfm_ghost_fill(vg_ctx *ctx, int dim, int ny, int nx, double *m1, double *m2, const double *phi,
+                           const vg_mhd_params *p1, const vg_mhd_params *p2);
+
 /* ---- Godunov-Peshkov-Romenski (HPR) continuum ---------------------------- */
 
 enum {

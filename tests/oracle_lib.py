@@ -103,6 +103,8 @@ def load():
                                        C.c_int]
     lib.orc_mhd_hllc_units.restype = None
     lib.orc_mhd_hllc_units.argtypes = [C.c_int, mq, mq, dptr, dptr]
+    lib.orc_mhd_rgfm_ghost_fill.restype = None
+    lib.orc_mhd_rgfm_ghost_fill.argtypes = [C.c_int, C.c_int, C.c_int, mq, mq, dptr, dptr, dptr]
     ep = C.POINTER(OrcEulerParams)
     lib.orc_mp_solve.restype = C.c_int
     lib.orc_mp_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
@@ -434,6 +436,18 @@ def mhd_rgfm_solve(m1, m2, phi, *, dim, dx, cfl, tfinal, max_steps, bias, limite
                                C.byref(OrcMHDParams(*p1)), C.byref(OrcMHDParams(*p2)), _p(a1), _p(a2), _p(ph),
                                _p(o1), _p(o2), _p(oph), _p(dts), dt_cap)
     return o1, o2, oph, dts[:min(n, dt_cap)].copy()
+
+
+def mhd_rgfm_ghost_fill(m1, m2, phi, p1, p2):
+    """applyRGFMBoundaryConditions[2D] on copies of m1, m2 (dim from phi)."""
+    lib = load()
+    a1 = np.array(m1, dtype=np.float64, order="C")
+    a2 = np.array(m2, dtype=np.float64, order="C")
+    ph = np.ascontiguousarray(phi, dtype=np.float64)
+    ny, nx = (1, ph.shape[0]) if ph.ndim == 1 else ph.shape
+    lib.orc_mhd_rgfm_ghost_fill(ph.ndim, ny, nx, C.byref(OrcMHDParams(*p1)), C.byref(OrcMHDParams(*p2)), _p(a1),
+                                _p(a2), _p(ph))
+    return a1, a2
 
 
 def mhd_hllc_units(states, p1, p2):

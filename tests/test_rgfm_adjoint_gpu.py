@@ -158,3 +158,21 @@ def test_adjoint_to_final_time_on_the_benchmark_geometry():
     lhs = float(np.sum(adj.grad1 * v1) + np.sum(adj.grad2 * v2))
     rhs = float(np.sum(s1 * t1) + np.sum(s2 * t2))
     assert abs(lhs - rhs) <= 1e-9 * float(np.sum(np.abs(adj.grad1 * v1)) + np.sum(np.abs(adj.grad2 * v2)))
+
+
+def test_overflowing_adjoint_names_the_step():
+    """An adjoint that overflows is reported, not returned: a seed of 1e308
+    overflows in the reversal of the last steps, and the entry point returns
+    VG_ERR_NONFINITE with the forward step (0-based) whose reversal first
+    produced a non-finite value (the per-step scan of the reverse sweep)."""
+    case = rgfm_case(2, tie=False)
+    m1, m2, phi, dx, dy = case
+    s1 = np.full(m1.shape, 1e308)
+    adj = S.rgfm_adjoint(S.MultimaterialSystem(m1, m2, phi), s1, np.zeros_like(m2), dx, 0.8, math.inf, HELIUM, AIR,
+                         dy=dy, max_steps=9, raise_on_error=False)
+    assert adj.status == S._lib.VG_ERR_NONFINITE
+    msg = S.default_context().lib.vg_last_error().decode()
+    print(msg, flush=True)
+    step = int(msg.split("reverse of step ")[1].split(" of ")[0])
+    assert msg.endswith(" of 9") and 5 <= step <= 8
+    assert adj.stats.nonfinite_cells > 0

@@ -38,6 +38,7 @@ struct ForcingArgs {
     typename P::Params m;
     Clock *clk;
     unsigned long long *partials;
+    int fixed_params = 0;   // keep m as given: no per-step parameters from the clock
 };
 
 constexpr int kForceBS = 128;
@@ -184,7 +185,7 @@ __global__ __launch_bounds__(kForceBS) void forcing_kernel(ForcingArgs<P> a)
     constexpr int NS = T::kMaxSlots;
     if (a.clk->done) return;
     typename P::Params m = a.m;
-    P::begin_step(m, a.clk->S);   // sub-cycles run in second-order steps only
+    if (!a.fixed_params) P::begin_step(m, a.clk->S);   // sub-cycles run in second-order steps only
     const double ts = 0.5 * (a.clk->dt / a.subcycles);
     const long long cells = (long long)a.ny * a.nx;
     const long long c = (long long)blockIdx.x * kForceBS + threadIdx.x;

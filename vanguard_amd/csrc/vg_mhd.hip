@@ -5,44 +5,13 @@
 #include <vector>
 
 #include "vg_batch_host.h"
-#include "vg_forcing.h"
-#include "vg_grid.h"
-#include "vg_phys_mhd.h"
-
-namespace vg {
-
-// Only psi's slot of the MHD source is non-zero (mhdstatevector.cpp:222-234).
-template <>
-struct ForcingTraits<MHDPhys> {
-    static constexpr int kMaxSlots = 1;
-    static constexpr const char *kTimer = "mhd_source_rk4";
-    __device__ static __forceinline__ constexpr int slot(int) { return 8; }
-    __device__ static __forceinline__ int count(const MHDPhys::Params &) { return 1; }
-    __device__ static __forceinline__ void finish(const MHDPhys::Params &, const double (&)[9], double (&)[9]) {}
-};
-
-// The GLM damping sub-cycle launch for GridState<MHDPhys>::enqueue_step.
-template <>
-struct SourceStage<MHDPhys> : ForcingStage<MHDPhys> {};
-
-}  // namespace vg
+#include "vg_mhd_host.h"
 
 namespace {
 
 using vg::MHDPhys;
 
-MHDPhys::Params to_params(const vg_mhd_params *p)
-{
-    MHDPhys::Params m;
-    m.gamma = p->gamma;
-    m.p_inf = p->p_inf;
-    m.ch = p->ch;
-    m.cp = p->cp;
-    m.gm1 = p->gamma - 1;
-    m.gpinf = p->gamma * p->p_inf;
-    m.inv_gm1 = 1.0 / (p->gamma - 1);
-    return m;
-}
+MHDPhys::Params to_params(const vg_mhd_params *p) { return vg::mhd_params(p); }
 
 constexpr int kBatchBS = 128;
 

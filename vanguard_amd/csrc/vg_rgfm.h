@@ -354,6 +354,26 @@ __device__ __forceinline__ void exact_interface(const State &a, const State &b, 
 
 // ---- line kernels -------------------------------------------------------------
 
+// Loop test of the step about to start (secondordersolver.cpp:295 style
+// `while (currentTime < finalTime)`): folds in the previous step and stops the
+// clock before anything of the next step runs.
+static __global__ void clock_check_kernel(Clock *c)
+{
+    if (c->done) return;
+    double t = c->t;
+    int it = c->it;
+    if (c->pending) {
+        t += c->dt;
+        it += 1;
+    }
+    if (!(t < c->t_final) || (c->max_steps > 0 && it >= c->max_steps)) {
+        c->t = t;
+        c->it = it;
+        c->pending = 0;
+        c->done = 1;
+    }
+}
+
 constexpr int kLineBS = 256;
 
 // Exclusive prefix count of `flag` over the block (kLineBS threads); `total`
@@ -546,7 +566,7 @@ __global__ __launch_bounds__(kLineBS) void rgfm_level_set_kernel(LineArgs a)
 // arithmetic are those of rgfm_level_set_kernel<1>, bit for bit.
 constexpr int kLsCols = 8, kLsSegs = 64;
 
-__global__ __launch_bounds__(kLsCols * kLsSegs) void rgfm_level_set_y_kernel(LineArgs a)
+static __global__ __launch_bounds__(kLsCols * kLsSegs) void rgfm_level_set_y_kernel(LineArgs a)
 {
     __shared__ int s_cnt[kLsSegs][kLsCols];
     if (a.clk->done) return;

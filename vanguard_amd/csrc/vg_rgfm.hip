@@ -23,26 +23,6 @@
 
 namespace vg {
 
-// Loop test of the step about to start (secondordersolver.cpp:295 style
-// `while (currentTime < finalTime)`): folds in the previous step and stops the
-// clock before anything of the next step runs.
-static __global__ void clock_check_kernel(Clock *c)
-{
-    if (c->done) return;
-    double t = c->t;
-    int it = c->it;
-    if (c->pending) {
-        t += c->dt;
-        it += 1;
-    }
-    if (!(t < c->t_final) || (c->max_steps > 0 && it >= c->max_steps)) {
-        c->t = t;
-        c->it = it;
-        c->pending = 0;
-        c->done = 1;
-    }
-}
-
 // The inside of one step, kept for the adjoint: both materials after the x
 // ghost fill (P1) and after the y fill (P2, the first x sweep's input), after
 // the first x sweep (Q1, the y sweep's input) and after the y sweep (Q2, the

@@ -1274,3 +1274,101 @@ class RGFMSolver:
                                  material1_parameters, material2_parameters, EXACT_RIEMANN, **kw)
 
     solveExact2D = solveExact
+
+
+# ---- two-material ideal MHD: real ghost fluid method --------------------------------
+
+@dataclass
+class MHDMultimaterialSystem(MultimaterialSystem):
+    """MHDMultimaterialSystem (RGFM/mhdmultimaterialsystem.h): both materials'
+    MHD primitive states (rho, u, v, w, p, Bx, By, Bz, psi) and the level set."""
+
+
+def _mhd_params(p):
+    if isinstance(p, MHDMaterialParameters):
+        return p
+    return MHDMaterialParameters(*p) if isinstance(p, (tuple, list)) else MHDMaterialParameters(float(p))
+
+
+def _mhd_system_arrays(system):
+    m1 = np.ascontiguousarray(system.material1, dtype=np.float64)
+    m2 = np.ascontiguousarray(system.material2, dtype=np.float64)
+    phi = np.ascontiguousarray(system.level_set, dtype=np.float64)
+    if m1.shape != m2.shape or m1.shape[:-1] != phi.shape or m1.shape[-1] != 9 or phi.ndim not in (1, 2):
+        raise ValueError("materials must be (..., 9) over the level set's 1D or 2D shape")
+    return m1, m2, phi
+
+
+def mhd_rgfm_solve(system: MultimaterialSystem, dx: float, cfl: float, t_final: float, params1, params2, *,
+                   bias: float = 0.0, limiter: int = 0, subcycles: int = 0, max_steps: int = 0, mode: int = EXACT,
+                   dt_cap: int = 200000, ctx: Context | None = None, raise_on_error: bool = True) -> RGFMResult:
+    """MHDRGFMSolver::solve / solve2D through vg_mhd_rgfm_solve_1d / _2d (the
+    1D solve ignores `subcycles`, as the reference's does)."""
+    ctx = ctx or default_context()
+    m1, m2, phi = _mhd_system_arrays(system)
+    o1, o2, oph = np.empty_like(m1), np.empty_like(m2), np.empty_like(phi)
+    dts = np.empty(dt_cap)
+    st = _lib.RunStats()
+    q1, q2 = _mhd_params(params1).c_struct(), _mhd_params(params2).c_struct()
+    P = _lib.as_dptr
+    if phi.ndim == 1:
+        rc = ctx.lib.vg_mhd_rgfm_solve_1d(ctx.handle, P(m1), P(m2), P(phi), P(o1), P(o2), P(oph), phi.shape[0], dx,
+                                          cfl, t_final, max_steps, bias, limiter, subcycles, C.byref(q1), C.byref(q2),
+                                          mode, P(dts), dt_cap, C.byref(st))
+    else:
+        rc = ctx.lib.vg_mhd_rgfm_solve_2d(ctx.handle, P(m1), P(m2), P(phi), P(o1), P(o2), P(oph), phi.shape[0],
+                                          phi.shape[1], dx, cfl, t_final, max_steps, bias, limiter, subcycles,
+                                          C.byref(q1), C.byref(q2), mode, P(dts), dt_cap, C.byref(st))
+    if raise_on_error and rc != _lib.VG_OK:
+        check(rc, "vg_mhd_rgfm_solve")
+    return RGFMResult(MHDMultimaterialSystem(o1, o2, oph), dts[:min(st.steps, dt_cap)].copy(), st, rc)
+
+
+def mhd_rgfm_ghost_fill(system: MultimaterialSystem, params1, params2, *,
+                        ctx: Context | None = None) -> MHDMultimaterialSystem:
+    """applyRGFMBoundaryConditions[2D] through vg_mhd_rgfm_ghost_fill; returns
+    a new system (the level set unchanged)."""
+    ctx = ctx or default_context()
+    m1, m2, phi = _mhd_system_arrays(system)
+    m1, m2 = m1.copy(), m2.copy()
+    q1, q2 = _mhd_params(params1).c_struct(), _mhd_params(params2).c_struct()
+    ny, nx = (1, phi.shape[0]) if phi.ndim == 1 else phi.shape
+    rc = ctx.lib.vg_mhd_rgfm_ghost_fill(ctx.handle, phi.ndim, ny, nx, _lib.as_dptr(m1), _lib.as_dptr(m2),
+                                        _lib.as_dptr(phi), C.byref(q1), C.byref(q2))
+    check(rc, "vg_mhd_rgfm_ghost_fill")
+    return MHDMultimaterialSystem(m1, m2, phi.copy())
+
+
+class MHDRGFMSolver:
+    """MHDRGFMSolver (RGFM/mhdrgfmsolver.h:8-31): ghost fills and the whole
+    solves (reinitialisation_frequency is unused by the reference's solves)."""
+
+    @staticmethod
+    def applyRGFMBoundaryConditions(system, material1_parameters, material2_parameters, **kw):
+        if np.ndim(system.level_set) != 1:
+            raise ValueError("applyRGFMBoundaryConditions takes a 1D system")
+        return mhd_rgfm_ghost_fill(system, material1_parameters, material2_parameters, **kw)
+
+    @staticmethod
+    def applyRGFMBoundaryConditions2D(system, material1_parameters, material2_parameters, **kw):
+        if np.ndim(system.level_set) != 2:
+            raise ValueError("applyRGFMBoundaryConditions2D takes a 2D system")
+        return mhd_rgfm_ghost_fill(system, material1_parameters, material2_parameters, **kw)
+
+    @staticmethod
+    def solve(system, cell_spacing, cfl_coefficient, final_time, bias, slope_limiter, subcycling_iterations,
+              reinitialisation_frequency, material1_parameters, material2_parameters, **kw):
+        if np.ndim(system.level_set) != 1:
+            raise ValueError("solve takes a 1D system; use solve2D")
+        return mhd_rgfm_solve(system, cell_spacing, cfl_coefficient, final_time, material1_parameters,
+                              material2_parameters, bias=bias, limiter=slope_limiter,
+                              subcycles=subcycling_iterations, **kw).system
+
+    @staticmethod
+    def solve2D(system, cell_spacing, cfl_coefficient, final_time, bias, slope_limiter, subcycling_iterations,
+                reinitialisation_frequency, material1_parameters, material2_parameters, **kw):
+        if np.ndim(system.level_set) != 2:
+            raise ValueError("solve2D takes a 2D system")
+        return mhd_rgfm_solve(system, cell_spacing, cfl_coefficient, final_time, material1_parameters,
+                              material2_parameters, bias=bias, limiter=slope_limiter,
+                              subcycles=subcycling_iterations, **kw).system
