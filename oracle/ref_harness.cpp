@@ -1580,6 +1580,33 @@ int mhdHllcUnits(int argc, char **argv)
     return 0;
 }
 
+// The reference's own test drivers, end to end: each builds its initial
+// condition, runs the reference's solve and writes its .dat files into the
+// working directory (Euler/eulertests.cpp:7-52, RGFM/eulerrgfmtests.cpp,
+// MHD/mhdtests.cpp:7-120, HPR/hprtests.cpp:7-57).
+//   driver <name> <cellCount> <subcycles or reinitialisation frequency>
+int driverRun(int argc, char **argv)
+{
+    if (argc < 5) { std::fprintf(stderr, "driver: too few arguments\n"); return 2; }
+    const std::string name = argv[2];
+    const int n = std::atoi(argv[3]), k = std::atoi(argv[4]);
+    if (name == "euler_toro1") EulerTests::solveToroTest1(n);
+    else if (name == "euler_2d_toro1") EulerTests::solve2DToroTest1(n);
+    else if (name == "rgfm_toro1_hllc") EulerRGFMTests::solveToroTest1HLLC(n, k);
+    else if (name == "rgfm_fedkiw_hllc") EulerRGFMTests::solveFedkiwTestHLLC(n, k);
+    else if (name == "rgfm_2d_toro1_hllc") EulerRGFMTests::solve2DToroTest1HLLC(n, k);
+    else if (name == "rgfm_2d_fedkiw_hllc") EulerRGFMTests::solve2DFedkiwTestHLLC(n, k);
+    else if (name == "mhd_dumbser1") MHDTests::solveDumbserTest1(n, k);
+    else if (name == "mhd_dumbser2") MHDTests::solveDumbserTest2(n, k);
+    else if (name == "mhd_dumbser3") MHDTests::solveDumbserTest3(n, k);
+    else if (name == "mhd_2d_dumbser1") MHDTests::solve2DDumbserTest1(n, k);
+    else if (name == "mhd_2d_dumbser2") MHDTests::solve2DDumbserTest2(n, k);
+    else if (name == "hpr_stokes") HPRTests::solveStokesFirstProblem(n, k);
+    else if (name == "hpr_heat") HPRTests::solveHeatConductionProblem(n, k);
+    else { std::fprintf(stderr, "driver: unknown %s\n", name.c_str()); return 2; }
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) { std::fprintf(stderr, "usage: ref_harness <mode> ...\n"); return 2; }
@@ -1598,6 +1625,7 @@ int main(int argc, char **argv)
     if (mode == "rgfm_run") return rgfmRun(argc, argv);
     if (mode == "hllc_units") return hllcUnits(argc, argv);
     if (mode == "dat") return datOut(argc, argv);
+    if (mode == "driver") return driverRun(argc, argv);
     if (mode == "mp_run") return mpRun(argc, argv);
     if (mode == "mhd_rgfm_run") return mhdRgfmRun(argc, argv);
     if (mode == "mhd_hllc_units") return mhdHllcUnits(argc, argv);

@@ -2,7 +2,9 @@
 (EulerTests / MHDTests / HPRTests / EulerRGFMTests ::outputSolution[2D]) run by
 the compiled reference harness (`oracle/_ref/ref_harness dat ...`) on seeded
 states, with the inputs and every file's bytes stored as plain arrays (no
-pickles).  Run in the build container only:
+pickles); and tests/golden/driver_fixtures.npz: the reference's test drivers
+run end to end (`ref_harness driver ...`: their initial conditions, solves and
+writers), every file's bytes.  Run in the build container only:
 
     python tests/golden/gen_dat.py
 """
@@ -84,7 +86,35 @@ def run(args, tmp):
         raise RuntimeError(f"harness failed: {r.stderr}")
 
 
+# (driver, cellCount, sub-cycles or reinitialisation frequency): the reference's
+# EulerTests, EulerRGFMTests (HLLC), MHDTests and HPRTests drivers
+DRIVERS = [("euler_toro1", 100, 0), ("euler_2d_toro1", 32, 0), ("rgfm_toro1_hllc", 100, 0),
+           ("rgfm_fedkiw_hllc", 100, 0), ("rgfm_2d_toro1_hllc", 32, 0), ("rgfm_2d_fedkiw_hllc", 32, 0),
+           ("mhd_dumbser1", 100, 0), ("mhd_dumbser2", 100, 1), ("mhd_dumbser3", 100, 0), ("mhd_2d_dumbser1", 32, 0),
+           ("mhd_2d_dumbser2", 32, 1), ("hpr_stokes", 50, 1), ("hpr_heat", 100, 1)]
+
+
+def drivers():
+    out = {}
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    for name, n, k in DRIVERS:
+        with tempfile.TemporaryDirectory() as tmp:
+            r = subprocess.run([HARNESS, "driver", name, str(n), str(k)], capture_output=True, text=True, cwd=tmp,
+                               env=env)
+            if r.returncode != 0:
+                raise RuntimeError(f"driver {name} failed: {r.stderr}")
+            files = sorted(f for f in os.listdir(tmp) if f.endswith(".dat"))
+            out[f"{name}__args"] = np.array([n, k])
+            out[f"{name}__files"] = np.array(files)
+            for f in files:
+                with open(os.path.join(tmp, f), "rb") as fh:
+                    out[f"{name}__file__{f}"] = np.frombuffer(fh.read(), dtype=np.uint8)
+        print(name, files)
+    np.savez_compressed(os.path.join(HERE, "driver_fixtures.npz"), **out)
+
+
 def main():
+    drivers()
     rng = np.random.default_rng(20261202)
     cases = {
         "euler1d": ("euler", 1, euler_states(rng, 23)[None]),

@@ -11,6 +11,8 @@ seed on any machine.
 
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 _MASK = (1 << 64) - 1
@@ -359,3 +361,120 @@ def mhd_rgfm_dumbser_2d(n: int):
     j = np.arange(n, dtype=np.float64)[None, :] - (0.5 * n)
     phi = (np.sqrt((i * i) + (j * j)) / n) - 0.2
     return m1, m2, phi
+
+
+# ---- the reference's test drivers' initial conditions ---------------------------------
+
+def _disc_radius(n: int) -> np.ndarray:
+    """sqrt((i - n/2)^2 + (j - n/2)^2) over an n x n grid, as the drivers form it
+    (i, j converted to double, squares summed left to right)."""
+    i = np.arange(n, dtype=np.float64)[:, None] - 0.5 * n
+    j = np.arange(n, dtype=np.float64)[None, :] - 0.5 * n
+    return np.sqrt((i * i) + (j * j))
+
+
+def euler_toro1_2d(n: int) -> np.ndarray:
+    """EulerTests::solve2DToroTest1 (Euler/eulertests.cpp:29-52): (1, 0, 0, 0, 1)
+    where the radius is <= 0.2 n, (0.125, 0, 0, 0, 0.1) elsewhere."""
+    inside = _disc_radius(n) <= 0.2 * n
+    w = np.empty((n, n, 6))
+    w[inside] = (1.0, 0.0, 0.0, 0.0, 1.0, 1.0)
+    w[~inside] = (0.125, 0.0, 0.0, 0.0, 0.1, 1.0)
+    return w
+
+
+def _euler_fill(shape, state) -> np.ndarray:
+    w = np.empty(tuple(shape) + (6,))
+    w[...] = tuple(state) + (1.0,)
+    return w
+
+
+# Fedkiw's shocked-air state of EulerRGFMTests (rgfm: 1.3333, 0.3535 sqrt(10^5),
+# 0, 0, 1.5 10^5); the ambient air and the helium (eulerrgfmtests.cpp:70-83).
+FEDKIW_SHOCKED = (1.3333, 0.3535 * math.sqrt(10.0 ** 5.0), 0.0, 0.0, 1.5 * 10.0 ** 5.0)
+FEDKIW_AIR = (1.0, 0.0, 0.0, 0.0, 1.0 * 10.0 ** 5.0)
+FEDKIW_HELIUM = (0.1379, 0.0, 0.0, 0.0, 1.0 * 10.0 ** 5.0)
+
+
+def rgfm_toro1_1d(n: int):
+    """EulerRGFMTests::solveToroTest1HLLC (RGFM/eulerrgfmtests.cpp:28-48): material
+    1 (1, 0, 0, 0, 1) and material 2 (0.125, 0, 0, 0, 0.1) everywhere, level set
+    (1/n) i - 0.5.  Returns (m1, m2, phi)."""
+    h = 1.0 / n
+    phi = (h * np.arange(n, dtype=np.float64)) - 0.5
+    return _euler_fill((n,), (1.0, 0.0, 0.0, 0.0, 1.0)), _euler_fill((n,), (0.125, 0.0, 0.0, 0.0, 0.1)), phi
+
+
+def rgfm_fedkiw_1d(n: int):
+    """EulerRGFMTests::solveFedkiwTestHLLC (RGFM/eulerrgfmtests.cpp:84-111): material
+    1 shocked for i <= 0.05 n and ambient beyond, material 2 helium, level set
+    (1/n) i - 0.5."""
+    h = 1.0 / n
+    m1 = _euler_fill((n,), FEDKIW_AIR)
+    m1[np.arange(n) <= n * 0.05, :5] = FEDKIW_SHOCKED
+    phi = (h * np.arange(n, dtype=np.float64)) - 0.5
+    return m1, _euler_fill((n,), FEDKIW_HELIUM), phi
+
+
+def rgfm_toro1_2d(n: int):
+    """EulerRGFMTests::solve2DToroTest1HLLC (RGFM/eulerrgfmtests.cpp:136-155): both
+    materials uniform, level set radius / n - 0.2."""
+    phi = (_disc_radius(n) / n) - 0.2
+    return (_euler_fill((n, n), (1.0, 0.0, 0.0, 0.0, 1.0)), _euler_fill((n, n), (0.125, 0.0, 0.0, 0.0, 0.1)), phi)
+
+
+def rgfm_fedkiw_2d(n: int):
+    """EulerRGFMTests::solve2DFedkiwTestHLLC (RGFM/eulerrgfmtests.cpp:183-213):
+    material 1 shocked where the radius is <= 0.02 n, ambient elsewhere; material
+    2 helium; level set radius / n - 0.2."""
+    r = _disc_radius(n)
+    m1 = _euler_fill((n, n), FEDKIW_AIR)
+    m1[r <= 0.02 * n, :5] = FEDKIW_SHOCKED
+    return m1, _euler_fill((n, n), FEDKIW_HELIUM), (r / n) - 0.2
+
+
+# MHDTests::solveDumbserTest1-3 (MHD/mhdtests.cpp:7-71): split fraction, left and
+# right states, gamma, final time.
+MHD_DUMBSER = {
+    1: (0.5, (1.0, 0.0, 0.0, 0.0, 1.0, 0.75, 1.0, 0.0, 0.0), (0.125, 0.0, 0.0, 0.0, 0.1, 0.75, -1.0, 0.0, 0.0),
+        2.0, 0.1),
+    2: (0.4, (1.08, 1.2, 0.01, 0.5, 0.95, 0.564189, 1.015541, 0.564189, 0.0),
+        (0.9891, -0.0131, 0.0269, 0.010037, 0.97159, 0.564189, 1.135262, 0.564923, 0.0), 5.0 / 3.0, 0.2),
+    3: (0.5, (1.0, 0.0, 0.0, 0.0, 1.0, 1.3, 1.0, 0.0, 0.0), (0.4, 0.0, 0.0, 0.0, 0.4, 1.3, -1.0, 0.0, 0.0),
+        5.0 / 3.0, 0.16),
+}
+
+# MHDTests::solve2DDumbserTest1-2 (MHD/mhdtests.cpp:73-120): disc radius fraction,
+# inside and outside states, gamma, CFL, final time.
+MHD_DUMBSER_2D = {
+    1: (0.2, MHD_DUMBSER[1][1], MHD_DUMBSER[1][2], 2.0, 0.6, 0.05),
+    2: (0.16, MHD_DUMBSER[2][1], MHD_DUMBSER[2][2], 5.0 / 3.0, 0.8, 0.1),
+}
+
+
+def mhd_dumbser_1d(n: int, test: int = 1) -> np.ndarray:
+    """MHDTests::solveDumbserTest<test>: left state for i <= f n, right beyond."""
+    f, left, right, _, _ = MHD_DUMBSER[test]
+    w = np.empty((n, 9))
+    mask = np.arange(n) <= f * n
+    w[mask] = left
+    w[~mask] = right
+    return w
+
+
+def mhd_dumbser_2d(n: int, test: int = 1) -> np.ndarray:
+    """MHDTests::solve2DDumbserTest<test>: inside state where the radius is <= f n."""
+    f, inside_state, outside_state, _, _, _ = MHD_DUMBSER_2D[test]
+    w = np.empty((n, n, 9))
+    inside = _disc_radius(n) <= f * n
+    w[inside] = inside_state
+    w[~inside] = outside_state
+    return w
+
+
+def hpr_heat_conduction_1d(n: int) -> np.ndarray:
+    """HPRTests::solveHeatConductionProblem (HPR/hprtests.cpp:33-57): rho 2, p 1,
+    A = 2^(1/3) I for i <= n/2; rho 0.5, p 1, A = 0.5^(1/3) I beyond; at rest."""
+    a_hot, a_cold = 2.0 ** (1.0 / 3.0), 0.5 ** (1.0 / 3.0)
+    return np.stack([hpr_state(2.0, 1.0, A=a_hot * np.eye(3)) if i <= 0.5 * n else
+                     hpr_state(0.5, 1.0, A=a_cold * np.eye(3)) for i in range(n)])

@@ -836,10 +836,17 @@ class HPRMaterialParameters:
                                               (self.reference_pressure * (self.transverse_wave_speed ** 2)))
         return self
 
-    def configure_thermal_impulse_relaxation_time(self, heat_conduction_coefficient: float):
-        """tau2 = kappa rho0 / (T0 alpha^2) (hprmaterialparameters.cpp:112-119)."""
-        self.thermal_impulse_relaxation_time = heat_conduction_coefficient * (
-            self.reference_density / (self.initial_temperature * (self.heat_wave_speed ** 2)))
+    def configure_thermal_impulse_relaxation_time(self, heat_conduction_coefficient: float | None = None):
+        """tau2 = kappa rho0 / (T0 alpha^2) (hprmaterialparameters.cpp:112-119);
+        without an argument kappa = mu gamma (c_v / Pr) (:106-114).  T0 = 0 gives
+        an infinite tau2, as in the reference."""
+        if heat_conduction_coefficient is None:
+            heat_conduction_coefficient = (self.dynamic_viscosity_coefficient * self.adiabatic_index) * (
+                self.specific_heat_capacity / self.prandtl_number)
+        den = np.float64(self.initial_temperature) * (np.float64(self.heat_wave_speed) * self.heat_wave_speed)
+        with np.errstate(divide="ignore", invalid="ignore"):   # IEEE division, as in C++
+            ratio = np.float64(self.reference_density) / den
+        self.thermal_impulse_relaxation_time = float(heat_conduction_coefficient * ratio)
         return self
 
     def c_struct(self) -> _lib.HPRParams:
