@@ -92,7 +92,7 @@ int vg_ctx_set_graphs(vg_ctx *ctx, int enabled);
 /* Tile-shape autotuner (default on): the first run on a grid shape times the
  * candidate y-sweep shapes (threads per block x rows per strip) on that grid
  * and keeps the fastest for the context.  vg_ctx_tuned_shape returns the chosen
- * variant for a key "physics/order/mode/ny/nx" (forward Y sweeps, variants 0-3;
+ * variant for a key "physics/order/mode/ny/nx" (forward Y sweeps, variants 0-5;
  * physics "euler", or "euler_planar" for grids swept with 4 fields)
  * or "euler_adjoint_x/ny/nx" / "euler_adjoint_y/ny/nx" (adjoint sweeps, variants
  * 0-3; a 1D grid of n cells has ny = 1, nx = n), -1 if not tuned
@@ -456,6 +456,15 @@ int vg_rgfm_solve_fixed_dt_2d(vg_ctx *ctx, const double *m1_in, const double *m2
                               double *m1_out, double *m2_out, double *phi_out, int ny, int nx, double dx, double dy,
                               const double *dts, int n_steps, double bias, int limiter, const vg_euler_params *p1,
                               const vg_euler_params *p2);
+
+/* Self-check of the EXACT mode's division sequences; no counterpart in the
+ * reference.  For n operand pairs (host arrays a, b) computes on the device the
+ * shared-reciprocal quotient of the conversions (shared_q), the short division
+ * sequence of the slope limiters (in_range_q) and the compiler's IEEE division
+ * (ieee_q), so that a test can show them equal over the range the solvers use
+ * them in (vanguard_amd/csrc/vg_quot.h).  n <= 2^26. */
+int vg_division_selfcheck(vg_ctx *ctx, int64_t n, const double *a, const double *b, double *shared_q,
+                          double *in_range_q, double *ieee_q);
 
 /* ---- two-material ideal MHD: real ghost fluid method ------------------- */
 

@@ -159,6 +159,7 @@ SIGNATURES = {
     "vg_mhd_solve_batch": (C.c_int, [vptr, C.c_int, C.POINTER(C.c_int64), dptr, dptr, dptr, C.POINTER(MHDParams),
                                      dptr, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
                                      C.POINTER(C.c_int32), dptr, C.POINTER(RunStats)]),
+    "vg_division_selfcheck": (C.c_int, [vptr, C.c_int64, dptr, dptr, dptr, dptr, dptr]),
     "vg_mhd_rgfm_solve_1d": (C.c_int, [vptr, dptr, dptr, dptr, dptr, dptr, dptr, C.c_int, C.c_double, C.c_double,
                                        C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(MHDParams),
                                        C.POINTER(MHDParams), C.c_int, dptr, C.c_int, C.POINTER(RunStats)]),

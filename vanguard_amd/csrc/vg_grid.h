@@ -102,16 +102,18 @@ __global__ __launch_bounds__(256) void constant_planes_kernel(const double *__re
 
 // ---- y-sweep shapes and their autotuner --------------------------------------
 
-// Candidate (threads per block, rows per strip) of the y sweep: variant 0 is
-// Tiles<P>'s shape; 1-3 trade strip length for more workgroups on small grids.
-constexpr int kYVariants = 4;
+// Candidate (threads per block, rows per strip, columns per thread) of the y
+// sweep: variant 0 is Tiles<P>'s shape; 1-3 trade strip length for more
+// workgroups on small grids; 4-5 give each thread two columns (two independent
+// dependency chains per wave).
+constexpr int kYVariants = 6;
 
-template <class P, int ORDER, int MODE, int BS, int H>
+template <class P, int ORDER, int MODE, int BS, int H, int CPT = 1>
 int launch_y_shape(vg_ctx *ctx, const SweepArgs<P> &a)
 {
-    dim3 grid((a.nx + BS - 1) / BS, (a.ny + H - 1) / H);
+    dim3 grid((a.nx + BS * CPT - 1) / (BS * CPT), (a.ny + H - 1) / H);
     return ctx->timed(ORDER == 2 ? "sweep_y_slic" : "sweep_y_force",
-                      [&] { sweep_y_kernel<P, ORDER, MODE, BS, H><<<grid, BS, 0, ctx->stream>>>(a); });
+                      [&] { sweep_y_kernel<P, ORDER, MODE, BS, H, CPT><<<grid, BS, 0, ctx->stream>>>(a); });
 }
 
 template <class P, int ORDER, int MODE>
@@ -122,6 +124,8 @@ int launch_y(vg_ctx *ctx, int variant, const SweepArgs<P> &a)
         case 1: return launch_y_shape<P, ORDER, MODE, 256, 16>(ctx, a);
         case 2: return launch_y_shape<P, ORDER, MODE, 128, 32>(ctx, a);
         case 3: return launch_y_shape<P, ORDER, MODE, 64, 64>(ctx, a);
+        case 4: return launch_y_shape<P, ORDER, MODE, 64, 32, 2>(ctx, a);
+        case 5: return launch_y_shape<P, ORDER, MODE, 64, 64, 2>(ctx, a);
         default: break;
         }
     }

@@ -1,41 +1,45 @@
-// Correctly rounded quotients a / b that share one reciprocal of b.
+// Correctly rounded quotients a / b for the EXACT conversions, cheaper than the
+// compiler's division where the operands are in range.
 //
-// Several quotients of a state divide by the same density (u = rho u / rho,
-// v = rho v / rho, e_t = E / rho, ...).  A correctly rounded IEEE division costs
-// a scale, a reciprocal estimate, Newton steps and a fix-up on the AMDGPU
-// (about 11 dependent instructions, two of them tied to VCC); here the
-// reciprocal y = RN(1 / b) is taken once with that division, and each quotient
-// is then refined from q0 = RN(a y) by two residual corrections with fused
-// multiply-adds:
+// The compiler's correctly rounded FP64 division on gfx950 is the sequence
 //
-//     r = a - b q          (exact: the FMA's product is not rounded, and the
-//                           residual of a quotient within 1.5 ulp fits a double)
-//     q <- RN(q + r y)
+//     s = div_scale(b), t = div_scale(a)       (scale out-of-range operands)
+//     y = rcp(s); two Newton steps y += y (1 - s y)
+//     q = t y;  r = t - s q                     (exact residual, one FMA)
+//     div_fixup(div_fmas(r, y, q), b, a)        (q + r y rounded once, then the
+//                                                special cases)
 //
-// q0 is within 1.5 ulp of a / b (y is within half an ulp of 1 / b).  After the
-// first correction q is within 1 ulp: its exact value is a/b + (a/b - q0)(b y - 1)
-// and |b y - 1| <= 2^-53.  The second correction then returns RN(a / b): this is
-// Markstein's theorem (P. W. Markstein, "Computation of elementary functions on
-// the IBM RISC System/6000 processor", IBM J. Res. Dev. 34(1), 1990; also J.-M.
-// Muller et al., Handbook of Floating-Point Arithmetic, the section on
-// Newton-Raphson-based division): if y is within half an ulp of 1/b and q within
-// one ulp of a/b, RN(q + (a - bq) y) = RN(a/b).
+// For operands in range -- a zero or 2^-600 <= |a| <= 2^600, 2^-100 <= |b| <=
+// 2^100, or more generally: b and 1/b normal, a / b normal, exponent(a) -
+// exponent(b) < 768 and a not within 2^53 of the smallest normal -- the two
+// scaling steps return their operands unchanged (and clear VCC), div_fmas is a
+// plain FMA and div_fixup returns its first operand, so the division is
 //
-// The residual is formed as b q - a and its product with y subtracted (the
-// negation on the operand y, never on a rounded result), so a zero numerator
-// keeps its sign (-0 / b = -0), which the RN(q + r y) form would lose.  The theorem
-// needs every intermediate normal: the shared reciprocal is used only for
-// 2^-100 <= b <= 2^100 (densities; NaN, negative and other divisors take the
-// IEEE division), and the result equals IEEE a / b for every numerator a that
-// is zero or of magnitude in [2^-800, 2^800] -- the momenta and energies of any
-// state the solvers can reach.  A numerator beyond 2^800 may give NaN where
-// IEEE gives an infinity (a state already lost), and one below 2^-800 (under
-// 1e-240, not a physical momentum) may differ in the last bit.
-// tests/test_quotient.py compares this code with IEEE division over random,
-// near-midpoint and edge operands.
+//     y = rcp(b); two Newton steps; q = a y; r = b q - a; q - r y
+//
+// and the reciprocal part depends on b only: several quotients by one density
+// (u = rho u / rho, v, E / rho, ...) share it and cost one multiply and two
+// FMAs each.  The residual is formed as b q - a and its product with y
+// subtracted, so a zero numerator keeps its sign (-0 / b = -0) as div_fixup
+// would give it.  tests/test_quotient_gpu.py compares these sequences on the
+// GPU with the compiler's division over random operands across the range,
+// quotients next to rounding midpoints and signed zeros.
+//
+// Outside that range the shared form falls back to the IEEE division (a / b)
+// for divisors beyond 2^-100 <= b <= 2^100 (NaN, negative and other
+// densities); numerators beyond 2^600 or below 2^-600 (and not zero) -- no
+// momentum or energy of a state the solvers can reach -- may differ from the
+// IEEE quotient.
+//
+// The host build (used by tests/quotient_check.cpp) keeps y = RN(1 / b) and two
+// residual corrections, q = RN(a y), q += (a - b q) y twice, which Markstein's
+// theorem (P. W. Markstein, IBM J. Res. Dev. 34(1), 1990; J.-M. Muller et al.,
+// Handbook of Floating-Point Arithmetic) makes equal to RN(a / b) in the same
+// range.
 #pragma once
 
 #ifdef __HIPCC__
+#include <hip/hip_runtime.h>
 #define VG_QUOT_FN __host__ __device__ __forceinline__
 #else
 #include <cmath>
@@ -46,23 +50,60 @@ namespace vg {
 
 struct SharedDivisor {
     double b;    // the divisor
-    double y;    // RN(1 / b)
-    bool fast;   // 2^-100 <= b <= 2^100: the FMA refinement applies
+    double y;    // device: the division sequence's reciprocal estimate; host: RN(1 / b)
+    bool fast;   // 2^-100 <= b <= 2^100: the shared form applies
 };
+
+#ifdef __HIPCC__
+// The reciprocal estimate of the division sequence: rcp and two Newton steps.
+__device__ __forceinline__ double div_reciprocal(double b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double y = __builtin_amdgcn_rcp(b);
+#else
+    double y = 1.0 / b;   // host pass of the HIP build: declared, never called
+#endif
+    double e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-b, y, 1.0);
+    return __builtin_fma(y, e, y);
+}
+
+// The rest of the division sequence: RN(a / b) from the estimate y of 1 / b.
+__device__ __forceinline__ double div_finish(double a, double b, double y)
+{
+    const double q = a * y;
+    const double r = __builtin_fma(b, q, -a);
+    return __builtin_fma(-r, y, q);
+}
+
+// RN(a / b) for operands in range (see the header comment); the caller
+// guarantees the range.
+__device__ __forceinline__ double div_in_range(double a, double b)
+{
+    return div_finish(a, b, div_reciprocal(b));
+}
+#endif
 
 VG_QUOT_FN SharedDivisor shared_divisor(double b)
 {
     SharedDivisor d;
     d.b = b;
-    d.y = 1.0 / b;
     d.fast = (b >= 0x1p-100) && (b <= 0x1p+100);
+#if defined(__HIP_DEVICE_COMPILE__)
+    d.y = div_reciprocal(b);   // unused when !fast
+#else
+    d.y = 1.0 / b;
+#endif
     return d;
 }
 
-// RN(a / d.b) by the two corrections; valid when d.fast.
+// RN(a / d.b); valid when d.fast and a is zero or 2^-600 <= |a| <= 2^600.
 VG_QUOT_FN double quot_refined(double a, const SharedDivisor &d)
 {
-#ifdef __HIPCC__
+#if defined(__HIP_DEVICE_COMPILE__)
+    return div_finish(a, d.b, d.y);
+#elif defined(__HIPCC__)
     double q = a * d.y;
     double r = __builtin_fma(d.b, q, -a);
     q = __builtin_fma(r, -d.y, q);

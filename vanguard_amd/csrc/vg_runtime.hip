@@ -1,4 +1,6 @@
 // Context, errors and kernel timing.
+#include <string>
+
 #include "vg_runtime.h"
 
 #include <cstring>
@@ -225,7 +227,10 @@ int vg_ctx_release_workspace(vg_ctx *ctx)
 int vg_ctx_set_tuned_shape(vg_ctx *ctx, const char *key, int variant)
 {
     VG_CHECK(ctx != nullptr && key != nullptr, "vg_ctx_set_tuned_shape: null argument");
-    VG_CHECK(variant >= -1 && variant < 4, "vg_ctx_set_tuned_shape: variant must be -1..3");
+    const bool adjoint = std::string(key).rfind("euler_adjoint", 0) == 0;
+    VG_CHECK(variant >= -1 && variant < (adjoint ? 4 : 6),
+             adjoint ? "vg_ctx_set_tuned_shape: adjoint variant must be -1..3"
+                     : "vg_ctx_set_tuned_shape: y-sweep variant must be -1..5");
     if (variant < 0) ctx->tuned.erase(key);
     else ctx->tuned[key] = variant;
     return VG_OK;

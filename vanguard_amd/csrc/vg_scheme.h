@@ -24,24 +24,46 @@ struct HasSharedDivisor<P, decltype((void)&P::template flux_d<0>)> {
     static constexpr bool value = true;
 };
 
+// Correctly rounded division: the compiler's IEEE division, or the division
+// sequence without its scaling steps where the caller has shown the operands
+// in range (vg_quot.h).
+struct IeeeDiv {
+    __device__ __forceinline__ double operator()(double a, double b) const { return a / b; }
+};
+struct InRangeDiv {
+    __device__ __forceinline__ double operator()(double a, double b) const { return div_in_range(a, b); }
+};
+
 // SlopeLimiters::computeRCoefficient and computeSlopeLimiter
 // (Solvers/slopelimiters.cpp:7-74): 0 superbee, 1 van Leer, anything else minbee.
-__device__ __forceinline__ double r_coefficient(double r, double bias)
+template <class Div = IeeeDiv>
+__device__ __forceinline__ double r_coefficient(double r, double bias, Div div = Div())
 {
-    return 2.0 / ((1.0 - bias) + ((1.0 + bias) * r));
+    return div(2.0, (1.0 - bias) + ((1.0 + bias) * r));
 }
 
-__device__ __forceinline__ double limiter_value(double r, double bias, int limiter)
+template <class Div = IeeeDiv>
+__device__ __forceinline__ double limiter_value(double r, double bias, int limiter, Div div = Div())
 {
     if (r < 0.0) return 0.0;
     if (limiter == 0) {
         if (r < 0.5) return 2.0 * r;
         if (r < 1.0) return 1.0;
-        return smin(smin(r, r_coefficient(r, bias)), 2.0);
+        return smin(smin(r, r_coefficient(r, bias, div)), 2.0);
     }
-    if (limiter == 1) return smin((2.0 * r) / (1.0 + r), r_coefficient(r, bias));
+    if (limiter == 1) return smin(div(2.0 * r, 1.0 + r), r_coefficient(r, bias, div));
     if (r < 1.0) return r;
-    return smin(1.0, r_coefficient(r, bias));
+    return smin(1.0, r_coefficient(r, bias, div));
+}
+
+// The limiter's divisions are in the range of the short division sequence when
+// every slope difference is at most 2^600 in magnitude (they are at least 1e-5
+// after the clamp) and -1 <= bias <= 1: then 2^-617 <= |r| <= 2^617, and the
+// denominators 1 + r and (1 - bias) + (1 + bias) r of the positive-r branches
+// lie in [2^-617, 2^619] with numerators 2 and 2r.
+__device__ __forceinline__ bool limiter_in_range(double num, double den)
+{
+    return smax(fabs(num), fabs(den)) <= 0x1p600;
 }
 
 // FAST-mode limiter: the same piecewise functions with Newton-refined reciprocals.
@@ -116,17 +138,29 @@ struct Scheme {
         double xl[NF], xr[NF];
         const double cl = 0.5 * (1.0 + bias), cr = 0.5 * (1.0 - bias);
         if constexpr (MODE == MODE_EXACT) {
+            double num[NF], den[NF], d[NF];
+            bool in_range = (bias >= -1.0) && (bias <= 1.0);
 #pragma unroll
             for (int k = 0; k < NF; k++) {
                 const double dl = UM[k] - UL[k];
                 const double dr = UR[k] - UM[k];
-                double d = (cl * dl) + (cr * dr);
+                d[k] = (cl * dl) + (cr * dr);
                 // |difference| < 1e-5 is replaced by +1e-5 (sign dropped) before the ratio.
-                const double num = (fabs(dl) < 1e-5) ? 1e-5 : dl;
-                const double den = (fabs(dr) < 1e-5) ? 1e-5 : dr;
-                d *= limiter_value(num / den, bias, limiter);
-                xl[k] = UM[k] - (0.5 * d);
-                xr[k] = UM[k] + (0.5 * d);
+                num[k] = (fabs(dl) < 1e-5) ? 1e-5 : dl;
+                den[k] = (fabs(dr) < 1e-5) ? 1e-5 : dr;
+                in_range = in_range && limiter_in_range(num[k], den[k]);
+            }
+            if (in_range) {
+#pragma unroll
+                for (int k = 0; k < NF; k++) d[k] *= limiter_value(div_in_range(num[k], den[k]), bias, limiter, InRangeDiv());
+            } else {
+#pragma unroll
+                for (int k = 0; k < NF; k++) d[k] *= limiter_value(num[k] / den[k], bias, limiter);
+            }
+#pragma unroll
+            for (int k = 0; k < NF; k++) {
+                xl[k] = UM[k] - (0.5 * d[k]);
+                xr[k] = UM[k] + (0.5 * d[k]);
             }
             double Fl[NF], Fr[NF];
             flux_of_cons(m, xl, Fl);

@@ -207,7 +207,12 @@ __global__ __launch_bounds__(BS) void sweep_x_kernel(SweepArgs<P> a)
     if (a.fuse_max) block_max_to<BS>(a.partials, key);
 }
 
-template <class P, int ORDER, int MODE, int BS, int H>
+// CPT columns per thread (j, j + BS, ...): their computations are independent,
+// so the instruction stream of a wave interleaves CPT dependency chains (the
+// divide sequences of EXACT mode are long chains); the arithmetic of every
+// cell is the same.  Columns past the grid's edge compute on the last column
+// and store nothing.
+template <class P, int ORDER, int MODE, int BS, int H, int CPT = 1>
 __global__ __launch_bounds__(BS) void sweep_y_kernel(SweepArgs<P> a)
 {
     constexpr int NF = P::NF;
@@ -219,74 +224,98 @@ __global__ __launch_bounds__(BS) void sweep_y_kernel(SweepArgs<P> a)
     typename P::Params m = a.m;
     if (ORDER == 2 && !a.fixed_params) P::begin_step(m, a.clk->S);   // first-order solvers never set c_h
 
-    const int j = blockIdx.x * BS + threadIdx.x;
+    const int j0 = blockIdx.x * BS * CPT + threadIdx.x;
     const int ny = a.ny, nx = a.nx;
     const int i0 = blockIdx.y * H;
     const int i1 = (i0 + H < ny) ? i0 + H : ny;
     unsigned long long key = 0;
 
-    if (j < nx) {
-        auto row_of = [&](int r) -> long long { return (long long)ghost_index(r, ny, a.bc) * nx + j; };
-        if (ORDER == 2) {
-            double A[NF], B[NF], C[NF], w[NF];
-            load_state<P>(a.in, a.plane, row_of(i0 - 2), w);
-            S::cons(m, w, A);
-            load_state<P>(a.in, a.plane, row_of(i0 - 1), w);
-            S::cons(m, w, B);
-            load_state<P>(a.in, a.plane, row_of(i0), w);   // row e + 1 of the first iteration
-            ConsFlux<NF> er_prev;
-            double Fprev[NF];
-            for (int e = i0 - 1; e <= i1; e++) {
-                S::cons(m, w, C);
-                if (e + 2 <= i1 + 1) load_state<P>(a.in, a.plane, row_of(e + 2), w);   // prefetch
-                ConsFlux<NF> L, R;
-                S::evolve(m, A, B, C, a.bias, a.limiter, sc.half_c, L, R);
-                double Fnew[NF];
-                if (e >= i0) S::force_face(m, er_prev, L, sc.lf_c, sc.ri_c, Fnew);   // face e - 1/2
-                if (e >= i0 + 1) {
-                    double wn[NF];
-                    S::update(m, A, Fprev, Fnew, sc.up_c, wn);                       // row e - 1
-                    store_state<P>(a.out, a.plane, (long long)(e - 1) * nx + j, wn);
-                    if (a.fuse_max) {
-                        const unsigned long long k2 = max_key(P::wave_speed(m, wn, a.dim));
-                        key = (k2 > key) ? k2 : key;
-                    }
-                }
+    if (j0 < nx) {
+        int jc[CPT];
+        bool st[CPT];
 #pragma unroll
-                for (int f = 0; f < NF; f++) {
-                    Fprev[f] = Fnew[f];
-                    er_prev.U[f] = R.U[f];
-                    er_prev.F[f] = R.F[f];
-                    A[f] = B[f];
-                    B[f] = C[f];
+        for (int c = 0; c < CPT; c++) {
+            const int j = j0 + c * BS;
+            st[c] = j < nx;
+            jc[c] = st[c] ? j : nx - 1;
+        }
+        auto row_of = [&](int r, int c) -> long long { return (long long)ghost_index(r, ny, a.bc) * nx + jc[c]; };
+        if (ORDER == 2) {
+            double A[CPT][NF], B[CPT][NF], C[CPT][NF], w[CPT][NF];
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                load_state<P>(a.in, a.plane, row_of(i0 - 2, c), w[c]);
+                S::cons(m, w[c], A[c]);
+                load_state<P>(a.in, a.plane, row_of(i0 - 1, c), w[c]);
+                S::cons(m, w[c], B[c]);
+                load_state<P>(a.in, a.plane, row_of(i0, c), w[c]);   // row e + 1 of the first iteration
+            }
+            ConsFlux<NF> er_prev[CPT];
+            double Fprev[CPT][NF];
+            for (int e = i0 - 1; e <= i1; e++) {
+#pragma unroll
+                for (int c = 0; c < CPT; c++) {
+                    S::cons(m, w[c], C[c]);
+                    if (e + 2 <= i1 + 1) load_state<P>(a.in, a.plane, row_of(e + 2, c), w[c]);   // prefetch
+                    ConsFlux<NF> L, R;
+                    S::evolve(m, A[c], B[c], C[c], a.bias, a.limiter, sc.half_c, L, R);
+                    double Fnew[NF];
+                    if (e >= i0) S::force_face(m, er_prev[c], L, sc.lf_c, sc.ri_c, Fnew);   // face e - 1/2
+                    if (e >= i0 + 1) {
+                        double wn[NF];
+                        S::update(m, A[c], Fprev[c], Fnew, sc.up_c, wn);                    // row e - 1
+                        if (st[c]) {
+                            store_state<P>(a.out, a.plane, (long long)(e - 1) * nx + jc[c], wn);
+                            if (a.fuse_max) {
+                                const unsigned long long k2 = max_key(P::wave_speed(m, wn, a.dim));
+                                key = (k2 > key) ? k2 : key;
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int f = 0; f < NF; f++) {
+                        Fprev[c][f] = Fnew[f];
+                        er_prev[c].U[f] = R.U[f];
+                        er_prev[c].F[f] = R.F[f];
+                        A[c][f] = B[c][f];
+                        B[c][f] = C[c][f];
+                    }
                 }
             }
         } else {
-            double w[NF];
-            ConsFlux<NF> prev;
-            load_state<P>(a.in, a.plane, row_of(i0 - 1), w);
-            S::cons_flux_of_prim(m, w, prev);
-            double Fprev[NF];
-            for (int e = i0; e <= i1; e++) {
-                ConsFlux<NF> cur;
-                load_state<P>(a.in, a.plane, row_of(e), w);
-                S::cons_flux_of_prim(m, w, cur);
-                double Fnew[NF];
-                S::force_face(m, prev, cur, sc.lf_c, sc.ri_c, Fnew);                 // face e - 1/2
-                if (e >= i0 + 1) {
-                    double wn[NF];
-                    S::update(m, prev.U, Fprev, Fnew, sc.up_c, wn);                  // row e - 1
-                    store_state<P>(a.out, a.plane, (long long)(e - 1) * nx + j, wn);
-                    if (a.fuse_max) {
-                        const unsigned long long k2 = max_key(P::wave_speed(m, wn, a.dim));
-                        key = (k2 > key) ? k2 : key;
-                    }
-                }
+            double w[CPT][NF];
+            ConsFlux<NF> prev[CPT];
+            double Fprev[CPT][NF];
 #pragma unroll
-                for (int f = 0; f < NF; f++) {
-                    Fprev[f] = Fnew[f];
-                    prev.U[f] = cur.U[f];
-                    prev.F[f] = cur.F[f];
+            for (int c = 0; c < CPT; c++) {
+                load_state<P>(a.in, a.plane, row_of(i0 - 1, c), w[c]);
+                S::cons_flux_of_prim(m, w[c], prev[c]);
+            }
+            for (int e = i0; e <= i1; e++) {
+#pragma unroll
+                for (int c = 0; c < CPT; c++) {
+                    ConsFlux<NF> cur;
+                    load_state<P>(a.in, a.plane, row_of(e, c), w[c]);
+                    S::cons_flux_of_prim(m, w[c], cur);
+                    double Fnew[NF];
+                    S::force_face(m, prev[c], cur, sc.lf_c, sc.ri_c, Fnew);                 // face e - 1/2
+                    if (e >= i0 + 1) {
+                        double wn[NF];
+                        S::update(m, prev[c].U, Fprev[c], Fnew, sc.up_c, wn);               // row e - 1
+                        if (st[c]) {
+                            store_state<P>(a.out, a.plane, (long long)(e - 1) * nx + jc[c], wn);
+                            if (a.fuse_max) {
+                                const unsigned long long k2 = max_key(P::wave_speed(m, wn, a.dim));
+                                key = (k2 > key) ? k2 : key;
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int f = 0; f < NF; f++) {
+                        Fprev[c][f] = Fnew[f];
+                        prev[c].U[f] = cur.U[f];
+                        prev[c].F[f] = cur.F[f];
+                    }
                 }
             }
         }
