@@ -217,10 +217,14 @@ int vg_euler_solve_batch(vg_ctx *ctx, int n_inst, const int64_t *offsets, const 
  * solve2D, EXACT arithmetic) with the time steps held at the values the forward
  * run chose.  No reference counterpart: the reference has no derivatives.
  * Runs the forward (prim_out receives its final state, dt_log its time steps),
- * keeping a checkpoint every `checkpoint_every` steps (0: every step when that
- * fits in free device memory and max_steps > 0, else ceil(sqrt(max_steps)), 16
- * for runs to t_final), then the adjoint backwards with recomputation per
- * segment.  seed: the
+ * then the adjoint backwards.  With max_steps > 0 the forward can write its
+ * sweeps into recording buffers of the context's workspace: checkpoint_every 0
+ * keeps every sweep input when they fit in free device memory (the reverse pass
+ * recomputes nothing), else every step's start (the reverse pass recomputes the
+ * step's two inner sweeps), as does checkpoint_every 1.  Otherwise a
+ * checkpoint every `checkpoint_every` steps (0: ceil(sqrt(max_steps)), 16 for
+ * runs to t_final) and recomputation per segment.  Every choice gives the same
+ * bits.  seed: the
  * derivative of a scalar loss with respect to the final primitive state
  * (array of structures, 6 doubles per cell); grad receives its derivative with
  * respect to the initial primitive state.  The slope limiter is differentiated
@@ -237,9 +241,7 @@ int vg_euler_adjoint_2d(vg_ctx *ctx, const double *prim_in, double *prim_out, co
 /* The same with prim_in, prim_out, seed and grad in device memory (same
  * array-of-structures layout); dt_log and stats stay on the host.  Checkpoints
  * and sweep inputs live in the context's workspace, kept between calls
- * (vg_ctx_release_workspace frees it); checkpoint_every 0 picks 1 when a grid
- * per step fits in free device memory and the step count is capped, else
- * ceil(sqrt(max_steps)) (16 for runs to t_final). */
+ * (vg_ctx_release_workspace frees it); checkpoint_every chooses as above. */
 int vg_euler_adjoint_1d_device(vg_ctx *ctx, const double *prim_in, double *prim_out, const double *seed, double *grad,
                                int n, double dx, double cfl, double t_final, int max_steps, double bias, int limiter,
                                int bc, const vg_euler_params *mp, int checkpoint_every, double *dt_log, int dt_log_cap,

@@ -197,6 +197,26 @@ def test_checkpoint_interval_changes_nothing(dim):
         O.assert_bitwise(other.cells, base.cells, f"final state, checkpoint every {k}")
 
 
+@pytest.mark.parametrize("dim", [1, 2])
+def test_recorded_forward_stops_at_the_final_time(dim):
+    """With a step cap the forward writes its sweeps into recording buffers:
+    every sweep input (checkpoint_every 0) or every step's start (1).  A final
+    time reached before the cap ends that forward early, and both results are
+    the checkpointed adjoint's bit for bit."""
+    w = sod_1d(120) if dim == 1 else sod_2d(20, 28)
+    seed = np.random.default_rng(5).standard_normal(w.shape)
+    dx = 1.0 / w.shape[-2]
+    t_final = 0.02
+    kw = dict(params=mat(IDEAL), max_steps=500)
+    ref = S.euler_adjoint(w, seed, dim, dx, 0.6, t_final, checkpoint_every=4, **kw)
+    for k in (0, 1):
+        r = S.euler_adjoint(w, seed, dim, dx, 0.6, t_final, checkpoint_every=k, **kw)
+        assert 0 < r.stats.steps < 500 and abs(float(np.sum(r.dt)) - t_final) < 1e-15
+        O.assert_bitwise(r.dt, ref.dt, f"time steps, layout {k}")
+        O.assert_bitwise(r.cells, ref.cells, f"final state, layout {k}")
+        O.assert_bitwise(r.grad, ref.grad, f"gradient, layout {k}")
+
+
 def test_adjoint_tile_variants_change_nothing():
     """Every tile variant of the adjoint sweeps (shape and occupancy target)
     gives the same gradient bit for bit (each output is computed with the same

@@ -196,6 +196,8 @@ __device__ __forceinline__ void lin_flux(const EPar &m, const double (&U)[6], do
 
 struct AdjointArgs {
     const double *w;        // forward input of the sweep (SoA planes)
+    const double *w35;      // where planes 3 and 5 (w, lambda) are read: w, or a grid holding the
+                            // constant planes the 4-field forward sweeps leave out of their outputs
     const double *wbar_out; // adjoint of the sweep's output (SoA planes)
     double *wbar_in;        // adjoint of the sweep's input (SoA planes), interior cells
     double *ghost;          // per line: 4 positions (-2, -1, n, n+1) x 6 fields
@@ -294,7 +296,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         const int k = (pos >= -2 && pos < n + 2) ? ghost_index(pos, n, a.bc) : (pos < 0 ? 0 : n - 1);
         const long long c = adj_cell<DIR>(line, k, a.nx);
 #pragma unroll
-        for (int f = 0; f < NF; f++) w[f] = a.w[f * a.plane + c];
+        for (int f = 0; f < NF; f++) w[f] = ((f == 3 || f == 5) ? a.w35 : a.w)[f * a.plane + c];
         linearisation_state(w);
         double U[NF];
         EulerPhys::prim_to_cons(m, w, U);

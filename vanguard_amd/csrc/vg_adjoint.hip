@@ -1,9 +1,11 @@
 // Adjoint C ABI for the Euler SLIC solve, and the fixed-time-step forward it
 // differentiates.
 //
-// vg_euler_adjoint_{1d,2d}[_device]: forward solve (the same kernels and
-// arithmetic as vg_euler_solve_*, EXACT mode) that records every dt and keeps a
-// checkpoint of the grid every K steps (K = 1 when a grid per step fits in free
+// vg_euler_adjoint_{1d,2d}[_device]: when every sweep input of a capped run
+// fits in device memory, the forward writes each sweep into its own buffer and
+// the adjoint sweeps read them (adjoint_recorded).  Otherwise: forward solve
+// (the same kernels and arithmetic as vg_euler_solve_*, EXACT mode) that
+// records every dt and keeps a checkpoint of the grid every K steps (K = 1 when a grid per step fits in free
 // device memory, else ceil(sqrt(steps)), unless given); then, segment by
 // segment from the end, the segment is recomputed from its checkpoint with the
 // recorded dts, storing the input of every sweep (the last sweep of a segment
@@ -124,6 +126,202 @@ int choose_interval(vg_ctx *ctx, int checkpoint_every, int max_steps, int per, s
     return std::max(1, (int)std::ceil(std::sqrt((double)max_steps)));
 }
 
+// Grids of the recorded adjoint's layouts for a run of max_steps steps: every
+// sweep input (per per step), or the step starts plus two recomputation buffers;
+// 0 when neither fits in free device memory beside the adjoint's own buffers.
+int record_layout(vg_ctx *ctx, int max_steps, int per, size_t gbytes)
+{
+    if (max_steps <= 0) return 0;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 0;
+    const double avail = 0.85 * (double)(free_b + ctx->workspace_bytes());
+    auto fits = [&](double grids) { return (double)gbytes * (grids + 4.0) + 64.0 * 1024 * 1024 < avail; };
+    if (fits((double)per * max_steps)) return per;
+    if (per > 1 && fits((double)max_steps + 2.0)) return 1;
+    return 0;
+}
+
+// The adjoint with the forward's states recorded by the forward itself: the
+// CFL forward runs its steps (the solve's clock, kernels and fused wave-speed
+// maximum) with its sweeps writing into recording buffers instead of the grid's
+// two, so nothing is copied.  Layout kept = per (2D: 3): every sweep input has
+// its own buffer, slots[3 s + k] the input of sweep k of step s, and the reverse
+// pass reads them.  kept = 1 in 2D: only the step starts slots[s] are kept, the
+// two inner sweeps of a step write two scratch buffers and the reverse pass
+// recomputes them for each step with its recorded dt.  slots[0] is the loaded
+// grid.  When the grid takes the 4-field sweeps, their outputs leave planes 3
+// and 5 unwritten and the adjoint sweeps read those planes (w = +0, lambda = 1)
+// from slot 0.
+int adjoint_recorded(vg_ctx *ctx, EGrid &g, int kept, int dim, int ny, int nx, double dx, double cfl, double t_final,
+                     int max_steps, double bias, int limiter, int bc, const EulerPhys::Params &m, const double *seed,
+                     double *prim_out, double *grad, int on_device, double *dt_log, int dt_log_cap,
+                     vg_run_stats *stats, std::chrono::steady_clock::time_point w0)
+{
+    hipStream_t s = ctx->stream;
+    const size_t gbytes = sizeof(double) * 6 * g.plane;
+    const int per = (dim == 2) ? 3 : 1;
+    auto wsp = [&](size_t i, size_t bytes, double **p) {
+        void *v = nullptr;
+        int r = ctx->workspace_buffer(i, bytes, &v);
+        *p = static_cast<double *>(v);
+        return r;
+    };
+    // workspace: 0 dts, 1-2 adjoint ping-pong, 3 ghost adjoints, 4-5 scratch
+    // (kept = 1), 6 .. the recording buffers after slot 0
+    std::vector<double *> slots((size_t)kept * max_steps + 1, nullptr);
+    slots[0] = g.buf[g.cur];
+    double *dts_dev = nullptr, *wb[2] = {nullptr, nullptr}, *ghost = nullptr, *T[2] = {nullptr, nullptr};
+    int rc = wsp(0, sizeof(double) * max_steps, &dts_dev);
+    if (!rc) rc = wsp(1, gbytes, &wb[0]);
+    if (!rc) rc = wsp(2, gbytes, &wb[1]);
+    if (!rc) rc = wsp(3, sizeof(double) * 4 * 6 * (size_t)std::max(nx, ny), &ghost);
+    if (!rc && kept != per) rc = wsp(4, gbytes, &T[0]);
+    if (!rc && kept != per) rc = wsp(5, gbytes, &T[1]);
+    for (size_t k = 1; k < slots.size() && !rc; k++) rc = wsp(6 + k - 1, gbytes, &slots[k]);
+    if (rc) return rc;
+    // ---- forward: the steps of GridState::run, into the recording buffers -------------
+    if ((rc = g.choose_physics())) return rc;
+    const bool reduced = g.reduced;
+    if ((rc = g.refresh_max(bc, m, g.fused_count(2)))) return rc;
+    if (dim == 2) {
+        if (!g.tune_clk) VG_HIP(hipMalloc(&g.tune_clk, sizeof(vg::Clock)));
+        vg::Clock tc{};
+        tc.dt = 1e-3 * dx;
+        VG_HIP(hipMemcpyAsync(g.tune_clk, &tc, sizeof(vg::Clock), hipMemcpyHostToDevice, s));
+        rc = reduced ? g.tune_as<vg::EulerPlanarPhys, 2, vg::MODE_EXACT>(bc, dx, bias, limiter, m)
+                     : g.tune_as<EulerPhys, 2, vg::MODE_EXACT>(bc, dx, bias, limiter, m);
+        if (rc) return rc;
+    }
+    vg::Clock h{};
+    h.t_final = t_final;
+    h.cfl = cfl;
+    h.dx = dx;
+    h.max_steps = max_steps;
+    VG_HIP(hipMemcpyAsync(g.clk, &h, sizeof(vg::Clock), hipMemcpyHostToDevice, s));
+    // the four buffers a step reads and writes: its start, the two inner sweep inputs, its end
+    auto step_bufs = [&](int st, double *(&b)[4]) {
+        if (kept == per) {
+            for (int k = 0; k <= per; k++) b[k] = slots[(size_t)per * st + k];
+        } else {
+            b[0] = slots[st];
+            b[1] = T[0];
+            b[2] = T[1];
+            b[3] = slots[st + 1];
+        }
+    };
+    for (int st = 0; st < max_steps && !rc; st++) {
+        vg::clock_begin_kernel<<<1, vg::kReduceThreads, 0, s>>>(g.clk, g.partials, g.n_partials, dts_dev, max_steps);
+        double *b[4];
+        step_bufs(st, b);
+        if (dim == 1) {
+            rc = g.launch_sweep<2, vg::MODE_EXACT>(0, b[0], b[1], 1.0, 1, bc, dx, bias, limiter, m);
+        } else {
+            rc = g.launch_sweep<2, vg::MODE_EXACT>(0, b[0], b[1], 0.5, 0, bc, dx, bias, limiter, m);
+            if (!rc) rc = g.launch_sweep<2, vg::MODE_EXACT>(1, b[1], b[2], 1.0, 0, bc, dx, bias, limiter, m);
+            if (!rc) rc = g.launch_sweep<2, vg::MODE_EXACT>(0, b[2], b[3], 0.5, 1, bc, dx, bias, limiter, m);
+        }
+    }
+    if (rc) return vg::launch_status("adjoint forward");
+    vg::clock_finish_kernel<<<1, 1, 0, s>>>(g.clk);
+    VG_HIP(hipMemcpyAsync(&h, g.clk, sizeof(vg::Clock), hipMemcpyDeviceToHost, s));
+    VG_HIP(hipStreamSynchronize(s));
+    if (h.stalled) {
+        vg::set_error("a time step could not advance t (non-finite or zero wave speed)");
+        return VG_ERR_NONFINITE;
+    }
+    const int steps = h.steps_this_run;
+    std::vector<double> dts(std::max(steps, 1));
+    if (steps > 0) VG_HIP(hipMemcpyAsync(dts.data(), dts_dev, sizeof(double) * steps, hipMemcpyDeviceToHost, s));
+    // the final state, with slot 0's constant planes when the sweeps left them out
+    double *fin = slots[(size_t)kept * steps];
+    if (reduced && steps > 0) {
+        using R = vg::ReducedPhys<EulerPhys>;
+        for (int i = 0; i < R::kConst; i++) {
+            const long long off = (long long)R::plane(i) * g.plane;
+            VG_HIP(hipMemcpyAsync(fin + off, slots[0] + off, sizeof(double) * g.cells, hipMemcpyDeviceToDevice, s));
+        }
+    }
+    {
+        double *dst = on_device ? prim_out : g.staging;
+        vg::soa_to_aos_kernel<6><<<EGrid::blocks_for(g.cells), 256, 0, s>>>(fin, dst, g.plane, g.cells);
+        if ((rc = vg::launch_status("adjoint forward store"))) return rc;
+        if (!on_device)
+            VG_HIP(hipMemcpyAsync(prim_out, g.staging, sizeof(double) * 6 * g.cells, hipMemcpyDeviceToHost, s));
+        VG_HIP(hipStreamSynchronize(s));
+    }
+    // ---- reverse -------------------------------------------------------------------------
+    const double *seed_aos = seed;
+    if (!on_device) {
+        VG_HIP(hipMemcpyAsync(g.staging, seed, sizeof(double) * 6 * g.cells, hipMemcpyHostToDevice, s));
+        seed_aos = g.staging;
+    }
+    vg::aos_to_soa_kernel<6><<<EGrid::blocks_for(g.cells), 256, 0, s>>>(seed_aos, wb[0], g.plane, g.cells);
+    const double *w35 = reduced ? slots[0] : nullptr;
+    int cw = 0;
+    int xv = kAdjDefault[0], yv = kAdjDefault[1];
+    if (steps > 0) {
+        const vg::AdjointArgs ta =
+            adjoint_args(slots[0], wb[0], wb[1], ghost, g.plane, ny, nx, bc, dx, bias, limiter, dts[0], m);
+        rc = tune_adjoint(ctx, 0, ta, xv);
+        if (!rc && dim == 2) rc = tune_adjoint(ctx, 1, ta, yv);
+    }
+    for (int st = steps - 1; st >= 0 && !rc; st--) {
+        double *b[4];
+        step_bufs(st, b);
+        if (kept != per) {
+            // the step's two inner sweep inputs again, with its recorded dt
+            const std::vector<double *> rs = {b[0], b[1], b[2], nullptr};
+            rc = forward_fixed(ctx, g, dts_dev, st, st + 1, dx, bias, limiter, bc, m, rs, true);
+            if (rc) break;
+        }
+        const double dt = dts[st];
+        if (dim == 1) {
+            rc = adj_sweep(ctx, 0, xv, b[0], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias, limiter, dt, m,
+                           w35);
+            cw = 1 - cw;
+        } else {
+            rc = adj_sweep(ctx, 0, xv, b[2], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias, limiter,
+                           dt * 0.5, m, w35);
+            cw = 1 - cw;
+            if (!rc)
+                rc = adj_sweep(ctx, 1, yv, b[1], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias, limiter, dt,
+                               m, w35);
+            cw = 1 - cw;
+            if (!rc)
+                rc = adj_sweep(ctx, 0, xv, b[0], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias, limiter,
+                               dt * 0.5, m, w35);
+            cw = 1 - cw;
+        }
+    }
+    if (!rc) {
+        double *dst = on_device ? grad : g.staging;
+        vg::soa_to_aos_kernel<6><<<EGrid::blocks_for(g.cells), 256, 0, s>>>(wb[cw], dst, g.plane, g.cells);
+        rc = vg::launch_status("adjoint store");
+    }
+    if (!rc) {
+        hipError_t e = hipSuccess;
+        if (!on_device) e = hipMemcpyAsync(grad, g.staging, sizeof(double) * 6 * g.cells, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            vg::set_error(std::string("adjoint copy: ") + hipGetErrorString(e));
+            rc = VG_ERR_HIP;
+        }
+    }
+    if (!rc && ctx->timing) rc = ctx->resolve_timing();
+    if (dt_log) std::memcpy(dt_log, dts.data(), sizeof(double) * std::min(steps, std::max(dt_log_cap, 0)));
+    if (stats) {
+        *stats = vg_run_stats{};
+        stats->steps = steps;
+        stats->total_steps = steps;
+        stats->t = h.t;
+        stats->dt_min = steps ? *std::min_element(dts.begin(), dts.begin() + steps) : 0.0;
+        stats->dt_max = steps ? *std::max_element(dts.begin(), dts.begin() + steps) : 0.0;
+        stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+        stats->first_nonfinite = -1;
+    }
+    return rc;
+}
+
 int adjoint(vg_ctx *ctx, int dim, const double *prim_in, double *prim_out, const double *seed, double *grad, int ny,
             int nx, double dx, double cfl, double t_final, int max_steps, double bias, int limiter, int bc,
             const vg_euler_params *mp, int checkpoint_every, double *dt_log, int dt_log_cap, vg_run_stats *stats,
@@ -151,6 +349,17 @@ int adjoint(vg_ctx *ctx, int dim, const double *prim_in, double *prim_out, const
     }
     const size_t gbytes = sizeof(double) * 6 * g.plane;
     const int per = (dim == 2) ? 3 : 1;
+    // checkpoint_every 0: the fullest recorded layout that fits; 1: the step
+    // starts (a checkpoint every step, written by the forward itself)
+    int kept = 0;
+    if (checkpoint_every == 0) kept = record_layout(ctx, max_steps, per, gbytes);
+    else if (checkpoint_every == 1 && record_layout(ctx, max_steps, per, gbytes) > 0) kept = 1;
+    if (kept > 0) {
+        rc = adjoint_recorded(ctx, g, kept, dim, ny, nx, dx, cfl, t_final, max_steps, bias, limiter, bc, m, seed,
+                              prim_out, grad, on_device, dt_log, dt_log_cap, stats, w0);
+        g.destroy();
+        return rc;
+    }
     const int K = choose_interval(ctx, checkpoint_every, max_steps, per, gbytes);
     // Workspace buffers: 0 dts, 1-2 adjoint ping-pong, 3 ghost adjoints,
     // 4 .. 4 + per * K - 1 sweep inputs, then the checkpoints.

@@ -50,6 +50,7 @@ inline AdjointArgs adjoint_args(const double *w, const double *wbar_out, double 
 {
     AdjointArgs a;
     a.w = w;
+    a.w35 = w;
     a.wbar_out = wbar_out;
     a.wbar_in = wbar_in;
     a.ghost = ghost;
@@ -70,9 +71,10 @@ inline AdjointArgs adjoint_args(const double *w, const double *wbar_out, double 
 // recomputed, not shared), so the variant changes nothing in the result.
 inline int adj_sweep(vg_ctx *ctx, int dir, int variant, const double *w, const double *wbar_out, double *wbar_in,
               double *ghost, long long plane, int ny, int nx, int bc, double h, double bias, int limiter, double ts,
-              const EulerPhys::Params &m)
+              const EulerPhys::Params &m, const double *w35 = nullptr)
 {
-    const AdjointArgs a = adjoint_args(w, wbar_out, wbar_in, ghost, plane, ny, nx, bc, h, bias, limiter, ts, m);
+    AdjointArgs a = adjoint_args(w, wbar_out, wbar_in, ghost, plane, ny, nx, bc, h, bias, limiter, ts, m);
+    if (w35) a.w35 = w35;
     const int lines = (dir == 0) ? ny : nx;
     return ctx->timed(dir == 0 ? "adjoint_sweep_x" : "adjoint_sweep_y", [&] {
         launch_adjoint_dir(ctx, dir, variant, a);
