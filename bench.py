@@ -392,7 +392,12 @@ def bench_adjoint(args):
     w = problems.disc_2d(n)
     dx = LENGTH / n
     ctx = S.default_context()
+    # d loss / d final state for a loss on the fields this 2D problem evolves
+    # (rho, u, v, p): its w and lambda stay +0 and 1, so their parts are zero
+    # and the adjoint runs on the four fields
     seed = np.random.default_rng(1).standard_normal(w.shape)
+    seed[..., 3] = 0.0
+    seed[..., 5] = 0.0
     wd = torch.from_numpy(w).cuda()
     sd = torch.from_numpy(seed).cuda()
     od = torch.empty_like(wd)

@@ -217,6 +217,44 @@ def test_recorded_forward_stops_at_the_final_time(dim):
         O.assert_bitwise(r.grad, ref.grad, f"gradient, layout {k}")
 
 
+def planar(w):
+    w = np.array(w)
+    w[..., 3] = 0.0
+    w[..., 5] = 1.0
+    return w
+
+
+@pytest.mark.parametrize("make,dim,limiter,bias,bc,k", [
+    (lambda: sod_1d(150), 1, 0, 0.0, 0, 0),
+    (lambda: planar(smooth_1d(200)), 1, 1, 0.3, 1, 3),
+    (lambda: sod_2d(24, 40), 2, 0, 0.0, 0, 0),
+    (lambda: planar(smooth_2d(36, 20)), 2, 2, -0.2, 1, 1),
+    (lambda: planar(smooth_2d(30, 30)), 2, 1, 0.0, 0, 4),
+])
+def test_four_field_adjoint_equals_six_field(make, dim, limiter, bias, bc, k):
+    """With w = +0, lambda = 1, E_f = 0 and a seed whose w and lambda parts are
+    zero, the adjoint sweeps run on the four fields (rho, u, v, p); with the
+    context's 4-field switch off the six-field sweeps run.  The gradients are
+    the same numbers (zeros may differ in sign), for every recording layout."""
+    w = make()
+    seed = np.random.default_rng(6).standard_normal(w.shape)
+    seed[..., 3] = 0.0
+    seed[..., 5] = 0.0
+    dx = 1.0 / w.shape[-2]
+    kw = dict(bias=bias, limiter=limiter, params=mat(IDEAL), max_steps=9, bc=bc, checkpoint_every=k)
+    ctx6 = S.Context(0)
+    ctx6.set_reduced_fields(False)
+    six = S.euler_adjoint(w, seed, dim, dx, 0.6, math.inf, ctx=ctx6, **kw)
+    four = S.euler_adjoint(w, seed, dim, dx, 0.6, math.inf, **kw)
+    O.assert_bitwise(four.dt, six.dt, "time steps")
+    O.assert_bitwise(four.cells, six.cells, "final state")
+    assert np.array_equal(four.grad, six.grad), float(np.max(np.abs(four.grad - six.grad)))
+    nz = four.grad != 0.0
+    O.assert_bitwise(four.grad[nz], six.grad[nz], "nonzero gradient entries")
+    assert not np.any(four.grad[..., 3]) and not np.any(four.grad[..., 5])
+    assert np.count_nonzero(four.grad[..., 0]) > 0
+
+
 def test_adjoint_tile_variants_change_nothing():
     """Every tile variant of the adjoint sweeps (shape and occupancy target)
     gives the same gradient bit for bit (each output is computed with the same

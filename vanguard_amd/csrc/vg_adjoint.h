@@ -192,6 +192,100 @@ __device__ __forceinline__ void lin_flux(const EPar &m, const double (&U)[6], do
     EulerPhys::template flux_prim_fast<DIR>(m, w, U, F);
 }
 
+// ---- the physics of an adjoint sweep --------------------------------------------------
+// AdjFull: the six Euler fields.  AdjPlanar: the four fields (rho, u, v, p) --
+// planes 0, 1, 2 and 4 -- of a grid whose forward took the 4-field sweeps
+// (w = +0 and lambda = 1 everywhere) with no formation energy, for seeds whose
+// w and lambda parts are zero.  Then the w and lambda adjoints stay zero (their
+// fluxes and the pressure carry no term that feeds them, and the terms they
+// add to the other fields are products with w = 0, with the zero adjoints or
+// with E_f = 0), so the 4-field VJPs are the 6-field ones with those zero
+// terms left out, in the same order.
+
+struct AdjFull {
+    using P = EulerPhys;
+    static constexpr int NF = 6;
+    static constexpr const char *key = "euler_adjoint";
+    __device__ static __forceinline__ constexpr int plane(int f) { return f; }
+    __device__ static __forceinline__ void vjp_p2c(const EPar &m, const double (&w)[NF], const double (&Ub)[NF],
+                                                   double (&wb)[NF])
+    {
+        vg::vjp_p2c(m, w, Ub, wb);
+    }
+    __device__ static __forceinline__ void vjp_c2p(const EPar &m, const double (&U)[NF], const double (&wb)[NF],
+                                                   double (&Ub)[NF])
+    {
+        vg::vjp_c2p(m, U, wb, Ub);
+    }
+    template <int DIR>
+    __device__ static __forceinline__ void vjp_flux(const EPar &m, const double (&U)[NF], const double (&Fb)[NF],
+                                                    double (&Ub)[NF])
+    {
+        vg::vjp_flux<DIR>(m, U, Fb, Ub);
+    }
+};
+
+struct AdjPlanar {
+    using P = EulerPlanarPhys;
+    static constexpr int NF = 4;
+    static constexpr const char *key = "euler_planar_adjoint";
+    __device__ static __forceinline__ constexpr int plane(int f) { return f < 3 ? f : 4; }
+    // vjp_p2c with z = 0, lambda = 1, E_f = 0 and the w, lambda adjoints zero.
+    __device__ static __forceinline__ void vjp_p2c(const EPar &m, const double (&w)[NF], const double (&Ub)[NF],
+                                                   double (&wb)[NF])
+    {
+#pragma clang fp contract(fast)
+        const double rho = w[0], u = w[1], v = w[2];
+        const double vsq = u * u + v * v;
+        wb[0] = Ub[0] + u * Ub[1] + v * Ub[2] + (0.5 * vsq) * Ub[3];
+        wb[1] = rho * Ub[1] + rho * u * Ub[3];
+        wb[2] = rho * Ub[2] + rho * v * Ub[3];
+        wb[3] = Ub[3] * m.inv_gm1;
+    }
+    __device__ static __forceinline__ void vjp_c2p(const EPar &m, const double (&U)[NF], const double (&wb)[NF],
+                                                   double (&Ub)[NF])
+    {
+#pragma clang fp contract(fast)
+        const double q = rcp_nr(U[0]);
+        const double u = U[1] * q, v = U[2] * q;
+        const double g1 = m.gm1;
+        const double vsq = u * u + v * v;
+        Ub[0] = wb[0] - (u * wb[1] + v * wb[2]) * q + g1 * (0.5 * vsq) * wb[3];
+        Ub[1] = wb[1] * q - g1 * u * wb[3];
+        Ub[2] = wb[2] * q - g1 * v * wb[3];
+        Ub[3] = g1 * wb[3];
+    }
+    template <int DIR>
+    __device__ static __forceinline__ void vjp_flux(const EPar &m, const double (&U)[NF], const double (&Fb)[NF],
+                                                    double (&Ub)[NF])
+    {
+#pragma clang fp contract(fast)
+        const double q = rcp_nr(U[0]);
+        const double vel[2] = {U[1] * q, U[2] * q};
+        const double un = vel[DIR];
+        const double g1 = m.gm1;
+        const double vsq = vel[0] * vel[0] + vel[1] * vel[1];
+        const double p = g1 * (U[3] - 0.5 * U[0] * vsq) - m.gpinf;
+        const double H = (U[3] + p) * q;
+        const double T = vel[0] * Fb[1] + vel[1] * Fb[2] + H * Fb[3];
+        const double sp = g1 * (Fb[1 + DIR] + un * Fb[3]);
+        Ub[0] = sp * (0.5 * vsq) - un * T;
+        Ub[1] = un * Fb[1] - sp * vel[0];
+        Ub[2] = un * Fb[2] - sp * vel[1];
+        Ub[1 + DIR] += T + Fb[0];
+        Ub[3] = sp + un * Fb[3];
+    }
+};
+
+// Flux of a conserved vector of physics Q with one refined reciprocal of rho.
+template <class Q, int DIR, int NF>
+__device__ __forceinline__ void lin_flux_of(const EPar &m, const double (&U)[NF], double (&F)[NF])
+{
+    double w[NF];
+    Q::cons_to_prim_fast(m, U, w);
+    Q::template flux_prim_fast<DIR>(m, w, U, F);
+}
+
 // ---- the adjoint tile kernel ----------------------------------------------------------
 
 struct AdjointArgs {
@@ -259,21 +353,21 @@ __device__ __forceinline__ void opaque(double (&v)[K])
 // a solve whose grid stays admissible, such as the plain Euler adjoint's -- are
 // left as they are.  In the two-material adjoint such states arise in a
 // material's grid far from the region the level set makes real.
-__device__ __forceinline__ void linearisation_state(double (&w)[6])
+template <class A>
+__device__ __forceinline__ void linearisation_state(double (&w)[A::NF])
 {
     bool ok = w[0] > 0.0;
 #pragma unroll
-    for (int f = 0; f < 6; f++) ok = ok && isfinite(w[f]);
-    if (!ok) {
-        w[0] = 1.0; w[1] = 0.0; w[2] = 0.0; w[3] = 0.0; w[4] = 1.0; w[5] = 1.0;
-    }
+    for (int f = 0; f < A::NF; f++) ok = ok && isfinite(w[f]);
+    if (!ok) A::P::default_state(w);
 }
 
-template <int DIR, int BS, int LW, int WPE>
+template <class A, int DIR, int BS, int LW, int WPE>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void euler_adjoint_sweep_kernel(
     AdjointArgs a)
 {
-    constexpr int NF = 6;
+    constexpr int NF = A::NF;
+    using Q = typename A::P;
     using T = AdjTile<BS, LW>;
     __shared__ double L0[NF * BS], L1[NF * BS], L2[NF * BS];
     const int tid = threadIdx.x;
@@ -296,10 +390,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         const int k = (pos >= -2 && pos < n + 2) ? ghost_index(pos, n, a.bc) : (pos < 0 ? 0 : n - 1);
         const long long c = adj_cell<DIR>(line, k, a.nx);
 #pragma unroll
-        for (int f = 0; f < NF; f++) w[f] = ((f == 3 || f == 5) ? a.w35 : a.w)[f * a.plane + c];
-        linearisation_state(w);
+        for (int f = 0; f < NF; f++) w[f] = ((f == 3 || f == 5) && NF == 6 ? a.w35 : a.w)[A::plane(f) * a.plane + c];
+        linearisation_state<A>(w);
         double U[NF];
-        EulerPhys::prim_to_cons(m, w, U);
+        Q::prim_to_cons(m, w, U);
 #pragma unroll
         for (int f = 0; f < NF; f++) L0[f * BS + tid] = U[f];
     }
@@ -324,8 +418,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void 
             xr[k] = u + (0.5 * d);
         }
         double Fl[NF], Fr[NF];
-        lin_flux<DIR>(m, xl, Fl);
-        lin_flux<DIR>(m, xr, Fr);
+        lin_flux_of<Q, DIR>(m, xl, Fl);
+        lin_flux_of<Q, DIR>(m, xr, Fr);
 #pragma unroll
         for (int k = 0; k < NF; k++) {
             const double e = half_c * (Fl[k] - Fr[k]);
@@ -335,7 +429,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     }
     {
         double Fp[NF];
-        lin_flux<DIR>(m, XL, Fp);
+        lin_flux_of<Q, DIR>(m, XL, Fp);
 #pragma unroll
         for (int f = 0; f < NF; f++) {
             L1[f * BS + tid] = XL[f];
@@ -343,7 +437,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         }
 #pragma unroll
         for (int f = 0; f < NF; f++) URp[f] = XR[f];
-        lin_flux<DIR>(m, URp, FRp);
+        lin_flux_of<Q, DIR>(m, URp, FRp);
     }
     __syncthreads();
     // ---- F3: face q+1/2, a = right state of this cell, b = left state of the next ----
@@ -356,7 +450,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void 
             lf[k] = 0.5 * ((FRp[k] + fb) + (lf_c * (URp[k] - ub)));
             Us[k] = 0.5 * ((URp[k] + ub) + (ri_c * (FRp[k] - fb)));
         }
-        lin_flux<DIR>(m, Us, ri);
+        lin_flux_of<Q, DIR>(m, Us, ri);
 #pragma unroll
         for (int k = 0; k < NF; k++) L0[k * BS + tid] = 0.5 * (lf[k] + ri[k]);
     }
@@ -365,18 +459,18 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     opaque(w);
     {
         double U[NF], N[NF], wb[NF], Nb[NF];
-        EulerPhys::prim_to_cons_fast(m, w, U);
+        Q::prim_to_cons_fast(m, w, U);
 #pragma unroll
         for (int k = 0; k < NF; k++) N[k] = U[k] + (up_c * (L0[k * BS + tl] - L0[k * BS + tid]));
         if (real) {
             const long long c = adj_cell<DIR>(line, pos, a.nx);
 #pragma unroll
-            for (int f = 0; f < NF; f++) wb[f] = a.wbar_out[f * a.plane + c];
+            for (int f = 0; f < NF; f++) wb[f] = a.wbar_out[A::plane(f) * a.plane + c];
         } else {
 #pragma unroll
             for (int f = 0; f < NF; f++) wb[f] = 0.0;
         }
-        vjp_c2p(m, N, wb, Nb);
+        A::vjp_c2p(m, N, wb, Nb);
 #pragma unroll
         for (int f = 0; f < NF; f++) L1[f * BS + tid] = Nb[f];
     }
@@ -390,7 +484,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void 
             Fbar[k] = has_r ? up_c * (L1[k * BS + tr] - L1[k * BS + tid]) : 0.0;
             RIb[k] = 0.5 * Fbar[k];
         }
-        vjp_flux<DIR>(m, Us, RIb, Usb);
+        A::template vjp_flux<DIR>(m, Us, RIb, Usb);
 #pragma unroll
         for (int k = 0; k < NF; k++) {
             const double LFb = 0.5 * Fbar[k];
@@ -415,8 +509,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     double Ubar[NF];
     {
         double XLb[NF], XRb[NF];
-        vjp_flux<DIR>(m, XL, Ft, XLb);
-        vjp_flux<DIR>(m, XR, Fab, XRb);
+        A::template vjp_flux<DIR>(m, XL, Ft, XLb);
+        A::template vjp_flux<DIR>(m, XR, Fab, XRb);
 #pragma unroll
         for (int k = 0; k < NF; k++) {
             XLb[k] += Ut[k];
@@ -424,7 +518,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         }
         // evolution and extrapolation -> U-bar (own) and d-bar, then the slope
         double U[NF], xl[NF], xr[NF], eb[NF], tl1[NF], tr1[NF];
-        EulerPhys::prim_to_cons_fast(m, w, U);
+        Q::prim_to_cons_fast(m, w, U);
 #pragma unroll
         for (int k = 0; k < NF; k++) {
             const SlopeLin sl = slope_lin(dl[k], dr[k], (pieces >> (3 * k)) & 7, cl, cr, a.bias, a.limiter);
@@ -433,8 +527,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void 
             xr[k] = U[k] + (0.5 * d);
             eb[k] = half_c * (XLb[k] + XRb[k]);
         }
-        vjp_flux<DIR>(m, xl, eb, tl1);
-        vjp_flux<DIR>(m, xr, eb, tr1);
+        A::template vjp_flux<DIR>(m, xl, eb, tl1);
+        A::template vjp_flux<DIR>(m, xr, eb, tr1);
 #pragma unroll
         for (int k = 0; k < NF; k++) {
             const double xlb = XLb[k] + tl1[k];
@@ -459,15 +553,15 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         if (has_l) Ubar[f] += L2[f * BS + tl];
     }
     double wb[NF];
-    vjp_p2c(m, w, Ubar, wb);
+    A::vjp_p2c(m, w, Ubar, wb);
     if (line_ok && q >= kAdjHalo && q < T::len - kAdjHalo && pos >= -2 && pos < n + 2) {
         if (real) {
             const long long c = adj_cell<DIR>(line, pos, a.nx);
 #pragma unroll
-            for (int f = 0; f < NF; f++) a.wbar_in[f * a.plane + c] = wb[f];
+            for (int f = 0; f < NF; f++) a.wbar_in[A::plane(f) * a.plane + c] = wb[f];
         } else {
             const int g = (pos < 0) ? pos + 2 : 2 + (pos - n);   // -2,-1,n,n+1 -> 0..3
-            double *dst = a.ghost + ((long long)line * 4 + g) * NF;
+            double *dst = a.ghost + ((long long)line * 4 + g) * 6;
 #pragma unroll
             for (int f = 0; f < NF; f++) dst[f] = wb[f];
         }
@@ -488,9 +582,20 @@ static __global__ __launch_bounds__(256) void adjoint_nonfinite_kernel(const dou
     if (bad) atomicMax(first, step);
 }
 
+// Clears *ok unless planes 3 and 5 (w, lambda) of the SoA buffer are zero in
+// every cell.
+static __global__ __launch_bounds__(256) void zero_planes_kernel(const double *__restrict__ a, long long plane,
+                                                                 long long cells, int *ok)
+{
+    bool zero = true;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += (long long)gridDim.x * blockDim.x)
+        zero = zero && a[3 * plane + c] == 0.0 && a[5 * plane + c] == 0.0;
+    if (!zero) *ok = 0;
+}
+
 // Adds the ghost positions' adjoint to the cells they copy: transmissive
 // g[-1] = c[0], g[-2] = c[1], g[n] = c[n-1], g[n+1] = c[n-2]; periodic wraps.
-template <int DIR>
+template <class A, int DIR>
 __global__ void euler_adjoint_fold_kernel(double *wbar_in, const double *ghost, long long plane, int ny, int nx,
                                           int bc)
 {
@@ -504,7 +609,7 @@ __global__ void euler_adjoint_fold_kernel(double *wbar_in, const double *ghost, 
         const int k = ghost_index(pos[g], n, bc);
         const long long c = adj_cell<DIR>(line, k, nx);
         const double *src = ghost + ((long long)line * 4 + g) * 6;
-        for (int f = 0; f < 6; f++) wbar_in[f * plane + c] += src[f];
+        for (int f = 0; f < A::NF; f++) wbar_in[A::plane(f) * plane + c] += src[f];
     }
 }
 

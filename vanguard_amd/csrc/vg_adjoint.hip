@@ -126,6 +126,56 @@ int choose_interval(vg_ctx *ctx, int checkpoint_every, int max_steps, int per, s
     return std::max(1, (int)std::ceil(std::sqrt((double)max_steps)));
 }
 
+// The adjoint sweeps' physics for this run: the 4-field ones (AdjPlanar) when
+// the forward took the 4-field sweeps, E_f = 0 and the seed (SoA, in seed_soa)
+// has zero w and lambda parts; the w and lambda adjoints then stay zero, so
+// planes 3 and 5 of the other adjoint buffer are cleared once and neither
+// buffer's planes 3 and 5 are written again.
+int choose_adjoint_physics(vg_ctx *ctx, const EGrid &g, const EulerPhys::Params &m, const double *seed_soa,
+                           double *other, bool &planar)
+{
+    planar = false;
+    if (!g.reduced || m.e_form != 0.0) return VG_OK;
+    hipStream_t s = ctx->stream;
+    int *flag = nullptr;
+    VG_HIP(hipMalloc(&flag, sizeof(int)));
+    const int one = 1;
+    int ok = 0;
+    hipError_t e = hipMemcpyAsync(flag, &one, sizeof(int), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        vg::zero_planes_kernel<<<EGrid::blocks_for(g.cells), 256, 0, s>>>(seed_soa, g.plane, g.cells, flag);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&ok, flag, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    hipFree(flag);
+    if (e != hipSuccess) {
+        vg::set_error(std::string("adjoint seed check: ") + hipGetErrorString(e));
+        return VG_ERR_HIP;
+    }
+    if (!ok) return VG_OK;
+    VG_HIP(hipMemsetAsync(other + 3 * g.plane, 0, sizeof(double) * g.cells, s));
+    VG_HIP(hipMemsetAsync(other + 5 * g.plane, 0, sizeof(double) * g.cells, s));
+    planar = true;
+    return VG_OK;
+}
+
+int adj_sweep_p(bool planar, vg_ctx *ctx, int dir, int variant, const double *w, const double *wbar_out,
+                double *wbar_in, double *ghost, long long plane, int ny, int nx, int bc, double h, double bias,
+                int limiter, double ts, const EulerPhys::Params &m, const double *w35 = nullptr)
+{
+    if (planar)
+        return adj_sweep<vg::AdjPlanar>(ctx, dir, variant, w, wbar_out, wbar_in, ghost, plane, ny, nx, bc, h, bias,
+                                        limiter, ts, m, w35);
+    return adj_sweep<vg::AdjFull>(ctx, dir, variant, w, wbar_out, wbar_in, ghost, plane, ny, nx, bc, h, bias, limiter,
+                                  ts, m, w35);
+}
+
+int tune_adjoint_p(bool planar, vg_ctx *ctx, int dir, const vg::AdjointArgs &a, int &variant)
+{
+    return planar ? tune_adjoint<vg::AdjPlanar>(ctx, dir, a, variant) : tune_adjoint<vg::AdjFull>(ctx, dir, a, variant);
+}
+
 // Grids of the recorded adjoint's layouts for a run of max_steps steps: every
 // sweep input (per per step), or the step starts plus two recomputation buffers;
 // 0 when neither fits in free device memory beside the adjoint's own buffers.
@@ -256,14 +306,16 @@ int adjoint_recorded(vg_ctx *ctx, EGrid &g, int kept, int dim, int ny, int nx, d
         seed_aos = g.staging;
     }
     vg::aos_to_soa_kernel<6><<<EGrid::blocks_for(g.cells), 256, 0, s>>>(seed_aos, wb[0], g.plane, g.cells);
+    bool planar = false;
+    if ((rc = choose_adjoint_physics(ctx, g, m, wb[0], wb[1], planar))) return rc;
     const double *w35 = reduced ? slots[0] : nullptr;
     int cw = 0;
     int xv = kAdjDefault[0], yv = kAdjDefault[1];
     if (steps > 0) {
         const vg::AdjointArgs ta =
             adjoint_args(slots[0], wb[0], wb[1], ghost, g.plane, ny, nx, bc, dx, bias, limiter, dts[0], m);
-        rc = tune_adjoint(ctx, 0, ta, xv);
-        if (!rc && dim == 2) rc = tune_adjoint(ctx, 1, ta, yv);
+        rc = tune_adjoint_p(planar, ctx, 0, ta, xv);
+        if (!rc && dim == 2) rc = tune_adjoint_p(planar, ctx, 1, ta, yv);
     }
     for (int st = steps - 1; st >= 0 && !rc; st--) {
         double *b[4];
@@ -276,19 +328,19 @@ int adjoint_recorded(vg_ctx *ctx, EGrid &g, int kept, int dim, int ny, int nx, d
         }
         const double dt = dts[st];
         if (dim == 1) {
-            rc = adj_sweep(ctx, 0, xv, b[0], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias, limiter, dt, m,
+            rc = adj_sweep_p(planar, ctx, 0, xv, b[0], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias, limiter, dt, m,
                            w35);
             cw = 1 - cw;
         } else {
-            rc = adj_sweep(ctx, 0, xv, b[2], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias, limiter,
+            rc = adj_sweep_p(planar, ctx, 0, xv, b[2], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias, limiter,
                            dt * 0.5, m, w35);
             cw = 1 - cw;
             if (!rc)
-                rc = adj_sweep(ctx, 1, yv, b[1], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias, limiter, dt,
+                rc = adj_sweep_p(planar, ctx, 1, yv, b[1], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias, limiter, dt,
                                m, w35);
             cw = 1 - cw;
             if (!rc)
-                rc = adj_sweep(ctx, 0, xv, b[0], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias, limiter,
+                rc = adj_sweep_p(planar, ctx, 0, xv, b[0], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias, limiter,
                                dt * 0.5, m, w35);
             cw = 1 - cw;
         }
@@ -429,6 +481,8 @@ int adjoint(vg_ctx *ctx, int dim, const double *prim_in, double *prim_out, const
         seed_aos = g.staging;
     }
     vg::aos_to_soa_kernel<6><<<EGrid::blocks_for(g.cells), 256, 0, s>>>(seed_aos, wb[0], g.plane, g.cells);
+    bool planar = false;
+    if (!rc) rc = choose_adjoint_physics(ctx, g, m, wb[0], wb[1], planar);
     int cw = 0;
     const double dy = dx;
     int xv = kAdjDefault[0], yv = kAdjDefault[1];
@@ -436,8 +490,8 @@ int adjoint(vg_ctx *ctx, int dim, const double *prim_in, double *prim_out, const
         // tune on the checkpointed initial grid with the first time step
         const vg::AdjointArgs ta =
             adjoint_args(ckpt[0], wb[0], wb[1], ghost, g.plane, ny, nx, bc, dx, bias, limiter, dts[0], m);
-        rc = tune_adjoint(ctx, 0, ta, xv);
-        if (!rc && dim == 2) rc = tune_adjoint(ctx, 1, ta, yv);
+        rc = tune_adjoint_p(planar, ctx, 0, ta, xv);
+        if (!rc && dim == 2) rc = tune_adjoint_p(planar, ctx, 1, ta, yv);
     }
     for (int seg = (int)ckpt.size() - 1; seg >= 0 && !rc; seg--) {
         const int s0 = seg * K, s1 = std::min(steps, s0 + K);
@@ -448,19 +502,19 @@ int adjoint(vg_ctx *ctx, int dim, const double *prim_in, double *prim_out, const
             const int base = (st - s0) * per;
             const double dt = dts[st];
             if (dim == 1) {
-                rc = adj_sweep(ctx, 0, xv, slots[base], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias,
+                rc = adj_sweep_p(planar, ctx, 0, xv, slots[base], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias,
                                limiter, dt, m);
                 cw = 1 - cw;
             } else {
-                rc = adj_sweep(ctx, 0, xv, slots[base + 2], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias,
+                rc = adj_sweep_p(planar, ctx, 0, xv, slots[base + 2], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias,
                                limiter, dt * 0.5, m);
                 cw = 1 - cw;
                 if (!rc)
-                    rc = adj_sweep(ctx, 1, yv, slots[base + 1], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dy,
+                    rc = adj_sweep_p(planar, ctx, 1, yv, slots[base + 1], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dy,
                                    bias, limiter, dt, m);
                 cw = 1 - cw;
                 if (!rc)
-                    rc = adj_sweep(ctx, 0, xv, slots[base], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias,
+                    rc = adj_sweep_p(planar, ctx, 0, xv, slots[base], wb[cw], wb[1 - cw], ghost, g.plane, ny, nx, bc, dx, bias,
                                    limiter, dt * 0.5, m);
                 cw = 1 - cw;
             }

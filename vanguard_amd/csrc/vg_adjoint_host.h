@@ -10,13 +10,13 @@
 
 namespace vg {
 
-template <int DIR, int BS, int LW, int WPE>
+template <class A, int DIR, int BS, int LW, int WPE>
 inline void launch_adjoint_tile(vg_ctx *ctx, const AdjointArgs &a)
 {
     const int n = (DIR == 0) ? a.nx : a.ny, lines = (DIR == 0) ? a.ny : a.nx;
     using T = AdjTile<BS, LW>;
     dim3 grid((n + 4 + T::out - 1) / T::out, (lines + LW - 1) / LW);
-    euler_adjoint_sweep_kernel<DIR, BS, LW, WPE><<<grid, BS, 0, ctx->stream>>>(a);
+    euler_adjoint_sweep_kernel<A, DIR, BS, LW, WPE><<<grid, BS, 0, ctx->stream>>>(a);
 }
 
 // Tile variants per direction (threads, lines per block, waves per SIMD).  X:
@@ -25,22 +25,23 @@ inline void launch_adjoint_tile(vg_ctx *ctx, const AdjointArgs &a)
 constexpr int kAdjVariants = 4;
 constexpr int kAdjDefault[2] = {0, 0};
 
+template <class A = AdjFull>
 inline void launch_adjoint_dir(vg_ctx *ctx, int dir, int variant, const AdjointArgs &a)
 {
     if (dir == 0) {
         switch (variant) {
-        case 1: launch_adjoint_tile<0, 128, 1, 3>(ctx, a); break;
-        case 2: launch_adjoint_tile<0, 256, 1, 2>(ctx, a); break;
-        case 3: launch_adjoint_tile<0, 64, 1, 2>(ctx, a); break;
-        default: launch_adjoint_tile<0, 128, 1, 2>(ctx, a); break;
+        case 1: launch_adjoint_tile<A, 0, 128, 1, 3>(ctx, a); break;
+        case 2: launch_adjoint_tile<A, 0, 256, 1, 2>(ctx, a); break;
+        case 3: launch_adjoint_tile<A, 0, 64, 1, 2>(ctx, a); break;
+        default: launch_adjoint_tile<A, 0, 128, 1, 2>(ctx, a); break;
         }
         return;
     }
     switch (variant) {
-    case 1: launch_adjoint_tile<1, 256, 8, 3>(ctx, a); break;
-    case 2: launch_adjoint_tile<1, 256, 16, 2>(ctx, a); break;
-    case 3: launch_adjoint_tile<1, 256, 4, 2>(ctx, a); break;
-    default: launch_adjoint_tile<1, 256, 8, 2>(ctx, a); break;
+    case 1: launch_adjoint_tile<A, 1, 256, 8, 3>(ctx, a); break;
+    case 2: launch_adjoint_tile<A, 1, 256, 16, 2>(ctx, a); break;
+    case 3: launch_adjoint_tile<A, 1, 256, 4, 2>(ctx, a); break;
+    default: launch_adjoint_tile<A, 1, 256, 8, 2>(ctx, a); break;
     }
 }
 
@@ -69,6 +70,7 @@ inline AdjointArgs adjoint_args(const double *w, const double *wbar_out, double 
 // One adjoint sweep: wbar_in = (d sweep / d w)^T wbar_out at input w.  Every
 // tile variant computes each output with the same arithmetic (the halo is
 // recomputed, not shared), so the variant changes nothing in the result.
+template <class A = AdjFull>
 inline int adj_sweep(vg_ctx *ctx, int dir, int variant, const double *w, const double *wbar_out, double *wbar_in,
               double *ghost, long long plane, int ny, int nx, int bc, double h, double bias, int limiter, double ts,
               const EulerPhys::Params &m, const double *w35 = nullptr)
@@ -77,28 +79,28 @@ inline int adj_sweep(vg_ctx *ctx, int dir, int variant, const double *w, const d
     if (w35) a.w35 = w35;
     const int lines = (dir == 0) ? ny : nx;
     return ctx->timed(dir == 0 ? "adjoint_sweep_x" : "adjoint_sweep_y", [&] {
-        launch_adjoint_dir(ctx, dir, variant, a);
+        launch_adjoint_dir<A>(ctx, dir, variant, a);
         if (dir == 0)
-            euler_adjoint_fold_kernel<0><<<(lines + 255) / 256, 256, 0, ctx->stream>>>(wbar_in, ghost, plane, ny,
-                                                                                            nx, bc);
+            euler_adjoint_fold_kernel<A, 0><<<(lines + 255) / 256, 256, 0, ctx->stream>>>(wbar_in, ghost, plane, ny,
+                                                                                               nx, bc);
         else
-            euler_adjoint_fold_kernel<1><<<(lines + 255) / 256, 256, 0, ctx->stream>>>(wbar_in, ghost, plane, ny,
-                                                                                            nx, bc);
+            euler_adjoint_fold_kernel<A, 1><<<(lines + 255) / 256, 256, 0, ctx->stream>>>(wbar_in, ghost, plane, ny,
+                                                                                               nx, bc);
     });
 }
 
-inline std::string adjoint_key(int dir, int ny, int nx)
+inline std::string adjoint_key(const char *phys, int dir, int ny, int nx)
 {
-    return std::string(dir == 0 ? "euler_adjoint_x/" : "euler_adjoint_y/") + std::to_string(ny) + "/" +
-           std::to_string(nx);
+    return std::string(phys) + (dir == 0 ? "_x/" : "_y/") + std::to_string(ny) + "/" + std::to_string(nx);
 }
 
 // Tile variant of one direction for an ny x nx grid: the context's table, else
 // each variant runs once to warm up and twice timed on the caller's buffers
 // (the real sweep overwrites what they write), and the fastest is recorded.
+template <class A = AdjFull>
 inline int tune_adjoint(vg_ctx *ctx, int dir, const AdjointArgs &a, int &variant)
 {
-    const std::string key = adjoint_key(dir, a.ny, a.nx);
+    const std::string key = adjoint_key(A::key, dir, a.ny, a.nx);
     auto it = ctx->tuned.find(key);
     if (it != ctx->tuned.end()) {
         variant = it->second;
@@ -111,9 +113,9 @@ inline int tune_adjoint(vg_ctx *ctx, int dir, const AdjointArgs &a, int &variant
     VG_HIP(hipEventCreate(&e1));
     float best = 0.0f;
     for (int v = 0; v < kAdjVariants; v++) {
-        launch_adjoint_dir(ctx, dir, v, a);
+        launch_adjoint_dir<A>(ctx, dir, v, a);
         VG_HIP(hipEventRecord(e0, ctx->stream));
-        for (int r = 0; r < 2; r++) launch_adjoint_dir(ctx, dir, v, a);
+        for (int r = 0; r < 2; r++) launch_adjoint_dir<A>(ctx, dir, v, a);
         VG_HIP(hipEventRecord(e1, ctx->stream));
         VG_HIP(hipEventSynchronize(e1));
         int rc = launch_status("adjoint autotune");

@@ -227,7 +227,7 @@ int vg_ctx_release_workspace(vg_ctx *ctx)
 int vg_ctx_set_tuned_shape(vg_ctx *ctx, const char *key, int variant)
 {
     VG_CHECK(ctx != nullptr && key != nullptr, "vg_ctx_set_tuned_shape: null argument");
-    const bool adjoint = std::string(key).rfind("euler_adjoint", 0) == 0;
+    const bool adjoint = std::string(key).find("adjoint") != std::string::npos;
     VG_CHECK(variant >= -1 && variant < (adjoint ? 4 : 6),
              adjoint ? "vg_ctx_set_tuned_shape: adjoint variant must be -1..3"
                      : "vg_ctx_set_tuned_shape: y-sweep variant must be -1..5");
