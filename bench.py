@@ -64,7 +64,7 @@ BYTES_PER_CELL_SWEEP = 96   # read 6 + write 6 FP64 primitive fields
 # measured with rocprofv3 --pmc; profiles/euler2d_4096_slic_fast_hbm_traffic.md and
 # profiles/euler2d_4096_slic_exact_hbm_traffic.md.
 MEASURED_TRAFFIC = {("fast", "sweep_x_slic"): 1116074640, ("fast", "sweep_y_slic"): 1106876032,
-                    ("exact", "sweep_x_slic"): 1116162867, ("exact", "sweep_y_slic"): 1107054592}
+                    ("exact", "sweep_x_slic"): 1116162048, ("exact", "sweep_y_slic"): 1106966528}
 METRIC = "Mcell-updates/s FP64 SLIC 2D Euler 4096²; %HBM peak; adjoint/forward cost"
 
 
