@@ -10,6 +10,8 @@ C ABI, time steps frozen.
    numbers and against central differences of Gamma(M) through the GPU's own
    fixed-time-step forward.
 4. The checkpoint interval changes nothing (bit for bit).
+5. With w = +0 and lambda = 1 in both materials and seeds without w and lambda
+   parts the adjoint runs on four fields; its gradients are the 6-field ones.
 """
 
 import math
@@ -141,6 +143,60 @@ def test_checkpoint_interval_changes_nothing():
         other = run_adjoint(case, s1, s2, 17, k=k)
         O.assert_bitwise(other.grad1, base.grad1, f"material 1, checkpoint every {k}")
         O.assert_bitwise(other.grad2, base.grad2, f"material 2, checkpoint every {k}")
+
+
+def planar_seeds(shape, seed):
+    rng = np.random.default_rng(seed)
+    s1, s2 = rng.standard_normal(shape) / SCALE, rng.standard_normal(shape) / SCALE
+    for s in (s1, s2):
+        s[..., 3] = 0.0
+        s[..., 5] = 0.0
+    return s1, s2
+
+
+@pytest.mark.parametrize("dim,tie,steps,k", [(1, True, 30, 0), (1, False, 25, 4), (2, True, 20, 0),
+                                             (2, False, 17, 3), (2, False, 9, 1)])
+def test_four_field_adjoint_equals_six_field(dim, tie, steps, k):
+    """The 4-field adjoint (default context) against the 6-field one (the
+    context's 4-field switch off, which also gives the forward six fields):
+    same time steps, same forward bits, the same gradient numbers (zeros may
+    differ in sign), zero w and lambda parts."""
+    case = rgfm_case(dim, tie=tie)
+    m1, m2, phi, dx, dy = case
+    assert not np.any(m1[..., 3]) and np.all(m1[..., 5] == 1.0) and np.all(m2[..., 5] == 1.0)
+    s1, s2 = planar_seeds(m1.shape, 80 + 2 * dim + int(tie))
+    ctx6 = S.Context(0)
+    ctx6.set_reduced_fields(False)
+    kw = dict(dy=dy, max_steps=steps, checkpoint_every=k)
+    six = S.rgfm_adjoint(S.MultimaterialSystem(m1, m2, phi), s1, s2, dx, 0.8, math.inf, HELIUM, AIR, ctx=ctx6, **kw)
+    four = S.rgfm_adjoint(S.MultimaterialSystem(m1, m2, phi), s1, s2, dx, 0.8, math.inf, HELIUM, AIR, **kw)
+    O.assert_bitwise(four.dt, six.dt, "time steps")
+    O.assert_bitwise(four.system.material1, six.system.material1, "forward 1")
+    O.assert_bitwise(four.system.material2, six.system.material2, "forward 2")
+    for g4, g6, name in ((four.grad1, six.grad1, "material 1"), (four.grad2, six.grad2, "material 2")):
+        assert np.array_equal(g4, g6), (name, float(np.max(np.abs(g4 - g6))))
+        nz = g4 != 0.0
+        O.assert_bitwise(g4[nz], g6[nz], f"{name}: nonzero gradient entries")
+        assert not np.any(g4[..., 3]) and not np.any(g4[..., 5])
+        assert np.count_nonzero(g4[..., 0]) > 0
+
+
+@pytest.mark.parametrize("dim,tie", [(1, False), (2, True)])
+def test_four_field_dot_product_against_dual_numbers(dim, tie):
+    """The 4-field adjoint in the dot-product test against the oracle's
+    forward-mode derivative, with tangents in all six fields."""
+    case = rgfm_case(dim, tie=tie)
+    m1, m2, phi, dx, dy = case
+    s1, s2 = planar_seeds(m1.shape, 90 + 2 * dim + int(tie))
+    adj = run_adjoint(case, s1, s2, 20)
+    rng = np.random.default_rng(91)
+    v1, v2 = rng.standard_normal(m1.shape) * SCALE, rng.standard_normal(m2.shape) * SCALE
+    _, _, _, t1, t2 = O.rgfm_fixed_dt(m1, m2, phi, adj.dt, dim=dim, dx=dx, dy=dy, bias=0.0, limiter=0,
+                                      p1=HELIUM, p2=AIR, tangent=(v1, v2))
+    lhs = float(np.sum(adj.grad1 * v1) + np.sum(adj.grad2 * v2))
+    rhs = float(np.sum(s1 * t1) + np.sum(s2 * t2))
+    scale = float(np.sum(np.abs(adj.grad1 * v1)) + np.sum(np.abs(adj.grad2 * v2)))
+    assert abs(lhs - rhs) <= 1e-9 * scale, (lhs, rhs, scale)
 
 
 def test_adjoint_to_final_time_on_the_benchmark_geometry():

@@ -256,6 +256,26 @@ struct RGFMGrid {
                                       : tune_y<Q, 2, MODE_EXACT>(ctx, key, ta, y_variant);
     }
 
+    // Device copies of both materials' grids: whole, or the planes the 4-field
+    // sweeps carry (rho, u, v: planes 0-2; p: plane 4).
+    int copy_grids(double *d1, const double *s1, double *d2, const double *s2, size_t bytes)
+    {
+        VG_HIP(hipMemcpyAsync(d1, s1, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        VG_HIP(hipMemcpyAsync(d2, s2, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        return VG_OK;
+    }
+    int copy_carried(double *d1, const double *s1, double *d2, const double *s2)
+    {
+        const size_t pb = sizeof(double) * plane;
+        for (int k = 0; k < 2; k++) {
+            double *d = k ? d2 : d1;
+            const double *src = k ? s2 : s1;
+            VG_HIP(hipMemcpyAsync(d, src, 3 * pb, hipMemcpyDeviceToDevice, ctx->stream));
+            VG_HIP(hipMemcpyAsync(d + 4 * plane, src + 4 * plane, pb, hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        return VG_OK;
+    }
+
     // One step; with fixed_dts the time step is fixed_dts[it] (the clock's
     // step counter) instead of the CFL formula.
     template <int MODE>
@@ -269,10 +289,11 @@ struct RGFMGrid {
         clock_check_kernel<<<1, 1, 0, s>>>(clk);
         LineArgs la = line_args(riemann, p1, p2);
         const size_t gbytes = sizeof(double) * 6 * plane;
+        // with the 4-field sweeps only rho, u, v and p are copied: the caller
+        // has set the other two planes of the destinations
         auto snapshot = [&](double *const (&dst)[2]) {
-            VG_HIP(hipMemcpyAsync(dst[0], m1[cur], gbytes, hipMemcpyDeviceToDevice, s));
-            VG_HIP(hipMemcpyAsync(dst[1], m2[cur], gbytes, hipMemcpyDeviceToDevice, s));
-            return VG_OK;
+            return reduced ? copy_carried(dst[0], m1[cur], dst[1], m2[cur])
+                           : copy_grids(dst[0], m1[cur], dst[1], m2[cur], gbytes);
         };
         int rc = ctx->timed("rgfm_fill_x", [&] { rgfm_fill_kernel<0><<<ny, kLineBS, 0, s>>>(la); });
         if (!rc && keep) rc = snapshot(keep->P1);
@@ -675,6 +696,11 @@ int adjoint(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, cons
     vg::RGFMGrid g;
     rc = g.create(ctx, dim, ny, nx);
     if (!rc) rc = g.load(m1_in, m2_in, phi_in);
+    // the forward takes the 4-field sweeps when both materials hold w = +0 and
+    // lambda = 1 in every cell (its run decides the same way)
+    if (!rc) rc = g.choose_physics();
+    const bool planar_fwd = g.reduced;
+    g.reduced = false;
     if (rc) {
         g.destroy();
         return rc;
@@ -691,16 +717,17 @@ int adjoint(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, cons
             K = std::max(1, std::min(K, (int)(avail / per)));
         }
     }
-    // workspace: 0 dts, 2 ghost adjoints, 3-6 adjoint ping-pong (A[b][q]),
-    // 7-10 reverse temporaries (T1, T2 per material), then K kept steps
-    // (kKept buffers each) and the checkpoints (m1, m2, phi)
+    // workspace: 0 dts, 1 flags, 2 ghost adjoints, 3-6 adjoint ping-pong
+    // (A[b][q]), 7-10 reverse temporaries (T1, T2 per material), 11 the fill
+    // adjoints' VJP scratch, then K kept steps (kKept buffers each) and the
+    // checkpoints (m1, m2, phi)
     auto wsp = [&](size_t i, size_t bytes, double **p) {
         void *v = nullptr;
         int r = ctx->workspace_buffer(i, bytes, &v);
         *p = static_cast<double *>(v);
         return r;
     };
-    const size_t st0 = 11, ck0 = st0 + kKept * (size_t)K;
+    const size_t st0 = 12, ck0 = st0 + kKept * (size_t)K;
     // ---- forward with checkpoints ---------------------------------------------------
     std::vector<double> dts;
     std::vector<double *> ck;   // 3 per checkpoint
@@ -756,16 +783,18 @@ int adjoint(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, cons
         }
     }
     // ---- buffers of the reverse pass --------------------------------------------------
-    double *dts_dev = nullptr, *ghost = nullptr, *A[2][2], *T1[2], *T2[2], *flag_buf = nullptr;
+    double *dts_dev = nullptr, *ghost = nullptr, *A[2][2], *T1[2], *T2[2], *flag_buf = nullptr, *vjp = nullptr;
     if (!rc) rc = wsp(0, sizeof(double) * std::max(steps, 1), &dts_dev);
     if (!rc) rc = wsp(1, sizeof(double), &flag_buf);
     int *bad_step = reinterpret_cast<int *>(flag_buf);   // latest reversed step after which the adjoint is non-finite
+    int *seed_flag = bad_step + 1;                       // the seeds' w and lambda parts are zero
     if (!rc) VG_HIP(hipMemsetAsync(bad_step, 0xFF, sizeof(int), s));
     if (!rc) rc = wsp(2, sizeof(double) * 4 * 6 * (size_t)std::max(nx, ny), &ghost);
     for (int b = 0; b < 2 && !rc; b++)
         for (int q = 0; q < 2 && !rc; q++) rc = wsp(3 + 2 * b + q, gb, &A[b][q]);
     for (int q = 0; q < 2 && !rc; q++) rc = wsp(7 + q, gb, &T1[q]);
     for (int q = 0; q < 2 && !rc; q++) rc = wsp(9 + q, gb, &T2[q]);
+    if (!rc) rc = wsp(11, 2 * gb, &vjp);
     // kept step k: sts[3k .. 3k+2] = starting m1, m2, phi; ins[k] = its inside
     std::vector<double *> sts(3 * (size_t)K, nullptr);
     std::vector<vg::StepInternals> ins((size_t)K);
@@ -789,16 +818,61 @@ int adjoint(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, cons
     VG_HIP(hipStreamSynchronize(s));
     VG_HIP(hipMemcpyAsync(g.staging, seed2, gb / g.plane * g.cells, hipMemcpyHostToDevice, s));
     vg::aos_to_soa_kernel<6><<<vg::RGFMGrid::blocks_for(g.cells), 256, 0, s>>>(g.staging, A[0][1], g.plane, g.cells);
+    // The 4-field adjoint sweeps (AdjPlanar) when the forward took the 4-field
+    // sweeps, E_f = 0 for both materials and both seeds have zero w and lambda
+    // parts: the w and lambda adjoints then stay zero (the ghost fill's VJP
+    // gives them exact zeros: the interface values do not depend on lambda,
+    // and on w only through squares at w = +0).  The recomputation also takes
+    // the 4-field sweeps and copies four planes; the kept buffers get planes 3
+    // and 5 once, from the first checkpoint.
+    bool planar = false;
+    if (planar_fwd && q1.e_form == 0.0 && q2.e_form == 0.0) {
+        const int one = 1;
+        int ok = 0;
+        VG_HIP(hipMemcpyAsync(seed_flag, &one, sizeof(int), hipMemcpyHostToDevice, s));
+        for (int q = 0; q < 2; q++)
+            vg::zero_planes_kernel<<<vg::RGFMGrid::blocks_for(g.cells), 256, 0, s>>>(A[0][q], g.plane, g.cells,
+                                                                                    seed_flag);
+        rc = vg::launch_status("adjoint seed check");
+        if (!rc) VG_HIP(hipMemcpyAsync(&ok, seed_flag, sizeof(int), hipMemcpyDeviceToHost, s));
+        if (!rc) VG_HIP(hipStreamSynchronize(s));
+        planar = !rc && ok;
+    }
+    if (planar) {
+        for (int q = 0; q < 2; q++)
+            for (int f = 3; f <= 5; f += 2)
+                VG_HIP(hipMemsetAsync(A[1][q] + f * g.plane, 0, sizeof(double) * g.cells, s));
+        for (int k = 0; k < K; k++) {
+            double *dst[6] = {sts[3 * k], sts[3 * k + 1], ins[k].P1[0], ins[k].P1[1], ins[k].P2[0], ins[k].P2[1]};
+            for (int j = 0; j < (dim == 2 ? 6 : 4); j++)
+                for (int f = 3; f <= 5; f += 2)
+                    VG_HIP(hipMemcpyAsync(dst[j] + f * g.plane, ck[j % 2] + f * g.plane, sizeof(double) * g.cells,
+                                          hipMemcpyDeviceToDevice, s));
+        }
+    }
+    auto sweep_adj = [&](int dir, int variant, const double *w, const double *wbar_out, double *wbar_in, double h,
+                         double ts, const vg::EP &m, const double *w35) {
+        if (planar)
+            return vg::adj_sweep<vg::AdjPlanar>(ctx, dir, variant, w, wbar_out, wbar_in, ghost, g.plane, ny, nx,
+                                                VG_BC_TRANSMISSIVE, h, bias, limiter, ts, m, w35);
+        return vg::adj_sweep<vg::AdjFull>(ctx, dir, variant, w, wbar_out, wbar_in, ghost, g.plane, ny, nx,
+                                          VG_BC_TRANSMISSIVE, h, bias, limiter, ts, m);
+    };
     int xv = vg::kAdjDefault[0], yv = vg::kAdjDefault[1];
-    if (steps > 0) {
+    if (steps > 0 && !rc) {
         const vg::AdjointArgs ta =
             vg::adjoint_args(ck[0], A[0][0], A[1][0], ghost, g.plane, ny, nx, VG_BC_TRANSMISSIVE, dx, bias, limiter,
                              dts[0], q1);
-        rc = vg::tune_adjoint(ctx, 0, ta, xv);
-        if (!rc && dim == 2) rc = vg::tune_adjoint(ctx, 1, ta, yv);
+        if (planar) {
+            rc = vg::tune_adjoint<vg::AdjPlanar>(ctx, 0, ta, xv);
+            if (!rc && dim == 2) rc = vg::tune_adjoint<vg::AdjPlanar>(ctx, 1, ta, yv);
+        } else {
+            rc = vg::tune_adjoint(ctx, 0, ta, xv);
+            if (!rc && dim == 2) rc = vg::tune_adjoint(ctx, 1, ta, yv);
+        }
     }
     const vg::EP *qs[2] = {&q1, &q2};
-    const int cw = 0;
+    int cw = 0;   // the adjoint of the state after the step being reversed is in A[cw]
     const int nseg = (int)ck.size() / 3;
     for (int seg = nseg - 1; seg >= 0 && !rc; seg--) {
         const int s0 = seg * K, s1 = std::min(steps, s0 + K);
@@ -814,14 +888,17 @@ int adjoint(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, cons
         h.t_final = INFINITY;
         h.max_steps = s1;
         VG_HIP(hipMemcpyAsync(g.clk, &h, sizeof(h), hipMemcpyHostToDevice, s));
+        g.reduced = planar;
         for (int st = s0; st < s1 && !rc; st++) {
             const int k = st - s0;
-            VG_HIP(hipMemcpyAsync(sts[3 * k], g.m1[g.cur], gb, hipMemcpyDeviceToDevice, s));
-            VG_HIP(hipMemcpyAsync(sts[3 * k + 1], g.m2[g.cur], gb, hipMemcpyDeviceToDevice, s));
+            rc = planar ? g.copy_carried(sts[3 * k], g.m1[g.cur], sts[3 * k + 1], g.m2[g.cur])
+                        : g.copy_grids(sts[3 * k], g.m1[g.cur], sts[3 * k + 1], g.m2[g.cur], gb);
             VG_HIP(hipMemcpyAsync(sts[3 * k + 2], g.phi[g.pcur], pb, hipMemcpyDeviceToDevice, s));
-            rc = g.enqueue_step<vg::MODE_EXACT>(VG_RIEMANN_HLLC, dx, dy, bias, limiter, q1, q2, nullptr, 0, dts_dev, s1,
-                                                &ins[k]);
+            if (!rc)
+                rc = g.enqueue_step<vg::MODE_EXACT>(VG_RIEMANN_HLLC, dx, dy, bias, limiter, q1, q2, nullptr, 0, dts_dev,
+                                                    s1, &ins[k]);
         }
+        g.reduced = false;
         // reverse the steps of the segment
         for (int st = s1 - 1; st >= s0 && !rc; st--) {
             const int k = st - s0;
@@ -830,55 +907,43 @@ int adjoint(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, cons
             const double dt = dts[st];
             double *const *P1 = ins[k].P1, *const *Q1 = ins[k].Q1, *const *Q2 = ins[k].Q2;
             double *post[2] = {dim == 2 ? ins[k].P2[0] : P1[0], dim == 2 ? ins[k].P2[1] : P1[1]};
-            // reverse: sweeps of each material
+            // reverse: sweeps of each material, A[cw] -> A[1 - cw]
             for (int q = 0; q < 2 && !rc; q++) {
+                const double *w35 = ck[q];   // w = +0, lambda = 1 (4-field sweeps only)
                 if (dim == 1) {
-                    rc = vg::adj_sweep(ctx, 0, xv, post[q], A[cw][q], A[1 - cw][q], ghost, g.plane, ny, nx,
-                                       VG_BC_TRANSMISSIVE, dx, bias, limiter, dt, *qs[q]);
+                    rc = sweep_adj(0, xv, post[q], A[cw][q], A[1 - cw][q], dx, dt, *qs[q], w35);
                 } else {
-                    rc = vg::adj_sweep(ctx, 0, xv, Q2[q], A[cw][q], T1[q], ghost, g.plane, ny, nx, VG_BC_TRANSMISSIVE,
-                                       dx, bias, limiter, 0.5 * dt, *qs[q]);
-                    if (!rc)
-                        rc = vg::adj_sweep(ctx, 1, yv, Q1[q], T1[q], T2[q], ghost, g.plane, ny, nx,
-                                           VG_BC_TRANSMISSIVE, dy, bias, limiter, dt, *qs[q]);
-                    if (!rc)
-                        rc = vg::adj_sweep(ctx, 0, xv, post[q], T2[q], A[1 - cw][q], ghost, g.plane, ny, nx,
-                                           VG_BC_TRANSMISSIVE, dx, bias, limiter, 0.5 * dt, *qs[q]);
+                    rc = sweep_adj(0, xv, Q2[q], A[cw][q], T1[q], dx, 0.5 * dt, *qs[q], w35);
+                    if (!rc) rc = sweep_adj(1, yv, Q1[q], T1[q], T2[q], dy, dt, *qs[q], w35);
+                    if (!rc) rc = sweep_adj(0, xv, post[q], T2[q], A[1 - cw][q], dx, 0.5 * dt, *qs[q], w35);
                 }
             }
             if (rc) break;
-            // reverse: ghost fills (the y pass read P1, the x pass read S)
+            // reverse: ghost fills (the y pass read P1, the x pass read S), in place
             vg::FillAdjArgs fa;
             fa.phi = phs;
+            fa.adj1 = A[1 - cw][0];
+            fa.adj2 = A[1 - cw][1];
+            fa.vjp = vjp;
             fa.plane = g.plane;
             fa.ny = ny;
             fa.nx = nx;
             fa.init_flag = (dim == 2) ? 1 : 0;
             fa.p1 = q1;
             fa.p2 = q2;
-            const double *xin[2] = {A[1 - cw][0], A[1 - cw][1]};
             if (dim == 2) {
                 fa.m1 = P1[0];
                 fa.m2 = P1[1];
-                fa.bar1 = A[1 - cw][0];
-                fa.bar2 = A[1 - cw][1];
-                fa.out1 = T1[0];
-                fa.out2 = T1[1];
                 rc = ctx->timed("rgfm_fill_adjoint",
                                 [&] { vg::rgfm_fill_adjoint_kernel<1><<<nx, vg::kLineBS, 0, s>>>(fa); });
-                xin[0] = T1[0];
-                xin[1] = T1[1];
             }
             if (!rc) {
                 fa.m1 = S[0];
                 fa.m2 = S[1];
-                fa.bar1 = xin[0];
-                fa.bar2 = xin[1];
-                fa.out1 = A[cw][0];
-                fa.out2 = A[cw][1];
                 rc = ctx->timed("rgfm_fill_adjoint",
                                 [&] { vg::rgfm_fill_adjoint_kernel<0><<<ny, vg::kLineBS, 0, s>>>(fa); });
             }
+            cw = 1 - cw;
             for (int q = 0; q < 2 && !rc; q++) {
                 vg::adjoint_nonfinite_kernel<<<vg::RGFMGrid::blocks_for(g.cells), 256, 0, s>>>(A[cw][q], g.plane,
                                                                                               g.cells, st, bad_step);
