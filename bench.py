@@ -542,11 +542,24 @@ def circulation_seed(m1, phi, dx, dy):
     return s
 
 
+def admissible(m1, m2, phi, p1, p2):
+    """The solvers' final-state check (every cell of the material the level
+    set selects finite, rho > 0, p + p_inf > 0)."""
+    one = phi < 0.0
+    w = np.where(one[..., None], m1, m2)
+    pinf = np.where(one, p1[1], p2[1])   # (gamma, p_inf, ...)
+    return bool(np.isfinite(phi).all() and np.isfinite(w).all() and (w[..., 0] > 0.0).all()
+                and (w[..., 4] + pinf > 0.0).all())
+
+
 def bench_rgfm_adjoint(args):
-    """dGamma/dM per shock-bubble instance: plain forward (for Gamma and its
-    seed), then vg_rgfm_adjoint_2d; value = adjoint wall (its own forward,
-    recomputation and reverse sweeps included) / plain forward wall, summed
-    over the instances."""
+    """dGamma/dM of shock-bubble instances, batched: their plain forwards
+    through vg_rgfm_solve_batch_2d (timed), circulation seeds from the final
+    states, then their adjoints through vg_rgfm_adjoint_batch_2d (timed: its
+    own forwards, the recomputation and the reverse sweeps), both on the same
+    worker streams; value = adjoint batch wall / forward batch wall over the
+    instances whose forward ends admissible (the others lose positivity, a
+    state the oracle reproduces bit for bit, and have no derivative)."""
     import torch
     from vanguard_amd import problems
     from vanguard_amd import solvers as S
@@ -554,64 +567,71 @@ def bench_rgfm_adjoint(args):
     nx, ny = args.rgfm_nx, args.rgfm_ny
     radii, machs = problems.shock_bubble_batch(16, seed=6)
     ctx = S.default_context()
-    fwd_total = adj_total = 0.0
-    per = []
-    for q in args.rgfm_instances:
-        r, M = float(radii[q]), float(machs[q])
-        m1, m2, phi, dx, dy = problems.shock_bubble_2d(nx, ny, radius=r, mach=M)
-        sysm = S.MultimaterialSystem(m1, m2, phi)
-        if args.warmup:
-            S.rgfm_solve(sysm, dx, 0.8, args.t_final, problems.HELIUM, problems.AIR, dy=dy, max_steps=args.warmup)
+    ctx.set_batch_streams(args.batch_streams)
+    qs = list(args.rgfm_instances)
+    cases = [problems.shock_bubble_2d(nx, ny, radius=float(radii[q]), mach=float(machs[q])) for q in qs]
+    dx, dy = cases[0][3], cases[0][4]
+    m1 = np.stack([c[0] for c in cases])
+    m2 = np.stack([c[1] for c in cases])
+    phi = np.stack([c[2] for c in cases])
+    H, A = problems.HELIUM, problems.AIR
+
+    def forward(sel):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        f = S.rgfm_solve(sysm, dx, 0.8, args.t_final, problems.HELIUM, problems.AIR, dy=dy, raise_on_error=False)
-        fwd = time.perf_counter() - t0
-        if f.stats.nonfinite_cells > 0:
-            # the method loses positivity on this instance at this resolution
-            # (a state the oracle reproduces bit for bit): no derivative to take
-            log(f"instance {q}: M={M:.4f} steps={f.stats.steps} forward {fwd:.2f} s: "
-                f"{f.stats.nonfinite_cells} inadmissible cells, no adjoint")
-            per.append({"instance": q, "mach": M, "radius": r, "steps": int(f.stats.steps),
-                        "forward_s": round(fwd, 3), "adjoint_s": None, "circulation": None,
-                        "dcirculation_dmach": None, "finite": False})
+        f = S.rgfm_solve_batch_2d(m1[sel], m2[sel], phi[sel], dx, dy, 0.8, args.t_final, H, A, raise_on_error=False)
+        return f, time.perf_counter() - t0
+
+    if args.warmup:
+        S.rgfm_solve_batch_2d(m1, m2, phi, dx, dy, 0.8, args.t_final, H, A, max_steps=args.warmup,
+                              raise_on_error=False)
+    f, fwd = forward(list(range(len(qs))))
+    ok = [admissible(f.material1[i], f.material2[i], f.level_set[i], H, A) for i in range(len(qs))]
+    per = []
+    for i, q in enumerate(qs):
+        per.append({"instance": q, "mach": float(machs[q]), "radius": float(radii[q]), "steps": int(f.steps[i]),
+                    "circulation": float(f.circulation[i]) if ok[i] else None, "admissible": ok[i]})
+        if not ok[i]:
+            log(f"instance {q}: M={machs[q]:.4f} steps={f.steps[i]}: the forward ends inadmissible, no derivative")
+    keep = [i for i in range(len(qs)) if ok[i]]
+    if len(keep) < len(qs):
+        f, fwd = forward(keep)   # the forward batch of the instances with a derivative
+    log(f"forward batch of {len(keep)} instances: {fwd:.2f} s, {int(np.sum(f.steps))} steps")
+    seed1 = np.stack([circulation_seed(f.material1[j], f.level_set[j], dx, dy) for j in range(len(keep))])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    a = S.rgfm_adjoint_batch_2d(m1[keep], m2[keep], phi[keep], seed1, np.zeros_like(seed1), dx, dy, 0.8,
+                                args.t_final, H, A, checkpoint_every=args.checkpoint_every, raise_on_error=False)
+    adj = time.perf_counter() - t0
+    log(f"adjoint batch: {adj:.2f} s, ratio {adj / fwd:.3f}")
+    h = 1e-6
+    for j, i in enumerate(keep):
+        q = qs[i]
+        rec = per[i]
+        if a.status[j] != 0:
+            rec.update({"dcirculation_dmach": None, "finite": False,
+                        "adjoint_error": "the adjoint overflows (the pathwise derivative exceeds the double range)"})
+            log(f"instance {q}: M={machs[q]:.4f}: adjoint status {int(a.status[j])}, no finite derivative")
             continue
-        seed1 = circulation_seed(f.system.material1, f.system.level_set, dx, dy)
-        t0 = time.perf_counter()
-        a = S.rgfm_adjoint(sysm, seed1, np.zeros_like(m2), dx, 0.8, args.t_final, problems.HELIUM, problems.AIR,
-                           dy=dy, checkpoint_every=args.checkpoint_every, raise_on_error=False)
-        adj = time.perf_counter() - t0
-        if a.status != 0:
-            # the adjoint overflowed (the forward is locally unstable at the
-            # interface, so the pathwise derivative exceeds the double range):
-            # reported with the step whose reversal first overflowed
-            why = ctx.lib.vg_last_error().decode()
-            log(f"instance {q}: M={M:.4f} steps={f.stats.steps} forward {fwd:.2f} s, adjoint {adj:.2f} s: {why}")
-            per.append({"instance": q, "mach": M, "radius": r, "steps": int(f.stats.steps),
-                        "forward_s": round(fwd, 3), "adjoint_s": None, "circulation": f.circulation,
-                        "dcirculation_dmach": None, "finite": False, "adjoint_error": why})
-            continue
-        h = 1e-6
-        pa = problems.shock_bubble_2d(nx, ny, radius=r, mach=M + h)
-        pb = problems.shock_bubble_2d(nx, ny, radius=r, mach=M - h)
+        pa = problems.shock_bubble_2d(nx, ny, radius=float(radii[q]), mach=float(machs[q]) + h)
+        pb = problems.shock_bubble_2d(nx, ny, radius=float(radii[q]), mach=float(machs[q]) - h)
         dm2 = (pa[1] - pb[1]) / (2 * h)   # d(initial air state)/dM (helium does not depend on M)
-        dgdm = float(np.sum(a.grad2 * dm2))
-        finite = bool(np.isfinite(a.grad1).all() and np.isfinite(a.grad2).all())
-        log(f"instance {q}: M={M:.4f} steps={f.stats.steps} forward {fwd:.2f} s, adjoint {adj:.2f} s, "
-            f"Gamma={f.circulation:.6g}, dGamma/dM={dgdm:.6g}, finite={finite}")
-        fwd_total += fwd
-        adj_total += adj
-        per.append({"instance": q, "mach": M, "radius": r, "steps": int(f.stats.steps), "forward_s": round(fwd, 3),
-                    "adjoint_s": round(adj, 3), "circulation": f.circulation, "dcirculation_dmach": dgdm,
-                    "finite": finite})
+        dgdm = float(np.sum(a.grad2[j] * dm2))
+        rec.update({"dcirculation_dmach": dgdm, "finite": bool(np.isfinite(a.grad1[j]).all()
+                                                                 and np.isfinite(a.grad2[j]).all()),
+                    "adjoint_steps": int(a.steps[j])})
+        log(f"instance {q}: M={machs[q]:.4f} steps={a.steps[j]} Gamma={rec['circulation']:.6g} dGamma/dM={dgdm:.6g}")
+    n_steps = int(np.sum(f.steps))
     line = {
-        "metric": "adjoint/forward cost FP64 2D RGFM shock-bubble d circulation / d Mach",
-        "value": round(adj_total / fwd_total, 3), "unit": "x", "n_gpus": 1,
-        "steps": int(sum(p["steps"] for p in per if p["adjoint_s"] is not None)), "warmup": args.warmup,
-        "ms_per_step": round(adj_total * 1e3 / max(1, sum(p["steps"] for p in per if p["adjoint_s"] is not None)), 4),
-        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "metric": "adjoint/forward cost FP64 2D RGFM shock-bubble d circulation / d Mach (batched)",
+        "value": round(adj / fwd, 3), "unit": "x", "n_gpus": 1, "steps": n_steps, "warmup": args.warmup,
+        "ms_per_step": round(adj * 1e3 / max(1, n_steps), 4), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"rgfm_shock_bubble_{nx}x{ny}_hllc_t{args.t_final:g}_dgamma_dmach",
-                   "instances": list(args.rgfm_instances), "checkpoint_every": int(args.checkpoint_every) or "auto"},
-        "instances": per, "target": 3.0, "cpu_baseline": None,
+                   "instances": qs, "checkpoint_every": int(args.checkpoint_every) or "auto",
+                   "batch_streams": args.batch_streams},
+        "forward_batch_s": round(fwd, 3), "adjoint_batch_s": round(adj, 3), "instances": per, "target": 3.0,
+        "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
 

@@ -448,6 +448,24 @@ int vg_rgfm_adjoint_2d(vg_ctx *ctx, const double *m1_in, const double *m2_in, co
                        double *grad2, int ny, int nx, double dx, double dy, double cfl, double t_final, int max_steps,
                        double bias, int limiter, const vg_euler_params *p1, const vg_euler_params *p2,
                        int checkpoint_every, double *dt_log, int dt_log_cap, vg_run_stats *stats);
+/* The adjoints of a batch of independent 2D instances (arrays concatenated
+ * instance by instance, as vg_rgfm_solve_batch_2d; each with its own final
+ * time and seeds), run like vg_rgfm_adjoint_2d on the context's worker streams
+ * (vg_ctx_set_batch_streams), longest predicted run first; each worker gets an
+ * equal share of the free device memory for its checkpoints, with fewer workers
+ * than streams when the longest predicted run would not fit in a share.  The
+ * workers' device memory is released before the call returns.  Results are the
+ * same bits as one vg_rgfm_adjoint_2d call per instance.  steps (may be NULL):
+ * each instance's step count; status (may be NULL): each instance's status.
+ * An instance without a derivative (an inadmissible forward, an overflowing
+ * adjoint) gets VG_ERR_NONFINITE in its status while the others run on; the
+ * call then returns VG_ERR_NONFINITE.  No reference counterpart. */
+int vg_rgfm_adjoint_batch_2d(vg_ctx *ctx, int n_inst, const double *m1_in, const double *m2_in, const double *phi_in,
+                             double *m1_out, double *m2_out, double *phi_out, const double *seed1,
+                             const double *seed2, double *grad1, double *grad2, int ny, int nx, double dx, double dy,
+                             double cfl, const double *t_final, int max_steps, double bias, int limiter,
+                             const vg_euler_params *p1, const vg_euler_params *p2, int checkpoint_every,
+                             int32_t *steps, int32_t *status, vg_run_stats *stats);
 /* The HLLC RGFM solve with prescribed time steps dts[0..n_steps) (EXACT
  * arithmetic): the map the adjoint differentiates, for finite-difference checks. */
 int vg_rgfm_solve_fixed_dt_1d(vg_ctx *ctx, const double *m1_in, const double *m2_in, const double *phi_in,

@@ -199,6 +199,41 @@ def test_four_field_dot_product_against_dual_numbers(dim, tie):
     assert abs(lhs - rhs) <= 1e-9 * scale, (lhs, rhs, scale)
 
 
+@pytest.mark.parametrize("streams", [1, 3])
+def test_batch_equals_one_call_per_instance(streams):
+    """vg_rgfm_adjoint_batch_2d on worker streams against vg_rgfm_adjoint_2d per
+    instance, bit for bit: four shock-bubble instances of different Mach
+    numbers and radii to their own final times, planar seeds (4-field
+    adjoint) for three and a full seed (6-field) for one, plus an instance
+    whose seed overflows, reported in its status alone."""
+    machs, radii, tfs = [1.22, 1.3, 1.15, 1.25, 1.2], [0.025, 0.02, 0.03, 0.022, 0.025], [2e-5, 3e-5, 1.5e-5,
+                                                                                      2.5e-5, 1e-5]
+    cases = [problems.shock_bubble_2d(48, 24, radius=r, mach=M) for r, M in zip(radii, machs)]
+    dx, dy = cases[0][3], cases[0][4]
+    m1 = np.stack([c[0] for c in cases])
+    m2 = np.stack([c[1] for c in cases])
+    phi = np.stack([c[2] for c in cases])
+    rng = np.random.default_rng(95)
+    s1, s2 = rng.standard_normal(m1.shape) / SCALE, rng.standard_normal(m2.shape) / SCALE
+    s1[:3, ..., 3] = s1[:3, ..., 5] = s2[:3, ..., 3] = s2[:3, ..., 5] = 0.0
+    s1[4] = 1e308
+    ctx = S.Context(0)
+    ctx.set_batch_streams(streams)
+    b = S.rgfm_adjoint_batch_2d(m1, m2, phi, s1, s2, dx, dy, 0.8, np.array(tfs), HELIUM, AIR, ctx=ctx,
+                                raise_on_error=False)
+    assert b.rc == S._lib.VG_ERR_NONFINITE
+    assert list(b.status) == [0, 0, 0, 0, S._lib.VG_ERR_NONFINITE]
+    for q in range(4):
+        one = S.rgfm_adjoint(S.MultimaterialSystem(m1[q], m2[q], phi[q]), s1[q], s2[q], dx, 0.8, tfs[q], HELIUM, AIR,
+                             dy=dy)
+        assert b.steps[q] == one.dt.size
+        O.assert_bitwise(b.material1[q], one.system.material1, f"instance {q}: forward 1")
+        O.assert_bitwise(b.material2[q], one.system.material2, f"instance {q}: forward 2")
+        O.assert_bitwise(b.level_set[q], one.system.level_set, f"instance {q}: level set")
+        O.assert_bitwise(b.grad1[q], one.grad1, f"instance {q}: gradient 1")
+        O.assert_bitwise(b.grad2[q], one.grad2, f"instance {q}: gradient 2")
+
+
 def test_adjoint_to_final_time_on_the_benchmark_geometry():
     """A short run to t_final (last step cut) of the shock-bubble problem at a
     small size: finite gradients and the dot-product test."""

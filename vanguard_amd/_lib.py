@@ -188,6 +188,11 @@ SIGNATURES = {
                                      C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double,
                                      C.c_int, C.POINTER(EulerParams), C.POINTER(EulerParams), C.c_int, dptr,
                                      C.c_int, C.POINTER(RunStats)]),
+    "vg_rgfm_adjoint_batch_2d": (C.c_int, [vptr, C.c_int, dptr, dptr, dptr, dptr, dptr, dptr, dptr, dptr, dptr,
+                                           dptr, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, dptr,
+                                           C.c_int, C.c_double, C.c_int, C.POINTER(EulerParams),
+                                           C.POINTER(EulerParams), C.c_int, C.POINTER(C.c_int32),
+                                           C.POINTER(C.c_int32), C.POINTER(RunStats)]),
     "vg_rgfm_solve_fixed_dt_1d": (C.c_int, [vptr, dptr, dptr, dptr, dptr, dptr, dptr, C.c_int, C.c_double, dptr,
                                             C.c_int, C.c_double, C.c_int, C.POINTER(EulerParams),
                                             C.POINTER(EulerParams)]),

@@ -679,7 +679,7 @@ int adjoint(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, cons
             double *m2_out, double *phi_out, const double *seed1, const double *seed2, double *grad1, double *grad2,
             int ny, int nx, double dx, double dy, double cfl, double t_final, int max_steps, double bias, int limiter,
             const vg_euler_params *p1, const vg_euler_params *p2, int checkpoint_every, double *dt_log, int dt_cap,
-            vg_run_stats *stats)
+            vg_run_stats *stats, double mem_budget = 0.0, int n_est = 0)
 {
     vg::clear_error();
     VG_CHECK(ctx != nullptr, "null context");
@@ -706,15 +706,24 @@ int adjoint(vg_ctx *ctx, int dim, const double *m1_in, const double *m2_in, cons
         return rc;
     }
     const size_t gb = sizeof(double) * 6 * g.plane, pb = sizeof(double) * g.cells;
+    const int n_run = max_steps > 0 ? max_steps : n_est;   // steps of the run, when known or predicted
     int K = checkpoint_every > 0 ? checkpoint_every
-                                 : (max_steps > 0 ? std::max(1, (int)std::ceil(std::sqrt((double)max_steps))) : 64);
+                                 : (n_run > 0 ? std::max(1, (int)std::ceil(std::sqrt((double)n_run))) : 64);
     constexpr int kKept = 11;   // per kept step: m1, m2, phi, P1 x2, P2 x2, Q1 x2, Q2 x2
     if (checkpoint_every == 0) {
-        size_t free_b = 0, total_b = 0;
-        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
-            const double per = 10.0 * (double)gb + (double)pb;
-            const double avail = 0.6 * (double)(free_b + ctx->workspace_bytes());
-            K = std::max(1, std::min(K, (int)(avail / per)));
+        const double per = 10.0 * (double)gb + (double)pb;
+        if (mem_budget > 0.0) {
+            // a share of the device (the batch): the kept steps and the
+            // checkpoints of a run of n_run steps within the budget
+            const double ckb = 2.0 * (double)gb + (double)pb;
+            auto need = [&](int k) { return (double)k * per + (n_run > 0 ? std::ceil((double)n_run / k) * ckb : 0.0); };
+            while (K > 1 && need(K) > mem_budget) K--;
+        } else {
+            size_t free_b = 0, total_b = 0;
+            if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+                const double avail = 0.6 * (double)(free_b + ctx->workspace_bytes());
+                K = std::max(1, std::min(K, (int)(avail / per)));
+            }
         }
     }
     // workspace: 0 dts, 1 flags, 2 ghost adjoints, 3-6 adjoint ping-pong
@@ -1194,6 +1203,174 @@ int vg_rgfm_adjoint_2d(vg_ctx *ctx, const double *m1_in, const double *m2_in, co
 {
     return adjoint(ctx, 2, m1_in, m2_in, phi_in, m1_out, m2_out, phi_out, seed1, seed2, grad1, grad2, ny, nx, dx, dy,
                    cfl, t_final, max_steps, bias, limiter, p1, p2, checkpoint_every, dt_log, dt_log_cap, stats);
+}
+
+int vg_rgfm_adjoint_batch_2d(vg_ctx *ctx, int n_inst, const double *m1_in, const double *m2_in, const double *phi_in,
+                             double *m1_out, double *m2_out, double *phi_out, const double *seed1,
+                             const double *seed2, double *grad1, double *grad2, int ny, int nx, double dx, double dy,
+                             double cfl, const double *t_final, int max_steps, double bias, int limiter,
+                             const vg_euler_params *p1, const vg_euler_params *p2, int checkpoint_every,
+                             int32_t *steps, int32_t *status, vg_run_stats *stats)
+{
+    vg::clear_error();
+    VG_CHECK(ctx != nullptr && n_inst >= 1 && t_final != nullptr, "null argument");
+    VG_CHECK(m1_in && m2_in && phi_in && m1_out && m2_out && phi_out && seed1 && seed2 && grad1 && grad2,
+             "null state pointer");
+    VG_CHECK(checkpoint_every >= 0, "checkpoint_every must be >= 0");
+    for (int q = 0; q < n_inst; q++) {
+        int rc = check_args(2, ny, nx, dx, dy, cfl, t_final[q], max_steps, limiter, VG_RIEMANN_HLLC, p1, p2,
+                            VG_MODE_EXACT);
+        if (rc) return rc;
+    }
+    int rc = vg::bind(ctx);
+    if (rc) return rc;
+    const size_t cells = (size_t)ny * nx, n6 = 6 * cells;
+    const vg::EP e1 = to_params(p1), e2 = to_params(p2);
+    // Predicted steps (t_final times the largest signal speed over the CFL
+    // spacing, with a margin for the start-up steps) order the instances,
+    // longest first, and size each one's checkpoints.
+    std::vector<std::pair<double, int>> order(n_inst);
+    std::vector<int> n_est(n_inst, 0);
+    for (int q = 0; q < n_inst; q++) {
+        double smax = 0.0;
+        const double *ms[2] = {m1_in + n6 * q, m2_in + n6 * q};
+        const vg_euler_params *ps[2] = {p1, p2};
+        for (int k = 0; k < 2; k++)
+            for (size_t c = 0; c < cells; c++) {
+                const double *w = ms[k] + 6 * c;
+                const double a2 = ps[k]->gamma * (w[4] + ps[k]->p_inf) / w[0];
+                if (a2 > 0.0 && std::isfinite(a2))
+                    smax = std::max(smax, std::max(std::fabs(w[1]), std::fabs(w[2])) + std::sqrt(a2));
+            }
+        const double n = 1.25 * t_final[q] * smax / (cfl * std::min(dx, dy)) + 16.0;
+        n_est[q] = (std::isfinite(n) && n < 1e8) ? (int)n : 0;
+        order[q] = {-(double)n_est[q], q};
+    }
+    std::stable_sort(order.begin(), order.end());
+    const int nw = std::min(n_inst, ctx->timing ? 1 : ctx->batch_streams);
+    // Tile shapes are tuned here, alone on the device, before the workers
+    // start (they take them from this context): the y sweeps of the forward,
+    // and the adjoint sweeps in both the 4- and 6-field forms.
+    if (ctx->autotune) {
+        vg::RGFMGrid g;
+        const int q = order[0].second;
+        double *buf = nullptr;
+        rc = g.create(ctx, 2, ny, nx);
+        if (!rc) rc = g.load(m1_in + n6 * q, m2_in + n6 * q, phi_in + cells * q);
+        if (!rc) rc = g.choose_physics();
+        if (!rc)
+            rc = g.reduced ? g.tune_as<vg::EulerPlanarPhys>(VG_MODE_EXACT, dx, dy, bias, limiter, e1)
+                           : g.tune_as<vg::EulerPhys>(VG_MODE_EXACT, dx, dy, bias, limiter, e1);
+        const size_t gb = sizeof(double) * 6 * g.plane;
+        if (!rc && hipMalloc(&buf, 2 * gb + sizeof(double) * 4 * 6 * std::max(nx, ny)) != hipSuccess) {
+            vg::set_error("RGFM adjoint batch: out of device memory for the tile tuning");
+            rc = VG_ERR_HIP;
+        }
+        if (!rc && hipMemsetAsync(buf, 0, gb, ctx->stream) != hipSuccess) rc = VG_ERR_HIP;
+        if (!rc) {
+            const vg::AdjointArgs ta = vg::adjoint_args(g.m1[g.cur], buf, buf + 6 * g.plane, buf + 12 * g.plane,
+                                                        g.plane, ny, nx, VG_BC_TRANSMISSIVE, dx, bias, limiter,
+                                                        1e-3 * std::min(dx, dy), e1);
+            int v = 0;
+            for (int dir = 0; dir < 2 && !rc; dir++) {
+                rc = vg::tune_adjoint<vg::AdjPlanar>(ctx, dir, ta, v);
+                if (!rc) rc = vg::tune_adjoint<vg::AdjFull>(ctx, dir, ta, v);
+            }
+        }
+        if (ctx->stream) hipStreamSynchronize(ctx->stream);
+        if (buf) hipFree(buf);
+        g.destroy();
+        if (rc) return rc;
+    }
+    rc = vg::batch_workers(ctx, nw);
+    if (rc) return rc;
+    // Each worker's share of the free device memory for its instance's
+    // checkpoints and kept steps; fewer workers when the longest predicted
+    // run would not fit in a share (an instance needs at least K kept steps
+    // and n / K checkpoints, least at K = sqrt(n checkpoint / kept step)).
+    for (int k = 0; k < (int)ctx->workers.size(); k++) ctx->workers[k]->release_workspace();
+    size_t free_b = 0, total_b = 0;
+    VG_HIP(hipMemGetInfo(&free_b, &total_b));
+    const double usable = 0.55 * (double)free_b;
+    int nw_mem = nw;
+    if (checkpoint_every == 0) {
+        const double plane = (double)(((long long)cells + 63) / 64 * 64), gb = 48.0 * plane, pb = 8.0 * cells;
+        const double kept = 10.0 * gb + pb, ckb = 2.0 * gb + pb;
+        const double n = std::max(1.0, (double)(max_steps > 0 ? max_steps : -order[0].first));
+        const double k_best = std::max(1.0, std::sqrt(n * ckb / kept));
+        const double least = k_best * kept + std::ceil(n / k_best) * ckb;
+        nw_mem = std::max(1, std::min(nw, (int)(usable / least)));
+    }
+    const int nwork = nw_mem;
+    const double budget = usable / nwork;
+    std::vector<vg_run_stats> per(n_inst);
+    std::vector<int> inst_rc(n_inst, VG_OK);
+    std::vector<std::string> inst_err(n_inst);
+    std::atomic<int> next{0};
+    std::atomic<bool> stop{false};
+    std::vector<int> wrc(nwork, VG_OK);
+    std::vector<std::string> werr(nwork);
+    // Each worker takes instances from the shared queue, longest first, and
+    // runs the single-instance adjoint on its own stream and workspace.
+    auto worker = [&](int k) {
+        vg_ctx *c = ctx->workers[k];
+        int r = vg::bind(c);
+        while (!r && !stop.load()) {
+            const int i = next.fetch_add(1);
+            if (i >= n_inst) break;
+            const int q = order[i].second;
+            const int s = adjoint(c, 2, m1_in + n6 * q, m2_in + n6 * q, phi_in + cells * q, m1_out + n6 * q,
+                                  m2_out + n6 * q, phi_out + cells * q, seed1 + n6 * q, seed2 + n6 * q,
+                                  grad1 + n6 * q, grad2 + n6 * q, ny, nx, dx, dy, cfl, t_final[q], max_steps, bias,
+                                  limiter, p1, p2, checkpoint_every, nullptr, 0, &per[q], budget, n_est[q]);
+            inst_rc[q] = s;
+            if (s) inst_err[q] = vg_last_error();
+            // an instance without a derivative is reported in its status
+            r = (s == VG_ERR_NONFINITE) ? VG_OK : s;
+        }
+        if (c->stream) hipStreamSynchronize(c->stream);
+        c->release_workspace();
+        if (r) {
+            stop = true;
+            werr[k] = vg_last_error();
+        }
+        wrc[k] = r;
+    };
+    {
+        std::vector<std::thread> threads;
+        for (int k = 0; k < nwork; k++) threads.emplace_back(worker, k);
+        for (auto &t : threads) t.join();
+    }
+    vg::merge_worker_tuning(ctx);
+    for (int k = 0; k < nwork && !rc; k++)
+        if (wrc[k]) {
+            rc = wrc[k];
+            vg::set_error(werr[k]);
+        }
+    vg_run_stats total{};
+    total.first_nonfinite = -1;
+    int n_bad = 0, first_bad = -1;
+    for (int q = 0; q < n_inst; q++) {
+        if (steps) steps[q] = per[q].steps;
+        if (status) status[q] = inst_rc[q];
+        total.steps += per[q].steps;
+        total.wall_ms += per[q].wall_ms;
+        if (inst_rc[q] == VG_ERR_NONFINITE) {
+            n_bad++;
+            if (first_bad < 0) first_bad = q;
+            total.nonfinite_cells += per[q].nonfinite_cells;
+            if (total.first_nonfinite < 0 && per[q].first_nonfinite >= 0)
+                total.first_nonfinite = (int64_t)cells * q + per[q].first_nonfinite;
+        }
+    }
+    total.total_steps = total.steps;
+    if (stats) *stats = total;
+    if (!rc && n_bad > 0) {
+        vg::set_error("RGFM adjoint batch: " + std::to_string(n_bad) + " instance(s) without a finite derivative; " +
+                      "first: instance " + std::to_string(first_bad) + ": " + inst_err[first_bad]);
+        return VG_ERR_NONFINITE;
+    }
+    return rc;
 }
 
 int vg_rgfm_solve_fixed_dt_1d(vg_ctx *ctx, const double *m1_in, const double *m2_in, const double *phi_in,

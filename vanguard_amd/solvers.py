@@ -1236,6 +1236,54 @@ def rgfm_adjoint(system: MultimaterialSystem, seed1: np.ndarray, seed2: np.ndarr
     return RGFMAdjointResult(MultimaterialSystem(o1, o2, oph), g1, g2, dts[:min(st.steps, dt_cap)].copy(), st, rc)
 
 
+@dataclass
+class RGFMAdjointBatchResult:
+    material1: np.ndarray         # final states of the forwards, (n_inst, ny, nx, 6)
+    material2: np.ndarray
+    level_set: np.ndarray         # (n_inst, ny, nx)
+    grad1: np.ndarray             # d loss / d initial material 1, per instance
+    grad2: np.ndarray
+    steps: np.ndarray             # step count per instance
+    status: np.ndarray            # per instance: VG_OK or VG_ERR_NONFINITE
+    stats: _lib.RunStats
+    rc: int
+
+
+def rgfm_adjoint_batch_2d(m1, m2, phi, seed1, seed2, dx, dy, cfl, t_final, params1, params2, *, bias=0.0, limiter=0,
+                          max_steps=0, checkpoint_every=0, ctx: Context | None = None,
+                          raise_on_error=True) -> RGFMAdjointBatchResult:
+    """The adjoints of n_inst instances (leading axis) through
+    vg_rgfm_adjoint_batch_2d: each instance as rgfm_adjoint, on the context's
+    worker streams.  An instance without a derivative has VG_ERR_NONFINITE in
+    `status`; the call's status is then VG_ERR_NONFINITE (raised unless
+    raise_on_error is False)."""
+    ctx = ctx or default_context()
+    a1 = np.ascontiguousarray(m1, dtype=np.float64)
+    a2 = np.ascontiguousarray(m2, dtype=np.float64)
+    ph = np.ascontiguousarray(phi, dtype=np.float64)
+    s1 = np.ascontiguousarray(seed1, dtype=np.float64)
+    s2 = np.ascontiguousarray(seed2, dtype=np.float64)
+    n_inst, ny, nx = ph.shape
+    if a1.shape != (n_inst, ny, nx, 6) or a2.shape != a1.shape or s1.shape != a1.shape or s2.shape != a1.shape:
+        raise ValueError("materials and seeds must be (n_inst, ny, nx, 6) over the level sets' shape")
+    tf = np.ascontiguousarray(np.broadcast_to(np.asarray(t_final, dtype=np.float64), (n_inst,)))
+    o1, o2, oph = np.empty_like(a1), np.empty_like(a2), np.empty_like(ph)
+    g1, g2 = np.empty_like(a1), np.empty_like(a2)
+    steps = np.zeros(n_inst, dtype=np.int32)
+    status = np.zeros(n_inst, dtype=np.int32)
+    st = _lib.RunStats()
+    q1, q2 = _params(params1).c_struct(), _params(params2).c_struct()
+    P = _lib.as_dptr
+    i32 = C.POINTER(C.c_int32)
+    rc = ctx.lib.vg_rgfm_adjoint_batch_2d(ctx.handle, n_inst, P(a1), P(a2), P(ph), P(o1), P(o2), P(oph), P(s1),
+                                          P(s2), P(g1), P(g2), ny, nx, dx, dy, cfl, P(tf), max_steps, bias, limiter,
+                                          C.byref(q1), C.byref(q2), checkpoint_every, steps.ctypes.data_as(i32),
+                                          status.ctypes.data_as(i32), C.byref(st))
+    if raise_on_error and rc != _lib.VG_OK:
+        check(rc, "vg_rgfm_adjoint_batch_2d")
+    return RGFMAdjointBatchResult(o1, o2, oph, g1, g2, steps, status, st, rc)
+
+
 def rgfm_solve_fixed_dt(system: MultimaterialSystem, dx: float, dts, params1, params2, *, dy: float | None = None,
                         bias: float = 0.0, limiter: int = 0, ctx: Context | None = None) -> MultimaterialSystem:
     """HLLC RGFM solve with prescribed time steps (vg_rgfm_solve_fixed_dt_1d / _2d)."""
