@@ -42,18 +42,37 @@ struct ForcingArgs {
 };
 
 constexpr int kForceBS = 128;
+#ifndef VG_FORCING_WAVES
+#define VG_FORCING_WAVES 2
+#endif
+
+// A neighbour's conserved vector in the workgroup's LDS tile (field f at
+// p[f * stride]), read where the fractional steps use it instead of being held
+// in registers for the whole sub-cycle.
+struct LdsNeighbour {
+    const double *p;
+    int stride;
+    __device__ __forceinline__ double operator[](int f) const { return p[f * stride]; }
+};
+// Before each fractional step: values read from the LDS tile are read again
+// (a compiler barrier, no instruction), so none stays live across steps.
+template <int NF>
+__device__ __forceinline__ void reload_point(const double (&)[NF])
+{
+}
+__device__ __forceinline__ void reload_point(const LdsNeighbour &) { asm volatile("" ::: "memory"); }
 
 // Solvers::evolveStateByFractionalX/YTimeStep (hprsolvers.cpp:606-620, 673-687;
 // mhdsolvers.cpp:612-626) on the conserved vectors of the three cells: slope,
 // half-slope extrapolations, flux difference times frac * (ts / dx) added to the
 // middle state, converted back to primitives (setConservedVariableVector).
-template <class P, int DIR>
-__device__ __forceinline__ void fractional_step(const typename P::Params &m, const double (&UL)[P::NF],
-                                                const double (&UM)[P::NF], const double (&UR)[P::NF], double bias,
-                                                int limiter, double c, double (&w)[P::NF])
+template <class P, int DIR, class NB>
+__device__ __forceinline__ void fractional_step(const typename P::Params &m, const NB &UL, const double (&UM)[P::NF],
+                                                const NB &UR, double bias, int limiter, double c, double (&w)[P::NF])
 {
     constexpr int NF = P::NF;
     static_assert(StoredFields<P>::n == NF, "this kernel reads and writes all NF primitive planes");
+    reload_point(UL);
     double xl[NF], xr[NF];
     const double cl = 0.5 * (1.0 + bias), cr = 0.5 * (1.0 - bias);
 #pragma unroll
@@ -96,11 +115,10 @@ struct NoStages {
 // the non-zero slots.  rec.mid(st, phase, UM) sees the middle state entering
 // each fractional step of stage st, rec.end(st, Ue) the state whose source gives
 // the next increment.
-template <class P, int DIM, class R>
-__device__ __forceinline__ void forcing_cell(const typename P::Params &m, const double (&UL)[P::NF],
-                                             const double (&UR)[P::NF], const double (&UT)[P::NF],
-                                             const double (&UB)[P::NF], const double (&Um)[P::NF], double ts,
-                                             double dx, double bias, int limiter, double (&N)[P::NF],
+template <class P, int DIM, class R, class NB>
+__device__ __forceinline__ void forcing_cell(const typename P::Params &m, const NB &UL, const NB &UR, const NB &UT,
+                                             const NB &UB, const double (&Um)[P::NF], double ts, double dx,
+                                             double bias, int limiter, double (&N)[P::NF],
                                              double (&K)[4][ForcingTraits<P>::kMaxSlots], R &rec)
 {
     constexpr int NF = P::NF;
@@ -176,30 +194,76 @@ __device__ __forceinline__ void forcing_neighbour(const typename P::Params &m, c
     P::prim_to_cons(m, w, U);
 }
 
+// Workgroup tile of the source stage: FTX x FTY cells (2D: rows of 32 cells,
+// 1D: 128 cells of the line).  The neighbours' conserved vectors of the tile
+// and its one-cell rim (corners excluded) are computed once into LDS, so a
+// cell's sub-cycle keeps only its own state in registers.
+template <int DIM>
+struct ForcingTile {
+    static constexpr int tx = (DIM == 2) ? 32 : kForceBS, ty = kForceBS / tx;
+    static constexpr int lx = tx + 2, ly = (DIM == 2) ? ty + 2 : 1, slots = lx * ly;
+};
+
+inline long long forcing_blocks(int dim, int ny, int nx)
+{
+    if (dim == 2) {
+        using T = ForcingTile<2>;
+        return (long long)((nx + T::tx - 1) / T::tx) * ((ny + T::ty - 1) / T::ty);
+    }
+    return ((long long)ny * nx + kForceBS - 1) / kForceBS;
+}
+
 template <class P, int DIM>
-__global__ __launch_bounds__(kForceBS) void forcing_kernel(ForcingArgs<P> a)
+__global__ __launch_bounds__(kForceBS, VG_FORCING_WAVES) void forcing_kernel(ForcingArgs<P> a)
 {
     constexpr int NF = P::NF;
     static_assert(StoredFields<P>::n == NF, "this kernel reads and writes all NF primitive planes");
     using T = ForcingTraits<P>;
-    constexpr int NS = T::kMaxSlots;
+    using Tile = ForcingTile<DIM>;
+    constexpr int NS = T::kMaxSlots, SL = Tile::slots;
+    __shared__ double s_nb[NF * SL];   // neighbour conserved vectors: field f of slot s at f * SL + s
     if (a.clk->done) return;
     typename P::Params m = a.m;
     if (!a.fixed_params) P::begin_step(m, a.clk->S);   // sub-cycles run in second-order steps only
     const double ts = 0.5 * (a.clk->dt / a.subcycles);
-    const long long cells = (long long)a.ny * a.nx;
-    const long long c = (long long)blockIdx.x * kForceBS + threadIdx.x;
+    const int tid = threadIdx.x;
+    int i, j, i0, j0;
+    if (DIM == 2) {
+        const int bx = (a.nx + Tile::tx - 1) / Tile::tx;
+        i0 = (blockIdx.x / bx) * Tile::ty;
+        j0 = (blockIdx.x % bx) * Tile::tx;
+        i = i0 + tid / Tile::tx;
+        j = j0 + tid % Tile::tx;
+    } else {
+        const long long c0 = (long long)blockIdx.x * kForceBS;
+        i0 = (int)(c0 / a.nx);
+        j0 = (int)(c0 - (long long)i0 * a.nx);
+        i = i0;
+        j = j0 + tid;   // 1D grids have one row (ny = 1)
+    }
+    // The tile's slots: slot (ly, lx) holds the cell at (i0 - 1 + ly, j0 - 1 + lx)
+    // (1D: (i0, j0 - 1 + lx)), positions outside the grid taking the boundary's
+    // ghost cell; slots no cell of the tile reads are skipped.
+    for (int sidx = tid; sidx < SL; sidx += kForceBS) {
+        const int sy = sidx / Tile::lx, sx = sidx - sy * Tile::lx;
+        const int gi = (DIM == 2) ? i0 - 1 + sy : i0, gj = j0 - 1 + sx;
+        const bool rim_x = (sx == 0 || sx == Tile::lx - 1), rim_y = (DIM == 2) && (sy == 0 || sy == Tile::ly - 1);
+        const bool need = !(rim_x && rim_y) && gi >= -1 && gi <= a.ny && gj >= -1 && gj <= a.nx;
+        if (!need) continue;
+        const long long idx = (long long)ghost_index(gi, a.ny, a.bc) * a.nx + ghost_index(gj, a.nx, a.bc);
+        double U[NF];
+        forcing_neighbour<P>(m, a.in, a.plane, idx, U);
+#pragma unroll
+        for (int f = 0; f < NF; f++) s_nb[f * SL + sidx] = U[f];
+    }
+    __syncthreads();
     unsigned long long key = 0;
-    if (c < cells) {
-        const int i = (int)(c / a.nx), j = (int)(c - (long long)i * a.nx);
-        const int jl = ghost_index(j - 1, a.nx, a.bc), jr = ghost_index(j + 1, a.nx, a.bc);
-        double UL[NF], UR[NF], UT[NF], UB[NF];
-        forcing_neighbour<P>(m, a.in, a.plane, (long long)i * a.nx + jl, UL);
-        forcing_neighbour<P>(m, a.in, a.plane, (long long)i * a.nx + jr, UR);
-        if (DIM == 2) {
-            forcing_neighbour<P>(m, a.in, a.plane, (long long)ghost_index(i - 1, a.ny, a.bc) * a.nx + j, UT);
-            forcing_neighbour<P>(m, a.in, a.plane, (long long)ghost_index(i + 1, a.ny, a.bc) * a.nx + j, UB);
-        }
+    if (i < a.ny && j < a.nx) {
+        const int sy = (DIM == 2) ? i - i0 + 1 : 0, sx = j - j0 + 1;
+        const int me = sy * Tile::lx + sx;
+        const LdsNeighbour UL{s_nb + me - 1, SL}, UR{s_nb + me + 1, SL};
+        const LdsNeighbour UT{s_nb + me - Tile::lx * (DIM == 2), SL}, UB{s_nb + me + Tile::lx * (DIM == 2), SL};
+        const long long c = (long long)i * a.nx + j;
         double w0[NF], Um[NF];
         load_prim<NF>(a.in, a.plane, c, w0);
         P::prim_to_cons(m, w0, Um);
@@ -223,7 +287,7 @@ struct ForcingStage {
     template <class GS>
     static long long blocks(const GS &g)
     {
-        return (g.cells + kForceBS - 1) / kForceBS;
+        return forcing_blocks(g.dim, g.ny, g.nx);
     }
     template <class GS>
     static int launch(GS &g, const double *in, double *out, int fuse, int bc, double dx, double bias, int limiter,

@@ -222,7 +222,10 @@ struct GridState {
         VG_HIP(hipMalloc(&clk, sizeof(Clock)));
         VG_HIP(hipMalloc(&scan, 2 * sizeof(unsigned long long)));
         const long long gx = (nx + kBSX - 4 - 1) / (kBSX - 4) + 1;
-        const long long slots = std::max<long long>(std::max<long long>(gx * ny, cells / 64 + 1), 4096);
+        // one per block of the largest producer: x sweep tiles, 64-cell wave-speed
+        // blocks, or the source stage's tiles of 32 x 4 cells
+        const long long tiles = (long long)((nx + 31) / 32) * ((ny + 3) / 4);
+        const long long slots = std::max<long long>(std::max<long long>(gx * ny, cells / 64 + 1), std::max(tiles, 4096LL));
         VG_HIP(hipMalloc(&partials, sizeof(unsigned long long) * slots));
         // stream-ordered copies only: a synchronous one would join the legacy
         // stream, which HIP refuses while another thread captures a graph

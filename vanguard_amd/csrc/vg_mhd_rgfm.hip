@@ -180,7 +180,7 @@ struct MHDRGFMGrid {
         a.clk = clk();
         a.partials = nullptr;
         a.fixed_params = 1;
-        const unsigned nb = (unsigned)((cells + kForceBS - 1) / kForceBS);
+        const unsigned nb = (unsigned)forcing_blocks(2, ny, nx);
         int rc = ctx->timed("mhd_rgfm_source_rk4",
                             [&] { forcing_kernel<MHDPhys, 2><<<nb, kForceBS, 0, ctx->stream>>>(a); });
         if (rc) return launch_status("mhd_rgfm_source");
