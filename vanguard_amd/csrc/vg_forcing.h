@@ -204,6 +204,69 @@ struct ForcingTile {
     static constexpr int lx = tx + 2, ly = (DIM == 2) ? ty + 2 : 1, slots = lx * ly;
 };
 
+// This workgroup's tile: its first row and column (i0, j0) and the thread's
+// cell (i, j).
+template <int DIM>
+__device__ __forceinline__ void forcing_tile_origin(int nx, int &i0, int &j0, int &i, int &j)
+{
+    using Tile = ForcingTile<DIM>;
+    const int tid = threadIdx.x;
+    if (DIM == 2) {
+        const int bx = (nx + Tile::tx - 1) / Tile::tx;
+        i0 = (blockIdx.x / bx) * Tile::ty;
+        j0 = (blockIdx.x % bx) * Tile::tx;
+        i = i0 + tid / Tile::tx;
+        j = j0 + tid % Tile::tx;
+    } else {
+        const long long c0 = (long long)blockIdx.x * kForceBS;
+        i0 = (int)(c0 / nx);
+        j0 = (int)(c0 - (long long)i0 * nx);
+        i = i0;
+        j = j0 + tid;   // 1D grids have one row (ny = 1)
+    }
+}
+
+// Fills the tile's slots: slot (ly, lx) holds the conserved vector that
+// cell_vector(idx, U) gives for the cell at (i0 - 1 + ly, j0 - 1 + lx) (1D:
+// (i0, j0 - 1 + lx)), positions outside the grid taking the boundary's ghost
+// cell; slots no cell of the tile reads are skipped.  Ends with a barrier.
+template <int NF, int DIM, class F>
+__device__ __forceinline__ void fill_neighbour_tile(double *s_nb, int i0, int j0, int ny, int nx, int bc,
+                                                    const F &cell_vector)
+{
+    using Tile = ForcingTile<DIM>;
+    constexpr int SL = Tile::slots;
+    for (int sidx = threadIdx.x; sidx < SL; sidx += kForceBS) {
+        const int sy = sidx / Tile::lx, sx = sidx - sy * Tile::lx;
+        const int gi = (DIM == 2) ? i0 - 1 + sy : i0, gj = j0 - 1 + sx;
+        const bool rim_x = (sx == 0 || sx == Tile::lx - 1), rim_y = (DIM == 2) && (sy == 0 || sy == Tile::ly - 1);
+        const bool need = !(rim_x && rim_y) && gi >= -1 && gi <= ny && gj >= -1 && gj <= nx;
+        if (!need) continue;
+        const long long idx = (long long)ghost_index(gi, ny, bc) * nx + ghost_index(gj, nx, bc);
+        double U[NF];
+        cell_vector(idx, U);
+#pragma unroll
+        for (int f = 0; f < NF; f++) s_nb[f * SL + sidx] = U[f];
+    }
+    __syncthreads();
+}
+
+// The four neighbours of cell (i, j) in the tile.
+template <int DIM>
+struct TileNeighbours {
+    LdsNeighbour l, r, t, b;
+    __device__ __forceinline__ TileNeighbours(const double *s_nb, int i, int j, int i0, int j0)
+    {
+        using Tile = ForcingTile<DIM>;
+        const int sy = (DIM == 2) ? i - i0 + 1 : 0, sx = j - j0 + 1;
+        const int me = sy * Tile::lx + sx;
+        l = {s_nb + me - 1, Tile::slots};
+        r = {s_nb + me + 1, Tile::slots};
+        t = {s_nb + me - Tile::lx * (DIM == 2), Tile::slots};
+        b = {s_nb + me + Tile::lx * (DIM == 2), Tile::slots};
+    }
+};
+
 inline long long forcing_blocks(int dim, int ny, int nx)
 {
     if (dim == 2) {
@@ -226,43 +289,15 @@ __global__ __launch_bounds__(kForceBS, VG_FORCING_WAVES) void forcing_kernel(For
     typename P::Params m = a.m;
     if (!a.fixed_params) P::begin_step(m, a.clk->S);   // sub-cycles run in second-order steps only
     const double ts = 0.5 * (a.clk->dt / a.subcycles);
-    const int tid = threadIdx.x;
     int i, j, i0, j0;
-    if (DIM == 2) {
-        const int bx = (a.nx + Tile::tx - 1) / Tile::tx;
-        i0 = (blockIdx.x / bx) * Tile::ty;
-        j0 = (blockIdx.x % bx) * Tile::tx;
-        i = i0 + tid / Tile::tx;
-        j = j0 + tid % Tile::tx;
-    } else {
-        const long long c0 = (long long)blockIdx.x * kForceBS;
-        i0 = (int)(c0 / a.nx);
-        j0 = (int)(c0 - (long long)i0 * a.nx);
-        i = i0;
-        j = j0 + tid;   // 1D grids have one row (ny = 1)
-    }
-    // The tile's slots: slot (ly, lx) holds the cell at (i0 - 1 + ly, j0 - 1 + lx)
-    // (1D: (i0, j0 - 1 + lx)), positions outside the grid taking the boundary's
-    // ghost cell; slots no cell of the tile reads are skipped.
-    for (int sidx = tid; sidx < SL; sidx += kForceBS) {
-        const int sy = sidx / Tile::lx, sx = sidx - sy * Tile::lx;
-        const int gi = (DIM == 2) ? i0 - 1 + sy : i0, gj = j0 - 1 + sx;
-        const bool rim_x = (sx == 0 || sx == Tile::lx - 1), rim_y = (DIM == 2) && (sy == 0 || sy == Tile::ly - 1);
-        const bool need = !(rim_x && rim_y) && gi >= -1 && gi <= a.ny && gj >= -1 && gj <= a.nx;
-        if (!need) continue;
-        const long long idx = (long long)ghost_index(gi, a.ny, a.bc) * a.nx + ghost_index(gj, a.nx, a.bc);
-        double U[NF];
+    forcing_tile_origin<DIM>(a.nx, i0, j0, i, j);
+    fill_neighbour_tile<NF, DIM>(s_nb, i0, j0, a.ny, a.nx, a.bc, [&](long long idx, double (&U)[NF]) {
         forcing_neighbour<P>(m, a.in, a.plane, idx, U);
-#pragma unroll
-        for (int f = 0; f < NF; f++) s_nb[f * SL + sidx] = U[f];
-    }
-    __syncthreads();
+    });
     unsigned long long key = 0;
     if (i < a.ny && j < a.nx) {
-        const int sy = (DIM == 2) ? i - i0 + 1 : 0, sx = j - j0 + 1;
-        const int me = sy * Tile::lx + sx;
-        const LdsNeighbour UL{s_nb + me - 1, SL}, UR{s_nb + me + 1, SL};
-        const LdsNeighbour UT{s_nb + me - Tile::lx * (DIM == 2), SL}, UB{s_nb + me + Tile::lx * (DIM == 2), SL};
+        const TileNeighbours<DIM> nb(s_nb, i, j, i0, j0);
+        const LdsNeighbour &UL = nb.l, &UR = nb.r, &UT = nb.t, &UB = nb.b;
         const long long c = (long long)i * a.nx + j;
         double w0[NF], Um[NF];
         load_prim<NF>(a.in, a.plane, c, w0);

@@ -769,30 +769,42 @@ __device__ __forceinline__ void tau_block_add(double *tau_part, double tb1, doub
     }
 }
 
-// 1. The forward sub-cycle of every cell with its stage states recorded, the
-// adjoint of its output conversion, and the first increment adjoints.
-template <int DIM>
-__global__ __launch_bounds__(kBS) void hpr_adj_force_record_kernel(ForcingAdjArgs a)
+// 0. Every cell's conserved vector as its neighbours' predictors see it
+// (forcing_neighbour), into a.rt.
+__global__ __launch_bounds__(kBS) void hpr_adj_force_rt_kernel(ForcingAdjArgs a)
 {
-    using T = ForcingTraits<HP>;
-    constexpr int NS = T::kMaxSlots;
-    const HPar &m = a.m;
     const long long cells = (long long)a.ny * a.nx;
     const long long c = (long long)blockIdx.x * kBS + threadIdx.x;
     if (c >= cells) return;
-    long long cl, cr, ct, cb;
-    cell_neighbours(a, c, cl, cr, ct, cb);
+    double RT[NF];
+    forcing_neighbour<HP>(a.m, a.w, a.plane, c, RT);
+    store_planes(a.rt, a.plane, c, RT);
+}
+
+// 1. The forward sub-cycle of every cell with its stage states recorded, the
+// adjoint of its output conversion, and the first increment adjoints.  As in
+// forcing_kernel, the workgroup covers a tile of cells (forcing_blocks) whose
+// neighbours' conserved vectors (from a.rt) sit in LDS, read where the
+// predictors use them: two waves per SIMD instead of one with all four held in
+// registers.
+template <int DIM>
+__global__ __launch_bounds__(kForceBS, 2) void hpr_adj_force_record_kernel(ForcingAdjArgs a)
+{
+    using T = ForcingTraits<HP>;
+    constexpr int NS = T::kMaxSlots;
+    __shared__ double s_nb[NF * ForcingTile<DIM>::slots];
+    const HPar &m = a.m;
+    int i, j, i0, j0;
+    forcing_tile_origin<DIM>(a.nx, i0, j0, i, j);
+    fill_neighbour_tile<NF, DIM>(s_nb, i0, j0, a.ny, a.nx, a.bc,
+                                 [&](long long idx, double (&U)[NF]) { load_planes(a.rt, a.plane, idx, U); });
+    if (i >= a.ny || j >= a.nx) return;
+    const long long c = (long long)i * a.nx + j;
     double N[NF], K[4][NS];
     {
-        double UL[NF], UR[NF], UT[NF], UB[NF], w0[NF], Um[NF], RT[NF];
-        forcing_neighbour<HP>(m, a.w, a.plane, cl, UL);
-        forcing_neighbour<HP>(m, a.w, a.plane, cr, UR);
-        if (DIM == 2) {
-            forcing_neighbour<HP>(m, a.w, a.plane, ct, UT);
-            forcing_neighbour<HP>(m, a.w, a.plane, cb, UB);
-        }
-        forcing_neighbour<HP>(m, a.w, a.plane, c, RT);
-        store_planes(a.rt, a.plane, c, RT);
+        const TileNeighbours<DIM> nb(s_nb, i, j, i0, j0);
+        const LdsNeighbour &UL = nb.l, &UR = nb.r, &UT = nb.t, &UB = nb.b;
+        double w0[NF], Um[NF];
         load_planes(a.w, a.plane, c, w0);
         HP::prim_to_cons(m, w0, Um);
         GlobalStages rec{a.um, a.ue, a.plane, c};

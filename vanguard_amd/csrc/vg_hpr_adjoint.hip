@@ -198,8 +198,10 @@ int forcing_adjoint(vg_ctx *ctx, const HPRGrid &g, const Work &w, const double *
     hipStream_t s = ctx->stream;
     const double cx = ts / dx;
     return ctx->timed("hpr_adjoint_source", [&] {
-        if (g.dim == 1) ha::hpr_adj_force_record_kernel<1><<<nb, ha::kBS, 0, s>>>(a);
-        else ha::hpr_adj_force_record_kernel<2><<<nb, ha::kBS, 0, s>>>(a);
+        ha::hpr_adj_force_rt_kernel<<<nb, ha::kBS, 0, s>>>(a);
+        const unsigned tiles = (unsigned)vg::forcing_blocks(g.dim, g.ny, g.nx);
+        if (g.dim == 1) ha::hpr_adj_force_record_kernel<1><<<tiles, vg::kForceBS, 0, s>>>(a);
+        else ha::hpr_adj_force_record_kernel<2><<<tiles, vg::kForceBS, 0, s>>>(a);
         for (int st = 2; st >= 0; st--) {
             const double coef = (st < 2) ? 0.5 : 1.0;
             ha::hpr_adj_force_source_kernel<<<nb, ha::kBS, 0, s>>>(a, st);
