@@ -92,10 +92,11 @@ int vg_ctx_set_graphs(vg_ctx *ctx, int enabled);
 /* Tile-shape autotuner (default on): the first run on a grid shape times the
  * candidate y-sweep shapes (threads per block x rows per strip) on that grid
  * and keeps the fastest for the context.  vg_ctx_tuned_shape returns the chosen
- * variant for a key "physics/order/mode/ny/nx" (forward Y sweeps, variants 0-5;
+ * variant for a key "physics/order/mode/ny/nx" (forward Y sweeps, variants 0-7;
  * physics "euler", or "euler_planar" for grids swept with 4 fields)
  * or "euler_adjoint_x/ny/nx" / "euler_adjoint_y/ny/nx" (adjoint sweeps, variants
- * 0-3; a 1D grid of n cells has ny = 1, nx = n), -1 if not tuned
+ * 0-3; "euler_planar_adjoint_..." for the 4-field adjoint sweeps; a 1D grid of n
+ * cells has ny = 1, nx = n), -1 if not tuned
  * yet.  vg_ctx_set_tuned_shape pins a variant for a key (for example one saved
  * from an earlier process); -1 forgets it.  Shapes change speed, never results. */
 int vg_ctx_set_autotune(vg_ctx *ctx, int enabled);

@@ -99,6 +99,23 @@ def test_autotuned_shape_gives_identical_results(fresh_ctx):
     r = S.euler_solve(w, 2, 2.0 / 300, 0.8, math.inf, max_steps=9, ctx=fresh_ctx)
     v = fresh_ctx.tuned_shape("euler_planar", 2, S.EXACT, 70, 300)
     print(f"tuned y-sweep shape for 70 x 300: variant {v}", flush=True)
-    assert 0 <= v < 4
+    assert 0 <= v < 8
     O.assert_bitwise(r.cells, ref.cells, "state")
     O.assert_bitwise(r.dt, ref.dt, "dt")
+
+
+@pytest.mark.parametrize("mode", [S.EXACT, S.FAST])
+def test_every_y_shape_gives_identical_results(fresh_ctx, mode):
+    """Each of the eight y-sweep shapes, pinned, on a grid whose rows and
+    columns none of them divides evenly (strips and blocks end inside the
+    grid): the same bits as the default shape."""
+    w = problems.disc_2d(301, ny=173)
+    key = f"euler_planar/2/{mode}/173/301"
+    runs = []
+    for v in range(8):
+        fresh_ctx.set_tuned_key(key, v)
+        runs.append(S.euler_solve(w, 2, 2.0 / 301, 0.8, math.inf, max_steps=7, mode=mode, ctx=fresh_ctx))
+        assert fresh_ctx.tuned_key(key) == v
+    for v in range(1, 8):
+        O.assert_bitwise(runs[v].cells, runs[0].cells, f"state, y shape {v}")
+        O.assert_bitwise(runs[v].dt, runs[0].dt, f"dt, y shape {v}")

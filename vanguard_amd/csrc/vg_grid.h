@@ -105,8 +105,10 @@ __global__ __launch_bounds__(256) void constant_planes_kernel(const double *__re
 // Candidate (threads per block, rows per strip, columns per thread) of the y
 // sweep: variant 0 is Tiles<P>'s shape; 1-3 trade strip length for more
 // workgroups on small grids; 4-5 give each thread two columns (two independent
-// dependency chains per wave).
-constexpr int kYVariants = 6;
+// dependency chains per wave); 6-7 are longer strips, whose wave counts on
+// 4096-row grids (3072 and 6144 per 4096 columns) fill a device running three
+// waves per SIMD in whole rounds.
+constexpr int kYVariants = 8;
 
 template <class P, int ORDER, int MODE, int BS, int H, int CPT = 1>
 int launch_y_shape(vg_ctx *ctx, const SweepArgs<P> &a)
@@ -126,6 +128,8 @@ int launch_y(vg_ctx *ctx, int variant, const SweepArgs<P> &a)
         case 3: return launch_y_shape<P, ORDER, MODE, 64, 64>(ctx, a);
         case 4: return launch_y_shape<P, ORDER, MODE, 64, 32, 2>(ctx, a);
         case 5: return launch_y_shape<P, ORDER, MODE, 64, 64, 2>(ctx, a);
+        case 6: return launch_y_shape<P, ORDER, MODE, 256, 86>(ctx, a);
+        case 7: return launch_y_shape<P, ORDER, MODE, 256, 43>(ctx, a);
         default: break;
         }
     }
