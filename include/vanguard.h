@@ -121,6 +121,11 @@ int vg_ctx_set_tuned_shape(vg_ctx *ctx, const char *key, int variant);
 /* Frees the device workspace the context keeps between calls (the adjoint's
  * checkpoints and sweep inputs). */
 int vg_ctx_release_workspace(vg_ctx *ctx);
+/* Device memory the Euler adjoint may use for its recorded forward (bytes; 0,
+ * the default: 85 % of the free device memory).  A smaller limit records fewer
+ * steps' inner sweep inputs and recomputes the rest; results are the same
+ * bits.  No reference counterpart. */
+int vg_ctx_set_adjoint_memory(vg_ctx *ctx, int64_t bytes);
 
 /* ---- Euler: whole-solve drop-ins ---------------------------------------- */
 
@@ -221,8 +226,12 @@ int vg_euler_solve_batch(vg_ctx *ctx, int n_inst, const int64_t *offsets, const 
  * then the adjoint backwards.  With max_steps > 0 the forward can write its
  * sweeps into recording buffers of the context's workspace: checkpoint_every 0
  * keeps every sweep input when they fit in free device memory (the reverse pass
- * recomputes nothing), else every step's start (the reverse pass recomputes the
- * step's two inner sweeps), as does checkpoint_every 1.  Otherwise a
+ * recomputes nothing), else every step's start and, for as many of the last
+ * steps as fit, their inner sweep inputs too (the reverse pass recomputes the
+ * two inner sweeps of the other steps); checkpoint_every 1 keeps the step
+ * starts only.  A grid swept with 4 fields records planes 0-4 only.  The
+ * memory used is at most 85 % of the free device memory, or the context's
+ * vg_ctx_set_adjoint_memory limit.  Otherwise a
  * checkpoint every `checkpoint_every` steps (0: ceil(sqrt(max_steps)), 16 for
  * runs to t_final) and recomputation per segment.  Every choice gives the same
  * bits.  seed: the

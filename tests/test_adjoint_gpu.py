@@ -217,6 +217,38 @@ def test_recorded_forward_stops_at_the_final_time(dim):
         O.assert_bitwise(r.grad, ref.grad, f"gradient, layout {k}")
 
 
+@pytest.mark.parametrize("make,full", [(lambda: smooth_2d(36, 20), 5), (lambda: planar(smooth_2d(36, 20)), 3),
+                                       (lambda: sod_2d(20, 28), -1)])
+def test_partly_recorded_forward_gives_the_same_bits(make, full):
+    """Under a device memory limit the recorded forward keeps every step's start
+    and the inner sweep inputs of only its last `full` steps (the reverse pass
+    recomputes the others'); a limit below the step starts (full = -1) falls
+    back to checkpoints.  Time steps, final state and gradient are the
+    unlimited run's bit for bit, for 6-field and 4-field grids (whose recording
+    buffers hold planes 0-4 only)."""
+    w = make()
+    ny, nx = w.shape[:2]
+    steps = 12
+    seed = np.random.default_rng(7).standard_normal(w.shape)
+    seed[..., 3] = 0.0
+    seed[..., 5] = 0.0
+    dx = 1.0 / nx
+    kw = dict(params=mat(IDEAL), max_steps=steps, checkpoint_every=0)
+    ref = S.euler_adjoint(w, seed, 2, dx, 0.6, math.inf, **kw)
+    plane = (ny * nx + 63) // 64 * 64
+    reduced = not np.any(w[..., 3]) and np.all(w[..., 5] == 1.0)
+    rb, gb = (5 if reduced else 6) * plane * 8, 6 * plane * 8
+    # record_layout: limit - 2 grids - 64 MiB left for (steps + 2) starts and scratch + 2 per full step
+    limit = 64 * 2 ** 20 + 2 * gb + (steps + 2) * rb + (2 * full + 1) * rb if full >= 0 else 64 * 2 ** 20
+    ctx = S.Context(0)
+    ctx.set_adjoint_memory(limit)
+    r = S.euler_adjoint(w, seed, 2, dx, 0.6, math.inf, ctx=ctx, **kw)
+    assert r.stats.steps == steps
+    O.assert_bitwise(r.dt, ref.dt, "time steps")
+    O.assert_bitwise(r.cells, ref.cells, "final state")
+    O.assert_bitwise(r.grad, ref.grad, "gradient")
+
+
 def planar(w):
     w = np.array(w)
     w[..., 3] = 0.0

@@ -77,6 +77,7 @@ SIGNATURES = {
     "vg_ctx_tuned_shape": (C.c_int, [vptr, C.c_char_p, C.POINTER(C.c_int)]),
     "vg_ctx_set_tuned_shape": (C.c_int, [vptr, C.c_char_p, C.c_int]),
     "vg_ctx_release_workspace": (C.c_int, [vptr]),
+    "vg_ctx_set_adjoint_memory": (C.c_int, [vptr, C.c_int64]),
     "vg_euler_solve_1d": (C.c_int, [vptr, dptr, dptr, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                                     C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(EulerParams), C.c_int,
                                     dptr, C.c_int, C.POINTER(RunStats)]),

@@ -176,62 +176,71 @@ int tune_adjoint_p(bool planar, vg_ctx *ctx, int dir, const vg::AdjointArgs &a, 
     return planar ? tune_adjoint<vg::AdjPlanar>(ctx, dir, a, variant) : tune_adjoint<vg::AdjFull>(ctx, dir, a, variant);
 }
 
-// Grids of the recorded adjoint's layouts for a run of max_steps steps: every
-// sweep input (per per step), or the step starts plus two recomputation buffers;
-// 0 when neither fits in free device memory beside the adjoint's own buffers.
-int record_layout(vg_ctx *ctx, int max_steps, int per, size_t gbytes)
+// The recorded adjoint's layout for a run of max_steps steps: the number of
+// steps, the last ones, that keep every sweep input (2D: their two inner sweep
+// inputs too); the others keep their start only and the reverse pass
+// recomputes their inner sweep inputs into two scratch buffers.  rbytes is one
+// recording buffer, gbytes one full grid (the adjoint's own buffers).  -1 when
+// not even the step starts fit in free device memory (or the context's
+// adjoint memory limit).
+int record_layout(vg_ctx *ctx, int max_steps, int dim, size_t rbytes, size_t gbytes)
 {
-    if (max_steps <= 0) return 0;
+    if (max_steps <= 0) return -1;
     size_t free_b = 0, total_b = 0;
-    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 0;
-    const double avail = 0.85 * (double)(free_b + ctx->workspace_bytes());
-    auto fits = [&](double grids) { return (double)gbytes * (grids + 4.0) + 64.0 * 1024 * 1024 < avail; };
-    if (fits((double)per * max_steps)) return per;
-    if (per > 1 && fits((double)max_steps + 2.0)) return 1;
-    return 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return -1;
+    double avail = 0.85 * (double)(free_b + ctx->workspace_bytes());
+    if (ctx->adjoint_memory > 0) avail = std::min(avail, (double)ctx->adjoint_memory);
+    const double left = avail - 2.0 * (double)gbytes - 64.0 * 1024 * 1024;
+    const double rb = (double)rbytes;
+    if (dim == 1) return (double)max_steps * rb <= left ? max_steps : -1;
+    if (3.0 * max_steps * rb <= left) return max_steps;
+    const double base = ((double)max_steps + 2.0) * rb;   // every start and the two scratch buffers
+    if (base > left) return -1;
+    return std::min(max_steps - 1, (int)((left - base) / (2.0 * rb)));
 }
 
 // The adjoint with the forward's states recorded by the forward itself: the
 // CFL forward runs its steps (the solve's clock, kernels and fused wave-speed
 // maximum) with its sweeps writing into recording buffers instead of the grid's
-// two, so nothing is copied.  Layout kept = per (2D: 3): every sweep input has
-// its own buffer, slots[3 s + k] the input of sweep k of step s, and the reverse
-// pass reads them.  kept = 1 in 2D: only the step starts slots[s] are kept, the
-// two inner sweeps of a step write two scratch buffers and the reverse pass
-// recomputes them for each step with its recorded dt.  slots[0] is the loaded
-// grid.  When the grid takes the 4-field sweeps, their outputs leave planes 3
-// and 5 unwritten and the adjoint sweeps read those planes (w = +0, lambda = 1)
-// from slot 0.
-int adjoint_recorded(vg_ctx *ctx, EGrid &g, int kept, int dim, int ny, int nx, double dx, double cfl, double t_final,
+// two, so nothing is copied.  start(s) = slots[s] is the input of step s
+// (slots[0] the loaded grid, slots[s + 1] its output); the last `full` steps
+// (from s0 = max_steps - full on) also keep their two inner sweep inputs
+// (2D) in slots of their own, the earlier ones write them into two scratch
+// buffers and the reverse pass recomputes them for each step with its
+// recorded dt.  When the grid takes the 4-field sweeps, their outputs leave
+// planes 3 and 5 unwritten, the recording buffers hold planes 0-4 only and the
+// adjoint sweeps read planes 3 and 5 (w = +0, lambda = 1) from slot 0.
+int adjoint_recorded(vg_ctx *ctx, EGrid &g, int full, int dim, int ny, int nx, double dx, double cfl, double t_final,
                      int max_steps, double bias, int limiter, int bc, const EulerPhys::Params &m, const double *seed,
                      double *prim_out, double *grad, int on_device, double *dt_log, int dt_log_cap,
                      vg_run_stats *stats, std::chrono::steady_clock::time_point w0)
 {
     hipStream_t s = ctx->stream;
     const size_t gbytes = sizeof(double) * 6 * g.plane;
-    const int per = (dim == 2) ? 3 : 1;
+    const bool reduced = g.reduced;   // decided by the caller's choose_physics
+    const size_t rbytes = reduced ? sizeof(double) * 5 * g.plane : gbytes;
     auto wsp = [&](size_t i, size_t bytes, double **p) {
         void *v = nullptr;
         int r = ctx->workspace_buffer(i, bytes, &v);
         *p = static_cast<double *>(v);
         return r;
     };
-    // workspace: 0 dts, 1-2 adjoint ping-pong, 3 ghost adjoints, 4-5 scratch
-    // (kept = 1), 6 .. the recording buffers after slot 0
-    std::vector<double *> slots((size_t)kept * max_steps + 1, nullptr);
+    const int s0 = max_steps - full;
+    const bool scratch = (dim == 2 && s0 > 0);
+    // workspace: 0 dts, 1-2 adjoint ping-pong, 3 ghost adjoints, 4-5 scratch,
+    // 6 .. the recording buffers after slot 0
+    std::vector<double *> slots((size_t)max_steps + 1 + (dim == 2 ? 2 * (size_t)full : 0), nullptr);
     slots[0] = g.buf[g.cur];
     double *dts_dev = nullptr, *wb[2] = {nullptr, nullptr}, *ghost = nullptr, *T[2] = {nullptr, nullptr};
     int rc = wsp(0, sizeof(double) * max_steps, &dts_dev);
     if (!rc) rc = wsp(1, gbytes, &wb[0]);
     if (!rc) rc = wsp(2, gbytes, &wb[1]);
     if (!rc) rc = wsp(3, sizeof(double) * 4 * 6 * (size_t)std::max(nx, ny), &ghost);
-    if (!rc && kept != per) rc = wsp(4, gbytes, &T[0]);
-    if (!rc && kept != per) rc = wsp(5, gbytes, &T[1]);
-    for (size_t k = 1; k < slots.size() && !rc; k++) rc = wsp(6 + k - 1, gbytes, &slots[k]);
+    if (!rc && scratch) rc = wsp(4, rbytes, &T[0]);
+    if (!rc && scratch) rc = wsp(5, rbytes, &T[1]);
+    for (size_t k = 1; k < slots.size() && !rc; k++) rc = wsp(6 + k - 1, rbytes, &slots[k]);
     if (rc) return rc;
     // ---- forward: the steps of GridState::run, into the recording buffers -------------
-    if ((rc = g.choose_physics())) return rc;
-    const bool reduced = g.reduced;
     if ((rc = g.refresh_max(bc, m, g.fused_count(2)))) return rc;
     if (dim == 2) {
         if (!g.tune_clk) VG_HIP(hipMalloc(&g.tune_clk, sizeof(vg::Clock)));
@@ -248,16 +257,19 @@ int adjoint_recorded(vg_ctx *ctx, EGrid &g, int kept, int dim, int ny, int nx, d
     h.dx = dx;
     h.max_steps = max_steps;
     VG_HIP(hipMemcpyAsync(g.clk, &h, sizeof(vg::Clock), hipMemcpyHostToDevice, s));
-    // the four buffers a step reads and writes: its start, the two inner sweep inputs, its end
+    // the buffers a step reads and writes: its start, (2D) the two inner sweep
+    // inputs and its end; 1D: its start and its end
     auto step_bufs = [&](int st, double *(&b)[4]) {
-        if (kept == per) {
-            for (int k = 0; k <= per; k++) b[k] = slots[(size_t)per * st + k];
-        } else {
-            b[0] = slots[st];
-            b[1] = T[0];
-            b[2] = T[1];
-            b[3] = slots[st + 1];
+        b[0] = slots[st];
+        if (dim == 1) {
+            b[1] = slots[st + 1];
+            b[2] = b[3] = nullptr;
+            return;
         }
+        const size_t in = (size_t)max_steps + 1 + 2 * (size_t)(st - s0);
+        b[1] = st >= s0 ? slots[in] : T[0];
+        b[2] = st >= s0 ? slots[in + 1] : T[1];
+        b[3] = slots[st + 1];
     };
     for (int st = 0; st < max_steps && !rc; st++) {
         vg::clock_begin_kernel<<<1, vg::kReduceThreads, 0, s>>>(g.clk, g.partials, g.n_partials, dts_dev, max_steps);
@@ -282,14 +294,17 @@ int adjoint_recorded(vg_ctx *ctx, EGrid &g, int kept, int dim, int ny, int nx, d
     const int steps = h.steps_this_run;
     std::vector<double> dts(std::max(steps, 1));
     if (steps > 0) VG_HIP(hipMemcpyAsync(dts.data(), dts_dev, sizeof(double) * steps, hipMemcpyDeviceToHost, s));
-    // the final state, with slot 0's constant planes when the sweeps left them out
-    double *fin = slots[(size_t)kept * steps];
+    // the final state; with the 4-field sweeps its four planes and slot 0's
+    // constant planes go into the grid's other buffer first
+    double *fin = slots[steps];
     if (reduced && steps > 0) {
-        using R = vg::ReducedPhys<EulerPhys>;
-        for (int i = 0; i < R::kConst; i++) {
-            const long long off = (long long)R::plane(i) * g.plane;
-            VG_HIP(hipMemcpyAsync(fin + off, slots[0] + off, sizeof(double) * g.cells, hipMemcpyDeviceToDevice, s));
-        }
+        double *out = g.buf[1 - g.cur];
+        const size_t pb = sizeof(double) * g.plane;
+        VG_HIP(hipMemcpyAsync(out, fin, 3 * pb, hipMemcpyDeviceToDevice, s));
+        VG_HIP(hipMemcpyAsync(out + 4 * g.plane, fin + 4 * g.plane, pb, hipMemcpyDeviceToDevice, s));
+        for (int f = 3; f <= 5; f += 2)
+            VG_HIP(hipMemcpyAsync(out + f * g.plane, slots[0] + f * g.plane, pb, hipMemcpyDeviceToDevice, s));
+        fin = out;
     }
     {
         double *dst = on_device ? prim_out : g.staging;
@@ -320,7 +335,7 @@ int adjoint_recorded(vg_ctx *ctx, EGrid &g, int kept, int dim, int ny, int nx, d
     for (int st = steps - 1; st >= 0 && !rc; st--) {
         double *b[4];
         step_bufs(st, b);
-        if (kept != per) {
+        if (dim == 2 && st < s0) {
             // the step's two inner sweep inputs again, with its recorded dt
             const std::vector<double *> rs = {b[0], b[1], b[2], nullptr};
             rc = forward_fixed(ctx, g, dts_dev, st, st + 1, dx, bias, limiter, bc, m, rs, true);
@@ -402,15 +417,21 @@ int adjoint(vg_ctx *ctx, int dim, const double *prim_in, double *prim_out, const
     const size_t gbytes = sizeof(double) * 6 * g.plane;
     const int per = (dim == 2) ? 3 : 1;
     // checkpoint_every 0: the fullest recorded layout that fits; 1: the step
-    // starts (a checkpoint every step, written by the forward itself)
-    int kept = 0;
-    if (checkpoint_every == 0) kept = record_layout(ctx, max_steps, per, gbytes);
-    else if (checkpoint_every == 1 && record_layout(ctx, max_steps, per, gbytes) > 0) kept = 1;
-    if (kept > 0) {
-        rc = adjoint_recorded(ctx, g, kept, dim, ny, nx, dx, cfl, t_final, max_steps, bias, limiter, bc, m, seed,
-                              prim_out, grad, on_device, dt_log, dt_log_cap, stats, w0);
-        g.destroy();
-        return rc;
+    // starts only (a checkpoint every step, written by the forward itself)
+    if (max_steps > 0 && checkpoint_every <= 1) {
+        if ((rc = g.choose_physics())) {
+            g.destroy();
+            return rc;
+        }
+        const size_t rbytes = g.reduced ? sizeof(double) * 5 * g.plane : gbytes;
+        int full = record_layout(ctx, max_steps, dim, rbytes, gbytes);
+        if (checkpoint_every == 1 && full >= 0) full = (dim == 1) ? max_steps : 0;
+        if (full >= 0) {
+            rc = adjoint_recorded(ctx, g, full, dim, ny, nx, dx, cfl, t_final, max_steps, bias, limiter, bc, m, seed,
+                                  prim_out, grad, on_device, dt_log, dt_log_cap, stats, w0);
+            g.destroy();
+            return rc;
+        }
     }
     const int K = choose_interval(ctx, checkpoint_every, max_steps, per, gbytes);
     // Workspace buffers: 0 dts, 1-2 adjoint ping-pong, 3 ghost adjoints,

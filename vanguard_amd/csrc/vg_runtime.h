@@ -56,6 +56,7 @@ struct vg_ctx {
     // after another on this context's stream), and the worker contexts (one
     // stream each) that carry them, kept between calls.
     int batch_streams = 4;
+    size_t adjoint_memory = 0;   // limit of the Euler adjoint's recording (0: 85 % of free memory)
     std::vector<vg_ctx *> workers;
     std::map<std::string, int> tuned;
     std::map<std::string, vg::KernelTimer> timers;

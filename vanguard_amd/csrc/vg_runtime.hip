@@ -167,6 +167,14 @@ int vg_ctx_set_batch_streams(vg_ctx *ctx, int streams)
     return VG_OK;
 }
 
+int vg_ctx_set_adjoint_memory(vg_ctx *ctx, int64_t bytes)
+{
+    VG_CHECK(ctx != nullptr, "vg_ctx_set_adjoint_memory: null context");
+    VG_CHECK(bytes >= 0, "vg_ctx_set_adjoint_memory: bytes must be >= 0");
+    ctx->adjoint_memory = (size_t)bytes;
+    return VG_OK;
+}
+
 int vg_ctx_set_timing(vg_ctx *ctx, int enabled)
 {
     VG_CHECK(ctx != nullptr, "vg_ctx_set_timing: null context");

@@ -89,6 +89,10 @@ class Context:
         """Free the device workspace kept between calls (adjoint checkpoints)."""
         check(self.lib.vg_ctx_release_workspace(self.handle), "vg_ctx_release_workspace")
 
+    def set_adjoint_memory(self, nbytes: int):
+        """Limit the Euler adjoint's recorded forward to nbytes of device memory (0: 85 % of free)."""
+        check(self.lib.vg_ctx_set_adjoint_memory(self.handle, int(nbytes)), "vg_ctx_set_adjoint_memory")
+
     def set_tuned_key(self, key: str, variant: int):
         """Pin a tile-shape variant for a key; -1 forgets it."""
         check(self.lib.vg_ctx_set_tuned_shape(self.handle, key.encode(), int(variant)), "vg_ctx_set_tuned_shape")
