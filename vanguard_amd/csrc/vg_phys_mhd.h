@@ -5,6 +5,7 @@
 #pragma once
 
 #include "vg_device.h"
+#include "vg_quot.h"
 
 namespace vg {
 
@@ -80,14 +81,29 @@ struct MHDPhys {
         U[8] = w[8];
     }
 
-    // setConservedVariableVector (mhdstatevector.cpp:49-68).
-    __device__ static __forceinline__ void cons_to_prim(const Params &m, const double (&U)[NF], double (&w)[NF])
+    // The quotients by rho of a conserved vector (u, v, w and the hydrodynamic
+    // specific total energy (E - |B|^2 / 2) / rho), each the correctly rounded
+    // IEEE quotient, from one shared reciprocal of rho.
+    __device__ static __forceinline__ void by_rho(const double (&U)[NF], const SharedDivisor &d, double b2,
+                                                  double (&q)[4])
+    {
+        const double a[4] = {U[1], U[2], U[3], U[4] - (0.5 * b2)};
+        quot_group<4>(a, d, q);
+    }
+
+    // setConservedVariableVector (mhdstatevector.cpp:49-68); returns the shared
+    // divisor of rho for a flux of the same density.
+    __device__ static __forceinline__ SharedDivisor cons_to_prim_d(const Params &m, const double (&U)[NF],
+                                                                   double (&w)[NF])
     {
         const double rho = U[0];
-        const double u = U[1] / rho, v = U[2] / rho, z = U[3] / rho;
+        const SharedDivisor d = shared_divisor(rho);
         const double bx = U[5], by = U[6], bz = U[7];
         const double b2 = (bx * bx) + (by * by) + (bz * bz);
-        const double et = (U[4] - (0.5 * b2)) / rho;
+        double q[4];
+        by_rho(U, d, b2, q);
+        const double u = q[0], v = q[1], z = q[2];
+        const double et = q[3];
         const double vsq = (u * u) + (v * v) + (z * z);
         const double e = et - (0.5 * vsq);
         w[0] = rho;
@@ -99,18 +115,27 @@ struct MHDPhys {
         w[6] = by;
         w[7] = bz;
         w[8] = U[8];
+        return d;
+    }
+    __device__ static __forceinline__ void cons_to_prim(const Params &m, const double (&U)[NF], double (&w)[NF])
+    {
+        (void)cons_to_prim_d(m, U, w);
     }
 
-    // computeXFluxVector / computeYFluxVector (mhdstatevector.cpp:108-158, 165-215).
+    // computeXFluxVector / computeYFluxVector (mhdstatevector.cpp:108-158,
+    // 165-215), given the shared divisor of the density U[0].
     template <int DIR>
-    __device__ static __forceinline__ void flux(const Params &m, const double (&U)[NF], double (&F)[NF])
+    __device__ static __forceinline__ void flux_d(const Params &m, const double (&U)[NF], const SharedDivisor &d,
+                                                  double (&F)[NF])
     {
         const double rho = U[0];
-        const double u = U[1] / rho, v = U[2] / rho, z = U[3] / rho;
         const double bx = U[5], by = U[6], bz = U[7], psi = U[8];
         const double b2 = (bx * bx) + (by * by) + (bz * bz);
+        double q[4];
+        by_rho(U, d, b2, q);
+        const double u = q[0], v = q[1], z = q[2];
         const double E = U[4];
-        const double het = (E - (0.5 * b2)) / rho;
+        const double het = q[3];
         const double vsq = (u * u) + (v * v) + (z * z);
         const double e = het - (0.5 * vsq);
         const double p = eos_p(m, rho, e);
@@ -137,6 +162,11 @@ struct MHDPhys {
             F[7] = (bz * v) - (by * z);
             F[8] = ch2 * by;
         }
+    }
+    template <int DIR>
+    __device__ static __forceinline__ void flux(const Params &m, const double (&U)[NF], double (&F)[NF])
+    {
+        flux_d<DIR>(m, U, shared_divisor(U[0]), F);
     }
 
     // MHDWaveSpeeds::computeX/YFastMagnetoAcousticSpeed with the Alfven speed
