@@ -11,6 +11,7 @@
 #pragma once
 
 #include "vg_device.h"
+#include "vg_quot.h"
 
 namespace vg {
 
@@ -276,24 +277,34 @@ struct HPRPhysT {
         }
     }
 
+    // The quotients by rho of a conserved vector (u, v, w, J and E), each the
+    // correctly rounded IEEE quotient, from one shared reciprocal of rho.
+    __device__ static __forceinline__ void by_rho(const double (&U)[NF], double (&q)[7])
+    {
+        const double a[7] = {U[11], U[12], U[13], U[14], U[15], U[16], U[1]};
+        quot_group<7>(a, shared_divisor(U[0]), q);
+    }
+
     // setConservedVariableVector (hprstatevector.cpp:59-76).
     __device__ static __forceinline__ void cons_to_prim(const Params &m, const double (&U)[NF], double (&w)[NF])
     {
         double A[3][3];
         load_A(U, A);
         const double rho = U[0];
-        const double u = U[11] / rho, v = U[12] / rho, z = U[13] / rho;
+        double q[7];
+        by_rho(U, q);
+        const double u = q[0], v = q[1], z = q[2];
         double jx, jy, jz;
         if (m.is_thermal) {
-            jx = U[14] / rho;
-            jy = U[15] / rho;
-            jz = U[16] / rho;
+            jx = q[3];
+            jy = q[4];
+            jz = q[5];
         } else {
             jx = U[14];
             jy = U[15];
             jz = U[16];
         }
-        const double E = U[1] / rho;
+        const double E = q[6];
         w[0] = rho;
         w[1] = pressure_of(m, rho, E, u, v, z, A, jx, jy, jz);
 #pragma unroll
@@ -314,11 +325,13 @@ struct HPRPhysT {
         load_A(U, A);
         const double rho = U[0];
         shear(m, rho, A, sig, dsr);
-        const double u = U[11] / rho, v = U[12] / rho, z = U[13] / rho;
-        const double jx = m.is_thermal ? U[14] / rho : 0.0;
-        const double jy = m.is_thermal ? U[15] / rho : 0.0;
-        const double jz = m.is_thermal ? U[16] / rho : 0.0;
-        const double E = U[1] / rho;
+        double q[7];
+        by_rho(U, q);
+        const double u = q[0], v = q[1], z = q[2];
+        const double jx = m.is_thermal ? q[3] : 0.0;
+        const double jy = m.is_thermal ? q[4] : 0.0;
+        const double jz = m.is_thermal ? q[5] : 0.0;
+        const double E = q[6];
         const double p = pressure_of(m, rho, E, u, v, z, A, jx, jy, jz);
 #pragma unroll
         for (int k = 0; k < NF; k++) F[k] = 0.0;
