@@ -557,7 +557,9 @@ using HPRPhys = HPRPhysT<0>;
 // 17 fields: 128-cell x tiles keep the LDS tile at 52 KB.
 template <int EOS>
 struct Tiles<HPRPhysT<EOS>> {
-    static constexpr int bsx = 128, rx = 4, bsy = 128, hy = 32;
+    // y strips of 16 rows: a 1024 x 1024 grid gives 64 strips x 8 blocks x 2
+    // waves = 1024 waves, one per SIMD (32-row strips leave half idle)
+    static constexpr int bsx = 128, rx = 4, bsy = 128, hy = 16;
     static constexpr bool tune = false;
 };
 
