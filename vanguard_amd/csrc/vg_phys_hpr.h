@@ -558,8 +558,10 @@ using HPRPhys = HPRPhysT<0>;
 template <int EOS>
 struct Tiles<HPRPhysT<EOS>> {
     // y strips of 16 rows: a 1024 x 1024 grid gives 64 strips x 8 blocks x 2
-    // waves = 1024 waves, one per SIMD (32-row strips leave half idle)
-    static constexpr int bsx = 128, rx = 4, bsy = 128, hy = 16;
+    // waves = 1024 waves, one per SIMD (32-row strips leave half idle); x
+    // blocks of 3 rows give 6,156 waves, three full rounds at two per SIMD
+    // (4 rows: 4,608 waves, 2.25 rounds)
+    static constexpr int bsx = 128, rx = 3, bsy = 128, hy = 16;
     static constexpr bool tune = false;
 };
 
