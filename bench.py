@@ -636,6 +636,59 @@ def bench_rgfm_adjoint(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_force2d(args):
+    """First-order FORCE on the headline's 4096^2 disc (EXACT, CFL 0.8):
+    value = cell-updates / wall of `steps` steps, graph replay, state resident
+    in HBM; per-kernel durations from a second, evented run of the same steps."""
+    import torch
+    from vanguard_amd import problems
+    from vanguard_amd import solvers as S
+
+    ctx = S.default_context()
+    w = problems.disc_2d(N)
+    dx = LENGTH / N
+
+    def timed(steps, events):
+        g = S.EulerGrid(2, N, N, ctx=ctx)
+        g.load(w)
+        g.max_wave_speed()
+        ctx.reset_timing()
+        ctx.set_timing(events)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st, _ = g.run(dx, CFL, max_steps=steps, order=1, mode=S.EXACT, dt_cap=steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        ctx.set_timing(False)
+        g.close()
+        return st, el
+
+    if args.warmup:
+        timed(args.warmup, False)
+        timed(args.steps, False)
+    st, el = timed(args.steps, False)
+    timed(args.steps, True)
+    kern = kernel_table(ctx, ["sweep_x_force", "sweep_y_force"])
+    cells = N * N
+    x = kern.get("sweep_x_force", {}).get("avg_ms")
+    bytes_launch = 96 * cells
+    line = {
+        "metric": "Mcell-updates/s FP64 FORCE (first order) 2D Euler 4096\u00b2",
+        "value": round(cells * st.steps / el / 1e6, 2), "unit": "Mcell-updates/s", "n_gpus": 1,
+        "steps": int(st.steps), "warmup": args.warmup, "ms_per_step": round(el * 1e3 / max(st.steps, 1), 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "euler2d_force_4096x4096_disc_64gauss", "grid": [N, N], "cfl": CFL, "order": 1,
+                   "mode": "exact"},
+        "roofline": {"bound": "hbm", "achieved": round(bytes_launch / (x * 1e-3) / 1e9, 1) if x else None,
+                     "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(bytes_launch / (x * 1e-3) / 8e12, 4) if x else None, "traffic": None,
+                     "kernel": "sweep_x_force", "avg_launch_ms": x,
+                     "bytes_basis": "algorithmic: 6 FP64 fields read and written per cell-sweep (96 B)"},
+        "kernels": kern, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_copy(args):
     """Plain device copy of 1 GiB, three times (torch's copy kernel): the known
     byte count that scales the FETCH_SIZE / WRITE_SIZE counters."""
@@ -656,7 +709,8 @@ def bench_copy(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["euler2d", "mhd_batch", "hpr2d", "rgfm", "adjoint", "rgfm_adjoint", "hpr_adjoint", "copy"],
+    ap.add_argument("--workload", choices=["euler2d", "force2d", "mhd_batch", "hpr2d", "rgfm", "adjoint", "rgfm_adjoint",
+                                           "hpr_adjoint", "copy"],
                     default="euler2d")
     ap.add_argument("--rgfm-instances", type=int, nargs="+", default=[0, 1])
     ap.add_argument("--adj-n", type=int, default=4096)
@@ -711,6 +765,8 @@ def main():
         return bench_hpr_adjoint(args)
     if args.workload == "copy":
         return bench_copy(args)
+    if args.workload == "force2d":
+        return bench_force2d(args)
     log(f"building initial condition {N}x{N} ...")
     w = problems.disc_2d(N)
     ctx = S.default_context()
