@@ -109,3 +109,27 @@ def test_out_of_range_divisors_fall_back():
     a = make(rng, b.size, -600, 599)
     shared, _, ieee = run(a, b)
     assert_same(shared, ieee, a, b, "fall-back")
+
+
+def test_tiny_numerators():
+    """Numerators below the range the header comment of vg_quot.h states
+    (2^-1000 .. 2^-600 in magnitude) by divisors within 2^-100 .. 2^100: the
+    shared form is the compiler's division bit for bit.  Deeper numerators
+    (2^-1074 .. 2^-1000, subnormals included) are the documented exception:
+    there it may differ, by at most one unit in the last place."""
+    rng = np.random.default_rng(104)
+    n = 1 << 20
+    a = make(rng, n, -1000, -601)
+    b = make(rng, a.size, -100, 99, signed=False)
+    shared, _, ieee = run(a, b)
+    assert_same(shared, ieee, a, b, "numerators 2^-1000 .. 2^-600")
+    deep = np.concatenate([make(rng, 1 << 18, -1022, -1001),
+                           rng.integers(1, 1 << 52, 4096, dtype=np.uint64).view(np.float64)])
+    deep = np.concatenate([deep, -deep[-4096:]])
+    bd = make(rng, deep.size, -100, 99, signed=False)
+    shared, _, ieee = run(deep, bd)
+    ulps = np.abs(bits(np.abs(shared)).astype(np.int64) - bits(np.abs(ieee)).astype(np.int64))
+    same_sign = np.signbit(shared) == np.signbit(ieee)
+    print(f"numerators below 2^-1000: {int((ulps > 0).sum())} of {deep.size} quotients differ, "
+          f"at most {int(ulps.max())} ulp", flush=True)
+    assert same_sign.all() and ulps.max() <= 1

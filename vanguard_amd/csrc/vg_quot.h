@@ -27,9 +27,11 @@
 //
 // Outside that range the shared form falls back to the IEEE division (a / b)
 // for divisors beyond 2^-100 <= b <= 2^100 (NaN, negative and other
-// densities); numerators beyond 2^600 or below 2^-600 (and not zero) -- no
-// momentum or energy of a state the solvers can reach -- may differ from the
-// IEEE quotient.
+// densities).  Numerators are not checked: the tests show the short form
+// equal to the IEEE quotient for every numerator from 2^-1000 to 2^600 in
+// magnitude (and zero); below 2^-1000 (subnormal territory) it may be one
+// unit in the last place off, and above 2^600 it is not covered -- no
+// momentum or energy of a state the solvers reach.
 //
 // The host build (used by tests/quotient_check.cpp) keeps y = RN(1 / b) and two
 // residual corrections, q = RN(a y), q += (a - b q) y twice, which Markstein's
