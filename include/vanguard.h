@@ -95,7 +95,7 @@ int vg_ctx_set_graphs(vg_ctx *ctx, int enabled);
  * variant for a key "physics/order/mode/ny/nx" (forward Y sweeps, variants 0-7;
  * physics "euler", or "euler_planar" for grids swept with 4 fields)
  * or "euler_adjoint_x/ny/nx" / "euler_adjoint_y/ny/nx" (adjoint sweeps, variants
- * 0-3; "euler_planar_adjoint_..." for the 4-field adjoint sweeps; a 1D grid of n
+ * 0-5; "euler_planar_adjoint_..." for the 4-field adjoint sweeps; a 1D grid of n
  * cells has ny = 1, nx = n), -1 if not tuned
  * yet.  vg_ctx_set_tuned_shape pins a variant for a key (for example one saved
  * from an earlier process); -1 forgets it.  Shapes change speed, never results. */

@@ -299,14 +299,29 @@ def test_adjoint_tile_variants_change_nothing():
     try:
         kx, ky, k1 = "euler_adjoint_x/70/45", "euler_adjoint_y/70/45", "euler_adjoint_x/1/300"
         base = base1 = None
-        for vx in range(4):
-            for vy in range(4):
+        for vx in range(6):
+            for vy in range(6):
                 ctx.set_tuned_key(kx, vx)
                 ctx.set_tuned_key(ky, vy)
                 r = S.euler_adjoint(w, seed, 2, 1.0 / 45, 0.6, math.inf, params=IDEAL_M, max_steps=6, bc=1, ctx=ctx)
                 if base is None:
                     base = r.grad
                 O.assert_bitwise(r.grad, base, f"tile variants x {vx} y {vy}")
+        # the same for the 4-field adjoint sweeps (seed without w and lambda parts)
+        seedp = seed.copy()
+        seedp[..., 3] = 0.0
+        seedp[..., 5] = 0.0
+        px, py = "euler_planar_adjoint_x/70/45", "euler_planar_adjoint_y/70/45"
+        basep = None
+        for v in range(6):
+            for vy in (v, 5 - v):
+                ctx.set_tuned_key(px, v)
+                ctx.set_tuned_key(py, vy)
+                r = S.euler_adjoint(w, seedp, 2, 1.0 / 45, 0.6, math.inf, params=IDEAL_M, max_steps=6, bc=1, ctx=ctx)
+                if basep is None:
+                    basep = r.grad
+                O.assert_bitwise(r.grad, basep, f"4-field tile variants x {v} y {vy}")
+        for vx in range(6):
             ctx.set_tuned_key(k1, vx)
             r1 = S.euler_adjoint(w1, seed1, 1, 1.0 / 300, 0.6, math.inf, params=IDEAL_M, max_steps=9, ctx=ctx)
             if base1 is None:
@@ -316,7 +331,7 @@ def test_adjoint_tile_variants_change_nothing():
             ctx.set_tuned_key(k, -1)
             assert ctx.tuned_key(k) == -1
         S.euler_adjoint(w, seed, 2, 1.0 / 45, 0.6, math.inf, params=IDEAL_M, max_steps=2, bc=1, ctx=ctx)
-        assert 0 <= ctx.tuned_key(kx) <= 3 and 0 <= ctx.tuned_key(ky) <= 3
+        assert 0 <= ctx.tuned_key(kx) <= 5 and 0 <= ctx.tuned_key(ky) <= 5
     finally:
         ctx.close()
 

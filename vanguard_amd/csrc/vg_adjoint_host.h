@@ -21,8 +21,9 @@ inline void launch_adjoint_tile(vg_ctx *ctx, const AdjointArgs &a)
 
 // Tile variants per direction (threads, lines per block, waves per SIMD).  X:
 // one line per block; Y: LW adjacent columns per block, so each row read is LW
-// consecutive doubles.
-constexpr int kAdjVariants = 4;
+// consecutive doubles.  Variants 4-5 trade halo for tile length: a tile of
+// BS / LW positions keeps BS / LW - 8 outputs.
+constexpr int kAdjVariants = 6;
 constexpr int kAdjDefault[2] = {0, 0};
 
 template <class A = AdjFull>
@@ -33,6 +34,8 @@ inline void launch_adjoint_dir(vg_ctx *ctx, int dir, int variant, const AdjointA
         case 1: launch_adjoint_tile<A, 0, 128, 1, 3>(ctx, a); break;
         case 2: launch_adjoint_tile<A, 0, 256, 1, 2>(ctx, a); break;
         case 3: launch_adjoint_tile<A, 0, 64, 1, 2>(ctx, a); break;
+        case 4: launch_adjoint_tile<A, 0, 256, 1, 3>(ctx, a); break;
+        case 5: launch_adjoint_tile<A, 0, 512, 1, 3>(ctx, a); break;
         default: launch_adjoint_tile<A, 0, 128, 1, 2>(ctx, a); break;
         }
         return;
@@ -41,6 +44,8 @@ inline void launch_adjoint_dir(vg_ctx *ctx, int dir, int variant, const AdjointA
     case 1: launch_adjoint_tile<A, 1, 256, 8, 3>(ctx, a); break;
     case 2: launch_adjoint_tile<A, 1, 256, 16, 2>(ctx, a); break;
     case 3: launch_adjoint_tile<A, 1, 256, 4, 2>(ctx, a); break;
+    case 4: launch_adjoint_tile<A, 1, 512, 8, 3>(ctx, a); break;
+    case 5: launch_adjoint_tile<A, 1, 512, 16, 3>(ctx, a); break;
     default: launch_adjoint_tile<A, 1, 256, 8, 2>(ctx, a); break;
     }
 }

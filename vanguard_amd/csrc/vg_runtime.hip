@@ -236,8 +236,8 @@ int vg_ctx_set_tuned_shape(vg_ctx *ctx, const char *key, int variant)
 {
     VG_CHECK(ctx != nullptr && key != nullptr, "vg_ctx_set_tuned_shape: null argument");
     const bool adjoint = std::string(key).find("adjoint") != std::string::npos;
-    VG_CHECK(variant >= -1 && variant < (adjoint ? 4 : 8),
-             adjoint ? "vg_ctx_set_tuned_shape: adjoint variant must be -1..3"
+    VG_CHECK(variant >= -1 && variant < (adjoint ? 6 : 8),
+             adjoint ? "vg_ctx_set_tuned_shape: adjoint variant must be -1..5"
                      : "vg_ctx_set_tuned_shape: y-sweep variant must be -1..7");
     if (variant < 0) ctx->tuned.erase(key);
     else ctx->tuned[key] = variant;
