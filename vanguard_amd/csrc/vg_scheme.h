@@ -151,8 +151,19 @@ struct Scheme {
                 in_range = in_range && limiter_in_range(num[k], den[k]);
             }
             if (in_range) {
+                // Two cases need no division: slope differences of opposite
+                // signs (r < 0, the limiters' zero piece) and equal ones (r = 1
+                // exactly, e.g. both clamped to 1e-5 in a flat region), whose
+                // limiter value is the same for every cell.
+                const double f1 = limiter_value(1.0, bias, limiter, InRangeDiv());
 #pragma unroll
-                for (int k = 0; k < NF; k++) d[k] *= limiter_value(div_in_range(num[k], den[k]), bias, limiter, InRangeDiv());
+                for (int k = 0; k < NF; k++) {
+                    double lim;
+                    if (num[k] * den[k] < 0.0) lim = 0.0;
+                    else if (num[k] == den[k]) lim = f1;
+                    else lim = limiter_value(div_in_range(num[k], den[k]), bias, limiter, InRangeDiv());
+                    d[k] *= lim;
+                }
             } else {
 #pragma unroll
                 for (int k = 0; k < NF; k++) d[k] *= limiter_value(num[k] / den[k], bias, limiter);
