@@ -75,6 +75,9 @@ __device__ __forceinline__ void fractional_step(const typename P::Params &m, con
     reload_point(UL);
     double xl[NF], xr[NF];
     const double cl = 0.5 * (1.0 + bias), cr = 0.5 * (1.0 - bias);
+    // finite slope differences of opposite signs (r < 0) or equal ones (r = 1
+    // exactly) need no division: the limiter's zero piece, or its value at 1
+    const double f1 = limiter_value(1.0, bias, limiter);
 #pragma unroll
     for (int k = 0; k < NF; k++) {
         const double dl = UM[k] - UL[k];
@@ -82,7 +85,11 @@ __device__ __forceinline__ void fractional_step(const typename P::Params &m, con
         double d = (cl * dl) + (cr * dr);
         const double num = (fabs(dl) < 1e-5) ? 1e-5 : dl;
         const double den = (fabs(dr) < 1e-5) ? 1e-5 : dr;
-        d *= limiter_value(num / den, bias, limiter);
+        double lim;
+        if (limiter_in_range(num, den) && num * den < 0.0) lim = 0.0;
+        else if (limiter_in_range(num, den) && num == den) lim = f1;
+        else lim = limiter_value(num / den, bias, limiter);
+        d *= lim;
         xl[k] = UM[k] - (0.5 * d);
         xr[k] = UM[k] + (0.5 * d);
     }
